@@ -1,0 +1,182 @@
+// Device helpers shared by the skygen and module kernels: column access,
+// texture sampling, coordinate transforms and the density models.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include "../../include/galfast_hip.h"
+#include "gf_math.h"
+#include "gf_rng.h"
+
+#define GF_D __device__ __forceinline__
+
+// Element (row, plane) of a structure-of-arrays column (common/column.h:131-136).
+template <typename T>
+GF_D T &col(const gf_col &c, uint32_t row, uint32_t plane = 0)
+{
+	return *((T *)((char *)c.ptr + (size_t)plane * c.pitch) + row);
+}
+
+GF_D bool bit_isset(const gf_bitmap &m, int bit)
+{
+	if (bit < 32) return ((1u << bit) & m.bits[0]) != 0;
+	return ((1u << (bit - 32)) & m.bits[1]) != 0;
+}
+
+// Linear filtering of a 1D table with clamping, in the texture coordinate
+// convention of sample_impl (common/cux.h:849-855): xi = (x - tc0)*tc1 + 0.5,
+// texel centres at integer + 0.5.  The reference's CUDA path filters in
+// hardware and its CPU path takes the nearest texel (common/cux.h:778-810);
+// this build evaluates the interpolation exactly in FP32.
+GF_D float lerp_coord(float x, float tc0, float tc1, int n, int &i)
+{
+	float xi = (x - tc0) * tc1 + 0.5f;
+	float t = xi - 0.5f;
+	if (!(t > 0.f)) { i = 0; return 0.f; }
+	float tmax = (float)(n - 1);
+	if (t >= tmax) { i = n > 1 ? n - 2 : 0; return n > 1 ? 1.f : 0.f; }
+	float fl = __builtin_floorf(t);
+	i = (int)fl;
+	return t - fl;
+}
+
+GF_D float tex1d(const float *data, int n, float tc0, float tc1, float x)
+{
+	int i;
+	float a = lerp_coord(x, tc0, tc1, n, i);
+	if (n == 1) return data[0];
+	float v0 = data[i], v1 = data[i + 1];
+	return v0 + a * (v1 - v0);
+}
+
+GF_D float tex1d(const gf_tex1d &t, float x) { return tex1d(t.data, t.n, t.tc[0], t.tc[1], x); }
+
+GF_D float tex3d(const gf_tex3d &t, float x, float y, float z)
+{
+	if (!t.data) return 0.f;
+	int i, j, k;
+	float a = lerp_coord(x, t.tc[0][0], t.tc[0][1], t.nx, i);
+	float b = lerp_coord(y, t.tc[1][0], t.tc[1][1], t.ny, j);
+	float c = lerp_coord(z, t.tc[2][0], t.tc[2][1], t.nz, k);
+	int i1 = t.nx > 1 ? i + 1 : i, j1 = t.ny > 1 ? j + 1 : j, k1 = t.nz > 1 ? k + 1 : k;
+	const size_t sx = 1, sy = (size_t)t.nx, sz = (size_t)t.nx * t.ny;
+	auto at = [&](int ii, int jj, int kk) { return t.data[ii * sx + jj * sy + kk * sz]; };
+	float c00 = at(i, j, k) + a * (at(i1, j, k) - at(i, j, k));
+	float c10 = at(i, j1, k) + a * (at(i1, j1, k) - at(i, j1, k));
+	float c01 = at(i, j, k1) + a * (at(i1, j, k1) - at(i, j, k1));
+	float c11 = at(i, j1, k1) + a * (at(i1, j1, k1) - at(i, j1, k1));
+	float c0 = c00 + b * (c10 - c00);
+	float c1 = c01 + b * (c11 - c01);
+	return c0 + c * (c1 - c0);
+}
+
+// transform(): translation followed by a 3x3 rotation (modules/module_lib.h:122-148).
+GF_D void transform3(float &x, float &y, float &z, const float T[3], const float M[3][3])
+{
+	float vx = x + T[0], vy = y + T[1], vz = z + T[2];
+	x = M[0][0] * vx + M[0][1] * vy + M[0][2] * vz;
+	y = M[1][0] * vx + M[1][1] * vy + M[1][2] * vz;
+	z = M[2][0] * vx + M[2][1] * vy + M[2][2] * vz;
+}
+
+// Density of a model at heliocentric Galactic (x, y, z), before the LF factor:
+// expDisk::rho (skygen/model_expDisk.h:71-121) and
+// powerLawEllipsoid::rho (skygen/model_powerLawEllipsoid.h:60-105).
+GF_D float model_rho_xyz(const gf_model &m, float x, float y, float z)
+{
+	transform3(x, y, z, m.T, m.M);
+	if (m.kind == GF_MODEL_EXPDISK) {
+		float r2 = x * x + y * y;
+		if (m.r_cut2 != 0.f && r2 + z * z > m.r_cut2) return 0.f;
+		float r = gf_sqrtf(r2);
+		float e = -r / m.l - gf_fabsf(z) / m.h;
+		return m.f * gf_expf(e);
+	} else {
+		float DSq = x * x + y * y + z * z;
+		if (m.rmaxSq != 0.f && (DSq < m.rminSq || DSq >= m.rmaxSq)) return 0.f;
+		float yb = y / m.ba, zc = z / m.ca;
+		float rSq = x * x + yb * yb + zc * zc;
+		return m.f * gf_powf(rSq, 0.5f * m.n);
+	}
+}
+
+// lambert::deproject (skygen/skygen.h:131-146), with the reference's mixed
+// float/double promotions kept.
+GF_D void lambert_deproject(const gf_lambert &p, double &l, double &b, const float x, const float y)
+{
+	double r2 = (double)(x * x + y * y);
+	if (r2 == 0.) { b = p.b0; l = 0.; return; }
+	double ir = 1. / sqrt(r2);
+	double sc = 1. / ir * sqrt(1. - 0.25 * r2);
+	double cc = 1. - 0.5 * r2;
+	b = asin(cc * (double)p.sb + (double)y * sc * (double)p.cb * ir);
+	float ysb = y * p.sb;
+	l = p.l0 + atan2((double)x * sc, (double)p.cb * cc / ir - (double)ysb * sc);
+}
+
+static constexpr float POGSON = 0.4605170185988091f;   // 0.2 ln 10 (skygen/skygen.cu.h:466)
+static constexpr double DBL_R2D = 57.2957795130823208767981548;
+static constexpr double DBL_D2R = 0.01745329251994329576923691;
+
+// Footprint predicate of the clipper: 1 when (l, b) [radians] is outside.
+GF_D int outside_footprint(const gf_footprint &f, double l, double b)
+{
+	if (f.kind == 1) {
+		double ld = l * DBL_R2D, bd = b * DBL_R2D;
+		while (ld < 0.) ld += 360.;
+		while (ld >= 360.) ld -= 360.;
+		if (bd < f.b0 || bd > f.b1) return 1;
+		if (f.bmin > 0. && fabs(bd) < f.bmin) return 1;
+		double lo = f.l0, hi = f.l1;
+		if (hi - lo >= 360.) return 0;
+		double t = ld - lo;
+		while (t < 0.) t += 360.;
+		while (t >= 360.) t -= 360.;
+		return t > (hi - lo) ? 1 : 0;
+	}
+	if (f.kind == 2) {
+		double cb = cos(b), sb = sin(b), cl = cos(l), sl = sin(l);
+		double d = cb * cl * f.cb * f.cl + cb * sl * f.cb * f.sl + sb * f.sb;
+		return d < f.cosr ? 1 : 0;
+	}
+	return 0;
+}
+
+// Valid absolute-magnitude-bin window of DM bin k of one pixel
+// (the cell skips of skygen/skygen.cu.h:551-559): cells (im = k - iM, iM)
+// with 0 <= im < nm, 0 <= iM < nM, not cut by the distance limits and with
+// m0 + dm*im + Am <= m1.  Returns false if the window is empty.
+GF_D bool dm_window(const gf_skygen_params &sp, int k, float D, float Am, int &iMlo, int &iMhi)
+{
+	if ((sp.dmin != 0.f || sp.dmax != 0.f) && (sp.dmin > D || sp.dmax <= D)) return false;
+	int lo = k - (sp.nm - 1); if (lo < 0) lo = 0;
+	int hi = k; if (hi > sp.nM - 1) hi = sp.nM - 1;
+	if (lo > hi) return false;
+	// flux limit: the largest im passing m0 + dm*im + Am <= m1, found by the
+	// same float expression the reference evaluates per cell
+	int imlo = k - hi, imhi = k - lo;
+	if (sp.m0 + sp.dm * imlo + Am > sp.m1) return false;
+	if (!(sp.m0 + sp.dm * imhi + Am > sp.m1)) { iMlo = lo; iMhi = hi; return true; }
+	int a = imlo, b = imhi;   // pass at a, fail at b
+	while (b - a > 1) {
+		int mid = (a + b) >> 1;
+		if (sp.m0 + sp.dm * mid + Am > sp.m1) b = mid; else a = mid;
+	}
+	iMlo = k - a; iMhi = hi;
+	return true;
+}
+
+// compute_pos (skygen/skygen.cu.h:216-229): distance and heliocentric XYZ of
+// DM bin (im, M) of a pixel, and the per-beam minimum extinction.
+GF_D void cell_pos(const gf_skygen_params &sp, const gf_extinction &ext, const gf_pencil_beam &pix,
+                   float M, int im, float &D, float &Am, float &x, float &y, float &z)
+{
+	float m = sp.m0 + im * sp.dm;
+	float DM = m - M;
+	D = gf_powf(10.f, 0.2f * DM + 1.f);
+	int xidx = pix.extIdx / 2048;
+	int yidx = pix.extIdx % 2048;
+	Am = ext.beam.data ? tex3d(ext.beam, (float)xidx, (float)yidx, DM) : 0.f;
+	x = D * pix.cl * pix.cb;
+	y = D * pix.sl * pix.cb;
+	z = D * pix.sb;
+}

@@ -1,0 +1,183 @@
+// Deterministic single-precision transcendentals for the catalog hot path.
+//
+// The reference calls the platform's expf/logf/tanf/powf/exp10f/log10f inside
+// the Poisson and Gaussian samplers (common/cuda_rng.h:165-376), the density
+// models (skygen/model_expDisk.h:113, model_powerLawEllipsoid.h:93) and the
+// per-cell distance (skygen/skygen.cu.h:222, 408).  Rejection loops compare
+// those results against uniforms, so integer star counts depend on the last
+// bit of every transcendental.  To make the GPU path and a CPU restatement
+// agree bit for bit, every such function here is evaluated in IEEE double
+// with explicit fma() and rounded once to float:
+//
+//   exp:  Cody-Waite reduction x = k*ln2 + r, |r| <= ln2/2, Taylor series
+//         of e^r to degree 13 (truncation < 5e-18), scaling by 2^k.
+//   log:  x = 2^e * m, m in [sqrt(1/2), sqrt(2)), s = (m-1)/(m+1),
+//         log m = 2*atanh(s) = 2s * sum z^n/(2n+1), z = s^2 <= 0.0295,
+//         summed to n = 12 (truncation < 1e-19).
+//   sin/cos: reduction by pi/2 (two-part constant, fma), Taylor series to
+//         degree 17/18 on |r| <= pi/4 (truncation < 1e-18).
+//
+// Only +, *, /, sqrt, fma and rint are used; all are correctly rounded
+// IEEE operations on gfx950 and on x86-64, so the results are identical on
+// both.  Errors are below one float ulp (rounding of an accurate double).
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define GF_HD __host__ __device__ inline
+#else
+#include <math.h>
+#include <string.h>
+#define GF_HD inline
+#endif
+
+namespace gfm {
+
+GF_HD double as_double(uint64_t u) { double d; __builtin_memcpy(&d, &u, 8); return d; }
+GF_HD uint64_t as_u64(double d) { uint64_t u; __builtin_memcpy(&u, &d, 8); return u; }
+GF_HD float as_float(uint32_t u) { float f; __builtin_memcpy(&f, &u, 4); return f; }
+GF_HD uint32_t as_u32(float f) { uint32_t u; __builtin_memcpy(&u, &f, 4); return u; }
+
+GF_HD double dfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+GF_HD double drint(double a) { return __builtin_rint(a); }
+
+static constexpr double LN2_HI = 6.93147180369123816490e-01;   // ln2 rounded to 32 significant bits
+static constexpr double LN2_LO = 1.90821492927058770002e-10;   // ln2 - LN2_HI
+static constexpr double INV_LN2 = 1.44269504088896338700e+00;
+static constexpr double LN10 = 2.30258509299404590109e+00;
+static constexpr double INV_LN10 = 4.34294481903251816668e-01;
+static constexpr double PIO2_HI = 1.57079632679489655800e+00;   // double nearest pi/2
+static constexpr double PIO2_LO = 6.12323399573676603587e-17;   // pi/2 - PIO2_HI
+static constexpr double INV_PIO2 = 6.36619772367581382433e-01;
+
+// 2^k for integer k in [-1022, 1023]
+GF_HD double pow2i(int k) { return as_double((uint64_t)(k + 1023) << 52); }
+
+// e^x in double
+GF_HD double exp(double x)
+{
+	if (x != x) return x;
+	if (x > 709.0) return as_double(0x7ff0000000000000ull);
+	if (x < -745.0) return 0.0;
+	double kd = drint(x * INV_LN2);
+	double r = dfma(-kd, LN2_HI, x);
+	r = dfma(-kd, LN2_LO, r);
+	double p = 1.0 / 6227020800.0;            // 1/13!
+	p = dfma(p, r, 1.0 / 479001600.0);        // 1/12!
+	p = dfma(p, r, 1.0 / 39916800.0);         // 1/11!
+	p = dfma(p, r, 1.0 / 3628800.0);          // 1/10!
+	p = dfma(p, r, 1.0 / 362880.0);           // 1/9!
+	p = dfma(p, r, 1.0 / 40320.0);            // 1/8!
+	p = dfma(p, r, 1.0 / 5040.0);             // 1/7!
+	p = dfma(p, r, 1.0 / 720.0);              // 1/6!
+	p = dfma(p, r, 1.0 / 120.0);              // 1/5!
+	p = dfma(p, r, 1.0 / 24.0);               // 1/4!
+	p = dfma(p, r, 1.0 / 6.0);                // 1/3!
+	p = dfma(p, r, 0.5);                      // 1/2!
+	p = dfma(p, r, 1.0);
+	p = dfma(p, r, 1.0);
+	int k = (int)kd;
+	if (k > 1023) { p *= pow2i(1023); k -= 1023; }
+	if (k < -1022) { p *= pow2i(-1022); k += 1022; }
+	return p * pow2i(k);
+}
+
+// natural log in double
+GF_HD double log(double x)
+{
+	if (x != x || x < 0.0) return as_double(0x7ff8000000000000ull);
+	if (x == 0.0) return -as_double(0x7ff0000000000000ull);
+	uint64_t u = as_u64(x);
+	int e = (int)((u >> 52) & 0x7ff);
+	if (e == 0x7ff) return x;   // +inf
+	if (e == 0) { x *= pow2i(54); u = as_u64(x); e = (int)((u >> 52) & 0x7ff) - 54; }
+	e -= 1023;
+	double m = as_double((u & 0x000fffffffffffffull) | 0x3ff0000000000000ull);   // [1,2)
+	if (m > 1.41421356237309504880) { m *= 0.5; e += 1; }
+	double s = (m - 1.0) / (m + 1.0);
+	double z = s * s;
+	double p = 1.0 / 25.0;
+	p = dfma(p, z, 1.0 / 23.0);
+	p = dfma(p, z, 1.0 / 21.0);
+	p = dfma(p, z, 1.0 / 19.0);
+	p = dfma(p, z, 1.0 / 17.0);
+	p = dfma(p, z, 1.0 / 15.0);
+	p = dfma(p, z, 1.0 / 13.0);
+	p = dfma(p, z, 1.0 / 11.0);
+	p = dfma(p, z, 1.0 / 9.0);
+	p = dfma(p, z, 1.0 / 7.0);
+	p = dfma(p, z, 1.0 / 5.0);
+	p = dfma(p, z, 1.0 / 3.0);
+	p = dfma(p, z, 1.0);
+	double lm = 2.0 * s * p;
+	double ed = (double)e;
+	return dfma(ed, LN2_HI, dfma(ed, LN2_LO, lm));
+}
+
+// sin and cos of x in double (|x| < 1e6)
+GF_HD void sincos(double x, double *sn, double *cs)
+{
+	double kd = drint(x * INV_PIO2);
+	double r = dfma(-kd, PIO2_HI, x);
+	r = dfma(-kd, PIO2_LO, r);
+	double z = r * r;
+	// sin r = r * (1 - z/3! + z^2/5! - ... ), to r^17/17!
+	double ps = -1.0 / 355687428096000.0;     // -1/17!
+	ps = dfma(ps, z, 1.0 / 1307674368000.0);  // 1/15!
+	ps = dfma(ps, z, -1.0 / 6227020800.0);    // -1/13!
+	ps = dfma(ps, z, 1.0 / 39916800.0);       // 1/11!
+	ps = dfma(ps, z, -1.0 / 362880.0);        // -1/9!
+	ps = dfma(ps, z, 1.0 / 5040.0);           // 1/7!
+	ps = dfma(ps, z, -1.0 / 120.0);           // -1/5!
+	ps = dfma(ps, z, 1.0 / 6.0);              // 1/3!  (subtracted below)
+	double s = dfma(-r * z, ps, r);
+	// cos r = 1 - z/2! + z^2/4! - ..., to r^18/18!
+	double pc = 1.0 / 6402373705728000.0;     // 1/18!
+	pc = dfma(pc, z, -1.0 / 20922789888000.0);// -1/16!
+	pc = dfma(pc, z, 1.0 / 87178291200.0);    // 1/14!
+	pc = dfma(pc, z, -1.0 / 479001600.0);     // -1/12!
+	pc = dfma(pc, z, 1.0 / 3628800.0);        // 1/10!
+	pc = dfma(pc, z, -1.0 / 40320.0);         // -1/8!
+	pc = dfma(pc, z, 1.0 / 720.0);            // 1/6!
+	pc = dfma(pc, z, -1.0 / 24.0);            // -1/4!
+	pc = dfma(pc, z, 0.5);                    // 1/2!  (subtracted below)
+	double c = dfma(-z, pc, 1.0);
+	int q = ((int)(int64_t)kd) & 3;
+	switch (q) {
+	case 0: *sn = s; *cs = c; break;
+	case 1: *sn = c; *cs = -s; break;
+	case 2: *sn = -s; *cs = -c; break;
+	default: *sn = -c; *cs = s; break;
+	}
+}
+
+} // namespace gfm
+
+// Single-precision entry points used by the kernels.
+GF_HD float gf_expf(float x) { return (float)gfm::exp((double)x); }
+GF_HD float gf_logf(float x) { return (float)gfm::log((double)x); }
+GF_HD float gf_exp10f(float x) { return (float)gfm::exp((double)x * gfm::LN10); }
+GF_HD float gf_log10f(float x) { return (float)(gfm::log((double)x) * gfm::INV_LN10); }
+GF_HD float gf_sinf(float x) { double s, c; gfm::sincos((double)x, &s, &c); return (float)s; }
+GF_HD float gf_cosf(float x) { double s, c; gfm::sincos((double)x, &s, &c); return (float)c; }
+GF_HD float gf_tanf(float x) { double s, c; gfm::sincos((double)x, &s, &c); return (float)(s / c); }
+GF_HD float gf_sqrtf(float x) { return __builtin_sqrtf(x); }
+GF_HD float gf_fabsf(float x) { return __builtin_fabsf(x); }
+
+// x^y for float x >= 0 (and the integer-y cases of x < 0)
+GF_HD float gf_powf(float x, float y)
+{
+	if (y == 0.0f || x == 1.0f) return 1.0f;
+	if (x != x || y != y) return x + y;
+	if (x == 0.0f) return y > 0.0f ? 0.0f : gfm::as_float(0x7f800000u);
+	if (x < 0.0f) {
+		float yi = __builtin_truncf(y);
+		if (yi != y) return gfm::as_float(0x7fc00000u);
+		double r = gfm::exp((double)y * gfm::log(-(double)x));
+		int64_t iy = (int64_t)yi;
+		return (float)((iy & 1) ? -r : r);
+	}
+	return (float)gfm::exp((double)y * gfm::log((double)x));
+}

@@ -1,0 +1,420 @@
+// Per-star column modules as device functions over one thread's row range.
+// Each follows the reference kernel named in its comment; the global kernels
+// (modules.hip) and the fused chain (chain.hip) call them.
+#pragma once
+
+#include "gf_device.h"
+
+static constexpr float ABSMAG_NOT_PRESENT = 99.999f;   // modules/module_lib.h:57
+
+// os_FeH_kernel body (modules/FeH_gpu.cu.h:55-86)
+GF_D void mod_FeH(const gf_FeH_data &par, gf_mwc &rng, uint32_t r0, uint32_t r1,
+                  const gf_col &comp, const gf_col &hidden, const gf_col &XYZ, const gf_col &FeH)
+{
+	for (uint32_t row = r0; row < r1; row++) {
+		if (col<int>(hidden, row)) continue;
+		int cmp = col<int>(comp, row);
+		if (bit_isset(par.comp_thin, cmp) || bit_isset(par.comp_thick, cmp)) {
+			float p = rng.uniform() * (par.A[0] + par.A[1]);
+			int i = p < par.A[0] ? 0 : 1;
+			float x = col<float>(XYZ, row, 0), y = col<float>(XYZ, row, 1), z = col<float>(XYZ, row, 2);
+			transform3(x, y, z, par.T, par.M);
+			float muD = par.muInf + par.DeltaMu * gf_expf(-gf_fabsf(z) / par.Hmu);
+			float aZ = muD - 0.067f;
+			float feh = rng.gaussian(par.sigma[i]) + aZ + par.offs[i];
+			col<float>(FeH, row) = feh;
+		} else if (bit_isset(par.comp_halo, cmp)) {
+			float feh = par.offs[2] + rng.gaussian(par.sigma[2]);
+			col<float>(FeH, row) = feh;
+		}
+	}
+}
+
+// os_GaussianFeH_kernel body (modules/GaussianFeH_gpu.cu.h:55-72)
+GF_D void mod_GaussianFeH(const gf_GaussianFeH_data &par, gf_bitmap apply, gf_mwc &rng,
+                          uint32_t r0, uint32_t r1, const gf_col &comp, const gf_col &hidden,
+                          const gf_col &FeH_comp, const gf_col &FeH)
+{
+	for (uint32_t row = r0; row < r1; row++) {
+		if (col<int>(hidden, row)) continue;
+		int cmp = col<int>(comp, row);
+		if (!bit_isset(apply, cmp)) continue;
+		int at = 0;
+		if (par.ncomp != 1) {
+			float u = rng.uniform();
+			while (at < par.ncomp - 1 && par.pcum[at] < u) ++at;
+		}
+		col<float>(FeH, row) = par.mean[at] + rng.gaussian(par.sigma[at]);
+		if (FeH_comp.ptr) col<int>(FeH_comp, row) = at;
+	}
+}
+
+// modfun (modules/Bond2010_gpu.cu.h:106-110): a + b|Z|^c + d R^e, with the
+// exponent 0.5*e formed in double and narrowed as in the reference.
+GF_D float bond_modfun(float Rsquared, float Z, const float val[5])
+{
+	float p1 = gf_powf(gf_fabsf(Z), val[2]);
+	float p2 = gf_powf(Rsquared, (float)(0.5 * (double)val[4]));
+	return val[0] + val[1] * p1 + val[3] * p2;
+}
+
+// trivar_gaussK_draw (modules/Bond2010_gpu.cu.h:81-103): s11, s22, s33 are
+// standard deviations, s12, s13, s23 covariances.
+GF_D void bond_trivar(float y[3], const float s[6], gf_mwc &rng)
+{
+	float b10 = s[1];
+	float b11 = s[3] * s[3];
+	float b20 = s[2];
+	float b21 = s[4];
+	float b22 = s[5] * s[5];
+	float a00 = s[0];
+	float oneDivA00 = 1.0f / a00;
+	float a10 = b10 * oneDivA00;
+	float a11 = gf_sqrtf(b11 - a10 * a10);
+	float a20 = b20 * oneDivA00;
+	float a21 = (b21 - a20 * a10) / a11;
+	float a22 = gf_sqrtf(b22 - a20 * a20 - a21 * a21);
+	float z0 = rng.gaussian(1.0f);
+	float z1 = rng.gaussian(1.0f);
+	float z2 = rng.gaussian(1.0f);
+	y[0] += a00 * z0;
+	y[1] += a10 * z0 + a11 * z1;
+	y[2] += a20 * z0 + a21 * z1 + a22 * z2;
+}
+
+// os_Bond2010_kernel body (modules/Bond2010_gpu.cu.h:181-217)
+GF_D void mod_Bond2010(const gf_Bond2010_data &par, gf_mwc &rng, uint32_t r0, uint32_t r1,
+                       const gf_col &comp, const gf_col &hidden, const gf_col &XYZ, const gf_col &vcyl)
+{
+	for (uint32_t row = r0; row < r1; row++) {
+		if (col<int>(hidden, row)) continue;
+		const int cmp = col<int>(comp, row);
+		float x = col<float>(XYZ, row, 0), y = col<float>(XYZ, row, 1), z = col<float>(XYZ, row, 2);
+		transform3(x, y, z, par.T, par.M);
+		const float Rsquared = (float)(1e-6 * (double)(x * x + y * y));
+		const float Z = (float)(1e-3 * (double)z);
+		float v[3];
+		if (bit_isset(par.comp_thin, cmp) || bit_isset(par.comp_thick, cmp)) {
+			// get_disk_kinematics (modules/Bond2010_gpu.cu.h:130-140)
+			float p = rng.uniform();
+			const gf_vmoments &disk = p < par.fk ? par.disk_1 : par.disk_2;
+			for (int i = 0; i < 3; i++) v[i] = bond_modfun(Rsquared, Z, disk.m[i]);
+			if (v[1] > 0.f) v[1] = 0.f;
+			float s[6];
+			for (int i = 0; i < 6; i++) s[i] = bond_modfun(Rsquared, Z, disk.ss[i]);
+			bond_trivar(v, s, rng);
+		} else if (bit_isset(par.comp_halo, cmp)) {
+			// get_halo_kinematics (modules/Bond2010_gpu.cu.h:142-159)
+			for (int i = 0; i < 3; i++) v[i] = bond_modfun(Rsquared, Z, par.halo.m[i]);
+			float s[6];
+			for (int i = 0; i < 6; i++) s[i] = bond_modfun(Rsquared, Z, par.halo.ss[i]);
+			bond_trivar(v, s, rng);
+			float invR = 1.f / gf_sqrtf(Rsquared + Z * Z);
+			float cosa = gf_sqrtf(Rsquared) * invR;
+			float sina = Z * invR;
+			float vR = cosa * v[0] - sina * v[1];
+			float vZ = sina * v[0] + cosa * v[1];
+			v[0] = vR;
+			v[1] = v[2];
+			v[2] = vZ;
+		} else {
+			continue;
+		}
+		col<float>(vcyl, row, 0) = v[0];
+		col<float>(vcyl, row, 1) = v[1];
+		col<float>(vcyl, row, 2) = v[2];
+	}
+}
+
+// J2000 Galactic <-> equatorial constants (modules/module_lib.h:62-71)
+namespace galequ_c {
+static constexpr double angp = 0.01745329251994329576923691 * 192.859508333;
+static constexpr double l0 = 0.01745329251994329576923691 * 32.932;
+static constexpr double ce = 0.88998740217659689;
+static constexpr double se = 0.45598511375586859;
+static constexpr double pi2 = 6.28318530717958647692528677;
+static constexpr double d2r = 0.01745329251994329576923691;
+}
+
+// transform_pm in double (modules/vel2pm_gpu.cu.h:134-164), with float
+// in/out parameters narrowed where the reference narrows them.
+GF_D void transform_pm(float &vlout, float &vbout, float l, float b, float vl, float vb,
+                       const double ce, const double se, const double l0)
+{
+	double cb = cos((double)b);
+	double sb = sin((double)b);
+	double cl = cos((double)l - l0);
+	double sl = sin((double)l - l0);
+	double tmp1 = cb * se - ce * sb * sl;
+	double tmp2 = ce * sb - cb * se * sl;
+	double tmp4 = cb * cl;
+	double tmp3 = sb * se + cb * ce * sl;
+	if (fabs(tmp3) > 0.9999999) {
+		vlout = vbout = 999.99f;
+	} else {
+		double denom1 = 1. / (tmp4 * tmp4 + tmp2 * tmp2);
+		double cd = sqrt(1. - tmp3 * tmp3);
+		vl = (float)((double)vl / cb);
+		vlout = (float)((-ce * cl * (double)vb + cb * tmp1 * (double)vl) * denom1);
+		vbout = (float)((tmp1 * (double)vb + cb * ce * cl * (double)vl) / cd);
+		vlout = (float)((double)vlout * cd);
+	}
+}
+
+// os_vel2pm_kernel body (modules/vel2pm_gpu.cu.h:268-300) with vcyl2pm
+// (:221-251), vel_cyl2xyz (:112-121) and vel_xyz2lbr (:81-94).
+GF_D void mod_vel2pm(const gf_vel2pm_data &par, uint32_t r0, uint32_t r1, const gf_col &lb0,
+                     const gf_col &XYZ, const gf_col &vcyl, const gf_col &pmout, const gf_col &hidden)
+{
+	for (uint32_t row = r0; row < r1; row++) {
+		if (col<int>(hidden, row)) continue;
+		float l = 0.017453292519943295769f * (float)col<double>(lb0, row, 0);
+		float b = 0.017453292519943295769f * (float)col<double>(lb0, row, 1);
+		float X = par.Rg - col<float>(XYZ, row, 0);
+		float Y = -col<float>(XYZ, row, 1);
+		float Z = col<float>(XYZ, row, 2);
+		float vr = col<float>(vcyl, row, 0);
+		float vphi = col<float>(vcyl, row, 1);
+		float vz0 = col<float>(vcyl, row, 2);
+
+		float rho = gf_sqrtf(X * X + Y * Y);
+		float cphi = X / rho;
+		float sphi = Y / rho;
+		float px = -sphi * vphi + cphi * vr;
+		float py = cphi * vphi + sphi * vr;
+		float pz = vz0;
+		px -= par.u0;
+		py -= par.v0 + par.vLSR;
+		pz -= par.w0;
+
+		float vx = -px, vy = -py, vz = pz;
+		float cl = gf_cosf(l), sl = gf_sinf(l), cb = gf_cosf(b), sb = gf_sinf(b);
+		float vl = -vx * sl + vy * cl;
+		float tmp = vx * cl + vy * sl;
+		float vrad = cb * tmp + vz * sb;
+		float vb = -sb * tmp + vz * cb;
+
+		float dx = 8000.f - X;
+		float D = gf_sqrtf(dx * dx + Y * Y + Z * Z);
+		double den = 4.74 * (double)D * 1e-3;
+		float pmx = (float)((double)vl / den);
+		float pmy = (float)((double)vb / den);
+		float pmz = vrad;
+
+		if (par.coordsys == 0) {
+		} else if (par.coordsys == 1) {
+			transform_pm(pmx, pmy, l, b, pmx, pmy, galequ_c::ce, galequ_c::se, galequ_c::l0);
+		} else {
+			pmx = pmy = pmz = -9.99f;
+		}
+		col<float>(pmout, row, 0) = pmx;
+		col<float>(pmout, row, 1) = pmy;
+		col<float>(pmout, row, 2) = pmz;
+	}
+}
+
+// os_gal2other_kernel body with galequ (modules/gal2other_gpu.cu.h:39-95)
+GF_D void mod_gal2other(int coordsys, uint32_t r0, uint32_t r1, const gf_col &hidden,
+                        const gf_col &lb0, const gf_col &out)
+{
+	using namespace galequ_c;
+	for (uint32_t row = r0; row < r1; row++) {
+		if (col<int>(hidden, row)) continue;
+		double lx = col<double>(lb0, row, 0) * d2r;
+		double ly = col<double>(lb0, row, 1) * d2r;
+		double rx, ry;
+		if (coordsys == 1) {
+			const double cb = cos(ly);
+			const double sb = sin(ly);
+			const double cl = cos(lx - l0);
+			const double sl = sin(lx - l0);
+			rx = atan2(cb * cl, sb * ce - cb * se * sl) + angp;
+			ry = asin(cb * ce * sl + sb * se);
+			while (rx < 0.) rx += pi2;
+			while (rx >= pi2) rx -= pi2;
+		} else {
+			rx = ry = -9999.;
+		}
+		col<double>(out, row, 0) = rx / d2r;
+		col<double>(out, row, 1) = ry / d2r;
+	}
+}
+
+// Bilinear sample of a float4 colour table (the TEX2D lookups of
+// sampleColors, modules/photometry_gpu.cu.h:56-71).
+GF_D float4 tex2d4(const float *tab, int nx, int ny, const float tcx[2], const float tcy[2], float x, float y)
+{
+	int i, j;
+	float a = lerp_coord(x, tcx[0], tcx[1], nx, i);
+	float b = lerp_coord(y, tcy[0], tcy[1], ny, j);
+	int i1 = nx > 1 ? i + 1 : i, j1 = ny > 1 ? j + 1 : j;
+	const float4 *t = (const float4 *)tab;
+	float4 t00 = t[(size_t)j * nx + i], t10 = t[(size_t)j * nx + i1];
+	float4 t01 = t[(size_t)j1 * nx + i], t11 = t[(size_t)j1 * nx + i1];
+	float4 r;
+#define GF_BL(c)                                             \
+	{                                                        \
+		float r0 = t00.c + a * (t10.c - t00.c);              \
+		float r1 = t01.c + a * (t11.c - t01.c);              \
+		r.c = r0 + b * (r1 - r0);                            \
+	}
+	GF_BL(x) GF_BL(y) GF_BL(z) GF_BL(w)
+#undef GF_BL
+	return r;
+}
+
+// sampleColors (modules/photometry_gpu.cu.h:56-71)
+GF_D uint32_t sample_colors(const gf_photometry_data &lt, float *c, float FeH, float Mr)
+{
+	uint32_t flags = 0, cidx = 0;
+	for (int t = 0; t < 4 && cidx < lt.ncolors; t++) {
+		float4 clr = tex2d4(lt.color[t], lt.nFeH, lt.nMr, lt.tcFeH, lt.tcMr, FeH, Mr);
+		float4 f = tex2d4(lt.cflags[t], lt.nFeH, lt.nMr, lt.tcFeH, lt.tcMr, FeH, Mr);
+		float cv[4] = {clr.x, clr.y, clr.z, clr.w};
+		float fv[4] = {f.x, f.y, f.z, f.w};
+		for (int q = 0; q < 4 && cidx < lt.ncolors; q++) {
+			c[cidx] = cv[q];
+			flags |= (uint32_t)(fv[q] != 0.f) << cidx;
+			cidx++;
+		}
+	}
+	return flags;
+}
+
+// os_photometry_kernel body (modules/photometry_gpu.cu.h:100-173)
+GF_D void mod_photometry(const gf_photometry_data &lt, gf_bitmap apply, uint32_t r0, uint32_t r1,
+                         const gf_col &Am, const gf_col &flags, const gf_col &DM, const gf_col &Mr,
+                         int nabsmag, const gf_col &mags, const gf_col &FeH, const gf_col &comp,
+                         const gf_col &hidden)
+{
+	float c[16];
+	const int nb = (int)lt.ncolors;
+	for (uint32_t row = r0; row < r1; row++) {
+		if (col<int>(hidden, row)) continue;
+		int cmp = col<int>(comp, row);
+		if (!bit_isset(apply, cmp)) continue;
+		float fFeH = col<float>(FeH, row);
+		uint32_t flag = 0;
+		float L[17];
+		for (int b = 0; b <= nb; b++) L[b] = 0.f;
+		for (int syscomp = 0; syscomp < nabsmag; syscomp++) {
+			float fMr = col<float>(Mr, row, syscomp);
+			if (fMr >= ABSMAG_NOT_PRESENT) break;
+			flag |= sample_colors(lt, c, fFeH, fMr);
+			for (int b = 0; b <= nb; b++) {
+				float M = fMr;
+				if (b < (int)lt.bidx) { for (int i = b; i != (int)lt.bidx; i++) M += c[i]; }
+				if (b > (int)lt.bidx) { for (int i = lt.bidx; i != b; i++) M -= c[i]; }
+				float Lb = gf_exp10f(-0.4f * M);
+				if (syscomp) Lb += L[b];
+				L[b] = Lb;
+			}
+		}
+		bool fl = false;
+		for (int i = (int)lt.bidx - 1; i >= 0; i--) {
+			uint32_t mask = 1u << i;
+			fl = fl || (flag & mask);
+			flag |= (uint32_t)fl << i;
+		}
+		fl = false;
+		for (int i = lt.bidx; i < nb; i++) {
+			uint32_t mask = 1u << i;
+			fl = fl || (flag & mask);
+			flag |= (uint32_t)fl << i << 1;
+		}
+		flag &= ~(1u << lt.bidx);
+		flag = __brev(flag);
+		flag >>= (32 - nb - 1);
+		col<int>(flags, row) = (int)flag;
+
+		float dm = col<float>(DM, row);
+		float Am0 = col<float>(Am, row);
+		for (int b = 0; b <= nb; b++) {
+			float Msys = -2.5f * gf_log10f(L[b]);
+			float msys = dm + Msys;
+			float am = Am0 * lt.reddening[b];
+			col<float>(mags, row, b) = msys + am;
+		}
+	}
+}
+
+// draw_companion and os_unresolvedMultiples_kernel body
+// (modules/unresolvedMultiples_gpu.cu.h:48-122)
+GF_D void mod_multiples(const gf_multiples_data &par, gf_bitmap apply, gf_mwc &rng, uint32_t r0, uint32_t r1,
+                        int nabsmag, const gf_col &M, const gf_col &Msys, const gf_col &ncomp,
+                        const gf_col &comp, const gf_col &hidden)
+{
+	for (uint32_t row = r0; row < r1; row++) {
+		if (col<int>(hidden, row)) continue;
+		int cmp = col<int>(comp, row);
+		if (!bit_isset(apply, cmp)) continue;
+		float M1 = col<float>(M, row);
+		col<float>(Msys, row, 0) = M1;
+		float Ltot = gf_exp10f(-0.4f * M1);
+		int ncomps = 1;
+		for (int i = 1; i < nabsmag; i++) {
+			float M2 = 0.f;
+			bool has = false;
+			float psec = tex1d(par.secProb, M1);
+			float u = rng.uniform();
+			if (!(u > psec)) {
+				has = true;
+				if (par.algo == 3) {
+					M2 = M1;
+				} else {
+					float pprim = tex1d(par.cumLF, M1);
+					u = rng.uniform();
+					if (par.algo == 1) u = (float)((double)pprim + (double)u * (1. - (double)pprim));
+					M2 = tex1d(par.invCumLF, u);
+					if (par.algo == 1 && M2 < M1) M2 = M1;
+				}
+			}
+			if (has) {
+				col<float>(Msys, row, i) = M2;
+				Ltot += gf_exp10f(-0.4f * M2);
+				ncomps++;
+			} else {
+				col<float>(Msys, row, i) = ABSMAG_NOT_PRESENT;
+			}
+		}
+		float Mtot = (float)(-2.5 * (double)gf_log10f(Ltot));
+		col<float>(M, row) = Mtot;
+		col<int>(ncomp, row) = ncomps;
+	}
+}
+
+// Linear spline with linear extrapolation past the end knots (the GSL
+// linear interpolation behind spline::operator(), common/spline.cpp:59-65).
+GF_D double spline_eval(const double *kx, const double *ky, int n, double x)
+{
+	if (n == 1) return ky[0];
+	int lo = 0, hi = n - 1;
+	while (hi > lo + 1) {
+		int mid = (lo + hi) >> 1;
+		if (kx[mid] > x) hi = mid; else lo = mid;
+	}
+	double dx = kx[lo + 1] - kx[lo];
+	double D = (ky[lo + 1] - ky[lo]) / dx;
+	return ky[lo] + D * (x - kx[lo]);
+}
+
+// Photometric errors for one row chunk, bands in order, rows in order
+// within a band (modules/photometricErrors_host.cpp:112-131), drawing from
+// the chunk's MWC stream.
+GF_D void mod_photoerr(const gf_photoerr_data &par, gf_mwc &rng, uint32_t r0, uint32_t r1,
+                       const gf_col &magTrue, const gf_col &magObs, const gf_col &hidden)
+{
+	for (int b = 0; b < par.nbands; b++) {
+		int k0 = par.koff[b], k1 = par.koff[b + 1];
+		for (uint32_t row = r0; row < r1; row++) {
+			if (col<int>(hidden, row)) continue;
+			float mag = col<float>(magTrue, row, b);
+			if (k1 > k0) {
+				float sigma = (float)spline_eval(par.kx + k0, par.ky + k0, k1 - k0, (double)mag);
+				col<float>(magObs, row, b) = mag + rng.gaussian(sigma);
+			} else {
+				col<float>(magObs, row, b) = mag;
+			}
+		}
+	}
+}

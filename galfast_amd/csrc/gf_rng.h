@@ -1,0 +1,176 @@
+// Multiply-with-carry random streams and the samplers built on them.
+//
+// Follows prngs::mwc_impl::uniform (common/cuda_rng.h:439-463) and the
+// samplers of prngs::rng<> (common/cuda_rng.h:158-376): polar Box-Muller
+// gaussian, GSL-derived gamma_large/gamma_int/gamma_frac/gamma/beta,
+// binomial (Knuth) and Poisson (mu > 10 split through gamma_int, product of
+// uniforms below).  Each lane keeps its stream (a, c, x) in registers; the
+// reference kept it in shared memory, which has no effect on the sequence.
+// All float arithmetic is written operation by operation (the library is
+// built with -ffp-contract=off) and transcendentals come from gf_math.h, so
+// a CPU evaluation of the same steps yields the same bits.
+#pragma once
+
+#include "gf_math.h"
+
+struct gf_mwc
+{
+	uint32_t a, c, x;
+
+	GF_HD float uniform()
+	{
+		uint64_t t = (uint64_t)a * (uint64_t)x + (uint64_t)c;
+		c = (uint32_t)(t >> 32);
+		x = (uint32_t)t;
+		return 2.32830643708e-10f * (float)x;
+	}
+
+	GF_HD float uniform_pos()
+	{
+		float u;
+		do { u = uniform(); } while (u == 0.f);
+		return u;
+	}
+
+	GF_HD float gaussian(const float sigma)
+	{
+		float xx, yy, r2;
+		do {
+			xx = -1.f + 2.f * uniform();
+			yy = -1.f + 2.f * uniform();
+			r2 = xx * xx + yy * yy;
+		} while (r2 > 1.0f || r2 == 0.f || r2 == 4.f);
+		float t = -2.0f * gf_logf(r2);
+		t = t / r2;
+		return sigma * yy * gf_sqrtf(t);
+	}
+
+	GF_HD float gamma_large(const float aa)
+	{
+		float sqa, xv, yv, v;
+		sqa = gf_sqrtf(2 * aa - 1);
+		for (;;) {
+			do {
+				// tanf(M_PI * u): the product is formed in double and narrowed
+				yv = gf_tanf((float)(3.14159265358979323846 * (double)uniform()));
+				xv = sqa * yv + aa - 1;
+			} while (xv <= 0);
+			v = uniform();
+			float e = (aa - 1) * gf_logf(xv / (aa - 1)) - sqa * yv;
+			if (!(v > (1 + yv * yv) * gf_expf(e))) break;
+		}
+		return xv;
+	}
+
+	GF_HD float gamma_int(const unsigned int aa)
+	{
+		if (aa < 3) {
+			float prod = 1;
+			for (unsigned int i = 0; i < aa; i++) prod *= uniform_pos();
+			return -gf_logf(prod);
+		}
+		return gamma_large((float)aa);
+	}
+
+	GF_HD float gamma_frac(const float aa)
+	{
+		// p = M_E / (a + M_E), evaluated in double as written in the reference
+		float p, q, xv, u, v;
+		p = (float)(2.71828182845904523536 / ((double)aa + 2.71828182845904523536));
+		do {
+			u = uniform();
+			v = uniform_pos();
+			if (u < p) {
+				xv = gf_expf((1 / aa) * gf_logf(v));
+				q = gf_expf(-xv);
+			} else {
+				xv = 1 - gf_logf(v);
+				q = gf_expf((aa - 1) * gf_logf(xv));
+			}
+		} while (uniform() >= q);
+		return xv;
+	}
+
+	GF_HD float gamma(const float aa, const float b)
+	{
+		unsigned int na = (unsigned int)__builtin_floorf(aa);
+		if (aa == (float)na) return b * gamma_int(na);
+		if (na == 0) return b * gamma_frac(aa);
+		// the reference leaves the order of these two draws unspecified;
+		// this build draws the integer part first
+		float gi = gamma_int(na);
+		float gf = gamma_frac(aa - (float)na);
+		return b * (gi + gf);
+	}
+
+	GF_HD float beta(const float aa, const float b)
+	{
+		float x1 = gamma(aa, 1.0f);
+		float x2 = gamma(b, 1.0f);
+		return x1 / (x1 + x2);
+	}
+
+	GF_HD unsigned int binomial(float p, unsigned int n)
+	{
+		unsigned int i, aa, b, k = 0;
+		while (n > 10) {
+			aa = 1 + (n / 2);
+			b = 1 + n - aa;
+			float X = beta((float)aa, (float)b);
+			if (X >= p) {
+				n = aa - 1;
+				p /= X;
+			} else {
+				k += aa;
+				n = b - 1;
+				p = (p - X) / (1 - X);
+			}
+		}
+		for (i = 0; i < n; i++) {
+			float u = uniform();
+			if (u < p) k++;
+		}
+		return k;
+	}
+
+	GF_HD unsigned int poisson(float mu)
+	{
+		float emu;
+		float prod = 1.0f;
+		unsigned int k = 0;
+		while (mu > 10) {
+			// m = mu * (7.0/8.0): double product truncated to unsigned
+			unsigned int m = (unsigned int)((double)mu * 0.875);
+			float X = gamma_int(m);
+			if (X >= mu) {
+				return k + binomial(mu / X, m - 1);
+			} else {
+				k += m;
+				mu -= X;
+			}
+		}
+		emu = gf_expf(-mu);
+		do {
+			prod *= uniform();
+			k++;
+		} while (prod > emu);
+		return k - 1;
+	}
+};
+
+// Plane layout of a stream table: [a | c | x], each plane nstreams words
+// (common/cuda_rng.h:45-69).
+GF_HD gf_mwc gf_mwc_load(const uint32_t *st, uint32_t nstreams, uint32_t s)
+{
+	gf_mwc r;
+	r.a = st[s];
+	r.c = st[nstreams + s];
+	r.x = st[2 * nstreams + s];
+	return r;
+}
+
+GF_HD void gf_mwc_store(uint32_t *st, uint32_t nstreams, uint32_t s, const gf_mwc &r)
+{
+	st[nstreams + s] = r.c;
+	st[2 * nstreams + s] = r.x;
+}

@@ -1,0 +1,50 @@
+// Host-side runtime helpers: error propagation, launch checks and a small
+// per-device cache of scratch allocations reused across calls.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include "../../include/galfast_hip.h"
+
+#define GF_HIP(expr)                                   \
+	do {                                               \
+		hipError_t e__ = (expr);                       \
+		if (e__ != hipSuccess) return (int)e__;        \
+	} while (0)
+
+#define GF_CHECK(cond, code)                           \
+	do {                                               \
+		if (!(cond)) return (code);                    \
+	} while (0)
+
+// Launch-status check after <<<>>>.
+#define GF_LAUNCHED() GF_HIP(hipGetLastError())
+
+namespace gf {
+
+// Grow-only scratch buffer cached per (device, slot).  Slots are owned by
+// the entry point that uses them; calls on one device that overlap in time
+// must not share a slot.
+int scratch(int slot, size_t bytes, void **ptr);
+
+inline hipStream_t as_stream(void *s) { return (hipStream_t)s; }
+
+inline unsigned blocks_for(uint64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// Exclusive prefix sum of n 32-bit counts into 64-bit offsets; offs has n+1
+// entries, offs[n] = total.  Deterministic (fixed block decomposition).
+int exclusive_scan_u32_u64(const uint32_t *in, uint64_t *offs, uint64_t n, hipStream_t st);
+
+} // namespace gf
+
+enum gf_scratch_slot {
+	GF_SLOT_LFTAB = 0,
+	GF_SLOT_PARTIALS,
+	GF_SLOT_SCAN,
+	GF_SLOT_COMPAT_COUNTS,
+	GF_SLOT_COMPAT_STATE,
+	GF_SLOT_SAMPLE,
+	GF_SLOT_COUNT_W,
+	GF_SLOT_NSLOTS
+};

@@ -1,0 +1,281 @@
+// Configuration parsing and host-side table construction.
+#include "host.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <fstream>
+#include <sstream>
+
+namespace gfh {
+
+void fail(const std::string &msg) { throw ConfigError{msg}; }
+
+static std::string trim(const std::string &s)
+{
+	size_t a = s.find_first_not_of(" \t\r\n");
+	if (a == std::string::npos) return "";
+	size_t b = s.find_last_not_of(" \t\r\n");
+	return s.substr(a, b - a + 1);
+}
+
+void Config::parse(const std::string &text)
+{
+	std::istringstream in(text);
+	std::string line;
+	while (std::getline(in, line)) {
+		size_t h = line.find('#');
+		if (h != std::string::npos) line = line.substr(0, h);
+		size_t eq = line.find('=');
+		if (eq == std::string::npos) continue;
+		std::string k = trim(line.substr(0, eq)), v = trim(line.substr(eq + 1));
+		if (!k.empty()) kv[k] = v;
+	}
+}
+
+bool Config::load(const std::string &file)
+{
+	std::ifstream f(file);
+	if (!f) return false;
+	std::stringstream ss;
+	ss << f.rdbuf();
+	path = file;
+	parse(ss.str());
+	return true;
+}
+
+bool Config::count(const std::string &key) const
+{
+	return kv.count(key) || (globals && globals->count(key));
+}
+
+std::string Config::get(const std::string &key) const
+{
+	auto it = kv.find(key);
+	if (it != kv.end()) return it->second;
+	if (globals && globals->count(key)) return globals->get(key);
+	fail("configuration key '" + key + "' missing" + (path.empty() ? "" : " in " + path));
+}
+
+std::string Config::get(const std::string &key, const std::string &dflt) const
+{
+	return count(key) ? get(key) : dflt;
+}
+
+double Config::getd(const std::string &key) const
+{
+	std::string v = get(key);
+	char *end = nullptr;
+	double d = strtod(v.c_str(), &end);
+	if (end == v.c_str()) fail("configuration key '" + key + "' is not a number: " + v);
+	return d;
+}
+
+double Config::getd(const std::string &key, double dflt) const { return count(key) ? getd(key) : dflt; }
+
+std::vector<double> Config::getv(const std::string &key, const std::string &dflt) const
+{
+	std::istringstream ss(get(key, dflt));
+	std::vector<double> out;
+	double d;
+	while (ss >> d) out.push_back(d);
+	return out;
+}
+
+std::string Config::resolve(const std::string &rel) const
+{
+	if (rel.empty() || rel[0] == '/' || path.empty()) return rel;
+	size_t s = path.find_last_of('/');
+	if (s == std::string::npos) return rel;
+	return path.substr(0, s + 1) + rel;
+}
+
+std::vector<std::vector<double>> load_columns(const std::string &file, int ncols)
+{
+	std::ifstream f(file);
+	if (!f) fail("cannot open " + file);
+	std::vector<std::vector<double>> rows;
+	std::string line;
+	while (std::getline(f, line)) {
+		size_t h = line.find('#');
+		if (h != std::string::npos) line = line.substr(0, h);
+		std::istringstream ss(line);
+		std::vector<double> r;
+		double d;
+		while ((int)r.size() < ncols && ss >> d) r.push_back(d);
+		if (r.empty()) continue;
+		if ((int)r.size() < ncols) fail("short row in " + file);
+		rows.push_back(r);
+	}
+	return rows;
+}
+
+double Spline::operator()(double v) const
+{
+	int n = (int)x.size();
+	if (n == 1) return y[0];
+	int lo = 0, hi = n - 1;
+	while (hi > lo + 1) {
+		int mid = (lo + hi) / 2;
+		if (x[mid] > v) hi = mid; else lo = mid;
+	}
+	double dx = x[lo + 1] - x[lo];
+	double D = (y[lo + 1] - y[lo]) / dx;
+	return y[lo] + D * (v - x[lo]);
+}
+
+Tex1D resample_1d(const std::vector<double> &X, const std::vector<double> &Y, int nsamp)
+{
+	Spline s(X, Y);
+	Tex1D t;
+	t.v.resize(nsamp);
+	double X0 = X.front(), X1 = X.back(), dX = (X1 - X0) / (nsamp - 1);
+	for (int i = 0; i < nsamp; i++) t.v[i] = (float)s(X0 + i * dX);
+	t.tc[0] = (float)X0;
+	t.tc[1] = (float)(1. / dX);
+	return t;
+}
+
+Tex1D resample_clip_1d(std::vector<double> X, std::vector<double> Y, float clip, int nsamp)
+{
+	X.insert(X.begin(), -123.0);
+	Y.insert(Y.begin(), (double)clip);
+	double xrange = X.back() - X[1];
+	X[0] = X[1] - 1e-5 * xrange;
+	X.push_back(X.back() + 1e-5 * xrange);
+	Y.push_back((double)clip);
+	return resample_1d(X, Y, nsamp);
+}
+
+Tex1D constant_1d(float val, float X0, float X1)
+{
+	Tex1D t;
+	t.v = {val, val};
+	t.tc[0] = X0;
+	t.tc[1] = (float)(1. / ((double)X1 - (double)X0));
+	return t;
+}
+
+PhotTables build_photometry_tables(const std::vector<std::vector<double>> &rows, int ncolors, double dFeH,
+                                   double dMr)
+{
+	// group rows by FeH knot; sort each group by Mr
+	std::map<double, std::vector<std::vector<double>>> byFeH;
+	for (const auto &r : rows) byFeH[r[1]].push_back(r);
+	if (byFeH.empty()) fail("empty photometric system table");
+	double FeH0 = byFeH.begin()->first, FeH1 = byFeH.rbegin()->first;
+	double Mr0 = rows[0][0], Mr1 = rows[0][0];
+	for (const auto &r : rows) { Mr0 = std::min(Mr0, r[0]); Mr1 = std::max(Mr1, r[0]); }
+	PhotTables pt;
+	pt.ncolors = ncolors;
+	pt.nFeH = (int)((FeH1 - FeH0) / dFeH + 1);
+	pt.nMr = (int)((Mr1 - Mr0) / dMr + 1);
+	pt.ntex = (ncolors + 3) / 4;
+	size_t per = (size_t)pt.nFeH * pt.nMr * 4;
+	pt.color.assign(per * pt.ntex, 0.f);
+	pt.flags.assign(per * pt.ntex, 0.f);
+
+	std::vector<double> keys;
+	std::vector<double> mrmin, mrmax;
+	std::vector<std::vector<Spline>> spl;   // [knot][colour] col(Mr)
+	for (auto &kvp : byFeH) {
+		auto &g = kvp.second;
+		std::stable_sort(g.begin(), g.end(), [](const std::vector<double> &a, const std::vector<double> &b) { return a[0] < b[0]; });
+		keys.push_back(kvp.first);
+		std::vector<double> mr;
+		for (auto &r : g) mr.push_back(r[0]);
+		std::vector<Spline> s(ncolors);
+		for (int c = 0; c < ncolors; c++) {
+			std::vector<double> cv;
+			for (auto &r : g) cv.push_back(r[2 + c]);
+			s[c] = Spline(mr, cv);
+		}
+		spl.push_back(std::move(s));
+		mrmin.push_back(mr.front());
+		mrmax.push_back(mr.back());
+	}
+	const int nk = (int)keys.size();
+	std::vector<double> vcol(nk), vecol(nk);
+	for (int ic = 0; ic < ncolors; ic++) {
+		int tex = ic / 4, comp = ic % 4;
+		for (int m = 0; m < pt.nMr; m++) {
+			double Mr = Mr0 + m * dMr;
+			for (int k = 0; k < nk; k++) {
+				vcol[k] = spl[k][ic](Mr);
+				vecol[k] = (mrmin[k] > Mr || Mr > mrmax[k]) ? 1.0 : 0.0;
+			}
+			Spline fc(keys, vcol), fe(keys, vecol);
+			for (int f = 0; f < pt.nFeH; f++) {
+				double FeH = FeH0 + f * dFeH;
+				float color = (float)fc(FeH);
+				bool ef = keys.front() > FeH || FeH > keys.back() || fe(FeH) != 0.;
+				size_t o = (((size_t)tex * pt.nMr + m) * pt.nFeH + f) * 4 + comp;
+				pt.color[o] = color;
+				pt.flags[o] = ef ? 1.f : 0.f;
+			}
+		}
+	}
+	// texcoord_from_range(0, n, x0, x0 + n*d), evaluated in float (common/cux.cpp:389-397)
+	auto tcr = [](float imgx, float imgy, float x, float y, float *o) {
+		o[0] = (-imgy * x + imgx * y) / (imgx - imgy);
+		o[1] = (imgx - imgy) / (x - y);
+	};
+	tcr(0.f, (float)pt.nFeH, (float)FeH0, (float)(FeH0 + pt.nFeH * dFeH), pt.tcFeH);
+	tcr(0.f, (float)pt.nMr, (float)Mr0, (float)(Mr0 + pt.nMr * dMr), pt.tcMr);
+	return pt;
+}
+
+// get_gal2xxx_matrix / xxx2galxyz / load_transform (common/transform.cpp:70-388)
+// for the coordinate systems the configurations use: gal and galplane.
+void load_transform(float T[3], float M[3][3], const std::string &center, const std::string &orientation,
+                    const Config &cfg)
+{
+	double ctr[3] = {0, 0, 0};
+	{
+		std::istringstream ss(center);
+		std::string cs, type;
+		ss >> cs >> type >> ctr[0] >> ctr[1] >> ctr[2];
+		if (type.empty()) ctr[0] = ctr[1] = ctr[2] = 0.;
+		if (cs == "gal") {
+			if (type == "sph") {
+				double l = ctr[0] * M_PI / 180., b = ctr[1] * M_PI / 180., d = ctr[2];
+				ctr[0] = d * cos(l) * cos(b); ctr[1] = d * sin(l) * cos(b); ctr[2] = d * sin(b);
+			} else if (!(type == "xyz" || type.empty())) fail("unknown center type " + type);
+		} else if (cs == "galcentric") {
+			double Rg = cfg.getd("Rg");
+			ctr[0] = Rg - ctr[0];
+			ctr[1] = -ctr[1];
+		} else if (cs == "galplane") {
+			double Rg = cfg.getd("Rg"), z0 = cfg.getd("z0");
+			double sina = -z0 / Rg, cosa = sqrt(1 - sina * sina);
+			double x = cosa * ctr[0] - sina * ctr[2];
+			double z = sina * ctr[0] + cosa * ctr[2];
+			ctr[0] = Rg - x;
+			ctr[1] = -ctr[1];
+			ctr[2] = z;
+		} else fail("unsupported center coordinate system " + cs);
+	}
+	T[0] = (float)-ctr[0];
+	T[1] = (float)-ctr[1];
+	T[2] = (float)-ctr[2];
+	double R[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+	std::istringstream ss(orientation);
+	std::string cs;
+	ss >> cs;
+	if (cs == "galplane") {
+		double Rg = cfg.getd("Rg"), z0 = cfg.getd("z0");
+		double sina = -z0 / Rg, cosa = sqrt(1 - sina * sina);
+		double G[3][3] = {{-cosa, 0, sina}, {0, -1, 0}, {sina, 0, cosa}};
+		memcpy(R, G, sizeof(R));
+	} else if (!(cs.empty() || cs == "gal" || cs == "galcentric")) {
+		fail("unsupported orientation " + cs);
+	}
+	std::string kind;
+	if (ss >> kind) fail("rotations on top of the coordinate system are not supported: " + orientation);
+	for (int i = 0; i < 3; i++)
+		for (int j = 0; j < 3; j++) M[i][j] = (float)R[i][j];
+}
+
+} // namespace gfh

@@ -1,0 +1,116 @@
+// Seeding and device management of the multiply-with-carry streams
+// (common/cuda_rng.h:74-103, 466-497; common/cux.cpp:404-435).
+#include "gf_runtime.h"
+#include "gf_rng.h"
+#include <random>
+
+namespace {
+
+// POSIX drand48 family: X <- (0x5DEECE66D * X + 0xB) mod 2^48.
+struct Rand48 {
+	uint64_t X;
+	void seed(uint32_t s) { X = ((uint64_t)s << 16) | 0x330Eull; }
+	void step() { X = (0x5DEECE66Dull * X + 0xBull) & 0xFFFFFFFFFFFFull; }
+	double drand() { step(); return (double)X * (1.0 / 281474976710656.0); }
+	uint32_t lrand() { step(); return (uint32_t)(X >> 17); }
+};
+
+// (int)v with the x86-64 cvttsd2si result for out-of-range values.
+inline int32_t trunc_to_int_x86(double v)
+{
+	if (!(v > -2147483649.0 && v < 2147483648.0)) return INT32_MIN;
+	return (int32_t)v;
+}
+
+__global__ void mwc_sample_kernel(uint32_t *state, uint32_t ns, int kind, float param, uint32_t n, void *out)
+{
+	uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= ns) return;
+	gf_mwc r = gf_mwc_load(state, ns, s);
+	for (uint32_t i = 0; i < n; i++) {
+		size_t o = (size_t)i * ns + s;
+		if (kind == 0) ((float *)out)[o] = r.uniform();
+		else if (kind == 1) ((float *)out)[o] = r.gaussian(param);
+		else if (kind == 2) ((uint32_t *)out)[o] = r.poisson(param);
+		else ((uint32_t *)out)[o] = r.poisson(param * (float)(1 + s));
+	}
+	gf_mwc_store(state, ns, s, r);
+}
+
+} // namespace
+
+extern "C" {
+
+uint32_t gf_seed_from_pipeline_seed(uint32_t pipeline_seed)
+{
+	// gsl_rng_set(mt19937, 0) seeds with 4357; the first output divided by
+	// 2^32 is gsl_rng_uniform, narrowed to float by rng_gsl_t::uniform
+	std::mt19937 mt(pipeline_seed == 0 ? 4357u : pipeline_seed);
+	float u = (float)((double)mt() / 4294967296.0);
+	return (uint32_t)(u * (float)(1 << 24));
+}
+
+int gf_mwc_seed_host(uint32_t seed, const uint32_t *primes, uint32_t nprimes,
+                     uint32_t nstreams, uint32_t *out_state)
+{
+	GF_CHECK(primes && out_state, GF_E_ARG);
+	if (nstreams % 64) nstreams += 64 - nstreams % 64;
+	GF_CHECK(nstreams <= nprimes, GF_E_ARG);
+	Rand48 r;
+	r.seed(seed);
+	for (uint32_t i = 0; i < nstreams; i++) {
+		uint32_t a = primes[i];
+		out_state[i] = a;
+		out_state[nstreams + i] = (uint32_t)trunc_to_int_x86(r.drand() * (double)a);
+		out_state[2 * nstreams + i] = r.lrand();
+	}
+	return (int)nstreams;
+}
+
+int gf_rng_alloc(gf_rng *rng, uint32_t nstreams)
+{
+	GF_CHECK(rng && nstreams > 0, GF_E_ARG);
+	rng->nstreams = nstreams;
+	rng->reserved = 0;
+	GF_HIP(hipMalloc((void **)&rng->state, (size_t)3 * nstreams * sizeof(uint32_t)));
+	return 0;
+}
+
+int gf_rng_upload(gf_rng *rng, const uint32_t *host_state, void *stream)
+{
+	GF_CHECK(rng && rng->state && host_state, GF_E_ARG);
+	GF_HIP(hipMemcpyAsync(rng->state, host_state, (size_t)3 * rng->nstreams * sizeof(uint32_t),
+	                      hipMemcpyHostToDevice, gf::as_stream(stream)));
+	GF_HIP(hipStreamSynchronize(gf::as_stream(stream)));
+	return 0;
+}
+
+int gf_rng_download(const gf_rng *rng, uint32_t *host_state, void *stream)
+{
+	GF_CHECK(rng && rng->state && host_state, GF_E_ARG);
+	GF_HIP(hipMemcpyAsync(host_state, rng->state, (size_t)3 * rng->nstreams * sizeof(uint32_t),
+	                      hipMemcpyDeviceToHost, gf::as_stream(stream)));
+	GF_HIP(hipStreamSynchronize(gf::as_stream(stream)));
+	return 0;
+}
+
+int gf_rng_free(gf_rng *rng)
+{
+	GF_CHECK(rng, GF_E_ARG);
+	if (rng->state) GF_HIP(hipFree(rng->state));
+	rng->state = nullptr;
+	rng->nstreams = 0;
+	return 0;
+}
+
+int gf_mwc_sample(gf_rng *rng, int kind, float param, uint32_t n, void *out, void *stream)
+{
+	GF_CHECK(rng && rng->state && out && kind >= 0 && kind <= 3, GF_E_ARG);
+	unsigned bs = 256;
+	mwc_sample_kernel<<<gf::blocks_for(rng->nstreams, bs), bs, 0, gf::as_stream(stream)>>>(
+	    rng->state, rng->nstreams, kind, param, n, out);
+	GF_LAUNCHED();
+	return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,653 @@
+// Sky generator kernels: expected-count integral, reference-order draw,
+// and the cumulative-table draw (plan + emit).
+//
+// Cell geometry (skygen/skygen.cu.h:216-229, 541-565): pixel p, apparent bin
+// im, absolute bin iM; M = M1 - iM*dM, m = m0 + im*dm, and because dm == dM
+// the distance modulus depends only on k = im + iM.  A (pixel, k) pair is a
+// "DM bin"; its density rho(xyz(D_k)) is shared by all its cells, and a cell
+// contributes LF(M) * rho * norm * D^3 * dA * POGSON * dm * dM.
+#include "gf_runtime.h"
+#include "gf_device.h"
+
+namespace {
+
+constexpr unsigned SK_BS = 256;
+
+// ---------------------------------------------------------------------------
+// Luminosity-function tables of one model: LFc[iM] = LF(M1 - iM*dM) and its
+// exclusive prefix sums in the three accumulation arithmetics.
+// ---------------------------------------------------------------------------
+struct LFTables {
+	float *lfc;     // nM
+	double *P;      // nM + 1, FP64 prefix
+	float *Pk;      // nM + 1, Kahan-compensated FP32 prefix
+	float *Pf;      // nM + 1, naive FP32 prefix
+};
+
+__global__ void k_lf_tables(gf_skygen_params sp, gf_model m, LFTables t)
+{
+	for (int iM = threadIdx.x; iM < sp.nM; iM += blockDim.x) {
+		float M = sp.M1 - iM * sp.dM;
+		t.lfc[iM] = tex1d(m.lf, M);
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		double s = 0.0;
+		float sk = 0.f, ck = 0.f, sf = 0.f;
+		t.P[0] = 0.0; t.Pk[0] = 0.f; t.Pf[0] = 0.f;
+		for (int iM = 0; iM < sp.nM; iM++) {
+			float v = t.lfc[iM];
+			s += (double)v;
+			float y = v - ck;
+			float tt = sk + y;
+			ck = (tt - sk) - y;
+			sk = tt;
+			sf += v;
+			t.P[iM + 1] = s; t.Pk[iM + 1] = sk; t.Pf[iM + 1] = sf;
+		}
+	}
+}
+
+int lf_tables(const gf_skygen_params &sp, const gf_model &m, LFTables &t, hipStream_t st)
+{
+	size_t n = (size_t)sp.nM + 1;
+	size_t bytes = n * 4 + n * 8 + n * 4 + n * 4 + 64;
+	void *p;
+	int rc = gf::scratch(GF_SLOT_LFTAB, bytes, &p);
+	if (rc) return rc;
+	char *c = (char *)p;
+	t.P = (double *)c; c += n * 8;
+	t.lfc = (float *)c; c += n * 4;
+	t.Pk = (float *)c; c += n * 4;
+	t.Pf = (float *)c;
+	k_lf_tables<<<1, SK_BS, 0, st>>>(sp, m, t);
+	GF_LAUNCHED();
+	return 0;
+}
+
+// Distance of DM bin k: compute_pos evaluated at the diagonal's first cell in
+// crawl order (largest iM), which is where the reference's advance() enters a
+// diagonal.
+GF_D void dm_bin(const gf_skygen_params &sp, const gf_extinction &ext, const gf_pencil_beam &pix, int k,
+                 float &D, float &Am, float &x, float &y, float &z)
+{
+	int iM0 = k < sp.nM - 1 ? k : sp.nM - 1;
+	int im0 = k - iM0;
+	float M = sp.M1 - iM0 * sp.dM;
+	cell_pos(sp, ext, pix, M, im0, D, Am, x, y, z);
+}
+
+// ---------------------------------------------------------------------------
+// Count integral: one workgroup per pixel, lanes over DM bins.
+// ---------------------------------------------------------------------------
+struct CountAcc {
+	double c, cov, mx, ncell;
+};
+
+template <int MODE>
+struct Acc;   // per-mode running sums
+
+template <> struct Acc<GF_ACC_FP64> {
+	double c = 0, cov = 0;
+	GF_D void add(double w, float cf) { c += w; cov += w * (double)cf; }
+	GF_D double total() const { return c; }
+	GF_D double covered() const { return cov; }
+};
+template <> struct Acc<GF_ACC_KAHAN32> {
+	float c = 0, cc = 0, cov = 0, ccov = 0;
+	GF_D static void kadd(float &s, float &comp, float v) { float y = v - comp; float t = s + y; comp = (t - s) - y; s = t; }
+	GF_D void add(double w, float cf) { float wf = (float)w; kadd(c, cc, wf); kadd(cov, ccov, wf * cf); }
+	GF_D double total() const { return (double)c; }
+	GF_D double covered() const { return (double)cov; }
+};
+template <> struct Acc<GF_ACC_FP32> {
+	float c = 0, cov = 0;
+	GF_D void add(double w, float cf) { float wf = (float)w; c += wf; cov += wf * cf; }
+	GF_D double total() const { return (double)c; }
+	GF_D double covered() const { return (double)cov; }
+};
+
+// Expectation of DM bin (pixel, k); returns -1 for an empty window.
+template <int MODE>
+GF_D double bin_expectation(const gf_skygen_params &sp, const gf_model &m, const gf_extinction &ext,
+                            const gf_pencil_beam &pix, const LFTables &t, int k, int &ncell)
+{
+	float D, Am, x, y, z;
+	dm_bin(sp, ext, pix, k, D, Am, x, y, z);
+	int lo, hi;
+	ncell = 0;
+	if (!dm_window(sp, k, D, Am, lo, hi)) return -1.0;
+	ncell = hi - lo + 1;
+	float R = model_rho_xyz(m, x, y, z);
+	if (MODE == GF_ACC_FP64) {
+		double vol = (double)pix.dA * (double)POGSON * (double)sp.dm * (double)sp.dM;
+		double fac = (double)R * (double)m.norm * ((double)D * (double)D * (double)D) * vol;
+		return fac * (t.P[hi + 1] - t.P[lo]);
+	} else {
+		float rho = R * m.norm;
+		rho *= D * D * D;
+		rho *= pix.dA * POGSON * sp.dm * sp.dM;
+		float S = MODE == GF_ACC_KAHAN32 ? (t.Pk[hi + 1] - t.Pk[lo]) : (t.Pf[hi + 1] - t.Pf[lo]);
+		return (double)(rho * S);
+	}
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(SK_BS) k_count(gf_skygen_params sp, gf_model m, const gf_pencil_beam *pixels,
+                                                 gf_extinction ext, LFTables t, CountAcc *pix_out,
+                                                 double *per_pixel, double *wpk)
+{
+	const int p = blockIdx.x;
+	const int K = sp.nm + sp.nM - 1;
+	const gf_pencil_beam pix = pixels[p];
+	Acc<MODE> acc;
+	double mx = 0.0, nc = 0.0;
+	for (int k = threadIdx.x; k < K; k += blockDim.x) {
+		int ncell;
+		double w = bin_expectation<MODE>(sp, m, ext, pix, t, k, ncell);
+		if (wpk) wpk[(size_t)p * K + k] = w;
+		if (w >= 0.0) {
+			acc.add(w, pix.coveredFraction);
+			if (w > mx) mx = w;
+			nc += (double)ncell;
+		}
+	}
+	__shared__ double sc[SK_BS], scov[SK_BS], smx[SK_BS], snc[SK_BS];
+	sc[threadIdx.x] = acc.total();
+	scov[threadIdx.x] = acc.covered();
+	smx[threadIdx.x] = mx;
+	snc[threadIdx.x] = nc;
+	__syncthreads();
+	for (int o = SK_BS / 2; o > 0; o >>= 1) {
+		if ((int)threadIdx.x < o) {
+			if (MODE == GF_ACC_FP64) {
+				sc[threadIdx.x] += sc[threadIdx.x + o];
+				scov[threadIdx.x] += scov[threadIdx.x + o];
+			} else {
+				sc[threadIdx.x] = (double)((float)sc[threadIdx.x] + (float)sc[threadIdx.x + o]);
+				scov[threadIdx.x] = (double)((float)scov[threadIdx.x] + (float)scov[threadIdx.x + o]);
+			}
+			smx[threadIdx.x] = fmax(smx[threadIdx.x], smx[threadIdx.x + o]);
+			snc[threadIdx.x] += snc[threadIdx.x + o];
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		pix_out[p] = CountAcc{sc[0], scov[0], smx[0], snc[0]};
+		if (per_pixel) per_pixel[p] = scov[0];
+	}
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(SK_BS) k_count_final(const CountAcc *pix_out, int np, gf_count_result *res)
+{
+	// the uncovered total is reduced in the mode's arithmetic (the precision
+	// study compares it); covered totals, maxima and cell counts in FP64
+	Acc<MODE> acc;
+	double cov = 0.0, mx = 0.0, nc = 0.0;
+	for (int p = threadIdx.x; p < np; p += blockDim.x) {
+		CountAcc a = pix_out[p];
+		acc.add(a.c, 1.0f);
+		cov += a.cov;
+		mx = fmax(mx, a.mx);
+		nc += a.ncell;
+	}
+	__shared__ double sc[SK_BS], scov[SK_BS], smx[SK_BS], snc[SK_BS];
+	sc[threadIdx.x] = acc.total();
+	scov[threadIdx.x] = cov;
+	smx[threadIdx.x] = mx;
+	snc[threadIdx.x] = nc;
+	__syncthreads();
+	for (int o = SK_BS / 2; o > 0; o >>= 1) {
+		if ((int)threadIdx.x < o) {
+			if (MODE == GF_ACC_FP64) sc[threadIdx.x] += sc[threadIdx.x + o];
+			else sc[threadIdx.x] = (double)((float)sc[threadIdx.x] + (float)sc[threadIdx.x + o]);
+			scov[threadIdx.x] += scov[threadIdx.x + o];
+			smx[threadIdx.x] = fmax(smx[threadIdx.x], smx[threadIdx.x + o]);
+			snc[threadIdx.x] += snc[threadIdx.x + o];
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		res->nstarsExpectedToGenerate = sc[0];
+		res->nstarsExpected = scov[0];
+		res->maxCount = smx[0];
+		res->ncells = snc[0];
+	}
+}
+
+// ---------------------------------------------------------------------------
+// Diagonal crawl of the reference (skygen/skygen.cu.h:231-358).
+// ---------------------------------------------------------------------------
+GF_D void diag_index_to_ij(int &ilb, int &i, int &j, int &k, const int x, const int y)
+{
+	int kmax = x * y;
+	if (k >= kmax) {
+		ilb += k / kmax;
+		k %= kmax;
+	}
+	int l = x < y ? x : y;
+	int m = x < y ? y : x;
+	int d;
+	if (2 * k < l * (l - 1)) {
+		d = int(0.5 * (double)(gf_sqrtf((float)(1 + 8 * k)) - 1));
+		i = k - d * (d + 1) / 2;
+	} else if (2 * k < l * (2 * m + 1 - l)) {
+		if (x >= y) {
+			int ka = k - y * (y - 1) / 2;
+			d = int(ka / y) + (y - 1);
+			i = (ka % y) + d - (y - 1);
+		} else {
+			int ka = k - x * (x - 1) / 2;
+			d = int(ka / x) + (x - 1);
+			i = ka % x;
+		}
+	} else {
+		int ka = x * y - k - 1;
+		int dd = int(0.5 * (double)(gf_sqrtf((float)(1 + 8 * ka)) - 1));
+		d = (x + y - 2) - dd;
+		i = (x - 1) - (ka - dd * (dd + 1) / 2);
+	}
+	j = d - i;
+}
+
+GF_D bool crawl_advance(int &ilb, int &i, int &j, gf_pencil_beam &pix, const gf_pencil_beam *pixels,
+                        int npixels, const int x, const int y)
+{
+	i++; j--;
+	if (j < 0) {
+		if (i < y) { j += i + 1; i = 0; }
+		else { i -= (y - 1); j += y; }
+	} else if (i >= x) {
+		if (x > y) { i -= y - j - 2; j = y - 1; }
+		else {
+			if (j + 1 < y - x) { i -= x; j += x + 1; }
+			else { i = x - y + j + 2; j = y - 1; }
+		}
+		if (i == x) {
+			ilb++;
+			if (ilb != npixels) pix = pixels[ilb];
+		}
+	} else {
+		return false;
+	}
+	return true;
+}
+
+// Destination of one generated star.
+struct StarSink {
+	gf_skygen_cols c;
+	bool write;
+	uint64_t row;
+};
+
+// One star of draw_stars (skygen/skygen.cu.h:373-456), followed by the
+// clipper's footprint test.
+GF_D void draw_one_star(const gf_skygen_params &sp, const gf_model &m, const gf_extinction &ext,
+                        const gf_footprint &foot, const gf_pencil_beam &pix, float M, int im, float AmMin,
+                        gf_mwc &rng, StarSink &s)
+{
+	float x = pix.X, y = pix.Y;
+	x += pix.dx * (rng.uniform() - 0.5f);
+	y += pix.dx * (rng.uniform() - 0.5f);
+	const uint32_t row = (uint32_t)s.row;
+	if (x * x + y * y > 2.f) {
+		if (s.write) {
+			col<int>(s.c.hidden, row) = 1;
+			col<int>(s.c.projIdx, row) = pix.projIdx;
+			col<float>(s.c.projXY, row, 0) = x; col<float>(s.c.projXY, row, 1) = y;
+			col<double>(s.c.lb, row, 0) = 0.; col<double>(s.c.lb, row, 1) = 0.;
+			col<float>(s.c.M, row) = 0.f; col<float>(s.c.DM, row) = 0.f;
+			col<float>(s.c.XYZ, row, 0) = 0.f; col<float>(s.c.XYZ, row, 1) = 0.f; col<float>(s.c.XYZ, row, 2) = 0.f;
+			col<int>(s.c.comp, row) = m.comp;
+			col<float>(s.c.Am, row) = 0.f; col<float>(s.c.AmInf, row) = 0.f;
+		}
+		return;
+	}
+	double l, b;
+	lambert_deproject(sp.proj[pix.projIdx], l, b, x, y);
+	int outside = outside_footprint(foot, l, b);
+	float dcl = (float)cos(l), dcb = (float)cos(b), dsl = (float)sin(l), dsb = (float)sin(b);
+	l *= DBL_R2D;
+	if (l < 0.) l += 360.;
+	if (l > 360.) l -= 360.;
+	b *= DBL_R2D;
+
+	float Mtmp = M + sp.dM * (rng.uniform() - 0.5f);
+	float mtmp = sp.m0 + sp.dm * ((float)im + rng.uniform() - 0.5f);
+	float DM = mtmp - Mtmp;
+	float D = gf_powf(10.f, 0.2f * DM + 1.f);
+	float X = D * dcl * dcb, Y = D * dsl * dcb, Z = D * dsb;
+	float Am0 = 0.f, Am1 = 0.f;
+	const gf_tex3d &em = pix.projIdx == 0 ? ext.north : ext.south;
+	if (em.data) {
+		Am0 = tex3d(em, x, y, DM);
+		Am1 = tex3d(em, x, y, 100.f);
+	}
+	float AmInf = Am1 > Am0 ? Am1 : Am0;
+	if (AmMin > Am0) AmInf = -AmMin;
+	int hidden = (mtmp + Am0 > sp.m1) ? 1 : 0;
+	hidden |= outside;
+	if (s.write) {
+		col<int>(s.c.projIdx, row) = pix.projIdx;
+		col<float>(s.c.projXY, row, 0) = x;
+		col<float>(s.c.projXY, row, 1) = y;
+		col<double>(s.c.lb, row, 0) = l;
+		col<double>(s.c.lb, row, 1) = b;
+		col<float>(s.c.M, row) = Mtmp;
+		col<float>(s.c.DM, row) = DM;
+		col<float>(s.c.XYZ, row, 0) = X;
+		col<float>(s.c.XYZ, row, 1) = Y;
+		col<float>(s.c.XYZ, row, 2) = Z;
+		col<int>(s.c.comp, row) = m.comp;
+		col<float>(s.c.Am, row) = Am0;
+		col<float>(s.c.AmInf, row) = AmInf;
+		col<int>(s.c.hidden, row) = hidden;
+	}
+}
+
+// kernel<1> of the reference with unlimited output capacity.  Pass 0 counts
+// the stars of each thread, pass 1 regenerates them from the same stream
+// state and writes them at the thread's offset.
+__global__ void __launch_bounds__(64) k_draw_compat(gf_skygen_params sp, gf_model m, const gf_pencil_beam *pixels,
+                                                    gf_extinction ext, gf_footprint foot, uint32_t *st, uint32_t ns,
+                                                    int pass, uint32_t *counts, const uint64_t *offs,
+                                                    gf_skygen_cols out)
+{
+	const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+	if (tid >= sp.nthreads) return;
+	const int block = 10;
+	const int nthreads = sp.nthreads, nm = sp.nm, nM = sp.nM;
+	int ilb = 0, im = 0, iM = 0, k = block * (tid - nthreads), bc = 0;
+	float D = 0.f, Am = 0.f, R = 0.f;
+	gf_pencil_beam pix;
+	gf_mwc rng = gf_mwc_load(st, ns, tid);
+	StarSink s{out, pass == 1, pass == 1 ? offs[tid] : 0};
+	uint32_t n = 0;
+	for (;;) {
+		bool moved;
+		if (bc == 0) {
+			bc = block;
+			k += block * nthreads;
+			diag_index_to_ij(ilb, im, iM, k, nm, nM);
+			if (ilb >= sp.npixels) break;
+			pix = pixels[ilb];
+			moved = true;
+		} else {
+			moved = crawl_advance(ilb, im, iM, pix, pixels, sp.npixels, nm, nM);
+		}
+		bc--;
+		float M = sp.M1 - iM * sp.dM;
+		if (moved) {
+			if (ilb >= sp.npixels) break;
+			float x, y, z;
+			cell_pos(sp, ext, pix, M, im, D, Am, x, y, z);
+			R = model_rho_xyz(m, x, y, z);
+		}
+		if ((sp.dmin != 0.f || sp.dmax != 0.f) && (sp.dmin > D || sp.dmax <= D)) continue;
+		float mm = sp.m0 + sp.dm * im + Am;
+		if (mm > sp.m1) continue;
+		float phi = tex1d(m.lf, M);
+		float rho = phi * R;
+		rho *= m.norm;
+		rho *= D * D * D;
+		rho *= pix.dA * POGSON * sp.dm * sp.dM;
+		unsigned int ndraw = rng.poisson(rho);
+		for (unsigned int q = 0; q < ndraw; q++) {
+			draw_one_star(sp, m, ext, foot, pix, M, im, Am, rng, s);
+			s.row++;
+			n++;
+		}
+	}
+	if (pass == 0) counts[tid] = n;
+	else gf_mwc_store(st, ns, tid, rng);
+}
+
+// ---------------------------------------------------------------------------
+// Cumulative-table draw.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(SK_BS) k_plan(const double *wpk, uint64_t U, uint32_t *st, uint32_t ns,
+                                                uint32_t *counts)
+{
+	uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= ns) return;
+	gf_mwc rng = gf_mwc_load(st, ns, s);
+	for (uint64_t u = s; u < U; u += ns) {
+		double w = wpk[u];
+		uint32_t n = 0;
+		if (w > 0.0) n = rng.poisson((float)w);
+		counts[u] = n;
+	}
+	gf_mwc_store(st, ns, s, rng);
+}
+
+constexpr int EMIT_MAX_NM = 4096;
+
+__global__ void __launch_bounds__(SK_BS) k_emit(gf_skygen_params sp, gf_model m, const gf_pencil_beam *pixels,
+                                                gf_extinction ext, gf_footprint foot, const double *Pg,
+                                                const uint64_t *offs, uint64_t U, uint64_t row_begin,
+                                                uint64_t row_end, uint64_t u_lo, uint32_t *st, uint32_t ns,
+                                                gf_skygen_cols out)
+{
+	extern __shared__ double P[];
+	for (int i = threadIdx.x; i <= sp.nM; i += blockDim.x) P[i] = Pg[i];
+	__syncthreads();
+	uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= ns) return;
+	const int K = sp.nm + sp.nM - 1;
+	uint64_t first = u_lo + ((uint64_t)s + ns - (u_lo % ns)) % ns;
+	gf_mwc rng = gf_mwc_load(st, ns, s);
+	bool touched = false;
+	for (uint64_t u = first; u < U; u += ns) {
+		uint64_t o0 = offs[u], o1 = offs[u + 1];
+		if (o0 >= row_end) break;
+		if (o1 <= row_begin || o1 == o0) continue;
+		touched = true;
+		int p = (int)(u / K), k = (int)(u % K);
+		const gf_pencil_beam pix = pixels[p];
+		float D, Am, x, y, z;
+		dm_bin(sp, ext, pix, k, D, Am, x, y, z);
+		int lo, hi;
+		dm_window(sp, k, D, Am, lo, hi);
+		double base = P[lo], span = P[hi + 1] - P[lo];
+		uint64_t j0 = row_begin > o0 ? row_begin : o0;
+		uint64_t j1 = row_end < o1 ? row_end : o1;
+		StarSink snk{out, true, j0 - row_begin};
+		for (uint64_t j = j0; j < j1; j++) {
+			// absolute-magnitude bin from the cumulative LF window
+			double tgt = base + (double)rng.uniform() * span;
+			int a = lo, b = hi;
+			while (a < b) {
+				int mid = (a + b) >> 1;
+				if (P[mid + 1] > tgt) b = mid; else a = mid + 1;
+			}
+			int iM = a;
+			int im = k - iM;
+			float M = sp.M1 - iM * sp.dM;
+			draw_one_star(sp, m, ext, foot, pix, M, im, Am, rng, snk);
+			snk.row++;
+		}
+	}
+	if (touched) gf_mwc_store(st, ns, s, rng);
+}
+
+__global__ void k_find_unit(const uint64_t *offs, uint64_t U, uint64_t row, uint64_t *res)
+{
+	// largest u with offs[u] <= row (u < U)
+	uint64_t a = 0, b = U;   // offs[a] <= row; search in [a, b)
+	while (b - a > 1) {
+		uint64_t mid = (a + b) / 2;
+		if (offs[mid] <= row) a = mid; else b = mid;
+	}
+	*res = a;
+}
+
+} // namespace
+
+struct gf_draw_plan {
+	gf_skygen_params sp;
+	gf_model model;
+	const gf_pencil_beam *pixels;
+	gf_extinction ext;
+	uint64_t U;
+	uint64_t total;
+	uint32_t *counts;
+	uint64_t *offs;
+	double *P;
+	uint64_t *tmp;
+};
+
+extern "C" {
+
+int gf_skygen_count(const gf_skygen_params *sp, const gf_model *model,
+                    const gf_pencil_beam *pixels, const gf_extinction *ext,
+                    int acc_mode, gf_count_result *out, double *per_pixel,
+                    double *wpk, void *stream)
+{
+	GF_CHECK(sp && model && pixels && out && sp->npixels >= 0 && sp->nm > 0 && sp->nM > 0, GF_E_ARG);
+	GF_CHECK(acc_mode >= 0 && acc_mode <= 2, GF_E_ARG);
+	hipStream_t st = gf::as_stream(stream);
+	gf_extinction ex{};
+	if (ext) ex = *ext;
+	if (sp->npixels == 0) { *out = gf_count_result{0, 0, 0, 0}; return 0; }
+	LFTables t;
+	int rc = lf_tables(*sp, *model, t, st);
+	if (rc) return rc;
+	void *p;
+	rc = gf::scratch(GF_SLOT_PARTIALS, (size_t)sp->npixels * sizeof(CountAcc) + sizeof(gf_count_result) + 64, &p);
+	if (rc) return rc;
+	CountAcc *pix_out = (CountAcc *)p;
+	gf_count_result *dres = (gf_count_result *)((char *)p + (size_t)sp->npixels * sizeof(CountAcc));
+	switch (acc_mode) {
+	case GF_ACC_FP64:
+		k_count<GF_ACC_FP64><<<sp->npixels, SK_BS, 0, st>>>(*sp, *model, pixels, ex, t, pix_out, per_pixel, wpk);
+		GF_LAUNCHED();
+		k_count_final<GF_ACC_FP64><<<1, SK_BS, 0, st>>>(pix_out, sp->npixels, dres);
+		break;
+	case GF_ACC_KAHAN32:
+		k_count<GF_ACC_KAHAN32><<<sp->npixels, SK_BS, 0, st>>>(*sp, *model, pixels, ex, t, pix_out, per_pixel, wpk);
+		GF_LAUNCHED();
+		k_count_final<GF_ACC_KAHAN32><<<1, SK_BS, 0, st>>>(pix_out, sp->npixels, dres);
+		break;
+	default:
+		k_count<GF_ACC_FP32><<<sp->npixels, SK_BS, 0, st>>>(*sp, *model, pixels, ex, t, pix_out, per_pixel, wpk);
+		GF_LAUNCHED();
+		k_count_final<GF_ACC_FP32><<<1, SK_BS, 0, st>>>(pix_out, sp->npixels, dres);
+		break;
+	}
+	GF_LAUNCHED();
+	GF_HIP(hipMemcpyAsync(out, dres, sizeof(gf_count_result), hipMemcpyDeviceToHost, st));
+	GF_HIP(hipStreamSynchronize(st));
+	return 0;
+}
+
+int gf_skygen_draw_compat(const gf_skygen_params *sp, const gf_model *model,
+                          const gf_pencil_beam *pixels, const gf_extinction *ext,
+                          const gf_footprint *foot, gf_rng *rng,
+                          const gf_skygen_cols *out, uint64_t capacity,
+                          uint64_t *nstars, void *stream)
+{
+	GF_CHECK(sp && model && pixels && rng && rng->state && out && nstars, GF_E_ARG);
+	GF_CHECK(sp->nthreads > 0 && (uint32_t)sp->nthreads <= rng->nstreams, GF_E_STREAMS);
+	hipStream_t st = gf::as_stream(stream);
+	gf_extinction ex{};
+	if (ext) ex = *ext;
+	gf_footprint fp{};
+	if (foot) fp = *foot;
+	const int nt = sp->nthreads;
+	void *pc, *po;
+	int rc = gf::scratch(GF_SLOT_COMPAT_COUNTS, (size_t)nt * sizeof(uint32_t), &pc);
+	if (rc) return rc;
+	rc = gf::scratch(GF_SLOT_COMPAT_STATE, ((size_t)nt + 1) * sizeof(uint64_t), &po);
+	if (rc) return rc;
+	uint32_t *counts = (uint32_t *)pc;
+	uint64_t *offs = (uint64_t *)po;
+	unsigned grid = gf::blocks_for(nt, 64);
+	k_draw_compat<<<grid, 64, 0, st>>>(*sp, *model, pixels, ex, fp, rng->state, rng->nstreams, 0, counts, offs, *out);
+	GF_LAUNCHED();
+	rc = gf::exclusive_scan_u32_u64(counts, offs, nt, st);
+	if (rc) return rc;
+	uint64_t total = 0;
+	GF_HIP(hipMemcpyAsync(&total, offs + nt, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+	GF_HIP(hipStreamSynchronize(st));
+	*nstars = total;
+	if (total > capacity) return GF_E_CAPACITY;
+	k_draw_compat<<<grid, 64, 0, st>>>(*sp, *model, pixels, ex, fp, rng->state, rng->nstreams, 1, counts, offs, *out);
+	GF_LAUNCHED();
+	return 0;
+}
+
+int gf_skygen_plan(const gf_skygen_params *sp, const gf_model *model,
+                   const gf_pencil_beam *pixels, const gf_extinction *ext,
+                   const double *wpk, gf_rng *rng, gf_draw_plan **plan,
+                   uint64_t *nstars, void *stream)
+{
+	GF_CHECK(sp && model && pixels && wpk && rng && rng->state && plan && nstars, GF_E_ARG);
+	GF_CHECK(sp->nM <= EMIT_MAX_NM, GF_E_ARG);
+	hipStream_t st = gf::as_stream(stream);
+	gf_draw_plan *pl = new gf_draw_plan{};
+	pl->sp = *sp;
+	pl->model = *model;
+	pl->pixels = pixels;
+	if (ext) pl->ext = *ext;
+	const uint64_t K = (uint64_t)(sp->nm + sp->nM - 1);
+	pl->U = (uint64_t)sp->npixels * K;
+	hipError_t e = hipMalloc((void **)&pl->counts, (pl->U + 1) * sizeof(uint32_t));
+	if (e == hipSuccess) e = hipMalloc((void **)&pl->offs, (pl->U + 1) * sizeof(uint64_t));
+	if (e == hipSuccess) e = hipMalloc((void **)&pl->P, ((size_t)sp->nM + 1) * sizeof(double));
+	if (e == hipSuccess) e = hipMalloc((void **)&pl->tmp, 2 * sizeof(uint64_t));
+	if (e != hipSuccess) { gf_skygen_plan_free(pl); return (int)e; }
+	LFTables t;
+	int rc = lf_tables(*sp, *model, t, st);
+	if (rc) { gf_skygen_plan_free(pl); return rc; }
+	e = hipMemcpyAsync(pl->P, t.P, ((size_t)sp->nM + 1) * sizeof(double), hipMemcpyDeviceToDevice, st);
+	if (e != hipSuccess) { gf_skygen_plan_free(pl); return (int)e; }
+	if (pl->U > 0) {
+		k_plan<<<gf::blocks_for(rng->nstreams, SK_BS), SK_BS, 0, st>>>(wpk, pl->U, rng->state, rng->nstreams, pl->counts);
+		e = hipGetLastError();
+		if (e != hipSuccess) { gf_skygen_plan_free(pl); return (int)e; }
+	}
+	rc = gf::exclusive_scan_u32_u64(pl->counts, pl->offs, pl->U, st);
+	if (rc) { gf_skygen_plan_free(pl); return rc; }
+	e = hipMemcpyAsync(&pl->total, pl->offs + pl->U, sizeof(uint64_t), hipMemcpyDeviceToHost, st);
+	if (e == hipSuccess) e = hipStreamSynchronize(st);
+	if (e != hipSuccess) { gf_skygen_plan_free(pl); return (int)e; }
+	*nstars = pl->total;
+	*plan = pl;
+	return 0;
+}
+
+int gf_skygen_emit(const gf_draw_plan *plan, uint64_t row_begin, uint64_t row_end,
+                   const gf_footprint *foot, gf_rng *rng,
+                   const gf_skygen_cols *out, void *stream)
+{
+	GF_CHECK(plan && rng && rng->state && out && row_begin <= row_end && row_end <= plan->total, GF_E_ARG);
+	if (row_begin == row_end) return 0;
+	hipStream_t st = gf::as_stream(stream);
+	gf_footprint fp{};
+	if (foot) fp = *foot;
+	k_find_unit<<<1, 1, 0, st>>>(plan->offs, plan->U, row_begin, plan->tmp);
+	GF_LAUNCHED();
+	uint64_t u_lo = 0;
+	GF_HIP(hipMemcpyAsync(&u_lo, plan->tmp, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+	GF_HIP(hipStreamSynchronize(st));
+	size_t shb = ((size_t)plan->sp.nM + 1) * sizeof(double);
+	k_emit<<<gf::blocks_for(rng->nstreams, SK_BS), SK_BS, shb, st>>>(
+	    plan->sp, plan->model, plan->pixels, plan->ext, fp, plan->P, plan->offs, plan->U, row_begin, row_end, u_lo,
+	    rng->state, rng->nstreams, *out);
+	GF_LAUNCHED();
+	return 0;
+}
+
+int gf_skygen_plan_free(gf_draw_plan *plan)
+{
+	if (!plan) return 0;
+	if (plan->counts) (void)hipFree(plan->counts);
+	if (plan->offs) (void)hipFree(plan->offs);
+	if (plan->P) (void)hipFree(plan->P);
+	if (plan->tmp) (void)hipFree(plan->tmp);
+	delete plan;
+	return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,444 @@
+/*
+ * galfast_hip.h -- C ABI of the MI355X (gfx950) catalog hot path.
+ *
+ * Every entry point takes plain pointers, sizes and POD parameter blocks.
+ * Device pointers are HIP device addresses; `stream` is a hipStream_t (NULL
+ * selects the null stream).  Functions return 0 on success, a positive
+ * hipError_t value when a HIP call failed, or a negative GF_E* code for an
+ * argument error.  The parameter blocks keep the field order and types of
+ * the reference's own structures so that a reference-side module fills them
+ * exactly as it fills its kernel arguments today.
+ *
+ * Reference paths are relative to the galfast source tree.
+ */
+#ifndef GALFAST_HIP_H
+#define GALFAST_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------- */
+/* Error codes                                                             */
+/* ---------------------------------------------------------------------- */
+#define GF_OK              0
+#define GF_E_ARG         (-1)   /* invalid argument (NULL pointer, bad size) */
+#define GF_E_CAPACITY    (-2)   /* output table too small */
+#define GF_E_STREAMS     (-3)   /* more 256-row chunks than RNG streams */
+#define GF_E_CONFIG      (-4)   /* configuration file missing a key or malformed */
+#define GF_E_IO          (-5)   /* file could not be read */
+#define GF_E_NODEVICE    (-6)   /* no HIP device */
+#define GF_E_BUDGET      (-7)   /* request exceeds the device-memory budget */
+#define GF_E_STATE       (-8)   /* call out of order */
+
+/* Human-readable text for an error code (static storage). */
+const char *gf_strerror(int code);
+
+/* ---------------------------------------------------------------------- */
+/* Data model                                                              */
+/* ---------------------------------------------------------------------- */
+
+/* Structure-of-arrays column view, the device side of a column<T>.
+ * Element (row r, plane j) lives at (char*)ptr + j*pitch + r*elem.
+ * Replaces gptr<T,2> = { T *ptr; uint32_t extent[1]; }
+ * (common/cux.h:183-216, common/column.h:131-136).  pitch is a multiple
+ * of 128 bytes for multi-plane columns (common/otable.h:275-288). */
+typedef struct gf_col {
+    void    *ptr;
+    uint32_t pitch;   /* bytes between planes */
+    uint32_t width;   /* number of planes */
+    uint32_t elem;    /* bytes per element (4 or 8) */
+    uint32_t reserved;
+} gf_col;
+
+/* Row range of one kernel call: replaces kernel_state / otable_ks
+ * (common/gpu.h:124-156).  Thread t of the reference owns rows
+ * [begin + step*t, min(begin + step*(t+1), end)) and RNG stream t. */
+typedef struct gf_rows {
+    uint32_t begin;
+    uint32_t step;    /* rows per stream; 256 in the reference */
+    uint32_t end;
+    uint32_t shmem;   /* unused, kept for layout parity */
+} gf_rows;
+
+/* Component mask: bit_map (common/column.h:51-113). */
+typedef struct gf_bitmap {
+    uint32_t bits[2];
+} gf_bitmap;
+
+/* Device-resident multiply-with-carry streams: three planes [a | c | x] of
+ * nstreams words each (common/cuda_rng.h:45-69, 414-500). */
+typedef struct gf_rng {
+    uint32_t *state;     /* device pointer, 3*nstreams words */
+    uint32_t  nstreams;
+    uint32_t  reserved;
+} gf_rng;
+
+/* ---------------------------------------------------------------------- */
+/* Runtime                                                                 */
+/* ---------------------------------------------------------------------- */
+
+/* Select the HIP device used by later calls (cux_init, common/cux.cpp:713). */
+int gf_init(int device);
+/* Wait for all work on `stream` (cudaThreadSynchronize in gpu.h:253). */
+int gf_sync(void *stream);
+/* Device memory helpers for callers without their own allocator. */
+int gf_malloc(void **ptr, size_t bytes);
+int gf_free(void *ptr);
+int gf_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int gf_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int gf_memset(void *dst, int value, size_t bytes, void *stream);
+
+/* ---------------------------------------------------------------------- */
+/* Random streams                                                          */
+/* ---------------------------------------------------------------------- */
+
+/* Seed derivation of gpu_rng_t::persistent_rng::get (common/cux.cpp:404-410)
+ * with rng_gsl_t (common/gpu.h:362-382): the first MT19937 output of
+ * `pipeline_seed`, divided by 2^32, rounded to float, times 2^24, truncated. */
+uint32_t gf_seed_from_pipeline_seed(uint32_t pipeline_seed);
+
+/* mwc_impl::srand (common/cuda_rng.h:466-497), on the host.
+ * primes: multipliers (first column of safeprimes32.txt), at least nstreams.
+ * nstreams is rounded up to a multiple of 64.  out_state receives
+ * 3*nstreams words [a | c | x].  Returns the rounded stream count, or a
+ * negative error code. */
+int gf_mwc_seed_host(uint32_t seed, const uint32_t *primes, uint32_t nprimes,
+                     uint32_t nstreams, uint32_t *out_state);
+
+/* Allocate / copy device stream state (rng_base::upload/download,
+ * common/cuda_rng.h:74-103). */
+int gf_rng_alloc(gf_rng *rng, uint32_t nstreams);
+int gf_rng_upload(gf_rng *rng, const uint32_t *host_state, void *stream);
+int gf_rng_download(const gf_rng *rng, uint32_t *host_state, void *stream);
+int gf_rng_free(gf_rng *rng);
+
+/* Sampler test kernel: stream s produces n values of the chosen sampler,
+ * written to out[i*nstreams + s] (float for uniform/gaussian, uint32 for
+ * poisson).  kind: 0 uniform, 1 gaussian(param), 2 poisson(param),
+ * 3 poisson(param * (1 + s)) -- a sweep of means across streams. */
+int gf_mwc_sample(gf_rng *rng, int kind, float param, uint32_t n, void *out, void *stream);
+
+/* ---------------------------------------------------------------------- */
+/* Per-star module kernels (the GPU twin of cpulaunch_/gpulaunch_ pairs)   */
+/* ---------------------------------------------------------------------- */
+
+/* os_FeH_data (modules/FeH_gpu.cu.h:27-36). */
+typedef struct gf_FeH_data {
+    float A[2], sigma[3], offs[3];
+    float Hmu, muInf, DeltaMu;
+    gf_bitmap comp_thin, comp_thick, comp_halo;
+    float M[3][3];
+    float T[3];
+} gf_FeH_data;
+
+/* os_GaussianFeH_data (modules/GaussianFeH_gpu.cu.h:27-36). */
+typedef struct gf_GaussianFeH_data {
+    int   ncomp;
+    float mean[10];
+    float sigma[10];
+    float pcum[10];
+} gf_GaussianFeH_data;
+
+/* Bond2010::vmoments and os_Bond2010_data (modules/Bond2010_gpu.cu.h:29-52):
+ * each quantity is a + b|Z|^c + d R^e, stored as 5 floats. */
+typedef struct gf_vmoments {
+    float m[3][5];
+    float ss[6][5];
+} gf_vmoments;
+
+typedef struct gf_Bond2010_data {
+    gf_bitmap comp_thin, comp_thick, comp_halo;
+    float fk;
+    gf_vmoments disk_1, disk_2;
+    gf_vmoments halo;
+    float M[3][3];
+    float T[3];
+} gf_Bond2010_data;
+
+/* os_vel2pm_data (modules/vel2pm_gpu.cu.h:27-35). coordsys: 0 GAL, 1 EQU. */
+typedef struct gf_vel2pm_data {
+    int   coordsys;
+    float vLSR, u0, v0, w0;
+    float Rg;
+} gf_vel2pm_data;
+
+/* os_photometry_data (modules/photometry.h:27-33) plus the colour tables the
+ * reference binds as textures color0..3 / cflags0..3.  Table i holds colours
+ * 4i..4i+3 as float4 texels at [m*nFeH + f]; FeH of texel f is
+ * (f - 0.5 + 0.5)/tcFeH[1] + tcFeH[0] (texcoord_from_range convention). */
+typedef struct gf_photometry_data {
+    uint32_t ncolors, bidx;
+    float    reddening[17];
+    int32_t  nFeH, nMr;
+    float    tcFeH[2], tcMr[2];
+    const float *color[4];   /* device, float4 texels */
+    const float *cflags[4];  /* device, float4 texels */
+} gf_photometry_data;
+
+/* Tabulated 1D function on a uniform grid, sampled with the reference's
+ * texture convention xi = (x - tc[0])*tc[1] + 0.5 and linear filtering with
+ * clamping (common/cux.h:849-855). */
+typedef struct gf_tex1d {
+    const float *data;   /* device */
+    int32_t n;
+    float   tc[2];
+} gf_tex1d;
+
+/* unresolvedMultiples tables (modules/unresolvedMultiples_host.cpp:140-180).
+ * algo: 1 LF_M2_gt_M1, 2 LF, 3 equal_mass. */
+typedef struct gf_multiples_data {
+    gf_tex1d secProb, cumLF, invCumLF;
+    int32_t algo;
+} gf_multiples_data;
+
+/* Photometric-error curves sigma(m) as linear splines over knots
+ * (modules/photometricErrors_host.cpp:112-131).  Band b uses knots
+ * [koff[b], koff[b+1]); nbands <= 17; a band with no knots gets no error. */
+typedef struct gf_photoerr_data {
+    int32_t nbands;
+    int32_t koff[18];
+    const double *kx;   /* device knots: magnitude */
+    const double *ky;   /* device knots: sigma */
+} gf_photoerr_data;
+
+/* os_FeH_kernel (modules/FeH_gpu.cu.h:43-87; called at FeH_host.cpp:76). */
+int gf_os_FeH_kernel(const gf_rows *ks, const gf_FeH_data *par, gf_rng *rng,
+                     gf_col comp, gf_col hidden, gf_col XYZ, gf_col FeH, void *stream);
+
+/* os_GaussianFeH_kernel (modules/GaussianFeH_gpu.cu.h:42-73;
+ * called at GaussianFeH_host.cpp:74). */
+int gf_os_GaussianFeH_kernel(const gf_rows *ks, gf_bitmap applyToComponents,
+                             const gf_GaussianFeH_data *par, gf_rng *rng,
+                             gf_col comp, gf_col hidden, gf_col XYZ,
+                             gf_col FeH_comp, gf_col FeH, void *stream);
+
+/* os_Bond2010_kernel (modules/Bond2010_gpu.cu.h:164-218; called at
+ * Bond2010_host.cpp:88; the parameters travel as a __constant__ there). */
+int gf_os_Bond2010_kernel(const gf_rows *ks, const gf_Bond2010_data *par, gf_rng *rng,
+                          gf_col comp, gf_col hidden, gf_col XYZ, gf_col vcyl, void *stream);
+
+/* os_vel2pm_kernel (modules/vel2pm_gpu.cu.h:253-301; called at
+ * vel2pm_host.cpp:72).  Draws no random numbers. */
+int gf_os_vel2pm_kernel(const gf_rows *ks, const gf_vel2pm_data *par,
+                        gf_col lb, gf_col XYZ, gf_col vcyl, gf_col pmout,
+                        gf_col hidden, void *stream);
+
+/* os_gal2other_kernel (modules/gal2other_gpu.cu.h:62-96; called at
+ * gal2other_host.cpp:62).  coordsys: 1 EQU. */
+int gf_os_gal2other_kernel(const gf_rows *ks, int coordsys, gf_col hidden,
+                           gf_col lb, gf_col out, void *stream);
+
+/* os_photometry_kernel (modules/photometry_gpu.cu.h:87-174; called at
+ * photometry_host.cpp:391). */
+int gf_os_photometry_kernel(const gf_rows *ks, gf_bitmap applyToComponents,
+                            const gf_photometry_data *par,
+                            gf_col Am, gf_col flags, gf_col DM, gf_col Mr,
+                            int nabsmag, gf_col mags, gf_col FeH, gf_col comp,
+                            gf_col hidden, void *stream);
+
+/* os_unresolvedMultiples_kernel (modules/unresolvedMultiples_gpu.cu.h:80-123;
+ * called at unresolvedMultiples_host.cpp:103). */
+int gf_os_unresolvedMultiples_kernel(const gf_rows *ks, gf_bitmap applyToComponents,
+                                     const gf_multiples_data *par, gf_rng *rng,
+                                     int nabsmag, gf_col M, gf_col Msys,
+                                     gf_col ncomp, gf_col comp, gf_col hidden,
+                                     void *stream);
+
+/* os_photometricErrors::process (modules/photometricErrors_host.cpp:112-131).
+ * The reference runs this on the host with the GSL seeder stream; this
+ * kernel draws the Gaussian errors from the per-chunk MWC streams instead
+ * (one stream per `ks->step` rows, bands in order within each row chunk). */
+int gf_photometric_errors_kernel(const gf_rows *ks, const gf_photoerr_data *par,
+                                 gf_rng *rng, gf_col magTrue, gf_col magObs,
+                                 gf_col hidden, void *stream);
+
+/* ---------------------------------------------------------------------- */
+/* Fused column chain                                                      */
+/* ---------------------------------------------------------------------- */
+
+/* One pass over a row range running, for each 256-row chunk, the enabled
+ * modules in pipeline order (pipeline.h:75-89): FeH or GaussianFeH,
+ * unresolvedMultiples, Bond2010, photometry, photometric errors, vel2pm,
+ * gal2other.  Each module consumes its random numbers from the chunk's
+ * stream in the same order as the unfused kernels, so the outputs are
+ * identical to calling the per-module entry points one after another.
+ * Disabled modules have a NULL parameter pointer. */
+typedef struct gf_chain {
+    const gf_FeH_data         *feh;
+    const gf_GaussianFeH_data *gfeh;
+    gf_bitmap                  gfeh_comps;
+    const gf_multiples_data   *mult;
+    gf_bitmap                  mult_comps;
+    const gf_Bond2010_data    *bond;
+    const gf_photometry_data  *phot;
+    gf_bitmap                  phot_comps;
+    const gf_photoerr_data    *perr;
+    const gf_vel2pm_data      *v2pm;
+    int32_t                    gal2other_coordsys;  /* 0 disables */
+    int32_t                    pad;
+} gf_chain;
+
+/* Column set of the catalog table (names as in otable.cpp:714-756). */
+typedef struct gf_table {
+    gf_col lb, projIdx, projXY, DM, absmag, XYZ, Am, AmInf, comp, hidden;
+    gf_col FeH, FeH_comp, absmagSys, absmagSysNcomp, vcyl, mags, photoFlags,
+           obsMags, pm, radec;
+} gf_table;
+
+int gf_module_chain(const gf_rows *ks, const gf_chain *chain, gf_rng *rng,
+                    const gf_table *t, void *stream);
+
+/* ---------------------------------------------------------------------- */
+/* Sky generator (skygen)                                                  */
+/* ---------------------------------------------------------------------- */
+
+/* lambert (skygen/skygen.h:108-147): pole (l0, b0) in radians plus the
+ * float cosines/sines of the pole. */
+typedef struct gf_lambert {
+    double l0, b0;
+    float  cl, cb, sl, sb;
+} gf_lambert;
+
+/* pencilBeam (skygen/skygen.h:89-103): direction cosines of the pixel
+ * centre, covered fraction, linear size (rad), area (sr), projection index,
+ * Lambert coordinates and the per-beam extinction index. */
+typedef struct gf_pencil_beam {
+    float   cl, cb, sl, sb;
+    float   coveredFraction;
+    float   dx, dA;
+    int32_t projIdx;
+    float   X, Y;
+    int32_t extIdx;
+    int32_t pad;
+} gf_pencil_beam;
+
+/* skygenParams (skygen/skygen.h:152-165). */
+typedef struct gf_skygen_params {
+    float   m0, m1, dm;
+    float   M0, M1, dM;
+    float   dmin, dmax;
+    int32_t npixels, nm, nM;
+    int32_t nthreads;
+    int32_t stopstars;
+    int32_t pad;
+    gf_lambert proj[2];
+} gf_skygen_params;
+
+#define GF_MODEL_EXPDISK   1
+#define GF_MODEL_POWERLAW  2
+
+/* One density model (skygen/model_expDisk.h:39-124,
+ * skygen/model_powerLawEllipsoid.h:30-108) with its luminosity function
+ * texture (1024 texels, skygen/model_lib.h:37-95). */
+typedef struct gf_model {
+    int32_t kind;
+    int32_t comp;            /* sequential component index written to `comp` */
+    float   f, l, h, r_cut2; /* exponential disk */
+    float   n, ba, ca;       /* power-law ellipsoid */
+    float   rminSq, rmaxSq;
+    float   T[3];
+    float   M[3][3];
+    float   norm;            /* overall density normalisation (setDensityNorm) */
+    gf_tex1d lf;
+} gf_model;
+
+/* Extinction: ext_beam minimum-extinction cube and north/south maps
+ * (skygen/skygen.cu.h:199-226).  A NULL data pointer means zero extinction. */
+typedef struct gf_tex3d {
+    const float *data;   /* device, [k][j][i] */
+    int32_t nx, ny, nz;
+    float   tc[3][2];
+} gf_tex3d;
+
+typedef struct gf_extinction {
+    gf_tex3d north, south, beam;
+} gf_extinction;
+
+/* Footprint predicate used to hide stars drawn outside the requested area
+ * (the clipper, skygen/os_skygen.cpp:999-1044).  kind: 0 none, 1 rect in
+ * Galactic l/b (degrees, with |b| >= bmin), 2 beam (centre l, b, radius). */
+typedef struct gf_footprint {
+    int32_t kind;
+    int32_t pad;
+    double  l0, l1, b0, b1, bmin;   /* rect, degrees */
+    double  cl, cb, sl, sb, cosr;   /* beam centre direction and cos(radius) */
+} gf_footprint;
+
+/* Count-pass results (skygen/skyconfig_impl.h:112-184). */
+typedef struct gf_count_result {
+    double nstarsExpectedToGenerate;   /* sum over pixels at full pixel area */
+    double nstarsExpected;             /* covered-fraction weighted */
+    double maxCount;                   /* largest single-cell expectation */
+    double ncells;                     /* cells inside flux and distance limits */
+} gf_count_result;
+
+/* Accumulation variants for the count integral. */
+#define GF_ACC_FP64    0
+#define GF_ACC_KAHAN32 1
+#define GF_ACC_FP32    2
+
+/* Expected star counts for one model, in one pass over all (pixel, DM-bin)
+ * pairs.  pixels: device array of npixels beams.  per_pixel (optional,
+ * device, npixels doubles) receives covered-fraction weighted counts per
+ * pixel.  wpk (optional, device, npixels*(nm+nM-1) doubles) receives the
+ * expectation of every (pixel, DM-bin) pair for the draw.
+ * Replaces skygenHost<T>::integrateCounts (skygen/skyconfig_impl.h:322-429)
+ * and the count branch of skygenGPU<T>::kernel<0> (skygen/skygen.cu.h:469-618). */
+int gf_skygen_count(const gf_skygen_params *sp, const gf_model *model,
+                    const gf_pencil_beam *pixels, const gf_extinction *ext,
+                    int acc_mode, gf_count_result *out, double *per_pixel,
+                    double *wpk, void *stream);
+
+/* Skygen output columns (skygen/skygen.h:170-177). */
+typedef struct gf_skygen_cols {
+    gf_col lb, projIdx, projXY, DM, M, XYZ, Am, AmInf, comp, hidden;
+} gf_skygen_cols;
+
+/* Reference-order draw: the diagonal crawl of skygenGPU<T>::kernel<1>
+ * (skygen/skygen.cu.h:366-457, 469-618) with `sp->nthreads` threads, each
+ * bound to RNG stream tid, Poisson draw per cell, 2 or 4 uniforms per star.
+ * Stars are written in (thread, crawl) order instead of atomicAdd order.
+ * Needs sp->nthreads * 12 bytes of scratch plus the output capacity; the
+ * number of stars is returned in *nstars; GF_E_CAPACITY if it exceeds
+ * `capacity` (nothing is written then, the streams are left unchanged). */
+int gf_skygen_draw_compat(const gf_skygen_params *sp, const gf_model *model,
+                          const gf_pencil_beam *pixels, const gf_extinction *ext,
+                          const gf_footprint *foot, gf_rng *rng,
+                          const gf_skygen_cols *out, uint64_t capacity,
+                          uint64_t *nstars, void *stream);
+
+/* Draw plan for the cumulative-table draw below: per (pixel, DM-bin) star
+ * counts and their exclusive prefix sums, device resident. */
+typedef struct gf_draw_plan gf_draw_plan;
+
+/* Poisson-draw the star count of every (pixel, DM-bin) pair from the
+ * expectations `wpk` written by gf_skygen_count.  Unit u = p*K + k is drawn
+ * from stream u mod nstreams, units of one stream in increasing u.
+ * Returns the total in *nstars. */
+int gf_skygen_plan(const gf_skygen_params *sp, const gf_model *model,
+                   const gf_pencil_beam *pixels, const gf_extinction *ext,
+                   const double *wpk, gf_rng *rng, gf_draw_plan **plan,
+                   uint64_t *nstars, void *stream);
+
+/* Emit the stars of rows [row_begin, row_end) of a plan (a chunk of the
+ * catalog) into `out` starting at row 0: each star's absolute-magnitude bin
+ * is drawn from the cumulative luminosity-function window of its (pixel,
+ * DM-bin) pair (FP64 prefix table), then position, M, m, DM, D, XYZ,
+ * extinction and the footprint flag are drawn as in draw_stars.  Chunks
+ * must be emitted in increasing row order (each stream continues where the
+ * previous chunk left it). */
+int gf_skygen_emit(const gf_draw_plan *plan, uint64_t row_begin, uint64_t row_end,
+                   const gf_footprint *foot, gf_rng *rng,
+                   const gf_skygen_cols *out, void *stream);
+
+int gf_skygen_plan_free(gf_draw_plan *plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GALFAST_HIP_H */

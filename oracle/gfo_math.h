@@ -1,0 +1,124 @@
+/*
+ * ORACLE -- test infrastructure only.  Never linked into the product.
+ *
+ * Single-precision transcendentals of the oracle.  The reference calls the
+ * platform expf/logf/tanf/powf/exp10f/log10f/sinf/cosf (common/cuda_rng.h,
+ * skygen/model_expDisk.h and the module kernels).  The oracle evaluates each one in
+ * IEEE double and rounds once to float:
+ *   exp: k = rint(x/ln2), r = x - k*ln2 (two-part ln2, fma), e^r by its
+ *        Taylor series to r^13/13!, times 2^k;
+ *   log: x = 2^e m, m in [1/sqrt2, sqrt2), log m = 2 atanh((m-1)/(m+1)) by
+ *        its series to z^12/25;
+ *   sin/cos: k = rint(x/(pi/2)), r = x - k*pi/2 (two-part, fma), Taylor
+ *        series to degree 17/18, quadrant by k mod 4.
+ * Pinned in tests/test_oracle_math.py against glibc (at most one float ulp,
+ * almost always exact) and by identities (exp(log x) = x, sin^2+cos^2 = 1).
+ */
+#ifndef GFO_MATH_H
+#define GFO_MATH_H
+
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+static inline double gfo_bits2d(uint64_t u) { double d; memcpy(&d, &u, 8); return d; }
+static inline uint64_t gfo_d2bits(double d) { uint64_t u; memcpy(&u, &d, 8); return u; }
+
+#define GFO_LN2_HI 6.93147180369123816490e-01
+#define GFO_LN2_LO 1.90821492927058770002e-10
+#define GFO_PIO2_HI 1.57079632679489655800e+00
+#define GFO_PIO2_LO 6.12323399573676603587e-17
+
+static inline double gfo_pow2i(int k) { return gfo_bits2d((uint64_t)(k + 1023) << 52); }
+
+static inline double gfo_exp(double x)
+{
+	if (x != x) return x;
+	if (x > 709.0) return INFINITY;
+	if (x < -745.0) return 0.0;
+	double kd = rint(x * 1.44269504088896338700e+00);
+	double r = fma(-kd, GFO_LN2_HI, x);
+	r = fma(-kd, GFO_LN2_LO, r);
+	static const double inv_fact[14] = {
+		1.0, 1.0, 1.0 / 2.0, 1.0 / 6.0, 1.0 / 24.0, 1.0 / 120.0, 1.0 / 720.0, 1.0 / 5040.0,
+		1.0 / 40320.0, 1.0 / 362880.0, 1.0 / 3628800.0, 1.0 / 39916800.0, 1.0 / 479001600.0,
+		1.0 / 6227020800.0};
+	double p = inv_fact[13];
+	for (int i = 12; i >= 0; i--) p = fma(p, r, inv_fact[i]);
+	int k = (int)kd;
+	if (k > 1023) { p *= gfo_pow2i(1023); k -= 1023; }
+	if (k < -1022) { p *= gfo_pow2i(-1022); k += 1022; }
+	return p * gfo_pow2i(k);
+}
+
+static inline double gfo_log(double x)
+{
+	if (x != x || x < 0.0) return NAN;
+	if (x == 0.0) return -INFINITY;
+	uint64_t u = gfo_d2bits(x);
+	int e = (int)((u >> 52) & 0x7ff);
+	if (e == 0x7ff) return x;
+	if (e == 0) { x *= gfo_pow2i(54); u = gfo_d2bits(x); e = (int)((u >> 52) & 0x7ff) - 54; }
+	e -= 1023;
+	double m = gfo_bits2d((u & 0x000fffffffffffffull) | 0x3ff0000000000000ull);
+	if (m > 1.41421356237309504880) { m *= 0.5; e += 1; }
+	double s = (m - 1.0) / (m + 1.0);
+	double z = s * s;
+	double p = 1.0 / 25.0;
+	for (int n = 11; n >= 0; n--) p = fma(p, z, 1.0 / (double)(2 * n + 1));
+	double lm = 2.0 * s * p;
+	double ed = (double)e;
+	return fma(ed, GFO_LN2_HI, fma(ed, GFO_LN2_LO, lm));
+}
+
+static inline void gfo_sincos(double x, double *sn, double *cs)
+{
+	double kd = rint(x * 6.36619772367581382433e-01);
+	double r = fma(-kd, GFO_PIO2_HI, x);
+	r = fma(-kd, GFO_PIO2_LO, r);
+	double z = r * r;
+	/* sin: r - r^3/3! + ... ; coefficients of z^n in (1/3! - z/5! + ...) */
+	static const double sc[8] = {1.0 / 6.0, -1.0 / 120.0, 1.0 / 5040.0, -1.0 / 362880.0,
+	                             1.0 / 39916800.0, -1.0 / 6227020800.0, 1.0 / 1307674368000.0,
+	                             -1.0 / 355687428096000.0};
+	double ps = sc[7];
+	for (int i = 6; i >= 0; i--) ps = fma(ps, z, sc[i]);
+	double s = fma(-r * z, ps, r);
+	static const double cc[9] = {0.5, -1.0 / 24.0, 1.0 / 720.0, -1.0 / 40320.0, 1.0 / 3628800.0,
+	                             -1.0 / 479001600.0, 1.0 / 87178291200.0, -1.0 / 20922789888000.0,
+	                             1.0 / 6402373705728000.0};
+	double pc = cc[8];
+	for (int i = 7; i >= 0; i--) pc = fma(pc, z, cc[i]);
+	double c = fma(-z, pc, 1.0);
+	int q = ((int)(int64_t)kd) & 3;
+	switch (q) {
+	case 0: *sn = s; *cs = c; break;
+	case 1: *sn = c; *cs = -s; break;
+	case 2: *sn = -s; *cs = -c; break;
+	default: *sn = -c; *cs = s; break;
+	}
+}
+
+static inline float gfo_expf(float x) { return (float)gfo_exp((double)x); }
+static inline float gfo_logf(float x) { return (float)gfo_log((double)x); }
+static inline float gfo_exp10f(float x) { return (float)gfo_exp((double)x * 2.30258509299404590109e+00); }
+static inline float gfo_log10f(float x) { return (float)(gfo_log((double)x) * 4.34294481903251816668e-01); }
+static inline float gfo_sinf(float x) { double s, c; gfo_sincos((double)x, &s, &c); return (float)s; }
+static inline float gfo_cosf(float x) { double s, c; gfo_sincos((double)x, &s, &c); return (float)c; }
+static inline float gfo_tanf(float x) { double s, c; gfo_sincos((double)x, &s, &c); return (float)(s / c); }
+
+static inline float gfo_powf(float x, float y)
+{
+	if (y == 0.0f || x == 1.0f) return 1.0f;
+	if (x != x || y != y) return x + y;
+	if (x == 0.0f) return y > 0.0f ? 0.0f : INFINITY;
+	if (x < 0.0f) {
+		float yi = truncf(y);
+		if (yi != y) return NAN;
+		double r = gfo_exp((double)y * gfo_log(-(double)x));
+		return (float)((((int64_t)yi) & 1) ? -r : r);
+	}
+	return (float)gfo_exp((double)y * gfo_log((double)x));
+}
+
+#endif

@@ -1,0 +1,219 @@
+"""Per-star module kernels on the GPU against the oracle, and the fused chain
+against the unfused sequence."""
+import ctypes as C
+import copy
+
+import numpy as np
+import pytest
+
+import galfast_amd as G
+from galfast_amd import _abi as A
+from galfast_amd.table import DeviceTable, HostTable
+import oracle as O
+from devutil import DevRng, dev_array, stream_ptr, sync
+import gfx
+
+pytestmark = pytest.mark.gpu
+
+N = 300_000     # 1172 chunks of 256 rows
+SEED = 77
+
+
+class Fixture:
+    def __init__(self, n=N, nbands=5):
+        self.host = gfx.synthetic_stars(n, seed=3, nbands=nbands)
+        self.dev = DeviceTable(n, self.host.spec)
+        self.dev.upload(self.host)
+        self.state = gfx.mwc_state(SEED, 65536)
+        self.drng = DevRng(self.state)
+        self.n = n
+        self.nbands = nbands
+        # photometry tables shared by device and oracle
+        rows = gfx.synthetic_photosys(ncolors=nbands - 1)
+        ntex, nF, nM, tc, colors, flags = gfx.photometry_tables(rows, nbands - 1)
+        sz = nF * nM * 4
+        self.phot_host_arrays = [colors[i * sz:(i + 1) * sz] for i in range(ntex)] + \
+                                [flags[i * sz:(i + 1) * sz] for i in range(ntex)]
+        self.phot_dev_arrays = [dev_array(a) for a in self.phot_host_arrays]
+        red = [1.8739, 1.3788, 1.0, 0.7583, 0.5376, 0.4]
+        self.phot_host = gfx.photometry_params(ntex, nF, nM, tc,
+                                               [a.ctypes.data for a in self.phot_host_arrays[:ntex]],
+                                               [a.ctypes.data for a in self.phot_host_arrays[ntex:]],
+                                               nbands - 1, 2, red)
+        self.phot_dev = gfx.photometry_params(ntex, nF, nM, tc,
+                                              [t.data_ptr() for t in self.phot_dev_arrays[:ntex]],
+                                              [t.data_ptr() for t in self.phot_dev_arrays[ntex:]],
+                                              nbands - 1, 2, red)
+        M, phi = gfx.seeded_lf()
+        (sec, sec_tc), (cum, cum_tc), (inv, inv_tc) = gfx.multiples_tables(M, phi)
+        self.mult_host_arrays = [sec, cum, inv]
+        self.mult_dev_arrays = [dev_array(a) for a in self.mult_host_arrays]
+        tcs = [sec_tc, cum_tc, inv_tc]
+
+        def mult(ptrs, algo):
+            p = A.gf_multiples_data()
+            p.secProb = gfx.tex1d(ptrs[0], len(sec), tcs[0])
+            p.cumLF = gfx.tex1d(ptrs[1], len(cum), tcs[1])
+            p.invCumLF = gfx.tex1d(ptrs[2], len(inv), tcs[2])
+            p.algo = algo
+            return p
+        self.mult = lambda dev, algo: mult([t.data_ptr() for t in self.mult_dev_arrays] if dev else
+                                           [a.ctypes.data for a in self.mult_host_arrays], algo)
+        kx, ky, koff = gfx.photoerr_knots(nbands)
+        self.pe_host_arrays = [kx, ky]
+        self.pe_dev_arrays = [dev_array(kx), dev_array(ky)]
+
+        def perr(ptrs):
+            p = A.gf_photoerr_data()
+            p.nbands = nbands
+            for i, k in enumerate(koff):
+                p.koff[i] = k
+            p.kx, p.ky = ptrs
+            return p
+        self.perr = lambda dev: perr([t.data_ptr() for t in self.pe_dev_arrays] if dev else
+                                     [a.ctypes.data for a in self.pe_host_arrays])
+
+    def rows(self):
+        return A.gf_rows(0, 256, self.n, 0)
+
+    def oracle_chain(self, **mods):
+        ch = A.gf_chain()
+        for k, v in mods.items():
+            if k in ("gfeh_comps", "mult_comps", "phot_comps", "gal2other_coordsys"):
+                setattr(ch, k, v)
+            else:
+                setattr(ch, k, C.pointer(v))
+        t = self.host.table()
+        rc = O.lib().gfo_module_chain(C.byref(self.rows()), C.byref(ch), O.ptr(self.state), 65536, C.byref(t))
+        assert rc == 0
+
+    def compare(self, cols, exact=True, rtol=0.0, atol=0.0):
+        sync()
+        got = self.dev.to_host()
+        vis = self.host.arr("hidden")[0] == 0
+        for c in cols:
+            a, b = got.arr(c)[:, vis], self.host.arr(c)[:, vis]
+            if exact:
+                bad = (a != b) & ~(np.isnan(a) & np.isnan(b))
+                assert not bad.any(), "%s: %d of %d differ, e.g. %r vs %r" % (
+                    c, bad.sum(), bad.size, a[bad][:3], b[bad][:3])
+            else:
+                np.testing.assert_allclose(a, b, rtol=rtol, atol=atol, err_msg=c)
+        assert np.array_equal(self.drng.state(), self.state), "stream states differ"
+
+
+@pytest.fixture
+def fx():
+    return Fixture()
+
+
+def test_FeH(fx):
+    p = gfx.feh_params()
+    G.check(G.lib().gf_os_FeH_kernel(C.byref(fx.rows()), C.byref(p), C.byref(fx.drng.rng), fx.dev.col("comp"),
+                                     fx.dev.col("hidden"), fx.dev.col("XYZ"), fx.dev.col("FeH"), stream_ptr()))
+    fx.oracle_chain(feh=p)
+    fx.compare(["FeH"])
+
+
+def test_GaussianFeH(fx):
+    p = A.gf_GaussianFeH_data()
+    p.ncomp = 3
+    for i, (m, s, w) in enumerate([(-0.5, 0.2, 0.6), (-1.0, 0.3, 0.3), (-1.5, 0.4, 0.1)]):
+        p.mean[i], p.sigma[i] = m, s
+    cs = np.cumsum([0.6, 0.3, 0.1])
+    for i in range(3):
+        p.pcum[i] = cs[i] / cs[-1]
+    comps = A.bitmap(0, 1, 2)
+    G.check(G.lib().gf_os_GaussianFeH_kernel(C.byref(fx.rows()), comps, C.byref(p), C.byref(fx.drng.rng),
+                                             fx.dev.col("comp"), fx.dev.col("hidden"), fx.dev.col("XYZ"),
+                                             fx.dev.col("FeH_comp"), fx.dev.col("FeH"), stream_ptr()))
+    fx.oracle_chain(gfeh=p, gfeh_comps=comps)
+    fx.compare(["FeH", "FeH_comp"])
+
+
+def test_Bond2010(fx):
+    p = gfx.bond_params()
+    G.check(G.lib().gf_os_Bond2010_kernel(C.byref(fx.rows()), C.byref(p), C.byref(fx.drng.rng), fx.dev.col("comp"),
+                                          fx.dev.col("hidden"), fx.dev.col("XYZ"), fx.dev.col("vcyl"), stream_ptr()))
+    fx.oracle_chain(bond=p)
+    fx.compare(["vcyl"])
+
+
+@pytest.mark.parametrize("coordsys", [0, 1])
+def test_vel2pm(fx, coordsys):
+    p = gfx.vel2pm_params(coordsys)
+    G.check(G.lib().gf_os_vel2pm_kernel(C.byref(fx.rows()), C.byref(p), fx.dev.col("lb"), fx.dev.col("XYZ"),
+                                        fx.dev.col("vcyl"), fx.dev.col("pm"), fx.dev.col("hidden"), stream_ptr()))
+    fx.oracle_chain(v2pm=p)
+    # the equatorial rotation uses double cos/sin of the platform
+    fx.compare(["pm"], exact=(coordsys == 0), rtol=2e-6, atol=1e-6)
+
+
+def test_gal2other(fx):
+    G.check(G.lib().gf_os_gal2other_kernel(C.byref(fx.rows()), 1, fx.dev.col("hidden"), fx.dev.col("lb"),
+                                           fx.dev.col("radec"), stream_ptr()))
+    fx.oracle_chain(gal2other_coordsys=1)
+    fx.compare(["radec"], exact=False, rtol=0, atol=1e-10)
+
+
+def test_photometry(fx):
+    comps = A.bitmap(0, 1, 2)
+    Mr = fx.dev.col("absmag")
+    G.check(G.lib().gf_os_photometry_kernel(C.byref(fx.rows()), comps, C.byref(fx.phot_dev), fx.dev.col("Am"),
+                                            fx.dev.col("photoFlags"), fx.dev.col("DM"), Mr, 1, fx.dev.col("mags"),
+                                            fx.dev.col("FeH"), fx.dev.col("comp"), fx.dev.col("hidden"),
+                                            stream_ptr()))
+    fx.oracle_chain(phot=fx.phot_host, phot_comps=comps)
+    fx.compare(["mags", "photoFlags"])
+
+
+@pytest.mark.parametrize("algo", [1, 2, 3])
+def test_unresolvedMultiples(fx, algo):
+    comps = A.bitmap(0, 1, 2)
+    pd, ph = fx.mult(True, algo), fx.mult(False, algo)
+    G.check(G.lib().gf_os_unresolvedMultiples_kernel(C.byref(fx.rows()), comps, C.byref(pd), C.byref(fx.drng.rng),
+                                                     2, fx.dev.col("absmag"), fx.dev.col("absmagSys"),
+                                                     fx.dev.col("absmagSysNcomp"), fx.dev.col("comp"),
+                                                     fx.dev.col("hidden"), stream_ptr()))
+    fx.oracle_chain(mult=ph, mult_comps=comps)
+    fx.compare(["absmag", "absmagSys", "absmagSysNcomp"])
+
+
+def test_photometric_errors(fx):
+    fx.dev.buf["mags"][0].copy_(__import__("torch").from_numpy(
+        np.random.default_rng(1).uniform(14, 26, fx.dev.buf["mags"][0].numel()).astype(np.float32)))
+    fx.host.buf["mags"][0][:] = fx.dev.buf["mags"][0].cpu().numpy()
+    pd, ph = fx.perr(True), fx.perr(False)
+    G.check(G.lib().gf_photometric_errors_kernel(C.byref(fx.rows()), C.byref(pd), C.byref(fx.drng.rng),
+                                                 fx.dev.col("mags"), fx.dev.col("obsMags"), fx.dev.col("hidden"),
+                                                 stream_ptr()))
+    fx.oracle_chain(perr=ph)
+    fx.compare(["obsMags"])
+
+
+def test_stream_limit_is_enforced(fx):
+    p = gfx.feh_params()
+    rows = A.gf_rows(0, 256, 65536 * 256 + 1, 0)
+    rc = G.lib().gf_os_FeH_kernel(C.byref(rows), C.byref(p), C.byref(fx.drng.rng), fx.dev.col("comp"),
+                                  fx.dev.col("hidden"), fx.dev.col("XYZ"), fx.dev.col("FeH"), stream_ptr())
+    assert rc == -3
+
+
+def test_fused_chain_equals_unfused_and_oracle():
+    fx = Fixture()
+    feh, bond, v2pm = gfx.feh_params(), gfx.bond_params(), gfx.vel2pm_params(1)
+    comps = A.bitmap(0, 1, 2)
+    pe_d, pe_h = fx.perr(True), fx.perr(False)
+    ch = A.gf_chain()
+    ch.feh, ch.bond, ch.v2pm = C.pointer(feh), C.pointer(bond), C.pointer(v2pm)
+    ch.phot, ch.phot_comps = C.pointer(fx.phot_dev), comps
+    ch.perr = C.pointer(pe_d)
+    ch.gal2other_coordsys = 1
+    t = fx.dev.table()
+    G.check(G.lib().gf_module_chain(C.byref(fx.rows()), C.byref(ch), C.byref(fx.drng.rng), C.byref(t),
+                                    stream_ptr()), "gf_module_chain")
+    fx.oracle_chain(feh=feh, bond=bond, phot=fx.phot_host, phot_comps=comps, perr=pe_h, v2pm=v2pm,
+                    gal2other_coordsys=1)
+    fx.compare(["FeH", "vcyl", "mags", "photoFlags", "obsMags"])
+    fx.compare(["pm"], exact=False, rtol=2e-6, atol=1e-6)
+    fx.compare(["radec"], exact=False, rtol=0, atol=1e-10)

@@ -1,0 +1,52 @@
+"""Multiply-with-carry streams on the GPU, bit-exact against the oracle."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import galfast_amd as G
+import oracle as O
+from devutil import DevRng, stream_ptr, torch, sync
+from gfx import mwc_state
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(kind, param, n, seed, ns=65536, chunks=1):
+    T = torch()
+    host = mwc_state(seed, ns)
+    dev = DevRng(host)
+    dt = np.float32 if kind <= 1 else np.uint32
+    for c in range(chunks):
+        out = T.zeros(n * ns, dtype=T.int32, device="cuda")
+        G.check(G.lib().gf_mwc_sample(C.byref(dev.rng), kind, C.c_float(param), n, C.c_void_p(out.data_ptr()),
+                                      stream_ptr()), "gf_mwc_sample")
+        ref = np.zeros(n * ns, dtype=dt)
+        O.lib().gfo_mwc_sample(O.ptr(host), ns, kind, C.c_float(param), n, O.ptr(ref))
+        got = out.cpu().numpy().view(dt)
+        assert np.array_equal(got, ref), "chunk %d: %d mismatches" % (c, (got != ref).sum())
+    assert np.array_equal(dev.state(), host)
+
+
+def test_uniform_all_streams_1e4_draws():
+    _run(0, 0.0, 1000, seed=42, chunks=10)
+
+
+def test_uniform_seed_with_large_multipliers():
+    _run(0, 0.0, 64, seed=3, ns=65536)
+
+
+@pytest.mark.parametrize("sigma", [1.0, 0.3])
+def test_gaussian(sigma):
+    _run(1, sigma, 100, seed=5)
+
+
+@pytest.mark.parametrize("mu", [0.0, 0.02, 0.7, 6.5, 10.5, 37.0, 2500.0])
+def test_poisson(mu):
+    _run(2, mu, 20, seed=9, ns=16384)
+
+
+def test_poisson_sweep_of_means():
+    # stream s draws with mean 0.01*(1+s): covers the product loop, the
+    # gamma_int/gamma_large split and the binomial tail
+    _run(3, 0.01, 4, seed=13, ns=65536)

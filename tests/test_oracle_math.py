@@ -1,0 +1,62 @@
+"""Pin the oracle's float transcendentals against glibc and identities."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+RNG = np.random.default_rng(5)
+
+
+def ulp_diff(a, b):
+    a = a.astype(np.float32).view(np.int32).astype(np.int64)
+    b = b.astype(np.float32).view(np.int32).astype(np.int64)
+    a = np.where(a < 0, -(a & 0x7FFFFFFF), a)
+    b = np.where(b < 0, -(b & 0x7FFFFFFF), b)
+    return np.abs(a - b)
+
+
+CASES = {
+    0: np.concatenate([RNG.uniform(-87, 88, 200000), RNG.uniform(-1e-3, 1e-3, 20000), [0.0, -0.0, 1.0, -1.0]]),
+    1: np.concatenate([10 ** RNG.uniform(-37, 38, 200000), RNG.uniform(0.5, 2, 50000), [1.0, 2.0]]),
+    2: RNG.uniform(-20, 20, 200000),
+    3: RNG.uniform(-20, 20, 200000),
+    4: RNG.uniform(0, 3.14159, 200000),
+    6: RNG.uniform(-37, 38, 200000),
+    7: 10 ** RNG.uniform(-37, 38, 200000),
+}
+
+
+NP64 = {0: np.exp, 1: np.log, 2: np.sin, 3: np.cos, 4: np.tan,
+        6: lambda v: 10.0 ** v, 7: np.log10}
+
+
+@pytest.mark.parametrize("fn", sorted(CASES))
+def test_against_glibc_and_double_rounding(fn):
+    x = CASES[fn].astype(np.float32)
+    mine = O.math_f(fn, x)
+    # glibc's float routines (themselves up to 2 ulp off for log10f)
+    assert ulp_diff(mine, O.math_f(fn, x, libm=True)).max() <= 2
+    # numpy's double routines rounded to float: equal almost everywhere
+    cr = NP64[fn](x.astype(np.float64)).astype(np.float32)
+    d = ulp_diff(mine, cr)
+    assert d.max() <= 1
+    assert (d == 0).mean() > 0.9999
+
+
+def test_powf_against_glibc():
+    x = (10 ** RNG.uniform(-10, 10, 200000)).astype(np.float32)
+    y = RNG.uniform(-4, 4, 200000).astype(np.float32)
+    d = ulp_diff(O.math_f(5, x, y), O.math_f(5, x, y, libm=True))
+    assert d.max() <= 1
+    # special cases used by the models
+    assert O.math_f(5, np.float32([0.0]), np.float32([0.0]))[0] == 1.0
+    assert O.math_f(5, np.float32([0.0]), np.float32([1.5]))[0] == 0.0
+    assert O.math_f(5, np.float32([10.0]), np.float32([2.0]))[0] == 100.0
+
+
+def test_identities():
+    x = RNG.uniform(-10, 10, 10000).astype(np.float32)
+    s, c = O.math_f(2, x), O.math_f(3, x)
+    assert np.abs(s.astype(np.float64) ** 2 + c.astype(np.float64) ** 2 - 1).max() < 3e-7
+    y = RNG.uniform(0.1, 10, 10000).astype(np.float32)
+    assert np.abs(O.math_f(0, O.math_f(1, y)) / y - 1).max() < 3e-7

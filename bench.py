@@ -1,0 +1,152 @@
+"""Catalog throughput of the MI355X pipeline on one GPU.
+
+A step is one complete catalog job of the chosen workload: the count pass
+over every (pixel, DM-bin) pair of every model, the Poisson plan, the star
+draw and the fused per-star module chain, chunk by chunk, with all inputs
+(pixels, luminosity functions, colour tables, error curves, RNG streams)
+resident in HBM before the timed region.  ``value`` is visible catalog
+stars per second.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1]
+
+The CPU baseline replays a bounded sample of the same job (a contiguous
+block of sky pixels) on the single-threaded oracle on the host.
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+WORKLOADS = {
+    "c1": "full sky |b|>10 deg, 0.5 deg Lambert pixels, r 14-24, dm 0.05, Mr 3-15; thin+thick disk + "
+          "power-law halo; Ivezic [Fe/H]; Bond2010 kinematics (halo 140/105/75 km/s); equatorial "
+          "proper motions and coordinates; 5 synthetic bands with sigma(m)=0.02+exp(0.8(m-24)) errors; "
+          "3e7-row output chunks",
+    "c0": "10x10 deg patch at (l,b)=(90,60), 1 deg pixels, r 15-21, dm 0.05, Mr 3-15; thin+thick disk + "
+          "halo; Ivezic [Fe/H]; 5 synthetic bands",
+}
+
+PEAK_HBM_GBS = 8000.0
+# algorithmic bytes per row (stated in DESIGN.md): the draw writes the 10
+# skygen columns; the module chain reads the 7 input columns it needs and
+# writes its 7 output columns once
+EMIT_BYTES_PER_ROW = 64
+CHAIN_BYTES_PER_VISIBLE_ROW = {"c1": 48 + 88, "c0": 4 + 4 + 12 + 4 + 4 + 4 + 4 + 20 + 4}
+CHAIN_BYTES_PER_HIDDEN_ROW = 4
+
+
+def cpu_baseline(cat, workload, budget_s):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import galfast_amd as G
+    import replay as R
+    view = R.host_view(G.lib(), cat._h)
+    npix = view.sp.npixels
+    # a contiguous block of pixels around the middle of the pixel list
+    block = max(1, min(npix, 400))
+    start = npix // 2
+    idx = list(range(start, min(npix, start + block)))
+    r = R.replay(view, chunk_rows=30_000_000, pixel_index=idx, time_budget=budget_s)
+    rate = r["visible"] / r["seconds"] if r["seconds"] > 0 else 0.0
+    return {"value": rate, "unit": "stars/s", "cores": 1, "kind": "port",
+            "sample": "%d of %d pixels (%d visible stars, %.1f s, single-threaded oracle)"
+                      % (len(idx), npix, r["visible"], r["seconds"])}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS))
+    ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the sky would shard by "
+              "pixels, see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import galfast_amd as G
+    from galfast_amd import configs
+    from galfast_amd.catalog import Catalog
+
+    G.check(G.lib().gf_init(0), "gf_init")
+    d = tempfile.mkdtemp(prefix="galfast_bench_%s_" % args.workload)
+    cmd = configs.WRITERS[args.workload](d, args.seed)
+    cat = Catalog(cmd, seed=args.seed, chunk_rows=30_000_000)
+    info = cat.info()
+
+    def step():
+        t0 = time.perf_counter()
+        expected = cat.count()
+        t1 = time.perf_counter()
+        st = cat.run()
+        return expected, t1 - t0, st
+
+    for _ in range(args.warmup):
+        step()
+    G.lib().gf_sync(None)
+    t0 = time.perf_counter()
+    acc = {"visible": 0, "generated": 0, "t_count": 0.0, "t_plan": 0.0, "t_emit": 0.0, "t_chain": 0.0,
+           "n_emit": 0, "n_chain": 0, "expected": 0.0}
+    for _ in range(args.steps):
+        expected, tc, st = step()
+        acc["visible"] += st.visible
+        acc["generated"] += st.generated
+        acc["t_count"] += tc
+        acc["t_plan"] += st.t_plan
+        acc["t_emit"] += st.t_emit
+        acc["t_chain"] += st.t_chain
+        acc["n_emit"] += st.n_emit
+        acc["n_chain"] += st.n_chain
+        acc["expected"] += expected
+    G.lib().gf_sync(None)
+    elapsed = time.perf_counter() - t0
+    value = acc["visible"] / elapsed
+
+    # roofline of the dominant kernel, from HIP-event durations on the catalog's stream
+    hidden = acc["generated"] - acc["visible"]
+    if acc["t_emit"] >= acc["t_chain"]:
+        name, t, n = "k_emit", acc["t_emit"], acc["n_emit"]
+        bytes_total = EMIT_BYTES_PER_ROW * acc["generated"]
+    else:
+        name, t, n = "k_chain", acc["t_chain"], acc["n_chain"]
+        bytes_total = CHAIN_BYTES_PER_VISIBLE_ROW[args.workload] * acc["visible"] + CHAIN_BYTES_PER_HIDDEN_ROW * hidden
+    per_launch_s = t / max(n, 1)
+    achieved = bytes_total / max(n, 1) / per_launch_s / 1e9 if per_launch_s > 0 else 0.0
+    out = {
+        "metric": "catalog stars/sec (full pipeline) on 1 GPU vs plain; p50/p99 request latency",
+        "value": value,
+        "unit": "stars/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "%s: %s" % (args.workload, WORKLOADS[args.workload]),
+                   "pixels": int(info.npixels), "nm": info.nm, "nM": info.nM,
+                   "stars_per_step": acc["visible"] // args.steps,
+                   "rows_per_step": acc["generated"] // args.steps,
+                   "expected_per_step": acc["expected"] / args.steps},
+        "breakdown_ms_per_step": {k: 1000.0 * acc[k] / args.steps for k in ("t_count", "t_plan", "t_emit", "t_chain")},
+        "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cat, args.workload, args.cpu_budget)
+    print(json.dumps(out))
+    cat.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -44,8 +44,6 @@ def _compile(src):
     rel = os.path.relpath(src, CSRC).replace(os.sep, "_")
     obj = os.path.join(OBJ, rel + ".o")
     cmd = [HIPCC] + COMMON + DEVICE + ["-c", src, "-o", obj]
-    if src.endswith(".cpp"):
-        cmd = [HIPCC] + COMMON + ["-D__HIP_PLATFORM_AMD__", "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     return src, obj, r
 

@@ -1,0 +1,180 @@
+"""Writers for the benchmark workloads as galfast configuration files.
+
+Each workload is written as the file set a reference user would pass to
+``galfast catalog cmd.conf``: definitions, skygen input, density models,
+footprint, module configurations, plus the synthetic tables they name
+(a seeded smooth luminosity function, a seeded polynomial photometric
+system and tabulated photometric-error curves).
+
+* ``c0``  -- thin disk (2600/300 pc) + thick disk (3600/900 pc, 0.12) +
+  power-law halo (n=-2.77, c/a=0.64, 0.005); Sun at Rg=8000, z0=25;
+  10x10 deg patch at (l,b)=(90,60) in 1 deg pixels; r in [15,21], dm=0.05;
+  Mr in [3,15]; 5 synthetic bands; Ivezic [Fe/H] (the photometry module
+  needs FeH).  About 1e5 stars.
+* ``c1``  -- the same model over |b| > 10 deg in 0.5 deg pixels, r in
+  [14,24]; Ivezic [Fe/H]; Bond2010 kinematics with halo dispersions
+  (sigma_r, sigma_phi, sigma_theta) = (140, 105, 75) km/s; equatorial proper
+  motions and coordinates; photometric errors sigma(m)=0.02+exp(0.8(m-24)).
+"""
+import os
+
+import numpy as np
+
+BANDS = ["SYNu", "SYNg", "SYNr", "SYNi", "SYNz"]
+
+
+def seeded_lf(seed=42, M0=3.0, M1=15.0, dM=0.1):
+    """Smooth luminosity function on [M0, M1] step dM (stars / pc^3 / mag)."""
+    rng = np.random.default_rng(seed)
+    M = np.round(np.arange(M0, M1 + dM / 2, dM), 10)
+    c = rng.uniform(-0.3, 0.3, 4)
+    x = (M - 9.0) / 6.0
+    logphi = -3.46 + 0.25 * (M - 3.0) - 0.02 * (M - 3.0) ** 2 + 0.1 * (c[0] * x + c[1] * x * x + c[2] * x ** 3)
+    return M, 10 ** logphi
+
+
+def seeded_photosys(seed=42, ncolors=4, Mr=(-1.0, 18.0, 0.1), FeH=(-3.0, 0.5, 0.25)):
+    """Rows (Mr, FeH, u-g, g-r, r-i, i-z): colours are seeded cubic polynomials
+    of an r-i proxy that grows with Mr, plus a linear metallicity term."""
+    rng = np.random.default_rng(seed)
+    Ms = np.round(np.arange(Mr[0], Mr[1] + Mr[2] / 2, Mr[2]), 8)
+    Fs = np.round(np.arange(FeH[0], FeH[1] + FeH[2] / 2, FeH[2]), 8)
+    coef = rng.uniform(-0.05, 0.05, (ncolors, 4))
+    coef[:, 1] += rng.uniform(0.5, 1.5, ncolors)
+    slope = rng.uniform(0.02, 0.2, ncolors)
+    rows = []
+    for f in Fs:
+        for m in Ms:
+            ri = 0.1 + 0.12 * (m - 3.0)
+            rows.append([m, f] + [coef[c, 0] + coef[c, 1] * ri + coef[c, 2] * ri ** 2 + coef[c, 3] * ri ** 3
+                                  + slope[c] * f for c in range(ncolors)])
+    return np.array(rows)
+
+
+def _w(path, text):
+    with open(path, "w") as f:
+        f.write(text.strip() + "\n")
+
+
+def write_common(d, seed=42):
+    os.makedirs(d, exist_ok=True)
+    M, phi = seeded_lf(seed)
+    np.savetxt(os.path.join(d, "lf.txt"), np.c_[M, phi], fmt="%.10g")
+    np.savetxt(os.path.join(d, "photosys.txt"), seeded_photosys(seed), fmt="%.8f")
+    m = np.round(np.arange(10.0, 30.0 + 0.005, 0.01), 6)
+    err = 0.02 + np.exp(0.8 * (m - 24.0))
+    for b in BANDS:
+        np.savetxt(os.path.join(d, "SYN.%s.photoerr.txt" % b), np.c_[m, err], fmt="%.8g")
+    _w(os.path.join(d, "definitions.conf"), "module = definitions\nRg = 8000\nz0 = 25")
+    _w(os.path.join(d, "model.thin.conf"), """
+module = model
+model = exponential disk
+comp = 0
+l = 2600
+h = 300
+lumfunc = lf.txt
+f = 1""")
+    _w(os.path.join(d, "model.thick.conf"), """
+module = model
+model = exponential disk
+comp = 1
+l = 3600
+h = 900
+lumfunc = lf.txt
+f = 0.12""")
+    _w(os.path.join(d, "model.halo.conf"), """
+module = model
+model = power law ellipsoid
+comp = 2
+n = -2.77
+c_a = 0.64
+lumfunc = lf.txt
+f = 0.005""")
+    _w(os.path.join(d, "feh.conf"), """
+module = FeH
+comp_thin = 0
+comp_thick = 1
+comp_halo = 2
+A0 = 0.63
+sigma0 = 0.2
+offs0 = 0.0
+A1 = 0.37
+sigma1 = 0.2
+offs1 = 0.14
+muInf = -0.82
+deltaMu = 0.55
+Hmu = 500
+offsH = -1.46
+sigmaH = 0.3""")
+    _w(os.path.join(d, "photometry.conf"), """
+module = photometry
+comp = 0-2
+bandset = SYN
+file = photosys.txt
+bands = %s
+bootstrap_band = SYNr
+absband = absSYNr
+dFeH = 0.01
+dM = 0.01
+reddening = 1.8739 1.3788 1 0.7583 0.5376""" % " ".join(BANDS))
+
+
+def write_c0(d, seed=42):
+    write_common(d, seed)
+    _w(os.path.join(d, "skygen.conf"), """
+module = skygen
+band = SYNr
+dm = 0.05
+m0 = 15
+m1 = 21
+M0 = 3
+M1 = 15
+dx = 1""")
+    _w(os.path.join(d, "foot.conf"), "module = footprint\ntype = rect\ncoordsys = gal\nrect = 85 95 55 65")
+    _w(os.path.join(d, "cmd.conf"), """
+module = config
+seed = %d
+definitions = definitions.conf
+input = skygen.conf
+models = model.thin.conf model.thick.conf model.halo.conf
+footprints = foot.conf
+modules = feh.conf photometry.conf""" % seed)
+    return os.path.join(d, "cmd.conf")
+
+
+def write_c1(d, seed=42):
+    write_common(d, seed)
+    _w(os.path.join(d, "skygen.conf"), """
+module = skygen
+band = SYNr
+dm = 0.05
+m0 = 14
+m1 = 24
+M0 = 3
+M1 = 15
+dx = 0.5""")
+    _w(os.path.join(d, "foot.conf"), "module = footprint\ntype = rect\ncoordsys = gal\nrect = 0 360 -90 90\nbmin = 10")
+    _w(os.path.join(d, "Bond2010.conf"), """
+module = Bond2010
+comp_thin = 0
+comp_thick = 1
+comp_halo = 2
+HsigmaR = 140 0 0 0 0
+HsigmaPhi = 105 0 0 0 0
+HsigmaTh = 75 0 0 0 0""")
+    _w(os.path.join(d, "vel2pm.conf"), "module = vel2pm\ncoordsys = equ")
+    _w(os.path.join(d, "gal2other.conf"), "module = gal2other\ncoordsys = equ")
+    _w(os.path.join(d, "photoerr.conf"), "module = photometricErrors\n" +
+       "\n".join("SYN.%s.file = SYN.%s.photoerr.txt" % (b, b) for b in BANDS))
+    _w(os.path.join(d, "cmd.conf"), """
+module = config
+seed = %d
+definitions = definitions.conf
+input = skygen.conf
+models = model.thin.conf model.thick.conf model.halo.conf
+footprints = foot.conf
+modules = feh.conf Bond2010.conf photometry.conf photoerr.conf vel2pm.conf gal2other.conf""" % seed)
+    return os.path.join(d, "cmd.conf")
+
+
+WRITERS = {"c0": write_c0, "c1": write_c1}

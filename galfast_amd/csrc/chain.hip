@@ -6,6 +6,7 @@
 // output equals the unfused sequence of gf_os_*_kernel calls.
 #include "gf_runtime.h"
 #include "gf_modules.h"
+#include <algorithm>
 
 namespace {
 
@@ -89,5 +90,42 @@ extern "C" int gf_module_chain(const gf_rows *ks, const gf_chain *chain, gf_rng 
 	GF_LAUNCHED();
 	// the staging copy must complete before the host buffer is reused
 	GF_HIP(hipStreamSynchronize(st));
+	return 0;
+}
+
+namespace {
+__global__ void __launch_bounds__(256) k_count_visible(gf_col hidden, uint64_t n, unsigned long long *out)
+{
+	__shared__ unsigned long long sh[256];
+	unsigned long long c = 0;
+	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+		c += col<int>(hidden, (uint32_t)i) == 0 ? 1ull : 0ull;
+	sh[threadIdx.x] = c;
+	__syncthreads();
+	for (int o = 128; o > 0; o >>= 1) {
+		if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) atomicAdd(out, sh[0]);
+}
+} // namespace
+
+extern "C" int gf_count_visible(gf_col hidden, uint64_t n, uint64_t *visible, void *stream)
+{
+	GF_CHECK(visible && hidden.ptr && n < (1ull << 32), GF_E_ARG);
+	hipStream_t st = gf::as_stream(stream);
+	void *p;
+	int rc = gf::scratch(GF_SLOT_NSLOTS + 3, sizeof(unsigned long long), &p);
+	if (rc) return rc;
+	GF_HIP(hipMemsetAsync(p, 0, sizeof(unsigned long long), st));
+	if (n) {
+		unsigned grid = (unsigned)std::min<uint64_t>(2048, (n + 255) / 256);
+		k_count_visible<<<grid, 256, 0, st>>>(hidden, n, (unsigned long long *)p);
+		GF_LAUNCHED();
+	}
+	unsigned long long h = 0;
+	GF_HIP(hipMemcpyAsync(&h, p, sizeof(h), hipMemcpyDeviceToHost, st));
+	GF_HIP(hipStreamSynchronize(st));
+	*visible = h;
 	return 0;
 }

@@ -1,0 +1,994 @@
+// The catalog pipeline behind gf_catalog_*: reads a galfast `catalog`
+// command configuration (pipeline.cpp:1313-1445), constructs the sky
+// generator (skygen/os_skygen.cpp:816-860) and the per-star modules
+// (modules/*_host.cpp construct()), keeps every table resident on the
+// device, and runs count -> draw -> module chain in output chunks.
+#include "host.h"
+#include "../gf_runtime.h"
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <fstream>
+#include <mutex>
+#include <sstream>
+
+using namespace gfh;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+std::vector<std::string> split_ws(const std::string &s)
+{
+	std::istringstream ss(s);
+	std::vector<std::string> out;
+	std::string w;
+	while (ss >> w) out.push_back(w);
+	return out;
+}
+
+// Device buffer owned by the catalog.
+struct DevBuf {
+	void *p = nullptr;
+	size_t bytes = 0;
+	void alloc(size_t b)
+	{
+		free();
+		if (hipMalloc(&p, b ? b : 1) != hipSuccess) fail("device allocation failed");
+		bytes = b;
+	}
+	template <typename T> void upload(const std::vector<T> &v)
+	{
+		alloc(v.size() * sizeof(T));
+		if (hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) fail("upload failed");
+	}
+	void free()
+	{
+		if (p) (void)hipFree(p);
+		p = nullptr;
+		bytes = 0;
+	}
+	~DevBuf() { free(); }
+};
+
+// componentMap (common/compmgr.h:35-49): user component IDs -> sequential indices.
+struct ComponentMap {
+	std::vector<uint32_t> seq2comp;
+	uint32_t seqIdx(uint32_t id)
+	{
+		for (size_t i = 0; i < seq2comp.size(); i++)
+			if (seq2comp[i] == id) return (uint32_t)i;
+		seq2comp.push_back(id);
+		return (uint32_t)seq2comp.size() - 1;
+	}
+};
+
+// read_component_map (pipeline.cpp:133-191): "a, b-c, ..." -> bitmap over
+// sequential indices of components that exist.
+gf_bitmap read_component_map(const Config &cfg, const std::string &key, ComponentMap &cm)
+{
+	gf_bitmap b{{0, 0}};
+	std::string line = cfg.get(key);
+	std::stringstream ss(line);
+	std::string item;
+	while (std::getline(ss, item, ',')) {
+		size_t dash = item.find('-', item.find_first_not_of(" \t") + 1);
+		uint32_t a, c;
+		if (dash != std::string::npos) { a = (uint32_t)atof(item.substr(0, dash).c_str()); c = (uint32_t)atof(item.substr(dash + 1).c_str()); }
+		else { a = c = (uint32_t)atof(item.c_str()); }
+		for (size_t i = 0; i < cm.seq2comp.size(); i++) {
+			uint32_t id = cm.seq2comp[i];
+			if (id >= a && id <= c && i < 64) b.bits[i / 32] |= 1u << (i % 32);
+		}
+	}
+	return b;
+}
+
+struct Model {
+	gf_model m{};
+	Tex1D lf;
+	DevBuf lf_dev;
+	std::string name;
+	int user_comp = 0;
+	gf_count_result counts{};
+	DevBuf wpk;
+};
+
+struct Footprint {
+	gf_footprint f{};
+	double area_sr = 0;
+};
+
+} // namespace
+
+struct gf_catalog {
+	Config globals, skygen;
+	std::vector<Config> model_cfgs, foot_cfgs, module_cfgs;
+	ComponentMap cm;
+	gf_skygen_params sp{};
+	std::vector<gf_pencil_beam> pixels;
+	DevBuf pix_dev;
+	std::vector<std::unique_ptr<Model>> models;
+	Footprint foot;
+	uint32_t seed = 42;
+	std::string datadir;
+	std::string band = "LSSTr";
+	uint64_t chunk_rows = 30000000;
+	int acc_mode = GF_ACC_FP64;
+	double nstars_forced = 0;
+
+	// modules
+	bool has_feh = false, has_gfeh = false, has_mult = false, has_bond = false, has_phot = false,
+	     has_perr = false, has_v2pm = false;
+	int gal2other = 0;
+	gf_FeH_data feh{};
+	gf_GaussianFeH_data gfeh{};
+	gf_bitmap gfeh_comps{}, mult_comps{}, phot_comps{};
+	gf_multiples_data mult{};
+	Tex1D mult_sec, mult_cum, mult_inv;
+	DevBuf mult_dev[3];
+	int nabsmag = 1;
+	gf_Bond2010_data bond{};
+	gf_photometry_data phot{};
+	PhotTables phot_tables;
+	DevBuf phot_dev[8];
+	int nbands = 0;
+	std::vector<std::string> band_names;
+	gf_photoerr_data perr{};
+	DevBuf perr_dev[2];
+	gf_vel2pm_data v2pm{};
+
+	// host copies for gf_catalog_host_view
+	std::vector<gf_model> models_h;
+	gf_photometry_data phot_h{};
+	gf_multiples_data mult_h{};
+	gf_photoerr_data perr_h{};
+	std::vector<double> perr_kx, perr_ky;
+	std::vector<uint32_t> rng_state0;
+
+	// device state
+	gf_rng rng{};
+	DevBuf table_dev;
+	uint64_t table_cap = 0;
+	gf_table table{};
+	hipStream_t stream = nullptr;
+	hipEvent_t ev[2] = {nullptr, nullptr};
+
+	~gf_catalog()
+	{
+		if (rng.state) gf_rng_free(&rng);
+		if (ev[0]) (void)hipEventDestroy(ev[0]);
+		if (ev[1]) (void)hipEventDestroy(ev[1]);
+	}
+};
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Sky pixelization: square pixels of side dx in the north (pole b = +90, l0 =
+// 90 deg) and south (b = -90, l0 = 270 deg) Lambert projections
+// (skygen/os_skygen.cpp:239-283, 940-995), covered fractions by 8x8 point
+// sampling of the footprint predicate.
+// ---------------------------------------------------------------------------
+bool in_footprint(const gf_footprint &f, double l, double b)
+{
+	if (f.kind == 1) {
+		double ld = l * 180. / M_PI, bd = b * 180. / M_PI;
+		ld = fmod(ld, 360.); if (ld < 0) ld += 360.;
+		if (bd < f.b0 || bd > f.b1) return false;
+		if (f.bmin > 0. && fabs(bd) < f.bmin) return false;
+		if (f.l1 - f.l0 >= 360.) return true;
+		double t = fmod(ld - f.l0, 360.); if (t < 0) t += 360.;
+		return t <= f.l1 - f.l0;
+	}
+	if (f.kind == 2) {
+		double d = cos(b) * cos(l) * f.cb * f.cl + cos(b) * sin(l) * f.cb * f.sl + sin(b) * f.sb;
+		return d >= f.cosr;
+	}
+	return true;
+}
+
+void lambert_init(gf_lambert &p, double l0, double b0)
+{
+	p.l0 = l0; p.b0 = b0;
+	p.cl = (float)cos(l0); p.cb = (float)cos(b0); p.sl = (float)sin(l0); p.sb = (float)sin(b0);
+}
+
+// deprojection of a pole projection (b0 = +-90 deg) in double
+void deproj_pole(int hemi, double x, double y, double &l, double &b)
+{
+	double r2 = x * x + y * y;
+	double sb = 1. - 0.5 * r2;
+	if (hemi == 0) { b = asin(std::max(-1.0, std::min(1.0, sb))); l = M_PI / 2 + atan2(x, -y); }
+	else { b = -asin(std::max(-1.0, std::min(1.0, sb))); l = 3 * M_PI / 2 + atan2(x, y); }
+	l = fmod(l, 2 * M_PI); if (l < 0) l += 2 * M_PI;
+}
+
+void pixelize(gf_catalog &c, double dx)
+{
+	c.pixels.clear();
+	const int sub = 8;
+	for (int hemi = 0; hemi < 2; hemi++) {
+		// bounding box: the whole projected hemisphere disk (radius sqrt 2),
+		// restricted to where the footprint has points on a coarse grid
+		const double R = sqrt(2.0);
+		int n = (int)ceil(2 * R / dx);
+		double x0 = -n * dx / 2, y0 = -n * dx / 2;
+		for (int i = 0; i < n; i++) {
+			for (int j = 0; j < n; j++) {
+				double xa = x0 + i * dx, ya = y0 + j * dx;
+				// quick reject: pixel entirely outside the hemisphere disk
+				double nx = std::max(fabs(xa), fabs(xa + dx)), ny = std::max(fabs(ya), fabs(ya + dx));
+				double fx = (xa > 0 || xa + dx < 0) ? std::min(fabs(xa), fabs(xa + dx)) : 0.;
+				double fy = (ya > 0 || ya + dx < 0) ? std::min(fabs(ya), fabs(ya + dx)) : 0.;
+				(void)nx; (void)ny;
+				if (fx * fx + fy * fy > 2.0) continue;
+				int inside = 0;
+				for (int a = 0; a < sub; a++)
+					for (int b2 = 0; b2 < sub; b2++) {
+						double x = xa + dx * (a + 0.5) / sub, y = ya + dx * (b2 + 0.5) / sub;
+						if (x * x + y * y > 2.0) continue;
+						double l, b;
+						deproj_pole(hemi, x, y, l, b);
+						if ((hemi == 0 && b < 0) || (hemi == 1 && b >= 0)) continue;
+						if (in_footprint(c.foot.f, l, b)) inside++;
+					}
+				if (!inside) continue;
+				double xc = xa + 0.5 * dx, yc = ya + 0.5 * dx;
+				double l, b;
+				deproj_pole(hemi, xc, yc, l, b);
+				gf_pencil_beam pb{};
+				pb.cl = (float)cos(l); pb.cb = (float)cos(b); pb.sl = (float)sin(l); pb.sb = (float)sin(b);
+				pb.coveredFraction = (float)inside / (float)(sub * sub);
+				pb.dx = (float)dx;
+				pb.dA = pb.dx * pb.dx;
+				pb.projIdx = hemi;
+				pb.X = (float)xc; pb.Y = (float)yc;
+				pb.extIdx = -1;
+				c.pixels.push_back(pb);
+			}
+		}
+	}
+	c.sp.npixels = (int)c.pixels.size();
+}
+
+void load_footprint(gf_catalog &c, const Config &fc)
+{
+	std::string type = fc.get("type");
+	std::string cs = fc.get("coordsys", "gal");
+	if (cs != "gal") fail("footprint coordsys '" + cs + "' is not supported (gal only)");
+	gf_footprint &f = c.foot.f;
+	if (type == "rect") {
+		std::vector<double> r = fc.getv("rect", "0 1 0 1");
+		if (r.size() < 4) fail("rect needs l0 l1 b0 b1");
+		double l0 = fmod(r[0], 360.), l1 = fmod(r[1], 360.);
+		if (l0 < 0) l0 += 360.;
+		if (l1 < 0) l1 += 360.;
+		if (l0 >= l1) { if (l0 == 0) l1 = 360.; else l0 -= 360.; }
+		f.kind = 1;
+		f.l0 = l0; f.l1 = l1; f.b0 = r[2]; f.b1 = r[3];
+	} else if (type == "beam") {
+		std::vector<double> v = fc.getv("footprint_beam", "");
+		if (v.size() < 3) fail("footprint_beam needs l b r");
+		if (v.size() > 3 && v[3] != 0.) fail("beams with holes are not supported");
+		double l = v[0] * M_PI / 180., b = v[1] * M_PI / 180.;
+		f.kind = 2;
+		f.cl = cos(l); f.cb = cos(b); f.sl = sin(l); f.sb = sin(b);
+		f.cosr = cos(v[2] * M_PI / 180.);
+	} else fail("unknown footprint type " + type);
+	f.bmin = fc.getd("bmin", 0.);
+}
+
+// os_skygen::load_skyPixelizationConfig + reset_absmag (skygen/os_skygen.cpp:401-430)
+void load_sky_pixelization(gf_catalog &c, double &dx)
+{
+	const Config &cfg = c.skygen;
+	dx = cfg.getd("dx", 1.0) * M_PI / 180.;
+	auto any = [&](const char *a, const char *b) { return cfg.count(a) ? cfg.getd(a) : cfg.getd(b); };
+	gf_skygen_params &sp = c.sp;
+	sp.M0 = (float)any("M0", "ri0");
+	sp.M1 = (float)any("M1", "ri1");
+	sp.m0 = (float)any("m0", "r0");
+	sp.m1 = (float)any("m1", "r1");
+	sp.dm = (float)cfg.getd("dm");
+	sp.dM = sp.dm;
+	sp.dmin = (float)cfg.getd("dmin", 0.);
+	sp.dmax = (float)cfg.getd("dmax", 0.);
+	sp.nM = (int)round((sp.M1 - sp.M0) / sp.dM);
+	sp.M1 = (float)(sp.M1 - 0.5 * sp.dM);
+	sp.nm = (int)round((sp.m1 - sp.m0) / sp.dm);
+	sp.m0 = (float)(sp.m0 + 0.5 * sp.dm);
+	sp.nthreads = 7680;   // the reference's 120 x 64 crawl (skygen/skyconfig_impl.h:276-289)
+	lambert_init(sp.proj[0], M_PI / 2, M_PI / 2);
+	lambert_init(sp.proj[1], 3 * M_PI / 2, -M_PI / 2);
+	c.band = cfg.get("band", "LSSTr");
+}
+
+// A model (skygen/model_expDisk.cpp, model_powerLawEllipsoid.cpp) with load_lf
+// (skygen/model_lib.h:37-95).
+void load_model(gf_catalog &c, const Config &mc)
+{
+	auto md = std::make_unique<Model>();
+	gf_model &m = md->m;
+	std::string kind = mc.get("model");
+	md->name = kind;
+	std::string ctr = mc.get("center", "galplane"), orient = mc.get("orientation", "galplane");
+	load_transform(m.T, m.M, ctr, orient, mc);
+	if (kind == "exponential disk") {
+		m.kind = GF_MODEL_EXPDISK;
+		m.l = (float)mc.getd("l");
+		m.h = (float)mc.getd("h");
+	} else if (kind == "power law ellipsoid") {
+		m.kind = GF_MODEL_POWERLAW;
+		m.ca = (float)mc.getd("c_a", 1.0);
+		m.ba = (float)mc.getd("b_a", 1.0);
+		m.n = (float)mc.getd("n", -3.0);
+	} else fail("unsupported density model '" + kind + "'");
+	md->user_comp = (int)mc.getd("comp");
+	m.comp = (int)c.cm.seqIdx(md->user_comp);
+	// luminosity function
+	if (mc.count("lumfunc")) {
+		auto rows = load_columns(mc.resolve(mc.get("lumfunc")), 2);
+		std::vector<double> X, Y;
+		for (auto &r : rows) { X.push_back(r[0]); Y.push_back(r[1]); }
+		md->lf = resample_clip_1d(X, Y, 0.f, 1024);
+	} else {
+		md->lf = constant_1d(1.f, -100.f, 100.f);
+	}
+	// renormalise f so rho(LF site) = f (the cuts are applied after)
+	float lfl = (float)mc.getd("lf_l", 0.), lfb = (float)mc.getd("lf_b", 0.), lfD = (float)mc.getd("lf_D", 0.);
+	double lr = (double)(0.01745329251994329576923691f * lfl), br = (double)(0.01745329251994329576923691f * lfb);
+	float dcl = (float)cos(lr), dcb = (float)cos(br), dsl = (float)sin(lr), dsb = (float)sin(br);
+	float px = lfD * dcl * dcb, py = lfD * dsl * dcb, pz = lfD * dsb;
+	m.f = 1.f;
+	float vx = px + m.T[0], vy = py + m.T[1], vz = pz + m.T[2];
+	float x = m.M[0][0] * vx + m.M[0][1] * vy + m.M[0][2] * vz;
+	float y = m.M[1][0] * vx + m.M[1][1] * vy + m.M[1][2] * vz;
+	float z = m.M[2][0] * vx + m.M[2][1] * vy + m.M[2][2] * vz;
+	float norm;
+	if (m.kind == GF_MODEL_EXPDISK) {
+		float r = sqrtf(x * x + y * y);
+		float e = -r / m.l - fabsf(z) / m.h;
+		norm = (float)exp((double)e);
+	} else {
+		float yb = y / m.ba, zc = z / m.ca;
+		float rSq = x * x + yb * yb + zc * zc;
+		norm = (float)exp((double)(0.5f * m.n) * log((double)rSq));
+	}
+	if (!(norm != 0.f) || !std::isfinite(norm)) fail("luminosity-function site has zero density");
+	float f0 = (float)mc.getd("f", 1.0);
+	m.f = f0 / norm;
+	if (m.kind == GF_MODEL_EXPDISK) {
+		float rc = (float)mc.getd("rcut", 0.);
+		m.r_cut2 = rc * rc;
+	} else {
+		float rmin = (float)mc.getd("rmin", 0.), rmax = (float)mc.getd("rmax", 0.);
+		m.rminSq = rmin * rmin;
+		m.rmaxSq = rmax * rmax;
+	}
+	m.norm = 1.f;
+	md->lf_dev.upload(md->lf.v);
+	m.lf.data = (const float *)md->lf_dev.p;
+	m.lf.n = (int)md->lf.v.size();
+	m.lf.tc[0] = md->lf.tc[0];
+	m.lf.tc[1] = md->lf.tc[1];
+	c.models.push_back(std::move(md));
+}
+
+void construct_FeH(gf_catalog &c, const Config &cfg)
+{
+	gf_FeH_data &p = c.feh;
+	load_transform(p.T, p.M, cfg.get("center", "galplane"), cfg.get("orientation", "galplane"), cfg);
+	p.A[0] = (float)cfg.getd("A0", 0.63);
+	p.sigma[0] = (float)cfg.getd("sigma0", 0.20);
+	p.offs[0] = (float)cfg.getd("offs0", 0.00);
+	p.A[1] = (float)cfg.getd("A1", 0.37);
+	p.sigma[1] = (float)cfg.getd("sigma1", 0.20);
+	p.offs[1] = (float)cfg.getd("offs1", 0.14);
+	p.Hmu = (float)cfg.getd("Hmu", 500.);
+	p.muInf = (float)cfg.getd("muInf", -0.82);
+	p.DeltaMu = (float)cfg.getd("deltaMu", 0.55);
+	double sumA = (double)(p.A[0] + p.A[1]);
+	p.A[0] = (float)(p.A[0] / sumA);
+	p.A[1] = (float)(p.A[1] / sumA);
+	p.sigma[2] = (float)cfg.getd("sigmaH", 0.30);
+	p.offs[2] = (float)cfg.getd("offsH", -1.46);
+	p.comp_thin = read_component_map(cfg, "comp_thin", c.cm);
+	p.comp_thick = read_component_map(cfg, "comp_thick", c.cm);
+	p.comp_halo = read_component_map(cfg, "comp_halo", c.cm);
+	c.has_feh = true;
+}
+
+void construct_GaussianFeH(gf_catalog &c, const Config &cfg)
+{
+	c.gfeh_comps = read_component_map(cfg, "comp", c.cm);
+	std::vector<double> mean = cfg.getv("mean", ""), sigma = cfg.getv("sigma", "");
+	if (mean.empty() || mean.size() >= 10 || sigma.size() != mean.size()) fail("GaussianFeH: bad mean/sigma");
+	gf_GaussianFeH_data &p = c.gfeh;
+	p.ncomp = (int)mean.size();
+	for (int i = 0; i < p.ncomp; i++) { p.mean[i] = (float)mean[i]; p.sigma[i] = (float)sigma[i]; }
+	std::vector<double> wt = p.ncomp > 1 ? cfg.getv("wt", "") : std::vector<double>(1, 1.);
+	if ((int)wt.size() != p.ncomp) fail("GaussianFeH: wt size");
+	double s = 0;
+	for (int i = 0; i < p.ncomp; i++) { s += wt[i]; p.pcum[i] = (float)s; }
+	for (int i = 0; i < p.ncomp; i++) p.pcum[i] /= p.pcum[p.ncomp - 1];
+	c.has_gfeh = true;
+}
+
+void farr5(float dst[5], const std::vector<double> &v)
+{
+	for (int i = 0; i < 5; i++) dst[i] = i < (int)v.size() ? (float)v[i] : 0.f;
+}
+
+// os_Bond2010::construct (modules/Bond2010_host.cpp:136-227)
+void construct_Bond2010(gf_catalog &c, const Config &cfg)
+{
+	gf_Bond2010_data &p = c.bond;
+	p.comp_thin = read_component_map(cfg, "comp_thin", c.cm);
+	p.comp_thick = read_component_map(cfg, "comp_thick", c.cm);
+	p.comp_halo = read_component_map(cfg, "comp_halo", c.cm);
+	load_transform(p.T, p.M, cfg.get("center", "galplane"), cfg.get("orientation", "galplane"), cfg);
+	auto g = [&](const char *k, const char *d) {
+		std::vector<double> v = cfg.getv(k, d);
+		std::vector<double> f;
+		for (double x : v) f.push_back((double)(float)x);
+		return f;
+	};
+	farr5(p.disk_1.m[0], g("vR", "0 0 0 0 0"));
+	farr5(p.disk_1.m[1], g("vPhi", "-194 19.2 1.25 0 0"));
+	farr5(p.disk_1.m[2], g("vZ", "0 0 0 0 0"));
+	farr5(p.disk_1.ss[0], g("sigmaR", "40 5 1.5 0 0"));
+	farr5(p.disk_1.ss[1], g("sigmaRPhi", "0 0 0 0 0"));
+	farr5(p.disk_1.ss[2], g("sigmaRZ", "0 0 0 0 0"));
+	farr5(p.disk_1.ss[3], g("sigmaPhi1", "12 1.8 2 0 0"));
+	farr5(p.disk_1.ss[4], g("sigmaPhiZ", "0 0 0 0 0"));
+	farr5(p.disk_1.ss[5], g("sigmaZ", "25 4 1.5 0 0"));
+	p.disk_2 = p.disk_1;
+	farr5(p.disk_2.ss[3], g("sigmaPhi2", "34 1.2 2 0 0"));
+	float DeltavPhi = (float)cfg.getd("DeltavPhi", 34.0);
+	p.disk_2.m[1][0] += DeltavPhi;
+	float fk = (float)cfg.getd("fk", 3.0);
+	p.fk = (float)(fk / (1. + fk));
+	farr5(p.halo.m[0], g("HvR", "0 0 0 0 0"));
+	farr5(p.halo.m[1], g("HvTh", "0 0 0 0 0"));
+	farr5(p.halo.m[2], g("HvPhi", "0 0 0 0 0"));
+	farr5(p.halo.ss[0], g("HsigmaR", "141 0 0 0 0"));
+	farr5(p.halo.ss[1], g("HsigmaRTh", "0 0 0 0 0"));
+	farr5(p.halo.ss[2], g("HsigmaRPhi", "0 0 0 0 0"));
+	farr5(p.halo.ss[3], g("HsigmaTh", "75 0 0 0 0"));
+	farr5(p.halo.ss[4], g("HsigmaThPhi", "0 0 0 0 0"));
+	farr5(p.halo.ss[5], g("HsigmaPhi", "85 0 0 0 0"));
+	c.has_bond = true;
+}
+
+// os_vel2pm::construct (modules/vel2pm_host.cpp:79-97)
+void construct_vel2pm(gf_catalog &c, const Config &cfg)
+{
+	std::string cs = cfg.get("coordsys", "gal");
+	gf_vel2pm_data &p = c.v2pm;
+	if (cs == "gal") p.coordsys = 0;
+	else if (cs == "equ") p.coordsys = 1;
+	else fail("vel2pm: unknown coordinate system " + cs);
+	p.Rg = (float)cfg.getd("Rg");
+	p.vLSR = (float)cfg.getd("vLSR", -220.0);
+	p.u0 = (float)cfg.getd("u0", -10.0);
+	p.v0 = (float)cfg.getd("v0", -5.3);
+	p.w0 = (float)cfg.getd("w0", 7.2);
+	c.has_v2pm = true;
+}
+
+void construct_gal2other(gf_catalog &c, const Config &cfg)
+{
+	std::string cs = cfg.get("coordsys", "gal");
+	if (cs == "equ") c.gal2other = 1;
+	else if (cs != "gal") fail("gal2other: unknown coordinate system " + cs);
+}
+
+// os_photometry::construct (modules/photometry_host.cpp:87-349)
+void construct_photometry(gf_catalog &c, const Config &cfg)
+{
+	c.phot_comps = read_component_map(cfg, "comp", c.cm);
+	std::string bandset = cfg.get("bandset", "LSSTugrizy");
+	c.band_names = split_ws(cfg.get("bands", "LSSTu LSSTg LSSTr LSSTi LSSTz LSSTy"));
+	c.nbands = (int)c.band_names.size();
+	int ncolors = c.nbands - 1;
+	if (ncolors < 1 || ncolors >= 16) fail("photometry: 2..16 bands supported");
+	std::string bband = cfg.get("bootstrap_band", "LSSTr");
+	auto it = std::find(c.band_names.begin(), c.band_names.end(), bband);
+	if (it == c.band_names.end()) fail("Bootstrap band must be listed in the 'bands' keyword.");
+	gf_photometry_data &p = c.phot;
+	p.ncolors = (uint32_t)ncolors;
+	p.bidx = (uint32_t)(it - c.band_names.begin());
+	for (int i = 0; i < 17; i++) p.reddening[i] = 1.f;
+	std::vector<double> red = cfg.getv("reddening", "");
+	for (int i = 0; i < c.nbands && i < (int)red.size(); i++) p.reddening[i] = (float)red[i];
+	std::string file = cfg.get("file", "");
+	if (file.empty()) file = c.datadir + "/" + bandset + ".photosys.txt";
+	else file = cfg.resolve(file);
+	auto rows = load_columns(file, 2 + ncolors);
+	double dFeH = cfg.getd("dFeH", 0.), dMr = cfg.getd("dM", 0.);
+	if (dFeH == 0.) { auto v = cfg.getv("FeH_grid", "-3 0.5 0.01"); dFeH = v.size() > 2 ? v[2] : 0.01; }
+	if (dMr == 0.) { auto v = cfg.getv("absmag_grid", "3 15 0.01"); dMr = v.size() > 2 ? v[2] : 0.01; }
+	c.phot_tables = build_photometry_tables(rows, ncolors, dFeH, dMr);
+	PhotTables &pt = c.phot_tables;
+	size_t per = (size_t)pt.nFeH * pt.nMr * 4;
+	for (int t = 0; t < pt.ntex; t++) {
+		std::vector<float> col(pt.color.begin() + t * per, pt.color.begin() + (t + 1) * per);
+		std::vector<float> fl(pt.flags.begin() + t * per, pt.flags.begin() + (t + 1) * per);
+		c.phot_dev[t].upload(col);
+		c.phot_dev[4 + t].upload(fl);
+		p.color[t] = (const float *)c.phot_dev[t].p;
+		p.cflags[t] = (const float *)c.phot_dev[4 + t].p;
+	}
+	p.nFeH = pt.nFeH;
+	p.nMr = pt.nMr;
+	p.tcFeH[0] = pt.tcFeH[0]; p.tcFeH[1] = pt.tcFeH[1];
+	p.tcMr[0] = pt.tcMr[0]; p.tcMr[1] = pt.tcMr[1];
+	c.has_phot = true;
+}
+
+// os_unresolvedMultiples::construct (modules/unresolvedMultiples_host.cpp:126-189)
+void construct_multiples(gf_catalog &c, const Config &cfg)
+{
+	c.mult_comps = read_component_map(cfg, "comp", c.cm);
+	std::string algo = cfg.get("algorithm");
+	if (algo == "LF_M2_gt_M1") c.mult.algo = 1;
+	else if (algo == "LF") c.mult.algo = 2;
+	else if (algo == "equal_mass") c.mult.algo = 3;
+	else fail("unknown secondary magnitude algorithm " + algo);
+	std::string ff = cfg.get("fraction_file", "");
+	if (!ff.empty()) {
+		auto rows = load_columns(cfg.resolve(ff), 2);
+		std::vector<double> X, Y;
+		for (auto &r : rows) { X.push_back(r[0]); Y.push_back(r[1]); }
+		c.mult_sec = resample_1d(X, Y, 64);
+	} else {
+		float frac = (float)cfg.getd("fraction", 1.0);
+		c.mult_sec = constant_1d(frac, -100.f, 100.f);
+	}
+	std::vector<double> x, y;
+	std::string lff = cfg.get("lumfunc", "");
+	if (!lff.empty()) {
+		auto rows = load_columns(cfg.resolve(lff), 2);
+		for (auto &r : rows) { x.push_back(r[0]); y.push_back(r[1]); }
+	} else {
+		x = {-100, 100};
+		y = {1, 1};
+	}
+	const int NPIX = 256;
+	Spline lf(x, y);
+	double dx = (x.back() - x.front()) / (NPIX - 1);
+	std::vector<double> xc(NPIX), yc(NPIX);
+	xc[0] = x.front();
+	yc[0] = 0;
+	double yprev = lf(x.front());
+	for (int i = 1; i < NPIX; i++) {
+		double xx = x.front() + i * dx, yy = lf(xx);
+		double dy = yy - yprev;
+		double dA = (yprev + 0.5 * dy) * dx;
+		xc[i] = xx;
+		yc[i] = yc[i - 1] + dA;
+		yprev = yy;
+	}
+	double norm = yc.back();
+	for (auto &v : yc) v /= norm;
+	c.mult_cum = resample_1d(xc, yc, NPIX);
+	c.mult_inv = resample_1d(yc, xc, NPIX);
+	Tex1D *ts[3] = {&c.mult_sec, &c.mult_cum, &c.mult_inv};
+	gf_tex1d *dst[3] = {&c.mult.secProb, &c.mult.cumLF, &c.mult.invCumLF};
+	for (int i = 0; i < 3; i++) {
+		c.mult_dev[i].upload(ts[i]->v);
+		dst[i]->data = (const float *)c.mult_dev[i].p;
+		dst[i]->n = (int)ts[i]->v.size();
+		dst[i]->tc[0] = ts[i]->tc[0];
+		dst[i]->tc[1] = ts[i]->tc[1];
+	}
+	c.nabsmag = (int)cfg.getd("ncomp", 2);
+	c.has_mult = true;
+}
+
+// os_photometricErrors::construct (modules/photometricErrors_host.cpp:232-283):
+// <bandset>.<band>.file = path ('internal' = <datadir>/<bandset>.<band>.photoerr.txt)
+void construct_photoerr(gf_catalog &c, const Config &cfg)
+{
+	if (!c.has_phot) fail("photometricErrors needs the photometry module");
+	std::vector<double> kx, ky;
+	c.perr.nbands = c.nbands;
+	c.perr.koff[0] = 0;
+	for (int b = 0; b < c.nbands; b++) {
+		std::string key;
+		for (auto &kvp : cfg.kv) {
+			const std::string &k = kvp.first;
+			std::string suffix = "." + c.band_names[b] + ".file";
+			if (k.size() > suffix.size() && k.compare(k.size() - suffix.size(), suffix.size(), suffix) == 0) key = k;
+		}
+		if (!key.empty()) {
+			std::string file = cfg.get(key);
+			if (file == "internal") {
+				std::string bandset = key.substr(0, key.find('.'));
+				file = c.datadir + "/" + bandset + "." + c.band_names[b] + ".photoerr.txt";
+			} else file = cfg.resolve(file);
+			auto rows = load_columns(file, 2);
+			for (auto &r : rows) { kx.push_back(r[0]); ky.push_back(r[1]); }
+		}
+		c.perr.koff[b + 1] = (int)kx.size();
+	}
+	c.perr_kx = kx;
+	c.perr_ky = ky;
+	c.perr_dev[0].upload(kx);
+	c.perr_dev[1].upload(ky);
+	c.perr.kx = (const double *)c.perr_dev[0].p;
+	c.perr.ky = (const double *)c.perr_dev[1].p;
+	c.has_perr = true;
+}
+
+std::string normalize_keyword(std::string s)
+{
+	std::string o;
+	for (char ch : s) if (ch != ' ' && ch != '\t' && ch != '_') o += (char)tolower((unsigned char)ch);
+	return o;
+}
+
+void open_catalog(gf_catalog &c, const std::string &cmdfile)
+{
+	Config cmd;
+	if (!cmd.load(cmdfile)) fail("cannot read " + cmdfile);
+	if (cmd.count("definitions")) {
+		for (auto &f : split_ws(cmd.get("definitions"))) {
+			Config d;
+			if (!d.load(cmd.resolve(f))) fail("cannot read " + f);
+			for (auto &kv : d.kv) if (kv.first != "module") c.globals.kv[kv.first] = kv.second;
+		}
+	}
+	auto loadc = [&](const std::string &f) {
+		Config x;
+		x.globals = &c.globals;
+		if (!x.load(cmd.resolve(f))) fail("cannot read " + cmd.resolve(f));
+		return x;
+	};
+	c.skygen = loadc(cmd.get("input"));
+	for (auto &f : split_ws(cmd.get("models"))) c.model_cfgs.push_back(loadc(f));
+	for (auto &f : split_ws(cmd.get("footprints"))) c.foot_cfgs.push_back(loadc(f));
+	for (auto &f : split_ws(cmd.get("modules", ""))) c.module_cfgs.push_back(loadc(f));
+	c.nstars_forced = cmd.getd("nstars", 0.);
+	if (c.foot_cfgs.size() != 1) fail("exactly one footprint configuration is supported");
+	for (auto &mc : c.model_cfgs) mc.globals = &c.globals;
+
+	double dx;
+	load_sky_pixelization(c, dx);
+	load_footprint(c, c.foot_cfgs[0]);
+	pixelize(c, dx);
+	for (auto &mc : c.model_cfgs) load_model(c, mc);
+	// hint_absmag: restrict the absolute-magnitude range to the LF's support
+	// (skygen/skyconfig_impl.h:258-263, skygen/model_lib.h:97-112); one range is
+	// shared by all models here, so only a common restriction is applied
+	for (auto &m : c.models) {
+		float X0 = m->lf.tc[0];
+		float X1 = (float)((double)(m->lf.v.size() - 1) / m->lf.tc[1] + X0);
+		float M0 = c.sp.M0, M1 = (float)(c.sp.M1 + 0.5 * c.sp.dM);
+		if (X0 > M0 || X1 < M1) fail("luminosity function does not cover the absolute-magnitude range");
+	}
+	// modules in pipeline order (pipeline.h:75-89)
+	for (auto &mc : c.module_cfgs) {
+		std::string mod = normalize_keyword(mc.get("module"));
+		if (mod == "feh") construct_FeH(c, mc);
+		else if (mod == "gaussianfeh") construct_GaussianFeH(c, mc);
+		else if (mod == "unresolvedmultiples") construct_multiples(c, mc);
+		else if (mod == "bond2010") construct_Bond2010(c, mc);
+		else if (mod == "photometry") construct_photometry(c, mc);
+		else if (mod == "vel2pm") construct_vel2pm(c, mc);
+		else if (mod == "gal2other") construct_gal2other(c, mc);
+		else if (mod == "photometricerrors") { /* needs photometry; constructed below */ }
+		else if (mod == "fitsout" || mod == "textout" || mod == "countsmap") { /* output stages: not on the hot path */ }
+		else fail("unsupported module '" + mod + "'");
+	}
+	for (auto &mc : c.module_cfgs)
+		if (normalize_keyword(mc.get("module")) == "photometricerrors") construct_photoerr(c, mc);
+	if (c.has_feh && c.has_gfeh) fail("FeH and GaussianFeH are both configured");
+
+	c.pix_dev.upload(c.pixels);
+	if (hipStreamCreate(&c.stream) != hipSuccess) fail("stream creation failed");
+	(void)hipEventCreate(&c.ev[0]);
+	(void)hipEventCreate(&c.ev[1]);
+	// persistent streams seeded from the pipeline seed (common/cux.cpp:404-435)
+	std::string pf = c.datadir + "/safeprimes32.u32";
+	std::ifstream f(pf, std::ios::binary);
+	if (!f) fail("cannot read " + pf);
+	std::vector<uint32_t> primes(70763);
+	f.read((char *)primes.data(), primes.size() * 4);
+	primes.resize(f.gcount() / 4);
+	uint32_t s32 = gf_seed_from_pipeline_seed(c.seed);
+	std::vector<uint32_t> st(3 * 65536);
+	if (gf_mwc_seed_host(s32, primes.data(), (uint32_t)primes.size(), 65536, st.data()) != 65536)
+		fail("seeding failed");
+	if (gf_rng_alloc(&c.rng, 65536) || gf_rng_upload(&c.rng, st.data(), c.stream)) fail("stream upload failed");
+	c.rng_state0 = st;
+
+	// host mirrors of the device parameter blocks
+	c.models_h.clear();
+	for (auto &m : c.models) {
+		gf_model h = m->m;
+		h.lf.data = m->lf.v.data();
+		c.models_h.push_back(h);
+	}
+	if (c.has_phot) {
+		c.phot_h = c.phot;
+		size_t per = (size_t)c.phot_tables.nFeH * c.phot_tables.nMr * 4;
+		for (int t = 0; t < 4; t++) {
+			c.phot_h.color[t] = t < c.phot_tables.ntex ? c.phot_tables.color.data() + t * per : nullptr;
+			c.phot_h.cflags[t] = t < c.phot_tables.ntex ? c.phot_tables.flags.data() + t * per : nullptr;
+		}
+	}
+	if (c.has_mult) {
+		c.mult_h = c.mult;
+		c.mult_h.secProb.data = c.mult_sec.v.data();
+		c.mult_h.cumLF.data = c.mult_cum.v.data();
+		c.mult_h.invCumLF.data = c.mult_inv.v.data();
+	}
+	if (c.has_perr) {
+		c.perr_h = c.perr;
+		c.perr_h.kx = c.perr_kx.data();
+		c.perr_h.ky = c.perr_ky.data();
+	}
+}
+
+// Allocate the device catalog table for `cap` rows.
+void ensure_table(gf_catalog &c, uint64_t cap)
+{
+	if (c.table_cap >= cap && c.table_dev.p) return;
+	struct ColDef { gf_col gf_table::*m; uint32_t elem, width; };
+	int nb = c.nbands > 0 ? c.nbands : 1;
+	ColDef defs[] = {
+		{&gf_table::lb, 8, 2}, {&gf_table::projIdx, 4, 1}, {&gf_table::projXY, 4, 2}, {&gf_table::DM, 4, 1},
+		{&gf_table::absmag, 4, 1}, {&gf_table::XYZ, 4, 3}, {&gf_table::Am, 4, 1}, {&gf_table::AmInf, 4, 1},
+		{&gf_table::comp, 4, 1}, {&gf_table::hidden, 4, 1}, {&gf_table::FeH, 4, 1}, {&gf_table::FeH_comp, 4, 1},
+		{&gf_table::absmagSys, 4, (uint32_t)std::max(c.nabsmag, 1)}, {&gf_table::absmagSysNcomp, 4, 1},
+		{&gf_table::vcyl, 4, 3}, {&gf_table::mags, 4, (uint32_t)nb}, {&gf_table::photoFlags, 4, 1},
+		{&gf_table::obsMags, 4, (uint32_t)nb}, {&gf_table::pm, 4, 3}, {&gf_table::radec, 8, 2},
+	};
+	size_t total = 0;
+	for (auto &d : defs) {
+		size_t pitch = ((cap * d.elem + 127) / 128) * 128;
+		total += pitch * d.width;
+	}
+	c.table_dev.alloc(total);
+	char *p = (char *)c.table_dev.p;
+	for (auto &d : defs) {
+		uint32_t pitch = (uint32_t)(((cap * d.elem + 127) / 128) * 128);
+		gf_col col{p, pitch, d.width, d.elem, 0};
+		c.table.*(d.m) = col;
+		p += (size_t)pitch * d.width;
+	}
+	c.table_cap = cap;
+}
+
+double now_s()
+{
+	return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+} // namespace
+
+extern "C" {
+
+const char *gf_last_error(void) { return g_last_error.c_str(); }
+
+int gf_catalog_open(const char *cmd_conf, uint32_t seed, const char *datadir, gf_catalog **out)
+{
+	GF_CHECK(cmd_conf && datadir && out, GF_E_ARG);
+	auto c = std::make_unique<gf_catalog>();
+	c->seed = seed;
+	c->datadir = datadir;
+	try {
+		open_catalog(*c, cmd_conf);
+	} catch (const ConfigError &e) {
+		g_last_error = e.msg;
+		return GF_E_CONFIG;
+	}
+	*out = c.release();
+	return 0;
+}
+
+int gf_catalog_set(gf_catalog *c, const char *key, double value)
+{
+	GF_CHECK(c && key, GF_E_ARG);
+	std::string k = key;
+	if (k == "chunk_rows") c->chunk_rows = (uint64_t)value;
+	else if (k == "acc_mode") c->acc_mode = (int)value;
+	else if (k == "nstars") c->nstars_forced = value;
+	else return GF_E_ARG;
+	return 0;
+}
+
+int gf_catalog_info(const gf_catalog *c, gf_catalog_info_t *info)
+{
+	GF_CHECK(c && info, GF_E_ARG);
+	info->npixels = (uint64_t)c->sp.npixels;
+	info->nm = c->sp.nm;
+	info->nM = c->sp.nM;
+	info->nmodels = (int)c->models.size();
+	info->nbands = c->nbands;
+	info->nabsmag = c->nabsmag;
+	info->footprint_covered_sr = 0;
+	for (auto &p : c->pixels) info->footprint_covered_sr += (double)p.coveredFraction * (double)p.dA;
+	return 0;
+}
+
+int gf_catalog_count(gf_catalog *c, gf_count_result *per_model, double *expected)
+{
+	GF_CHECK(c, GF_E_ARG);
+	double tot = 0;
+	const uint64_t K = (uint64_t)(c->sp.nm + c->sp.nM - 1);
+	for (size_t i = 0; i < c->models.size(); i++) {
+		Model &m = *c->models[i];
+		m.wpk.alloc((size_t)c->sp.npixels * K * sizeof(double));
+		int rc = gf_skygen_count(&c->sp, &m.m, (const gf_pencil_beam *)c->pix_dev.p, nullptr, c->acc_mode, &m.counts,
+		                         nullptr, (double *)m.wpk.p, c->stream);
+		if (rc) return rc;
+		if (per_model) per_model[i] = m.counts;
+		tot += m.counts.nstarsExpected;
+	}
+	// nstars renormalisation (skygen/os_skygen.cpp:886-901)
+	if (c->nstars_forced > 0 && tot > 0) {
+		double norm = c->nstars_forced / tot;
+		for (auto &m : c->models) m->m.norm = (float)norm;
+		tot = 0;
+		for (size_t i = 0; i < c->models.size(); i++) {
+			Model &m = *c->models[i];
+			int rc = gf_skygen_count(&c->sp, &m.m, (const gf_pencil_beam *)c->pix_dev.p, nullptr, c->acc_mode,
+			                         &m.counts, nullptr, (double *)m.wpk.p, c->stream);
+			if (rc) return rc;
+			if (per_model) per_model[i] = m.counts;
+			tot += m.counts.nstarsExpected;
+		}
+	}
+	if (expected) *expected = tot;
+	return 0;
+}
+
+int gf_catalog_run(gf_catalog *c, gf_chunk_fn fn, void *user, gf_run_stats *stats)
+{
+	GF_CHECK(c, GF_E_ARG);
+	gf_run_stats s{};
+	double t0 = now_s();
+	hipStream_t st = c->stream;
+	gf_chain ch{};
+	if (c->has_feh) ch.feh = &c->feh;
+	if (c->has_gfeh) { ch.gfeh = &c->gfeh; ch.gfeh_comps = c->gfeh_comps; }
+	if (c->has_mult) { ch.mult = &c->mult; ch.mult_comps = c->mult_comps; }
+	if (c->has_bond) ch.bond = &c->bond;
+	if (c->has_phot) { ch.phot = &c->phot; ch.phot_comps = c->phot_comps; }
+	if (c->has_perr) ch.perr = &c->perr;
+	if (c->has_v2pm) ch.v2pm = &c->v2pm;
+	ch.gal2other_coordsys = c->gal2other;
+	const uint64_t sub = 65536ull * 256ull;   // rows the 65,536 streams cover per call
+	for (auto &mp : c->models) {
+		Model &m = *mp;
+		if (!m.wpk.p) return GF_E_STATE;
+		gf_draw_plan *plan = nullptr;
+		uint64_t total = 0;
+		(void)hipEventRecord(c->ev[0], st);
+		int rc = gf_skygen_plan(&c->sp, &m.m, (const gf_pencil_beam *)c->pix_dev.p, nullptr, (const double *)m.wpk.p,
+		                        &c->rng, &plan, &total, st);
+		if (rc) return rc;
+		(void)hipEventRecord(c->ev[1], st);
+		(void)hipEventSynchronize(c->ev[1]);
+		float ms = 0;
+		(void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+		s.t_plan += ms * 1e-3;
+		s.generated += total;
+		s.expected += m.counts.nstarsExpectedToGenerate;
+		uint64_t chunk = std::min<uint64_t>(c->chunk_rows, std::max<uint64_t>(total, 1));
+		ensure_table(*c, chunk);
+		gf_skygen_cols sc{c->table.lb, c->table.projIdx, c->table.projXY, c->table.DM, c->table.absmag,
+		                  c->table.XYZ, c->table.Am, c->table.AmInf, c->table.comp, c->table.hidden};
+		for (uint64_t r0 = 0; r0 < total; r0 += chunk) {
+			uint64_t r1 = std::min(total, r0 + chunk);
+			(void)hipEventRecord(c->ev[0], st);
+			rc = gf_skygen_emit(plan, r0, r1, &c->foot.f, &c->rng, &sc, st);
+			if (rc) { gf_skygen_plan_free(plan); return rc; }
+			(void)hipEventRecord(c->ev[1], st);
+			(void)hipEventSynchronize(c->ev[1]);
+			(void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+			s.t_emit += ms * 1e-3;
+			s.n_emit++;
+			uint64_t n = r1 - r0;
+			(void)hipEventRecord(c->ev[0], st);
+			for (uint64_t b0 = 0; b0 < n; b0 += sub) {
+				gf_rows ks{(uint32_t)b0, 256, (uint32_t)std::min(n, b0 + sub), 0};
+				rc = gf_module_chain(&ks, &ch, &c->rng, &c->table, st);
+				if (rc) { gf_skygen_plan_free(plan); return rc; }
+				s.n_chain++;
+			}
+			(void)hipEventRecord(c->ev[1], st);
+			(void)hipEventSynchronize(c->ev[1]);
+			(void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+			s.t_chain += ms * 1e-3;
+			s.rows_chain += n;
+			uint64_t vis = 0;
+			rc = gf_count_visible(c->table.hidden, n, &vis, st);
+			if (rc) { gf_skygen_plan_free(plan); return rc; }
+			s.visible += vis;
+			if (fn) {
+				rc = fn(user, &c->table, n, m.m.comp);
+				if (rc) { gf_skygen_plan_free(plan); return rc; }
+			}
+		}
+		gf_skygen_plan_free(plan);
+	}
+	GF_HIP(hipStreamSynchronize(st));
+	s.t_total = now_s() - t0;
+	if (stats) *stats = s;
+	return 0;
+}
+
+int gf_catalog_host_view_get(const gf_catalog *c, gf_catalog_host_view *v)
+{
+	GF_CHECK(c && v, GF_E_ARG);
+	*v = gf_catalog_host_view{};
+	v->sp = c->sp;
+	v->pixels = c->pixels.data();
+	v->nmodels = (int)c->models_h.size();
+	v->nbands = c->nbands;
+	v->models = c->models_h.data();
+	v->foot = c->foot.f;
+	gf_chain &ch = v->chain;
+	if (c->has_feh) ch.feh = &c->feh;
+	if (c->has_gfeh) { ch.gfeh = &c->gfeh; ch.gfeh_comps = c->gfeh_comps; }
+	if (c->has_mult) { ch.mult = &c->mult_h; ch.mult_comps = c->mult_comps; }
+	if (c->has_bond) ch.bond = &c->bond;
+	if (c->has_phot) { ch.phot = &c->phot_h; ch.phot_comps = c->phot_comps; }
+	if (c->has_perr) ch.perr = &c->perr_h;
+	if (c->has_v2pm) ch.v2pm = &c->v2pm;
+	ch.gal2other_coordsys = c->gal2other;
+	v->nabsmag = c->nabsmag;
+	v->nstreams = 65536;
+	v->rng_state0 = c->rng_state0.data();
+	return 0;
+}
+
+int gf_lf_texture_host(const double *X, const double *Y, int n, int nsamp, float *tex, float tc[2])
+{
+	GF_CHECK(X && Y && tex && tc && n >= 1 && nsamp >= 2, GF_E_ARG);
+	Tex1D t = resample_clip_1d(std::vector<double>(X, X + n), std::vector<double>(Y, Y + n), 0.f, nsamp);
+	std::copy(t.v.begin(), t.v.end(), tex);
+	tc[0] = t.tc[0];
+	tc[1] = t.tc[1];
+	return 0;
+}
+
+int gf_photometry_tables_host(const double *rows, int nrows, int ncolors, double dFeH, double dMr,
+                              float *colors, float *flags, size_t cap_floats, int *nFeH, int *nMr, float tc[4])
+{
+	GF_CHECK(rows && nrows > 0 && ncolors >= 1 && ncolors <= 16 && colors && flags && nFeH && nMr && tc, GF_E_ARG);
+	std::vector<std::vector<double>> rr;
+	for (int r = 0; r < nrows; r++) rr.emplace_back(rows + (size_t)r * (2 + ncolors), rows + (size_t)(r + 1) * (2 + ncolors));
+	PhotTables pt;
+	try {
+		pt = build_photometry_tables(rr, ncolors, dFeH, dMr);
+	} catch (const ConfigError &e) {
+		g_last_error = e.msg;
+		return GF_E_CONFIG;
+	}
+	if (pt.color.size() > cap_floats) return GF_E_CAPACITY;
+	std::copy(pt.color.begin(), pt.color.end(), colors);
+	std::copy(pt.flags.begin(), pt.flags.end(), flags);
+	*nFeH = pt.nFeH;
+	*nMr = pt.nMr;
+	tc[0] = pt.tcFeH[0]; tc[1] = pt.tcFeH[1]; tc[2] = pt.tcMr[0]; tc[3] = pt.tcMr[1];
+	return pt.ntex;
+}
+
+void gf_catalog_close(gf_catalog *c)
+{
+	if (!c) return;
+	if (c->stream) (void)hipStreamDestroy(c->stream);
+	delete c;
+}
+
+} // extern "C"

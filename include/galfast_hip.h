@@ -437,6 +437,89 @@ int gf_skygen_emit(const gf_draw_plan *plan, uint64_t row_begin, uint64_t row_en
 
 int gf_skygen_plan_free(gf_draw_plan *plan);
 
+/* ---------------------------------------------------------------------- */
+/* Catalog pipeline (generate_catalog, pipeline.cpp:1313-1445)             */
+/* ---------------------------------------------------------------------- */
+
+/* A loaded `galfast catalog cmd.conf` job: definitions, skygen input,
+ * models, footprint and per-star modules, with every table resident on the
+ * device and the persistent RNG streams seeded from `seed`
+ * (common/cux.cpp:404-435).  Output stages (textout, fitsout, countsMap)
+ * are accepted in the module list and ignored: rows are handed to the
+ * caller's chunk callback instead. */
+typedef struct gf_catalog gf_catalog;
+
+typedef struct gf_catalog_info_t {
+    uint64_t npixels;
+    int32_t  nm, nM, nmodels, nbands, nabsmag, pad;
+    double   footprint_covered_sr;
+} gf_catalog_info_t;
+
+/* Per-run statistics; times in seconds, measured with HIP events on the
+ * catalog's stream. */
+typedef struct gf_run_stats {
+    double   expected;      /* sum of nstarsExpectedToGenerate over models */
+    uint64_t generated;     /* rows drawn (hidden included) */
+    uint64_t rows_chain;    /* rows passed through the module chain */
+    uint64_t n_emit, n_chain;
+    double   t_plan, t_emit, t_chain, t_total;
+    uint64_t visible;       /* rows with hidden == 0 */
+} gf_run_stats;
+
+/* Called once per output chunk with device columns holding `nrows` rows
+ * (hidden rows included; hidden != 0 marks rows the reference drops). */
+typedef int (*gf_chunk_fn)(void *user, const gf_table *cols, uint64_t nrows, int comp);
+
+/* Message of the last GF_E_CONFIG failure on this thread. */
+const char *gf_last_error(void);
+int  gf_catalog_open(const char *cmd_conf, uint32_t seed, const char *datadir, gf_catalog **out);
+/* Keys: "chunk_rows" (rows per output chunk, default 3e7), "acc_mode"
+ * (GF_ACC_*), "nstars" (force the expected total, os_skygen.cpp:886-901). */
+int  gf_catalog_set(gf_catalog *c, const char *key, double value);
+int  gf_catalog_info(const gf_catalog *c, gf_catalog_info_t *info);
+/* Count pass for every model (os_skygen::run, skygen/os_skygen.cpp:876-884);
+ * per_model (optional) receives nmodels results. */
+int  gf_catalog_count(gf_catalog *c, gf_count_result *per_model, double *expected);
+/* Draw every model and run the module chain chunk by chunk
+ * (skygenHost<T>::drawSources, skygen/skyconfig_impl.h:434-545). */
+int  gf_catalog_run(gf_catalog *c, gf_chunk_fn fn, void *user, gf_run_stats *stats);
+void gf_catalog_close(gf_catalog *c);
+
+/* Host-side copies of everything a catalog job was built from: the sky
+ * parameters, pixels, models (luminosity-function texels in host memory),
+ * footprint, module parameter blocks (tables in host memory) and the
+ * initial stream table.  Valid until gf_catalog_close.  Lets a CPU
+ * implementation replay the same job. */
+typedef struct gf_catalog_host_view {
+    gf_skygen_params      sp;
+    const gf_pencil_beam *pixels;
+    int32_t               nmodels;
+    int32_t               nbands;
+    const gf_model       *models;
+    gf_footprint          foot;
+    gf_chain              chain;
+    int32_t               nabsmag;
+    uint32_t              nstreams;
+    const uint32_t       *rng_state0;
+} gf_catalog_host_view;
+
+int gf_catalog_host_view_get(const gf_catalog *c, gf_catalog_host_view *view);
+
+/* Number of rows with hidden == 0 among the first n (synchronous). */
+int gf_count_visible(gf_col hidden, uint64_t n, uint64_t *visible, void *stream);
+
+/* Host-side table construction, exposed for hosts that build their own
+ * parameter blocks:
+ * - the 1024-texel luminosity-function texture with zero clipping
+ *   (load_resample_and_clip_texture_1D, common/cux.cpp:368-384);
+ * - the photometric-system colour/flag tables
+ *   (modules/photometry_host.cpp:175-349), [tex][nMr][nFeH][4] floats,
+ *   returning the number of float4 tables or a negative error. */
+int gf_lf_texture_host(const double *X, const double *Y, int n, int nsamp, float *tex, float tc[2]);
+int gf_photometry_tables_host(const double *rows, int nrows, int ncolors, double dFeH, double dMr,
+                              float *colors, float *flags, size_t cap_floats, int *nFeH, int *nMr,
+                              float tc[4]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,89 @@
+"""Replay a catalog job on the oracle (CPU).
+
+Given the host view of a ``gf_catalog`` (``gf_catalog_host_view_get``), run
+the same steps as ``gf_catalog_run`` on the oracle: per model the
+(pixel, DM-bin) expectations, the Poisson plan, the emission in the same
+output chunks and the module chain in the same 65,536 x 256-row sub-batches.
+Used by the end-to-end parity test, by smoke() and as bench.py's CPU
+baseline.  Test infrastructure.
+"""
+import ctypes as C
+import time
+
+import numpy as np
+
+from galfast_amd import _abi as A
+from galfast_amd.table import HostTable, catalog_spec
+import oracle as O
+
+
+class gf_catalog_host_view(C.Structure):
+    _fields_ = [("sp", A.gf_skygen_params), ("pixels", C.c_void_p), ("nmodels", C.c_int32),
+                ("nbands", C.c_int32), ("models", C.POINTER(A.gf_model)), ("foot", A.gf_footprint),
+                ("chain", A.gf_chain), ("nabsmag", C.c_int32), ("nstreams", C.c_uint32),
+                ("rng_state0", C.POINTER(C.c_uint32))]
+
+
+def host_view(lib, handle):
+    lib.gf_catalog_host_view_get.argtypes = [C.c_void_p, C.POINTER(gf_catalog_host_view)]
+    v = gf_catalog_host_view()
+    rc = lib.gf_catalog_host_view_get(handle, C.byref(v))
+    assert rc == 0
+    return v
+
+
+def replay(view, chunk_rows=30_000_000, pixel_index=None, on_chunk=None, time_budget=None):
+    """Run the job on the oracle.  pixel_index: optional subset of pixels.
+    Returns dict(generated, visible, seconds, stopped_early)."""
+    o = O.lib()
+    ns = view.nstreams
+    state = np.ctypeslib.as_array(view.rng_state0, shape=(3 * ns,)).copy()
+    allpix = (A.gf_pencil_beam * view.sp.npixels).from_address(view.pixels)
+    if pixel_index is None:
+        pix = allpix
+    else:
+        pix = (A.gf_pencil_beam * len(pixel_index))(*[allpix[i] for i in pixel_index])
+    sp = A.gf_skygen_params.from_buffer_copy(view.sp)
+    sp.npixels = len(pix)
+    K = sp.nm + sp.nM - 1
+    U = sp.npixels * K
+    spec = catalog_spec(nbands=max(view.nbands, 1), nabsmag=max(view.nabsmag, 1))
+    sub = 65536 * 256
+    t0 = time.perf_counter()
+    generated = visible = 0
+    stopped = False
+    foot = A.gf_footprint.from_buffer_copy(view.foot)
+    for mi in range(view.nmodels):
+        m = view.models[mi]
+        res = A.gf_count_result()
+        wpk = np.zeros(U)
+        o.gfo_count_bins(C.byref(sp), C.byref(m), C.cast(pix, C.c_void_p), None, C.byref(res), None, O.ptr(wpk))
+        counts = np.zeros(U, dtype=np.uint32)
+        o.gfo_plan(O.ptr(wpk), U, O.ptr(state), ns, O.ptr(counts))
+        offs = np.zeros(U + 1, dtype=np.uint64)
+        offs[1:] = np.cumsum(counts, dtype=np.uint64)
+        total = int(offs[-1])
+        generated += total
+        chunk = max(1, min(chunk_rows, total))
+        for r0 in range(0, total, chunk):
+            r1 = min(total, r0 + chunk)
+            n = r1 - r0
+            t = HostTable(n, spec)
+            cols = t.skygen_cols()
+            o.gfo_emit(C.byref(sp), C.byref(m), C.cast(pix, C.c_void_p), None, C.byref(foot), O.ptr(offs), U, r0,
+                       r1, O.ptr(state), ns, C.byref(cols))
+            tab = t.table()
+            for b0 in range(0, n, sub):
+                ks = A.gf_rows(b0, 256, min(n, b0 + sub), 0)
+                rc = o.gfo_module_chain(C.byref(ks), C.byref(view.chain), O.ptr(state), ns, C.byref(tab))
+                assert rc == 0
+            visible += int((t.arr("hidden")[0] == 0).sum())
+            if on_chunk is not None:
+                on_chunk(t, n, m.comp)
+            if time_budget is not None and time.perf_counter() - t0 > time_budget:
+                stopped = True
+                break
+        if stopped:
+            break
+    return {"generated": generated, "visible": visible, "seconds": time.perf_counter() - t0,
+            "stopped_early": stopped}

@@ -1,0 +1,76 @@
+"""End-to-end catalog job (config 0) on the GPU against the oracle replay."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import galfast_amd as G
+from galfast_amd import configs
+from galfast_amd.catalog import Catalog
+from galfast_amd.table import HostTable, catalog_spec
+from galfast_amd._abi import TABLE_COLUMNS
+import replay as R
+
+pytestmark = pytest.mark.gpu
+
+
+def fetch(table, n, spec):
+    """Copy n rows of every device column into a host table."""
+    L = G.lib()
+    L.gf_memcpy_d2h.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
+    h = HostTable(n, spec)
+    for name in spec:
+        col = getattr(table, name)
+        if not col.ptr:
+            continue
+        raw, p = h.buf[name]
+        assert col.elem == raw.dtype.itemsize
+        for j in range(col.width):
+            dst = raw.ctypes.data + j * p
+            G.check(L.gf_memcpy_d2h(dst, col.ptr + j * col.pitch, n * col.elem, None))
+    return h
+
+
+def collect(cat, spec):
+    out = []
+    cat.run(lambda t, n, comp: out.append((fetch(t, n, spec), n, comp)))
+    return out
+
+
+@pytest.fixture(scope="module")
+def c0(tmp_path_factory):
+    d = str(tmp_path_factory.mktemp("c0"))
+    return configs.write_c0(d)
+
+
+def test_c0_catalog_matches_oracle_replay(c0):
+    cat = Catalog(c0, seed=42, chunk_rows=40000)
+    info = cat.info()
+    assert info.nmodels == 3 and info.nbands == 5
+    expected = cat.count()
+    assert 5e4 < expected < 2e5
+    view = R.host_view(G.lib(), cat._h)
+    spec = catalog_spec(nbands=view.nbands, nabsmag=view.nabsmag)
+    got = collect(cat, spec)
+    ref = []
+    R.replay(view, chunk_rows=40000, on_chunk=lambda t, n, comp: ref.append((t, n, comp)))
+    assert [(n, c) for _, n, c in got] == [(n, c) for _, n, c in ref]
+    exact = ["projIdx", "projXY", "DM", "absmag", "comp", "Am", "AmInf", "hidden", "FeH", "mags", "photoFlags"]
+    nvis = 0
+    for (a, n, _), (b, _, _) in zip(got, ref):
+        vis = b.arr("hidden")[0] == 0
+        nvis += vis.sum()
+        for c in exact:
+            assert np.array_equal(a.arr(c)[:, vis], b.arr(c)[:, vis]), c
+        np.testing.assert_allclose(a.arr("lb")[:, vis], b.arr("lb")[:, vis], rtol=0, atol=1e-11)
+        np.testing.assert_allclose(a.arr("XYZ")[:, vis], b.arr("XYZ")[:, vis], rtol=3e-7, atol=1e-6)
+    assert nvis > 0.5 * expected
+
+
+def test_c0_generated_matches_expected(c0):
+    cat = Catalog(c0, seed=7)
+    cat.count()
+    st = cat.run()
+    assert abs(st.generated - st.expected) < 5 * np.sqrt(st.expected)
+    assert 0 < st.visible <= st.generated

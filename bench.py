@@ -32,12 +32,22 @@ WORKLOADS = {
 }
 
 PEAK_HBM_GBS = 8000.0
-# algorithmic bytes per row (stated in DESIGN.md): the draw writes the 10
-# skygen columns; the module chain reads the 7 input columns it needs and
-# writes its 7 output columns once
-EMIT_BYTES_PER_ROW = 64
-CHAIN_BYTES_PER_VISIBLE_ROW = {"c1": 48 + 88, "c0": 4 + 4 + 12 + 4 + 4 + 4 + 4 + 20 + 4}
-CHAIN_BYTES_PER_HIDDEN_ROW = 4
+# Algorithmic bytes per row of each kernel (stated in DESIGN.md): every
+# column a kernel must read, and every column it writes, once.  Hidden rows
+# cost the 4-byte hidden flag in the module kernels.
+EMIT_BYTES_PER_ROW = 64                      # the 10 skygen columns
+STAGE_BYTES_PER_VISIBLE_ROW = {
+    # stage A (FeH + Bond2010): read hidden, comp, XYZ; write FeH, vcyl
+    "k_chain_rng_a": 4 + 4 + 12 + 4 + 12,
+    # stage B (photometry, 5 bands): read hidden, comp, FeH, absmag, DM, Am; write mags, flags
+    "k_rows_phot": 4 + 4 + 4 + 4 + 4 + 4 + 20 + 4,
+    # stage C (photometric errors, 5 bands): read hidden, mags; write obsMags
+    "k_chain_rng_c": 4 + 20 + 20,
+    # stage D (vel2pm + gal2other): read hidden, lb, XYZ, vcyl; write pm, radec
+    "k_rows_astro": 4 + 16 + 12 + 12 + 12 + 16,
+}
+STAGE_NAMES = ["k_chain_rng_a", "k_rows_phot", "k_chain_rng_c", "k_rows_astro"]
+HIDDEN_ROW_BYTES = 4
 
 
 def cpu_baseline(cat, workload, budget_s):
@@ -94,7 +104,7 @@ def main():
     G.lib().gf_sync(None)
     t0 = time.perf_counter()
     acc = {"visible": 0, "generated": 0, "t_count": 0.0, "t_plan": 0.0, "t_emit": 0.0, "t_chain": 0.0,
-           "n_emit": 0, "n_chain": 0, "expected": 0.0}
+           "n_emit": 0, "n_chain": 0, "expected": 0.0, "t_stage": [0.0] * 4}
     for _ in range(args.steps):
         expected, tc, st = step()
         acc["visible"] += st.visible
@@ -106,18 +116,22 @@ def main():
         acc["n_emit"] += st.n_emit
         acc["n_chain"] += st.n_chain
         acc["expected"] += expected
+        for q in range(4):
+            acc["t_stage"][q] += st.t_stage[q]
     G.lib().gf_sync(None)
     elapsed = time.perf_counter() - t0
     value = acc["visible"] / elapsed
 
-    # roofline of the dominant kernel, from HIP-event durations on the catalog's stream
+    # roofline of the dominant kernel, from HIP-event durations recorded on
+    # the catalog's stream around each launch
     hidden = acc["generated"] - acc["visible"]
-    if acc["t_emit"] >= acc["t_chain"]:
-        name, t, n = "k_emit", acc["t_emit"], acc["n_emit"]
-        bytes_total = EMIT_BYTES_PER_ROW * acc["generated"]
-    else:
-        name, t, n = "k_chain", acc["t_chain"], acc["n_chain"]
-        bytes_total = CHAIN_BYTES_PER_VISIBLE_ROW[args.workload] * acc["visible"] + CHAIN_BYTES_PER_HIDDEN_ROW * hidden
+    kernels = {"k_emit": (acc["t_emit"], acc["n_emit"], EMIT_BYTES_PER_ROW * acc["generated"])}
+    for q, name in enumerate(STAGE_NAMES):
+        if acc["t_stage"][q] > 0:
+            kernels[name] = (acc["t_stage"][q], acc["n_chain"],
+                             STAGE_BYTES_PER_VISIBLE_ROW[name] * acc["visible"] + HIDDEN_ROW_BYTES * hidden)
+    name = max(kernels, key=lambda k: kernels[k][0])
+    t, n, bytes_total = kernels[name]
     per_launch_s = t / max(n, 1)
     achieved = bytes_total / max(n, 1) / per_launch_s / 1e9 if per_launch_s > 0 else 0.0
     out = {
@@ -138,7 +152,9 @@ def main():
                    "stars_per_step": acc["visible"] // args.steps,
                    "rows_per_step": acc["generated"] // args.steps,
                    "expected_per_step": acc["expected"] / args.steps},
-        "breakdown_ms_per_step": {k: 1000.0 * acc[k] / args.steps for k in ("t_count", "t_plan", "t_emit", "t_chain")},
+        "breakdown_ms_per_step": dict(
+            [(k, 1000.0 * acc[k] / args.steps) for k in ("t_count", "t_plan", "t_emit", "t_chain")] +
+            [(STAGE_NAMES[q], 1000.0 * acc["t_stage"][q] / args.steps) for q in range(4)]),
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": None},
     }

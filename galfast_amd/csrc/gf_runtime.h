@@ -36,6 +36,10 @@ inline unsigned blocks_for(uint64_t n, unsigned bs) { return (unsigned)((n + bs 
 // entries, offs[n] = total.  Deterministic (fixed block decomposition).
 int exclusive_scan_u32_u64(const uint32_t *in, uint64_t *offs, uint64_t n, hipStream_t st);
 
+struct gf_chain_fwd;
+int module_chain(const gf_rows *ks, const gf_chain *chain, gf_rng *rng, const gf_table *t, hipStream_t st,
+                 hipEvent_t *ev, float *ms);
+
 } // namespace gf
 
 enum gf_scratch_slot {

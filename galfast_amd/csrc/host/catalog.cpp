@@ -158,12 +158,13 @@ struct gf_catalog {
 	gf_table table{};
 	hipStream_t stream = nullptr;
 	hipEvent_t ev[2] = {nullptr, nullptr};
+	hipEvent_t sev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
 
 	~gf_catalog()
 	{
 		if (rng.state) gf_rng_free(&rng);
-		if (ev[0]) (void)hipEventDestroy(ev[0]);
-		if (ev[1]) (void)hipEventDestroy(ev[1]);
+		for (auto e : ev) if (e) (void)hipEventDestroy(e);
+		for (auto e : sev) if (e) (void)hipEventDestroy(e);
 	}
 };
 
@@ -695,6 +696,7 @@ void open_catalog(gf_catalog &c, const std::string &cmdfile)
 	if (hipStreamCreate(&c.stream) != hipSuccess) fail("stream creation failed");
 	(void)hipEventCreate(&c.ev[0]);
 	(void)hipEventCreate(&c.ev[1]);
+	for (auto &e : c.sev) (void)hipEventCreate(&e);
 	// persistent streams seeded from the pipeline seed (common/cux.cpp:404-435)
 	std::string pf = c.datadir + "/safeprimes32.u32";
 	std::ifstream f(pf, std::ios::binary);
@@ -901,8 +903,10 @@ int gf_catalog_run(gf_catalog *c, gf_chunk_fn fn, void *user, gf_run_stats *stat
 			(void)hipEventRecord(c->ev[0], st);
 			for (uint64_t b0 = 0; b0 < n; b0 += sub) {
 				gf_rows ks{(uint32_t)b0, 256, (uint32_t)std::min(n, b0 + sub), 0};
-				rc = gf_module_chain(&ks, &ch, &c->rng, &c->table, st);
+				float sms[4];
+				rc = gf::module_chain(&ks, &ch, &c->rng, &c->table, st, c->sev, sms);
 				if (rc) { gf_skygen_plan_free(plan); return rc; }
+				for (int q = 0; q < 4; q++) s.t_stage[q] += sms[q] * 1e-3;
 				s.n_chain++;
 			}
 			(void)hipEventRecord(c->ev[1], st);

@@ -464,6 +464,8 @@ typedef struct gf_run_stats {
     uint64_t n_emit, n_chain;
     double   t_plan, t_emit, t_chain, t_total;
     uint64_t visible;       /* rows with hidden == 0 */
+    double   t_stage[4];    /* module chain stages: RNG modules before photometry,
+                               photometry, photometric errors, vel2pm + gal2other */
 } gf_run_stats;
 
 /* Called once per output chunk with device columns holding `nrows` rows

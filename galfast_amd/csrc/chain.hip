@@ -41,22 +41,84 @@ GF_D bool chunk_rows(const ChainDev &c, uint32_t t, uint32_t &r0, uint32_t &r1)
 	return true;
 }
 
+// Rows per tile of the RNG stages.  A lane walks its 256-row chunk in
+// tiles: the tile's inputs are fetched with independent vector loads into
+// the lane's own LDS slots (so the loads of 16 rows are in flight together),
+// then the rows are processed in order from LDS.  Only the load schedule
+// changes; each row's arithmetic and draws are those of the module.
+constexpr int TILE = 16;
+
+template <typename T>
+GF_D void tile_load(const gf_col &c, uint32_t plane, uint32_t row0, int n, T (*dst)[CHAIN_BS], int lane, T pad)
+{
+	const T *src = &col<T>(c, row0, plane);
+	if (n == TILE && ((uintptr_t)src & 15) == 0) {
+		const uint4 *v = (const uint4 *)src;
+		uint4 q[TILE / 4];
+#pragma unroll
+		for (int i = 0; i < TILE / 4; i++) q[i] = v[i];
+#pragma unroll
+		for (int i = 0; i < TILE / 4; i++) {
+			uint32_t w[4] = {q[i].x, q[i].y, q[i].z, q[i].w};
+#pragma unroll
+			for (int k = 0; k < 4; k++) dst[4 * i + k][lane] = __builtin_bit_cast(T, w[k]);
+		}
+	} else {
+		for (int i = 0; i < TILE; i++) dst[i][lane] = i < n ? src[i] : pad;
+	}
+}
+
 // Stage A: the modules that draw random numbers before photometry
 // (FeH or GaussianFeH, unresolvedMultiples, Bond2010), one lane per chunk.
 __global__ void __launch_bounds__(CHAIN_BS) k_chain_rng_a(const ChainDev *cd, uint32_t nthr, uint32_t *st, uint32_t ns)
 {
+	__shared__ int s_hid[TILE][CHAIN_BS], s_cmp[TILE][CHAIN_BS];
+	__shared__ float s_x[TILE][CHAIN_BS], s_y[TILE][CHAIN_BS], s_z[TILE][CHAIN_BS];
 	uint32_t t = blockIdx.x * blockDim.x + threadIdx.x, r0, r1;
 	if (t >= nthr) return;
 	const ChainDev &c = *cd;
 	if (!chunk_rows(c, t, r0, r1)) return;
 	const gf_table &T = c.t;
+	const int lane = threadIdx.x;
 	gf_mwc r = gf_mwc_load(st, ns, t);
-	if (c.has_feh) mod_FeH(c.feh, r, r0, r1, T.comp, T.hidden, T.XYZ, T.FeH);
+	if (c.has_feh) {
+		for (uint32_t base = r0; base < r1; base += TILE) {
+			int n = (int)(r1 - base < (uint32_t)TILE ? r1 - base : TILE);
+			tile_load<int>(T.hidden, 0, base, n, s_hid, lane, 1);
+			tile_load<int>(T.comp, 0, base, n, s_cmp, lane, 0);
+			tile_load<float>(T.XYZ, 0, base, n, s_x, lane, 0.f);
+			tile_load<float>(T.XYZ, 1, base, n, s_y, lane, 0.f);
+			tile_load<float>(T.XYZ, 2, base, n, s_z, lane, 0.f);
+			for (int i = 0; i < n; i++) {
+				if (s_hid[i][lane]) continue;
+				float feh;
+				if (feh_row(c.feh, r, s_cmp[i][lane], s_x[i][lane], s_y[i][lane], s_z[i][lane], feh))
+					col<float>(T.FeH, base + i) = feh;
+			}
+		}
+	}
 	if (c.has_gfeh) mod_GaussianFeH(c.gfeh, c.gfeh_comps, r, r0, r1, T.comp, T.hidden, T.FeH_comp, T.FeH);
 	if (c.has_mult)
 		mod_multiples(c.mult, c.mult_comps, r, r0, r1, (int)T.absmagSys.width, T.absmag, T.absmagSys,
 		              T.absmagSysNcomp, T.comp, T.hidden);
-	if (c.has_bond) mod_Bond2010(c.bond, r, r0, r1, T.comp, T.hidden, T.XYZ, T.vcyl);
+	if (c.has_bond) {
+		for (uint32_t base = r0; base < r1; base += TILE) {
+			int n = (int)(r1 - base < (uint32_t)TILE ? r1 - base : TILE);
+			tile_load<int>(T.hidden, 0, base, n, s_hid, lane, 1);
+			tile_load<int>(T.comp, 0, base, n, s_cmp, lane, 0);
+			tile_load<float>(T.XYZ, 0, base, n, s_x, lane, 0.f);
+			tile_load<float>(T.XYZ, 1, base, n, s_y, lane, 0.f);
+			tile_load<float>(T.XYZ, 2, base, n, s_z, lane, 0.f);
+			for (int i = 0; i < n; i++) {
+				if (s_hid[i][lane]) continue;
+				float v[3];
+				if (!bond_row(c.bond, r, s_cmp[i][lane], s_x[i][lane], s_y[i][lane], s_z[i][lane], v)) continue;
+				col<float>(T.vcyl, base + i, 0) = v[0];
+				col<float>(T.vcyl, base + i, 1) = v[1];
+				col<float>(T.vcyl, base + i, 2) = v[2];
+			}
+		}
+	}
 	gf_mwc_store(st, ns, t, r);
 }
 
@@ -72,15 +134,38 @@ __global__ void __launch_bounds__(CHAIN_BS) k_rows_phot(const ChainDev *cd)
 	               T.FeH, T.comp, T.hidden);
 }
 
-// Stage C: photometric errors, one lane per chunk (draws random numbers).
+// Stage C: photometric errors, one lane per chunk (draws random numbers),
+// bands outer and rows inner as mod_photoerr, inputs fetched per tile.
 __global__ void __launch_bounds__(CHAIN_BS) k_chain_rng_c(const ChainDev *cd, uint32_t nthr, uint32_t *st, uint32_t ns)
 {
+	__shared__ int s_hid[TILE][CHAIN_BS];
+	__shared__ float s_mag[TILE][CHAIN_BS];
 	uint32_t t = blockIdx.x * blockDim.x + threadIdx.x, r0, r1;
 	if (t >= nthr) return;
 	const ChainDev &c = *cd;
 	if (!chunk_rows(c, t, r0, r1)) return;
+	const gf_table &T = c.t;
+	const gf_photoerr_data &par = c.perr;
+	const int lane = threadIdx.x;
 	gf_mwc r = gf_mwc_load(st, ns, t);
-	mod_photoerr(c.perr, r, r0, r1, c.t.mags, c.t.obsMags, c.t.hidden);
+	for (int b = 0; b < par.nbands; b++) {
+		int k0 = par.koff[b], k1 = par.koff[b + 1];
+		for (uint32_t base = r0; base < r1; base += TILE) {
+			int n = (int)(r1 - base < (uint32_t)TILE ? r1 - base : TILE);
+			tile_load<int>(T.hidden, 0, base, n, s_hid, lane, 1);
+			tile_load<float>(T.mags, b, base, n, s_mag, lane, 0.f);
+			for (int i = 0; i < n; i++) {
+				if (s_hid[i][lane]) continue;
+				float mag = s_mag[i][lane];
+				float obs = mag;
+				if (k1 > k0) {
+					float sigma = (float)spline_eval(par.kx + k0, par.ky + k0, k1 - k0, (double)mag);
+					obs = mag + r.gaussian(sigma);
+				}
+				col<float>(T.obsMags, base + i, b) = obs;
+			}
+		}
+	}
 	gf_mwc_store(st, ns, t, r);
 }
 

@@ -108,9 +108,9 @@ GF_D void lambert_deproject(const gf_lambert &p, double &l, double &b, const flo
 	double ir = 1. / sqrt(r2);
 	double sc = 1. / ir * sqrt(1. - 0.25 * r2);
 	double cc = 1. - 0.5 * r2;
-	b = asin(cc * (double)p.sb + (double)y * sc * (double)p.cb * ir);
+	b = gfm::asin(cc * (double)p.sb + (double)y * sc * (double)p.cb * ir);
 	float ysb = y * p.sb;
-	l = p.l0 + atan2((double)x * sc, (double)p.cb * cc / ir - (double)ysb * sc);
+	l = p.l0 + gfm::atan2((double)x * sc, (double)p.cb * cc / ir - (double)ysb * sc);
 }
 
 static constexpr float POGSON = 0.4605170185988091f;   // 0.2 ln 10 (skygen/skygen.cu.h:466)
@@ -134,7 +134,9 @@ GF_D int outside_footprint(const gf_footprint &f, double l, double b)
 		return t > (hi - lo) ? 1 : 0;
 	}
 	if (f.kind == 2) {
-		double cb = cos(b), sb = sin(b), cl = cos(l), sl = sin(l);
+		double cb, sb, cl, sl;
+		gfm::sincos(b, &sb, &cb);
+		gfm::sincos(l, &sl, &cl);
 		double d = cb * cl * f.cb * f.cl + cb * sl * f.cb * f.sl + sb * f.sb;
 		return d < f.cosr ? 1 : 0;
 	}

@@ -7,26 +7,36 @@
 
 static constexpr float ABSMAG_NOT_PRESENT = 99.999f;   // modules/module_lib.h:57
 
+// One row of os_FeH_kernel (modules/FeH_gpu.cu.h:60-84); false when the row's
+// component is not handled by the module.
+GF_D bool feh_row(const gf_FeH_data &par, gf_mwc &rng, int cmp, float x, float y, float z, float &feh)
+{
+	if (bit_isset(par.comp_thin, cmp) || bit_isset(par.comp_thick, cmp)) {
+		float p = rng.uniform() * (par.A[0] + par.A[1]);
+		int i = p < par.A[0] ? 0 : 1;
+		transform3(x, y, z, par.T, par.M);
+		float muD = par.muInf + par.DeltaMu * gf_expf(-gf_fabsf(z) / par.Hmu);
+		float aZ = muD - 0.067f;
+		feh = rng.gaussian(par.sigma[i]) + aZ + par.offs[i];
+		return true;
+	}
+	if (bit_isset(par.comp_halo, cmp)) {
+		feh = par.offs[2] + rng.gaussian(par.sigma[2]);
+		return true;
+	}
+	return false;
+}
+
 // os_FeH_kernel body (modules/FeH_gpu.cu.h:55-86)
 GF_D void mod_FeH(const gf_FeH_data &par, gf_mwc &rng, uint32_t r0, uint32_t r1,
                   const gf_col &comp, const gf_col &hidden, const gf_col &XYZ, const gf_col &FeH)
 {
 	for (uint32_t row = r0; row < r1; row++) {
 		if (col<int>(hidden, row)) continue;
-		int cmp = col<int>(comp, row);
-		if (bit_isset(par.comp_thin, cmp) || bit_isset(par.comp_thick, cmp)) {
-			float p = rng.uniform() * (par.A[0] + par.A[1]);
-			int i = p < par.A[0] ? 0 : 1;
-			float x = col<float>(XYZ, row, 0), y = col<float>(XYZ, row, 1), z = col<float>(XYZ, row, 2);
-			transform3(x, y, z, par.T, par.M);
-			float muD = par.muInf + par.DeltaMu * gf_expf(-gf_fabsf(z) / par.Hmu);
-			float aZ = muD - 0.067f;
-			float feh = rng.gaussian(par.sigma[i]) + aZ + par.offs[i];
+		float feh;
+		if (feh_row(par, rng, col<int>(comp, row), col<float>(XYZ, row, 0), col<float>(XYZ, row, 1),
+		            col<float>(XYZ, row, 2), feh))
 			col<float>(FeH, row) = feh;
-		} else if (bit_isset(par.comp_halo, cmp)) {
-			float feh = par.offs[2] + rng.gaussian(par.sigma[2]);
-			col<float>(FeH, row) = feh;
-		}
 	}
 }
 
@@ -82,44 +92,53 @@ GF_D void bond_trivar(float y[3], const float s[6], gf_mwc &rng)
 	y[2] += a20 * z0 + a21 * z1 + a22 * z2;
 }
 
+// One row of os_Bond2010_kernel (modules/Bond2010_gpu.cu.h:185-216); false
+// when the row's component is not handled by the module.
+GF_D bool bond_row(const gf_Bond2010_data &par, gf_mwc &rng, int cmp, float x, float y, float z, float v[3])
+{
+	transform3(x, y, z, par.T, par.M);
+	const float Rsquared = (float)(1e-6 * (double)(x * x + y * y));
+	const float Z = (float)(1e-3 * (double)z);
+	if (bit_isset(par.comp_thin, cmp) || bit_isset(par.comp_thick, cmp)) {
+		// get_disk_kinematics (modules/Bond2010_gpu.cu.h:130-140)
+		float p = rng.uniform();
+		const gf_vmoments &disk = p < par.fk ? par.disk_1 : par.disk_2;
+		for (int i = 0; i < 3; i++) v[i] = bond_modfun(Rsquared, Z, disk.m[i]);
+		if (v[1] > 0.f) v[1] = 0.f;
+		float s[6];
+		for (int i = 0; i < 6; i++) s[i] = bond_modfun(Rsquared, Z, disk.ss[i]);
+		bond_trivar(v, s, rng);
+		return true;
+	}
+	if (bit_isset(par.comp_halo, cmp)) {
+		// get_halo_kinematics (modules/Bond2010_gpu.cu.h:142-159)
+		for (int i = 0; i < 3; i++) v[i] = bond_modfun(Rsquared, Z, par.halo.m[i]);
+		float s[6];
+		for (int i = 0; i < 6; i++) s[i] = bond_modfun(Rsquared, Z, par.halo.ss[i]);
+		bond_trivar(v, s, rng);
+		float invR = 1.f / gf_sqrtf(Rsquared + Z * Z);
+		float cosa = gf_sqrtf(Rsquared) * invR;
+		float sina = Z * invR;
+		float vR = cosa * v[0] - sina * v[1];
+		float vZ = sina * v[0] + cosa * v[1];
+		v[0] = vR;
+		v[1] = v[2];
+		v[2] = vZ;
+		return true;
+	}
+	return false;
+}
+
 // os_Bond2010_kernel body (modules/Bond2010_gpu.cu.h:181-217)
 GF_D void mod_Bond2010(const gf_Bond2010_data &par, gf_mwc &rng, uint32_t r0, uint32_t r1,
                        const gf_col &comp, const gf_col &hidden, const gf_col &XYZ, const gf_col &vcyl)
 {
 	for (uint32_t row = r0; row < r1; row++) {
 		if (col<int>(hidden, row)) continue;
-		const int cmp = col<int>(comp, row);
-		float x = col<float>(XYZ, row, 0), y = col<float>(XYZ, row, 1), z = col<float>(XYZ, row, 2);
-		transform3(x, y, z, par.T, par.M);
-		const float Rsquared = (float)(1e-6 * (double)(x * x + y * y));
-		const float Z = (float)(1e-3 * (double)z);
 		float v[3];
-		if (bit_isset(par.comp_thin, cmp) || bit_isset(par.comp_thick, cmp)) {
-			// get_disk_kinematics (modules/Bond2010_gpu.cu.h:130-140)
-			float p = rng.uniform();
-			const gf_vmoments &disk = p < par.fk ? par.disk_1 : par.disk_2;
-			for (int i = 0; i < 3; i++) v[i] = bond_modfun(Rsquared, Z, disk.m[i]);
-			if (v[1] > 0.f) v[1] = 0.f;
-			float s[6];
-			for (int i = 0; i < 6; i++) s[i] = bond_modfun(Rsquared, Z, disk.ss[i]);
-			bond_trivar(v, s, rng);
-		} else if (bit_isset(par.comp_halo, cmp)) {
-			// get_halo_kinematics (modules/Bond2010_gpu.cu.h:142-159)
-			for (int i = 0; i < 3; i++) v[i] = bond_modfun(Rsquared, Z, par.halo.m[i]);
-			float s[6];
-			for (int i = 0; i < 6; i++) s[i] = bond_modfun(Rsquared, Z, par.halo.ss[i]);
-			bond_trivar(v, s, rng);
-			float invR = 1.f / gf_sqrtf(Rsquared + Z * Z);
-			float cosa = gf_sqrtf(Rsquared) * invR;
-			float sina = Z * invR;
-			float vR = cosa * v[0] - sina * v[1];
-			float vZ = sina * v[0] + cosa * v[1];
-			v[0] = vR;
-			v[1] = v[2];
-			v[2] = vZ;
-		} else {
+		if (!bond_row(par, rng, col<int>(comp, row), col<float>(XYZ, row, 0), col<float>(XYZ, row, 1),
+		              col<float>(XYZ, row, 2), v))
 			continue;
-		}
 		col<float>(vcyl, row, 0) = v[0];
 		col<float>(vcyl, row, 1) = v[1];
 		col<float>(vcyl, row, 2) = v[2];
@@ -141,10 +160,9 @@ static constexpr double d2r = 0.01745329251994329576923691;
 GF_D void transform_pm(float &vlout, float &vbout, float l, float b, float vl, float vb,
                        const double ce, const double se, const double l0)
 {
-	double cb = cos((double)b);
-	double sb = sin((double)b);
-	double cl = cos((double)l - l0);
-	double sl = sin((double)l - l0);
+	double cb, sb, cl, sl;
+	gfm::sincos((double)b, &sb, &cb);
+	gfm::sincos((double)l - l0, &sl, &cl);
 	double tmp1 = cb * se - ce * sb * sl;
 	double tmp2 = ce * sb - cb * se * sl;
 	double tmp4 = cb * cl;
@@ -224,12 +242,11 @@ GF_D void mod_gal2other(int coordsys, uint32_t r0, uint32_t r1, const gf_col &hi
 		double ly = col<double>(lb0, row, 1) * d2r;
 		double rx, ry;
 		if (coordsys == 1) {
-			const double cb = cos(ly);
-			const double sb = sin(ly);
-			const double cl = cos(lx - l0);
-			const double sl = sin(lx - l0);
-			rx = atan2(cb * cl, sb * ce - cb * se * sl) + angp;
-			ry = asin(cb * ce * sl + sb * se);
+			double cb, sb, cl, sl;
+			gfm::sincos(ly, &sb, &cb);
+			gfm::sincos(lx - l0, &sl, &cl);
+			rx = gfm::atan2(cb * cl, sb * ce - cb * se * sl) + angp;
+			ry = gfm::asin(cb * ce * sl + sb * se);
 			while (rx < 0.) rx += pi2;
 			while (rx >= pi2) rx -= pi2;
 		} else {
@@ -385,13 +402,26 @@ GF_D void mod_multiples(const gf_multiples_data &par, gf_bitmap apply, gf_mwc &r
 
 // Linear spline with linear extrapolation past the end knots (the GSL
 // linear interpolation behind spline::operator(), common/spline.cpp:59-65).
+// The interval is the largest lo in [0, n-2] with kx[lo] <= x (0 below the
+// first knot), as a binary search finds it; it is guessed from the mean knot
+// spacing first and the search runs only when the guess is not that interval.
 GF_D double spline_eval(const double *kx, const double *ky, int n, double x)
 {
 	if (n == 1) return ky[0];
-	int lo = 0, hi = n - 1;
-	while (hi > lo + 1) {
-		int mid = (lo + hi) >> 1;
-		if (kx[mid] > x) hi = mid; else lo = mid;
+	int lo;
+	double x0 = kx[0], xn = kx[n - 1];
+	double g = (x - x0) * ((double)(n - 1) / (xn - x0));
+	int gi = g < 0. ? 0 : (g > (double)(n - 2) ? n - 2 : (int)g);
+	bool ok = (gi == 0 || kx[gi] <= x) && (gi == n - 2 || kx[gi + 1] > x);
+	if (ok) {
+		lo = gi;
+	} else {
+		lo = 0;
+		int hi = n - 1;
+		while (hi > lo + 1) {
+			int mid = (lo + hi) >> 1;
+			if (kx[mid] > x) hi = mid; else lo = mid;
+		}
 	}
 	double dx = kx[lo + 1] - kx[lo];
 	double D = (ky[lo + 1] - ky[lo]) / dx;

@@ -281,40 +281,55 @@ struct StarSink {
 	uint64_t row;
 };
 
-// One star of draw_stars (skygen/skygen.cu.h:373-456), followed by the
-// clipper's footprint test.
-GF_D void draw_one_star(const gf_skygen_params &sp, const gf_model &m, const gf_extinction &ext,
-                        const gf_footprint &foot, const gf_pencil_beam &pix, float M, int im, float AmMin,
-                        gf_mwc &rng, StarSink &s)
+// The random part of one star of draw_stars (skygen/skygen.cu.h:376-385,
+// 404-405): position inside the pixel (2 uniforms) and, when that stays
+// inside the projection, the absolute and apparent magnitude (2 more).
+GF_D bool star_draws(const gf_skygen_params &sp, const gf_pencil_beam &pix, float M, int im, gf_mwc &rng,
+                     float &x, float &y, float &Mtmp, float &mtmp)
 {
-	float x = pix.X, y = pix.Y;
+	x = pix.X;
+	y = pix.Y;
 	x += pix.dx * (rng.uniform() - 0.5f);
 	y += pix.dx * (rng.uniform() - 0.5f);
-	const uint32_t row = (uint32_t)s.row;
-	if (x * x + y * y > 2.f) {
-		if (s.write) {
-			col<int>(s.c.hidden, row) = 1;
-			col<int>(s.c.projIdx, row) = pix.projIdx;
-			col<float>(s.c.projXY, row, 0) = x; col<float>(s.c.projXY, row, 1) = y;
-			col<double>(s.c.lb, row, 0) = 0.; col<double>(s.c.lb, row, 1) = 0.;
-			col<float>(s.c.M, row) = 0.f; col<float>(s.c.DM, row) = 0.f;
-			col<float>(s.c.XYZ, row, 0) = 0.f; col<float>(s.c.XYZ, row, 1) = 0.f; col<float>(s.c.XYZ, row, 2) = 0.f;
-			col<int>(s.c.comp, row) = m.comp;
-			col<float>(s.c.Am, row) = 0.f; col<float>(s.c.AmInf, row) = 0.f;
-		}
-		return;
-	}
+	if (x * x + y * y > 2.f) return false;
+	Mtmp = M + sp.dM * (rng.uniform() - 0.5f);
+	mtmp = sp.m0 + sp.dm * ((float)im + rng.uniform() - 0.5f);
+	return true;
+}
+
+// Columns of a star outside the projection: only `hidden` is meaningful
+// (the reference writes nothing else); the rest are set to zero.
+GF_D void write_star_outside(const gf_skygen_cols &c, uint32_t row, const gf_pencil_beam &pix, int comp, float x,
+                             float y)
+{
+	col<int>(c.hidden, row) = 1;
+	col<int>(c.projIdx, row) = pix.projIdx;
+	col<float>(c.projXY, row, 0) = x; col<float>(c.projXY, row, 1) = y;
+	col<double>(c.lb, row, 0) = 0.; col<double>(c.lb, row, 1) = 0.;
+	col<float>(c.M, row) = 0.f; col<float>(c.DM, row) = 0.f;
+	col<float>(c.XYZ, row, 0) = 0.f; col<float>(c.XYZ, row, 1) = 0.f; col<float>(c.XYZ, row, 2) = 0.f;
+	col<int>(c.comp, row) = comp;
+	col<float>(c.Am, row) = 0.f; col<float>(c.AmInf, row) = 0.f;
+}
+
+// The deterministic rest of draw_stars (skygen/skygen.cu.h:387-456):
+// deprojection, distance, XYZ, extinction and the flux limit, followed by
+// the clipper's footprint test.
+GF_D void write_star(const gf_skygen_params &sp, int comp, const gf_extinction &ext, const gf_footprint &foot,
+                     const gf_pencil_beam &pix, float x, float y, float Mtmp, float mtmp, float AmMin,
+                     const gf_skygen_cols &c, uint32_t row)
+{
 	double l, b;
 	lambert_deproject(sp.proj[pix.projIdx], l, b, x, y);
 	int outside = outside_footprint(foot, l, b);
-	float dcl = (float)cos(l), dcb = (float)cos(b), dsl = (float)sin(l), dsb = (float)sin(b);
+	double sld, cld, sbd, cbd;
+	gfm::sincos(l, &sld, &cld);
+	gfm::sincos(b, &sbd, &cbd);
+	float dcl = (float)cld, dcb = (float)cbd, dsl = (float)sld, dsb = (float)sbd;
 	l *= DBL_R2D;
 	if (l < 0.) l += 360.;
 	if (l > 360.) l -= 360.;
 	b *= DBL_R2D;
-
-	float Mtmp = M + sp.dM * (rng.uniform() - 0.5f);
-	float mtmp = sp.m0 + sp.dm * ((float)im + rng.uniform() - 0.5f);
 	float DM = mtmp - Mtmp;
 	float D = gf_powf(10.f, 0.2f * DM + 1.f);
 	float X = D * dcl * dcb, Y = D * dsl * dcb, Z = D * dsb;
@@ -328,22 +343,34 @@ GF_D void draw_one_star(const gf_skygen_params &sp, const gf_model &m, const gf_
 	if (AmMin > Am0) AmInf = -AmMin;
 	int hidden = (mtmp + Am0 > sp.m1) ? 1 : 0;
 	hidden |= outside;
-	if (s.write) {
-		col<int>(s.c.projIdx, row) = pix.projIdx;
-		col<float>(s.c.projXY, row, 0) = x;
-		col<float>(s.c.projXY, row, 1) = y;
-		col<double>(s.c.lb, row, 0) = l;
-		col<double>(s.c.lb, row, 1) = b;
-		col<float>(s.c.M, row) = Mtmp;
-		col<float>(s.c.DM, row) = DM;
-		col<float>(s.c.XYZ, row, 0) = X;
-		col<float>(s.c.XYZ, row, 1) = Y;
-		col<float>(s.c.XYZ, row, 2) = Z;
-		col<int>(s.c.comp, row) = m.comp;
-		col<float>(s.c.Am, row) = Am0;
-		col<float>(s.c.AmInf, row) = AmInf;
-		col<int>(s.c.hidden, row) = hidden;
+	col<int>(c.projIdx, row) = pix.projIdx;
+	col<float>(c.projXY, row, 0) = x;
+	col<float>(c.projXY, row, 1) = y;
+	col<double>(c.lb, row, 0) = l;
+	col<double>(c.lb, row, 1) = b;
+	col<float>(c.M, row) = Mtmp;
+	col<float>(c.DM, row) = DM;
+	col<float>(c.XYZ, row, 0) = X;
+	col<float>(c.XYZ, row, 1) = Y;
+	col<float>(c.XYZ, row, 2) = Z;
+	col<int>(c.comp, row) = comp;
+	col<float>(c.Am, row) = Am0;
+	col<float>(c.AmInf, row) = AmInf;
+	col<int>(c.hidden, row) = hidden;
+}
+
+// One star of draw_stars with the clipper's footprint test.
+GF_D void draw_one_star(const gf_skygen_params &sp, const gf_model &m, const gf_extinction &ext,
+                        const gf_footprint &foot, const gf_pencil_beam &pix, float M, int im, float AmMin,
+                        gf_mwc &rng, StarSink &s)
+{
+	float x, y, Mtmp, mtmp;
+	const uint32_t row = (uint32_t)s.row;
+	if (!star_draws(sp, pix, M, im, rng, x, y, Mtmp, mtmp)) {
+		if (s.write) write_star_outside(s.c, row, pix, m.comp, x, y);
+		return;
 	}
+	if (s.write) write_star(sp, m.comp, ext, foot, pix, x, y, Mtmp, mtmp, AmMin, s.c, row);
 }
 
 // kernel<1> of the reference with unlimited output capacity.  Pass 0 counts
@@ -469,6 +496,79 @@ __global__ void __launch_bounds__(SK_BS) k_emit(gf_skygen_params sp, gf_model m,
 		}
 	}
 	if (touched) gf_mwc_store(st, ns, s, rng);
+}
+
+// Random half of the emit for maps without extinction: per star the
+// absolute-magnitude bin and the star_draws values, parked in the output
+// columns (projXY = x, y; M = Mtmp; DM = mtmp; comp = pixel index;
+// hidden = 1 outside the projection).  k_emit_rows finishes every row.
+__global__ void __launch_bounds__(SK_BS) k_emit_rng(gf_skygen_params sp, gf_model m, const gf_pencil_beam *pixels,
+                                                    gf_extinction ext, const double *Pg, const uint64_t *offs,
+                                                    uint64_t U, uint64_t row_begin, uint64_t row_end, uint64_t u_lo,
+                                                    uint32_t *st, uint32_t ns, gf_skygen_cols out)
+{
+	extern __shared__ double P[];
+	for (int i = threadIdx.x; i <= sp.nM; i += blockDim.x) P[i] = Pg[i];
+	__syncthreads();
+	uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= ns) return;
+	const int K = sp.nm + sp.nM - 1;
+	uint64_t first = u_lo + ((uint64_t)s + ns - (u_lo % ns)) % ns;
+	gf_mwc rng = gf_mwc_load(st, ns, s);
+	bool touched = false;
+	for (uint64_t u = first; u < U; u += ns) {
+		uint64_t o0 = offs[u], o1 = offs[u + 1];
+		if (o0 >= row_end) break;
+		if (o1 <= row_begin || o1 == o0) continue;
+		touched = true;
+		int p = (int)(u / K), k = (int)(u % K);
+		const gf_pencil_beam pix = pixels[p];
+		float D, Am, x, y, z;
+		dm_bin(sp, ext, pix, k, D, Am, x, y, z);
+		int lo, hi;
+		dm_window(sp, k, D, Am, lo, hi);
+		double base = P[lo], span = P[hi + 1] - P[lo];
+		uint64_t j0 = row_begin > o0 ? row_begin : o0;
+		uint64_t j1 = row_end < o1 ? row_end : o1;
+		for (uint64_t j = j0; j < j1; j++) {
+			double tgt = base + (double)rng.uniform() * span;
+			int a = lo, b = hi;
+			while (a < b) {
+				int mid = (a + b) >> 1;
+				if (P[mid + 1] > tgt) b = mid; else a = mid + 1;
+			}
+			int iM = a;
+			float M = sp.M1 - iM * sp.dM;
+			float sx, sy, Mtmp = 0.f, mtmp = 0.f;
+			bool in = star_draws(sp, pix, M, k - iM, rng, sx, sy, Mtmp, mtmp);
+			uint32_t row = (uint32_t)(j - row_begin);
+			col<float>(out.projXY, row, 0) = sx;
+			col<float>(out.projXY, row, 1) = sy;
+			col<float>(out.M, row) = Mtmp;
+			col<float>(out.DM, row) = mtmp;
+			col<int>(out.comp, row) = p;
+			col<int>(out.hidden, row) = in ? 0 : 1;
+		}
+	}
+	if (touched) gf_mwc_store(st, ns, s, rng);
+}
+
+// Deterministic half of the emit, one lane per row.
+__global__ void __launch_bounds__(SK_BS) k_emit_rows(gf_skygen_params sp, int comp, const gf_pencil_beam *pixels,
+                                                     gf_extinction ext, gf_footprint foot, uint32_t nrows,
+                                                     gf_skygen_cols out)
+{
+	uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+	if (row >= nrows) return;
+	int p = col<int>(out.comp, row);
+	const gf_pencil_beam pix = pixels[p];
+	float x = col<float>(out.projXY, row, 0), y = col<float>(out.projXY, row, 1);
+	if (col<int>(out.hidden, row)) {
+		write_star_outside(out, row, pix, comp, x, y);
+		return;
+	}
+	float Mtmp = col<float>(out.M, row), mtmp = col<float>(out.DM, row);
+	write_star(sp, comp, ext, foot, pix, x, y, Mtmp, mtmp, 0.f, out, row);
 }
 
 __global__ void k_find_unit(const uint64_t *offs, uint64_t U, uint64_t row, uint64_t *res)
@@ -632,9 +732,22 @@ int gf_skygen_emit(const gf_draw_plan *plan, uint64_t row_begin, uint64_t row_en
 	GF_HIP(hipMemcpyAsync(&u_lo, plan->tmp, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
 	GF_HIP(hipStreamSynchronize(st));
 	size_t shb = ((size_t)plan->sp.nM + 1) * sizeof(double);
-	k_emit<<<gf::blocks_for(rng->nstreams, SK_BS), SK_BS, shb, st>>>(
-	    plan->sp, plan->model, plan->pixels, plan->ext, fp, plan->P, plan->offs, plan->U, row_begin, row_end, u_lo,
+	if (plan->ext.beam.data || plan->ext.north.data || plan->ext.south.data) {
+		// with extinction the star needs its DM bin's minimum extinction: one pass
+		k_emit<<<gf::blocks_for(rng->nstreams, SK_BS), SK_BS, shb, st>>>(
+		    plan->sp, plan->model, plan->pixels, plan->ext, fp, plan->P, plan->offs, plan->U, row_begin, row_end,
+		    u_lo, rng->state, rng->nstreams, *out);
+		GF_LAUNCHED();
+		return 0;
+	}
+	k_emit_rng<<<gf::blocks_for(rng->nstreams, SK_BS), SK_BS, shb, st>>>(
+	    plan->sp, plan->model, plan->pixels, plan->ext, plan->P, plan->offs, plan->U, row_begin, row_end, u_lo,
 	    rng->state, rng->nstreams, *out);
+	GF_LAUNCHED();
+	uint64_t n = row_end - row_begin;
+	GF_CHECK(n < (1ull << 32), GF_E_ARG);
+	k_emit_rows<<<gf::blocks_for(n, SK_BS), SK_BS, 0, st>>>(plan->sp, plan->model.comp, plan->pixels, plan->ext, fp,
+	                                                          (uint32_t)n, *out);
 	GF_LAUNCHED();
 	return 0;
 }

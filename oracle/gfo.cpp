@@ -425,9 +425,9 @@ static void deproject(const gf_lambert &p, double &l, double &b, float x, float 
 	double ir = 1. / sqrt(r2);
 	double sc = 1. / ir * sqrt(1. - 0.25 * r2);
 	double cc = 1. - 0.5 * r2;
-	b = asin(cc * (double)p.sb + (double)y * sc * (double)p.cb * ir);
+	b = gfo_asin(cc * (double)p.sb + (double)y * sc * (double)p.cb * ir);
 	float ysb = y * p.sb;
-	l = p.l0 + atan2((double)x * sc, (double)p.cb * cc / ir - (double)ysb * sc);
+	l = p.l0 + gfo_atan2((double)x * sc, (double)p.cb * cc / ir - (double)ysb * sc);
 }
 
 GFO_API void gfo_deproject(const gf_lambert *p, float x, float y, double *l, double *b) { deproject(*p, *l, *b, x, y); }
@@ -447,7 +447,9 @@ static int outside_foot(const gf_footprint &f, double l, double b)
 		return t > (f.l1 - f.l0) ? 1 : 0;
 	}
 	if (f.kind == 2) {
-		double cb = cos(b), sb = sin(b), cl = cos(l), sl = sin(l);
+		double cb, sb, cl, sl;
+		gfo_sincos(b, &sb, &cb);
+		gfo_sincos(l, &sl, &cl);
 		double d = cb * cl * f.cb * f.cl + cb * sl * f.cb * f.sl + sb * f.sb;
 		return d < f.cosr ? 1 : 0;
 	}
@@ -537,7 +539,10 @@ static void one_star(const gf_skygen_params &sp, const gf_model &m, const gf_ext
 	double l, b;
 	deproject(sp.proj[pix.projIdx], l, b, x, y);
 	int outside = outside_foot(foot, l, b);
-	float cl = (float)cos(l), cb = (float)cos(b), sl = (float)sin(l), sb = (float)sin(b);
+	double sld, cld, sbd, cbd;
+	gfo_sincos(l, &sld, &cld);
+	gfo_sincos(b, &sbd, &cbd);
+	float cl = (float)cld, cb = (float)cbd, sl = (float)sld, sb = (float)sbd;
 	l *= R2D;
 	if (l < 0.) l += 360.;
 	if (l > 360.) l -= 360.;
@@ -941,7 +946,9 @@ static const double D2R = 0.01745329251994329576923691;
 /* transform_pm (modules/vel2pm_gpu.cu.h:134-164) */
 static void transform_pm(float &vlout, float &vbout, float l, float b, float vl, float vb, double ce, double se, double l0)
 {
-	double cb = cos((double)b), sb = sin((double)b), cl = cos((double)l - l0), sl = sin((double)l - l0);
+	double cb, sb, cl, sl;
+	gfo_sincos((double)b, &sb, &cb);
+	gfo_sincos((double)l - l0, &sl, &cl);
 	double tmp1 = cb * se - ce * sb * sl;
 	double tmp2 = ce * sb - cb * se * sl;
 	double tmp4 = cb * cl;
@@ -1003,9 +1010,11 @@ static void vel2pm_rows(const gf_vel2pm_data &par, uint32_t r0, uint32_t r1, con
 /* galequ (modules/gal2other_gpu.cu.h:39-60), radians in and out */
 GFO_API void gfo_galequ(double l, double b, double *ra, double *dec)
 {
-	double cb = cos(b), sb = sin(b), cl = cos(l - GE_L0), sl = sin(l - GE_L0);
-	double rx = atan2(cb * cl, sb * GE_CE - cb * GE_SE * sl) + GE_ANGP;
-	double ry = asin(cb * GE_CE * sl + sb * GE_SE);
+	double cb, sb, cl, sl;
+	gfo_sincos(b, &sb, &cb);
+	gfo_sincos(l - GE_L0, &sl, &cl);
+	double rx = gfo_atan2(cb * cl, sb * GE_CE - cb * GE_SE * sl) + GE_ANGP;
+	double ry = gfo_asin(cb * GE_CE * sl + sb * GE_SE);
 	while (rx < 0.) rx += 6.28318530717958647692528677;
 	while (rx >= 6.28318530717958647692528677) rx -= 6.28318530717958647692528677;
 	*ra = rx;
@@ -1251,6 +1260,21 @@ GFO_API int gfo_photometry_tables(const double *rows, int nrows, int ncolors, do
 	*nFeH_out = nFeH;
 	*nMr_out = nMr;
 	return ntex;
+}
+
+/* the oracle's double functions: 0 sin, 1 cos, 2 atan2(x, y), 3 asin, 4 atan */
+GFO_API void gfo_math_d(int fn, const double *x, const double *y, double *out, size_t n)
+{
+	for (size_t i = 0; i < n; i++) {
+		double s, c;
+		switch (fn) {
+		case 0: gfo_sincos(x[i], &s, &c); out[i] = s; break;
+		case 1: gfo_sincos(x[i], &s, &c); out[i] = c; break;
+		case 2: out[i] = gfo_atan2(x[i], y[i]); break;
+		case 3: out[i] = gfo_asin(x[i]); break;
+		case 4: out[i] = gfo_atan(x[i]); break;
+		}
+	}
 }
 
 /* glibc's float functions, for pinning the oracle's own */

@@ -99,6 +99,45 @@ static inline void gfo_sincos(double x, double *sn, double *cs)
 	}
 }
 
+/* double atan: two half-angle reductions t -> t/(1+sqrt(1+t^2)) take
+ * |t| <= 1 below tan(pi/16); odd Taylor series to t^25; |x| > 1 via
+ * pi/2 - atan(1/x). */
+static inline double gfo_atan(double x)
+{
+	if (x != x) return x;
+	double ax = fabs(x);
+	int inv = ax > 1.0;
+	double t = inv ? 1.0 / ax : ax;
+	for (int i = 0; i < 2; i++) t = t / (1.0 + sqrt(fma(t, t, 1.0)));
+	double z = t * t;
+	double p = 1.0 / 25.0;
+	for (int n = 10; n >= 0; n--) p = fma(p, z, ((n & 1) ? 1.0 : -1.0) / (double)(2 * n + 3));
+	double a = fma(t * z, p, t) * 4.0;
+	if (inv) a = (GFO_PIO2_HI - a) + GFO_PIO2_LO;
+	return x < 0 ? -a : a;
+}
+
+static inline double gfo_atan2(double y, double x)
+{
+	const double PI_HI = 3.14159265358979311600e+00, PI_LO = 1.22464679914735317720e-16;
+	if (x != x || y != y) return x + y;
+	if (x == 0.0 && y == 0.0) {
+		double r = signbit(x) ? PI_HI : 0.0;
+		return signbit(y) ? -r : r;
+	}
+	double ax = fabs(x), ay = fabs(y), a;
+	if (ay <= ax) a = gfo_atan(ay / ax);
+	else a = (GFO_PIO2_HI - gfo_atan(ax / ay)) + GFO_PIO2_LO;
+	if (x < 0) a = (PI_HI - a) + PI_LO;
+	return (y < 0 || (y == 0.0 && signbit(y))) ? -a : a;
+}
+
+static inline double gfo_asin(double x)
+{
+	if (!(x >= -1.0 && x <= 1.0)) return NAN;
+	return gfo_atan2(x, sqrt((1.0 - x) * (1.0 + x)));
+}
+
 static inline float gfo_expf(float x) { return (float)gfo_exp((double)x); }
 static inline float gfo_logf(float x) { return (float)gfo_log((double)x); }
 static inline float gfo_exp10f(float x) { return (float)gfo_exp((double)x * 2.30258509299404590109e+00); }

@@ -66,7 +66,7 @@ def test_count_matches_oracle(sky, mi):
     assert res.nstarsExpectedToGenerate == pytest.approx(ref.nstarsExpectedToGenerate, rel=1e-12)
     assert res.nstarsExpected == pytest.approx(ref.nstarsExpected, rel=1e-12)
     assert res.ncells == ref.ncells
-    np.testing.assert_allclose(w, rw, rtol=1e-13, atol=0)
+    assert np.array_equal(w, rw)
     np.testing.assert_allclose(per, rper, rtol=1e-12, atol=0)
     # the reference's own cell-by-cell float expectations summed in the
     # crawl order of kernel<0> agree to the stated 1e-6
@@ -86,14 +86,10 @@ def test_count_precision_variants(sky):
     assert rf.nstarsExpectedToGenerate == pytest.approx(r64.nstarsExpectedToGenerate, rel=1e-4)
 
 
-def _cols_equal(a, b, n, double_tol=1e-11):
-    hid = a.arr("hidden")[0, :n]
-    assert np.array_equal(hid, b.arr("hidden")[0, :n])
-    for c in ("projIdx", "projXY", "DM", "absmag", "comp", "Am", "AmInf"):
-        assert np.array_equal(a.arr(c)[:, :n], b.arr(c)[:, :n]), c
-    np.testing.assert_allclose(a.arr("lb")[:, :n], b.arr("lb")[:, :n], rtol=0, atol=double_tol)
-    # XYZ uses float cosines of the double (l, b): one float ulp at most
-    np.testing.assert_allclose(a.arr("XYZ")[:, :n], b.arr("XYZ")[:, :n], rtol=3e-7, atol=1e-6)
+def _cols_equal(a, b, n):
+    for c in ("hidden", "projIdx", "projXY", "lb", "DM", "absmag", "XYZ", "comp", "Am", "AmInf"):
+        bad = a.arr(c)[:, :n] != b.arr(c)[:, :n]
+        assert not bad.any(), "%s: %d of %d differ" % (c, bad.sum(), bad.size)
 
 
 @pytest.mark.parametrize("mi", [0, 2])

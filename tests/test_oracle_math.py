@@ -60,3 +60,30 @@ def test_identities():
     assert np.abs(s.astype(np.float64) ** 2 + c.astype(np.float64) ** 2 - 1).max() < 3e-7
     y = RNG.uniform(0.1, 10, 10000).astype(np.float32)
     assert np.abs(O.math_f(0, O.math_f(1, y)) / y - 1).max() < 3e-7
+
+
+def test_double_functions_against_numpy():
+    import ctypes as C
+    o = O.lib()
+    o.gfo_math_d.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+
+    def run(fn, x, y=None):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y if y is not None else np.zeros_like(x), dtype=np.float64)
+        out = np.empty_like(x)
+        o.gfo_math_d(fn, O.ptr(x), O.ptr(y), O.ptr(out), x.size)
+        return out
+
+    x = RNG.uniform(-30, 30, 100000)
+    assert np.abs(run(0, x) - np.sin(x)).max() < 5e-16
+    assert np.abs(run(1, x) - np.cos(x)).max() < 5e-16
+    y, z = RNG.normal(size=100000), RNG.normal(size=100000)
+    y[:10] = 0.0
+    ref = np.arctan2(y, z)
+    assert np.abs(run(2, y, z) - ref).max() < 2e-15
+    assert np.all(np.sign(run(2, y, z)) == np.sign(ref))
+    a = RNG.uniform(-1, 1, 100000)
+    a[:4] = [-1.0, 1.0, 0.0, 0.999999999999]
+    assert np.abs(run(3, a) - np.arcsin(a)).max() < 2e-15
+    t = np.tan(RNG.uniform(-1.5, 1.5, 100000))
+    assert np.abs(run(4, t) - np.arctan(t)).max() < 2e-15

@@ -210,24 +210,69 @@ void deproj_pole(int hemi, double x, double y, double &l, double &b)
 	l = fmod(l, 2 * M_PI); if (l < 0) l += 2 * M_PI;
 }
 
-void pixelize(gf_catalog &c, double dx)
+// Project (l, b) [radians] into hemisphere `hemi`'s Lambert plane.
+void proj_pole(int hemi, double l, double b, double &x, double &y)
 {
-	c.pixels.clear();
+	if (hemi == 0) {
+		double kp = sqrt(2.0 / (1.0 + sin(b)));
+		x = kp * cos(b) * sin(l - M_PI / 2);
+		y = -kp * cos(b) * cos(l - M_PI / 2);
+	} else {
+		double kp = sqrt(2.0 / (1.0 - sin(b)));
+		x = kp * cos(b) * sin(l - 3 * M_PI / 2);
+		y = kp * cos(b) * cos(l - 3 * M_PI / 2);
+	}
+}
+
+// Pixels of side dx covering a footprint, in both hemispheres, on grids
+// centred on the poles.  For a beam only the grid cells around the beam are
+// examined.
+std::vector<gf_pencil_beam> pixelize_footprint(const gf_footprint &fp, double dx)
+{
+	std::vector<gf_pencil_beam> out;
 	const int sub = 8;
+	const double R = sqrt(2.0);
+	int n = (int)ceil(2 * R / dx);
+	double x0 = -n * dx / 2, y0 = -n * dx / 2;
 	for (int hemi = 0; hemi < 2; hemi++) {
-		// bounding box: the whole projected hemisphere disk (radius sqrt 2),
-		// restricted to where the footprint has points on a coarse grid
-		const double R = sqrt(2.0);
-		int n = (int)ceil(2 * R / dx);
-		double x0 = -n * dx / 2, y0 = -n * dx / 2;
-		for (int i = 0; i < n; i++) {
-			for (int j = 0; j < n; j++) {
+		int i0 = 0, i1 = n, j0 = 0, j1 = n;
+		if (fp.kind == 2) {
+			// grid cells around the beam: project a polar sampling of the cap
+			double cl0 = fp.cl, sl0 = fp.sl, cb0 = fp.cb, sb0 = fp.sb;
+			double r = acos(std::max(-1.0, std::min(1.0, fp.cosr)));
+			double xmin = 1e9, xmax = -1e9, ymin = 1e9, ymax = -1e9;
+			bool any = false;
+			// unit vectors of the centre and two tangent directions
+			double c[3] = {cb0 * cl0, cb0 * sl0, sb0};
+			double e1[3] = {-sl0, cl0, 0};
+			double e2[3] = {-sb0 * cl0, -sb0 * sl0, cb0};
+			for (int ir = 0; ir <= 24; ir++) {
+				double rr = r * ir / 24.0;
+				int na = ir == 0 ? 1 : 96;
+				for (int ia = 0; ia < na; ia++) {
+					double a = 2 * M_PI * ia / na;
+					double v[3];
+					for (int q = 0; q < 3; q++) v[q] = cos(rr) * c[q] + sin(rr) * (cos(a) * e1[q] + sin(a) * e2[q]);
+					double b = asin(std::max(-1.0, std::min(1.0, v[2]))), l = atan2(v[1], v[0]);
+					if ((hemi == 0 && b < 0) || (hemi == 1 && b >= 0)) continue;
+					double x, y;
+					proj_pole(hemi, l, b, x, y);
+					xmin = std::min(xmin, x); xmax = std::max(xmax, x);
+					ymin = std::min(ymin, y); ymax = std::max(ymax, y);
+					any = true;
+				}
+			}
+			if (!any) continue;
+			i0 = std::max(0, (int)floor((xmin - x0) / dx) - 2);
+			i1 = std::min(n, (int)floor((xmax - x0) / dx) + 3);
+			j0 = std::max(0, (int)floor((ymin - y0) / dx) - 2);
+			j1 = std::min(n, (int)floor((ymax - y0) / dx) + 3);
+		}
+		for (int i = i0; i < i1; i++) {
+			for (int j = j0; j < j1; j++) {
 				double xa = x0 + i * dx, ya = y0 + j * dx;
-				// quick reject: pixel entirely outside the hemisphere disk
-				double nx = std::max(fabs(xa), fabs(xa + dx)), ny = std::max(fabs(ya), fabs(ya + dx));
 				double fx = (xa > 0 || xa + dx < 0) ? std::min(fabs(xa), fabs(xa + dx)) : 0.;
 				double fy = (ya > 0 || ya + dx < 0) ? std::min(fabs(ya), fabs(ya + dx)) : 0.;
-				(void)nx; (void)ny;
 				if (fx * fx + fy * fy > 2.0) continue;
 				int inside = 0;
 				for (int a = 0; a < sub; a++)
@@ -237,7 +282,7 @@ void pixelize(gf_catalog &c, double dx)
 						double l, b;
 						deproj_pole(hemi, x, y, l, b);
 						if ((hemi == 0 && b < 0) || (hemi == 1 && b >= 0)) continue;
-						if (in_footprint(c.foot.f, l, b)) inside++;
+						if (in_footprint(fp, l, b)) inside++;
 					}
 				if (!inside) continue;
 				double xc = xa + 0.5 * dx, yc = ya + 0.5 * dx;
@@ -251,10 +296,16 @@ void pixelize(gf_catalog &c, double dx)
 				pb.projIdx = hemi;
 				pb.X = (float)xc; pb.Y = (float)yc;
 				pb.extIdx = -1;
-				c.pixels.push_back(pb);
+				out.push_back(pb);
 			}
 		}
 	}
+	return out;
+}
+
+void pixelize(gf_catalog &c, double dx)
+{
+	c.pixels = pixelize_footprint(c.foot.f, dx);
 	c.sp.npixels = (int)c.pixels.size();
 }
 

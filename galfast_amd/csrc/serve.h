@@ -1,0 +1,63 @@
+// Device-side descriptors of the request server (host layer: host/server.cpp,
+// kernels: serve.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include "../../include/galfast_hip.h"
+
+// One (request, model) segment of a batch.
+struct gf_serve_seg {
+	gf_skygen_params sp;    // the request's binning (its own m1)
+	gf_footprint foot;      // the request's footprint
+	const double *lfP;      // FP64 prefix of the model's LF at the bin centres (nM + 1)
+	uint32_t *state;        // the request's stream table [a | c | x], S streams
+	uint64_t unit0;         // first unit (pixel x DM bin) of the segment in the batch
+	uint64_t nunits;
+	uint64_t row0, row1;    // output rows (after the plan scan)
+	uint32_t pix0, npix;    // the request's pixels in the batch pixel array
+	uint32_t slot0;         // first count-kernel workgroup of the segment in its launch
+	uint32_t lane0;         // first plan/emit lane of the segment in its launch
+	uint32_t S;             // streams of the request
+	int32_t model;          // index into the model array
+	int32_t req;            // request index in the batch
+	int32_t pad;
+};
+
+// One request of a batch, for the module chain.
+struct gf_serve_req {
+	uint32_t *state;
+	uint64_t row0, row1;
+	uint32_t lane0;
+	uint32_t S;
+};
+
+// Module parameter blocks of the server (device copy).
+struct gf_serve_chain {
+	gf_FeH_data feh;
+	gf_GaussianFeH_data gfeh;
+	gf_multiples_data mult;
+	gf_Bond2010_data bond;
+	gf_photometry_data phot;
+	gf_photoerr_data perr;
+	gf_vel2pm_data v2pm;
+	gf_bitmap gfeh_comps, mult_comps, phot_comps;
+	int has_feh, has_gfeh, has_mult, has_bond, has_phot, has_perr, has_v2pm;
+	int gal2other;
+};
+
+namespace gf {
+int serve_count(const gf_serve_seg *dsegs, int nseg, uint32_t nslots, const gf_model *dmodels,
+                const gf_pencil_beam *dpix, double *W, double *pix_sum, double *seg_expected, hipStream_t st);
+int serve_plan(const gf_serve_seg *dsegs, int nseg, uint32_t nlanes, const double *W, uint32_t *counts,
+               hipStream_t st);
+int serve_seg_rows(gf_serve_seg *dsegs, int nseg, const uint64_t *offs, hipStream_t st);
+int serve_emit(const gf_serve_seg *dsegs, int nseg, uint32_t nlanes, const gf_pencil_beam *dpix, const uint64_t *offs,
+               const gf_skygen_cols &out, hipStream_t st);
+int serve_emit_rows(const gf_serve_seg *dsegs, int nseg, const gf_model *dmodels, const gf_pencil_beam *dpix,
+                    uint64_t row0, uint64_t row1, const gf_skygen_cols &out, hipStream_t st);
+int serve_chain(const gf_serve_req *dreqs, int nreq, uint32_t nlanes, const gf_serve_chain *dch,
+                const gf_serve_chain &hch, uint64_t nrows, const gf_table &T, hipStream_t st);
+// FP64 exclusive prefix of a model's LF at the absolute-magnitude bin centres
+// (nM + 1 doubles) into dst (device).
+int lf_prefix(const gf_skygen_params &sp, const gf_model &m, double *dst, hipStream_t st);
+} // namespace gf

@@ -1,0 +1,312 @@
+// Batched kernels of the request server: many footprint requests, each with
+// its own sky binning, pixels and random-stream table, processed by one
+// launch per pipeline stage.  A "segment" is one (request, model) pair.
+// Every request's arithmetic and draws depend only on that request (its
+// pixels, binning, seed and stream table), never on which other requests
+// share the launch, so a request's rows are identical alone and in a batch.
+#include "gf_runtime.h"
+#include "gf_modules.h"
+#include "serve.h"
+
+#include <algorithm>
+
+namespace {
+
+constexpr unsigned SB = 256;
+
+template <typename T, typename F>
+GF_D int upper_index(const T *arr, int n, F key, uint64_t v)
+{
+	// largest i in [0, n) with key(arr[i]) <= v
+	int lo = 0, hi = n;
+	while (hi - lo > 1) {
+		int mid = (lo + hi) >> 1;
+		if (key(arr[mid]) <= v) lo = mid; else hi = mid;
+	}
+	return lo;
+}
+
+// ---------------------------------------------------------------------------
+// Count: one workgroup per (segment pixel), lanes over the segment's DM bins.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(SB) k_b_count(const gf_serve_seg *segs, int nseg, const gf_model *models,
+                                                const gf_pencil_beam *pixels, double *W, double *pix_sum)
+{
+	const uint32_t gp = blockIdx.x;   // batch pixel-slot index over this launch's segments
+	int si = upper_index(segs, nseg, [](const gf_serve_seg &s) { return (uint64_t)s.slot0; }, gp);
+	const gf_serve_seg &sg = segs[si];
+	const uint32_t p = gp - sg.slot0;
+	if (p >= sg.npix) return;
+	const gf_skygen_params &sp = sg.sp;
+	const gf_model &m = models[sg.model];
+	const int K = sp.nm + sp.nM - 1;
+	const gf_pencil_beam pix = pixels[sg.pix0 + p];
+	const double *P = sg.lfP;
+	gf_extinction ext{};
+	double acc = 0.0;
+	for (int k = threadIdx.x; k < K; k += blockDim.x) {
+		float D, Am, x, y, z;
+		int iM0 = k < sp.nM - 1 ? k : sp.nM - 1;
+		float M = sp.M1 - iM0 * sp.dM;
+		cell_pos(sp, ext, pix, M, k - iM0, D, Am, x, y, z);
+		int lo, hi;
+		double w = -1.0;
+		if (dm_window(sp, k, D, Am, lo, hi)) {
+			float R = model_rho_xyz(m, x, y, z);
+			double vol = (double)pix.dA * (double)POGSON * (double)sp.dm * (double)sp.dM;
+			double fac = (double)R * (double)m.norm * ((double)D * (double)D * (double)D) * vol;
+			w = fac * (P[hi + 1] - P[lo]);
+			acc += w;
+		}
+		W[sg.unit0 + (uint64_t)p * K + k] = w;
+	}
+	__shared__ double sh[SB];
+	sh[threadIdx.x] = acc;
+	__syncthreads();
+	for (int o = SB / 2; o > 0; o >>= 1) {
+		if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) pix_sum[gp] = sh[0];
+}
+
+// Per-segment expected totals, summed over the segment's pixels in order.
+__global__ void k_b_seg_expected(const gf_serve_seg *segs, int nseg, const double *pix_sum, double *expected)
+{
+	int si = blockIdx.x * blockDim.x + threadIdx.x;
+	if (si >= nseg) return;
+	double s = 0.0;
+	for (uint32_t p = 0; p < segs[si].npix; p++) s += pix_sum[segs[si].slot0 + p];
+	expected[si] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Plan: lane (segment, s) draws the units u = s, s + S, ... of its segment
+// from stream s of the segment's request.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(SB) k_b_plan(const gf_serve_seg *segs, int nseg, uint32_t nlanes, const double *W,
+                                               uint32_t *counts)
+{
+	uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+	if (g >= nlanes) return;
+	int si = upper_index(segs, nseg, [](const gf_serve_seg &s) { return (uint64_t)s.lane0; }, g);
+	const gf_serve_seg &sg = segs[si];
+	uint32_t s = g - sg.lane0;
+	if (s >= sg.S) return;
+	gf_mwc rng = gf_mwc_load(sg.state, sg.S, s);
+	for (uint64_t u = s; u < sg.nunits; u += sg.S) {
+		double w = W[sg.unit0 + u];
+		counts[sg.unit0 + u] = w > 0.0 ? rng.poisson((float)w) : 0u;
+	}
+	gf_mwc_store(sg.state, sg.S, s, rng);
+}
+
+// ---------------------------------------------------------------------------
+// Emit, random half: lane (segment, s) walks its units and parks the draws
+// of each star (as k_emit_rng does for the catalog).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(SB) k_b_emit_rng(const gf_serve_seg *segs, int nseg, uint32_t nlanes,
+                                                   const gf_pencil_beam *pixels, const uint64_t *offs,
+                                                   gf_skygen_cols out)
+{
+	uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+	if (g >= nlanes) return;
+	int si = upper_index(segs, nseg, [](const gf_serve_seg &s) { return (uint64_t)s.lane0; }, g);
+	const gf_serve_seg &sg = segs[si];
+	uint32_t s = g - sg.lane0;
+	if (s >= sg.S) return;
+	const gf_skygen_params &sp = sg.sp;
+	const int K = sp.nm + sp.nM - 1;
+	const double *P = sg.lfP;
+	gf_extinction ext{};
+	gf_mwc rng = gf_mwc_load(sg.state, sg.S, s);
+	for (uint64_t u = s; u < sg.nunits; u += sg.S) {
+		uint64_t o0 = offs[sg.unit0 + u], o1 = offs[sg.unit0 + u + 1];
+		if (o1 == o0) continue;
+		int p = (int)(u / K), k = (int)(u % K);
+		const gf_pencil_beam pix = pixels[sg.pix0 + p];
+		float D, Am, x, y, z;
+		int iM0 = k < sp.nM - 1 ? k : sp.nM - 1;
+		cell_pos(sp, ext, pix, sp.M1 - iM0 * sp.dM, k - iM0, D, Am, x, y, z);
+		int lo, hi;
+		dm_window(sp, k, D, Am, lo, hi);
+		double base = P[lo], span = P[hi + 1] - P[lo];
+		for (uint64_t j = o0; j < o1; j++) {
+			double tgt = base + (double)rng.uniform() * span;
+			int a = lo, b = hi;
+			while (a < b) {
+				int mid = (a + b) >> 1;
+				if (P[mid + 1] > tgt) b = mid; else a = mid + 1;
+			}
+			float M = sp.M1 - a * sp.dM;
+			float sx, sy, Mtmp = 0.f, mtmp = 0.f;
+			bool in = star_draws(sp, pix, M, k - a, rng, sx, sy, Mtmp, mtmp);
+			uint32_t row = (uint32_t)j;
+			col<float>(out.projXY, row, 0) = sx;
+			col<float>(out.projXY, row, 1) = sy;
+			col<float>(out.M, row) = Mtmp;
+			col<float>(out.DM, row) = mtmp;
+			col<int>(out.comp, row) = (int)(sg.pix0 + p);
+			col<int>(out.hidden, row) = in ? 0 : 1;
+		}
+	}
+	gf_mwc_store(sg.state, sg.S, s, rng);
+}
+
+// Emit, deterministic half: one lane per batch row.
+__global__ void __launch_bounds__(SB) k_b_emit_rows(const gf_serve_seg *segs, int nseg, const gf_model *models,
+                                                    const gf_pencil_beam *pixels, uint64_t row0, uint64_t row1,
+                                                    gf_skygen_cols out)
+{
+	uint64_t row = row0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (row >= row1) return;
+	int si = upper_index(segs, nseg, [](const gf_serve_seg &s) { return s.row0; }, row);
+	const gf_serve_seg &sg = segs[si];
+	uint32_t r = (uint32_t)row;
+	int gp = col<int>(out.comp, r);
+	const gf_pencil_beam pix = pixels[gp];
+	int comp = models[sg.model].comp;
+	float x = col<float>(out.projXY, r, 0), y = col<float>(out.projXY, r, 1);
+	if (col<int>(out.hidden, r)) {
+		write_star_outside(out, r, pix, comp, x, y);
+		return;
+	}
+	gf_extinction ext{};
+	write_star(sg.sp, comp, ext, sg.foot, pix, x, y, col<float>(out.M, r), col<float>(out.DM, r), 0.f, out, r);
+}
+
+// Row starts of the segments from the scanned offsets.
+__global__ void k_b_seg_rows(gf_serve_seg *segs, int nseg, const uint64_t *offs)
+{
+	int si = blockIdx.x * blockDim.x + threadIdx.x;
+	if (si >= nseg) return;
+	segs[si].row0 = offs[segs[si].unit0];
+	segs[si].row1 = offs[segs[si].unit0 + segs[si].nunits];
+}
+
+// ---------------------------------------------------------------------------
+// Module chain over requests: lane (request, s) owns the request's 256-row
+// chunks s, s + S, s + 2S, ... and runs the random-number modules on them in
+// that order; the deterministic modules run one lane per row.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(SB) k_b_chain_rng(const gf_serve_req *reqs, int nreq, uint32_t nlanes,
+                                                    const gf_serve_chain *ch, int stage, gf_table T)
+{
+	uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+	if (g >= nlanes) return;
+	int ri = upper_index(reqs, nreq, [](const gf_serve_req &q) { return (uint64_t)q.lane0; }, g);
+	const gf_serve_req &rq = reqs[ri];
+	uint32_t s = g - rq.lane0;
+	if (s >= rq.S) return;
+	gf_mwc rng = gf_mwc_load(rq.state, rq.S, s);
+	const uint64_t nrows = rq.row1 - rq.row0;
+	for (uint64_t c = s; c * 256 < nrows; c += rq.S) {
+		uint32_t r0 = (uint32_t)(rq.row0 + c * 256);
+		uint32_t r1 = (uint32_t)(rq.row0 + std::min<uint64_t>(nrows, (c + 1) * 256));
+		if (stage == 0) {
+			if (ch->has_feh) mod_FeH(ch->feh, rng, r0, r1, T.comp, T.hidden, T.XYZ, T.FeH);
+			if (ch->has_gfeh) mod_GaussianFeH(ch->gfeh, ch->gfeh_comps, rng, r0, r1, T.comp, T.hidden, T.FeH_comp, T.FeH);
+			if (ch->has_mult)
+				mod_multiples(ch->mult, ch->mult_comps, rng, r0, r1, (int)T.absmagSys.width, T.absmag, T.absmagSys,
+				              T.absmagSysNcomp, T.comp, T.hidden);
+			if (ch->has_bond) mod_Bond2010(ch->bond, rng, r0, r1, T.comp, T.hidden, T.XYZ, T.vcyl);
+		} else {
+			if (ch->has_perr) mod_photoerr(ch->perr, rng, r0, r1, T.mags, T.obsMags, T.hidden);
+		}
+	}
+	gf_mwc_store(rq.state, rq.S, s, rng);
+}
+
+__global__ void __launch_bounds__(SB) k_b_rows(const gf_serve_chain *ch, int stage, uint64_t row0, uint64_t row1,
+                                               gf_table T)
+{
+	uint64_t row = row0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (row >= row1) return;
+	uint32_t r = (uint32_t)row;
+	if (stage == 0) {
+		if (ch->has_phot) {
+			const gf_col &Mr = ch->has_mult ? T.absmagSys : T.absmag;
+			mod_photometry(ch->phot, ch->phot_comps, r, r + 1, T.Am, T.photoFlags, T.DM, Mr, (int)Mr.width, T.mags,
+			               T.FeH, T.comp, T.hidden);
+		}
+	} else {
+		if (ch->has_v2pm) mod_vel2pm(ch->v2pm, r, r + 1, T.lb, T.XYZ, T.vcyl, T.pm, T.hidden);
+		if (ch->gal2other) mod_gal2other(ch->gal2other, r, r + 1, T.hidden, T.lb, T.radec);
+	}
+}
+
+} // namespace
+
+namespace gf {
+
+int serve_count(const gf_serve_seg *dsegs, int nseg, uint32_t nslots, const gf_model *dmodels,
+                const gf_pencil_beam *dpix, double *W, double *pix_sum, double *seg_expected, hipStream_t st)
+{
+	if (nslots) {
+		k_b_count<<<nslots, SB, 0, st>>>(dsegs, nseg, dmodels, dpix, W, pix_sum);
+		GF_LAUNCHED();
+	}
+	k_b_seg_expected<<<gf::blocks_for(nseg, 64), 64, 0, st>>>(dsegs, nseg, pix_sum, seg_expected);
+	GF_LAUNCHED();
+	return 0;
+}
+
+int serve_plan(const gf_serve_seg *dsegs, int nseg, uint32_t nlanes, const double *W, uint32_t *counts,
+               hipStream_t st)
+{
+	if (!nlanes) return 0;
+	k_b_plan<<<gf::blocks_for(nlanes, SB), SB, 0, st>>>(dsegs, nseg, nlanes, W, counts);
+	GF_LAUNCHED();
+	return 0;
+}
+
+int serve_seg_rows(gf_serve_seg *dsegs, int nseg, const uint64_t *offs, hipStream_t st)
+{
+	k_b_seg_rows<<<gf::blocks_for(nseg, 64), 64, 0, st>>>(dsegs, nseg, offs);
+	GF_LAUNCHED();
+	return 0;
+}
+
+int serve_emit(const gf_serve_seg *dsegs, int nseg, uint32_t nlanes, const gf_pencil_beam *dpix, const uint64_t *offs,
+               const gf_skygen_cols &out, hipStream_t st)
+{
+	if (!nlanes) return 0;
+	k_b_emit_rng<<<gf::blocks_for(nlanes, SB), SB, 0, st>>>(dsegs, nseg, nlanes, dpix, offs, out);
+	GF_LAUNCHED();
+	return 0;
+}
+
+int serve_emit_rows(const gf_serve_seg *dsegs, int nseg, const gf_model *dmodels, const gf_pencil_beam *dpix,
+                    uint64_t row0, uint64_t row1, const gf_skygen_cols &out, hipStream_t st)
+{
+	if (row1 <= row0) return 0;
+	k_b_emit_rows<<<gf::blocks_for(row1 - row0, SB), SB, 0, st>>>(dsegs, nseg, dmodels, dpix, row0, row1, out);
+	GF_LAUNCHED();
+	return 0;
+}
+
+int serve_chain(const gf_serve_req *dreqs, int nreq, uint32_t nlanes, const gf_serve_chain *dch,
+                const gf_serve_chain &hch, uint64_t nrows, const gf_table &T, hipStream_t st)
+{
+	if (nrows == 0) return 0;
+	unsigned gl = gf::blocks_for(nlanes, SB), gr = gf::blocks_for(nrows, SB);
+	if (hch.has_feh || hch.has_gfeh || hch.has_mult || hch.has_bond) {
+		k_b_chain_rng<<<gl, SB, 0, st>>>(dreqs, nreq, nlanes, dch, 0, T);
+		GF_LAUNCHED();
+	}
+	if (hch.has_phot) {
+		k_b_rows<<<gr, SB, 0, st>>>(dch, 0, 0, nrows, T);
+		GF_LAUNCHED();
+	}
+	if (hch.has_perr) {
+		k_b_chain_rng<<<gl, SB, 0, st>>>(dreqs, nreq, nlanes, dch, 1, T);
+		GF_LAUNCHED();
+	}
+	if (hch.has_v2pm || hch.gal2other) {
+		k_b_rows<<<gr, SB, 0, st>>>(dch, 1, 0, nrows, T);
+		GF_LAUNCHED();
+	}
+	return 0;
+}
+
+} // namespace gf

@@ -281,84 +281,6 @@ struct StarSink {
 	uint64_t row;
 };
 
-// The random part of one star of draw_stars (skygen/skygen.cu.h:376-385,
-// 404-405): position inside the pixel (2 uniforms) and, when that stays
-// inside the projection, the absolute and apparent magnitude (2 more).
-GF_D bool star_draws(const gf_skygen_params &sp, const gf_pencil_beam &pix, float M, int im, gf_mwc &rng,
-                     float &x, float &y, float &Mtmp, float &mtmp)
-{
-	x = pix.X;
-	y = pix.Y;
-	x += pix.dx * (rng.uniform() - 0.5f);
-	y += pix.dx * (rng.uniform() - 0.5f);
-	if (x * x + y * y > 2.f) return false;
-	Mtmp = M + sp.dM * (rng.uniform() - 0.5f);
-	mtmp = sp.m0 + sp.dm * ((float)im + rng.uniform() - 0.5f);
-	return true;
-}
-
-// Columns of a star outside the projection: only `hidden` is meaningful
-// (the reference writes nothing else); the rest are set to zero.
-GF_D void write_star_outside(const gf_skygen_cols &c, uint32_t row, const gf_pencil_beam &pix, int comp, float x,
-                             float y)
-{
-	col<int>(c.hidden, row) = 1;
-	col<int>(c.projIdx, row) = pix.projIdx;
-	col<float>(c.projXY, row, 0) = x; col<float>(c.projXY, row, 1) = y;
-	col<double>(c.lb, row, 0) = 0.; col<double>(c.lb, row, 1) = 0.;
-	col<float>(c.M, row) = 0.f; col<float>(c.DM, row) = 0.f;
-	col<float>(c.XYZ, row, 0) = 0.f; col<float>(c.XYZ, row, 1) = 0.f; col<float>(c.XYZ, row, 2) = 0.f;
-	col<int>(c.comp, row) = comp;
-	col<float>(c.Am, row) = 0.f; col<float>(c.AmInf, row) = 0.f;
-}
-
-// The deterministic rest of draw_stars (skygen/skygen.cu.h:387-456):
-// deprojection, distance, XYZ, extinction and the flux limit, followed by
-// the clipper's footprint test.
-GF_D void write_star(const gf_skygen_params &sp, int comp, const gf_extinction &ext, const gf_footprint &foot,
-                     const gf_pencil_beam &pix, float x, float y, float Mtmp, float mtmp, float AmMin,
-                     const gf_skygen_cols &c, uint32_t row)
-{
-	double l, b;
-	lambert_deproject(sp.proj[pix.projIdx], l, b, x, y);
-	int outside = outside_footprint(foot, l, b);
-	double sld, cld, sbd, cbd;
-	gfm::sincos(l, &sld, &cld);
-	gfm::sincos(b, &sbd, &cbd);
-	float dcl = (float)cld, dcb = (float)cbd, dsl = (float)sld, dsb = (float)sbd;
-	l *= DBL_R2D;
-	if (l < 0.) l += 360.;
-	if (l > 360.) l -= 360.;
-	b *= DBL_R2D;
-	float DM = mtmp - Mtmp;
-	float D = gf_powf(10.f, 0.2f * DM + 1.f);
-	float X = D * dcl * dcb, Y = D * dsl * dcb, Z = D * dsb;
-	float Am0 = 0.f, Am1 = 0.f;
-	const gf_tex3d &em = pix.projIdx == 0 ? ext.north : ext.south;
-	if (em.data) {
-		Am0 = tex3d(em, x, y, DM);
-		Am1 = tex3d(em, x, y, 100.f);
-	}
-	float AmInf = Am1 > Am0 ? Am1 : Am0;
-	if (AmMin > Am0) AmInf = -AmMin;
-	int hidden = (mtmp + Am0 > sp.m1) ? 1 : 0;
-	hidden |= outside;
-	col<int>(c.projIdx, row) = pix.projIdx;
-	col<float>(c.projXY, row, 0) = x;
-	col<float>(c.projXY, row, 1) = y;
-	col<double>(c.lb, row, 0) = l;
-	col<double>(c.lb, row, 1) = b;
-	col<float>(c.M, row) = Mtmp;
-	col<float>(c.DM, row) = DM;
-	col<float>(c.XYZ, row, 0) = X;
-	col<float>(c.XYZ, row, 1) = Y;
-	col<float>(c.XYZ, row, 2) = Z;
-	col<int>(c.comp, row) = comp;
-	col<float>(c.Am, row) = Am0;
-	col<float>(c.AmInf, row) = AmInf;
-	col<int>(c.hidden, row) = hidden;
-}
-
 // One star of draw_stars with the clipper's footprint test.
 GF_D void draw_one_star(const gf_skygen_params &sp, const gf_model &m, const gf_extinction &ext,
                         const gf_footprint &foot, const gf_pencil_beam &pix, float M, int im, float AmMin,
@@ -764,3 +686,15 @@ int gf_skygen_plan_free(gf_draw_plan *plan)
 }
 
 } // extern "C"
+
+namespace gf {
+int lf_prefix(const gf_skygen_params &sp, const gf_model &m, double *dst, hipStream_t st)
+{
+	LFTables t;
+	int rc = lf_tables(sp, m, t, st);
+	if (rc) return rc;
+	GF_HIP(hipMemcpyAsync(dst, t.P, ((size_t)sp.nM + 1) * sizeof(double), hipMemcpyDeviceToDevice, st));
+	GF_HIP(hipStreamSynchronize(st));
+	return 0;
+}
+} // namespace gf

@@ -67,6 +67,65 @@ def cpu_baseline(cat, workload, budget_s):
                       % (len(idx), npix, r["visible"], r["seconds"])}
 
 
+def _serve_once(conf, reqs, rate, warm, **kw):
+    """Serve reqs (after `warm` warm-up requests) with Poisson arrivals at
+    `rate` req/s (None: all at once); latency = done - submit on the
+    server's clock."""
+    import numpy as np
+    from galfast_amd.server import Server
+    srv = Server(conf, None, hbm_budget=200 << 30, **kw)
+    try:
+        for r in warm:
+            srv.submit(*r)
+        srv.wait()
+        nwarm = len(srv.results)
+        rng = np.random.default_rng(1)
+        t = Server.clock()
+        for r in reqs:
+            if rate:
+                t += rng.exponential(1.0 / rate)
+                while True:
+                    dt = t - Server.clock()
+                    if dt <= 0:
+                        break
+                    time.sleep(min(dt, 0.001))
+            srv.submit(*r)
+        srv.wait()
+        res = srv.results[nwarm:]
+    finally:
+        srv.close()
+    lat = np.array([x.t_done - x.t_submit for x in res])
+    wall = max(x.t_done for x in res) - min(x.t_submit for x in res)
+    stars = sum(x.visible for x in res)
+    nb = len(set((x.t_start, x.batch_size) for x in res))
+    return {"requests": len(res), "failed": sum(1 for x in res if x.status != 0), "stars": stars,
+            "rows": sum(x.generated for x in res), "wall_s": wall, "stars_per_s": stars / wall,
+            "p50_ms": 1000.0 * float(np.percentile(lat, 50)), "p99_ms": 1000.0 * float(np.percentile(lat, 99)),
+            "mean_batch": len(res) / max(nb, 1)}
+
+
+def serving(args):
+    """Request-server measurement on the serving workload: the same request
+    set served one at a time (plain: no batching, no overlap) and with
+    continuous batching (all submitted at once), then Poisson arrivals at
+    each rate for the latency percentiles."""
+    from galfast_amd import configs
+    d = tempfile.mkdtemp(prefix="galfast_serve_")
+    conf = configs.write_c3(d, args.seed)
+    reqs = configs.c3_requests(args.serve_requests, seed=args.seed)
+    warm = configs.c3_requests(16, seed=args.seed + 1, id0=10 ** 7)
+    out = {"workload": "%d requests: circular beams, area log-uniform 0.01-100 deg^2, centres uniform with "
+                       "|b|>5 deg, r 14-m1 with m1 uniform in [20,25], seed per request id; config-1 models "
+                       "and modules, 0.25 deg pixels; HBM budget 200 GB" % len(reqs)}
+    out["plain"] = _serve_once(conf, reqs, None, warm, plain=True)
+    out["batched"] = _serve_once(conf, reqs, None, warm, max_batch=args.serve_batch, workers=2)
+    out["served_vs_plain"] = out["batched"]["stars_per_s"] / out["plain"]["stars_per_s"]
+    out["poisson"] = {}
+    for rate in args.serve_rates:
+        out["poisson"]["%g" % rate] = _serve_once(conf, reqs, rate, warm, max_batch=args.serve_batch, workers=2)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +135,10 @@ def main():
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-batch", type=int, default=256)
+    ap.add_argument("--serve-rates", type=float, nargs="*", default=[100.0, 1000.0])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the sky would shard by "
@@ -160,8 +223,13 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cat, args.workload, args.cpu_budget)
-    print(json.dumps(out))
     cat.close()
+    if not args.no_serve:
+        sv = serving(args)
+        out["serving"] = sv
+        out["p50_ms"] = sv["poisson"][max(sv["poisson"], key=float)]["p50_ms"] if sv["poisson"] else None
+        out["p99_ms"] = sv["poisson"][max(sv["poisson"], key=float)]["p99_ms"] if sv["poisson"] else None
+    print(json.dumps(out))
 
 
 if __name__ == "__main__":

@@ -177,4 +177,57 @@ modules = feh.conf Bond2010.conf photometry.conf photoerr.conf vel2pm.conf gal2o
     return os.path.join(d, "cmd.conf")
 
 
-WRITERS = {"c0": write_c0, "c1": write_c1}
+def write_c3(d, seed=42, m1=25.0, beam=(90.0, 60.0, 1.0), dx=0.25, table_seed=42):
+    """Serving base configuration: the config-1 models, tables (seeded with
+    table_seed) and modules, a beam footprint, r in [14, m1], dx-degree
+    pixels and pipeline seed `seed`.  The server takes each request's own
+    beam, faint limit and seed; a catalog opened on this configuration with
+    a request's values is that request alone."""
+    path = write_c1(d, table_seed)
+    with open(path) as f:
+        text = f.read()
+    _w(path, text.replace("seed = %d\n" % table_seed, "seed = %d\n" % seed))
+    _w(os.path.join(d, "skygen.conf"), """
+module = skygen
+band = SYNr
+dm = 0.05
+m0 = 14
+m1 = %r
+M0 = 3
+M1 = 15
+dx = %r""" % (float(m1), float(dx)))
+    _w(os.path.join(d, "foot.conf"), "module = footprint\ntype = beam\ncoordsys = gal\nfootprint_beam = %r %r %r" %
+       tuple(float(v) for v in beam))
+    return os.path.join(d, "cmd.conf")
+
+
+def request_seed(rid):
+    """32-bit seed of request rid (SplitMix64 finaliser of the id)."""
+    z = (rid + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return int((z ^ (z >> 31)) & 0xFFFFFFFF)
+
+
+def c3_requests(n, seed=0, area=(0.01, 100.0), bmin=5.0, m1=(20.0, 25.0), id0=1):
+    """n serving requests (id, l, b, radius, m1, seed): circular beams with
+    area log-uniform in `area` (deg^2), centres uniform on the sphere with
+    |b| > bmin, faint limit uniform in m1, seed = request_seed(id)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < n:
+        z = rng.uniform(-1.0, 1.0)
+        lon = rng.uniform(0.0, 360.0)
+        b = np.degrees(np.arcsin(z))
+        a = np.exp(rng.uniform(np.log(area[0]), np.log(area[1])))
+        mag = rng.uniform(*m1)
+        if abs(b) <= bmin:
+            continue
+        # cap of solid angle a: 2 pi (1 - cos r)
+        r = np.degrees(np.arccos(1.0 - a * (np.pi / 180.0) ** 2 / (2.0 * np.pi)))
+        rid = id0 + len(out)
+        out.append((rid, float(lon), float(b), float(r), float(mag), request_seed(rid)))
+    return out
+
+
+WRITERS = {"c0": write_c0, "c1": write_c1, "c3": write_c3}

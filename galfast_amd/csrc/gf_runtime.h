@@ -23,9 +23,9 @@
 
 namespace gf {
 
-// Grow-only scratch buffer cached per (device, slot).  Slots are owned by
-// the entry point that uses them; calls on one device that overlap in time
-// must not share a slot.
+// Grow-only scratch buffer cached per (host thread, device, slot).  Slots
+// are owned by the entry point that uses them; calls from one host thread
+// are ordered, calls from different threads use different buffers.
 int scratch(int slot, size_t bytes, void **ptr);
 
 inline hipStream_t as_stream(void *s) { return (hipStream_t)s; }

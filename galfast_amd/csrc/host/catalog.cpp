@@ -1047,3 +1047,509 @@ void gf_catalog_close(gf_catalog *c)
 }
 
 } // extern "C"
+
+// ===========================================================================
+// Request server: continuous batching of footprint requests.
+//
+// Submitted requests are prepared on a host thread (pixelization of the
+// beam, seeding of the request's own stream table) and queued.  Each worker
+// owns a HIP stream and device buffers; it takes every ready request up to
+// max_batch, runs count -> plan -> emit -> module chain for the whole batch
+// with one launch per stage (serve.hip), hands each request its rows and
+// takes the next batch.  With two workers one batch's host work overlaps the
+// other's kernels.  A request's streams, pixels and binning are its own, so
+// its rows do not depend on the batch it travels in.
+// ===========================================================================
+#include "../serve.h"
+#include <condition_variable>
+#include <deque>
+#include <thread>
+
+namespace {
+
+double server_clock() { return now_s(); }
+
+struct SReq {
+	gf_request r;
+	double t_submit = 0, t_start = 0;
+	gf_footprint foot{};
+	std::vector<gf_pencil_beam> pix;
+	gf_skygen_params sp{};
+	uint32_t S = 0;
+	std::vector<uint32_t> state;   // [a | c | x] of S streams
+	uint64_t nunits_model = 0;     // npix * K
+	double expected = 0;
+};
+
+struct Worker {
+	hipStream_t st = nullptr;
+	DevBuf W, counts, offs, pixsum, segexp, pix, state, segs, segall, reqs, table;
+	uint64_t table_cap = 0;
+	gf_table T{};
+	std::thread th;
+};
+
+} // namespace
+
+struct gf_server {
+	gf_catalog base;                 // models, binning, module parameters
+	double dx = 0;
+	float m0raw = 0;
+	std::vector<uint32_t> primes;
+	gf_server_config cfg{};
+	gf_result_fn fn = nullptr;
+	void *user = nullptr;
+	DevBuf dmodels, dlfP, dchain;
+	std::vector<gf_model> hmodels;
+	gf_serve_chain hchain{};
+	std::mutex mu;
+	std::condition_variable cv_sub, cv_ready, cv_idle;
+	std::deque<std::unique_ptr<SReq>> submitted, ready;
+	uint64_t n_submitted = 0, n_completed = 0, n_batches = 0, n_rows = 0, n_visible = 0;
+	double t_first = 0, t_last = 0;
+	bool stop = false;
+	std::vector<std::thread> prep;
+	std::vector<std::unique_ptr<Worker>> workers;
+	std::mutex cb_mu;
+};
+
+namespace {
+
+void prepare(gf_server &s, SReq &q)
+{
+	gf_footprint &f = q.foot;
+	double l = q.r.l * M_PI / 180., b = q.r.b * M_PI / 180.;
+	f.kind = 2;
+	f.cl = cos(l); f.cb = cos(b); f.sl = sin(l); f.sb = sin(b);
+	f.cosr = cos(q.r.radius * M_PI / 180.);
+	q.pix = pixelize_footprint(f, s.dx);
+	q.sp = s.base.sp;
+	q.sp.m1 = q.r.m1;
+	q.sp.nm = (int)round((q.sp.m1 - s.m0raw) / q.sp.dm);
+	q.sp.npixels = (int)q.pix.size();
+	const uint64_t K = (uint64_t)(q.sp.nm + q.sp.nM - 1);
+	q.nunits_model = (uint64_t)q.pix.size() * K;
+	uint64_t want = std::max<uint64_t>(64, q.nunits_model);
+	want = (want + 63) / 64 * 64;
+	q.S = (uint32_t)std::min<uint64_t>(65536, want);
+	q.state.resize(3 * (size_t)q.S);
+	gf_mwc_seed_host(gf_seed_from_pipeline_seed(q.r.seed), s.primes.data(), (uint32_t)s.primes.size(), q.S,
+	                 q.state.data());
+}
+
+size_t table_row_bytes(const gf_server &s)
+{
+	int nb = std::max(1, s.base.nbands), na = std::max(1, s.base.nabsmag);
+	return 16 + 4 + 8 + 4 + 4 + 12 + 4 + 4 + 4 + 4 + 4 + 4 + 4 * na + 4 + 12 + 4 * nb + 4 + 4 * nb + 12 + 16 + 128;
+}
+
+void worker_table(gf_server &s, Worker &w, uint64_t cap)
+{
+	if (w.table_cap >= cap && w.table.p) return;
+	cap = std::max<uint64_t>(cap, 1 << 16);
+	struct ColDef { gf_col gf_table::*m; uint32_t elem, width; };
+	int nb = s.base.nbands > 0 ? s.base.nbands : 1;
+	ColDef defs[] = {
+		{&gf_table::lb, 8, 2}, {&gf_table::projIdx, 4, 1}, {&gf_table::projXY, 4, 2}, {&gf_table::DM, 4, 1},
+		{&gf_table::absmag, 4, 1}, {&gf_table::XYZ, 4, 3}, {&gf_table::Am, 4, 1}, {&gf_table::AmInf, 4, 1},
+		{&gf_table::comp, 4, 1}, {&gf_table::hidden, 4, 1}, {&gf_table::FeH, 4, 1}, {&gf_table::FeH_comp, 4, 1},
+		{&gf_table::absmagSys, 4, (uint32_t)std::max(s.base.nabsmag, 1)}, {&gf_table::absmagSysNcomp, 4, 1},
+		{&gf_table::vcyl, 4, 3}, {&gf_table::mags, 4, (uint32_t)nb}, {&gf_table::photoFlags, 4, 1},
+		{&gf_table::obsMags, 4, (uint32_t)nb}, {&gf_table::pm, 4, 3}, {&gf_table::radec, 8, 2},
+	};
+	size_t total = 0;
+	for (auto &d : defs) total += ((cap * d.elem + 127) / 128) * 128 * d.width;
+	w.table.alloc(total);
+	char *p = (char *)w.table.p;
+	for (auto &d : defs) {
+		uint32_t pitch = (uint32_t)(((cap * d.elem + 127) / 128) * 128);
+		w.T.*(d.m) = gf_col{p, pitch, d.width, d.elem, 0};
+		p += (size_t)pitch * d.width;
+	}
+	w.table_cap = cap;
+}
+
+template <typename T> void grow(DevBuf &b, size_t n)
+{
+	if (b.bytes < n * sizeof(T)) b.alloc(n * sizeof(T) + n * sizeof(T) / 4 + 256);
+}
+
+int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch)
+{
+	hipStream_t st = w.st;
+	const int nm = (int)s.hmodels.size();
+	const int nr = (int)batch.size();
+	// unit layout: request-major, model-minor
+	std::vector<uint64_t> uoff(nr + 1, 0);
+	std::vector<uint32_t> poff(nr + 1, 0), loff(nr + 1, 0);
+	for (int r = 0; r < nr; r++) {
+		uoff[r + 1] = uoff[r] + batch[r]->nunits_model * nm;
+		poff[r + 1] = poff[r] + (uint32_t)batch[r]->pix.size();
+		loff[r + 1] = loff[r] + batch[r]->S;
+	}
+	const uint64_t U = uoff[nr];
+	std::vector<gf_pencil_beam> pix;
+	pix.reserve(poff[nr]);
+	std::vector<uint32_t> state;
+	std::vector<size_t> soff(nr + 1, 0);
+	for (int r = 0; r < nr; r++) {
+		pix.insert(pix.end(), batch[r]->pix.begin(), batch[r]->pix.end());
+		soff[r + 1] = soff[r] + batch[r]->state.size();
+	}
+	state.resize(soff[nr]);
+	for (int r = 0; r < nr; r++) std::copy(batch[r]->state.begin(), batch[r]->state.end(), state.begin() + soff[r]);
+	grow<gf_pencil_beam>(w.pix, std::max<size_t>(pix.size(), 1));
+	grow<uint32_t>(w.state, std::max<size_t>(state.size(), 1));
+	grow<double>(w.W, U + 1);
+	grow<uint32_t>(w.counts, U + 1);
+	grow<uint64_t>(w.offs, U + 2);
+	grow<double>(w.pixsum, (size_t)poff[nr] + 1);
+	grow<double>(w.segexp, (size_t)nr * nm + 1);
+	grow<gf_serve_seg>(w.segs, (size_t)nr * nm + 1);
+	grow<gf_serve_seg>(w.segall, (size_t)nr * nm + 1);
+	grow<gf_serve_req>(w.reqs, (size_t)nr + 1);
+	GF_HIP(hipMemcpyAsync(w.pix.p, pix.data(), pix.size() * sizeof(gf_pencil_beam), hipMemcpyHostToDevice, st));
+	GF_HIP(hipMemcpyAsync(w.state.p, state.data(), state.size() * 4, hipMemcpyHostToDevice, st));
+	uint32_t *dstate = (uint32_t *)w.state.p;
+	// segments per model (launch order: requests), and all segments in row order
+	std::vector<gf_serve_seg> segs((size_t)nm * nr), segall((size_t)nr * nm);
+	for (int m = 0; m < nm; m++) {
+		for (int r = 0; r < nr; r++) {
+			SReq &q = *batch[r];
+			gf_serve_seg g{};
+			g.sp = q.sp;
+			g.foot = q.foot;
+			g.lfP = (const double *)s.dlfP.p + (size_t)m * (s.base.sp.nM + 1);
+			g.state = dstate + soff[r];
+			g.unit0 = uoff[r] + (uint64_t)m * q.nunits_model;
+			g.nunits = q.nunits_model;
+			g.pix0 = poff[r];
+			g.npix = (uint32_t)q.pix.size();
+			g.slot0 = poff[r];
+			g.lane0 = loff[r];
+			g.S = q.S;
+			g.model = m;
+			g.req = r;
+			segs[(size_t)m * nr + r] = g;
+			segall[(size_t)r * nm + m] = g;
+		}
+	}
+	GF_HIP(hipMemcpyAsync(w.segs.p, segs.data(), segs.size() * sizeof(gf_serve_seg), hipMemcpyHostToDevice, st));
+	gf_serve_seg *dsegs = (gf_serve_seg *)w.segs.p;
+	const gf_model *dmodels = (const gf_model *)s.dmodels.p;
+	const gf_pencil_beam *dpix = (const gf_pencil_beam *)w.pix.p;
+	// count
+	for (int m = 0; m < nm; m++) {
+		int rc = gf::serve_count(dsegs + (size_t)m * nr, nr, poff[nr], dmodels, dpix, (double *)w.W.p,
+		                         (double *)w.pixsum.p, (double *)w.segexp.p + (size_t)m * nr, st);
+		if (rc) return rc;
+	}
+	std::vector<double> segexp((size_t)nm * nr);
+	GF_HIP(hipMemcpyAsync(segexp.data(), w.segexp.p, segexp.size() * 8, hipMemcpyDeviceToHost, st));
+	// plan (one launch per model: a request's models use its streams in turn)
+	for (int m = 0; m < nm; m++) {
+		int rc = gf::serve_plan(dsegs + (size_t)m * nr, nr, loff[nr], (const double *)w.W.p, (uint32_t *)w.counts.p, st);
+		if (rc) return rc;
+	}
+	int rc = gf::exclusive_scan_u32_u64((const uint32_t *)w.counts.p, (uint64_t *)w.offs.p, U, st);
+	if (rc) return rc;
+	for (int m = 0; m < nm; m++) {
+		rc = gf::serve_seg_rows(dsegs + (size_t)m * nr, nr, (const uint64_t *)w.offs.p, st);
+		if (rc) return rc;
+	}
+	std::vector<uint64_t> roff(nr + 1);
+	for (int r = 0; r <= nr; r++)
+		GF_HIP(hipMemcpyAsync(&roff[r], (const uint64_t *)w.offs.p + uoff[r], 8, hipMemcpyDeviceToHost, st));
+	GF_HIP(hipStreamSynchronize(st));
+	for (int r = 0; r < nr; r++) {
+		batch[r]->expected = 0;
+		for (int m = 0; m < nm; m++) batch[r]->expected += segexp[(size_t)m * nr + r];
+	}
+	const uint64_t nrows = roff[nr];
+	if (nrows >= (1ull << 32)) return GF_E_BUDGET;
+	worker_table(s, w, nrows);
+	GF_HIP(hipMemcpyAsync(w.segall.p, segall.data(), segall.size() * sizeof(gf_serve_seg), hipMemcpyHostToDevice, st));
+	// row0/row1 of the row-ordered table from the scanned offsets
+	rc = gf::serve_seg_rows((gf_serve_seg *)w.segall.p, nr * nm, (const uint64_t *)w.offs.p, st);
+	if (rc) return rc;
+	gf_skygen_cols sc{w.T.lb, w.T.projIdx, w.T.projXY, w.T.DM, w.T.absmag, w.T.XYZ, w.T.Am, w.T.AmInf, w.T.comp, w.T.hidden};
+	for (int m = 0; m < nm; m++) {
+		rc = gf::serve_emit(dsegs + (size_t)m * nr, nr, loff[nr], dpix, (const uint64_t *)w.offs.p, sc, st);
+		if (rc) return rc;
+	}
+	rc = gf::serve_emit_rows((const gf_serve_seg *)w.segall.p, nr * nm, dmodels, dpix, 0, nrows, sc, st);
+	if (rc) return rc;
+	std::vector<gf_serve_req> rq(nr);
+	for (int r = 0; r < nr; r++) {
+		rq[r].state = dstate + soff[r];
+		rq[r].row0 = roff[r];
+		rq[r].row1 = roff[r + 1];
+		rq[r].lane0 = loff[r];
+		rq[r].S = batch[r]->S;
+	}
+	GF_HIP(hipMemcpyAsync(w.reqs.p, rq.data(), rq.size() * sizeof(gf_serve_req), hipMemcpyHostToDevice, st));
+	rc = gf::serve_chain((const gf_serve_req *)w.reqs.p, nr, loff[nr], (const gf_serve_chain *)s.dchain.p, s.hchain,
+	                     nrows, w.T, st);
+	if (rc) return rc;
+	GF_HIP(hipStreamSynchronize(st));
+	// results
+	uint64_t vis_total = 0;
+	for (int r = 0; r < nr; r++) {
+		SReq &q = *batch[r];
+		uint64_t n = roff[r + 1] - roff[r], vis = 0;
+		if (n) {
+			gf_col h = w.T.hidden;
+			h.ptr = (char *)h.ptr + roff[r] * 4;
+			rc = gf_count_visible(h, n, &vis, st);
+			if (rc) return rc;
+		}
+		vis_total += vis;
+		gf_result res{};
+		res.id = q.r.id;
+		res.status = 0;
+		res.batch_size = nr;
+		res.npixels = q.pix.size();
+		res.generated = n;
+		res.visible = vis;
+		res.expected = q.expected;
+		res.t_submit = q.t_submit;
+		res.t_start = q.t_start;
+		res.t_done = server_clock();
+		if (s.fn) {
+			std::lock_guard<std::mutex> lk(s.cb_mu);
+			s.fn(s.user, &res, &w.T, roff[r], n);
+		}
+		std::lock_guard<std::mutex> lk(s.mu);
+		s.n_completed++;
+		s.t_last = res.t_done;
+	}
+	{
+		std::lock_guard<std::mutex> lk(s.mu);
+		s.n_batches++;
+		s.n_rows += nrows;
+		s.n_visible += vis_total;
+	}
+	s.cv_idle.notify_all();
+	return 0;
+}
+
+void fail_batch(gf_server &s, std::vector<std::unique_ptr<SReq>> &batch, int code)
+{
+	for (auto &q : batch) {
+		gf_result res{};
+		res.id = q->r.id;
+		res.status = code;
+		res.batch_size = (int)batch.size();
+		res.t_submit = q->t_submit;
+		res.t_start = q->t_start;
+		res.t_done = server_clock();
+		if (s.fn) {
+			std::lock_guard<std::mutex> lk(s.cb_mu);
+			s.fn(s.user, &res, nullptr, 0, 0);
+		}
+		std::lock_guard<std::mutex> lk(s.mu);
+		s.n_completed++;
+	}
+	s.cv_idle.notify_all();
+}
+
+void prep_loop(gf_server &s)
+{
+	for (;;) {
+		std::unique_ptr<SReq> q;
+		{
+			std::unique_lock<std::mutex> lk(s.mu);
+			s.cv_sub.wait(lk, [&] { return s.stop || !s.submitted.empty(); });
+			if (s.stop && s.submitted.empty()) return;
+			q = std::move(s.submitted.front());
+			s.submitted.pop_front();
+		}
+		prepare(s, *q);
+		{
+			std::lock_guard<std::mutex> lk(s.mu);
+			s.ready.push_back(std::move(q));
+		}
+		s.cv_ready.notify_all();
+	}
+}
+
+void worker_loop(gf_server &s, Worker &w)
+{
+	(void)hipSetDevice(0);
+	const size_t rowb = table_row_bytes(s);
+	for (;;) {
+		std::vector<std::unique_ptr<SReq>> batch;
+		{
+			std::unique_lock<std::mutex> lk(s.mu);
+			auto &src = s.cfg.plain ? s.submitted : s.ready;
+			s.cv_ready.wait(lk, [&] { return s.stop || !src.empty(); });
+			if (s.stop && src.empty()) return;
+			if (!s.cfg.plain && s.cfg.linger_us > 0 && (int)src.size() < s.cfg.max_batch)
+				s.cv_ready.wait_for(lk, std::chrono::microseconds(s.cfg.linger_us),
+				                    [&] { return s.stop || (int)src.size() >= s.cfg.max_batch; });
+			// admission: up to max_batch requests whose estimated rows fit the
+			// worker's share of the device budget
+			uint64_t share = s.cfg.hbm_budget / (uint64_t)std::max(1, s.cfg.workers);
+			uint64_t used = 0;
+			while (!src.empty() && (int)batch.size() < std::max(1, s.cfg.max_batch)) {
+				SReq &q = *src.front();
+				double area = 0;
+				for (auto &p : q.pix) area += (double)p.dA;
+				// rough bound: unit arrays plus rows at 3e8 stars per steradian
+				uint64_t est = (q.nunits_model + 1) * (uint64_t)(24 + 8 * s.hmodels.size()) +
+				               (uint64_t)(area * 3.0e8) * rowb;
+				if (!batch.empty() && used + est > share) break;
+				used += est;
+				batch.push_back(std::move(src.front()));
+				src.pop_front();
+			}
+		}
+		double t0 = server_clock();
+		for (auto &q : batch) {
+			if (s.cfg.plain) prepare(s, *q);
+			q->t_start = t0;
+		}
+		int rc = run_batch(s, w, batch);
+		if (rc) fail_batch(s, batch, rc);
+	}
+}
+
+} // namespace
+
+extern "C" {
+
+double gf_server_clock(void) { return server_clock(); }
+
+int gf_server_create(const char *cmd_conf, const char *datadir, const gf_server_config *cfg, gf_result_fn fn,
+                     void *user, gf_server **out)
+{
+	GF_CHECK(cmd_conf && datadir && cfg && out, GF_E_ARG);
+	auto s = std::make_unique<gf_server>();
+	s->base.datadir = datadir;
+	s->cfg = *cfg;
+	if (s->cfg.workers < 1) s->cfg.workers = 1;
+	if (s->cfg.max_batch < 1) s->cfg.max_batch = 1;
+	if (s->cfg.plain) { s->cfg.workers = 1; s->cfg.max_batch = 1; }
+	s->fn = fn;
+	s->user = user;
+	try {
+		open_catalog(s->base, cmd_conf);
+		s->dx = s->base.skygen.getd("dx", 1.0) * M_PI / 180.;
+		s->m0raw = (float)(s->base.skygen.count("m0") ? s->base.skygen.getd("m0") : s->base.skygen.getd("r0"));
+	} catch (const ConfigError &e) {
+		g_last_error = e.msg;
+		return GF_E_CONFIG;
+	}
+	gf_catalog &c = s->base;
+	s->primes.resize(70763);
+	{
+		std::ifstream f(c.datadir + "/safeprimes32.u32", std::ios::binary);
+		if (!f) { g_last_error = "cannot read safeprimes32.u32"; return GF_E_CONFIG; }
+		f.read((char *)s->primes.data(), s->primes.size() * 4);
+		s->primes.resize(f.gcount() / 4);
+	}
+	// models and their LF prefix tables (device)
+	for (auto &m : c.models) s->hmodels.push_back(m->m);
+	try {
+		s->dmodels.upload(s->hmodels);
+		s->dlfP.alloc(s->hmodels.size() * ((size_t)c.sp.nM + 1) * sizeof(double));
+	} catch (const ConfigError &e) {
+		g_last_error = e.msg;
+		return GF_E_CONFIG;
+	}
+	for (size_t i = 0; i < s->hmodels.size(); i++) {
+		int rc = gf::lf_prefix(c.sp, s->hmodels[i], (double *)s->dlfP.p + i * ((size_t)c.sp.nM + 1), c.stream);
+		if (rc) return rc;
+	}
+	// module chain
+	gf_serve_chain &h = s->hchain;
+	if (c.has_feh) { h.feh = c.feh; h.has_feh = 1; }
+	if (c.has_gfeh) { h.gfeh = c.gfeh; h.has_gfeh = 1; h.gfeh_comps = c.gfeh_comps; }
+	if (c.has_mult) { h.mult = c.mult; h.has_mult = 1; h.mult_comps = c.mult_comps; }
+	if (c.has_bond) { h.bond = c.bond; h.has_bond = 1; }
+	if (c.has_phot) { h.phot = c.phot; h.has_phot = 1; h.phot_comps = c.phot_comps; }
+	if (c.has_perr) { h.perr = c.perr; h.has_perr = 1; }
+	if (c.has_v2pm) { h.v2pm = c.v2pm; h.has_v2pm = 1; }
+	h.gal2other = c.gal2other;
+	try {
+		s->dchain.alloc(sizeof(gf_serve_chain));
+	} catch (const ConfigError &e) {
+		return GF_E_CONFIG;
+	}
+	GF_HIP(hipMemcpy(s->dchain.p, &h, sizeof(h), hipMemcpyHostToDevice));
+	gf_server *sp = s.get();
+	for (int i = 0; i < s->cfg.workers; i++) {
+		auto w = std::make_unique<Worker>();
+		GF_HIP(hipStreamCreate(&w->st));
+		s->workers.push_back(std::move(w));
+	}
+	if (!s->cfg.plain)
+		for (int i = 0; i < 4; i++) s->prep.emplace_back(prep_loop, std::ref(*sp));
+	for (auto &w : s->workers) {
+		Worker *wp = w.get();
+		w->th = std::thread([sp, wp] { worker_loop(*sp, *wp); });
+	}
+	*out = s.release();
+	return 0;
+}
+
+int gf_server_submit(gf_server *s, const gf_request *req)
+{
+	GF_CHECK(s && req && req->radius > 0 && req->radius < 90, GF_E_ARG);
+	auto q = std::make_unique<SReq>();
+	q->r = *req;
+	q->t_submit = server_clock();
+	{
+		std::lock_guard<std::mutex> lk(s->mu);
+		if (s->n_submitted == 0) s->t_first = q->t_submit;
+		s->n_submitted++;
+		s->submitted.push_back(std::move(q));
+	}
+	if (s->cfg.plain) s->cv_ready.notify_one();
+	else s->cv_sub.notify_one();
+	return 0;
+}
+
+int gf_server_wait_idle(gf_server *s)
+{
+	GF_CHECK(s, GF_E_ARG);
+	std::unique_lock<std::mutex> lk(s->mu);
+	s->cv_idle.wait(lk, [&] { return s->n_completed >= s->n_submitted; });
+	return 0;
+}
+
+int gf_server_stats_get(gf_server *s, gf_server_stats *st)
+{
+	GF_CHECK(s && st, GF_E_ARG);
+	std::lock_guard<std::mutex> lk(s->mu);
+	st->submitted = s->n_submitted;
+	st->completed = s->n_completed;
+	st->batches = s->n_batches;
+	st->rows = s->n_rows;
+	st->visible = s->n_visible;
+	st->t_first_submit = s->t_first;
+	st->t_last_done = s->t_last;
+	return 0;
+}
+
+void gf_server_destroy(gf_server *s)
+{
+	if (!s) return;
+	{
+		std::lock_guard<std::mutex> lk(s->mu);
+		s->stop = true;
+	}
+	s->cv_sub.notify_all();
+	s->cv_ready.notify_all();
+	for (auto &t : s->prep)
+		if (t.joinable()) t.join();
+	for (auto &w : s->workers) {
+		if (w->th.joinable()) w->th.join();
+		if (w->st) (void)hipStreamDestroy(w->st);
+	}
+	if (s->base.stream) (void)hipStreamDestroy(s->base.stream);
+	s->base.stream = nullptr;
+	delete s;
+}
+
+} // extern "C"

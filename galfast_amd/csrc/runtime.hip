@@ -8,15 +8,15 @@ namespace gf {
 
 namespace {
 struct ScratchEntry { void *ptr = nullptr; size_t bytes = 0; };
-std::mutex g_mu;
-std::vector<std::vector<ScratchEntry>> g_scratch;   // [device][slot]
+// per host thread, so that threads driving different streams never share a
+// buffer; [device][slot]
+thread_local std::vector<std::vector<ScratchEntry>> g_scratch;
 }
 
 int scratch(int slot, size_t bytes, void **ptr)
 {
 	int dev = 0;
 	GF_HIP(hipGetDevice(&dev));
-	std::lock_guard<std::mutex> lk(g_mu);
 	if ((int)g_scratch.size() <= dev) g_scratch.resize(dev + 1);
 	auto &v = g_scratch[dev];
 	if ((int)v.size() <= slot) v.resize(slot + 1);

@@ -507,6 +507,66 @@ typedef struct gf_catalog_host_view {
 
 int gf_catalog_host_view_get(const gf_catalog *c, gf_catalog_host_view *view);
 
+/* ---------------------------------------------------------------------- */
+/* Request server (the build's own; the reference has no serving mode)     */
+/* ---------------------------------------------------------------------- */
+
+/* A footprint-generation request: a circular beam (Galactic l, b, radius in
+ * degrees), its own faint limit and its own seed.  Everything else (models,
+ * absolute-magnitude binning, bright limit, pixel size, modules) comes from
+ * the server's base configuration. */
+typedef struct gf_request {
+    uint64_t id;
+    double   l, b, radius;
+    float    m1;
+    uint32_t seed;
+} gf_request;
+
+typedef struct gf_result {
+    uint64_t id;
+    int32_t  status;        /* 0, or a GF_E_* code */
+    int32_t  batch_size;    /* requests in the batch that carried it */
+    uint64_t npixels;
+    uint64_t generated;     /* rows drawn */
+    uint64_t visible;       /* rows with hidden == 0 */
+    double   expected;      /* expected stars (pixelized footprint) */
+    double   t_submit, t_start, t_done;   /* seconds on the server's clock */
+} gf_result;
+
+/* Called on a server thread when a request completes.  cols are the batch's
+ * device columns; the request owns rows [row0, row0 + nrows).  They are
+ * valid only during the call. */
+typedef int (*gf_result_fn)(void *user, const gf_result *res, const gf_table *cols, uint64_t row0,
+                            uint64_t nrows);
+
+typedef struct gf_server_config {
+    uint64_t hbm_budget;    /* device bytes the server may use for batches */
+    int32_t  max_batch;     /* requests per batch */
+    int32_t  workers;       /* batches in flight (one HIP stream each) */
+    int32_t  plain;         /* 1: no batching and no overlap (one request at a time) */
+    int32_t  linger_us;     /* a worker with fewer than max_batch ready requests waits up
+                             * to this long for more before it launches */
+} gf_server_config;
+
+typedef struct gf_server_stats {
+    uint64_t submitted, completed, batches, rows, visible;
+    double   t_first_submit, t_last_done;
+} gf_server_stats;
+
+typedef struct gf_server gf_server;
+
+/* cmd_conf: a catalog command configuration (its models, skygen binning and
+ * modules are used; its footprint is ignored). */
+int    gf_server_create(const char *cmd_conf, const char *datadir, const gf_server_config *cfg,
+                        gf_result_fn fn, void *user, gf_server **out);
+/* Thread-safe; the submission time is taken from the server's clock. */
+int    gf_server_submit(gf_server *s, const gf_request *req);
+/* Block until every submitted request has completed. */
+int    gf_server_wait_idle(gf_server *s);
+int    gf_server_stats_get(gf_server *s, gf_server_stats *st);
+double gf_server_clock(void);
+void   gf_server_destroy(gf_server *s);
+
 /* Number of rows with hidden == 0 among the first n (synchronous). */
 int gf_count_visible(gf_col hidden, uint64_t n, uint64_t *visible, void *stream);
 

@@ -87,3 +87,71 @@ def replay(view, chunk_rows=30_000_000, pixel_index=None, on_chunk=None, time_bu
             break
     return {"generated": generated, "visible": visible, "seconds": time.perf_counter() - t0,
             "stopped_early": stopped}
+
+
+def _shift(cols, row0):
+    """Copy of a gf_skygen_cols / gf_table whose columns start row0 rows later."""
+    out = type(cols).from_buffer_copy(cols)
+    for name, _ in type(cols)._fields_:
+        c = getattr(out, name)
+        if c.ptr:
+            c.ptr = c.ptr + row0 * c.elem
+    return out
+
+
+def request_streams(view, npix):
+    """Streams of a server request: S = 64-rounded (pixels x DM bins), at
+    least 64 and at most 65,536; the state is the first S streams of the
+    pipeline seed's sequence, planes [a | c | x]."""
+    K = view.sp.nm + view.sp.nM - 1
+    S = min(65536, (max(64, npix * K) + 63) // 64 * 64)
+    full = np.ctypeslib.as_array(view.rng_state0, shape=(3 * view.nstreams,))
+    ns = view.nstreams
+    st = np.concatenate([full[j * ns: j * ns + S] for j in range(3)]).copy()
+    return S, st
+
+
+def replay_request(view):
+    """One server request on the oracle, given the host view of a catalog
+    opened on the request's own beam, faint limit and seed: every model is
+    planned (in model order) on the request's S streams, then every model is
+    emitted, model after model, into one table; then the module chain runs
+    over the request's 256-row chunks, chunk c on stream c.
+    Returns (HostTable, expected)."""
+    o = O.lib()
+    sp = A.gf_skygen_params.from_buffer_copy(view.sp)
+    pix = (A.gf_pencil_beam * sp.npixels).from_address(view.pixels)
+    K = sp.nm + sp.nM - 1
+    U = sp.npixels * K
+    S, state = request_streams(view, sp.npixels)
+    foot = A.gf_footprint.from_buffer_copy(view.foot)
+    plans = []
+    expected = 0.0
+    for mi in range(view.nmodels):
+        m = view.models[mi]
+        res = A.gf_count_result()
+        wpk = np.zeros(U)
+        o.gfo_count_bins(C.byref(sp), C.byref(m), C.cast(pix, C.c_void_p), None, C.byref(res), None, O.ptr(wpk))
+        expected += float(wpk[wpk > 0].sum())
+        counts = np.zeros(U, dtype=np.uint32)
+        o.gfo_plan(O.ptr(wpk), U, O.ptr(state), S, O.ptr(counts))
+        offs = np.zeros(U + 1, dtype=np.uint64)
+        offs[1:] = np.cumsum(counts, dtype=np.uint64)
+        plans.append(offs)
+    total = int(sum(int(p[-1]) for p in plans))
+    spec = catalog_spec(nbands=max(view.nbands, 1), nabsmag=max(view.nabsmag, 1))
+    t = HostTable(total, spec)
+    row = 0
+    for mi, offs in enumerate(plans):
+        n = int(offs[-1])
+        if n:
+            cols = _shift(t.skygen_cols(), row)
+            o.gfo_emit(C.byref(sp), C.byref(view.models[mi]), C.cast(pix, C.c_void_p), None, C.byref(foot),
+                       O.ptr(offs), U, 0, n, O.ptr(state), S, C.byref(cols))
+        row += n
+    if total:
+        assert (total + 255) // 256 <= S, "replay needs at most one chunk per stream"
+        ks = A.gf_rows(0, 256, total, 0)
+        rc = o.gfo_module_chain(C.byref(ks), C.byref(view.chain), O.ptr(state), S, C.byref(t.table()))
+        assert rc == 0
+    return t, expected

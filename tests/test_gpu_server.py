@@ -1,0 +1,124 @@
+"""Request server on the GPU: every request's rows are the same alone and
+in a batch, and match the oracle replay of that request."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import galfast_amd as G
+from galfast_amd import configs
+from galfast_amd.catalog import Catalog
+from galfast_amd.server import Server
+from galfast_amd.table import HostTable, catalog_spec
+import replay as R
+
+pytestmark = pytest.mark.gpu
+
+SPEC = catalog_spec(nbands=5, nabsmag=1)
+
+# (id, l, b, radius deg, m1, seed): uneven sizes, both hemispheres, one beam
+# across l = 0 and one near the pole
+REQUESTS = [
+    (1, 90.0, 60.0, 0.6, 21.0, 11),
+    (2, 200.0, -35.0, 1.5, 20.5, 12),
+    (3, 359.8, 25.0, 0.3, 22.0, 13),
+    (4, 30.0, 85.0, 2.0, 20.0, 14),
+    (5, 120.0, -70.0, 0.1, 23.0, 15),
+    (6, 250.0, 12.0, 1.0, 20.2, 16),
+    (7, 10.0, -8.0, 0.4, 21.5, 17),
+    (8, 300.0, 45.0, 0.05, 24.0, 18),
+]
+
+
+def fetch_rows(table, row0, n):
+    L = G.lib()
+    L.gf_memcpy_d2h.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
+    h = HostTable(n, SPEC)
+    if n == 0:
+        return h
+    for name in SPEC:
+        col = getattr(table, name)
+        raw, p = h.buf[name]
+        for j in range(col.width):
+            G.check(L.gf_memcpy_d2h(raw.ctypes.data + j * p, col.ptr + j * col.pitch + row0 * col.elem,
+                                    n * col.elem, None))
+    return h
+
+
+def serve(conf, reqs, **kw):
+    got = {}
+
+    def on_result(res, table, row0, nrows):
+        assert res.status == 0, res.status
+        got[res.id] = (res, fetch_rows(table, row0, nrows))
+
+    srv = Server(conf, on_result, **kw)
+    try:
+        for r in reqs:
+            srv.submit(*r)
+        srv.wait()
+        st = srv.stats()
+    finally:
+        srv.close()
+    return got, st
+
+
+def same_rows(a, b, what):
+    assert a.n == b.n, "%s: %d vs %d rows" % (what, a.n, b.n)
+    for c in SPEC:
+        x, y = a.arr(c), b.arr(c)
+        bad = (x != y) & ~(np.isnan(x) & np.isnan(y)) if x.dtype.kind == "f" else (x != y)
+        assert not bad.any(), "%s: column %s, %d of %d differ" % (what, c, bad.sum(), bad.size)
+
+
+@pytest.fixture(scope="module")
+def base(tmp_path_factory):
+    return configs.write_c3(str(tmp_path_factory.mktemp("c3")), m1=25.0)
+
+
+@pytest.fixture(scope="module")
+def alone(base):
+    return serve(base, REQUESTS, plain=True)
+
+
+def test_plain_server_serves_one_at_a_time(alone):
+    got, st = alone
+    assert st.submitted == st.completed == len(REQUESTS)
+    assert st.batches == len(REQUESTS)
+    assert all(res.batch_size == 1 for res, _ in got.values())
+    assert st.rows == sum(res.generated for res, _ in got.values())
+    assert st.visible == sum(res.visible for res, _ in got.values())
+
+
+def test_request_rows_same_alone_and_in_mix(base, alone):
+    ref, _ = alone
+    mix, st = serve(base, REQUESTS, max_batch=16, workers=2, linger_us=300000)
+    assert st.completed == len(REQUESTS)
+    assert max(res.batch_size for res, _ in mix.values()) > 1, "no batching happened"
+    for rid, (res, t) in mix.items():
+        r0, t0 = ref[rid]
+        assert (res.generated, res.visible, res.npixels) == (r0.generated, r0.visible, r0.npixels)
+        assert res.expected == r0.expected
+        same_rows(t, t0, "request %d" % rid)
+    # a second mix in another order and batch shape gives the same rows again
+    mix2, _ = serve(base, REQUESTS[::-1], max_batch=3, workers=2, linger_us=100000)
+    for rid, (res, t) in mix2.items():
+        same_rows(t, ref[rid][1], "request %d (reversed)" % rid)
+
+
+@pytest.mark.parametrize("rid", [1, 3, 5])
+def test_request_matches_oracle_replay(tmp_path, alone, rid):
+    got, _ = alone
+    _, l, b, rad, m1, seed = next(r for r in REQUESTS if r[0] == rid)
+    m1 = float(np.float32(m1))
+    conf = configs.write_c3(str(tmp_path), m1=m1, beam=(l, b, rad))
+    cat = Catalog(conf, seed=seed)
+    cat.count()
+    view = R.host_view(G.lib(), cat._h)
+    res, t = got[rid]
+    assert res.npixels == view.sp.npixels
+    ref, expected = R.replay_request(view)
+    assert abs(res.expected - expected) <= 1e-12 * expected
+    assert res.generated == ref.n
+    same_rows(t, ref, "request %d vs oracle" % rid)
+    assert res.visible == int((ref.arr("hidden")[0] == 0).sum())

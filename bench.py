@@ -36,18 +36,18 @@ PEAK_HBM_GBS = 8000.0
 # column a kernel must read, and every column it writes, once.  Hidden rows
 # cost the 4-byte hidden flag in the module kernels.
 EMIT_BYTES_PER_ROW = 64                      # the 10 skygen columns
-STAGE_BYTES_PER_VISIBLE_ROW = {
-    # stage A (FeH + Bond2010): read hidden, comp, XYZ; write FeH, vcyl
-    "k_chain_rng_a": 4 + 4 + 12 + 4 + 12,
-    # stage B (photometry, 5 bands): read hidden, comp, FeH, absmag, DM, Am; write mags, flags
-    "k_rows_phot": 4 + 4 + 4 + 4 + 4 + 4 + 20 + 4,
-    # stage C (photometric errors, 5 bands): read hidden, mags; write obsMags
-    "k_chain_rng_c": 4 + 20 + 20,
-    # stage D (vel2pm + gal2other): read hidden, lb, XYZ, vcyl; write pm, radec
-    "k_rows_astro": 4 + 16 + 12 + 12 + 12 + 16,
+# (bytes per visible row, bytes per hidden row) of the two module-chain
+# passes for the config-1 module set (FeH, Bond2010, photometry with 5
+# bands, photometric errors, vel2pm, gal2other)
+STAGE_BYTES = {
+    # draw pass: read hidden + comp of every row; write the 8-byte stream
+    # position of each of the 3 random-number modules for a visible row
+    "k_chain_draws": (4 + 4 + 3 * 8, 4 + 4),
+    # row pass: read hidden, comp, XYZ, lb, DM, Am, absmag and the 3
+    # positions; write FeH, vcyl, mags, photoFlags, obsMags, pm, radec
+    "k_chain_rows": (4 + 4 + 12 + 16 + 4 + 4 + 4 + 3 * 8 + 4 + 12 + 20 + 4 + 20 + 12 + 16, 4),
 }
-STAGE_NAMES = ["k_chain_rng_a", "k_rows_phot", "k_chain_rng_c", "k_rows_astro"]
-HIDDEN_ROW_BYTES = 4
+STAGE_NAMES = ["k_chain_draws", "k_chain_rows"]
 
 
 def cpu_baseline(cat, workload, budget_s):
@@ -191,8 +191,8 @@ def main():
     kernels = {"k_emit": (acc["t_emit"], acc["n_emit"], EMIT_BYTES_PER_ROW * acc["generated"])}
     for q, name in enumerate(STAGE_NAMES):
         if acc["t_stage"][q] > 0:
-            kernels[name] = (acc["t_stage"][q], acc["n_chain"],
-                             STAGE_BYTES_PER_VISIBLE_ROW[name] * acc["visible"] + HIDDEN_ROW_BYTES * hidden)
+            bv, bh = STAGE_BYTES[name]
+            kernels[name] = (acc["t_stage"][q], acc["n_chain"], bv * acc["visible"] + bh * hidden)
     name = max(kernels, key=lambda k: kernels[k][0])
     t, n, bytes_total = kernels[name]
     per_launch_s = t / max(n, 1)
@@ -217,7 +217,7 @@ def main():
                    "expected_per_step": acc["expected"] / args.steps},
         "breakdown_ms_per_step": dict(
             [(k, 1000.0 * acc[k] / args.steps) for k in ("t_count", "t_plan", "t_emit", "t_chain")] +
-            [(STAGE_NAMES[q], 1000.0 * acc["t_stage"][q] / args.steps) for q in range(4)]),
+            [(STAGE_NAMES[q], 1000.0 * acc["t_stage"][q] / args.steps) for q in range(len(STAGE_NAMES))]),
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": None},
     }

@@ -1,0 +1,226 @@
+// The per-star module chain as two passes over a table.
+//
+// Draw pass (one lane per 256-row chunk, the chunk's MWC stream in
+// registers): for every module that draws random numbers the lane walks the
+// chunk's visible rows in module order, as the module's own kernel would,
+// records the stream position (c, x) at the start of each row's draws and
+// moves the stream past them (feh_skip, bond_skip, ...: the uniform draws
+// and rejection loops only, none of the arithmetic that turns them into
+// values).  unresolvedMultiples, whose number of draws depends on the row's
+// data, runs in full here.
+//
+// Row pass (one lane per row): every module in pipeline order (FeH,
+// GaussianFeH, Bond2010, photometry, photometric errors, vel2pm, gal2other)
+// for the row, the random-number modules restarting the stream from the
+// recorded position.  A row's draws are the same numbers in the same order
+// as in the sequential module kernels, so the output is identical; the
+// transcendental arithmetic runs at full occupancy instead of on one lane
+// per stream.
+#pragma once
+
+#include "gf_modules.h"
+
+// Module parameter blocks of a chain (device copy), with the index of each
+// random-number module's position plane (-1 when the module is off).
+struct gf_chain_mods {
+	gf_FeH_data feh;
+	gf_GaussianFeH_data gfeh;
+	gf_multiples_data mult;
+	gf_Bond2010_data bond;
+	gf_photometry_data phot;
+	gf_photoerr_data perr;
+	gf_vel2pm_data v2pm;
+	gf_bitmap gfeh_comps, mult_comps, phot_comps;
+	int has_feh, has_gfeh, has_mult, has_bond, has_phot, has_perr, has_v2pm;
+	int gal2other;
+	int pl_feh, pl_gfeh, pl_bond, pl_perr;
+	int nplanes;
+};
+
+// Assign the position planes of the enabled modules.
+inline void gf_chain_planes(gf_chain_mods &m)
+{
+	int n = 0;
+	m.pl_feh = m.has_feh ? n++ : -1;
+	m.pl_gfeh = m.has_gfeh ? n++ : -1;
+	m.pl_bond = m.has_bond ? n++ : -1;
+	m.pl_perr = m.has_perr ? n++ : -1;
+	m.nplanes = n;
+}
+
+inline bool gf_chain_has_rows(const gf_chain_mods &m)
+{
+	return m.has_feh || m.has_gfeh || m.has_bond || m.has_phot || m.has_perr || m.has_v2pm || m.gal2other;
+}
+
+// Stream positions: plane p holds the (c, x) of row i at pos[p * pitch + i].
+struct gf_positions {
+	uint2 *pos;
+	uint64_t pitch;
+};
+
+namespace gfc {
+
+constexpr int BS = 256;      // lanes per workgroup of both passes
+constexpr int TILE = 16;     // rows per tile of the draw pass
+constexpr int MAXC = 256;    // rows of a chunk kept in LDS
+constexpr uint8_t HIDDEN = 0xFF, NOCOMP = 0xFE;
+
+// LDS of one draw-pass workgroup: a tile of two 32-bit values per lane, and
+// the (hidden, comp) code of every row of each lane's chunk.
+struct DrawLds {
+	uint32_t a[TILE][BS];
+	uint32_t b[TILE][BS];
+	uint8_t hc[MAXC][BS];
+};
+
+// Fetch n <= TILE consecutive 32-bit values starting at src into the lane's
+// LDS column (vector loads when the run is whole and aligned).
+GF_D void tile_fetch(const uint32_t *src, int n, uint32_t (*dst)[BS], int lane, uint32_t pad)
+{
+	if (n == TILE && ((uintptr_t)src & 15) == 0) {
+		const uint4 *v = (const uint4 *)src;
+		uint4 q[TILE / 4];
+#pragma unroll
+		for (int i = 0; i < TILE / 4; i++) q[i] = v[i];
+#pragma unroll
+		for (int i = 0; i < TILE / 4; i++) {
+			dst[4 * i + 0][lane] = q[i].x;
+			dst[4 * i + 1][lane] = q[i].y;
+			dst[4 * i + 2][lane] = q[i].z;
+			dst[4 * i + 3][lane] = q[i].w;
+		}
+	} else {
+		for (int i = 0; i < TILE; i++) dst[i][lane] = i < n ? src[i] : pad;
+	}
+}
+
+GF_D uint8_t hc_code(uint32_t hidden, uint32_t comp)
+{
+	if (hidden) return HIDDEN;
+	return comp < 64u ? (uint8_t)comp : NOCOMP;
+}
+
+// Write the lane's n recorded positions (LDS columns a = c, b = x) to dst.
+GF_D void tile_store(uint2 *dst, int n, const DrawLds &L, int lane)
+{
+	if (n == TILE && ((uintptr_t)dst & 15) == 0) {
+		uint4 *v = (uint4 *)dst;
+#pragma unroll
+		for (int i = 0; i < TILE / 2; i++)
+			v[i] = make_uint4(L.a[2 * i][lane], L.b[2 * i][lane], L.a[2 * i + 1][lane], L.b[2 * i + 1][lane]);
+	} else {
+		for (int i = 0; i < n; i++) dst[i] = make_uint2(L.a[i][lane], L.b[i][lane]);
+	}
+}
+
+// One module's walk over the chunk: record each visible row's position in
+// plane pos, then step past the row's draws with skip(rng, comp).
+template <bool CACHED, typename Skip>
+GF_D void draw_walk(gf_mwc &rng, uint32_t r0, uint32_t r1, uint64_t prow0, uint2 *plane, const gf_table &T,
+                    DrawLds &L, int lane, Skip skip)
+{
+	for (uint32_t base = r0; base < r1; base += TILE) {
+		int n = (int)(r1 - base < (uint32_t)TILE ? r1 - base : TILE);
+		if (!CACHED) {
+			tile_fetch(&col<uint32_t>(T.hidden, base), n, L.a, lane, 1u);
+			tile_fetch(&col<uint32_t>(T.comp, base), n, L.b, lane, 0u);
+			for (int i = 0; i < n; i++) L.hc[i][lane] = hc_code(L.a[i][lane], L.b[i][lane]);
+		}
+		for (int i = 0; i < n; i++) {
+			uint8_t code = CACHED ? L.hc[base - r0 + i][lane] : L.hc[i][lane];
+			if (code == HIDDEN) continue;
+			L.a[i][lane] = rng.c;
+			L.b[i][lane] = rng.x;
+			skip(rng, code == NOCOMP ? -1 : (int)code);
+		}
+		tile_store(plane + prow0 + (base - r0), n, L, lane);
+	}
+}
+
+// The draw pass over one chunk [r0, r1) whose rows have positions at
+// prow0 + (row - r0) of every plane.
+template <bool CACHED>
+GF_D void chunk_draws(const gf_chain_mods &m, gf_mwc &rng, uint32_t r0, uint32_t r1, uint64_t prow0,
+                      const gf_table &T, const gf_positions &P, DrawLds &L, int lane)
+{
+	if (CACHED) {
+		for (uint32_t base = r0; base < r1; base += TILE) {
+			int n = (int)(r1 - base < (uint32_t)TILE ? r1 - base : TILE);
+			tile_fetch(&col<uint32_t>(T.hidden, base), n, L.a, lane, 1u);
+			tile_fetch(&col<uint32_t>(T.comp, base), n, L.b, lane, 0u);
+			for (int i = 0; i < n; i++) L.hc[base - r0 + i][lane] = hc_code(L.a[i][lane], L.b[i][lane]);
+		}
+	}
+	if (m.has_feh)
+		draw_walk<CACHED>(rng, r0, r1, prow0, P.pos + m.pl_feh * P.pitch, T, L, lane,
+		                  [&](gf_mwc &r, int cmp) { feh_skip(m.feh, r, cmp); });
+	if (m.has_gfeh)
+		draw_walk<CACHED>(rng, r0, r1, prow0, P.pos + m.pl_gfeh * P.pitch, T, L, lane,
+		                  [&](gf_mwc &r, int cmp) { gfeh_skip(m.gfeh, m.gfeh_comps, r, cmp); });
+	if (m.has_mult)
+		mod_multiples(m.mult, m.mult_comps, rng, r0, r1, (int)T.absmagSys.width, T.absmag, T.absmagSys,
+		              T.absmagSysNcomp, T.comp, T.hidden);
+	if (m.has_bond)
+		draw_walk<CACHED>(rng, r0, r1, prow0, P.pos + m.pl_bond * P.pitch, T, L, lane,
+		                  [&](gf_mwc &r, int cmp) { bond_skip(m.bond, r, cmp); });
+	if (m.has_perr)
+		draw_walk<CACHED>(rng, r0, r1, prow0, P.pos + m.pl_perr * P.pitch, T, L, lane,
+		                  [&](gf_mwc &r, int) { perr_skip(m.perr, r); });
+}
+
+GF_D gf_mwc restart(uint32_t a, const gf_positions &P, int plane, uint64_t prow)
+{
+	uint2 q = P.pos[plane * P.pitch + prow];
+	gf_mwc r;
+	r.a = a;
+	r.c = q.x;
+	r.x = q.y;
+	return r;
+}
+
+// The row pass for one row; a is the multiplier of the row's stream and
+// prow its index in the position planes.
+GF_D void chain_row(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t prow, const gf_table &T,
+                    const gf_positions &P)
+{
+	if (col<int>(T.hidden, row)) return;
+	const int cmp = col<int>(T.comp, row);
+	if (m.has_feh || m.has_bond) {
+		const float x = col<float>(T.XYZ, row, 0), y = col<float>(T.XYZ, row, 1), z = col<float>(T.XYZ, row, 2);
+		if (m.has_feh) {
+			gf_mwc r = restart(a, P, m.pl_feh, prow);
+			float feh;
+			if (feh_row(m.feh, r, cmp, x, y, z, feh)) col<float>(T.FeH, row) = feh;
+		}
+		if (m.has_gfeh) {
+			gf_mwc r = restart(a, P, m.pl_gfeh, prow);
+			gfeh_row(m.gfeh, m.gfeh_comps, r, cmp, row, T.FeH_comp, T.FeH);
+		}
+		if (m.has_bond) {
+			gf_mwc r = restart(a, P, m.pl_bond, prow);
+			float v[3];
+			if (bond_row(m.bond, r, cmp, x, y, z, v)) {
+				col<float>(T.vcyl, row, 0) = v[0];
+				col<float>(T.vcyl, row, 1) = v[1];
+				col<float>(T.vcyl, row, 2) = v[2];
+			}
+		}
+	} else if (m.has_gfeh) {
+		gf_mwc r = restart(a, P, m.pl_gfeh, prow);
+		gfeh_row(m.gfeh, m.gfeh_comps, r, cmp, row, T.FeH_comp, T.FeH);
+	}
+	if (m.has_phot) {
+		const gf_col &Mr = m.has_mult ? T.absmagSys : T.absmag;
+		mod_photometry(m.phot, m.phot_comps, row, row + 1, T.Am, T.photoFlags, T.DM, Mr, (int)Mr.width, T.mags,
+		               T.FeH, T.comp, T.hidden);
+	}
+	if (m.has_perr) {
+		gf_mwc r = restart(a, P, m.pl_perr, prow);
+		perr_row(m.perr, r, row, T.mags, T.obsMags);
+	}
+	if (m.has_v2pm) mod_vel2pm(m.v2pm, row, row + 1, T.lb, T.XYZ, T.vcyl, T.pm, T.hidden);
+	if (m.gal2other) mod_gal2other(m.gal2other, row, row + 1, T.hidden, T.lb, T.radec);
+}
+
+} // namespace gfc

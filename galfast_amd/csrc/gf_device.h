@@ -18,6 +18,7 @@ GF_D T &col(const gf_col &c, uint32_t row, uint32_t plane = 0)
 
 GF_D bool bit_isset(const gf_bitmap &m, int bit)
 {
+	if ((unsigned)bit >= 64u) return false;   // no component outside [0, 64)
 	if (bit < 32) return ((1u << bit) & m.bits[0]) != 0;
 	return ((1u << (bit - 32)) & m.bits[1]) != 0;
 }

@@ -27,6 +27,19 @@ GF_D bool feh_row(const gf_FeH_data &par, gf_mwc &rng, int cmp, float x, float y
 	return false;
 }
 
+// The draws feh_row makes for a row of component cmp, without the
+// arithmetic: the stream ends where feh_row leaves it.
+GF_D void feh_skip(const gf_FeH_data &par, gf_mwc &rng, int cmp)
+{
+	float yy, r2;
+	if (bit_isset(par.comp_thin, cmp) || bit_isset(par.comp_thick, cmp)) {
+		(void)rng.uniform();
+		rng.polar(yy, r2);
+	} else if (bit_isset(par.comp_halo, cmp)) {
+		rng.polar(yy, r2);
+	}
+}
+
 // os_FeH_kernel body (modules/FeH_gpu.cu.h:55-86)
 GF_D void mod_FeH(const gf_FeH_data &par, gf_mwc &rng, uint32_t r0, uint32_t r1,
                   const gf_col &comp, const gf_col &hidden, const gf_col &XYZ, const gf_col &FeH)
@@ -40,6 +53,29 @@ GF_D void mod_FeH(const gf_FeH_data &par, gf_mwc &rng, uint32_t r0, uint32_t r1,
 	}
 }
 
+// One row of os_GaussianFeH_kernel (modules/GaussianFeH_gpu.cu.h:55-72) for
+// a visible row of component cmp.
+GF_D void gfeh_row(const gf_GaussianFeH_data &par, gf_bitmap apply, gf_mwc &rng, int cmp, uint32_t row,
+                   const gf_col &FeH_comp, const gf_col &FeH)
+{
+	if (!bit_isset(apply, cmp)) return;
+	int at = 0;
+	if (par.ncomp != 1) {
+		float u = rng.uniform();
+		while (at < par.ncomp - 1 && par.pcum[at] < u) ++at;
+	}
+	col<float>(FeH, row) = par.mean[at] + rng.gaussian(par.sigma[at]);
+	if (FeH_comp.ptr) col<int>(FeH_comp, row) = at;
+}
+
+GF_D void gfeh_skip(const gf_GaussianFeH_data &par, gf_bitmap apply, gf_mwc &rng, int cmp)
+{
+	if (!bit_isset(apply, cmp)) return;
+	float yy, r2;
+	if (par.ncomp != 1) (void)rng.uniform();
+	rng.polar(yy, r2);
+}
+
 // os_GaussianFeH_kernel body (modules/GaussianFeH_gpu.cu.h:55-72)
 GF_D void mod_GaussianFeH(const gf_GaussianFeH_data &par, gf_bitmap apply, gf_mwc &rng,
                           uint32_t r0, uint32_t r1, const gf_col &comp, const gf_col &hidden,
@@ -47,15 +83,7 @@ GF_D void mod_GaussianFeH(const gf_GaussianFeH_data &par, gf_bitmap apply, gf_mw
 {
 	for (uint32_t row = r0; row < r1; row++) {
 		if (col<int>(hidden, row)) continue;
-		int cmp = col<int>(comp, row);
-		if (!bit_isset(apply, cmp)) continue;
-		int at = 0;
-		if (par.ncomp != 1) {
-			float u = rng.uniform();
-			while (at < par.ncomp - 1 && par.pcum[at] < u) ++at;
-		}
-		col<float>(FeH, row) = par.mean[at] + rng.gaussian(par.sigma[at]);
-		if (FeH_comp.ptr) col<int>(FeH_comp, row) = at;
+		gfeh_row(par, apply, rng, col<int>(comp, row), row, FeH_comp, FeH);
 	}
 }
 
@@ -127,6 +155,20 @@ GF_D bool bond_row(const gf_Bond2010_data &par, gf_mwc &rng, int cmp, float x, f
 		return true;
 	}
 	return false;
+}
+
+// The draws bond_row makes for a row of component cmp.
+GF_D void bond_skip(const gf_Bond2010_data &par, gf_mwc &rng, int cmp)
+{
+	float yy, r2;
+	if (bit_isset(par.comp_thin, cmp) || bit_isset(par.comp_thick, cmp)) {
+		(void)rng.uniform();
+	} else if (!bit_isset(par.comp_halo, cmp)) {
+		return;
+	}
+	rng.polar(yy, r2);
+	rng.polar(yy, r2);
+	rng.polar(yy, r2);
 }
 
 // os_Bond2010_kernel body (modules/Bond2010_gpu.cu.h:181-217)
@@ -428,23 +470,36 @@ GF_D double spline_eval(const double *kx, const double *ky, int n, double x)
 	return ky[lo] + D * (x - kx[lo]);
 }
 
-// Photometric errors for one row chunk, bands in order, rows in order
-// within a band (modules/photometricErrors_host.cpp:112-131), drawing from
-// the chunk's MWC stream.
-GF_D void mod_photoerr(const gf_photoerr_data &par, gf_mwc &rng, uint32_t r0, uint32_t r1,
-                       const gf_col &magTrue, const gf_col &magObs, const gf_col &hidden)
+// Photometric errors (modules/photometricErrors_host.cpp:112-131) drawn
+// from the chunk's MWC stream, rows in order and the bands of a row in
+// order.  One visible row: a deviate for each band that has an error curve.
+GF_D void perr_row(const gf_photoerr_data &par, gf_mwc &rng, uint32_t row, const gf_col &magTrue,
+                   const gf_col &magObs)
 {
 	for (int b = 0; b < par.nbands; b++) {
 		int k0 = par.koff[b], k1 = par.koff[b + 1];
-		for (uint32_t row = r0; row < r1; row++) {
-			if (col<int>(hidden, row)) continue;
-			float mag = col<float>(magTrue, row, b);
-			if (k1 > k0) {
-				float sigma = (float)spline_eval(par.kx + k0, par.ky + k0, k1 - k0, (double)mag);
-				col<float>(magObs, row, b) = mag + rng.gaussian(sigma);
-			} else {
-				col<float>(magObs, row, b) = mag;
-			}
+		float mag = col<float>(magTrue, row, b);
+		if (k1 > k0) {
+			float sigma = (float)spline_eval(par.kx + k0, par.ky + k0, k1 - k0, (double)mag);
+			col<float>(magObs, row, b) = mag + rng.gaussian(sigma);
+		} else {
+			col<float>(magObs, row, b) = mag;
 		}
+	}
+}
+
+GF_D void perr_skip(const gf_photoerr_data &par, gf_mwc &rng)
+{
+	float yy, r2;
+	for (int b = 0; b < par.nbands; b++)
+		if (par.koff[b + 1] > par.koff[b]) rng.polar(yy, r2);
+}
+
+GF_D void mod_photoerr(const gf_photoerr_data &par, gf_mwc &rng, uint32_t r0, uint32_t r1,
+                       const gf_col &magTrue, const gf_col &magObs, const gf_col &hidden)
+{
+	for (uint32_t row = r0; row < r1; row++) {
+		if (col<int>(hidden, row)) continue;
+		perr_row(par, rng, row, magTrue, magObs);
 	}
 }

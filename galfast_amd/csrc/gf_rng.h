@@ -32,14 +32,23 @@ struct gf_mwc
 		return u;
 	}
 
-	GF_HD float gaussian(const float sigma)
+	// The draws of one polar Box-Muller deviate: uniform pairs until the
+	// point falls inside the unit disc.  gaussian() finishes it; modules
+	// that only need to move the stream past a deviate stop here.
+	GF_HD void polar(float &yy, float &r2)
 	{
-		float xx, yy, r2;
+		float xx;
 		do {
 			xx = -1.f + 2.f * uniform();
 			yy = -1.f + 2.f * uniform();
 			r2 = xx * xx + yy * yy;
 		} while (r2 > 1.0f || r2 == 0.f || r2 == 4.f);
+	}
+
+	GF_HD float gaussian(const float sigma)
+	{
+		float yy, r2;
+		polar(yy, r2);
 		float t = -2.0f * gf_logf(r2);
 		t = t / r2;
 		return sigma * yy * gf_sqrtf(t);

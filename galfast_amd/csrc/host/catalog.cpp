@@ -1083,7 +1083,7 @@ struct SReq {
 
 struct Worker {
 	hipStream_t st = nullptr;
-	DevBuf W, counts, offs, pixsum, segexp, pix, state, segs, segall, reqs, table;
+	DevBuf W, counts, offs, pixsum, segexp, pix, state, segs, segall, reqs, table, pos;
 	uint64_t table_cap = 0;
 	gf_table T{};
 	std::thread th;
@@ -1288,8 +1288,9 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 		rq[r].S = batch[r]->S;
 	}
 	GF_HIP(hipMemcpyAsync(w.reqs.p, rq.data(), rq.size() * sizeof(gf_serve_req), hipMemcpyHostToDevice, st));
+	grow<uint2>(w.pos, std::max<size_t>(1, (size_t)s.hchain.nplanes * nrows));
 	rc = gf::serve_chain((const gf_serve_req *)w.reqs.p, nr, loff[nr], (const gf_serve_chain *)s.dchain.p, s.hchain,
-	                     nrows, w.T, st);
+	                     nrows, w.T, (uint2 *)w.pos.p, st);
 	if (rc) return rc;
 	GF_HIP(hipStreamSynchronize(st));
 	// results
@@ -1471,6 +1472,7 @@ int gf_server_create(const char *cmd_conf, const char *datadir, const gf_server_
 	if (c.has_perr) { h.perr = c.perr; h.has_perr = 1; }
 	if (c.has_v2pm) { h.v2pm = c.v2pm; h.has_v2pm = 1; }
 	h.gal2other = c.gal2other;
+	gf_chain_planes(h);
 	try {
 		s->dchain.alloc(sizeof(gf_serve_chain));
 	} catch (const ConfigError &e) {

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 #include "../../include/galfast_hip.h"
+#include "gf_chain.h"
 
 // One (request, model) segment of a batch.
 struct gf_serve_seg {
@@ -32,18 +33,7 @@ struct gf_serve_req {
 };
 
 // Module parameter blocks of the server (device copy).
-struct gf_serve_chain {
-	gf_FeH_data feh;
-	gf_GaussianFeH_data gfeh;
-	gf_multiples_data mult;
-	gf_Bond2010_data bond;
-	gf_photometry_data phot;
-	gf_photoerr_data perr;
-	gf_vel2pm_data v2pm;
-	gf_bitmap gfeh_comps, mult_comps, phot_comps;
-	int has_feh, has_gfeh, has_mult, has_bond, has_phot, has_perr, has_v2pm;
-	int gal2other;
-};
+typedef gf_chain_mods gf_serve_chain;
 
 namespace gf {
 int serve_count(const gf_serve_seg *dsegs, int nseg, uint32_t nslots, const gf_model *dmodels,
@@ -55,8 +45,9 @@ int serve_emit(const gf_serve_seg *dsegs, int nseg, uint32_t nlanes, const gf_pe
                const gf_skygen_cols &out, hipStream_t st);
 int serve_emit_rows(const gf_serve_seg *dsegs, int nseg, const gf_model *dmodels, const gf_pencil_beam *dpix,
                     uint64_t row0, uint64_t row1, const gf_skygen_cols &out, hipStream_t st);
+// pos: hch.nplanes * nrows stream positions (device scratch of the caller)
 int serve_chain(const gf_serve_req *dreqs, int nreq, uint32_t nlanes, const gf_serve_chain *dch,
-                const gf_serve_chain &hch, uint64_t nrows, const gf_table &T, hipStream_t st);
+                const gf_serve_chain &hch, uint64_t nrows, const gf_table &T, uint2 *pos, hipStream_t st);
 // FP64 exclusive prefix of a model's LF at the absolute-magnitude bin centres
 // (nM + 1 doubles) into dst (device).
 int lf_prefix(const gf_skygen_params &sp, const gf_model &m, double *dst, hipStream_t st);

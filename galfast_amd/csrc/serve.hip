@@ -186,12 +186,13 @@ __global__ void k_b_seg_rows(gf_serve_seg *segs, int nseg, const uint64_t *offs)
 
 // ---------------------------------------------------------------------------
 // Module chain over requests: lane (request, s) owns the request's 256-row
-// chunks s, s + S, s + 2S, ... and runs the random-number modules on them in
-// that order; the deterministic modules run one lane per row.
+// chunks s, s + S, s + 2S, ... and runs the draw pass on them in that order
+// (gf_chain.h); the row pass runs one lane per batch row.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(SB) k_b_chain_rng(const gf_serve_req *reqs, int nreq, uint32_t nlanes,
-                                                    const gf_serve_chain *ch, int stage, gf_table T)
+__global__ void __launch_bounds__(gfc::BS) k_b_draws(const gf_serve_req *reqs, int nreq, uint32_t nlanes,
+                                                     const gf_serve_chain *ch, gf_table T, gf_positions P)
 {
+	__shared__ gfc::DrawLds L;
 	uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
 	if (g >= nlanes) return;
 	int ri = upper_index(reqs, nreq, [](const gf_serve_req &q) { return (uint64_t)q.lane0; }, g);
@@ -203,36 +204,24 @@ __global__ void __launch_bounds__(SB) k_b_chain_rng(const gf_serve_req *reqs, in
 	for (uint64_t c = s; c * 256 < nrows; c += rq.S) {
 		uint32_t r0 = (uint32_t)(rq.row0 + c * 256);
 		uint32_t r1 = (uint32_t)(rq.row0 + std::min<uint64_t>(nrows, (c + 1) * 256));
-		if (stage == 0) {
-			if (ch->has_feh) mod_FeH(ch->feh, rng, r0, r1, T.comp, T.hidden, T.XYZ, T.FeH);
-			if (ch->has_gfeh) mod_GaussianFeH(ch->gfeh, ch->gfeh_comps, rng, r0, r1, T.comp, T.hidden, T.FeH_comp, T.FeH);
-			if (ch->has_mult)
-				mod_multiples(ch->mult, ch->mult_comps, rng, r0, r1, (int)T.absmagSys.width, T.absmag, T.absmagSys,
-				              T.absmagSysNcomp, T.comp, T.hidden);
-			if (ch->has_bond) mod_Bond2010(ch->bond, rng, r0, r1, T.comp, T.hidden, T.XYZ, T.vcyl);
-		} else {
-			if (ch->has_perr) mod_photoerr(ch->perr, rng, r0, r1, T.mags, T.obsMags, T.hidden);
-		}
+		gfc::chunk_draws<true>(*ch, rng, r0, r1, r0, T, P, L, threadIdx.x);
 	}
 	gf_mwc_store(rq.state, rq.S, s, rng);
 }
 
-__global__ void __launch_bounds__(SB) k_b_rows(const gf_serve_chain *ch, int stage, uint64_t row0, uint64_t row1,
-                                               gf_table T)
+__global__ void __launch_bounds__(gfc::BS) k_b_rows(const gf_serve_req *reqs, int nreq, const gf_serve_chain *ch,
+                                                    uint64_t nrows, gf_table T, gf_positions P)
 {
-	uint64_t row = row0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (row >= row1) return;
-	uint32_t r = (uint32_t)row;
-	if (stage == 0) {
-		if (ch->has_phot) {
-			const gf_col &Mr = ch->has_mult ? T.absmagSys : T.absmag;
-			mod_photometry(ch->phot, ch->phot_comps, r, r + 1, T.Am, T.photoFlags, T.DM, Mr, (int)Mr.width, T.mags,
-			               T.FeH, T.comp, T.hidden);
-		}
-	} else {
-		if (ch->has_v2pm) mod_vel2pm(ch->v2pm, r, r + 1, T.lb, T.XYZ, T.vcyl, T.pm, T.hidden);
-		if (ch->gal2other) mod_gal2other(ch->gal2other, r, r + 1, T.hidden, T.lb, T.radec);
+	uint64_t row = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (row >= nrows) return;
+	uint32_t a = 0;
+	if (ch->nplanes) {
+		int ri = upper_index(reqs, nreq, [](const gf_serve_req &q) { return q.row0; }, row);
+		const gf_serve_req &rq = reqs[ri];
+		uint64_t c = (row - rq.row0) / 256;
+		a = rq.state[c % rq.S];
 	}
+	gfc::chain_row(*ch, a, (uint32_t)row, row, T, P);
 }
 
 } // namespace
@@ -286,24 +275,16 @@ int serve_emit_rows(const gf_serve_seg *dsegs, int nseg, const gf_model *dmodels
 }
 
 int serve_chain(const gf_serve_req *dreqs, int nreq, uint32_t nlanes, const gf_serve_chain *dch,
-                const gf_serve_chain &hch, uint64_t nrows, const gf_table &T, hipStream_t st)
+                const gf_serve_chain &hch, uint64_t nrows, const gf_table &T, uint2 *pos, hipStream_t st)
 {
 	if (nrows == 0) return 0;
-	unsigned gl = gf::blocks_for(nlanes, SB), gr = gf::blocks_for(nrows, SB);
-	if (hch.has_feh || hch.has_gfeh || hch.has_mult || hch.has_bond) {
-		k_b_chain_rng<<<gl, SB, 0, st>>>(dreqs, nreq, nlanes, dch, 0, T);
+	gf_positions P{pos, nrows};
+	if (hch.nplanes || hch.has_mult) {
+		k_b_draws<<<gf::blocks_for(nlanes, gfc::BS), gfc::BS, 0, st>>>(dreqs, nreq, nlanes, dch, T, P);
 		GF_LAUNCHED();
 	}
-	if (hch.has_phot) {
-		k_b_rows<<<gr, SB, 0, st>>>(dch, 0, 0, nrows, T);
-		GF_LAUNCHED();
-	}
-	if (hch.has_perr) {
-		k_b_chain_rng<<<gl, SB, 0, st>>>(dreqs, nreq, nlanes, dch, 1, T);
-		GF_LAUNCHED();
-	}
-	if (hch.has_v2pm || hch.gal2other) {
-		k_b_rows<<<gr, SB, 0, st>>>(dch, 1, 0, nrows, T);
+	if (gf_chain_has_rows(hch)) {
+		k_b_rows<<<gf::blocks_for(nrows, gfc::BS), gfc::BS, 0, st>>>(dreqs, nreq, dch, nrows, T, P);
 		GF_LAUNCHED();
 	}
 	return 0;

@@ -832,6 +832,7 @@ GFO_API void gfo_emit(const gf_skygen_params *spp, const gf_model *mp, const gf_
 /* ======================================================================== */
 static bool isset(const gf_bitmap &m, int bit)
 {
+	if (bit < 0 || bit >= 64) return false;   /* no component outside [0, 64) */
 	return bit < 32 ? ((1u << bit) & m.bits[0]) != 0 : ((1u << (bit - 32)) & m.bits[1]) != 0;
 }
 
@@ -1145,14 +1146,15 @@ static void mult_rows(const gf_multiples_data &par, gf_bitmap apply, gfo_rng &rn
 	}
 }
 
-/* photometric errors drawn from the chunk stream, bands outer, rows inner
- * (modules/photometricErrors_host.cpp:112-131) */
+/* photometric errors drawn from the chunk stream, rows outer, bands inner
+ * (modules/photometricErrors_host.cpp:112-131 draws from one serial stream
+ * instead) */
 static void perr_rows(const gf_photoerr_data &par, gfo_rng &rng, uint32_t r0, uint32_t r1, const gf_table &t)
 {
-	for (int b = 0; b < par.nbands; b++) {
-		int k0 = par.koff[b], k1 = par.koff[b + 1];
-		for (uint32_t row = r0; row < r1; row++) {
-			if (C<int>(t.hidden, row)) continue;
+	for (uint32_t row = r0; row < r1; row++) {
+		if (C<int>(t.hidden, row)) continue;
+		for (int b = 0; b < par.nbands; b++) {
+			int k0 = par.koff[b], k1 = par.koff[b + 1];
 			float mag = C<float>(t.mags, row, b);
 			if (k1 > k0) {
 				float sg = (float)spline_at(par.kx + k0, par.ky + k0, k1 - k0, (double)mag);

@@ -63,10 +63,18 @@ def serve(conf, reqs, **kw):
     return got, st
 
 
+SKYGEN = ("lb", "projIdx", "projXY", "DM", "absmag", "XYZ", "Am", "AmInf", "comp", "hidden")
+
+
 def same_rows(a, b, what):
+    """Skygen columns on every row; module columns on visible rows (the
+    modules skip hidden rows and leave those cells as they were)."""
     assert a.n == b.n, "%s: %d vs %d rows" % (what, a.n, b.n)
+    vis = b.arr("hidden")[0] == 0
     for c in SPEC:
         x, y = a.arr(c), b.arr(c)
+        if c not in SKYGEN:
+            x, y = x[:, vis], y[:, vis]
         bad = (x != y) & ~(np.isnan(x) & np.isnan(y)) if x.dtype.kind == "f" else (x != y)
         assert not bad.any(), "%s: column %s, %d of %d differ" % (what, c, bad.sum(), bad.size)
 

@@ -22,7 +22,10 @@
  *   vel2pm ........................... commented known answer in
  *                                      modules/vel2pm_gpu.cu.h:223-231 and the
  *                                      pmlb golden column
- *   FeH/Bond2010/GaussianFeH/multiples moments vs analytic distributions
+ *   FeH, GaussianFeH, Bond2010, ...... KS / chi-square / covariance against the
+ *   unresolvedMultiples, photometric   distributions their formulas fix, each
+ *   errors                             test also rejecting a planted mistake
+ *                                      (tests/test_oracle_modules.py)
  *   photometry ....................... closed form on polynomial colour tables
  */
 #include <math.h>

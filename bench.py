@@ -36,18 +36,23 @@ PEAK_HBM_GBS = 8000.0
 # column a kernel must read, and every column it writes, once.  Hidden rows
 # cost the 4-byte hidden flag in the module kernels.
 EMIT_BYTES_PER_ROW = 64                      # the 10 skygen columns
-# (bytes per visible row, bytes per hidden row) of the two module-chain
-# passes for the config-1 module set (FeH, Bond2010, photometry with 5
-# bands, photometric errors, vel2pm, gal2other)
+# (bytes per visible row, bytes per hidden row) of the module-chain launches
+# for the config-1 module set (FeH, Bond2010, photometry with 5 bands,
+# photometric errors, vel2pm, gal2other): every column a launch must read,
+# and every column it writes, once.
 STAGE_BYTES = {
     # draw pass: read hidden + comp of every row; write the 8-byte stream
     # position of each of the 3 random-number modules for a visible row
     "k_chain_draws": (4 + 4 + 3 * 8, 4 + 4),
-    # row pass: read hidden, comp, XYZ, lb, DM, Am, absmag and the 3
-    # positions; write FeH, vcyl, mags, photoFlags, obsMags, pm, radec
-    "k_chain_rows": (4 + 4 + 12 + 16 + 4 + 4 + 4 + 3 * 8 + 4 + 12 + 20 + 4 + 20 + 12 + 16, 4),
+    # kinematics rows: read hidden, comp, XYZ, 2 positions; write FeH, vcyl
+    "k_chain_rows<KIN>": (4 + 4 + 12 + 2 * 8 + 4 + 12, 4),
+    # photometry rows: read hidden, comp, FeH, absmag, DM, Am, 1 position;
+    # write mags, photoFlags, obsMags
+    "k_chain_rows<PHOT>": (4 + 4 + 4 + 4 + 4 + 4 + 8 + 20 + 4 + 20, 4),
+    # astrometry rows: read hidden, lb, XYZ, vcyl; write pm, radec
+    "k_chain_rows<ASTRO>": (4 + 16 + 12 + 12 + 12 + 16, 4),
 }
-STAGE_NAMES = ["k_chain_draws", "k_chain_rows"]
+STAGE_NAMES = ["k_chain_draws", "k_chain_rows<KIN>", "k_chain_rows<PHOT>", "k_chain_rows<ASTRO>"]
 
 
 def cpu_baseline(cat, workload, budget_s):
@@ -139,7 +144,19 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-batch", type=int, default=256)
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[100.0, 1000.0])
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (the reference kernel for scaling "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the sky would shard by "
               "pixels, see DESIGN.md); run with --gpus 1", file=sys.stderr)

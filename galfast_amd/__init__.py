@@ -13,7 +13,9 @@ from . import _abi
 from ._abi import *  # noqa: F401,F403
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgalfast_hip.so")
+# GALFAST_LIB names another build of the library (e.g. one built with
+# different compile options) to load instead of the in-tree one
+LIB_PATH = os.environ.get("GALFAST_LIB") or os.path.join(HERE, "libgalfast_hip.so")
 DATA_DIR = os.path.join(HERE, "data")
 
 _lib = None

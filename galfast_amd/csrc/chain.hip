@@ -33,22 +33,30 @@ __global__ void __launch_bounds__(gfc::BS) k_chain_draws(const ChainDev *cd, uin
 }
 
 // Row pass: one lane per row; the row's stream is its chunk's.
-__global__ void __launch_bounds__(gfc::BS) k_chain_rows(const ChainDev *cd, const uint32_t *st)
+template <int GROUP, int NT>
+__global__ void __launch_bounds__(gfc::BS) k_chain_rows(const ChainDev *cd, const uint32_t *st, int need_stream)
 {
 	const ChainDev &c = *cd;
 	uint32_t row = c.begin + blockIdx.x * blockDim.x + threadIdx.x;
 	if (row >= c.end) return;
 	uint32_t i = row - c.begin;
-	gfc::chain_row(c.m, c.m.nplanes ? st[i / c.step] : 0u, row, i, c.t, c.P);
+	gfc::chain_rows<GROUP, NT>(c.m, need_stream ? st[i / c.step] : 0u, row, i, c.t, c.P);
+}
+
+template <int NT>
+void launch_phot(unsigned grid, hipStream_t st, const ChainDev *dp, const uint32_t *state, int need_stream)
+{
+	k_chain_rows<gfc::ROWS_PHOT, NT><<<grid, gfc::BS, 0, st>>>(dp, state, need_stream);
 }
 
 } // namespace
 
 namespace gf {
 
-// The chain with optional per-pass timing: when ev is non-NULL it holds 5
-// events; ms[0] receives the draw pass's duration, ms[1] the row pass's
-// (0 for a pass with nothing to do), ms[2] and ms[3] are 0.
+// The chain with optional per-launch timing: when ev is non-NULL it holds 5
+// events; ms[0] receives the draw pass's duration, ms[1..3] the row pass's
+// three groups (kinematics, photometry, astrometry); 0 for a launch with
+// nothing to do.
 int module_chain(const gf_rows *ks, const gf_chain *chain, gf_rng *rng, const gf_table *t, hipStream_t st,
                  hipEvent_t *ev, float *ms)
 {
@@ -88,26 +96,43 @@ int module_chain(const gf_rows *ks, const gf_chain *chain, gf_rng *rng, const gf
 	if (rc) return rc;
 	GF_HIP(hipMemcpyAsync(p, &h, sizeof(ChainDev), hipMemcpyHostToDevice, st));
 	const ChainDev *dp = (const ChainDev *)p;
-	const bool draws = m.nplanes || m.has_mult, rows = gf_chain_has_rows(m);
+	const bool on[4] = {m.nplanes || m.has_mult, gfc::rows_on(m, gfc::ROWS_KIN), gfc::rows_on(m, gfc::ROWS_PHOT),
+	                    gfc::rows_on(m, gfc::ROWS_ASTRO)};
+	const unsigned grows = gf::blocks_for(n, gfc::BS);
 	if (ev) GF_HIP(hipEventRecord(ev[0], st));
-	if (draws) {
+	if (on[0]) {
 		unsigned g = gf::blocks_for(nthr, gfc::BS);
 		if (ks->step <= (uint32_t)gfc::MAXC) k_chain_draws<true><<<g, gfc::BS, 0, st>>>(dp, nthr, rng->state, rng->nstreams);
 		else k_chain_draws<false><<<g, gfc::BS, 0, st>>>(dp, nthr, rng->state, rng->nstreams);
 		GF_LAUNCHED();
 	}
 	if (ev) GF_HIP(hipEventRecord(ev[1], st));
-	if (rows) {
-		k_chain_rows<<<gf::blocks_for(n, gfc::BS), gfc::BS, 0, st>>>(dp, rng->state);
+	if (on[1]) {
+		k_chain_rows<gfc::ROWS_KIN, 1><<<grows, gfc::BS, 0, st>>>(dp, rng->state, 1);
 		GF_LAUNCHED();
 	}
 	if (ev) GF_HIP(hipEventRecord(ev[2], st));
+	if (on[2]) {
+		const int ns = (int)gfc::rows_stream(m, gfc::ROWS_PHOT);
+		switch (gfc::phot_tables(m)) {
+		case 1: launch_phot<1>(grows, st, dp, rng->state, ns); break;
+		case 2: launch_phot<2>(grows, st, dp, rng->state, ns); break;
+		case 3: launch_phot<3>(grows, st, dp, rng->state, ns); break;
+		default: launch_phot<4>(grows, st, dp, rng->state, ns); break;
+		}
+		GF_LAUNCHED();
+	}
+	if (ev) GF_HIP(hipEventRecord(ev[3], st));
+	if (on[3]) {
+		k_chain_rows<gfc::ROWS_ASTRO, 1><<<grows, gfc::BS, 0, st>>>(dp, rng->state, 0);
+		GF_LAUNCHED();
+	}
+	if (ev) GF_HIP(hipEventRecord(ev[4], st));
 	// the staging copy must complete before the host buffer is reused
 	GF_HIP(hipStreamSynchronize(st));
-	if (ev && ms) {
-		if (draws) GF_HIP(hipEventElapsedTime(&ms[0], ev[0], ev[1]));
-		if (rows) GF_HIP(hipEventElapsedTime(&ms[1], ev[1], ev[2]));
-	}
+	if (ev && ms)
+		for (int i = 0; i < 4; i++)
+			if (on[i]) GF_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
 	return 0;
 }
 

@@ -9,10 +9,10 @@
 // values).  unresolvedMultiples, whose number of draws depends on the row's
 // data, runs in full here.
 //
-// Row pass (one lane per row): every module in pipeline order (FeH,
-// GaussianFeH, Bond2010, photometry, photometric errors, vel2pm, gal2other)
-// for the row, the random-number modules restarting the stream from the
-// recorded position.  A row's draws are the same numbers in the same order
+// Row pass (one lane per row, three launches): every module in pipeline
+// order (FeH, GaussianFeH, Bond2010 | photometry, photometric errors |
+// vel2pm, gal2other) for the row, the random-number modules restarting the
+// stream from the recorded position.  A row's draws are the same numbers in the same order
 // as in the sequential module kernels, so the output is identical; the
 // transcendental arithmetic runs at full occupancy instead of on one lane
 // per stream.
@@ -46,11 +46,6 @@ inline void gf_chain_planes(gf_chain_mods &m)
 	m.pl_bond = m.has_bond ? n++ : -1;
 	m.pl_perr = m.has_perr ? n++ : -1;
 	m.nplanes = n;
-}
-
-inline bool gf_chain_has_rows(const gf_chain_mods &m)
-{
-	return m.has_feh || m.has_gfeh || m.has_bond || m.has_phot || m.has_perr || m.has_v2pm || m.gal2other;
 }
 
 // Stream positions: plane p holds the (c, x) of row i at pos[p * pitch + i].
@@ -164,9 +159,11 @@ GF_D void chunk_draws(const gf_chain_mods &m, gf_mwc &rng, uint32_t r0, uint32_t
 	if (m.has_bond)
 		draw_walk<CACHED>(rng, r0, r1, prow0, P.pos + m.pl_bond * P.pitch, T, L, lane,
 		                  [&](gf_mwc &r, int cmp) { bond_skip(m.bond, r, cmp); });
-	if (m.has_perr)
+	if (m.has_perr) {
+		const int nc = perr_ncurves(m.perr);
 		draw_walk<CACHED>(rng, r0, r1, prow0, P.pos + m.pl_perr * P.pitch, T, L, lane,
-		                  [&](gf_mwc &r, int) { perr_skip(m.perr, r); });
+		                  [&](gf_mwc &r, int) { perr_skip(nc, r); });
+	}
 }
 
 GF_D gf_mwc restart(uint32_t a, const gf_positions &P, int plane, uint64_t prow)
@@ -179,10 +176,30 @@ GF_D gf_mwc restart(uint32_t a, const gf_positions &P, int plane, uint64_t prow)
 	return r;
 }
 
-// The row pass for one row; a is the multiplier of the row's stream and
-// prow its index in the position planes.
-GF_D void chain_row(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t prow, const gf_table &T,
-                    const gf_positions &P)
+// The row pass, in three groups so that each launch stays small enough in
+// registers to keep several waves per SIMD: KIN (FeH, GaussianFeH,
+// Bond2010), PHOT (photometry, photometric errors), ASTRO (vel2pm,
+// gal2other).  a is the multiplier of the row's stream and prow its index in
+// the position planes.
+enum { ROWS_KIN = 0, ROWS_PHOT = 1, ROWS_ASTRO = 2 };
+
+inline bool rows_on(const gf_chain_mods &m, int group)
+{
+	if (group == ROWS_KIN) return m.has_feh || m.has_gfeh || m.has_bond;
+	if (group == ROWS_PHOT) return m.has_phot || m.has_perr;
+	return m.has_v2pm || m.gal2other;
+}
+
+// whether the group restarts streams (needs the row's multiplier)
+inline bool rows_stream(const gf_chain_mods &m, int group)
+{
+	if (group == ROWS_KIN) return m.has_feh || m.has_gfeh || m.has_bond;
+	if (group == ROWS_PHOT) return m.has_perr;
+	return false;
+}
+
+GF_D void rows_kin(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t prow, const gf_table &T,
+                   const gf_positions &P)
 {
 	if (col<int>(T.hidden, row)) return;
 	const int cmp = col<int>(T.comp, row);
@@ -210,17 +227,44 @@ GF_D void chain_row(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t p
 		gf_mwc r = restart(a, P, m.pl_gfeh, prow);
 		gfeh_row(m.gfeh, m.gfeh_comps, r, cmp, row, T.FeH_comp, T.FeH);
 	}
+}
+
+template <int NT>
+GF_D void rows_phot(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t prow, const gf_table &T,
+                    const gf_positions &P)
+{
+	if (col<int>(T.hidden, row)) return;
 	if (m.has_phot) {
 		const gf_col &Mr = m.has_mult ? T.absmagSys : T.absmag;
-		mod_photometry(m.phot, m.phot_comps, row, row + 1, T.Am, T.photoFlags, T.DM, Mr, (int)Mr.width, T.mags,
-		               T.FeH, T.comp, T.hidden);
+		mod_photometry<NT>(m.phot, m.phot_comps, row, row + 1, T.Am, T.photoFlags, T.DM, Mr, (int)Mr.width,
+		                   T.mags, T.FeH, T.comp, T.hidden);
 	}
 	if (m.has_perr) {
 		gf_mwc r = restart(a, P, m.pl_perr, prow);
 		perr_row(m.perr, r, row, T.mags, T.obsMags);
 	}
+}
+
+GF_D void rows_astro(const gf_chain_mods &m, uint32_t row, const gf_table &T)
+{
 	if (m.has_v2pm) mod_vel2pm(m.v2pm, row, row + 1, T.lb, T.XYZ, T.vcyl, T.pm, T.hidden);
 	if (m.gal2other) mod_gal2other(m.gal2other, row, row + 1, T.hidden, T.lb, T.radec);
+}
+
+// GROUP: ROWS_KIN, ROWS_PHOT or ROWS_ASTRO; NT: colour tables of the
+// photometry (1..4, at least ceil(ncolors / 4); see phot_tables())
+template <int GROUP, int NT>
+GF_D void chain_rows(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t prow, const gf_table &T,
+                     const gf_positions &P)
+{
+	if (GROUP == ROWS_KIN) rows_kin(m, a, row, prow, T, P);
+	else if (GROUP == ROWS_PHOT) rows_phot<NT>(m, a, row, prow, T, P);
+	else rows_astro(m, row, T);
+}
+
+inline int phot_tables(const gf_chain_mods &m)
+{
+	return m.has_phot ? ((int)m.phot.ncolors + 3) / 4 : 1;
 }
 
 } // namespace gfc

@@ -10,16 +10,21 @@
 // with explicit fma() and rounded once to float:
 //
 //   exp:  Cody-Waite reduction x = k*ln2 + r, |r| <= ln2/2, Taylor series
-//         of e^r to degree 13 (truncation < 5e-18), scaling by 2^k.
+//         of e^r to degree 10 (truncation < 3e-13 relative), scaling by 2^k.
 //   log:  x = 2^e * m, m in [sqrt(1/2), sqrt(2)), s = (m-1)/(m+1),
 //         log m = 2*atanh(s) = 2s * sum z^n/(2n+1), z = s^2 <= 0.0295,
-//         summed to n = 12 (truncation < 1e-19).
+//         summed to n = 7 (truncation < 4e-14 relative).
+//   (exp and log only feed the float functions, so their double results
+//   need ~40 bits: rounding to float is then exact except within 2^-40 of a
+//   rounding boundary.)
 //   sin/cos: reduction by pi/2 (two-part constant, fma), Taylor series to
 //         degree 17/18 on |r| <= pi/4 (truncation < 1e-18).
 //
 // Only +, *, /, sqrt, fma and rint are used; all are correctly rounded
 // IEEE operations on gfx950 and on x86-64, so the results are identical on
-// both.  Errors are below one float ulp (rounding of an accurate double).
+// both.  Errors are below one float ulp (rounding of an accurate double),
+// and the float result is the correctly rounded one on all but a few in a
+// million arguments.
 // The double-precision sin/cos/atan/atan2/asin here (a few 1e-16 relative)
 // replace the platform's in the deprojection, direction cosines and sky
 // rotations for the same reason: the GPU and a CPU evaluation agree to the
@@ -68,10 +73,7 @@ GF_HD double exp(double x)
 	double kd = drint(x * INV_LN2);
 	double r = dfma(-kd, LN2_HI, x);
 	r = dfma(-kd, LN2_LO, r);
-	double p = 1.0 / 6227020800.0;            // 1/13!
-	p = dfma(p, r, 1.0 / 479001600.0);        // 1/12!
-	p = dfma(p, r, 1.0 / 39916800.0);         // 1/11!
-	p = dfma(p, r, 1.0 / 3628800.0);          // 1/10!
+	double p = 1.0 / 3628800.0;               // 1/10!
 	p = dfma(p, r, 1.0 / 362880.0);           // 1/9!
 	p = dfma(p, r, 1.0 / 40320.0);            // 1/8!
 	p = dfma(p, r, 1.0 / 5040.0);             // 1/7!
@@ -102,12 +104,7 @@ GF_HD double log(double x)
 	if (m > 1.41421356237309504880) { m *= 0.5; e += 1; }
 	double s = (m - 1.0) / (m + 1.0);
 	double z = s * s;
-	double p = 1.0 / 25.0;
-	p = dfma(p, z, 1.0 / 23.0);
-	p = dfma(p, z, 1.0 / 21.0);
-	p = dfma(p, z, 1.0 / 19.0);
-	p = dfma(p, z, 1.0 / 17.0);
-	p = dfma(p, z, 1.0 / 15.0);
+	double p = 1.0 / 15.0;
 	p = dfma(p, z, 1.0 / 13.0);
 	p = dfma(p, z, 1.0 / 11.0);
 	p = dfma(p, z, 1.0 / 9.0);
@@ -230,18 +227,27 @@ GF_HD float gf_tanf(float x) { double s, c; gfm::sincos((double)x, &s, &c); retu
 GF_HD float gf_sqrtf(float x) { return __builtin_sqrtf(x); }
 GF_HD float gf_fabsf(float x) { return __builtin_fabsf(x); }
 
-// x^y for float x >= 0 (and the integer-y cases of x < 0)
-GF_HD float gf_powf(float x, float y)
+// x^y for float x >= 0 (and NaN), with log(x) supplied by logx() only
+// when it is needed: callers that raise one x to several powers pass a
+// cached log and get exactly gf_powf's results.
+template <typename LogX>
+GF_HD float gf_powf_nonneg(float x, float y, LogX logx)
 {
 	if (y == 0.0f || x == 1.0f) return 1.0f;
 	if (x != x || y != y) return x + y;
 	if (x == 0.0f) return y > 0.0f ? 0.0f : gfm::as_float(0x7f800000u);
-	if (x < 0.0f) {
+	return (float)gfm::exp((double)y * logx());
+}
+
+// x^y for float x >= 0 (and the integer-y cases of x < 0)
+GF_HD float gf_powf(float x, float y)
+{
+	if (x < 0.0f && y != 0.0f && y == y) {
 		float yi = __builtin_truncf(y);
 		if (yi != y) return gfm::as_float(0x7fc00000u);
 		double r = gfm::exp((double)y * gfm::log(-(double)x));
 		int64_t iy = (int64_t)yi;
 		return (float)((iy & 1) ? -r : r);
 	}
-	return (float)gfm::exp((double)y * gfm::log((double)x));
+	return gf_powf_nonneg(x, y, [x]() { return gfm::log((double)x); });
 }

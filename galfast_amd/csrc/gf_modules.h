@@ -87,12 +87,35 @@ GF_D void mod_GaussianFeH(const gf_GaussianFeH_data &par, gf_bitmap apply, gf_mw
 	}
 }
 
+// Powers of one row's |Z| and R^2 (both >= 0): each log is taken once, on
+// first use, and every power equals gf_powf's.
+struct BondPow {
+	float aZ, R2;
+	double lZ = 0.0, lR = 0.0;
+	bool hZ = false, hR = false;
+	GF_D BondPow(float Rsquared, float Z) : aZ(gf_fabsf(Z)), R2(Rsquared) {}
+	GF_D float z(float y)
+	{
+		return gf_powf_nonneg(aZ, y, [this]() {
+			if (!hZ) { lZ = gfm::log((double)aZ); hZ = true; }
+			return lZ;
+		});
+	}
+	GF_D float r(float y)
+	{
+		return gf_powf_nonneg(R2, y, [this]() {
+			if (!hR) { lR = gfm::log((double)R2); hR = true; }
+			return lR;
+		});
+	}
+};
+
 // modfun (modules/Bond2010_gpu.cu.h:106-110): a + b|Z|^c + d R^e, with the
 // exponent 0.5*e formed in double and narrowed as in the reference.
-GF_D float bond_modfun(float Rsquared, float Z, const float val[5])
+GF_D float bond_modfun(BondPow &bp, const float val[5])
 {
-	float p1 = gf_powf(gf_fabsf(Z), val[2]);
-	float p2 = gf_powf(Rsquared, (float)(0.5 * (double)val[4]));
+	float p1 = bp.z(val[2]);
+	float p2 = bp.r((float)(0.5 * (double)val[4]));
 	return val[0] + val[1] * p1 + val[3] * p2;
 }
 
@@ -127,22 +150,23 @@ GF_D bool bond_row(const gf_Bond2010_data &par, gf_mwc &rng, int cmp, float x, f
 	transform3(x, y, z, par.T, par.M);
 	const float Rsquared = (float)(1e-6 * (double)(x * x + y * y));
 	const float Z = (float)(1e-3 * (double)z);
+	BondPow bp(Rsquared, Z);
 	if (bit_isset(par.comp_thin, cmp) || bit_isset(par.comp_thick, cmp)) {
 		// get_disk_kinematics (modules/Bond2010_gpu.cu.h:130-140)
 		float p = rng.uniform();
 		const gf_vmoments &disk = p < par.fk ? par.disk_1 : par.disk_2;
-		for (int i = 0; i < 3; i++) v[i] = bond_modfun(Rsquared, Z, disk.m[i]);
+		for (int i = 0; i < 3; i++) v[i] = bond_modfun(bp, disk.m[i]);
 		if (v[1] > 0.f) v[1] = 0.f;
 		float s[6];
-		for (int i = 0; i < 6; i++) s[i] = bond_modfun(Rsquared, Z, disk.ss[i]);
+		for (int i = 0; i < 6; i++) s[i] = bond_modfun(bp, disk.ss[i]);
 		bond_trivar(v, s, rng);
 		return true;
 	}
 	if (bit_isset(par.comp_halo, cmp)) {
 		// get_halo_kinematics (modules/Bond2010_gpu.cu.h:142-159)
-		for (int i = 0; i < 3; i++) v[i] = bond_modfun(Rsquared, Z, par.halo.m[i]);
+		for (int i = 0; i < 3; i++) v[i] = bond_modfun(bp, par.halo.m[i]);
 		float s[6];
-		for (int i = 0; i < 6; i++) s[i] = bond_modfun(Rsquared, Z, par.halo.ss[i]);
+		for (int i = 0; i < 6; i++) s[i] = bond_modfun(bp, par.halo.ss[i]);
 		bond_trivar(v, s, rng);
 		float invR = 1.f / gf_sqrtf(Rsquared + Z * Z);
 		float cosa = gf_sqrtf(Rsquared) * invR;
@@ -160,15 +184,12 @@ GF_D bool bond_row(const gf_Bond2010_data &par, gf_mwc &rng, int cmp, float x, f
 // The draws bond_row makes for a row of component cmp.
 GF_D void bond_skip(const gf_Bond2010_data &par, gf_mwc &rng, int cmp)
 {
-	float yy, r2;
 	if (bit_isset(par.comp_thin, cmp) || bit_isset(par.comp_thick, cmp)) {
 		(void)rng.uniform();
 	} else if (!bit_isset(par.comp_halo, cmp)) {
 		return;
 	}
-	rng.polar(yy, r2);
-	rng.polar(yy, r2);
-	rng.polar(yy, r2);
+	rng.polar_skip(3);
 }
 
 // os_Bond2010_kernel body (modules/Bond2010_gpu.cu.h:181-217)
@@ -301,15 +322,8 @@ GF_D void mod_gal2other(int coordsys, uint32_t r0, uint32_t r1, const gf_col &hi
 
 // Bilinear sample of a float4 colour table (the TEX2D lookups of
 // sampleColors, modules/photometry_gpu.cu.h:56-71).
-GF_D float4 tex2d4(const float *tab, int nx, int ny, const float tcx[2], const float tcy[2], float x, float y)
+GF_D float4 bilerp4(float4 t00, float4 t10, float4 t01, float4 t11, float a, float b)
 {
-	int i, j;
-	float a = lerp_coord(x, tcx[0], tcx[1], nx, i);
-	float b = lerp_coord(y, tcy[0], tcy[1], ny, j);
-	int i1 = nx > 1 ? i + 1 : i, j1 = ny > 1 ? j + 1 : j;
-	const float4 *t = (const float4 *)tab;
-	float4 t00 = t[(size_t)j * nx + i], t10 = t[(size_t)j * nx + i1];
-	float4 t01 = t[(size_t)j1 * nx + i], t11 = t[(size_t)j1 * nx + i1];
 	float4 r;
 #define GF_BL(c)                                             \
 	{                                                        \
@@ -322,13 +336,42 @@ GF_D float4 tex2d4(const float *tab, int nx, int ny, const float tcx[2], const f
 	return r;
 }
 
-// sampleColors (modules/photometry_gpu.cu.h:56-71)
+GF_D float4 tex2d4(const float *tab, int nx, int ny, const float tcx[2], const float tcy[2], float x, float y)
+{
+	int i, j;
+	float a = lerp_coord(x, tcx[0], tcx[1], nx, i);
+	float b = lerp_coord(y, tcy[0], tcy[1], ny, j);
+	int i1 = nx > 1 ? i + 1 : i, j1 = ny > 1 ? j + 1 : j;
+	const float4 *t = (const float4 *)tab;
+	return bilerp4(t[(size_t)j * nx + i], t[(size_t)j * nx + i1], t[(size_t)j1 * nx + i], t[(size_t)j1 * nx + i1],
+	               a, b);
+}
+
+// sampleColors (modules/photometry_gpu.cu.h:56-71).  NT: the most colour
+// tables (float4 each) the caller's arrays hold; the loops are unrolled to
+// it, so a small NT keeps the caller's register footprint small.
+template <int NT = 4>
 GF_D uint32_t sample_colors(const gf_photometry_data &lt, float *c, float FeH, float Mr)
 {
 	uint32_t flags = 0, cidx = 0;
-	for (int t = 0; t < 4 && cidx < lt.ncolors; t++) {
-		float4 clr = tex2d4(lt.color[t], lt.nFeH, lt.nMr, lt.tcFeH, lt.tcMr, FeH, Mr);
-		float4 f = tex2d4(lt.cflags[t], lt.nFeH, lt.nMr, lt.tcFeH, lt.tcMr, FeH, Mr);
+	// texel coordinates, shared by every table (interleaved layout)
+	int i, j;
+	const float a = lerp_coord(FeH, lt.tcFeH[0], lt.tcFeH[1], lt.nFeH, i);
+	const float b = lerp_coord(Mr, lt.tcMr[0], lt.tcMr[1], lt.nMr, j);
+	const int i1 = lt.nFeH > 1 ? i + 1 : i, j1 = lt.nMr > 1 ? j + 1 : j;
+	const int ntex = ((int)lt.ncolors + 3) / 4;
+	const float4 *tx = (const float4 *)lt.texels;
+	const size_t k00 = ((size_t)j * lt.nFeH + i) * 2 * ntex, k10 = ((size_t)j * lt.nFeH + i1) * 2 * ntex;
+	const size_t k01 = ((size_t)j1 * lt.nFeH + i) * 2 * ntex, k11 = ((size_t)j1 * lt.nFeH + i1) * 2 * ntex;
+	for (int t = 0; t < NT && cidx < lt.ncolors; t++) {
+		float4 clr, f;
+		if (tx) {
+			clr = bilerp4(tx[k00 + t], tx[k10 + t], tx[k01 + t], tx[k11 + t], a, b);
+			f = bilerp4(tx[k00 + ntex + t], tx[k10 + ntex + t], tx[k01 + ntex + t], tx[k11 + ntex + t], a, b);
+		} else {
+			clr = tex2d4(lt.color[t], lt.nFeH, lt.nMr, lt.tcFeH, lt.tcMr, FeH, Mr);
+			f = tex2d4(lt.cflags[t], lt.nFeH, lt.nMr, lt.tcFeH, lt.tcMr, FeH, Mr);
+		}
 		float cv[4] = {clr.x, clr.y, clr.z, clr.w};
 		float fv[4] = {f.x, f.y, f.z, f.w};
 		for (int q = 0; q < 4 && cidx < lt.ncolors; q++) {
@@ -340,34 +383,59 @@ GF_D uint32_t sample_colors(const gf_photometry_data &lt, float *c, float FeH, f
 	return flags;
 }
 
-// os_photometry_kernel body (modules/photometry_gpu.cu.h:100-173)
+// os_photometry_kernel body (modules/photometry_gpu.cu.h:100-173), for at
+// most 4*NT colours (the caller checks ncolors <= 4*NT)
+template <int NT = 4>
 GF_D void mod_photometry(const gf_photometry_data &lt, gf_bitmap apply, uint32_t r0, uint32_t r1,
                          const gf_col &Am, const gf_col &flags, const gf_col &DM, const gf_col &Mr,
                          int nabsmag, const gf_col &mags, const gf_col &FeH, const gf_col &comp,
                          const gf_col &hidden)
 {
-	float c[16];
-	const int nb = (int)lt.ncolors;
+	constexpr int NC = 4 * NT;
+	float c[NC];
+	const int nb = (int)lt.ncolors < NC ? (int)lt.ncolors : NC;
 	for (uint32_t row = r0; row < r1; row++) {
 		if (col<int>(hidden, row)) continue;
 		int cmp = col<int>(comp, row);
 		if (!bit_isset(apply, cmp)) continue;
 		float fFeH = col<float>(FeH, row);
+		const float dm = col<float>(DM, row), Am0 = col<float>(Am, row);
 		uint32_t flag = 0;
-		float L[17];
-		for (int b = 0; b <= nb; b++) L[b] = 0.f;
-		for (int syscomp = 0; syscomp < nabsmag; syscomp++) {
-			float fMr = col<float>(Mr, row, syscomp);
-			if (fMr >= ABSMAG_NOT_PRESENT) break;
-			flag |= sample_colors(lt, c, fFeH, fMr);
+		const float fMr0 = col<float>(Mr, row, 0);
+		const bool system = fMr0 < ABSMAG_NOT_PRESENT && nabsmag > 1 && col<float>(Mr, row, 1) < ABSMAG_NOT_PRESENT;
+		if (!system) {
+			// a lone star: its magnitudes are the colour sums themselves
+			// (-2.5 log10 10^(-0.4 M) = M, without the round trip)
+			if (fMr0 < ABSMAG_NOT_PRESENT) flag = sample_colors<NT>(lt, c, fFeH, fMr0);
 			for (int b = 0; b <= nb; b++) {
-				float M = fMr;
-				if (b < (int)lt.bidx) { for (int i = b; i != (int)lt.bidx; i++) M += c[i]; }
-				if (b > (int)lt.bidx) { for (int i = lt.bidx; i != b; i++) M -= c[i]; }
-				float Lb = gf_exp10f(-0.4f * M);
-				if (syscomp) Lb += L[b];
-				L[b] = Lb;
+				float M = fMr0;
+				if (fMr0 < ABSMAG_NOT_PRESENT) {
+					if (b < (int)lt.bidx) { for (int i = b; i != (int)lt.bidx; i++) M += c[i]; }
+					if (b > (int)lt.bidx) { for (int i = lt.bidx; i != b; i++) M -= c[i]; }
+				} else {
+					M = gfm::as_float(0x7f800000u);   // no component: zero luminosity
+				}
+				col<float>(mags, row, b) = (dm + M) + Am0 * lt.reddening[b];
 			}
+		} else {
+			// a system: luminosities summed over its components, newest first
+			float L[NC + 1];
+			for (int b = 0; b <= nb; b++) L[b] = 0.f;
+			for (int syscomp = 0; syscomp < nabsmag; syscomp++) {
+				float fMr = col<float>(Mr, row, syscomp);
+				if (fMr >= ABSMAG_NOT_PRESENT) break;
+				flag |= sample_colors<NT>(lt, c, fFeH, fMr);
+				for (int b = 0; b <= nb; b++) {
+					float M = fMr;
+					if (b < (int)lt.bidx) { for (int i = b; i != (int)lt.bidx; i++) M += c[i]; }
+					if (b > (int)lt.bidx) { for (int i = lt.bidx; i != b; i++) M -= c[i]; }
+					float Lb = gf_exp10f(-0.4f * M);
+					if (syscomp) Lb += L[b];
+					L[b] = Lb;
+				}
+			}
+			for (int b = 0; b <= nb; b++)
+				col<float>(mags, row, b) = (dm + -2.5f * gf_log10f(L[b])) + Am0 * lt.reddening[b];
 		}
 		bool fl = false;
 		for (int i = (int)lt.bidx - 1; i >= 0; i--) {
@@ -385,15 +453,6 @@ GF_D void mod_photometry(const gf_photometry_data &lt, gf_bitmap apply, uint32_t
 		flag = __brev(flag);
 		flag >>= (32 - nb - 1);
 		col<int>(flags, row) = (int)flag;
-
-		float dm = col<float>(DM, row);
-		float Am0 = col<float>(Am, row);
-		for (int b = 0; b <= nb; b++) {
-			float Msys = -2.5f * gf_log10f(L[b]);
-			float msys = dm + Msys;
-			float am = Am0 * lt.reddening[b];
-			col<float>(mags, row, b) = msys + am;
-		}
 	}
 }
 
@@ -488,11 +547,17 @@ GF_D void perr_row(const gf_photoerr_data &par, gf_mwc &rng, uint32_t row, const
 	}
 }
 
-GF_D void perr_skip(const gf_photoerr_data &par, gf_mwc &rng)
+// ncurves: the number of bands with an error curve
+GF_D void perr_skip(int ncurves, gf_mwc &rng)
 {
-	float yy, r2;
-	for (int b = 0; b < par.nbands; b++)
-		if (par.koff[b + 1] > par.koff[b]) rng.polar(yy, r2);
+	rng.polar_skip(ncurves);
+}
+
+GF_D int perr_ncurves(const gf_photoerr_data &par)
+{
+	int n = 0;
+	for (int b = 0; b < par.nbands; b++) n += par.koff[b + 1] > par.koff[b] ? 1 : 0;
+	return n;
 }
 
 GF_D void mod_photoerr(const gf_photoerr_data &par, gf_mwc &rng, uint32_t r0, uint32_t r1,

@@ -45,6 +45,19 @@ struct gf_mwc
 		} while (r2 > 1.0f || r2 == 0.f || r2 == 4.f);
 	}
 
+	// Move the stream past n polar deviates: the same draws as n calls of
+	// polar(), in one loop, so that lanes of a wave that need different
+	// numbers of tries for each deviate wait once instead of n times.
+	GF_HD void polar_skip(int n)
+	{
+		while (n > 0) {
+			float xx = -1.f + 2.f * uniform();
+			float yy = -1.f + 2.f * uniform();
+			float r2 = xx * xx + yy * yy;
+			n -= (r2 > 1.0f || r2 == 0.f || r2 == 4.f) ? 0 : 1;
+		}
+	}
+
 	GF_HD float gaussian(const float sigma)
 	{
 		float yy, r2;

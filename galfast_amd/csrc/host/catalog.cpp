@@ -136,7 +136,7 @@ struct gf_catalog {
 	gf_Bond2010_data bond{};
 	gf_photometry_data phot{};
 	PhotTables phot_tables;
-	DevBuf phot_dev[8];
+	DevBuf phot_dev[8], phot_il;
 	int nbands = 0;
 	std::vector<std::string> band_names;
 	gf_photoerr_data perr{};
@@ -577,6 +577,19 @@ void construct_photometry(gf_catalog &c, const Config &cfg)
 		p.color[t] = (const float *)c.phot_dev[t].p;
 		p.cflags[t] = (const float *)c.phot_dev[4 + t].p;
 	}
+	// the interleaved copy: per texel the ntex colour float4s, then the flags
+	{
+		const size_t ntexel = (size_t)pt.nFeH * pt.nMr, nt = (size_t)pt.ntex;
+		std::vector<float> il(ntexel * 8 * nt);
+		for (size_t k = 0; k < ntexel; k++)
+			for (size_t t = 0; t < nt; t++)
+				for (int q = 0; q < 4; q++) {
+					il[(k * 2 * nt + t) * 4 + q] = pt.color[t * per + k * 4 + q];
+					il[(k * 2 * nt + nt + t) * 4 + q] = pt.flags[t * per + k * 4 + q];
+				}
+		c.phot_il.upload(il);
+		p.texels = (const float *)c.phot_il.p;
+	}
 	p.nFeH = pt.nFeH;
 	p.nMr = pt.nMr;
 	p.tcFeH[0] = pt.tcFeH[0]; p.tcFeH[1] = pt.tcFeH[1];
@@ -776,6 +789,7 @@ void open_catalog(gf_catalog &c, const std::string &cmdfile)
 			c.phot_h.color[t] = t < c.phot_tables.ntex ? c.phot_tables.color.data() + t * per : nullptr;
 			c.phot_h.cflags[t] = t < c.phot_tables.ntex ? c.phot_tables.flags.data() + t * per : nullptr;
 		}
+		c.phot_h.texels = nullptr;
 	}
 	if (c.has_mult) {
 		c.mult_h = c.mult;

@@ -209,19 +209,20 @@ __global__ void __launch_bounds__(gfc::BS) k_b_draws(const gf_serve_req *reqs, i
 	gf_mwc_store(rq.state, rq.S, s, rng);
 }
 
+template <int GROUP, int NT>
 __global__ void __launch_bounds__(gfc::BS) k_b_rows(const gf_serve_req *reqs, int nreq, const gf_serve_chain *ch,
-                                                    uint64_t nrows, gf_table T, gf_positions P)
+                                                    uint64_t nrows, gf_table T, gf_positions P, int need_stream)
 {
 	uint64_t row = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (row >= nrows) return;
 	uint32_t a = 0;
-	if (ch->nplanes) {
+	if (need_stream) {
 		int ri = upper_index(reqs, nreq, [](const gf_serve_req &q) { return q.row0; }, row);
 		const gf_serve_req &rq = reqs[ri];
 		uint64_t c = (row - rq.row0) / 256;
 		a = rq.state[c % rq.S];
 	}
-	gfc::chain_row(*ch, a, (uint32_t)row, row, T, P);
+	gfc::chain_rows<GROUP, NT>(*ch, a, (uint32_t)row, row, T, P);
 }
 
 } // namespace
@@ -283,8 +284,23 @@ int serve_chain(const gf_serve_req *dreqs, int nreq, uint32_t nlanes, const gf_s
 		k_b_draws<<<gf::blocks_for(nlanes, gfc::BS), gfc::BS, 0, st>>>(dreqs, nreq, nlanes, dch, T, P);
 		GF_LAUNCHED();
 	}
-	if (gf_chain_has_rows(hch)) {
-		k_b_rows<<<gf::blocks_for(nrows, gfc::BS), gfc::BS, 0, st>>>(dreqs, nreq, dch, nrows, T, P);
+	const unsigned g = gf::blocks_for(nrows, gfc::BS);
+	if (gfc::rows_on(hch, gfc::ROWS_KIN)) {
+		k_b_rows<gfc::ROWS_KIN, 1><<<g, gfc::BS, 0, st>>>(dreqs, nreq, dch, nrows, T, P, 1);
+		GF_LAUNCHED();
+	}
+	if (gfc::rows_on(hch, gfc::ROWS_PHOT)) {
+		const int ns = (int)gfc::rows_stream(hch, gfc::ROWS_PHOT);
+		switch (gfc::phot_tables(hch)) {
+		case 1: k_b_rows<gfc::ROWS_PHOT, 1><<<g, gfc::BS, 0, st>>>(dreqs, nreq, dch, nrows, T, P, ns); break;
+		case 2: k_b_rows<gfc::ROWS_PHOT, 2><<<g, gfc::BS, 0, st>>>(dreqs, nreq, dch, nrows, T, P, ns); break;
+		case 3: k_b_rows<gfc::ROWS_PHOT, 3><<<g, gfc::BS, 0, st>>>(dreqs, nreq, dch, nrows, T, P, ns); break;
+		default: k_b_rows<gfc::ROWS_PHOT, 4><<<g, gfc::BS, 0, st>>>(dreqs, nreq, dch, nrows, T, P, ns); break;
+		}
+		GF_LAUNCHED();
+	}
+	if (gfc::rows_on(hch, gfc::ROWS_ASTRO)) {
+		k_b_rows<gfc::ROWS_ASTRO, 1><<<g, gfc::BS, 0, st>>>(dreqs, nreq, dch, nrows, T, P, 0);
 		GF_LAUNCHED();
 	}
 	return 0;

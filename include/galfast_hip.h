@@ -177,6 +177,11 @@ typedef struct gf_photometry_data {
     float    tcFeH[2], tcMr[2];
     const float *color[4];   /* device, float4 texels */
     const float *cflags[4];  /* device, float4 texels */
+    /* optional (device): the same texels interleaved, texel k holding the
+     * ntex colour float4s then the ntex flag float4s at texels + 8*ntex*k,
+     * so that the two texels a lookup needs in one Mr row are one
+     * contiguous fetch; used instead of color/cflags when set */
+    const float *texels;
 } gf_photometry_data;
 
 /* Tabulated 1D function on a uniform grid, sampled with the reference's

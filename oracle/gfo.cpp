@@ -1072,8 +1072,9 @@ static void phot_rows(const gf_photometry_data &lt, gf_bitmap apply, uint32_t r0
 		if (!isset(apply, cmp)) continue;
 		float FeH = C<float>(t.FeH, row);
 		uint32_t flag = 0;
-		float L[17] = {0};
+		float L[17] = {0}, Mfirst[17] = {0};
 		float c[16];
+		int ncomp = 0;
 		for (int sc = 0; sc < (int)Mr.width; sc++) {
 			float fMr = C<float>(Mr, row, sc);
 			if (fMr >= 99.999f) break;
@@ -1092,10 +1093,12 @@ static void phot_rows(const gf_photometry_data &lt, gf_bitmap apply, uint32_t r0
 				float M = fMr;
 				if (b < bidx) for (int i = b; i != bidx; i++) M += c[i];
 				if (b > bidx) for (int i = bidx; i != b; i++) M -= c[i];
-				float Lb = gfo_exp10f(-0.4f * M);
-				if (sc) Lb += L[b];
-				L[b] = Lb;
+				/* a lone star keeps its magnitude; a system sums luminosities,
+				 * newest component first as in the reference's running sum */
+				if (sc == 0) Mfirst[b] = M;
+				else L[b] = gfo_exp10f(-0.4f * M) + (sc == 1 ? gfo_exp10f(-0.4f * Mfirst[b]) : L[b]);
 			}
+			ncomp++;
 		}
 		bool fl = false;
 		for (int i = bidx - 1; i >= 0; i--) { fl = fl || (flag & (1u << i)); flag |= (uint32_t)fl << i; }
@@ -1107,7 +1110,7 @@ static void phot_rows(const gf_photometry_data &lt, gf_bitmap apply, uint32_t r0
 		C<int>(t.photoFlags, row) = (int)flag;
 		float dm = C<float>(t.DM, row), Am0 = C<float>(t.Am, row);
 		for (int b = 0; b <= nb; b++) {
-			float Msys = -2.5f * gfo_log10f(L[b]);
+			float Msys = ncomp == 1 ? Mfirst[b] : -2.5f * gfo_log10f(L[b]);
 			float msys = dm + Msys;
 			C<float>(t.mags, row, b) = msys + Am0 * lt.reddening[b];
 		}

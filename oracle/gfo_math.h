@@ -6,9 +6,9 @@
  * skygen/model_expDisk.h and the module kernels).  The oracle evaluates each one in
  * IEEE double and rounds once to float:
  *   exp: k = rint(x/ln2), r = x - k*ln2 (two-part ln2, fma), e^r by its
- *        Taylor series to r^13/13!, times 2^k;
+ *        Taylor series to r^10/10!, times 2^k;
  *   log: x = 2^e m, m in [1/sqrt2, sqrt2), log m = 2 atanh((m-1)/(m+1)) by
- *        its series to z^12/25;
+ *        its series to z^7/15 (both ~40 bits: they only feed float results);
  *   sin/cos: k = rint(x/(pi/2)), r = x - k*pi/2 (two-part, fma), Taylor
  *        series to degree 17/18, quadrant by k mod 4.
  * Pinned in tests/test_oracle_math.py against glibc (at most one float ulp,
@@ -39,12 +39,11 @@ static inline double gfo_exp(double x)
 	double kd = rint(x * 1.44269504088896338700e+00);
 	double r = fma(-kd, GFO_LN2_HI, x);
 	r = fma(-kd, GFO_LN2_LO, r);
-	static const double inv_fact[14] = {
+	static const double inv_fact[11] = {
 		1.0, 1.0, 1.0 / 2.0, 1.0 / 6.0, 1.0 / 24.0, 1.0 / 120.0, 1.0 / 720.0, 1.0 / 5040.0,
-		1.0 / 40320.0, 1.0 / 362880.0, 1.0 / 3628800.0, 1.0 / 39916800.0, 1.0 / 479001600.0,
-		1.0 / 6227020800.0};
-	double p = inv_fact[13];
-	for (int i = 12; i >= 0; i--) p = fma(p, r, inv_fact[i]);
+		1.0 / 40320.0, 1.0 / 362880.0, 1.0 / 3628800.0};
+	double p = inv_fact[10];
+	for (int i = 9; i >= 0; i--) p = fma(p, r, inv_fact[i]);
 	int k = (int)kd;
 	if (k > 1023) { p *= gfo_pow2i(1023); k -= 1023; }
 	if (k < -1022) { p *= gfo_pow2i(-1022); k += 1022; }
@@ -64,8 +63,8 @@ static inline double gfo_log(double x)
 	if (m > 1.41421356237309504880) { m *= 0.5; e += 1; }
 	double s = (m - 1.0) / (m + 1.0);
 	double z = s * s;
-	double p = 1.0 / 25.0;
-	for (int n = 11; n >= 0; n--) p = fma(p, z, 1.0 / (double)(2 * n + 1));
+	double p = 1.0 / 15.0;
+	for (int n = 6; n >= 0; n--) p = fma(p, z, 1.0 / (double)(2 * n + 1));
 	double lm = 2.0 * s * p;
 	double ed = (double)e;
 	return fma(ed, GFO_LN2_HI, fma(ed, GFO_LN2_LO, lm));

@@ -46,13 +46,16 @@ STAGE_BYTES = {
     "k_chain_draws": (4 + 4 + 3 * 8, 4 + 4),
     # kinematics rows: read hidden, comp, XYZ, 2 positions; write FeH, vcyl
     "k_chain_rows<KIN>": (4 + 4 + 12 + 2 * 8 + 4 + 12, 4),
-    # photometry rows: read hidden, comp, FeH, absmag, DM, Am, 1 position;
-    # write mags, photoFlags, obsMags
-    "k_chain_rows<PHOT>": (4 + 4 + 4 + 4 + 4 + 4 + 8 + 20 + 4 + 20, 4),
+    # photometry rows: read hidden, comp, FeH, absmag, DM, Am; write mags,
+    # photoFlags
+    "k_chain_rows<PHOT>": (4 + 4 + 4 + 4 + 4 + 4 + 20 + 4, 4),
+    # photometric-error rows: read hidden, mags, 1 position; write obsMags
+    "k_chain_rows<PERR>": (4 + 20 + 8 + 20, 4),
     # astrometry rows: read hidden, lb, XYZ, vcyl; write pm, radec
     "k_chain_rows<ASTRO>": (4 + 16 + 12 + 12 + 12 + 16, 4),
 }
-STAGE_NAMES = ["k_chain_draws", "k_chain_rows<KIN>", "k_chain_rows<PHOT>", "k_chain_rows<ASTRO>"]
+STAGE_NAMES = ["k_chain_draws", "k_chain_rows<KIN>", "k_chain_rows<PHOT>", "k_chain_rows<PERR>",
+               "k_chain_rows<ASTRO>"]
 
 
 def cpu_baseline(cat, workload, budget_s):
@@ -184,7 +187,7 @@ def main():
     G.lib().gf_sync(None)
     t0 = time.perf_counter()
     acc = {"visible": 0, "generated": 0, "t_count": 0.0, "t_plan": 0.0, "t_emit": 0.0, "t_chain": 0.0,
-           "n_emit": 0, "n_chain": 0, "expected": 0.0, "t_stage": [0.0] * 4}
+           "n_emit": 0, "n_chain": 0, "expected": 0.0, "t_stage": [0.0] * len(STAGE_NAMES)}
     for _ in range(args.steps):
         expected, tc, st = step()
         acc["visible"] += st.visible
@@ -196,7 +199,7 @@ def main():
         acc["n_emit"] += st.n_emit
         acc["n_chain"] += st.n_chain
         acc["expected"] += expected
-        for q in range(4):
+        for q in range(len(STAGE_NAMES)):
             acc["t_stage"][q] += st.t_stage[q]
     G.lib().gf_sync(None)
     elapsed = time.perf_counter() - t0

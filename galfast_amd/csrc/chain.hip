@@ -53,10 +53,10 @@ void launch_phot(unsigned grid, hipStream_t st, const ChainDev *dp, const uint32
 
 namespace gf {
 
-// The chain with optional per-launch timing: when ev is non-NULL it holds 5
-// events; ms[0] receives the draw pass's duration, ms[1..3] the row pass's
-// three groups (kinematics, photometry, astrometry); 0 for a launch with
-// nothing to do.
+// The chain with optional per-launch timing: when ev is non-NULL it holds
+// GF_CHAIN_LAUNCHES + 1 events; ms[0] receives the draw pass's duration,
+// ms[1..4] the row pass's groups (kinematics, photometry, photometric
+// errors, astrometry); 0 for a launch with nothing to do.
 int module_chain(const gf_rows *ks, const gf_chain *chain, gf_rng *rng, const gf_table *t, hipStream_t st,
                  hipEvent_t *ev, float *ms)
 {
@@ -64,7 +64,7 @@ int module_chain(const gf_rows *ks, const gf_chain *chain, gf_rng *rng, const gf
 	GF_CHECK(!(chain->feh && chain->gfeh), GF_E_ARG);
 	uint32_t n = ks->end - ks->begin;
 	uint32_t nthr = n / ks->step + (n % ks->step ? 1 : 0);
-	if (ms) for (int i = 0; i < 4; i++) ms[i] = 0.f;
+	if (ms) for (int i = 0; i < GF_CHAIN_LAUNCHES; i++) ms[i] = 0.f;
 	if (nthr == 0) return 0;
 	GF_CHECK(nthr <= rng->nstreams, GF_E_STREAMS);
 	if (chain->phot) GF_CHECK(chain->phot->ncolors >= 1 && chain->phot->ncolors <= 16, GF_E_ARG);
@@ -96,8 +96,9 @@ int module_chain(const gf_rows *ks, const gf_chain *chain, gf_rng *rng, const gf
 	if (rc) return rc;
 	GF_HIP(hipMemcpyAsync(p, &h, sizeof(ChainDev), hipMemcpyHostToDevice, st));
 	const ChainDev *dp = (const ChainDev *)p;
-	const bool on[4] = {m.nplanes || m.has_mult, gfc::rows_on(m, gfc::ROWS_KIN), gfc::rows_on(m, gfc::ROWS_PHOT),
-	                    gfc::rows_on(m, gfc::ROWS_ASTRO)};
+	const bool on[GF_CHAIN_LAUNCHES] = {m.nplanes || m.has_mult, gfc::rows_on(m, gfc::ROWS_KIN),
+	                                    gfc::rows_on(m, gfc::ROWS_PHOT), gfc::rows_on(m, gfc::ROWS_PERR),
+	                                    gfc::rows_on(m, gfc::ROWS_ASTRO)};
 	const unsigned grows = gf::blocks_for(n, gfc::BS);
 	if (ev) GF_HIP(hipEventRecord(ev[0], st));
 	if (on[0]) {
@@ -113,25 +114,29 @@ int module_chain(const gf_rows *ks, const gf_chain *chain, gf_rng *rng, const gf
 	}
 	if (ev) GF_HIP(hipEventRecord(ev[2], st));
 	if (on[2]) {
-		const int ns = (int)gfc::rows_stream(m, gfc::ROWS_PHOT);
 		switch (gfc::phot_tables(m)) {
-		case 1: launch_phot<1>(grows, st, dp, rng->state, ns); break;
-		case 2: launch_phot<2>(grows, st, dp, rng->state, ns); break;
-		case 3: launch_phot<3>(grows, st, dp, rng->state, ns); break;
-		default: launch_phot<4>(grows, st, dp, rng->state, ns); break;
+		case 1: launch_phot<1>(grows, st, dp, rng->state, 0); break;
+		case 2: launch_phot<2>(grows, st, dp, rng->state, 0); break;
+		case 3: launch_phot<3>(grows, st, dp, rng->state, 0); break;
+		default: launch_phot<4>(grows, st, dp, rng->state, 0); break;
 		}
 		GF_LAUNCHED();
 	}
 	if (ev) GF_HIP(hipEventRecord(ev[3], st));
 	if (on[3]) {
-		k_chain_rows<gfc::ROWS_ASTRO, 1><<<grows, gfc::BS, 0, st>>>(dp, rng->state, 0);
+		k_chain_rows<gfc::ROWS_PERR, 1><<<grows, gfc::BS, 0, st>>>(dp, rng->state, 1);
 		GF_LAUNCHED();
 	}
 	if (ev) GF_HIP(hipEventRecord(ev[4], st));
+	if (on[4]) {
+		k_chain_rows<gfc::ROWS_ASTRO, 1><<<grows, gfc::BS, 0, st>>>(dp, rng->state, 0);
+		GF_LAUNCHED();
+	}
+	if (ev) GF_HIP(hipEventRecord(ev[5], st));
 	// the staging copy must complete before the host buffer is reused
 	GF_HIP(hipStreamSynchronize(st));
 	if (ev && ms)
-		for (int i = 0; i < 4; i++)
+		for (int i = 0; i < GF_CHAIN_LAUNCHES; i++)
 			if (on[i]) GF_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
 	return 0;
 }

@@ -9,8 +9,8 @@
 // values).  unresolvedMultiples, whose number of draws depends on the row's
 // data, runs in full here.
 //
-// Row pass (one lane per row, three launches): every module in pipeline
-// order (FeH, GaussianFeH, Bond2010 | photometry, photometric errors |
+// Row pass (one lane per row, four launches): every module in pipeline
+// order (FeH, GaussianFeH, Bond2010 | photometry | photometric errors |
 // vel2pm, gal2other) for the row, the random-number modules restarting the
 // stream from the recorded position.  A row's draws are the same numbers in the same order
 // as in the sequential module kernels, so the output is identical; the
@@ -176,26 +176,25 @@ GF_D gf_mwc restart(uint32_t a, const gf_positions &P, int plane, uint64_t prow)
 	return r;
 }
 
-// The row pass, in three groups so that each launch stays small enough in
+// The row pass, in four groups so that each launch stays small enough in
 // registers to keep several waves per SIMD: KIN (FeH, GaussianFeH,
-// Bond2010), PHOT (photometry, photometric errors), ASTRO (vel2pm,
+// Bond2010), PHOT (photometry), PERR (photometric errors), ASTRO (vel2pm,
 // gal2other).  a is the multiplier of the row's stream and prow its index in
 // the position planes.
-enum { ROWS_KIN = 0, ROWS_PHOT = 1, ROWS_ASTRO = 2 };
+enum { ROWS_KIN = 0, ROWS_PHOT = 1, ROWS_PERR = 2, ROWS_ASTRO = 3, ROWS_GROUPS = 4 };
 
 inline bool rows_on(const gf_chain_mods &m, int group)
 {
 	if (group == ROWS_KIN) return m.has_feh || m.has_gfeh || m.has_bond;
-	if (group == ROWS_PHOT) return m.has_phot || m.has_perr;
+	if (group == ROWS_PHOT) return m.has_phot;
+	if (group == ROWS_PERR) return m.has_perr;
 	return m.has_v2pm || m.gal2other;
 }
 
 // whether the group restarts streams (needs the row's multiplier)
 inline bool rows_stream(const gf_chain_mods &m, int group)
 {
-	if (group == ROWS_KIN) return m.has_feh || m.has_gfeh || m.has_bond;
-	if (group == ROWS_PHOT) return m.has_perr;
-	return false;
+	return group == ROWS_KIN || group == ROWS_PERR;
 }
 
 GF_D void rows_kin(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t prow, const gf_table &T,
@@ -230,19 +229,19 @@ GF_D void rows_kin(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t pr
 }
 
 template <int NT>
-GF_D void rows_phot(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t prow, const gf_table &T,
+GF_D void rows_phot(const gf_chain_mods &m, uint32_t row, const gf_table &T)
+{
+	const gf_col &Mr = m.has_mult ? T.absmagSys : T.absmag;
+	mod_photometry<NT>(m.phot, m.phot_comps, row, row + 1, T.Am, T.photoFlags, T.DM, Mr, (int)Mr.width, T.mags,
+	                   T.FeH, T.comp, T.hidden);
+}
+
+GF_D void rows_perr(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t prow, const gf_table &T,
                     const gf_positions &P)
 {
 	if (col<int>(T.hidden, row)) return;
-	if (m.has_phot) {
-		const gf_col &Mr = m.has_mult ? T.absmagSys : T.absmag;
-		mod_photometry<NT>(m.phot, m.phot_comps, row, row + 1, T.Am, T.photoFlags, T.DM, Mr, (int)Mr.width,
-		                   T.mags, T.FeH, T.comp, T.hidden);
-	}
-	if (m.has_perr) {
-		gf_mwc r = restart(a, P, m.pl_perr, prow);
-		perr_row(m.perr, r, row, T.mags, T.obsMags);
-	}
+	gf_mwc r = restart(a, P, m.pl_perr, prow);
+	perr_row(m.perr, r, row, T.mags, T.obsMags);
 }
 
 GF_D void rows_astro(const gf_chain_mods &m, uint32_t row, const gf_table &T)
@@ -251,14 +250,15 @@ GF_D void rows_astro(const gf_chain_mods &m, uint32_t row, const gf_table &T)
 	if (m.gal2other) mod_gal2other(m.gal2other, row, row + 1, T.hidden, T.lb, T.radec);
 }
 
-// GROUP: ROWS_KIN, ROWS_PHOT or ROWS_ASTRO; NT: colour tables of the
+// GROUP: ROWS_KIN, ROWS_PHOT, ROWS_PERR or ROWS_ASTRO; NT: colour tables of the
 // photometry (1..4, at least ceil(ncolors / 4); see phot_tables())
 template <int GROUP, int NT>
 GF_D void chain_rows(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t prow, const gf_table &T,
                      const gf_positions &P)
 {
 	if (GROUP == ROWS_KIN) rows_kin(m, a, row, prow, T, P);
-	else if (GROUP == ROWS_PHOT) rows_phot<NT>(m, a, row, prow, T, P);
+	else if (GROUP == ROWS_PHOT) rows_phot<NT>(m, row, T);
+	else if (GROUP == ROWS_PERR) rows_perr(m, a, row, prow, T, P);
 	else rows_astro(m, row, T);
 }
 

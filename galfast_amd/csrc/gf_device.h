@@ -184,6 +184,54 @@ GF_D void cell_pos(const gf_skygen_params &sp, const gf_extinction &ext, const g
 	z = D * pix.sb;
 }
 
+// Absolute-magnitude bin of a draw: the smallest a in [lo, hi] with
+// P[a + 1] > tgt (hi if there is none) for the nondecreasing FP64 LF prefix
+// P, as a binary search over [lo, hi] finds it.  A guess table G of
+// LF_GUESS entries (G[q] = that bin over the whole range for the q-th
+// equally spaced value of P) gives the start; walking down while P[a] > tgt
+// and up while P[a + 1] <= tgt then ends on the same bin as the binary
+// search from any start, in a step or two instead of log2(nM).
+constexpr int LF_GUESS = 1024;
+
+GF_D void lf_guess_build(const double *P, int nM, uint16_t *G, int tid, int nthr)
+{
+	const double P0 = P[0], Pn = P[nM];
+	for (int q = tid; q < LF_GUESS; q += nthr) {
+		double t = P0 + (Pn - P0) * ((double)q / (double)LF_GUESS);
+		int a = 0, b = nM - 1;
+		while (a < b) {
+			int mid = (a + b) >> 1;
+			if (P[mid + 1] > t) b = mid; else a = mid + 1;
+		}
+		G[q] = (uint16_t)a;
+	}
+}
+
+GF_D double lf_guess_scale(const double *P, int nM)
+{
+	const double r = P[nM] - P[0];
+	return r > 0.0 ? (double)LF_GUESS / r : 0.0;
+}
+
+GF_D int lf_bin(const double *P, const uint16_t *G, double scale, int lo, int hi, double tgt)
+{
+	double qd = (tgt - P[0]) * scale;
+	int q = qd > 0.0 ? (qd < (double)(LF_GUESS - 1) ? (int)qd : LF_GUESS - 1) : 0;
+	int a = G[q];
+	a = a < lo ? lo : (a > hi ? hi : a);
+	while (a > lo && P[a] > tgt) a--;
+	while (a < hi && P[a + 1] <= tgt) a++;
+	return a;
+}
+
+// A star parked by the random half of the emit for the row-parallel half:
+// position in the projection plane, M and m, and the pixel with bit 31 set
+// when the star fell outside the projection (hidden).
+struct gf_parked {
+	float4 *xyMm;
+	uint32_t *pix;
+};
+
 // The random part of one star of draw_stars (skygen/skygen.cu.h:376-385,
 // 404-405): position inside the pixel (2 uniforms) and, when that stays
 // inside the projection, the absolute and apparent magnitude (2 more).

@@ -37,10 +37,14 @@ inline unsigned blocks_for(uint64_t n, unsigned bs) { return (unsigned)((n + bs 
 int exclusive_scan_u32_u64(const uint32_t *in, uint64_t *offs, uint64_t n, hipStream_t st);
 
 struct gf_chain_fwd;
+// ev: GF_CHAIN_LAUNCHES + 1 events (or NULL); ms: GF_CHAIN_LAUNCHES
+// durations (draw pass, then the row pass's groups)
 int module_chain(const gf_rows *ks, const gf_chain *chain, gf_rng *rng, const gf_table *t, hipStream_t st,
                  hipEvent_t *ev, float *ms);
 
 } // namespace gf
+
+constexpr int GF_CHAIN_LAUNCHES = 5;
 
 enum gf_scratch_slot {
 	GF_SLOT_LFTAB = 0,
@@ -50,5 +54,6 @@ enum gf_scratch_slot {
 	GF_SLOT_COMPAT_STATE,
 	GF_SLOT_SAMPLE,
 	GF_SLOT_COUNT_W,
+	GF_SLOT_PARK,
 	GF_SLOT_NSLOTS
 };

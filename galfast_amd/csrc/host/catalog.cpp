@@ -158,7 +158,7 @@ struct gf_catalog {
 	gf_table table{};
 	hipStream_t stream = nullptr;
 	hipEvent_t ev[2] = {nullptr, nullptr};
-	hipEvent_t sev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+	hipEvent_t sev[GF_CHAIN_LAUNCHES + 1] = {};
 
 	~gf_catalog()
 	{
@@ -968,10 +968,10 @@ int gf_catalog_run(gf_catalog *c, gf_chunk_fn fn, void *user, gf_run_stats *stat
 			(void)hipEventRecord(c->ev[0], st);
 			for (uint64_t b0 = 0; b0 < n; b0 += sub) {
 				gf_rows ks{(uint32_t)b0, 256, (uint32_t)std::min(n, b0 + sub), 0};
-				float sms[4];
+				float sms[GF_CHAIN_LAUNCHES];
 				rc = gf::module_chain(&ks, &ch, &c->rng, &c->table, st, c->sev, sms);
 				if (rc) { gf_skygen_plan_free(plan); return rc; }
-				for (int q = 0; q < 4; q++) s.t_stage[q] += sms[q] * 1e-3;
+				for (int q = 0; q < GF_CHAIN_LAUNCHES; q++) s.t_stage[q] += sms[q] * 1e-3;
 				s.n_chain++;
 			}
 			(void)hipEventRecord(c->ev[1], st);
@@ -1097,7 +1097,7 @@ struct SReq {
 
 struct Worker {
 	hipStream_t st = nullptr;
-	DevBuf W, counts, offs, pixsum, segexp, pix, state, segs, segall, reqs, table, pos;
+	DevBuf W, counts, offs, pixsum, segexp, pix, state, segs, segall, reqs, table, pos, park;
 	uint64_t table_cap = 0;
 	gf_table T{};
 	std::thread th;
@@ -1287,11 +1287,14 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	rc = gf::serve_seg_rows((gf_serve_seg *)w.segall.p, nr * nm, (const uint64_t *)w.offs.p, st);
 	if (rc) return rc;
 	gf_skygen_cols sc{w.T.lb, w.T.projIdx, w.T.projXY, w.T.DM, w.T.absmag, w.T.XYZ, w.T.Am, w.T.AmInf, w.T.comp, w.T.hidden};
+	grow<char>(w.park, std::max<size_t>(1, (size_t)nrows * (sizeof(float4) + sizeof(uint32_t))));
+	gf_parked park{(float4 *)w.park.p, (uint32_t *)((char *)w.park.p + (size_t)nrows * sizeof(float4))};
 	for (int m = 0; m < nm; m++) {
-		rc = gf::serve_emit(dsegs + (size_t)m * nr, nr, loff[nr], dpix, (const uint64_t *)w.offs.p, sc, st);
+		rc = gf::serve_emit(dsegs + (size_t)m * nr, nr, loff[nr], s.base.sp.nM, dpix, (const uint64_t *)w.offs.p,
+		                    park, st);
 		if (rc) return rc;
 	}
-	rc = gf::serve_emit_rows((const gf_serve_seg *)w.segall.p, nr * nm, dmodels, dpix, 0, nrows, sc, st);
+	rc = gf::serve_emit_rows((const gf_serve_seg *)w.segall.p, nr * nm, dmodels, dpix, 0, nrows, park, sc, st);
 	if (rc) return rc;
 	std::vector<gf_serve_req> rq(nr);
 	for (int r = 0; r < nr; r++) {

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include "../../include/galfast_hip.h"
 #include "gf_chain.h"
+#include "gf_device.h"
 
 // One (request, model) segment of a batch.
 struct gf_serve_seg {
@@ -41,10 +42,11 @@ int serve_count(const gf_serve_seg *dsegs, int nseg, uint32_t nslots, const gf_m
 int serve_plan(const gf_serve_seg *dsegs, int nseg, uint32_t nlanes, const double *W, uint32_t *counts,
                hipStream_t st);
 int serve_seg_rows(gf_serve_seg *dsegs, int nseg, const uint64_t *offs, hipStream_t st);
-int serve_emit(const gf_serve_seg *dsegs, int nseg, uint32_t nlanes, const gf_pencil_beam *dpix, const uint64_t *offs,
-               const gf_skygen_cols &out, hipStream_t st);
+// park: n rows of parked stars (device scratch of the caller)
+int serve_emit(const gf_serve_seg *dsegs, int nseg, uint32_t nlanes, int nM, const gf_pencil_beam *dpix,
+               const uint64_t *offs, const gf_parked &park, hipStream_t st);
 int serve_emit_rows(const gf_serve_seg *dsegs, int nseg, const gf_model *dmodels, const gf_pencil_beam *dpix,
-                    uint64_t row0, uint64_t row1, const gf_skygen_cols &out, hipStream_t st);
+                    uint64_t row0, uint64_t row1, const gf_parked &park, const gf_skygen_cols &out, hipStream_t st);
 // pos: hch.nplanes * nrows stream positions (device scratch of the caller)
 int serve_chain(const gf_serve_req *dreqs, int nreq, uint32_t nlanes, const gf_serve_chain *dch,
                 const gf_serve_chain &hch, uint64_t nrows, const gf_table &T, uint2 *pos, hipStream_t st);

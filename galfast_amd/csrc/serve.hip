@@ -107,8 +107,18 @@ __global__ void __launch_bounds__(SB) k_b_plan(const gf_serve_seg *segs, int nse
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(SB) k_b_emit_rng(const gf_serve_seg *segs, int nseg, uint32_t nlanes,
                                                    const gf_pencil_beam *pixels, const uint64_t *offs,
-                                                   gf_skygen_cols out)
+                                                   gf_parked park)
 {
+	// every segment of a launch is of one model: its LF prefix and the bin
+	// guess table go to LDS
+	extern __shared__ double P[];
+	const int nM = segs[0].sp.nM;
+	uint16_t *G = (uint16_t *)(P + nM + 1);
+	for (int i = threadIdx.x; i <= nM; i += blockDim.x) P[i] = segs[0].lfP[i];
+	__syncthreads();
+	lf_guess_build(P, nM, G, threadIdx.x, blockDim.x);
+	__syncthreads();
+	const double gscale = lf_guess_scale(P, nM);
 	uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
 	if (g >= nlanes) return;
 	int si = upper_index(segs, nseg, [](const gf_serve_seg &s) { return (uint64_t)s.lane0; }, g);
@@ -117,7 +127,6 @@ __global__ void __launch_bounds__(SB) k_b_emit_rng(const gf_serve_seg *segs, int
 	if (s >= sg.S) return;
 	const gf_skygen_params &sp = sg.sp;
 	const int K = sp.nm + sp.nM - 1;
-	const double *P = sg.lfP;
 	gf_extinction ext{};
 	gf_mwc rng = gf_mwc_load(sg.state, sg.S, s);
 	for (uint64_t u = s; u < sg.nunits; u += sg.S) {
@@ -133,21 +142,12 @@ __global__ void __launch_bounds__(SB) k_b_emit_rng(const gf_serve_seg *segs, int
 		double base = P[lo], span = P[hi + 1] - P[lo];
 		for (uint64_t j = o0; j < o1; j++) {
 			double tgt = base + (double)rng.uniform() * span;
-			int a = lo, b = hi;
-			while (a < b) {
-				int mid = (a + b) >> 1;
-				if (P[mid + 1] > tgt) b = mid; else a = mid + 1;
-			}
+			int a = lf_bin(P, G, gscale, lo, hi, tgt);
 			float M = sp.M1 - a * sp.dM;
 			float sx, sy, Mtmp = 0.f, mtmp = 0.f;
 			bool in = star_draws(sp, pix, M, k - a, rng, sx, sy, Mtmp, mtmp);
-			uint32_t row = (uint32_t)j;
-			col<float>(out.projXY, row, 0) = sx;
-			col<float>(out.projXY, row, 1) = sy;
-			col<float>(out.M, row) = Mtmp;
-			col<float>(out.DM, row) = mtmp;
-			col<int>(out.comp, row) = (int)(sg.pix0 + p);
-			col<int>(out.hidden, row) = in ? 0 : 1;
+			park.xyMm[j] = make_float4(sx, sy, Mtmp, mtmp);
+			park.pix[j] = (sg.pix0 + (uint32_t)p) | (in ? 0u : 0x80000000u);
 		}
 	}
 	gf_mwc_store(sg.state, sg.S, s, rng);
@@ -156,23 +156,23 @@ __global__ void __launch_bounds__(SB) k_b_emit_rng(const gf_serve_seg *segs, int
 // Emit, deterministic half: one lane per batch row.
 __global__ void __launch_bounds__(SB) k_b_emit_rows(const gf_serve_seg *segs, int nseg, const gf_model *models,
                                                     const gf_pencil_beam *pixels, uint64_t row0, uint64_t row1,
-                                                    gf_skygen_cols out)
+                                                    gf_parked park, gf_skygen_cols out)
 {
 	uint64_t row = row0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (row >= row1) return;
 	int si = upper_index(segs, nseg, [](const gf_serve_seg &s) { return s.row0; }, row);
 	const gf_serve_seg &sg = segs[si];
 	uint32_t r = (uint32_t)row;
-	int gp = col<int>(out.comp, r);
-	const gf_pencil_beam pix = pixels[gp];
+	const float4 v = park.xyMm[row];
+	const uint32_t ph = park.pix[row];
+	const gf_pencil_beam pix = pixels[ph & 0x7fffffffu];
 	int comp = models[sg.model].comp;
-	float x = col<float>(out.projXY, r, 0), y = col<float>(out.projXY, r, 1);
-	if (col<int>(out.hidden, r)) {
-		write_star_outside(out, r, pix, comp, x, y);
+	if (ph & 0x80000000u) {
+		write_star_outside(out, r, pix, comp, v.x, v.y);
 		return;
 	}
 	gf_extinction ext{};
-	write_star(sg.sp, comp, ext, sg.foot, pix, x, y, col<float>(out.M, r), col<float>(out.DM, r), 0.f, out, r);
+	write_star(sg.sp, comp, ext, sg.foot, pix, v.x, v.y, v.z, v.w, 0.f, out, r);
 }
 
 // Row starts of the segments from the scanned offsets.
@@ -257,20 +257,22 @@ int serve_seg_rows(gf_serve_seg *dsegs, int nseg, const uint64_t *offs, hipStrea
 	return 0;
 }
 
-int serve_emit(const gf_serve_seg *dsegs, int nseg, uint32_t nlanes, const gf_pencil_beam *dpix, const uint64_t *offs,
-               const gf_skygen_cols &out, hipStream_t st)
+int serve_emit(const gf_serve_seg *dsegs, int nseg, uint32_t nlanes, int nM, const gf_pencil_beam *dpix,
+               const uint64_t *offs, const gf_parked &park, hipStream_t st)
 {
 	if (!nlanes) return 0;
-	k_b_emit_rng<<<gf::blocks_for(nlanes, SB), SB, 0, st>>>(dsegs, nseg, nlanes, dpix, offs, out);
+	const size_t shb = ((size_t)nM + 1) * sizeof(double) + LF_GUESS * sizeof(uint16_t);
+	k_b_emit_rng<<<gf::blocks_for(nlanes, SB), SB, shb, st>>>(dsegs, nseg, nlanes, dpix, offs, park);
 	GF_LAUNCHED();
 	return 0;
 }
 
 int serve_emit_rows(const gf_serve_seg *dsegs, int nseg, const gf_model *dmodels, const gf_pencil_beam *dpix,
-                    uint64_t row0, uint64_t row1, const gf_skygen_cols &out, hipStream_t st)
+                    uint64_t row0, uint64_t row1, const gf_parked &park, const gf_skygen_cols &out, hipStream_t st)
 {
 	if (row1 <= row0) return 0;
-	k_b_emit_rows<<<gf::blocks_for(row1 - row0, SB), SB, 0, st>>>(dsegs, nseg, dmodels, dpix, row0, row1, out);
+	k_b_emit_rows<<<gf::blocks_for(row1 - row0, SB), SB, 0, st>>>(dsegs, nseg, dmodels, dpix, row0, row1, park,
+	                                                               out);
 	GF_LAUNCHED();
 	return 0;
 }
@@ -290,13 +292,16 @@ int serve_chain(const gf_serve_req *dreqs, int nreq, uint32_t nlanes, const gf_s
 		GF_LAUNCHED();
 	}
 	if (gfc::rows_on(hch, gfc::ROWS_PHOT)) {
-		const int ns = (int)gfc::rows_stream(hch, gfc::ROWS_PHOT);
 		switch (gfc::phot_tables(hch)) {
-		case 1: k_b_rows<gfc::ROWS_PHOT, 1><<<g, gfc::BS, 0, st>>>(dreqs, nreq, dch, nrows, T, P, ns); break;
-		case 2: k_b_rows<gfc::ROWS_PHOT, 2><<<g, gfc::BS, 0, st>>>(dreqs, nreq, dch, nrows, T, P, ns); break;
-		case 3: k_b_rows<gfc::ROWS_PHOT, 3><<<g, gfc::BS, 0, st>>>(dreqs, nreq, dch, nrows, T, P, ns); break;
-		default: k_b_rows<gfc::ROWS_PHOT, 4><<<g, gfc::BS, 0, st>>>(dreqs, nreq, dch, nrows, T, P, ns); break;
+		case 1: k_b_rows<gfc::ROWS_PHOT, 1><<<g, gfc::BS, 0, st>>>(dreqs, nreq, dch, nrows, T, P, 0); break;
+		case 2: k_b_rows<gfc::ROWS_PHOT, 2><<<g, gfc::BS, 0, st>>>(dreqs, nreq, dch, nrows, T, P, 0); break;
+		case 3: k_b_rows<gfc::ROWS_PHOT, 3><<<g, gfc::BS, 0, st>>>(dreqs, nreq, dch, nrows, T, P, 0); break;
+		default: k_b_rows<gfc::ROWS_PHOT, 4><<<g, gfc::BS, 0, st>>>(dreqs, nreq, dch, nrows, T, P, 0); break;
 		}
+		GF_LAUNCHED();
+	}
+	if (gfc::rows_on(hch, gfc::ROWS_PERR)) {
+		k_b_rows<gfc::ROWS_PERR, 1><<<g, gfc::BS, 0, st>>>(dreqs, nreq, dch, nrows, T, P, 1);
 		GF_LAUNCHED();
 	}
 	if (gfc::rows_on(hch, gfc::ROWS_ASTRO)) {

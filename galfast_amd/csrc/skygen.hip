@@ -427,11 +427,15 @@ __global__ void __launch_bounds__(SK_BS) k_emit(gf_skygen_params sp, gf_model m,
 __global__ void __launch_bounds__(SK_BS) k_emit_rng(gf_skygen_params sp, gf_model m, const gf_pencil_beam *pixels,
                                                     gf_extinction ext, const double *Pg, const uint64_t *offs,
                                                     uint64_t U, uint64_t row_begin, uint64_t row_end, uint64_t u_lo,
-                                                    uint32_t *st, uint32_t ns, gf_skygen_cols out)
+                                                    uint32_t *st, uint32_t ns, gf_parked park)
 {
 	extern __shared__ double P[];
+	uint16_t *G = (uint16_t *)(P + sp.nM + 1);
 	for (int i = threadIdx.x; i <= sp.nM; i += blockDim.x) P[i] = Pg[i];
 	__syncthreads();
+	lf_guess_build(P, sp.nM, G, threadIdx.x, blockDim.x);
+	__syncthreads();
+	const double gscale = lf_guess_scale(P, sp.nM);
 	uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
 	if (s >= ns) return;
 	const int K = sp.nm + sp.nM - 1;
@@ -454,22 +458,13 @@ __global__ void __launch_bounds__(SK_BS) k_emit_rng(gf_skygen_params sp, gf_mode
 		uint64_t j1 = row_end < o1 ? row_end : o1;
 		for (uint64_t j = j0; j < j1; j++) {
 			double tgt = base + (double)rng.uniform() * span;
-			int a = lo, b = hi;
-			while (a < b) {
-				int mid = (a + b) >> 1;
-				if (P[mid + 1] > tgt) b = mid; else a = mid + 1;
-			}
-			int iM = a;
+			int iM = lf_bin(P, G, gscale, lo, hi, tgt);
 			float M = sp.M1 - iM * sp.dM;
 			float sx, sy, Mtmp = 0.f, mtmp = 0.f;
 			bool in = star_draws(sp, pix, M, k - iM, rng, sx, sy, Mtmp, mtmp);
-			uint32_t row = (uint32_t)(j - row_begin);
-			col<float>(out.projXY, row, 0) = sx;
-			col<float>(out.projXY, row, 1) = sy;
-			col<float>(out.M, row) = Mtmp;
-			col<float>(out.DM, row) = mtmp;
-			col<int>(out.comp, row) = p;
-			col<int>(out.hidden, row) = in ? 0 : 1;
+			uint64_t row = j - row_begin;
+			park.xyMm[row] = make_float4(sx, sy, Mtmp, mtmp);
+			park.pix[row] = (uint32_t)p | (in ? 0u : 0x80000000u);
 		}
 	}
 	if (touched) gf_mwc_store(st, ns, s, rng);
@@ -478,19 +473,18 @@ __global__ void __launch_bounds__(SK_BS) k_emit_rng(gf_skygen_params sp, gf_mode
 // Deterministic half of the emit, one lane per row.
 __global__ void __launch_bounds__(SK_BS) k_emit_rows(gf_skygen_params sp, int comp, const gf_pencil_beam *pixels,
                                                      gf_extinction ext, gf_footprint foot, uint32_t nrows,
-                                                     gf_skygen_cols out)
+                                                     gf_parked park, gf_skygen_cols out)
 {
 	uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
 	if (row >= nrows) return;
-	int p = col<int>(out.comp, row);
-	const gf_pencil_beam pix = pixels[p];
-	float x = col<float>(out.projXY, row, 0), y = col<float>(out.projXY, row, 1);
-	if (col<int>(out.hidden, row)) {
-		write_star_outside(out, row, pix, comp, x, y);
+	const float4 v = park.xyMm[row];
+	const uint32_t ph = park.pix[row];
+	const gf_pencil_beam pix = pixels[ph & 0x7fffffffu];
+	if (ph & 0x80000000u) {
+		write_star_outside(out, row, pix, comp, v.x, v.y);
 		return;
 	}
-	float Mtmp = col<float>(out.M, row), mtmp = col<float>(out.DM, row);
-	write_star(sp, comp, ext, foot, pix, x, y, Mtmp, mtmp, 0.f, out, row);
+	write_star(sp, comp, ext, foot, pix, v.x, v.y, v.z, v.w, 0.f, out, row);
 }
 
 __global__ void k_find_unit(const uint64_t *offs, uint64_t U, uint64_t row, uint64_t *res)
@@ -654,6 +648,7 @@ int gf_skygen_emit(const gf_draw_plan *plan, uint64_t row_begin, uint64_t row_en
 	GF_HIP(hipMemcpyAsync(&u_lo, plan->tmp, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
 	GF_HIP(hipStreamSynchronize(st));
 	size_t shb = ((size_t)plan->sp.nM + 1) * sizeof(double);
+	const size_t shb_guess = shb + LF_GUESS * sizeof(uint16_t);
 	if (plan->ext.beam.data || plan->ext.north.data || plan->ext.south.data) {
 		// with extinction the star needs its DM bin's minimum extinction: one pass
 		k_emit<<<gf::blocks_for(rng->nstreams, SK_BS), SK_BS, shb, st>>>(
@@ -662,14 +657,18 @@ int gf_skygen_emit(const gf_draw_plan *plan, uint64_t row_begin, uint64_t row_en
 		GF_LAUNCHED();
 		return 0;
 	}
-	k_emit_rng<<<gf::blocks_for(rng->nstreams, SK_BS), SK_BS, shb, st>>>(
-	    plan->sp, plan->model, plan->pixels, plan->ext, plan->P, plan->offs, plan->U, row_begin, row_end, u_lo,
-	    rng->state, rng->nstreams, *out);
-	GF_LAUNCHED();
 	uint64_t n = row_end - row_begin;
 	GF_CHECK(n < (1ull << 32), GF_E_ARG);
+	void *pk;
+	int rc = gf::scratch(GF_SLOT_PARK, (size_t)n * (sizeof(float4) + sizeof(uint32_t)), &pk);
+	if (rc) return rc;
+	gf_parked park{(float4 *)pk, (uint32_t *)((char *)pk + (size_t)n * sizeof(float4))};
+	k_emit_rng<<<gf::blocks_for(rng->nstreams, SK_BS), SK_BS, shb_guess, st>>>(
+	    plan->sp, plan->model, plan->pixels, plan->ext, plan->P, plan->offs, plan->U, row_begin, row_end, u_lo,
+	    rng->state, rng->nstreams, park);
+	GF_LAUNCHED();
 	k_emit_rows<<<gf::blocks_for(n, SK_BS), SK_BS, 0, st>>>(plan->sp, plan->model.comp, plan->pixels, plan->ext, fp,
-	                                                          (uint32_t)n, *out);
+	                                                          (uint32_t)n, park, *out);
 	GF_LAUNCHED();
 	return 0;
 }

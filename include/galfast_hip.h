@@ -469,8 +469,9 @@ typedef struct gf_run_stats {
     uint64_t n_emit, n_chain;
     double   t_plan, t_emit, t_chain, t_total;
     uint64_t visible;       /* rows with hidden == 0 */
-    double   t_stage[4];    /* module chain stages: RNG modules before photometry,
-                               photometry, photometric errors, vel2pm + gal2other */
+    double   t_stage[6];    /* module chain launches: draw pass; row pass for
+                               FeH/GaussianFeH/Bond2010, photometry, photometric
+                               errors, vel2pm + gal2other; (spare) */
 } gf_run_stats;
 
 /* Called once per output chunk with device columns holding `nrows` rows

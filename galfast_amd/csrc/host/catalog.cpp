@@ -268,6 +268,18 @@ std::vector<gf_pencil_beam> pixelize_footprint(const gf_footprint &fp, double dx
 			j0 = std::max(0, (int)floor((ymin - y0) / dx) - 2);
 			j1 = std::min(n, (int)floor((ymax - y0) / dx) + 3);
 		}
+		// a beam's membership test straight from the Lambert-plane point:
+		// the unit vector of (x, y) is (-x s, -y s, 1 - r^2/2) in the north
+		// and (x s, -y s, r^2/2 - 1) in the south, s = sqrt(1 - r^2/4)
+		const double bc[3] = {fp.cb * fp.cl, fp.cb * fp.sl, fp.sb};
+		auto in_beam = [&](double x, double y) {
+			double r2 = x * x + y * y;
+			if (r2 > 2.0) return false;
+			double s = sqrt(std::max(0.0, 1.0 - 0.25 * r2)), z = 1.0 - 0.5 * r2;
+			double v0 = hemi == 0 ? -x * s : x * s, v1 = -y * s, v2 = hemi == 0 ? z : -z;
+			if ((hemi == 0 && v2 < 0) || (hemi == 1 && v2 >= 0)) return false;
+			return v0 * bc[0] + v1 * bc[1] + v2 * bc[2] >= fp.cosr;
+		};
 		for (int i = i0; i < i1; i++) {
 			for (int j = j0; j < j1; j++) {
 				double xa = x0 + i * dx, ya = y0 + j * dx;
@@ -275,15 +287,24 @@ std::vector<gf_pencil_beam> pixelize_footprint(const gf_footprint &fp, double dx
 				double fy = (ya > 0 || ya + dx < 0) ? std::min(fabs(ya), fabs(ya + dx)) : 0.;
 				if (fx * fx + fy * fy > 2.0) continue;
 				int inside = 0;
-				for (int a = 0; a < sub; a++)
-					for (int b2 = 0; b2 < sub; b2++) {
-						double x = xa + dx * (a + 0.5) / sub, y = ya + dx * (b2 + 0.5) / sub;
-						if (x * x + y * y > 2.0) continue;
-						double l, b;
-						deproj_pole(hemi, x, y, l, b);
-						if ((hemi == 0 && b < 0) || (hemi == 1 && b >= 0)) continue;
-						if (in_footprint(fp, l, b)) inside++;
-					}
+				if (fp.kind == 2 && in_beam(xa, ya) && in_beam(xa + dx, ya) && in_beam(xa, ya + dx) &&
+				    in_beam(xa + dx, ya + dx)) {
+					inside = sub * sub;   // a cell with all four corners in the cap is covered
+				} else {
+					for (int a = 0; a < sub; a++)
+						for (int b2 = 0; b2 < sub; b2++) {
+							double x = xa + dx * (a + 0.5) / sub, y = ya + dx * (b2 + 0.5) / sub;
+							if (x * x + y * y > 2.0) continue;
+							if (fp.kind == 2) {
+								if (in_beam(x, y)) inside++;
+								continue;
+							}
+							double l, b;
+							deproj_pole(hemi, x, y, l, b);
+							if ((hemi == 0 && b < 0) || (hemi == 1 && b >= 0)) continue;
+							if (in_footprint(fp, l, b)) inside++;
+						}
+				}
 				if (!inside) continue;
 				double xc = xa + 0.5 * dx, yc = ya + 0.5 * dx;
 				double l, b;
@@ -1097,7 +1118,7 @@ struct SReq {
 
 struct Worker {
 	hipStream_t st = nullptr;
-	DevBuf W, counts, offs, pixsum, segexp, pix, state, segs, segall, reqs, table, pos, park;
+	DevBuf W, counts, offs, pixsum, segexp, pix, state, segs, segall, reqs, table, pos, park, vis;
 	uint64_t table_cap = 0;
 	gf_table T{};
 	std::thread th;
@@ -1120,6 +1141,7 @@ struct gf_server {
 	std::condition_variable cv_sub, cv_ready, cv_idle;
 	std::deque<std::unique_ptr<SReq>> submitted, ready;
 	uint64_t n_submitted = 0, n_completed = 0, n_batches = 0, n_rows = 0, n_visible = 0;
+	double t_prep = 0, t_batch = 0;
 	double t_first = 0, t_last = 0;
 	bool stop = false;
 	std::vector<std::thread> prep;
@@ -1309,18 +1331,19 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	rc = gf::serve_chain((const gf_serve_req *)w.reqs.p, nr, loff[nr], (const gf_serve_chain *)s.dchain.p, s.hchain,
 	                     nrows, w.T, (uint2 *)w.pos.p, st);
 	if (rc) return rc;
+	uint64_t maxn = 0;
+	for (int r = 0; r < nr; r++) maxn = std::max(maxn, roff[r + 1] - roff[r]);
+	grow<unsigned long long>(w.vis, (size_t)nr + 1);
+	rc = gf::serve_visible((const gf_serve_req *)w.reqs.p, nr, maxn, w.T.hidden, (unsigned long long *)w.vis.p, st);
+	if (rc) return rc;
+	std::vector<unsigned long long> visv(nr);
+	GF_HIP(hipMemcpyAsync(visv.data(), w.vis.p, (size_t)nr * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
 	GF_HIP(hipStreamSynchronize(st));
 	// results
 	uint64_t vis_total = 0;
 	for (int r = 0; r < nr; r++) {
 		SReq &q = *batch[r];
-		uint64_t n = roff[r + 1] - roff[r], vis = 0;
-		if (n) {
-			gf_col h = w.T.hidden;
-			h.ptr = (char *)h.ptr + roff[r] * 4;
-			rc = gf_count_visible(h, n, &vis, st);
-			if (rc) return rc;
-		}
+		uint64_t n = roff[r + 1] - roff[r], vis = visv[r];
 		vis_total += vis;
 		gf_result res{};
 		res.id = q.r.id;
@@ -1382,9 +1405,12 @@ void prep_loop(gf_server &s)
 			q = std::move(s.submitted.front());
 			s.submitted.pop_front();
 		}
+		double t0 = server_clock();
 		prepare(s, *q);
+		double t1 = server_clock();
 		{
 			std::lock_guard<std::mutex> lk(s.mu);
+			s.t_prep += t1 - t0;
 			s.ready.push_back(std::move(q));
 		}
 		s.cv_ready.notify_all();
@@ -1427,8 +1453,14 @@ void worker_loop(gf_server &s, Worker &w)
 			if (s.cfg.plain) prepare(s, *q);
 			q->t_start = t0;
 		}
+		double t1 = server_clock();
 		int rc = run_batch(s, w, batch);
 		if (rc) fail_batch(s, batch, rc);
+		{
+			std::lock_guard<std::mutex> lk(s.mu);
+			s.t_batch += server_clock() - t1;
+			if (s.cfg.plain) s.t_prep += t1 - t0;
+		}
 	}
 }
 
@@ -1503,7 +1535,7 @@ int gf_server_create(const char *cmd_conf, const char *datadir, const gf_server_
 		s->workers.push_back(std::move(w));
 	}
 	if (!s->cfg.plain)
-		for (int i = 0; i < 4; i++) s->prep.emplace_back(prep_loop, std::ref(*sp));
+		for (int i = 0; i < 8; i++) s->prep.emplace_back(prep_loop, std::ref(*sp));
 	for (auto &w : s->workers) {
 		Worker *wp = w.get();
 		w->th = std::thread([sp, wp] { worker_loop(*sp, *wp); });
@@ -1548,6 +1580,8 @@ int gf_server_stats_get(gf_server *s, gf_server_stats *st)
 	st->visible = s->n_visible;
 	st->t_first_submit = s->t_first;
 	st->t_last_done = s->t_last;
+	st->t_prep = s->t_prep;
+	st->t_batch = s->t_batch;
 	return 0;
 }
 

@@ -50,6 +50,9 @@ int serve_emit_rows(const gf_serve_seg *dsegs, int nseg, const gf_model *dmodels
 // pos: hch.nplanes * nrows stream positions (device scratch of the caller)
 int serve_chain(const gf_serve_req *dreqs, int nreq, uint32_t nlanes, const gf_serve_chain *dch,
                 const gf_serve_chain &hch, uint64_t nrows, const gf_table &T, uint2 *pos, hipStream_t st);
+// vis[r] = rows of request r with hidden == 0 (max_rows: the largest request)
+int serve_visible(const gf_serve_req *dreqs, int nreq, uint64_t max_rows, const gf_col &hidden,
+                  unsigned long long *vis, hipStream_t st);
 // FP64 exclusive prefix of a model's LF at the absolute-magnitude bin centres
 // (nM + 1 doubles) into dst (device).
 int lf_prefix(const gf_skygen_params &sp, const gf_model &m, double *dst, hipStream_t st);

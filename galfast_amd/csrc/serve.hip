@@ -225,9 +225,48 @@ __global__ void __launch_bounds__(gfc::BS) k_b_rows(const gf_serve_req *reqs, in
 	gfc::chain_rows<GROUP, NT>(*ch, a, (uint32_t)row, row, T, P);
 }
 
+// Visible rows of each request: a workgroup per (request, 64K-row slice).
+__global__ void __launch_bounds__(SB) k_b_visible(const gf_serve_req *reqs, gf_col hidden,
+                                                  unsigned long long *vis)
+{
+	const gf_serve_req &rq = reqs[blockIdx.y];
+	uint64_t b0 = rq.row0 + (uint64_t)blockIdx.x * 65536;
+	uint64_t b1 = std::min<uint64_t>(rq.row1, b0 + 65536);
+	unsigned long long c = 0;
+	for (uint64_t r = b0 + threadIdx.x; r < b1; r += SB) c += col<int>(hidden, (uint32_t)r) == 0 ? 1ull : 0ull;
+	__shared__ unsigned long long sh[SB];
+	sh[threadIdx.x] = c;
+	__syncthreads();
+	for (int o = SB / 2; o > 0; o >>= 1) {
+		if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0 && b0 < b1) atomicAdd(&vis[blockIdx.y], sh[0]);
+}
+
+__global__ void k_b_zero(unsigned long long *v, int n)
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) v[i] = 0;
+}
+
 } // namespace
 
 namespace gf {
+
+int serve_visible(const gf_serve_req *dreqs, int nreq, uint64_t max_rows, const gf_col &hidden,
+                  unsigned long long *vis, hipStream_t st)
+{
+	if (nreq == 0) return 0;
+	k_b_zero<<<gf::blocks_for(nreq, 64), 64, 0, st>>>(vis, nreq);
+	GF_LAUNCHED();
+	if (max_rows == 0) return 0;
+	dim3 grid((unsigned)((max_rows + 65535) / 65536), (unsigned)nreq);
+	k_b_visible<<<grid, SB, 0, st>>>(dreqs, hidden, vis);
+	GF_LAUNCHED();
+	return 0;
+}
+
 
 int serve_count(const gf_serve_seg *dsegs, int nseg, uint32_t nslots, const gf_model *dmodels,
                 const gf_pencil_beam *dpix, double *W, double *pix_sum, double *seg_expected, hipStream_t st)

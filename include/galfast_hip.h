@@ -557,6 +557,8 @@ typedef struct gf_server_config {
 typedef struct gf_server_stats {
     uint64_t submitted, completed, batches, rows, visible;
     double   t_first_submit, t_last_done;
+    double   t_prep;    /* host seconds spent preparing requests (all threads) */
+    double   t_batch;   /* seconds workers spent running batches (all workers) */
 } gf_server_stats;
 
 typedef struct gf_server gf_server;

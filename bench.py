@@ -35,7 +35,11 @@ PEAK_HBM_GBS = 8000.0
 # Algorithmic bytes per row of each kernel (stated in DESIGN.md): every
 # column a kernel must read, and every column it writes, once.  Hidden rows
 # cost the 4-byte hidden flag in the module kernels.
-EMIT_BYTES_PER_ROW = 64                      # the 10 skygen columns
+# emit, per generated row: the random half writes the 20-byte parked star
+# (x, y, M, m, pixel|hidden); the row half reads it and writes the 10
+# skygen columns (64 B)
+EMIT_RNG_BYTES_PER_ROW = 20
+EMIT_ROWS_BYTES_PER_ROW = 20 + 64
 # (bytes per visible row, bytes per hidden row) of the module-chain launches
 # for the config-1 module set (FeH, Bond2010, photometry with 5 bands,
 # photometric errors, vel2pm, gal2other): every column a launch must read,
@@ -227,7 +231,8 @@ def main():
     G.lib().gf_sync(None)
     t0 = time.perf_counter()
     acc = {"visible": 0, "generated": 0, "t_count": 0.0, "t_plan": 0.0, "t_emit": 0.0, "t_chain": 0.0,
-           "n_emit": 0, "n_chain": 0, "expected": 0.0, "t_stage": [0.0] * len(STAGE_NAMES)}
+           "n_emit": 0, "n_chain": 0, "expected": 0.0, "t_stage": [0.0] * len(STAGE_NAMES),
+           "t_emit_rng": 0.0, "t_emit_rows": 0.0}
     for _ in range(args.steps):
         expected, tc, st = step()
         acc["visible"] += st.visible
@@ -235,6 +240,8 @@ def main():
         acc["t_count"] += tc
         acc["t_plan"] += st.t_plan
         acc["t_emit"] += st.t_emit
+        acc["t_emit_rng"] += st.t_emit_rng
+        acc["t_emit_rows"] += st.t_emit_rows
         acc["t_chain"] += st.t_chain
         acc["n_emit"] += st.n_emit
         acc["n_chain"] += st.n_chain
@@ -248,13 +255,23 @@ def main():
     # roofline of the dominant kernel, from HIP-event durations recorded on
     # the catalog's stream around each launch
     hidden = acc["generated"] - acc["visible"]
-    kernels = {"k_emit": (acc["t_emit"], acc["n_emit"], EMIT_BYTES_PER_ROW * acc["generated"])}
+    kernels = {"k_emit_rng": (acc["t_emit_rng"], acc["n_emit"], EMIT_RNG_BYTES_PER_ROW * acc["generated"]),
+               "k_emit_rows": (acc["t_emit_rows"], acc["n_emit"], EMIT_ROWS_BYTES_PER_ROW * acc["generated"])}
     for q, name in enumerate(STAGE_NAMES):
         if acc["t_stage"][q] > 0:
             bv, bh = STAGE_BYTES[name]
             kernels[name] = (acc["t_stage"][q], acc["n_chain"], bv * acc["visible"] + bh * hidden)
     name = max(kernels, key=lambda k: kernels[k][0])
     t, n, bytes_total = kernels[name]
+    # HBM bytes per launch of that kernel from the committed rocprofv3
+    # counter runs of this benchmark (profiles/traffic.json), if measured
+    traffic = None
+    tj = os.path.join(ROOT, "profiles", "traffic.json")
+    if args.workload == "c1" and os.path.exists(tj):
+        with open(tj) as f:
+            tr = json.load(f)
+        if name in tr:
+            traffic = tr[name]["bytes"]
     per_launch_s = t / max(n, 1)
     achieved = bytes_total / max(n, 1) / per_launch_s / 1e9 if per_launch_s > 0 else 0.0
     out = {
@@ -277,9 +294,12 @@ def main():
                    "expected_per_step": acc["expected"] / args.steps},
         "breakdown_ms_per_step": dict(
             [(k, 1000.0 * acc[k] / args.steps) for k in ("t_count", "t_plan", "t_emit", "t_chain")] +
+            [("k_emit_rng", 1000.0 * acc["t_emit_rng"] / args.steps),
+             ("k_emit_rows", 1000.0 * acc["t_emit_rows"] / args.steps)] +
             [(STAGE_NAMES[q], 1000.0 * acc["t_stage"][q] / args.steps) for q in range(len(STAGE_NAMES))]),
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None},
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": bytes_total / max(n, 1)},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cat, args.workload, args.cpu_budget)

@@ -26,7 +26,8 @@ class gf_run_stats(C.Structure):
     _fields_ = [("expected", C.c_double), ("generated", C.c_uint64), ("rows_chain", C.c_uint64),
                 ("n_emit", C.c_uint64), ("n_chain", C.c_uint64), ("t_plan", C.c_double),
                 ("t_emit", C.c_double), ("t_chain", C.c_double), ("t_total", C.c_double),
-                ("visible", C.c_uint64), ("t_stage", C.c_double * 6)]
+                ("visible", C.c_uint64), ("t_stage", C.c_double * 6),
+                ("t_emit_rng", C.c_double), ("t_emit_rows", C.c_double)]
 
 
 def _declare():

@@ -37,6 +37,12 @@ inline unsigned blocks_for(uint64_t n, unsigned bs) { return (unsigned)((n + bs 
 int exclusive_scan_u32_u64(const uint32_t *in, uint64_t *offs, uint64_t n, hipStream_t st);
 
 struct gf_chain_fwd;
+// gf_skygen_emit with optional timing: ev holds 3 events, ms[0] and ms[1]
+// receive the durations of the random half (k_emit_rng) and the row half
+// (k_emit_rows); both 0 on the extinction path (one fused kernel).
+int skygen_emit(const gf_draw_plan *plan, uint64_t row_begin, uint64_t row_end, const gf_footprint *foot,
+                gf_rng *rng, const gf_skygen_cols *out, hipStream_t st, hipEvent_t *ev, float *ms);
+
 // ev: GF_CHAIN_LAUNCHES + 1 events (or NULL); ms: GF_CHAIN_LAUNCHES
 // durations (draw pass, then the row pass's groups)
 int module_chain(const gf_rows *ks, const gf_chain *chain, gf_rng *rng, const gf_table *t, hipStream_t st,

@@ -978,8 +978,11 @@ int gf_catalog_run(gf_catalog *c, gf_chunk_fn fn, void *user, gf_run_stats *stat
 		for (uint64_t r0 = 0; r0 < total; r0 += chunk) {
 			uint64_t r1 = std::min(total, r0 + chunk);
 			(void)hipEventRecord(c->ev[0], st);
-			rc = gf_skygen_emit(plan, r0, r1, &c->foot.f, &c->rng, &sc, st);
+			float ems[2];
+			rc = gf::skygen_emit(plan, r0, r1, &c->foot.f, &c->rng, &sc, st, c->sev, ems);
 			if (rc) { gf_skygen_plan_free(plan); return rc; }
+			s.t_emit_rng += ems[0] * 1e-3;
+			s.t_emit_rows += ems[1] * 1e-3;
 			(void)hipEventRecord(c->ev[1], st);
 			(void)hipEventSynchronize(c->ev[1]);
 			(void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);

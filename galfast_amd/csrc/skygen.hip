@@ -637,9 +637,19 @@ int gf_skygen_emit(const gf_draw_plan *plan, uint64_t row_begin, uint64_t row_en
                    const gf_footprint *foot, gf_rng *rng,
                    const gf_skygen_cols *out, void *stream)
 {
+	return gf::skygen_emit(plan, row_begin, row_end, foot, rng, out, gf::as_stream(stream), nullptr, nullptr);
+}
+
+} // extern "C"
+
+namespace gf {
+
+int skygen_emit(const gf_draw_plan *plan, uint64_t row_begin, uint64_t row_end, const gf_footprint *foot,
+                gf_rng *rng, const gf_skygen_cols *out, hipStream_t st, hipEvent_t *ev, float *ms)
+{
 	GF_CHECK(plan && rng && rng->state && out && row_begin <= row_end && row_end <= plan->total, GF_E_ARG);
+	if (ms) ms[0] = ms[1] = 0.f;
 	if (row_begin == row_end) return 0;
-	hipStream_t st = gf::as_stream(stream);
 	gf_footprint fp{};
 	if (foot) fp = *foot;
 	k_find_unit<<<1, 1, 0, st>>>(plan->offs, plan->U, row_begin, plan->tmp);
@@ -663,15 +673,29 @@ int gf_skygen_emit(const gf_draw_plan *plan, uint64_t row_begin, uint64_t row_en
 	int rc = gf::scratch(GF_SLOT_PARK, (size_t)n * (sizeof(float4) + sizeof(uint32_t)), &pk);
 	if (rc) return rc;
 	gf_parked park{(float4 *)pk, (uint32_t *)((char *)pk + (size_t)n * sizeof(float4))};
+	if (ev) GF_HIP(hipEventRecord(ev[0], st));
 	k_emit_rng<<<gf::blocks_for(rng->nstreams, SK_BS), SK_BS, shb_guess, st>>>(
 	    plan->sp, plan->model, plan->pixels, plan->ext, plan->P, plan->offs, plan->U, row_begin, row_end, u_lo,
 	    rng->state, rng->nstreams, park);
 	GF_LAUNCHED();
+	if (ev) GF_HIP(hipEventRecord(ev[1], st));
 	k_emit_rows<<<gf::blocks_for(n, SK_BS), SK_BS, 0, st>>>(plan->sp, plan->model.comp, plan->pixels, plan->ext, fp,
 	                                                          (uint32_t)n, park, *out);
 	GF_LAUNCHED();
+	if (ev) {
+		GF_HIP(hipEventRecord(ev[2], st));
+		if (ms) {
+			GF_HIP(hipEventSynchronize(ev[2]));
+			GF_HIP(hipEventElapsedTime(&ms[0], ev[0], ev[1]));
+			GF_HIP(hipEventElapsedTime(&ms[1], ev[1], ev[2]));
+		}
+	}
 	return 0;
 }
+
+} // namespace gf
+
+extern "C" {
 
 int gf_skygen_plan_free(gf_draw_plan *plan)
 {

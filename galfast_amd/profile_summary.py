@@ -1,0 +1,139 @@
+"""Summaries of rocprofv3 runs of the catalog benchmark.
+
+    python -m galfast_amd.profile_summary --kt DIR [--fetch DIR --write DIR
+        --copy-fetch DIR --copy-write DIR] --out profiles --tag catalog_c1
+        --program "python3 bench.py --steps 1 --warmup 0 --no-cpu --no-serve"
+
+--kt is a ``rocprofv3 --kernel-trace --stats --output-format csv`` output
+directory; --fetch / --write are ``--pmc FETCH_SIZE`` / ``--pmc WRITE_SIZE``
+runs of the same command; --copy-fetch / --copy-write are the same counters
+over ``bench.py --copy-calibration`` (a plain 1 GiB device copy), which fix
+the factor between each counter and bytes.  Writes <tag>_kernel_stats.md,
+<tag>_traffic.md (when the counter runs are given) and traffic.json (HBM
+bytes per launch of each kernel, read by bench.py for roofline.traffic).
+"""
+import argparse
+import csv
+import glob
+import json
+import os
+import re
+
+COPY_BYTES = 1 << 30            # bench.py --copy-calibration copies 1 GiB
+ROW_GROUPS = {"0": "KIN", "1": "PHOT", "2": "PERR", "3": "ASTRO"}
+
+
+def short_name(full):
+    """'void (anonymous namespace)::k_chain_rows<1, 1>(...)' -> 'k_chain_rows<PHOT>'."""
+    s = full.replace("(anonymous namespace)::", "")
+    if s.startswith("void "):
+        s = s[5:]
+    m = re.match(r"([A-Za-z_:][\w:]*)(<[^()]*>)?", s)
+    if not m:
+        return s
+    name, targs = m.group(1), m.group(2)
+    if name.endswith("k_chain_rows") or name.endswith("k_b_rows"):
+        g = targs.strip("<>").split(",")[0].strip() if targs else ""
+        return "%s<%s>" % (name, ROW_GROUPS.get(g, g))
+    return name
+
+
+def _csv(d, suffix):
+    f = glob.glob(os.path.join(d, "**", "*" + suffix), recursive=True)
+    if not f:
+        raise SystemExit("no *%s under %s" % (suffix, d))
+    with open(f[0]) as fh:
+        return list(csv.DictReader(fh))
+
+
+def kernel_stats(d):
+    out = {}
+    for r in _csv(d, "kernel_stats.csv"):
+        k = short_name(r["Name"])
+        e = out.setdefault(k, {"calls": 0, "total_ns": 0.0})
+        e["calls"] += int(r["Calls"])
+        e["total_ns"] += float(r["TotalDurationNs"])
+    for e in out.values():
+        e["avg_ns"] = e["total_ns"] / max(e["calls"], 1)
+    return out
+
+
+def counter_per_dispatch(d, counter):
+    acc = {}
+    for r in _csv(d, "counter_collection.csv"):
+        if r["Counter_Name"] != counter:
+            continue
+        k = short_name(r["Kernel_Name"])
+        e = acc.setdefault(k, {})
+        e[r["Dispatch_Id"]] = e.get(r["Dispatch_Id"], 0.0) + float(r["Counter_Value"])
+    return {k: (sum(v.values()) / len(v), len(v)) for k, v in acc.items()}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--kt", required=True)
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--copy-fetch")
+    ap.add_argument("--copy-write")
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--tag", required=True)
+    ap.add_argument("--program", required=True, help="the profiled program (after rocprofv3's --)")
+    ap.add_argument("--workload", default="")
+    a = ap.parse_args()
+    os.makedirs(a.out, exist_ok=True)
+
+    ks = kernel_stats(a.kt)
+    total = sum(e["total_ns"] for e in ks.values())
+    kt_cmd = "rocprofv3 --kernel-trace --stats --output-format csv -d %s -o kt -- %s" % (a.kt, a.program)
+    lines = ["# Kernel trace: %s" % a.tag, "", "Command: `%s`" % kt_cmd, ""]
+    if a.workload:
+        lines += ["Workload: %s" % a.workload, ""]
+    lines += ["| kernel | calls | total ms | avg us | share |", "|---|---|---|---|---|"]
+    for k, e in sorted(ks.items(), key=lambda kv: -kv[1]["total_ns"]):
+        lines.append("| `%s` | %d | %.2f | %.1f | %.1f%% |" % (k, e["calls"], e["total_ns"] / 1e6, e["avg_ns"] / 1e3,
+                                                            100.0 * e["total_ns"] / total))
+    with open(os.path.join(a.out, "%s_kernel_stats.md" % a.tag), "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+    if not (a.fetch and a.write and a.copy_fetch and a.copy_write):
+        return
+    fetch = counter_per_dispatch(a.fetch, "FETCH_SIZE")
+    write = counter_per_dispatch(a.write, "WRITE_SIZE")
+    cf = counter_per_dispatch(a.copy_fetch, "FETCH_SIZE")
+    cw = counter_per_dispatch(a.copy_write, "WRITE_SIZE")
+    copy = "__amd_rocclr_copyBuffer"
+    f_fetch = COPY_BYTES / (cf[copy][0] * 1024.0)
+    f_write = COPY_BYTES / (cw[copy][0] * 1024.0)
+    traffic = {"_factors": {"fetch_bytes_per_KiB_count": f_fetch * 1024.0, "write_bytes_per_KiB_count": f_write * 1024.0,
+                            "copy_fetch_KiB": cf[copy][0], "copy_write_KiB": cw[copy][0], "copy_bytes": COPY_BYTES},
+               "_program": a.program}
+    pmc = "rocprofv3 --pmc %s --output-format csv -d %s -o %s -- %s"
+    lines = ["# HBM traffic per launch: %s" % a.tag, "", "Commands:", "",
+             "* `%s`" % (pmc % ("FETCH_SIZE", a.fetch, "fetch", a.program)),
+             "* `%s`" % (pmc % ("WRITE_SIZE", a.write, "write", a.program)),
+             "* `%s`" % (pmc % ("FETCH_SIZE", a.copy_fetch, "cf", "python3 bench.py --copy-calibration")),
+             "* `%s`" % (pmc % ("WRITE_SIZE", a.copy_write, "cw", "python3 bench.py --copy-calibration")),
+             "", "Calibration: `bench.py --copy-calibration` copies 1 GiB with `hipMemcpy` "
+             "(kernel `%s`): FETCH_SIZE reads %.0f KiB and WRITE_SIZE %.0f KiB for 1,048,576 KiB moved each way, "
+             "so bytes = FETCH_SIZE x %.3f KiB and WRITE_SIZE x %.3f KiB." % (copy, cf[copy][0], cw[copy][0],
+                                                                          f_fetch, f_write), "",
+             "| kernel | dispatches | read MB / launch | written MB / launch | total MB / launch | avg us | "
+             "GB/s |", "|---|---|---|---|---|---|---|"]
+    for k in sorted(set(fetch) & set(write), key=lambda k: -ks.get(k, {"total_ns": 0})["total_ns"]):
+        rb = fetch[k][0] * 1024.0 * f_fetch
+        wb = write[k][0] * 1024.0 * f_write
+        avg = ks.get(k, {}).get("avg_ns")
+        traffic[k] = {"read_bytes": rb, "write_bytes": wb, "bytes": rb + wb, "dispatches": fetch[k][1],
+                      "avg_ns": avg}
+        gbs = (rb + wb) / avg if avg else 0.0
+        lines.append("| `%s` | %d | %.1f | %.1f | %.1f | %s | %.0f |" % (
+            k, fetch[k][1], rb / 1e6, wb / 1e6, (rb + wb) / 1e6, "%.1f" % (avg / 1e3) if avg else "-", gbs))
+    with open(os.path.join(a.out, "%s_traffic.md" % a.tag), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    with open(os.path.join(a.out, "traffic.json"), "w") as f:
+        json.dump(traffic, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()

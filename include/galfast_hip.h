@@ -472,6 +472,7 @@ typedef struct gf_run_stats {
     double   t_stage[6];    /* module chain launches: draw pass; row pass for
                                FeH/GaussianFeH/Bond2010, photometry, photometric
                                errors, vel2pm + gal2other; (spare) */
+    double   t_emit_rng, t_emit_rows;   /* the two launches inside t_emit */
 } gf_run_stats;
 
 /* Called once per output chunk with device columns holding `nrows` rows

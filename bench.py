@@ -62,21 +62,37 @@ STAGE_NAMES = ["k_chain_draws", "k_chain_rows<KIN>", "k_chain_rows<PHOT>", "k_ch
                "k_chain_rows<ASTRO>"]
 
 
-def cpu_baseline(cat, workload, budget_s):
+def cpu_baseline(cat, workload, budget_s, threads):
+    """The oracle (a single-threaded C++ restatement of the reference's CPU
+    path) timed on the host: once on one thread over a block of pixels, and
+    on `threads` threads, each replaying its own block of pixels (the oracle
+    calls release the GIL, so the blocks run in parallel)."""
+    from concurrent.futures import ThreadPoolExecutor
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import galfast_amd as G
     import replay as R
     view = R.host_view(G.lib(), cat._h)
     npix = view.sp.npixels
-    # a contiguous block of pixels around the middle of the pixel list
-    block = max(1, min(npix, 400))
-    start = npix // 2
-    idx = list(range(start, min(npix, start + block)))
-    r = R.replay(view, chunk_rows=30_000_000, pixel_index=idx, time_budget=budget_s)
-    rate = r["visible"] / r["seconds"] if r["seconds"] > 0 else 0.0
-    return {"value": rate, "unit": "stars/s", "cores": 1, "kind": "port",
-            "sample": "%d of %d pixels (%d visible stars, %.1f s, single-threaded oracle)"
-                      % (len(idx), npix, r["visible"], r["seconds"])}
+    block = max(1, min(npix // max(threads, 1), 400))
+    start = npix // 4
+
+    def run(i, budget):
+        idx = list(range(start + i * block, min(npix, start + (i + 1) * block)))
+        return R.replay(view, chunk_rows=30_000_000, pixel_index=idx, time_budget=budget), len(idx)
+
+    one, n1 = run(0, budget_s)
+    rate1 = one["visible"] / one["seconds"] if one["seconds"] > 0 else 0.0
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        res = list(ex.map(lambda i: run(i, budget_s), range(threads)))
+    wall = time.perf_counter() - t0
+    vis = sum(r["visible"] for r, _ in res)
+    return {"value": vis / wall, "unit": "stars/s", "cores": threads, "kind": "port",
+            "sample": "%d threads, each replaying its own block of %d of %d pixels on the oracle (%d visible stars "
+                      "in %.1f s wall)" % (threads, block, npix, vis, wall),
+            "single_thread": {"value": rate1, "cores": 1,
+                              "sample": "%d pixels (%d visible stars, %.1f s)" % (n1, one["visible"],
+                                                                                    one["seconds"])}}
 
 
 def _serve_once(conf, reqs, rate, warm, **kw):
@@ -171,9 +187,12 @@ def serving(args):
     out["batched"] = _serve_once(conf, reqs, None, warm, max_batch=args.serve_batch, workers=2)
     out["served_vs_plain"] = out["batched"]["stars_per_s"] / out["plain_job"]["stars_per_s"]
     out["served_vs_unbatched"] = out["batched"]["stars_per_s"] / out["unbatched"]["stars_per_s"]
+    # Poisson arrivals: at each rate the first min(all, rate x 10 s) requests
     out["poisson"] = {}
     for rate in args.serve_rates:
-        out["poisson"]["%g" % rate] = _serve_once(conf, reqs, rate, warm, max_batch=args.serve_batch, workers=2)
+        n = max(20, min(len(reqs), int(rate * args.serve_seconds)))
+        out["poisson"]["%g" % rate] = _serve_once(conf, reqs[:n], rate, warm, max_batch=args.serve_batch,
+                                                  workers=2)
     return out
 
 
@@ -185,11 +204,13 @@ def main():
     ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS))
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-batch", type=int, default=256)
-    ap.add_argument("--serve-rates", type=float, nargs="*", default=[100.0, 1000.0])
+    ap.add_argument("--serve-rates", type=float, nargs="*", default=[10.0, 100.0, 1000.0])
+    ap.add_argument("--serve-seconds", type=float, default=10.0, help="arrival window per Poisson rate")
     ap.add_argument("--plain-jobs", type=int, default=40, help="requests timed as separate catalog jobs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (the reference kernel for scaling "
@@ -302,7 +323,7 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_total / max(n, 1)},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cat, args.workload, args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(cat, args.workload, args.cpu_budget, args.cpu_threads)
     cat.close()
     if not args.no_serve:
         sv = serving(args)

@@ -32,6 +32,21 @@ struct gf_mwc
 		return u;
 	}
 
+	// One coordinate of a polar try, -1 + 2*uniform(): uniform() is
+	// 2^-32 * (float)x exactly (the constant is 2^-32), so the product by 2
+	// is exact and one fma with 2^-31 rounds the same single sum.
+	GF_HD float polar_coord()
+	{
+		uint64_t t = (uint64_t)a * (uint64_t)x + (uint64_t)c;
+		c = (uint32_t)(t >> 32);
+		x = (uint32_t)t;
+		return __builtin_fmaf((float)x, 0x1p-31f, -1.f);
+	}
+
+	// The acceptance test of a try, !(r2 > 1 || r2 == 0 || r2 == 4) of the
+	// reference's loop, for the r2 >= 0 a try produces.
+	GF_HD static bool polar_accept(float r2) { return r2 <= 1.0f && r2 != 0.f; }
+
 	// The draws of one polar Box-Muller deviate: uniform pairs until the
 	// point falls inside the unit disc.  gaussian() finishes it; modules
 	// that only need to move the stream past a deviate stop here.
@@ -39,10 +54,18 @@ struct gf_mwc
 	{
 		float xx;
 		do {
-			xx = -1.f + 2.f * uniform();
-			yy = -1.f + 2.f * uniform();
+			xx = polar_coord();
+			yy = polar_coord();
 			r2 = xx * xx + yy * yy;
-		} while (r2 > 1.0f || r2 == 0.f || r2 == 4.f);
+		} while (!polar_accept(r2));
+	}
+
+	// One try of a polar deviate; true when it is accepted.
+	GF_HD bool polar_try()
+	{
+		float xx = polar_coord();
+		float yy = polar_coord();
+		return polar_accept(xx * xx + yy * yy);
 	}
 
 	// Move the stream past n polar deviates: the same draws as n calls of
@@ -50,12 +73,7 @@ struct gf_mwc
 	// numbers of tries for each deviate wait once instead of n times.
 	GF_HD void polar_skip(int n)
 	{
-		while (n > 0) {
-			float xx = -1.f + 2.f * uniform();
-			float yy = -1.f + 2.f * uniform();
-			float r2 = xx * xx + yy * yy;
-			n -= (r2 > 1.0f || r2 == 0.f || r2 == 4.f) ? 0 : 1;
-		}
+		while (n > 0) n -= polar_try() ? 1 : 0;
 	}
 
 	GF_HD float gaussian(const float sigma)

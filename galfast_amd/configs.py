@@ -230,4 +230,118 @@ def c3_requests(n, seed=0, area=(0.01, 100.0), bmin=5.0, m1=(20.0, 25.0), id0=1)
     return out
 
 
-WRITERS = {"c0": write_c0, "c1": write_c1, "c3": write_c3}
+BANDS6 = BANDS + ["SYNy"]
+
+
+def lambert_deproject_grid(x, y, hemi):
+    """(l, b) in radians of Lambert-plane points of the north (hemi 0, pole
+    l=90, b=90) or south (hemi 1, pole l=270, b=-90) projection."""
+    r2 = x * x + y * y
+    sb = np.clip(1.0 - 0.5 * r2, -1.0, 1.0)
+    if hemi == 0:
+        return np.pi / 2 + np.arctan2(x, -y), np.arcsin(sb)
+    return 1.5 * np.pi + np.arctan2(x, y), -np.arcsin(sb)
+
+
+def dust_disk_cube(hemi, n=256, ndm=61, dm_max=30.0, a0=0.5e-3, h=100.0):
+    """r-band extinction of an exponential dust disk (a0 mag/pc in the
+    plane, scale height h pc, the Sun in the plane) integrated along each
+    line of sight: A(D, b) = a0 h / |sin b| (1 - exp(-D |sin b| / h)).
+    Axes X, Y in [-2, 2] (Lambert plane of hemisphere `hemi`) and DM in
+    [0, dm_max]; zero outside the hemisphere (r^2 > 2).  Returns the cube as
+    [DM][Y][X] and its (CRVAL, CDELT) per axis."""
+    xs = np.linspace(-2.0, 2.0, n)
+    X, Y = np.meshgrid(xs, xs)                       # [y][x]
+    _, b = lambert_deproject_grid(X, Y, hemi)
+    sb = np.abs(np.sin(b))
+    out = np.zeros((ndm, n, n), dtype=np.float64)
+    for k in range(ndm):
+        D = 10 ** (0.2 * (dm_max * k / (ndm - 1)) + 1.0)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            A = np.where(sb > 1e-9, a0 * h / sb * (1.0 - np.exp(-D * sb / h)), a0 * D)
+        out[k] = A
+    out[:, (X * X + Y * Y) > 2.0] = 0.0
+    axes = [(-2.0, 4.0 / (n - 1)), (-2.0, 4.0 / (n - 1)), (0.0, dm_max / (ndm - 1))]
+    return out.astype(np.float32), axes
+
+
+def write_fits_cube(path, cube, axes, extra=None):
+    """A FITS primary image of 32-bit floats (cube indexed [z][y][x]) with
+    CRPIXn = 1, CRVALn, CDELTn coordinates."""
+    nz, ny, nx = cube.shape
+    cards = [("SIMPLE", "T"), ("BITPIX", "-32"), ("NAXIS", "3"), ("NAXIS1", str(nx)), ("NAXIS2", str(ny)),
+             ("NAXIS3", str(nz))]
+    for i, (crval, cdelt) in enumerate(axes):
+        cards += [("CRPIX%d" % (i + 1), "1"), ("CRVAL%d" % (i + 1), repr(float(crval))),
+                  ("CDELT%d" % (i + 1), repr(float(cdelt)))]
+    for k, v in (extra or {}).items():
+        cards.append((k, repr(float(v))))
+    hdr = "".join(("%-8s= %20s" % (k, v)).ljust(80) for k, v in cards) + "END".ljust(80)
+    hdr = hdr.ljust((len(hdr) + 2879) // 2880 * 2880)
+    data = np.ascontiguousarray(cube, dtype=">f4").tobytes()
+    data += b"\0" * ((-len(data)) % 2880)
+    with open(path, "wb") as f:
+        f.write(hdr.encode("ascii"))
+        f.write(data)
+
+
+def write_c4(d, seed=42, small=False):
+    """Config 4: 6 synthetic bands, unresolved binaries (fraction 0.5,
+    companion magnitudes from the LF, fainter than the primary) and an
+    analytic exponential dust disk (h = 100 pc, 0.5 mag/kpc in r) as north
+    and south (X, Y, DM) extinction cubes, r in [14, 27.5] over
+    l in [60, 100], b in [20, 50] (~970 deg^2) in 0.25 deg pixels.  small:
+    the config-0 sky (10x10 deg, 1 deg pixels, r 15-21) moved to b in
+    [10, 20] with 64x64x31 cubes, for tests."""
+    write_common(d, seed)
+    np.savetxt(os.path.join(d, "photosys.txt"), seeded_photosys(seed, ncolors=5), fmt="%.8f")
+    m = np.round(np.arange(10.0, 30.0 + 0.005, 0.01), 6)
+    err = 0.02 + np.exp(0.8 * (m - 24.0))
+    np.savetxt(os.path.join(d, "SYN.SYNy.photoerr.txt"), np.c_[m, err], fmt="%.8g")
+    _w(os.path.join(d, "photometry.conf"), """
+module = photometry
+comp = 0-2
+bandset = SYN
+file = photosys.txt
+bands = %s
+bootstrap_band = SYNr
+absband = absSYNr
+dFeH = 0.01
+dM = 0.01
+reddening = 1.8739 1.3788 1 0.7583 0.5376 0.4""" % " ".join(BANDS6))
+    _w(os.path.join(d, "multiples.conf"), """
+module = unresolvedMultiples
+comp = 0-2
+algorithm = LF_M2_gt_M1
+fraction = 0.5
+lumfunc = lf.txt""")
+    _w(os.path.join(d, "photoerr.conf"), "module = photometricErrors\n" +
+       "\n".join("SYN.%s.file = SYN.%s.photoerr.txt" % (b, b) for b in BANDS6))
+    n, ndm = (64, 31) if small else (256, 61)
+    for hemi, name, pole in ((0, "north", (90.0, 90.0)), (1, "south", (-90.0, -90.0))):
+        cube, axes = dust_disk_cube(hemi, n=n, ndm=ndm)
+        write_fits_cube(os.path.join(d, "ext.%s.fits" % name), cube, axes, {"LAMBDA0": pole[0], "PHI1": pole[1]})
+    _w(os.path.join(d, "ext.conf"), "north = ext.north.fits\nsouth = ext.south.fits")
+    if small:
+        sky, foot = ("m0 = 15\nm1 = 21\ndx = 1", "rect = 85 95 10 20")
+    else:
+        sky, foot = ("m0 = 14\nm1 = 27.5\ndx = 0.25", "rect = 60 100 20 50")
+    _w(os.path.join(d, "skygen.conf"), "module = skygen\nband = SYNr\ndm = 0.05\nM0 = 3\nM1 = 15\n%s\n"
+       "extmaps = ext.conf" % sky)
+    _w(os.path.join(d, "foot.conf"), "module = footprint\ntype = rect\ncoordsys = gal\n%s" % foot)
+    _w(os.path.join(d, "cmd.conf"), """
+module = config
+seed = %d
+definitions = definitions.conf
+input = skygen.conf
+models = model.thin.conf model.thick.conf model.halo.conf
+footprints = foot.conf
+modules = feh.conf multiples.conf photometry.conf photoerr.conf""" % seed)
+    return os.path.join(d, "cmd.conf")
+
+
+def write_c4_small(d, seed=42):
+    return write_c4(d, seed, small=True)
+
+
+WRITERS = {"c0": write_c0, "c1": write_c1, "c3": write_c3, "c4": write_c4, "c4s": write_c4_small}

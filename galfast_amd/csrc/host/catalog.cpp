@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <chrono>
 #include <fstream>
+#include <limits>
 #include <mutex>
 #include <sstream>
 
@@ -142,6 +143,14 @@ struct gf_catalog {
 	gf_photoerr_data perr{};
 	DevBuf perr_dev[2];
 	gf_vel2pm_data v2pm{};
+
+	// extinction (os_skygen::load_extinction_maps): north/south maps and the
+	// per-pixel minimum along each DM slice; ext holds device pointers,
+	// ext_h host pointers
+	bool has_ext = false;
+	Cube ext_north, ext_south, ext_beam;
+	DevBuf ext_dev[3];
+	gf_extinction ext{}, ext_h{};
 
 	// host copies for gf_catalog_host_view
 	std::vector<gf_model> models_h;
@@ -355,6 +364,102 @@ void load_footprint(gf_catalog &c, const Config &fc)
 		f.cosr = cos(v[2] * M_PI / 180.);
 	} else fail("unknown footprint type " + type);
 	f.bmin = fc.getd("bmin", 0.);
+}
+
+// ---------------------------------------------------------------------------
+// Extinction maps (skygen/os_skygen.cpp:285-342, 699-814)
+// ---------------------------------------------------------------------------
+
+// find_extinction_minima (skygen/os_skygen.cpp:285-341): for every DM slice
+// z, the smallest extinction one slice closer over the map texels that can
+// overlap the pixel (texels outside the hemisphere with zero extinction
+// skipped); proj_x, proj_y: the pixel centre in the map's projection.
+void extinction_minima(Cube &beam, int extIdx, double proj_x, double proj_y, double dx, const Cube &tex)
+{
+	const int xidx = extIdx / 2048, yidx = extIdx % 2048;
+	const float *tcx = tex.tc[0], *tcy = tex.tc[1];
+	double dr = 1.0001 / 2. * sqrt(2.) * (dx + std::max(1. / tcx[1], 1. / tcy[1]));
+	int xi = (int)((proj_x - dr - tcx[0]) * tcx[1]);  xi = std::max(0, std::min(xi - 1, tex.nx));
+	int yi = (int)((proj_y - dr - tcy[0]) * tcy[1]);  yi = std::max(0, std::min(yi - 1, tex.ny));
+	int xie = (int)((proj_x + dr - tcx[0]) * tcx[1]); xie = std::max(0, std::min(xie + 2, tex.nx));
+	int yie = (int)((proj_y + dr - tcy[0]) * tcy[1]); yie = std::max(0, std::min(yie + 2, tex.ny));
+	for (int x = xi; x < xie; x++) {
+		for (int y = yi; y < yie; y++) {
+			if (tex.at(x, y, tex.nz - 1) == 0.f) {
+				double xx = (double)x / tcx[1] + tcx[0], yy = (double)y / tcy[1] + tcy[0];
+				if (xx * xx + yy * yy > 2.) continue;
+			}
+			for (int z = 0; z < tex.nz; z++) {
+				float Ar = z == 0 ? 0.f : tex.at(x, y, z - 1);
+				float &b = beam.at(xidx, yidx, z);
+				b = std::min(b, Ar);
+			}
+		}
+	}
+}
+
+gf_tex3d tex3d_of(const Cube &c, const float *data)
+{
+	gf_tex3d t{};
+	t.data = data;
+	t.nx = c.nx; t.ny = c.ny; t.nz = c.nz;
+	for (int a = 0; a < 3; a++) { t.tc[a][0] = c.tc[a][0]; t.tc[a][1] = c.tc[a][1]; }
+	return t;
+}
+
+// os_skygen::load_extinction_maps: "north", "south" map files, "nscale",
+// "sscale" factors; pixels get extIdx = their index.
+void load_extinction(gf_catalog &c, const std::string &econf)
+{
+	if (econf.empty()) return;
+	Config cfg;
+	if (!cfg.load(c.skygen.resolve(econf))) fail("cannot read " + c.skygen.resolve(econf));
+	std::string nf = cfg.get("north", ""), sf = cfg.get("south", "");
+	float nscale = (float)cfg.getd("nscale", 1.), sscale = (float)cfg.getd("sscale", 1.);
+	if (nf.empty() && sf.empty()) return;
+	if (!nf.empty()) {
+		c.ext_north = load_fits_cube(cfg.resolve(nf));
+		for (auto &v : c.ext_north.v) v *= nscale;
+	}
+	if (!sf.empty()) {
+		c.ext_south = load_fits_cube(cfg.resolve(sf));
+		for (auto &v : c.ext_south.v) v *= sscale;
+	}
+	if (!nf.empty() && !sf.empty() && c.ext_north.nz != c.ext_south.nz)
+		fail("north and south extinction maps differ in depth");
+	const Cube &ref = nf.empty() ? c.ext_south : c.ext_north;
+	Cube &beam = c.ext_beam;
+	const int np = (int)c.pixels.size();
+	beam.nx = std::max(1, np / 2048 + (np % 2048 ? 1 : 0));
+	beam.ny = std::max(1, std::min(np, 2048));
+	beam.nz = ref.nz;
+	beam.v.assign((size_t)beam.nx * beam.ny * beam.nz, std::numeric_limits<float>::infinity());
+	texcoord_from_range(0, (float)beam.nx, 0, (float)beam.nx, beam.tc[0]);
+	texcoord_from_range(0, (float)beam.ny, 0, (float)beam.ny, beam.tc[1]);
+	beam.tc[2][0] = ref.tc[2][0];
+	beam.tc[2][1] = ref.tc[2][1];
+	for (int i = 0; i < np; i++) {
+		gf_pencil_beam &p = c.pixels[i];
+		p.extIdx = i;
+		double l = atan2((double)p.sl, (double)p.cl), b = atan2((double)p.sb, (double)p.cb);
+		double x, y;
+		if (!nf.empty()) { proj_pole(0, l, b, x, y); extinction_minima(beam, i, x, y, p.dx, c.ext_north); }
+		if (!sf.empty()) { proj_pole(1, l, b, x, y); extinction_minima(beam, i, x, y, p.dx, c.ext_south); }
+	}
+	c.ext_dev[0].upload(c.ext_north.v);
+	c.ext_dev[1].upload(c.ext_south.v);
+	c.ext_dev[2].upload(beam.v);
+	if (!nf.empty()) {
+		c.ext.north = tex3d_of(c.ext_north, (const float *)c.ext_dev[0].p);
+		c.ext_h.north = tex3d_of(c.ext_north, c.ext_north.v.data());
+	}
+	if (!sf.empty()) {
+		c.ext.south = tex3d_of(c.ext_south, (const float *)c.ext_dev[1].p);
+		c.ext_h.south = tex3d_of(c.ext_south, c.ext_south.v.data());
+	}
+	c.ext.beam = tex3d_of(beam, (const float *)c.ext_dev[2].p);
+	c.ext_h.beam = tex3d_of(beam, beam.v.data());
+	c.has_ext = true;
 }
 
 // os_skygen::load_skyPixelizationConfig + reset_absmag (skygen/os_skygen.cpp:401-430)
@@ -749,6 +854,7 @@ void open_catalog(gf_catalog &c, const std::string &cmdfile)
 	load_sky_pixelization(c, dx);
 	load_footprint(c, c.foot_cfgs[0]);
 	pixelize(c, dx);
+	load_extinction(c, c.skygen.get("extmaps", ""));
 	for (auto &mc : c.model_cfgs) load_model(c, mc);
 	// hint_absmag: restrict the absolute-magnitude range to the LF's support
 	// (skygen/skyconfig_impl.h:258-263, skygen/model_lib.h:97-112); one range is
@@ -915,7 +1021,8 @@ int gf_catalog_count(gf_catalog *c, gf_count_result *per_model, double *expected
 	for (size_t i = 0; i < c->models.size(); i++) {
 		Model &m = *c->models[i];
 		m.wpk.alloc((size_t)c->sp.npixels * K * sizeof(double));
-		int rc = gf_skygen_count(&c->sp, &m.m, (const gf_pencil_beam *)c->pix_dev.p, nullptr, c->acc_mode, &m.counts,
+		int rc = gf_skygen_count(&c->sp, &m.m, (const gf_pencil_beam *)c->pix_dev.p, c->has_ext ? &c->ext : nullptr,
+		                         c->acc_mode, &m.counts,
 		                         nullptr, (double *)m.wpk.p, c->stream);
 		if (rc) return rc;
 		if (per_model) per_model[i] = m.counts;
@@ -928,7 +1035,8 @@ int gf_catalog_count(gf_catalog *c, gf_count_result *per_model, double *expected
 		tot = 0;
 		for (size_t i = 0; i < c->models.size(); i++) {
 			Model &m = *c->models[i];
-			int rc = gf_skygen_count(&c->sp, &m.m, (const gf_pencil_beam *)c->pix_dev.p, nullptr, c->acc_mode,
+			int rc = gf_skygen_count(&c->sp, &m.m, (const gf_pencil_beam *)c->pix_dev.p,
+			                         c->has_ext ? &c->ext : nullptr, c->acc_mode,
 			                         &m.counts, nullptr, (double *)m.wpk.p, c->stream);
 			if (rc) return rc;
 			if (per_model) per_model[i] = m.counts;
@@ -961,7 +1069,8 @@ int gf_catalog_run(gf_catalog *c, gf_chunk_fn fn, void *user, gf_run_stats *stat
 		gf_draw_plan *plan = nullptr;
 		uint64_t total = 0;
 		(void)hipEventRecord(c->ev[0], st);
-		int rc = gf_skygen_plan(&c->sp, &m.m, (const gf_pencil_beam *)c->pix_dev.p, nullptr, (const double *)m.wpk.p,
+		int rc = gf_skygen_plan(&c->sp, &m.m, (const gf_pencil_beam *)c->pix_dev.p, c->has_ext ? &c->ext : nullptr,
+		                        (const double *)m.wpk.p,
 		                        &c->rng, &plan, &total, st);
 		if (rc) return rc;
 		(void)hipEventRecord(c->ev[1], st);
@@ -1042,6 +1151,7 @@ int gf_catalog_host_view_get(const gf_catalog *c, gf_catalog_host_view *v)
 	v->nabsmag = c->nabsmag;
 	v->nstreams = 65536;
 	v->rng_state0 = c->rng_state0.data();
+	if (c->has_ext) v->ext = c->ext_h;
 	return 0;
 }
 

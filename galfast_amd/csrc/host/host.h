@@ -70,6 +70,24 @@ struct PhotTables {
 PhotTables build_photometry_tables(const std::vector<std::vector<double>> &rows, int ncolors, double dFeH,
                                    double dMr);
 
+// A 3D float cube (x fastest) with the texture coordinates of each axis
+// (xi = (x - tc[0]) * tc[1] + 0.5 at texel centres).
+struct Cube {
+	int nx = 0, ny = 0, nz = 0;
+	std::vector<float> v;
+	float tc[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+	double l0 = -100., phi1 = -100.;   // LAMBDA0 / PHI1 (degrees) when present
+	float &at(int x, int y, int z) { return v[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)]; }
+	float at(int x, int y, int z) const { return v[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)]; }
+};
+// load_extinction_map (skygen/os_skygen.cpp:518-568): a FITS primary image
+// of 32-bit floats, NAXIS = 3, axes X, Y (Lambert plane) and DM, with
+// CRPIXn / CRVALn / CDELTn coordinates (texcoord_from_wcs, :465-488) or the
+// defaults X, Y in [-2, 2] and DM in [10, 15].
+Cube load_fits_cube(const std::string &file);
+// texcoord_from_range (common/cux.cpp:389-397)
+void texcoord_from_range(float imgx, float imgy, float x, float y, float tc[2]);
+
 // Coordinate transform of a model or module (common/transform.cpp:345-388).
 void load_transform(float T[3], float M[3][3], const std::string &center, const std::string &orientation,
                     const Config &cfg);

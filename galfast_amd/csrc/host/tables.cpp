@@ -7,6 +7,8 @@
 
 #include <algorithm>
 #include <fstream>
+#include <map>
+#include <stdlib.h>
 #include <sstream>
 
 namespace gfh {
@@ -276,6 +278,103 @@ void load_transform(float T[3], float M[3][3], const std::string &center, const 
 	if (ss >> kind) fail("rotations on top of the coordinate system are not supported: " + orientation);
 	for (int i = 0; i < 3; i++)
 		for (int j = 0; j < 3; j++) M[i][j] = (float)R[i][j];
+}
+
+void texcoord_from_range(float imgx, float imgy, float x, float y, float tc[2])
+{
+	tc[0] = (-imgy * x + imgx * y) / (imgx - imgy);
+	tc[1] = (imgx - imgy) / (x - y);
+}
+
+namespace {
+
+// FITS header card value: the text after "= " up to a '/' comment
+bool card_value(const std::string &card, const std::string &key, std::string &val)
+{
+	if (card.compare(0, key.size(), key) != 0) return false;
+	size_t k = key.size();
+	while (k < 8 && card[k] == ' ') k++;
+	if (k != 8 || card.size() < 10 || card[8] != '=' ) return false;
+	std::string v = card.substr(10);
+	size_t q = v.find('/');
+	if (q != std::string::npos && v.find('\'') == std::string::npos) v = v.substr(0, q);
+	size_t a = v.find_first_not_of(' '), b = v.find_last_not_of(' ');
+	val = a == std::string::npos ? std::string() : v.substr(a, b - a + 1);
+	return true;
+}
+
+} // namespace
+
+Cube load_fits_cube(const std::string &file)
+{
+	std::ifstream in(file, std::ios::binary);
+	if (!in) fail("cannot open extinction map " + file);
+	std::map<std::string, std::string> kv;
+	bool end = false;
+	char block[2880];
+	while (!end) {
+		in.read(block, 2880);
+		if (in.gcount() != 2880) fail("truncated FITS header in " + file);
+		for (int i = 0; i < 36 && !end; i++) {
+			std::string card(block + 80 * i, 80);
+			if (card.compare(0, 3, "END") == 0 && card.find_first_not_of(' ', 3) == std::string::npos) {
+				end = true;
+				break;
+			}
+			std::string key = card.substr(0, 8);
+			key = key.substr(0, key.find_last_not_of(' ') + 1);
+			std::string val;
+			if (!key.empty() && card_value(card, key, val)) kv[key] = val;
+		}
+	}
+	auto num = [&](const std::string &k, bool &ok) {
+		auto it = kv.find(k);
+		ok = it != kv.end();
+		return ok ? strtod(it->second.c_str(), nullptr) : 0.0;
+	};
+	bool ok;
+	if ((int)num("NAXIS", ok) != 3 || !ok)
+		fail("Supplied extinction map file " + file + " is not 3-dimensional.");
+	if ((int)num("BITPIX", ok) != -32 || !ok)
+		fail("Supplied extinction map file " + file + " is not a data cube of 32-bit floats");
+	Cube c;
+	c.nx = (int)num("NAXIS1", ok);
+	c.ny = (int)num("NAXIS2", ok);
+	c.nz = (int)num("NAXIS3", ok);
+	if (c.nx < 1 || c.ny < 1 || c.nz < 1) fail("bad extinction map dimensions in " + file);
+	size_t n = (size_t)c.nx * c.ny * c.nz;
+	std::vector<unsigned char> raw(n * 4);
+	in.read((char *)raw.data(), (std::streamsize)raw.size());
+	if ((size_t)in.gcount() != raw.size()) fail("truncated FITS data in " + file);
+	c.v.resize(n);
+	for (size_t i = 0; i < n; i++) {
+		uint32_t u = (uint32_t)raw[4 * i] << 24 | (uint32_t)raw[4 * i + 1] << 16 | (uint32_t)raw[4 * i + 2] << 8 |
+		             (uint32_t)raw[4 * i + 3];
+		memcpy(&c.v[i], &u, 4);
+	}
+	// texture coordinates from the WCS keys (all three axes) or the defaults
+	bool have = kv.count("CRPIX1") != 0;
+	const int dims[3] = {c.nx, c.ny, c.nz};
+	for (int a = 0; a < 3; a++) {
+		std::string s = std::to_string(a + 1);
+		if (kv.count("CROTA" + s)) fail("Axis rotations not supported (CROTA" + s + " keyword present in " + file + ")");
+		if (have) {
+			bool o1, o2, o3;
+			float imgx = (float)(num("CRPIX" + s, o1) - 1);
+			float x = (float)num("CRVAL" + s, o2), dx = (float)num("CDELT" + s, o3);
+			if (!(o1 && o2 && o3)) fail("incomplete coordinate keys for axis " + s + " in " + file);
+			c.tc[a][0] = x - imgx * dx;
+			c.tc[a][1] = 1.f / dx;
+		} else if (a < 2) {
+			texcoord_from_range(0, (float)(dims[a] - 1), -2, 2, c.tc[a]);
+		} else {
+			texcoord_from_range(0, (float)(dims[a] - 1), 10, 15, c.tc[a]);
+		}
+	}
+	bool o1, o2;
+	double l0 = num("LAMBDA0", o1), phi1 = num("PHI1", o2);
+	if (o1 && o2) { c.l0 = l0; c.phi1 = phi1; }
+	return c;
 }
 
 } // namespace gfh

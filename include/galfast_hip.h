@@ -510,6 +510,7 @@ typedef struct gf_catalog_host_view {
     int32_t               nabsmag;
     uint32_t              nstreams;
     const uint32_t       *rng_state0;
+    gf_extinction         ext;      /* host tables; all data NULL without extinction */
 } gf_catalog_host_view;
 
 int gf_catalog_host_view_get(const gf_catalog *c, gf_catalog_host_view *view);

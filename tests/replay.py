@@ -21,7 +21,13 @@ class gf_catalog_host_view(C.Structure):
     _fields_ = [("sp", A.gf_skygen_params), ("pixels", C.c_void_p), ("nmodels", C.c_int32),
                 ("nbands", C.c_int32), ("models", C.POINTER(A.gf_model)), ("foot", A.gf_footprint),
                 ("chain", A.gf_chain), ("nabsmag", C.c_int32), ("nstreams", C.c_uint32),
-                ("rng_state0", C.POINTER(C.c_uint32))]
+                ("rng_state0", C.POINTER(C.c_uint32)), ("ext", A.gf_extinction)]
+
+
+def _ext(view):
+    """The view's extinction tables (None without extinction)."""
+    e = view.ext
+    return C.byref(e) if (e.beam.data or e.north.data or e.south.data) else None
 
 
 def host_view(lib, handle):
@@ -57,7 +63,8 @@ def replay(view, chunk_rows=30_000_000, pixel_index=None, on_chunk=None, time_bu
         m = view.models[mi]
         res = A.gf_count_result()
         wpk = np.zeros(U)
-        o.gfo_count_bins(C.byref(sp), C.byref(m), C.cast(pix, C.c_void_p), None, C.byref(res), None, O.ptr(wpk))
+        o.gfo_count_bins(C.byref(sp), C.byref(m), C.cast(pix, C.c_void_p), _ext(view), C.byref(res), None,
+                         O.ptr(wpk))
         counts = np.zeros(U, dtype=np.uint32)
         o.gfo_plan(O.ptr(wpk), U, O.ptr(state), ns, O.ptr(counts))
         offs = np.zeros(U + 1, dtype=np.uint64)
@@ -70,8 +77,8 @@ def replay(view, chunk_rows=30_000_000, pixel_index=None, on_chunk=None, time_bu
             n = r1 - r0
             t = HostTable(n, spec)
             cols = t.skygen_cols()
-            o.gfo_emit(C.byref(sp), C.byref(m), C.cast(pix, C.c_void_p), None, C.byref(foot), O.ptr(offs), U, r0,
-                       r1, O.ptr(state), ns, C.byref(cols))
+            o.gfo_emit(C.byref(sp), C.byref(m), C.cast(pix, C.c_void_p), _ext(view), C.byref(foot), O.ptr(offs),
+                       U, r0, r1, O.ptr(state), ns, C.byref(cols))
             tab = t.table()
             for b0 in range(0, n, sub):
                 ks = A.gf_rows(b0, 256, min(n, b0 + sub), 0)

@@ -76,3 +76,40 @@ def test_c0_generated_matches_expected(c0):
     st = cat.run()
     assert abs(st.generated - st.expected) < 5 * np.sqrt(st.expected)
     assert 0 < st.visible <= st.generated
+
+
+@pytest.fixture(scope="module")
+def c4s(tmp_path_factory):
+    d = str(tmp_path_factory.mktemp("c4s"))
+    return configs.write_c4_small(d)
+
+
+def test_c4_extinction_multiples_six_bands_match_oracle_replay(c4s):
+    """Config 4's ingredients on a small sky at low latitude: extinction
+    cubes (FITS, analytic dust disk) sampled in the count (per-pixel
+    minimum) and the draw (Am, AmInf), unresolved binaries and 6 bands;
+    every column of every visible row bit-identical to the oracle replay."""
+    cat = Catalog(c4s, seed=5, chunk_rows=60000)
+    info = cat.info()
+    assert info.nbands == 6
+    expected = cat.count()
+    view = R.host_view(G.lib(), cat._h)
+    assert view.ext.beam.data and view.ext.north.data and view.ext.south.data
+    spec = catalog_spec(nbands=view.nbands, nabsmag=view.nabsmag)
+    got = collect(cat, spec)
+    ref = []
+    R.replay(view, chunk_rows=60000, on_chunk=lambda t, n, comp: ref.append((t, n, comp)))
+    assert [(n, c) for _, n, c in got] == [(n, c) for _, n, c in ref]
+    nvis = 0
+    am_pos = 0
+    for (a, n, _), (b, _, _) in zip(got, ref):
+        assert np.array_equal(a.arr("hidden"), b.arr("hidden"))
+        vis = b.arr("hidden")[0] == 0
+        nvis += vis.sum()
+        am_pos += (b.arr("Am")[0][vis] > 0).sum()
+        for c in spec:
+            x, y = a.arr(c)[:, vis], b.arr(c)[:, vis]
+            bad = (x != y) & ~(np.isnan(x) & np.isnan(y)) if x.dtype.kind == "f" else (x != y)
+            assert not bad.any(), "%s: %d of %d differ" % (c, bad.sum(), bad.size)
+    assert nvis > 0.3 * expected
+    assert am_pos > 0.9 * nvis          # the dust disk reddens nearly every star

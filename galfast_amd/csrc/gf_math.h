@@ -6,25 +6,24 @@
 // per-cell distance (skygen/skygen.cu.h:222, 408).  Rejection loops compare
 // those results against uniforms, so integer star counts depend on the last
 // bit of every transcendental.  To make the GPU path and a CPU restatement
-// agree bit for bit, every such function here is evaluated in IEEE double
-// with explicit fma() and rounded once to float:
+// agree bit for bit, every such function here is built only from +, *, /,
+// sqrt, fma and rint, which are correctly rounded IEEE operations on gfx950
+// and on x86-64, so the results are identical on both:
 //
-//   exp:  Cody-Waite reduction x = k*ln2 + r, |r| <= ln2/2, Taylor series
-//         of e^r to degree 10 (truncation < 3e-13 relative), scaling by 2^k.
-//   log:  x = 2^e * m, m in [sqrt(1/2), sqrt(2)), s = (m-1)/(m+1),
-//         log m = 2*atanh(s) = 2s * sum z^n/(2n+1), z = s^2 <= 0.0295,
-//         summed to n = 7 (truncation < 4e-14 relative).
-//   (exp and log only feed the float functions, so their double results
-//   need ~40 bits: rounding to float is then exact except within 2^-40 of a
-//   rounding boundary.)
-//   sin/cos: reduction by pi/2 (two-part constant, fma), Taylor series to
-//         degree 17/18 on |r| <= pi/4 (truncation < 1e-18).
-//
-// Only +, *, /, sqrt, fma and rint are used; all are correctly rounded
-// IEEE operations on gfx950 and on x86-64, so the results are identical on
-// both.  Errors are below one float ulp (rounding of an accurate double),
-// and the float result is the correctly rounded one on all but a few in a
-// million arguments.
+//   expf, logf (the samplers, the density models): float arithmetic, see
+//         gf_expf / gf_logf; at most 1 ulp from the correctly rounded value.
+//   exp10f, log10f, powf, sinf, cosf, tanf: evaluated in IEEE double and
+//         rounded once to float (the argument scaling of exp10/pow needs the
+//         extra precision):
+//     exp:  Cody-Waite reduction x = k*ln2 + r, |r| <= ln2/2, Taylor series
+//           of e^r to degree 10 (truncation < 3e-13 relative), scaling by 2^k.
+//     log:  x = 2^e * m, m in [sqrt(1/2), sqrt(2)), s = (m-1)/(m+1),
+//           log m = 2*atanh(s) = 2s * sum z^n/(2n+1), z = s^2 <= 0.0295,
+//           summed to n = 7 (truncation < 4e-14 relative).
+//     sin/cos: reduction by pi/2 (two-part constant, fma), Taylor series to
+//           degree 17/18 on |r| <= pi/4 (truncation < 1e-18).
+//     The float result is the correctly rounded one except within 2^-40 of
+//     a rounding boundary.
 // The double-precision sin/cos/atan/atan2/asin here (a few 1e-16 relative)
 // replace the platform's in the deprojection, direction cosines and sky
 // rotations for the same reason: the GPU and a CPU evaluation agree to the
@@ -217,8 +216,62 @@ GF_HD double asin(double x)
 } // namespace gfm
 
 // Single-precision entry points used by the kernels.
-GF_HD float gf_expf(float x) { return (float)gfm::exp((double)x); }
-GF_HD float gf_logf(float x) { return (float)gfm::log((double)x); }
+// e^x in float arithmetic: k = rint(x / ln2), r = x - k ln2 with ln2 split
+// into a 16-bit head (k * head is exact) and a tail, e^r by its Taylor
+// series to r^7/7! (|r| <= ln2/2), scaled by 2^k.  At most 1 ulp from the
+// correctly rounded value.
+GF_HD float gf_expf(float x)
+{
+	const float LN2_HI = 0.693145751953125f, LN2_LO = 1.428606765330187e-06f;
+	if (x != x) return x;
+	if (x > 88.72283935546875f) return gfm::as_float(0x7f800000u);
+	if (x < -103.97208404541016f) return 0.f;
+	float kf = __builtin_rintf(x * 1.44269502162933349609375f);
+	float r = __builtin_fmaf(-kf, LN2_HI, x);
+	r = __builtin_fmaf(-kf, LN2_LO, r);
+	float p = 1.f / 5040.f;
+	p = __builtin_fmaf(p, r, 1.f / 720.f);
+	p = __builtin_fmaf(p, r, 1.f / 120.f);
+	p = __builtin_fmaf(p, r, 1.f / 24.f);
+	p = __builtin_fmaf(p, r, 1.f / 6.f);
+	p = __builtin_fmaf(p, r, 0.5f);
+	p = __builtin_fmaf(p, r, 1.f);
+	p = __builtin_fmaf(p, r, 1.f);
+	int k = (int)kf;
+	if (k > 127) { p *= gfm::as_float(0x7f000000u); k -= 127; }
+	if (k < -126) { p *= gfm::as_float(0x00800000u); k += 126; }
+	return p * gfm::as_float((uint32_t)(k + 127) << 23);
+}
+
+// log x in float arithmetic: x = 2^e m, m in [sqrt(1/2), sqrt(2)), f = m-1
+// (exact), s = f/(2+f), log(1+f) = f - (f^2/2 - s (f^2/2 + R)) with
+// R = 2 atanh(s)/s - 2 = z (2/3 + 2z/5 + ... + 2z^4/11), z = s^2 <= 0.0295,
+// plus e ln2 (head exact, tail folded into the correction).  At most 1 ulp.
+GF_HD float gf_logf(float x)
+{
+	const float LN2_HI = 0.693145751953125f, LN2_LO = 1.428606765330187e-06f;
+	if (x != x || x < 0.f) return gfm::as_float(0x7fc00000u);
+	if (x == 0.f) return gfm::as_float(0xff800000u);
+	uint32_t ix = gfm::as_u32(x);
+	if ((ix >> 23) == 0xffu) return x;
+	int e = 0;
+	if ((ix >> 23) == 0u) { x *= gfm::as_float(0x4c000000u); ix = gfm::as_u32(x); e = -25; }
+	e += (int)(ix >> 23) - 127;
+	float m = gfm::as_float((ix & 0x7fffffu) | 0x3f800000u);
+	if (m > 1.41421353816986083984375f) { m *= 0.5f; e += 1; }
+	float f = m - 1.f;
+	float hfsq = 0.5f * f * f;
+	float s = f / (2.f + f);
+	float z = s * s;
+	float R = 2.f / 11.f;
+	R = __builtin_fmaf(R, z, 2.f / 9.f);
+	R = __builtin_fmaf(R, z, 2.f / 7.f);
+	R = __builtin_fmaf(R, z, 2.f / 5.f);
+	R = __builtin_fmaf(R, z, 2.f / 3.f);
+	R = R * z;
+	float ef = (float)e;
+	return __builtin_fmaf(ef, LN2_HI, -((hfsq - __builtin_fmaf(s, hfsq + R, ef * LN2_LO)) - f));
+}
 GF_HD float gf_exp10f(float x) { return (float)gfm::exp((double)x * gfm::LN10); }
 GF_HD float gf_log10f(float x) { return (float)(gfm::log((double)x) * gfm::INV_LN10); }
 GF_HD float gf_sinf(float x) { double s, c; gfm::sincos((double)x, &s, &c); return (float)s; }

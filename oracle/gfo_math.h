@@ -5,6 +5,8 @@
  * platform expf/logf/tanf/powf/exp10f/log10f/sinf/cosf (common/cuda_rng.h,
  * skygen/model_expDisk.h and the module kernels).  The oracle evaluates each one in
  * IEEE double and rounds once to float:
+ *   expf, logf: in float arithmetic (see gfo_expf / gfo_logf), <= 1 ulp;
+ *   the others in IEEE double, rounded once to float:
  *   exp: k = rint(x/ln2), r = x - k*ln2 (two-part ln2, fma), e^r by its
  *        Taylor series to r^10/10!, times 2^k;
  *   log: x = 2^e m, m in [1/sqrt2, sqrt2), log m = 2 atanh((m-1)/(m+1)) by
@@ -137,8 +139,55 @@ static inline double gfo_asin(double x)
 	return gfo_atan2(x, sqrt((1.0 - x) * (1.0 + x)));
 }
 
-static inline float gfo_expf(float x) { return (float)gfo_exp((double)x); }
-static inline float gfo_logf(float x) { return (float)gfo_log((double)x); }
+/* e^x in float arithmetic: k = rint(x/ln2), r = x - k*ln2 with ln2 in a
+ * 16-bit head (k*head exact) and a tail, Taylor series to r^7/7!, times 2^k */
+static inline float gfo_expf(float x)
+{
+	const float hi = 0.693145751953125f, lo = 1.428606765330187e-06f;
+	if (x != x) return x;
+	if (x > 88.72283935546875f) return INFINITY;
+	if (x < -103.97208404541016f) return 0.f;
+	float k = rintf(x * 1.44269502162933349609375f);
+	float r = fmaf(-k, hi, x);
+	r = fmaf(-k, lo, r);
+	static const float c[8] = {1.f / 5040.f, 1.f / 720.f, 1.f / 120.f, 1.f / 24.f, 1.f / 6.f, 0.5f, 1.f, 1.f};
+	float p = c[0];
+	for (int i = 1; i < 8; i++) p = fmaf(p, r, c[i]);
+	int ki = (int)k;
+	float s;
+	uint32_t u;
+	if (ki > 127) { u = 0x7f000000u; memcpy(&s, &u, 4); p *= s; ki -= 127; }
+	if (ki < -126) { u = 0x00800000u; memcpy(&s, &u, 4); p *= s; ki += 126; }
+	u = (uint32_t)(ki + 127) << 23;
+	memcpy(&s, &u, 4);
+	return p * s;
+}
+/* log x in float arithmetic: x = 2^e m, m in [1/sqrt2, sqrt2), f = m - 1,
+ * s = f/(2+f), log(1+f) = f - (f^2/2 - s (f^2/2 + R)),
+ * R = z (2/3 + 2z/5 + 2z^2/7 + 2z^3/9 + 2z^4/11), z = s^2; plus e*ln2 */
+static inline float gfo_logf(float x)
+{
+	const float hi = 0.693145751953125f, lo = 1.428606765330187e-06f;
+	if (x != x || x < 0.f) return NAN;
+	if (x == 0.f) return -INFINITY;
+	if (isinf(x)) return x;
+	int e;
+	float m = frexpf(x, &e) * 2.f;   /* x = m 2^(e-1), m in [1, 2) */
+	e -= 1;
+	if (m > 1.41421353816986083984375f) { m *= 0.5f; e += 1; }
+	float f = m - 1.f;
+	float h = 0.5f * f * f;
+	float s = f / (2.f + f);
+	float z = s * s;
+	float R = 2.f / 11.f;
+	R = fmaf(R, z, 2.f / 9.f);
+	R = fmaf(R, z, 2.f / 7.f);
+	R = fmaf(R, z, 2.f / 5.f);
+	R = fmaf(R, z, 2.f / 3.f);
+	R = R * z;
+	float ef = (float)e;
+	return fmaf(ef, hi, -((h - fmaf(s, h + R, ef * lo)) - f));
+}
 static inline float gfo_exp10f(float x) { return (float)gfo_exp((double)x * 2.30258509299404590109e+00); }
 static inline float gfo_log10f(float x) { return (float)(gfo_log((double)x) * 4.34294481903251816668e-01); }
 static inline float gfo_sinf(float x) { double s, c; gfo_sincos((double)x, &s, &c); return (float)s; }

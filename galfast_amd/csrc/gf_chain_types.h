@@ -1,0 +1,52 @@
+// Launch descriptor of the per-star module chain: module parameter blocks,
+// position planes and the row range, shared by the kernels and the host code
+// that queues them.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/galfast_hip.h"
+
+// Module parameter blocks of a chain (device copy), with the index of each
+// random-number module's position plane (-1 when the module is off).
+struct gf_chain_mods {
+	gf_FeH_data feh;
+	gf_GaussianFeH_data gfeh;
+	gf_multiples_data mult;
+	gf_Bond2010_data bond;
+	gf_photometry_data phot;
+	gf_photoerr_data perr;
+	gf_vel2pm_data v2pm;
+	gf_bitmap gfeh_comps, mult_comps, phot_comps;
+	int has_feh, has_gfeh, has_mult, has_bond, has_phot, has_perr, has_v2pm;
+	int gal2other;
+	int pl_feh, pl_gfeh, pl_bond, pl_perr;
+	int nplanes;
+};
+
+// Assign the position planes of the enabled modules.
+inline void gf_chain_planes(gf_chain_mods &m)
+{
+	int n = 0;
+	m.pl_feh = m.has_feh ? n++ : -1;
+	m.pl_gfeh = m.has_gfeh ? n++ : -1;
+	m.pl_bond = m.has_bond ? n++ : -1;
+	m.pl_perr = m.has_perr ? n++ : -1;
+	m.nplanes = n;
+}
+
+// Stream positions: plane p holds the (c, x) of row i at pos[p * pitch + i].
+struct gf_positions {
+	uint2 *pos;
+	uint64_t pitch;
+};
+
+// One chain over rows [begin, end) in step-row chunks: the module
+// parameters, the table and the position planes, passed to every launch by
+// value.
+struct gf_chain_launch {
+	gf_chain_mods m;
+	gf_table t;
+	gf_positions P;
+	uint32_t begin, step, end;
+};

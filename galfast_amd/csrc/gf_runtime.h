@@ -39,14 +39,36 @@ int exclusive_scan_u32_u64(const uint32_t *in, uint64_t *offs, uint64_t n, hipSt
 struct gf_chain_fwd;
 // gf_skygen_emit with optional timing: ev holds 3 events, ms[0] and ms[1]
 // receive the durations of the random half (k_emit_rng) and the row half
-// (k_emit_rows); both 0 on the extinction path (one fused kernel).
+// (k_emit_rows) (and make the call wait for them).  On the extinction path
+// (one fused kernel) the fused kernel's time is the first and the second is
+// 0.  unit: the plan unit holding row_begin (from
+// skygen_chunk_units), or UINT64_MAX to look it up, which waits for the
+// stream.
 int skygen_emit(const gf_draw_plan *plan, uint64_t row_begin, uint64_t row_end, const gf_footprint *foot,
-                gf_rng *rng, const gf_skygen_cols *out, hipStream_t st, hipEvent_t *ev, float *ms);
+                gf_rng *rng, const gf_skygen_cols *out, hipStream_t st, hipEvent_t *ev, float *ms, uint64_t unit);
+// units[q] = the plan unit holding row q * chunk, q < nq (waits for the stream).
+int skygen_chunk_units(const gf_draw_plan *plan, uint64_t chunk, uint32_t nq, uint64_t *units, hipStream_t st);
 
 // ev: GF_CHAIN_LAUNCHES + 1 events (or NULL); ms: GF_CHAIN_LAUNCHES
 // durations (draw pass, then the row pass's groups)
 int module_chain(const gf_rows *ks, const gf_chain *chain, gf_rng *rng, const gf_table *t, hipStream_t st,
                  hipEvent_t *ev, float *ms);
+} // namespace gf
+
+// The module chain in separate launches (launch 0: the draw pass, which
+// advances the streams; launches 1..4: the row pass's groups, which only
+// read the streams' multipliers and may run on another HIP stream once the
+// draw pass of their rows is done).  pos: the position planes
+// (chain_positions_bytes), or NULL for a per-thread scratch buffer.
+struct gf_chain_launch;
+namespace gf {
+size_t chain_positions_bytes(const gf_chain *chain, uint32_t nrows);
+int chain_setup(const gf_rows *ks, const gf_chain *chain, const gf_rng *rng, const gf_table *t, uint2 *pos,
+                gf_chain_launch &L);
+bool chain_on(const gf_chain_launch &L, int launch);
+int chain_launch(const gf_chain_launch &L, const gf_rng *rng, int launch, hipStream_t st);
+// Adds the number of rows [0, n) with hidden == 0 to *counter (device memory).
+int count_visible(gf_col hidden, uint64_t n, unsigned long long *counter, hipStream_t st);
 
 } // namespace gf
 
@@ -61,5 +83,6 @@ enum gf_scratch_slot {
 	GF_SLOT_SAMPLE,
 	GF_SLOT_COUNT_W,
 	GF_SLOT_PARK,
+	GF_SLOT_UNITS,
 	GF_SLOT_NSLOTS
 };

@@ -5,6 +5,7 @@
 // device, and runs count -> draw -> module chain in output chunks.
 #include "host.h"
 #include "../gf_runtime.h"
+#include "../gf_chain_types.h"
 
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -162,18 +163,42 @@ struct gf_catalog {
 
 	// device state
 	gf_rng rng{};
-	DevBuf table_dev;
-	uint64_t table_cap = 0;
-	gf_table table{};
-	hipStream_t stream = nullptr;
+	hipStream_t stream = nullptr;    // stream-order work: plans, emission, draw passes
+	hipStream_t rows = nullptr;      // the module chain's row passes
 	hipEvent_t ev[2] = {nullptr, nullptr};
-	hipEvent_t sev[GF_CHAIN_LAUNCHES + 1] = {};
+	// Two output chunks are in flight: while the row passes of chunk k run
+	// on `rows`, chunk k+1 is emitted and drawn on `stream` into the other
+	// slot.
+	struct Slot {
+		DevBuf table_dev;
+		uint64_t cap = 0;
+		gf_table table{};
+		hipEvent_t done = nullptr;        // the chunk's last launch (on `rows`)
+		std::vector<hipEvent_t> ev;       // timing events of the chunk's launches
+		bool pending = false;
+		uint64_t n = 0;
+		int comp = 0, nsub = 0;
+	} slot[2];
+	// Position planes of the draw passes, alternating per chain call: the
+	// draw pass into buffer b waits for the row passes that read it before.
+	DevBuf pos[2];
+	hipEvent_t drawn[2] = {}, pos_free[2] = {};
+	DevBuf vis_dev;                            // visible-row counter per slot
+	unsigned long long *vis_host = nullptr;    // pinned copies of the counters
+	int overlap = 1;
 
 	~gf_catalog()
 	{
 		if (rng.state) gf_rng_free(&rng);
 		for (auto e : ev) if (e) (void)hipEventDestroy(e);
-		for (auto e : sev) if (e) (void)hipEventDestroy(e);
+		for (auto &sl : slot) {
+			if (sl.done) (void)hipEventDestroy(sl.done);
+			for (auto e : sl.ev) (void)hipEventDestroy(e);
+		}
+		for (auto e : drawn) if (e) (void)hipEventDestroy(e);
+		for (auto e : pos_free) if (e) (void)hipEventDestroy(e);
+		if (vis_host) (void)hipHostFree(vis_host);
+		if (rows) (void)hipStreamDestroy(rows);
 	}
 };
 
@@ -884,10 +909,21 @@ void open_catalog(gf_catalog &c, const std::string &cmdfile)
 	if (c.has_feh && c.has_gfeh) fail("FeH and GaussianFeH are both configured");
 
 	c.pix_dev.upload(c.pixels);
-	if (hipStreamCreate(&c.stream) != hipSuccess) fail("stream creation failed");
+	if (hipStreamCreate(&c.stream) != hipSuccess || hipStreamCreate(&c.rows) != hipSuccess)
+		fail("stream creation failed");
 	(void)hipEventCreate(&c.ev[0]);
 	(void)hipEventCreate(&c.ev[1]);
-	for (auto &e : c.sev) (void)hipEventCreate(&e);
+	for (auto &sl : c.slot)
+		if (hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) fail("event creation failed");
+	for (int b = 0; b < 2; b++)
+		if (hipEventCreateWithFlags(&c.drawn[b], hipEventDisableTiming) != hipSuccess ||
+		    hipEventCreateWithFlags(&c.pos_free[b], hipEventDisableTiming) != hipSuccess)
+			fail("event creation failed");
+	for (int b = 0; b < 2; b++)   // the position buffers start free
+		if (hipEventRecord(c.pos_free[b], c.rows) != hipSuccess) fail("event record failed");
+	c.vis_dev.alloc(2 * sizeof(unsigned long long));
+	if (hipHostMalloc((void **)&c.vis_host, 2 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+		fail("pinned allocation failed");
 	// persistent streams seeded from the pipeline seed (common/cux.cpp:404-435)
 	std::string pf = c.datadir + "/safeprimes32.u32";
 	std::ifstream f(pf, std::ios::binary);
@@ -932,9 +968,9 @@ void open_catalog(gf_catalog &c, const std::string &cmdfile)
 }
 
 // Allocate the device catalog table for `cap` rows.
-void ensure_table(gf_catalog &c, uint64_t cap)
+void ensure_table(gf_catalog &c, gf_catalog::Slot &sl, uint64_t cap)
 {
-	if (c.table_cap >= cap && c.table_dev.p) return;
+	if (sl.cap >= cap && sl.table_dev.p) return;
 	struct ColDef { gf_col gf_table::*m; uint32_t elem, width; };
 	int nb = c.nbands > 0 ? c.nbands : 1;
 	ColDef defs[] = {
@@ -950,20 +986,197 @@ void ensure_table(gf_catalog &c, uint64_t cap)
 		size_t pitch = ((cap * d.elem + 127) / 128) * 128;
 		total += pitch * d.width;
 	}
-	c.table_dev.alloc(total);
-	char *p = (char *)c.table_dev.p;
+	sl.table_dev.alloc(total);
+	char *p = (char *)sl.table_dev.p;
 	for (auto &d : defs) {
 		uint32_t pitch = (uint32_t)(((cap * d.elem + 127) / 128) * 128);
 		gf_col col{p, pitch, d.width, d.elem, 0};
-		c.table.*(d.m) = col;
+		sl.table.*(d.m) = col;
 		p += (size_t)pitch * d.width;
 	}
-	c.table_cap = cap;
+	sl.cap = cap;
 }
 
 double now_s()
 {
 	return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Events of one chunk: 3 around emission (on the catalog stream), then per
+// chain call 2 around the draw pass (catalog stream) and
+// GF_CHAIN_LAUNCHES around the row passes (row stream).
+constexpr int EV_EMIT = 3, EV_CALL = 2 + GF_CHAIN_LAUNCHES;
+
+void ensure_events(gf_catalog::Slot &sl, int ncalls)
+{
+	size_t want = EV_EMIT + (size_t)EV_CALL * ncalls;
+	while (sl.ev.size() < want) {
+		hipEvent_t e;
+		if (hipEventCreate(&e) != hipSuccess) fail("event creation failed");
+		sl.ev.push_back(e);
+	}
+}
+
+double elapsed_s(hipEvent_t a, hipEvent_t b)
+{
+	float ms = 0.f;
+	(void)hipEventElapsedTime(&ms, a, b);
+	return ms * 1e-3;
+}
+
+// Waits for the chunk in slot si, adds its visible rows and launch times to
+// the statistics and hands its columns to the callback.
+int deliver(gf_catalog &c, int si, gf_chunk_fn fn, void *user, gf_run_stats &s)
+{
+	gf_catalog::Slot &sl = c.slot[si];
+	GF_HIP(hipEventSynchronize(sl.done));
+	sl.pending = false;
+	s.visible += c.vis_host[si];
+	const hipEvent_t *e = sl.ev.data();
+	double a = elapsed_s(e[0], e[1]), b = elapsed_s(e[1], e[2]);
+	s.t_emit_rng += a;
+	s.t_emit_rows += b;
+	s.t_emit += a + b;
+	for (int j = 0; j < sl.nsub; j++) {
+		const hipEvent_t *q = e + EV_EMIT + EV_CALL * j;
+		double d = elapsed_s(q[0], q[1]);
+		s.t_stage[0] += d;
+		s.t_chain += d;
+		for (int g = 1; g < GF_CHAIN_LAUNCHES; g++) {
+			double r = elapsed_s(q[g + 1], q[g + 2]);
+			s.t_stage[g] += r;
+			s.t_chain += r;
+		}
+	}
+	if (fn) return fn(user, &sl.table, sl.n, sl.comp);
+	return 0;
+}
+
+// The catalog in output chunks.  Stream order: emission and the draw passes
+// advance the RNG streams and run in order on c.stream; the row passes only
+// read the streams' multipliers and run on c.rows as soon as the draw pass
+// of their rows is done, so that they overlap the next chunk's emission and
+// draws (the draw-type kernels run one lane per stream and leave most of the
+// device idle).  Two chunk slots: chunk k is handed to the callback after
+// chunk k+1 has been queued, and before chunk k+2 overwrites its slot.
+int run_catalog(gf_catalog &c, gf_chunk_fn fn, void *user, gf_run_stats *stats)
+{
+	gf_run_stats s{};
+	double t0 = now_s();
+	hipStream_t A = c.stream, B = c.overlap ? c.rows : c.stream;
+	gf_chain ch{};
+	if (c.has_feh) ch.feh = &c.feh;
+	if (c.has_gfeh) { ch.gfeh = &c.gfeh; ch.gfeh_comps = c.gfeh_comps; }
+	if (c.has_mult) { ch.mult = &c.mult; ch.mult_comps = c.mult_comps; }
+	if (c.has_bond) ch.bond = &c.bond;
+	if (c.has_phot) { ch.phot = &c.phot; ch.phot_comps = c.phot_comps; }
+	if (c.has_perr) ch.perr = &c.perr;
+	if (c.has_v2pm) ch.v2pm = &c.v2pm;
+	ch.gal2other_coordsys = c.gal2other;
+	const uint64_t sub = 65536ull * 256ull;   // rows the 65,536 streams cover per chain call
+	const size_t pos_bytes = gf::chain_positions_bytes(&ch, (uint32_t)sub);
+	for (auto &pb : c.pos)
+		if (pos_bytes && pb.bytes < pos_bytes) pb.alloc(pos_bytes);
+	unsigned long long *vis_dev = (unsigned long long *)c.vis_dev.p;
+	uint32_t pb = 0;      // position buffer of the next chain call
+	uint64_t k = 0;       // chunks queued
+	int rc;
+	for (auto &mp : c.models) {
+		Model &m = *mp;
+		if (!m.wpk.p) return GF_E_STATE;
+		gf_draw_plan *plan = nullptr;
+		uint64_t total = 0;
+		GF_HIP(hipEventRecord(c.ev[0], A));
+		rc = gf_skygen_plan(&c.sp, &m.m, (const gf_pencil_beam *)c.pix_dev.p, c.has_ext ? &c.ext : nullptr,
+		                    (const double *)m.wpk.p, &c.rng, &plan, &total, A);
+		if (rc) return rc;
+		// the plan is read by the emission kernels on the catalog stream
+		auto release = [A](gf_draw_plan *p) {
+			(void)hipStreamSynchronize(A);
+			gf_skygen_plan_free(p);
+		};
+		std::unique_ptr<gf_draw_plan, decltype(release)> hold(plan, release);
+		GF_HIP(hipEventRecord(c.ev[1], A));
+		GF_HIP(hipEventSynchronize(c.ev[1]));
+		s.t_plan += elapsed_s(c.ev[0], c.ev[1]);
+		s.generated += total;
+		s.expected += m.counts.nstarsExpectedToGenerate;
+		if (!total) continue;
+		const uint64_t chunk = std::min<uint64_t>(c.chunk_rows, total);
+		GF_CHECK(chunk < (1ull << 32), GF_E_ARG);
+		const uint64_t nchunks = (total + chunk - 1) / chunk;
+		GF_CHECK(nchunks < (1ull << 32), GF_E_ARG);
+		std::vector<uint64_t> units(nchunks);
+		rc = gf::skygen_chunk_units(plan, chunk, (uint32_t)nchunks, units.data(), A);
+		if (rc) return rc;
+		for (uint64_t q = 0; q < nchunks; q++, k++) {
+			const int si = (int)(k & 1);
+			gf_catalog::Slot &sl = c.slot[si];
+			if (sl.pending) {
+				rc = deliver(c, si, fn, user, s);
+				if (rc) return rc;
+			}
+			const uint64_t r0 = q * chunk, r1 = std::min(total, r0 + chunk), n = r1 - r0;
+			const int nsub = (int)((n + sub - 1) / sub);
+			ensure_table(c, sl, chunk);
+			ensure_events(sl, nsub);
+			const hipEvent_t *e = sl.ev.data();
+			gf_skygen_cols sc{sl.table.lb, sl.table.projIdx, sl.table.projXY, sl.table.DM, sl.table.absmag,
+			                  sl.table.XYZ, sl.table.Am, sl.table.AmInf, sl.table.comp, sl.table.hidden};
+			rc = gf::skygen_emit(plan, r0, r1, &c.foot.f, &c.rng, &sc, A, sl.ev.data(), nullptr, units[q]);
+			if (rc) return rc;
+			s.n_emit++;
+			for (int j = 0; j < nsub; j++, pb ^= 1) {
+				const uint64_t b0 = (uint64_t)j * sub;
+				gf_rows ks{(uint32_t)b0, 256, (uint32_t)std::min(n, b0 + sub), 0};
+				gf_chain_launch L;
+				rc = gf::chain_setup(&ks, &ch, &c.rng, &sl.table, pos_bytes ? (uint2 *)c.pos[pb].p : nullptr, L);
+				if (rc) return rc;
+				const hipEvent_t *qe = e + EV_EMIT + EV_CALL * j;
+				// the draw pass overwrites the planes the row passes two calls back read
+				GF_HIP(hipStreamWaitEvent(A, c.pos_free[pb], 0));
+				GF_HIP(hipEventRecord(qe[0], A));
+				rc = gf::chain_launch(L, &c.rng, 0, A);
+				if (rc) return rc;
+				GF_HIP(hipEventRecord(qe[1], A));
+				GF_HIP(hipEventRecord(c.drawn[pb], A));
+				GF_HIP(hipStreamWaitEvent(B, c.drawn[pb], 0));
+				for (int g = 1; g < GF_CHAIN_LAUNCHES; g++) {
+					GF_HIP(hipEventRecord(qe[g + 1], B));
+					rc = gf::chain_launch(L, &c.rng, g, B);
+					if (rc) return rc;
+				}
+				GF_HIP(hipEventRecord(qe[GF_CHAIN_LAUNCHES + 1], B));
+				GF_HIP(hipEventRecord(c.pos_free[pb], B));
+				s.n_chain++;
+			}
+			s.rows_chain += n;
+			GF_HIP(hipMemsetAsync(vis_dev + si, 0, sizeof(unsigned long long), B));
+			rc = gf::count_visible(sl.table.hidden, n, vis_dev + si, B);
+			if (rc) return rc;
+			GF_HIP(hipMemcpyAsync(c.vis_host + si, vis_dev + si, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+			                      B));
+			GF_HIP(hipEventRecord(sl.done, B));
+			// the next chunk's emission rewrites the rows' inputs only in the
+			// other slot; the streams are not touched by the row passes
+			sl.pending = true;
+			sl.n = n;
+			sl.comp = m.m.comp;
+			sl.nsub = nsub;
+		}
+	}
+	for (uint64_t q = 0; q < 2; q++, k++) {
+		const int si = (int)(k & 1);
+		if (c.slot[si].pending) {
+			rc = deliver(c, si, fn, user, s);
+			if (rc) return rc;
+		}
+	}
+	GF_HIP(hipStreamSynchronize(A));
+	GF_HIP(hipStreamSynchronize(B));
+	s.t_total = now_s() - t0;
+	if (stats) *stats = s;
+	return 0;
 }
 
 } // namespace
@@ -993,6 +1206,7 @@ int gf_catalog_set(gf_catalog *c, const char *key, double value)
 	GF_CHECK(c && key, GF_E_ARG);
 	std::string k = key;
 	if (k == "chunk_rows") c->chunk_rows = (uint64_t)value;
+	else if (k == "overlap") c->overlap = value != 0.;
 	else if (k == "acc_mode") c->acc_mode = (int)value;
 	else if (k == "nstars") c->nstars_forced = value;
 	else return GF_E_ARG;
@@ -1050,83 +1264,20 @@ int gf_catalog_count(gf_catalog *c, gf_count_result *per_model, double *expected
 int gf_catalog_run(gf_catalog *c, gf_chunk_fn fn, void *user, gf_run_stats *stats)
 {
 	GF_CHECK(c, GF_E_ARG);
-	gf_run_stats s{};
-	double t0 = now_s();
-	hipStream_t st = c->stream;
-	gf_chain ch{};
-	if (c->has_feh) ch.feh = &c->feh;
-	if (c->has_gfeh) { ch.gfeh = &c->gfeh; ch.gfeh_comps = c->gfeh_comps; }
-	if (c->has_mult) { ch.mult = &c->mult; ch.mult_comps = c->mult_comps; }
-	if (c->has_bond) ch.bond = &c->bond;
-	if (c->has_phot) { ch.phot = &c->phot; ch.phot_comps = c->phot_comps; }
-	if (c->has_perr) ch.perr = &c->perr;
-	if (c->has_v2pm) ch.v2pm = &c->v2pm;
-	ch.gal2other_coordsys = c->gal2other;
-	const uint64_t sub = 65536ull * 256ull;   // rows the 65,536 streams cover per call
-	for (auto &mp : c->models) {
-		Model &m = *mp;
-		if (!m.wpk.p) return GF_E_STATE;
-		gf_draw_plan *plan = nullptr;
-		uint64_t total = 0;
-		(void)hipEventRecord(c->ev[0], st);
-		int rc = gf_skygen_plan(&c->sp, &m.m, (const gf_pencil_beam *)c->pix_dev.p, c->has_ext ? &c->ext : nullptr,
-		                        (const double *)m.wpk.p,
-		                        &c->rng, &plan, &total, st);
-		if (rc) return rc;
-		(void)hipEventRecord(c->ev[1], st);
-		(void)hipEventSynchronize(c->ev[1]);
-		float ms = 0;
-		(void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
-		s.t_plan += ms * 1e-3;
-		s.generated += total;
-		s.expected += m.counts.nstarsExpectedToGenerate;
-		uint64_t chunk = std::min<uint64_t>(c->chunk_rows, std::max<uint64_t>(total, 1));
-		ensure_table(*c, chunk);
-		gf_skygen_cols sc{c->table.lb, c->table.projIdx, c->table.projXY, c->table.DM, c->table.absmag,
-		                  c->table.XYZ, c->table.Am, c->table.AmInf, c->table.comp, c->table.hidden};
-		for (uint64_t r0 = 0; r0 < total; r0 += chunk) {
-			uint64_t r1 = std::min(total, r0 + chunk);
-			(void)hipEventRecord(c->ev[0], st);
-			float ems[2];
-			rc = gf::skygen_emit(plan, r0, r1, &c->foot.f, &c->rng, &sc, st, c->sev, ems);
-			if (rc) { gf_skygen_plan_free(plan); return rc; }
-			s.t_emit_rng += ems[0] * 1e-3;
-			s.t_emit_rows += ems[1] * 1e-3;
-			(void)hipEventRecord(c->ev[1], st);
-			(void)hipEventSynchronize(c->ev[1]);
-			(void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
-			s.t_emit += ms * 1e-3;
-			s.n_emit++;
-			uint64_t n = r1 - r0;
-			(void)hipEventRecord(c->ev[0], st);
-			for (uint64_t b0 = 0; b0 < n; b0 += sub) {
-				gf_rows ks{(uint32_t)b0, 256, (uint32_t)std::min(n, b0 + sub), 0};
-				float sms[GF_CHAIN_LAUNCHES];
-				rc = gf::module_chain(&ks, &ch, &c->rng, &c->table, st, c->sev, sms);
-				if (rc) { gf_skygen_plan_free(plan); return rc; }
-				for (int q = 0; q < GF_CHAIN_LAUNCHES; q++) s.t_stage[q] += sms[q] * 1e-3;
-				s.n_chain++;
-			}
-			(void)hipEventRecord(c->ev[1], st);
-			(void)hipEventSynchronize(c->ev[1]);
-			(void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
-			s.t_chain += ms * 1e-3;
-			s.rows_chain += n;
-			uint64_t vis = 0;
-			rc = gf_count_visible(c->table.hidden, n, &vis, st);
-			if (rc) { gf_skygen_plan_free(plan); return rc; }
-			s.visible += vis;
-			if (fn) {
-				rc = fn(user, &c->table, n, m.m.comp);
-				if (rc) { gf_skygen_plan_free(plan); return rc; }
-			}
-		}
-		gf_skygen_plan_free(plan);
+	int rc;
+	try {
+		rc = run_catalog(*c, fn, user, stats);
+	} catch (const ConfigError &e) {
+		g_last_error = e.msg;
+		rc = GF_E_CONFIG;
 	}
-	GF_HIP(hipStreamSynchronize(st));
-	s.t_total = now_s() - t0;
-	if (stats) *stats = s;
-	return 0;
+	if (rc) {
+		// drop the chunks in flight
+		(void)hipStreamSynchronize(c->stream);
+		(void)hipStreamSynchronize(c->rows);
+		for (auto &sl : c->slot) sl.pending = false;
+	}
+	return rc;
 }
 
 int gf_catalog_host_view_get(const gf_catalog *c, gf_catalog_host_view *v)

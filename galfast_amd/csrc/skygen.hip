@@ -498,6 +498,20 @@ __global__ void k_find_unit(const uint64_t *offs, uint64_t U, uint64_t row, uint
 	*res = a;
 }
 
+// The unit of row q * chunk for q < nq (the first unit of every output chunk).
+__global__ void k_find_units(const uint64_t *offs, uint64_t U, uint64_t chunk, uint32_t nq, uint64_t *res)
+{
+	uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+	if (q >= nq) return;
+	uint64_t row = (uint64_t)q * chunk;
+	uint64_t a = 0, b = U;
+	while (b - a > 1) {
+		uint64_t mid = (a + b) / 2;
+		if (offs[mid] <= row) a = mid; else b = mid;
+	}
+	res[q] = a;
+}
+
 } // namespace
 
 struct gf_draw_plan {
@@ -637,34 +651,61 @@ int gf_skygen_emit(const gf_draw_plan *plan, uint64_t row_begin, uint64_t row_en
                    const gf_footprint *foot, gf_rng *rng,
                    const gf_skygen_cols *out, void *stream)
 {
-	return gf::skygen_emit(plan, row_begin, row_end, foot, rng, out, gf::as_stream(stream), nullptr, nullptr);
+	return gf::skygen_emit(plan, row_begin, row_end, foot, rng, out, gf::as_stream(stream), nullptr, nullptr,
+	                       UINT64_MAX);
 }
 
 } // extern "C"
 
 namespace gf {
 
+int skygen_chunk_units(const gf_draw_plan *plan, uint64_t chunk, uint32_t nq, uint64_t *units, hipStream_t st)
+{
+	GF_CHECK(plan && units && chunk > 0, GF_E_ARG);
+	if (!nq) return 0;
+	void *p;
+	int rc = gf::scratch(GF_SLOT_UNITS, (size_t)nq * sizeof(uint64_t), &p);
+	if (rc) return rc;
+	k_find_units<<<gf::blocks_for(nq, 256), 256, 0, st>>>(plan->offs, plan->U, chunk, nq, (uint64_t *)p);
+	GF_LAUNCHED();
+	GF_HIP(hipMemcpyAsync(units, p, (size_t)nq * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+	GF_HIP(hipStreamSynchronize(st));
+	return 0;
+}
+
 int skygen_emit(const gf_draw_plan *plan, uint64_t row_begin, uint64_t row_end, const gf_footprint *foot,
-                gf_rng *rng, const gf_skygen_cols *out, hipStream_t st, hipEvent_t *ev, float *ms)
+                gf_rng *rng, const gf_skygen_cols *out, hipStream_t st, hipEvent_t *ev, float *ms, uint64_t unit)
 {
 	GF_CHECK(plan && rng && rng->state && out && row_begin <= row_end && row_end <= plan->total, GF_E_ARG);
 	if (ms) ms[0] = ms[1] = 0.f;
 	if (row_begin == row_end) return 0;
 	gf_footprint fp{};
 	if (foot) fp = *foot;
-	k_find_unit<<<1, 1, 0, st>>>(plan->offs, plan->U, row_begin, plan->tmp);
-	GF_LAUNCHED();
-	uint64_t u_lo = 0;
-	GF_HIP(hipMemcpyAsync(&u_lo, plan->tmp, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-	GF_HIP(hipStreamSynchronize(st));
+	uint64_t u_lo = unit;
+	if (u_lo == UINT64_MAX) {
+		k_find_unit<<<1, 1, 0, st>>>(plan->offs, plan->U, row_begin, plan->tmp);
+		GF_LAUNCHED();
+		GF_HIP(hipMemcpyAsync(&u_lo, plan->tmp, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+		GF_HIP(hipStreamSynchronize(st));
+	}
+	GF_CHECK(u_lo < plan->U, GF_E_ARG);
 	size_t shb = ((size_t)plan->sp.nM + 1) * sizeof(double);
 	const size_t shb_guess = shb + LF_GUESS * sizeof(uint16_t);
 	if (plan->ext.beam.data || plan->ext.north.data || plan->ext.south.data) {
 		// with extinction the star needs its DM bin's minimum extinction: one pass
+		if (ev) GF_HIP(hipEventRecord(ev[0], st));
 		k_emit<<<gf::blocks_for(rng->nstreams, SK_BS), SK_BS, shb, st>>>(
 		    plan->sp, plan->model, plan->pixels, plan->ext, fp, plan->P, plan->offs, plan->U, row_begin, row_end,
 		    u_lo, rng->state, rng->nstreams, *out);
 		GF_LAUNCHED();
+		if (ev) {
+			GF_HIP(hipEventRecord(ev[1], st));
+			GF_HIP(hipEventRecord(ev[2], st));
+			if (ms) {
+				GF_HIP(hipEventSynchronize(ev[2]));
+				GF_HIP(hipEventElapsedTime(&ms[0], ev[0], ev[1]));
+			}
+		}
 		return 0;
 	}
 	uint64_t n = row_end - row_begin;

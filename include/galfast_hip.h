@@ -475,14 +475,19 @@ typedef struct gf_run_stats {
     double   t_emit_rng, t_emit_rows;   /* the two launches inside t_emit */
 } gf_run_stats;
 
-/* Called once per output chunk with device columns holding `nrows` rows
- * (hidden rows included; hidden != 0 marks rows the reference drops). */
+/* Called once per output chunk, in row order, with device columns holding
+ * `nrows` rows (hidden rows included; hidden != 0 marks rows the reference
+ * drops).  The work that wrote them has completed; the columns stay valid
+ * until the callback returns (the run may already be computing the next
+ * chunk into other buffers). */
 typedef int (*gf_chunk_fn)(void *user, const gf_table *cols, uint64_t nrows, int comp);
 
 /* Message of the last GF_E_CONFIG failure on this thread. */
 const char *gf_last_error(void);
 int  gf_catalog_open(const char *cmd_conf, uint32_t seed, const char *datadir, gf_catalog **out);
-/* Keys: "chunk_rows" (rows per output chunk, default 3e7), "acc_mode"
+/* Keys: "chunk_rows" (rows per output chunk, default 3e7), "overlap" (1,
+ * the default: the module chain's row passes run on a second HIP stream
+ * while the next rows are emitted and drawn; 0: one stream), "acc_mode"
  * (GF_ACC_*), "nstars" (force the expected total, os_skygen.cpp:886-901). */
 int  gf_catalog_set(gf_catalog *c, const char *key, double value);
 int  gf_catalog_info(const gf_catalog *c, gf_catalog_info_t *info);

@@ -11,6 +11,14 @@ over ``bench.py --copy-calibration`` (a plain 1 GiB device copy), which fix
 the factor between each counter and bytes.  Writes <tag>_kernel_stats.md,
 <tag>_traffic.md (when the counter runs are given) and traffic.json (HBM
 bytes per launch of each kernel, read by bench.py for roofline.traffic).
+
+--pmc DIR... are further counter runs of the same command (any SQ_* /
+GRBM_GUI_ACTIVE counters, at most 8 SQ counters a run); with SQ_WAVES and
+SQ_INSTS_VALU among them they give <tag>_instruction_mix.md and valu.json:
+the vector instructions each launch issues and, over the kernel-trace
+duration, the issue rate against the device's: 256 CUs x 4 SIMDs, one
+wave64 vector instruction per SIMD every 4 cycles (16 lanes a cycle) at
+2.4 GHz = 614.4 G wave-instructions/s.
 """
 import argparse
 import csv
@@ -20,6 +28,7 @@ import os
 import re
 
 COPY_BYTES = 1 << 30            # bench.py --copy-calibration copies 1 GiB
+PEAK_VALU_GINSTS = 256 * 4 * 2.4 / 4   # G wave64 vector instructions per second
 ROW_GROUPS = {"0": "KIN", "1": "PHOT", "2": "PERR", "3": "ASTRO"}
 
 
@@ -58,15 +67,59 @@ def kernel_stats(d):
     return out
 
 
-def counter_per_dispatch(d, counter):
+def counter_per_dispatch(d, counter, reduce=None):
+    """{kernel: (mean over dispatches, dispatches)} of a counter summed (or,
+    for GRBM_GUI_ACTIVE, the maximum) over the instances of one dispatch."""
+    reduce = reduce or ("max" if counter.startswith("GRBM_") else "sum")
     acc = {}
     for r in _csv(d, "counter_collection.csv"):
         if r["Counter_Name"] != counter:
             continue
         k = short_name(r["Kernel_Name"])
         e = acc.setdefault(k, {})
-        e[r["Dispatch_Id"]] = e.get(r["Dispatch_Id"], 0.0) + float(r["Counter_Value"])
+        v = float(r["Counter_Value"])
+        i = r["Dispatch_Id"]
+        e[i] = (max(e[i], v) if reduce == "max" else e[i] + v) if i in e else v
     return {k: (sum(v.values()) / len(v), len(v)) for k, v in acc.items()}
+
+
+def counters_in(d):
+    return sorted({r["Counter_Name"] for r in _csv(d, "counter_collection.csv")})
+
+
+def instruction_mix(a, ks, kt_cmd):
+    per = {}
+    pmc_cmds = []
+    for d in a.pmc:
+        names = counters_in(d)
+        pmc_cmds.append("rocprofv3 --pmc %s --output-format csv -d %s -o pmc -- %s" % (" ".join(names), d, a.program))
+        for c in names:
+            for k, (v, n) in counter_per_dispatch(d, c).items():
+                per.setdefault(k, {})[c] = v
+    kernels = [k for k in sorted(ks, key=lambda k: -ks[k]["total_ns"]) if k in per and per[k].get("SQ_WAVES")]
+    classes = [c for c in sorted({c for k in kernels for c in per[k]}) if c.startswith("SQ_INSTS_")]
+    lines = ["# Instruction mix per launch: %s" % a.tag, "", "Commands:", ""] + ["* `%s`" % c for c in pmc_cmds] + \
+            ["* durations: `%s`" % kt_cmd, "",
+             "Issue rate = SQ_INSTS_VALU per launch / mean kernel-trace duration, against 256 CUs x 4 SIMDs x "
+             "2.4 GHz / 4 cycles per wave64 vector instruction = %.1f G/s.  Counts are per wave (64 lanes)." %
+             PEAK_VALU_GINSTS, "",
+             "| kernel | waves / launch | VALU / launch | VALU issue G/s | of peak | " +
+             " | ".join(c.replace("SQ_INSTS_", "") + " / wave" for c in classes) + " |",
+             "|---|---|---|---|---|" + "---|" * len(classes)]
+    out = {"_peak_G_wave_insts_per_s": PEAK_VALU_GINSTS, "_program": a.program}
+    for k in kernels:
+        e = per[k]
+        w = e["SQ_WAVES"]
+        valu = e.get("SQ_INSTS_VALU", 0.0)
+        avg = ks[k]["avg_ns"]
+        rate = valu / avg if avg else 0.0
+        out[k] = {"waves": w, "valu_insts": valu, "avg_ns": avg}
+        lines.append("| `%s` | %.0f | %.3g | %.0f | %.0f%% | " % (k, w, valu, rate, 100.0 * rate / PEAK_VALU_GINSTS) +
+                     " | ".join("%.1f" % (e.get(c, 0.0) / w) for c in classes) + " |")
+    with open(os.path.join(a.out, "%s_instruction_mix.md" % a.tag), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    with open(os.path.join(a.out, "valu.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
 
 
 def main():
@@ -80,6 +133,7 @@ def main():
     ap.add_argument("--tag", required=True)
     ap.add_argument("--program", required=True, help="the profiled program (after rocprofv3's --)")
     ap.add_argument("--workload", default="")
+    ap.add_argument("--pmc", nargs="*", default=[])
     a = ap.parse_args()
     os.makedirs(a.out, exist_ok=True)
 
@@ -95,6 +149,8 @@ def main():
                                                             100.0 * e["total_ns"] / total))
     with open(os.path.join(a.out, "%s_kernel_stats.md" % a.tag), "w") as f:
         f.write("\n".join(lines) + "\n")
+    if a.pmc:
+        instruction_mix(a, ks, kt_cmd)
 
     if not (a.fetch and a.write and a.copy_fetch and a.copy_write):
         return

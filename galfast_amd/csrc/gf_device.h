@@ -175,7 +175,7 @@ GF_D void cell_pos(const gf_skygen_params &sp, const gf_extinction &ext, const g
 {
 	float m = sp.m0 + im * sp.dm;
 	float DM = m - M;
-	D = gf_powf(10.f, 0.2f * DM + 1.f);
+	D = gf_pow10f(0.2f * DM + 1.f);
 	int xidx = pix.extIdx / 2048;
 	int yidx = pix.extIdx % 2048;
 	Am = ext.beam.data ? tex3d(ext.beam, (float)xidx, (float)yidx, DM) : 0.f;
@@ -282,7 +282,7 @@ GF_D void write_star(const gf_skygen_params &sp, int comp, const gf_extinction &
 	if (l > 360.) l -= 360.;
 	b *= DBL_R2D;
 	float DM = mtmp - Mtmp;
-	float D = gf_powf(10.f, 0.2f * DM + 1.f);
+	float D = gf_pow10f(0.2f * DM + 1.f);
 	float X = D * dcl * dcb, Y = D * dsl * dcb, Z = D * dsb;
 	float Am0 = 0.f, Am1 = 0.f;
 	const gf_tex3d &em = pix.projIdx == 0 ? ext.north : ext.south;

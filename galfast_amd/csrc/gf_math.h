@@ -156,31 +156,37 @@ GF_HD void sincos(double x, double *sn, double *cs)
 GF_HD double sin(double x) { double s, c; sincos(x, &s, &c); return s; }
 GF_HD double cos(double x) { double s, c; sincos(x, &s, &c); return c; }
 
-// atan(x) for double x: two half-angle reductions
-// atan(x) = 2 atan(x / (1 + sqrt(1 + x^2))) bring |x| <= 1 below tan(pi/16),
-// then the Taylor series to x^25; |x| > 1 uses pi/2 - atan(1/x).
-GF_HD double atan(double x)
+// atan(k/8), k = 0..8, each rounded to the nearest double (computed with
+// 200-bit arithmetic).
+static constexpr double ATAN_K8[9] = {
+	0.0, 0x1.fd5ba9aac2f6ep-4, 0x1.f5b75f92c80ddp-3, 0x1.6f61941e4def1p-2, 0x1.dac670561bb4fp-2,
+	0x1.1e00babdefeb4p-1, 0x1.4978fa3269ee1p-1, 0x1.700a7c5784634p-1, 0x1.921fb54442d18p-1};
+
+// atan t for 0 <= t <= 1: c = k/8 nearest t, atan t = atan c + atan u with
+// u = (t - c) / (1 + t c) (t - c is exact, |u| <= 1/16), atan u by its
+// Taylor series to u^13/13 (truncation < 6e-20).  One division.
+GF_HD double atan_unit(double t)
 {
-	if (x != x) return x;
-	double ax = x < 0 ? -x : x;
-	bool inv = ax > 1.0;
-	double t = inv ? 1.0 / ax : ax;
-	t = t / (1.0 + __builtin_sqrt(dfma(t, t, 1.0)));
-	t = t / (1.0 + __builtin_sqrt(dfma(t, t, 1.0)));
-	double z = t * t;
-	double p = 1.0 / 25.0;
-	p = dfma(p, z, -1.0 / 23.0);
-	p = dfma(p, z, 1.0 / 21.0);
-	p = dfma(p, z, -1.0 / 19.0);
-	p = dfma(p, z, 1.0 / 17.0);
-	p = dfma(p, z, -1.0 / 15.0);
-	p = dfma(p, z, 1.0 / 13.0);
+	double kd = drint(t * 8.0);
+	double c = kd * 0.125;
+	double u = (t - c) / dfma(t, c, 1.0);
+	double z = u * u;
+	double p = 1.0 / 13.0;
 	p = dfma(p, z, -1.0 / 11.0);
 	p = dfma(p, z, 1.0 / 9.0);
 	p = dfma(p, z, -1.0 / 7.0);
 	p = dfma(p, z, 1.0 / 5.0);
 	p = dfma(p, z, -1.0 / 3.0);
-	double a = dfma(t * z, p, t) * 4.0;
+	return ATAN_K8[(int)kd] + dfma(u * z, p, u);
+}
+
+// atan(x) for double x; |x| > 1 uses pi/2 - atan(1/x).
+GF_HD double atan(double x)
+{
+	if (x != x) return x;
+	double ax = x < 0 ? -x : x;
+	bool inv = ax > 1.0;
+	double a = atan_unit(inv ? 1.0 / ax : ax);
 	if (inv) a = (PIO2_HI - a) + PIO2_LO;
 	return x < 0 ? -a : a;
 }
@@ -198,9 +204,9 @@ GF_HD double atan2(double y, double x)
 	double ax = x < 0 ? -x : x, ay = y < 0 ? -y : y;
 	double a;
 	if (ay <= ax) {
-		a = atan(ay / ax);
+		a = atan_unit(ay / ax);
 	} else {
-		a = (PIO2_HI - atan(ax / ay)) + PIO2_LO;
+		a = (PIO2_HI - atan_unit(ax / ay)) + PIO2_LO;
 	}
 	if (x < 0) a = (PI_HI - a) + PI_LO;
 	return y < 0 || (y == 0.0 && __builtin_signbit(y)) ? -a : a;
@@ -290,6 +296,16 @@ GF_HD float gf_powf_nonneg(float x, float y, LogX logx)
 	if (x != x || y != y) return x + y;
 	if (x == 0.0f) return y > 0.0f ? 0.0f : gfm::as_float(0x7f800000u);
 	return (float)gfm::exp((double)y * logx());
+}
+
+// powf(10, y), the distance modulus's 10^(0.2 DM + 1): gf_powf(10, y) takes
+// the log of 10 in double, which is LN10 exactly (0x40026bb1bbb55516), so
+// this is the same exp of the same product without computing the log.
+GF_HD float gf_pow10f(float y)
+{
+	if (y == 0.0f) return 1.0f;
+	if (y != y) return 10.0f + y;
+	return (float)gfm::exp((double)y * gfm::LN10);
 }
 
 // x^y for float x >= 0 (and the integer-y cases of x < 0)

@@ -100,20 +100,36 @@ static inline void gfo_sincos(double x, double *sn, double *cs)
 	}
 }
 
-/* double atan: two half-angle reductions t -> t/(1+sqrt(1+t^2)) take
- * |t| <= 1 below tan(pi/16); odd Taylor series to t^25; |x| > 1 via
- * pi/2 - atan(1/x). */
+/* atan(k/8), k = 0..8, each rounded to the nearest double (computed with
+ * 200-bit arithmetic) */
+static const double GFO_ATAN_K8[9] = {
+	0.0, 0x1.fd5ba9aac2f6ep-4, 0x1.f5b75f92c80ddp-3, 0x1.6f61941e4def1p-2, 0x1.dac670561bb4fp-2,
+	0x1.1e00babdefeb4p-1, 0x1.4978fa3269ee1p-1, 0x1.700a7c5784634p-1, 0x1.921fb54442d18p-1};
+
+/* atan t for 0 <= t <= 1: c = k/8 nearest t, atan t = atan c + atan u with
+ * u = (t - c) / (1 + t c) (t - c exact, |u| <= 1/16), atan u by its Taylor
+ * series to u^13/13 (truncation < 6e-20) */
+static inline double gfo_atan_unit(double t)
+{
+	double k = rint(t * 8.0);
+	double c = k * 0.125;
+	double u = (t - c) / fma(t, c, 1.0);
+	double z = u * u;
+	double p = 1.0 / 13.0;
+	p = fma(p, z, -1.0 / 11.0);
+	p = fma(p, z, 1.0 / 9.0);
+	p = fma(p, z, -1.0 / 7.0);
+	p = fma(p, z, 1.0 / 5.0);
+	p = fma(p, z, -1.0 / 3.0);
+	return GFO_ATAN_K8[(int)k] + fma(u * z, p, u);
+}
+
 static inline double gfo_atan(double x)
 {
 	if (x != x) return x;
 	double ax = fabs(x);
 	int inv = ax > 1.0;
-	double t = inv ? 1.0 / ax : ax;
-	for (int i = 0; i < 2; i++) t = t / (1.0 + sqrt(fma(t, t, 1.0)));
-	double z = t * t;
-	double p = 1.0 / 25.0;
-	for (int n = 10; n >= 0; n--) p = fma(p, z, ((n & 1) ? 1.0 : -1.0) / (double)(2 * n + 3));
-	double a = fma(t * z, p, t) * 4.0;
+	double a = gfo_atan_unit(inv ? 1.0 / ax : ax);
 	if (inv) a = (GFO_PIO2_HI - a) + GFO_PIO2_LO;
 	return x < 0 ? -a : a;
 }
@@ -127,8 +143,8 @@ static inline double gfo_atan2(double y, double x)
 		return signbit(y) ? -r : r;
 	}
 	double ax = fabs(x), ay = fabs(y), a;
-	if (ay <= ax) a = gfo_atan(ay / ax);
-	else a = (GFO_PIO2_HI - gfo_atan(ax / ay)) + GFO_PIO2_LO;
+	if (ay <= ax) a = gfo_atan_unit(ay / ax);
+	else a = (GFO_PIO2_HI - gfo_atan_unit(ax / ay)) + GFO_PIO2_LO;
 	if (x < 0) a = (PI_HI - a) + PI_LO;
 	return (y < 0 || (y == 0.0 && signbit(y))) ? -a : a;
 }

@@ -89,3 +89,12 @@ def test_double_functions_against_numpy():
     assert np.abs(run(3, a) - np.arcsin(a)).max() < 2e-15
     t = np.tan(RNG.uniform(-1.5, 1.5, 100000))
     assert np.abs(run(4, t) - np.arctan(t)).max() < 2e-15
+
+
+def test_pow10_is_exp10():
+    """powf(10, y) takes log(10) in double, which rounds to LN10 exactly, so
+    it equals exp10f(y) bit for bit; the distance modulus uses exp10 form."""
+    y = np.concatenate([RNG.uniform(-8, 8, 200000), [0.0, -0.0, 1.0, 2.5]]).astype(np.float32)
+    ten = np.full_like(y, 10.0)
+    a, b = O.math_f(5, ten, y), O.math_f(6, y)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))

@@ -149,7 +149,13 @@ GF_D gf_mwc restart(uint32_t a, const gf_positions &P, int plane, uint64_t prow)
 // Bond2010), PHOT (photometry), PERR (photometric errors), ASTRO (vel2pm,
 // gal2other).  a is the multiplier of the row's stream and prow its index in
 // the position planes.
-enum { ROWS_KIN = 0, ROWS_PHOT = 1, ROWS_PERR = 2, ROWS_ASTRO = 3, ROWS_GROUPS = 4 };
+enum {
+	ROWS_KIN = GF_ROWS_KIN,
+	ROWS_PHOT = GF_ROWS_PHOT,
+	ROWS_PERR = GF_ROWS_PERR,
+	ROWS_ASTRO = GF_ROWS_ASTRO,
+	ROWS_GROUPS = GF_ROWS_GROUPS
+};
 
 inline bool rows_on(const gf_chain_mods &m, int group)
 {

@@ -50,3 +50,9 @@ struct gf_chain_launch {
 	gf_positions P;
 	uint32_t begin, step, end;
 };
+
+// The row pass's launch groups, in pipeline order: kinematics (FeH,
+// GaussianFeH, Bond2010), photometry, photometric errors, astrometry
+// (vel2pm, gal2other).  Photometry needs the kinematics' FeH, photometric
+// errors the photometry; astrometry needs only the kinematics.
+enum gf_row_group { GF_ROWS_KIN = 0, GF_ROWS_PHOT = 1, GF_ROWS_PERR = 2, GF_ROWS_ASTRO = 3, GF_ROWS_GROUPS = 4 };

@@ -165,6 +165,7 @@ struct gf_catalog {
 	gf_rng rng{};
 	hipStream_t stream = nullptr;    // stream-order work: plans, emission, draw passes
 	hipStream_t rows = nullptr;      // the module chain's row passes
+	hipStream_t side = nullptr;      // the astrometry row pass, beside photometry
 	hipEvent_t ev[2] = {nullptr, nullptr};
 	// Two output chunks are in flight: while the row passes of chunk k run
 	// on `rows`, chunk k+1 is emitted and drawn on `stream` into the other
@@ -182,7 +183,7 @@ struct gf_catalog {
 	// Position planes of the draw passes, alternating per chain call: the
 	// draw pass into buffer b waits for the row passes that read it before.
 	DevBuf pos[2];
-	hipEvent_t drawn[2] = {}, pos_free[2] = {};
+	hipEvent_t drawn[2] = {}, pos_free[2] = {}, kin_done[2] = {}, side_done[2] = {};
 	DevBuf vis_dev;                            // visible-row counter per slot
 	unsigned long long *vis_host = nullptr;    // pinned copies of the counters
 	int overlap = 1;
@@ -197,8 +198,11 @@ struct gf_catalog {
 		}
 		for (auto e : drawn) if (e) (void)hipEventDestroy(e);
 		for (auto e : pos_free) if (e) (void)hipEventDestroy(e);
+		for (auto e : kin_done) if (e) (void)hipEventDestroy(e);
+		for (auto e : side_done) if (e) (void)hipEventDestroy(e);
 		if (vis_host) (void)hipHostFree(vis_host);
 		if (rows) (void)hipStreamDestroy(rows);
+		if (side) (void)hipStreamDestroy(side);
 	}
 };
 
@@ -909,7 +913,8 @@ void open_catalog(gf_catalog &c, const std::string &cmdfile)
 	if (c.has_feh && c.has_gfeh) fail("FeH and GaussianFeH are both configured");
 
 	c.pix_dev.upload(c.pixels);
-	if (hipStreamCreate(&c.stream) != hipSuccess || hipStreamCreate(&c.rows) != hipSuccess)
+	if (hipStreamCreate(&c.stream) != hipSuccess || hipStreamCreate(&c.rows) != hipSuccess ||
+	    hipStreamCreate(&c.side) != hipSuccess)
 		fail("stream creation failed");
 	(void)hipEventCreate(&c.ev[0]);
 	(void)hipEventCreate(&c.ev[1]);
@@ -917,7 +922,9 @@ void open_catalog(gf_catalog &c, const std::string &cmdfile)
 		if (hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) fail("event creation failed");
 	for (int b = 0; b < 2; b++)
 		if (hipEventCreateWithFlags(&c.drawn[b], hipEventDisableTiming) != hipSuccess ||
-		    hipEventCreateWithFlags(&c.pos_free[b], hipEventDisableTiming) != hipSuccess)
+		    hipEventCreateWithFlags(&c.pos_free[b], hipEventDisableTiming) != hipSuccess ||
+		    hipEventCreateWithFlags(&c.kin_done[b], hipEventDisableTiming) != hipSuccess ||
+		    hipEventCreateWithFlags(&c.side_done[b], hipEventDisableTiming) != hipSuccess)
 			fail("event creation failed");
 	for (int b = 0; b < 2; b++)   // the position buffers start free
 		if (hipEventRecord(c.pos_free[b], c.rows) != hipSuccess) fail("event record failed");
@@ -1003,9 +1010,9 @@ double now_s()
 }
 
 // Events of one chunk: 3 around emission (on the catalog stream), then per
-// chain call 2 around the draw pass (catalog stream) and
-// GF_CHAIN_LAUNCHES around the row passes (row stream).
-constexpr int EV_EMIT = 3, EV_CALL = 2 + GF_CHAIN_LAUNCHES;
+// chain call 2 around each launch of the module chain (on the stream that
+// runs it).
+constexpr int EV_EMIT = 3, EV_CALL = 2 * GF_CHAIN_LAUNCHES;
 
 void ensure_events(gf_catalog::Slot &sl, int ncalls)
 {
@@ -1039,11 +1046,8 @@ int deliver(gf_catalog &c, int si, gf_chunk_fn fn, void *user, gf_run_stats &s)
 	s.t_emit += a + b;
 	for (int j = 0; j < sl.nsub; j++) {
 		const hipEvent_t *q = e + EV_EMIT + EV_CALL * j;
-		double d = elapsed_s(q[0], q[1]);
-		s.t_stage[0] += d;
-		s.t_chain += d;
-		for (int g = 1; g < GF_CHAIN_LAUNCHES; g++) {
-			double r = elapsed_s(q[g + 1], q[g + 2]);
+		for (int g = 0; g < GF_CHAIN_LAUNCHES; g++) {
+			double r = elapsed_s(q[2 * g], q[2 * g + 1]);
 			s.t_stage[g] += r;
 			s.t_chain += r;
 		}
@@ -1057,13 +1061,15 @@ int deliver(gf_catalog &c, int si, gf_chunk_fn fn, void *user, gf_run_stats &s)
 // read the streams' multipliers and run on c.rows as soon as the draw pass
 // of their rows is done, so that they overlap the next chunk's emission and
 // draws (the draw-type kernels run one lane per stream and leave most of the
-// device idle).  Two chunk slots: chunk k is handed to the callback after
+// device idle).  The astrometry pass needs only the kinematics and runs on
+// c.side beside photometry and photometric errors (memory-latency bound
+// texture lookups beside double-precision arithmetic).  Two chunk slots: chunk k is handed to the callback after
 // chunk k+1 has been queued, and before chunk k+2 overwrites its slot.
 int run_catalog(gf_catalog &c, gf_chunk_fn fn, void *user, gf_run_stats *stats)
 {
 	gf_run_stats s{};
 	double t0 = now_s();
-	hipStream_t A = c.stream, B = c.overlap ? c.rows : c.stream;
+	hipStream_t A = c.stream, B = c.overlap ? c.rows : c.stream, C = c.overlap ? c.side : c.stream;
 	gf_chain ch{};
 	if (c.has_feh) ch.feh = &c.feh;
 	if (c.has_gfeh) { ch.gfeh = &c.gfeh; ch.gfeh_comps = c.gfeh_comps; }
@@ -1079,6 +1085,7 @@ int run_catalog(gf_catalog &c, gf_chunk_fn fn, void *user, gf_run_stats *stats)
 		if (pos_bytes && pb.bytes < pos_bytes) pb.alloc(pos_bytes);
 	unsigned long long *vis_dev = (unsigned long long *)c.vis_dev.p;
 	uint32_t pb = 0;      // position buffer of the next chain call
+	uint32_t last = 0;    // position buffer of the last chain call queued
 	uint64_t k = 0;       // chunks queued
 	int rc;
 	for (auto &mp : c.models) {
@@ -1135,21 +1142,29 @@ int run_catalog(gf_catalog &c, gf_chunk_fn fn, void *user, gf_run_stats *stats)
 				const hipEvent_t *qe = e + EV_EMIT + EV_CALL * j;
 				// the draw pass overwrites the planes the row passes two calls back read
 				GF_HIP(hipStreamWaitEvent(A, c.pos_free[pb], 0));
-				GF_HIP(hipEventRecord(qe[0], A));
-				rc = gf::chain_launch(L, &c.rng, 0, A);
-				if (rc) return rc;
-				GF_HIP(hipEventRecord(qe[1], A));
+				auto launch = [&](int g, hipStream_t st) -> int {
+					GF_HIP(hipEventRecord(qe[2 * g], st));
+					int r = gf::chain_launch(L, &c.rng, g, st);
+					if (r) return r;
+					GF_HIP(hipEventRecord(qe[2 * g + 1], st));
+					return 0;
+				};
+				if ((rc = launch(0, A))) return rc;
 				GF_HIP(hipEventRecord(c.drawn[pb], A));
 				GF_HIP(hipStreamWaitEvent(B, c.drawn[pb], 0));
-				for (int g = 1; g < GF_CHAIN_LAUNCHES; g++) {
-					GF_HIP(hipEventRecord(qe[g + 1], B));
-					rc = gf::chain_launch(L, &c.rng, g, B);
-					if (rc) return rc;
-				}
-				GF_HIP(hipEventRecord(qe[GF_CHAIN_LAUNCHES + 1], B));
+				if ((rc = launch(1 + GF_ROWS_KIN, B))) return rc;
+				GF_HIP(hipEventRecord(c.kin_done[pb], B));
+				GF_HIP(hipStreamWaitEvent(C, c.kin_done[pb], 0));
+				if ((rc = launch(1 + GF_ROWS_ASTRO, C))) return rc;
+				GF_HIP(hipEventRecord(c.side_done[pb], C));
+				if ((rc = launch(1 + GF_ROWS_PHOT, B))) return rc;
+				if ((rc = launch(1 + GF_ROWS_PERR, B))) return rc;
 				GF_HIP(hipEventRecord(c.pos_free[pb], B));
+				last = pb;
 				s.n_chain++;
 			}
+			// the chunk is complete when the astrometry passes are
+			GF_HIP(hipStreamWaitEvent(B, c.side_done[last], 0));
 			s.rows_chain += n;
 			GF_HIP(hipMemsetAsync(vis_dev + si, 0, sizeof(unsigned long long), B));
 			rc = gf::count_visible(sl.table.hidden, n, vis_dev + si, B);
@@ -1174,6 +1189,7 @@ int run_catalog(gf_catalog &c, gf_chunk_fn fn, void *user, gf_run_stats *stats)
 	}
 	GF_HIP(hipStreamSynchronize(A));
 	GF_HIP(hipStreamSynchronize(B));
+	GF_HIP(hipStreamSynchronize(C));
 	s.t_total = now_s() - t0;
 	if (stats) *stats = s;
 	return 0;
@@ -1275,6 +1291,7 @@ int gf_catalog_run(gf_catalog *c, gf_chunk_fn fn, void *user, gf_run_stats *stat
 		// drop the chunks in flight
 		(void)hipStreamSynchronize(c->stream);
 		(void)hipStreamSynchronize(c->rows);
+		(void)hipStreamSynchronize(c->side);
 		for (auto &sl : c->slot) sl.pending = false;
 	}
 	return rc;

@@ -51,8 +51,10 @@ STAGE_BYTES = {
     # draw pass: read hidden + comp of every row; write the 8-byte stream
     # position of each of the 3 random-number modules for a visible row
     "k_chain_draws": (4 + 4 + 3 * 8, 4 + 4),
-    # kinematics rows: read hidden, comp, XYZ, 2 positions; write FeH, vcyl
-    "k_chain_rows<KIN>": (4 + 4 + 12 + 2 * 8 + 4 + 12, 4),
+    # metallicity rows: read hidden, comp, XYZ, 1 position; write FeH
+    "k_chain_rows<FEH>": (4 + 4 + 12 + 8 + 4, 4),
+    # kinematics rows: read hidden, comp, XYZ, 1 position; write vcyl
+    "k_chain_rows<BOND>": (4 + 4 + 12 + 8 + 12, 4),
     # photometry rows: read hidden, comp, FeH, absmag, DM, Am; write mags,
     # photoFlags
     "k_chain_rows<PHOT>": (4 + 4 + 4 + 4 + 4 + 4 + 20 + 4, 4),
@@ -61,8 +63,8 @@ STAGE_BYTES = {
     # astrometry rows: read hidden, lb, XYZ, vcyl; write pm, radec
     "k_chain_rows<ASTRO>": (4 + 16 + 12 + 12 + 12 + 16, 4),
 }
-STAGE_NAMES = ["k_chain_draws", "k_chain_rows<KIN>", "k_chain_rows<PHOT>", "k_chain_rows<PERR>",
-               "k_chain_rows<ASTRO>"]
+STAGE_NAMES = ["k_chain_draws", "k_chain_rows<FEH>", "k_chain_rows<BOND>", "k_chain_rows<PHOT>",
+               "k_chain_rows<PERR>", "k_chain_rows<ASTRO>"]
 
 
 def cpu_baseline(cat, workload, budget_s, threads):

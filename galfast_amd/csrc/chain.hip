@@ -104,8 +104,11 @@ int chain_launch(const gf_chain_launch &L, const gf_rng *rng, int launch, hipStr
 			k_chain_draws<false><<<g, gfc::BS, 0, st>>>(L, nthr, rng->state, rng->nstreams);
 		break;
 	}
-	case 1 + gfc::ROWS_KIN:
-		k_chain_rows<gfc::ROWS_KIN, 1><<<grows, gfc::BS, 0, st>>>(L, rng->state, 1);
+	case 1 + gfc::ROWS_FEH:
+		k_chain_rows<gfc::ROWS_FEH, 1><<<grows, gfc::BS, 0, st>>>(L, rng->state, 1);
+		break;
+	case 1 + gfc::ROWS_BOND:
+		k_chain_rows<gfc::ROWS_BOND, 1><<<grows, gfc::BS, 0, st>>>(L, rng->state, 1);
 		break;
 	case 1 + gfc::ROWS_PHOT:
 		switch (gfc::phot_tables(L.m)) {
@@ -128,8 +131,9 @@ int chain_launch(const gf_chain_launch &L, const gf_rng *rng, int launch, hipStr
 
 // The chain on one stream, with optional per-launch timing: when ev is
 // non-NULL it holds GF_CHAIN_LAUNCHES + 1 events; ms[0] receives the draw
-// pass's duration, ms[1..4] the row pass's groups (kinematics, photometry,
-// photometric errors, astrometry); 0 for a launch with nothing to do.
+// pass's duration, ms[1..5] the row pass's groups (metallicity, kinematics,
+// photometry, photometric errors, astrometry); 0 for a launch with nothing
+// to do.
 int module_chain(const gf_rows *ks, const gf_chain *chain, gf_rng *rng, const gf_table *t, hipStream_t st,
                  hipEvent_t *ev, float *ms)
 {

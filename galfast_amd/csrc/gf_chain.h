@@ -144,13 +144,15 @@ GF_D gf_mwc restart(uint32_t a, const gf_positions &P, int plane, uint64_t prow)
 	return r;
 }
 
-// The row pass, in four groups so that each launch stays small enough in
-// registers to keep several waves per SIMD: KIN (FeH, GaussianFeH,
-// Bond2010), PHOT (photometry), PERR (photometric errors), ASTRO (vel2pm,
-// gal2other).  a is the multiplier of the row's stream and prow its index in
-// the position planes.
+// The row pass, in five groups so that each launch stays small enough in
+// registers to keep several waves per SIMD, and so that independent groups
+// can run side by side: FEH (FeH, GaussianFeH), BOND (Bond2010), PHOT
+// (photometry), PERR (photometric errors), ASTRO (vel2pm, gal2other).  a is
+// the multiplier of the row's stream and prow its index in the position
+// planes.
 enum {
-	ROWS_KIN = GF_ROWS_KIN,
+	ROWS_FEH = GF_ROWS_FEH,
+	ROWS_BOND = GF_ROWS_BOND,
 	ROWS_PHOT = GF_ROWS_PHOT,
 	ROWS_PERR = GF_ROWS_PERR,
 	ROWS_ASTRO = GF_ROWS_ASTRO,
@@ -159,7 +161,8 @@ enum {
 
 inline bool rows_on(const gf_chain_mods &m, int group)
 {
-	if (group == ROWS_KIN) return m.has_feh || m.has_gfeh || m.has_bond;
+	if (group == ROWS_FEH) return m.has_feh || m.has_gfeh;
+	if (group == ROWS_BOND) return m.has_bond;
 	if (group == ROWS_PHOT) return m.has_phot;
 	if (group == ROWS_PERR) return m.has_perr;
 	return m.has_v2pm || m.gal2other;
@@ -168,37 +171,38 @@ inline bool rows_on(const gf_chain_mods &m, int group)
 // whether the group restarts streams (needs the row's multiplier)
 inline bool rows_stream(const gf_chain_mods &m, int group)
 {
-	return group == ROWS_KIN || group == ROWS_PERR;
+	return group == ROWS_FEH || group == ROWS_BOND || group == ROWS_PERR;
 }
 
-GF_D void rows_kin(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t prow, const gf_table &T,
+GF_D void rows_feh(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t prow, const gf_table &T,
                    const gf_positions &P)
 {
 	if (col<int>(T.hidden, row)) return;
 	const int cmp = col<int>(T.comp, row);
-	if (m.has_feh || m.has_bond) {
+	if (m.has_feh) {
 		const float x = col<float>(T.XYZ, row, 0), y = col<float>(T.XYZ, row, 1), z = col<float>(T.XYZ, row, 2);
-		if (m.has_feh) {
-			gf_mwc r = restart(a, P, m.pl_feh, prow);
-			float feh;
-			if (feh_row(m.feh, r, cmp, x, y, z, feh)) col<float>(T.FeH, row) = feh;
-		}
-		if (m.has_gfeh) {
-			gf_mwc r = restart(a, P, m.pl_gfeh, prow);
-			gfeh_row(m.gfeh, m.gfeh_comps, r, cmp, row, T.FeH_comp, T.FeH);
-		}
-		if (m.has_bond) {
-			gf_mwc r = restart(a, P, m.pl_bond, prow);
-			float v[3];
-			if (bond_row(m.bond, r, cmp, x, y, z, v)) {
-				col<float>(T.vcyl, row, 0) = v[0];
-				col<float>(T.vcyl, row, 1) = v[1];
-				col<float>(T.vcyl, row, 2) = v[2];
-			}
-		}
-	} else if (m.has_gfeh) {
+		gf_mwc r = restart(a, P, m.pl_feh, prow);
+		float feh;
+		if (feh_row(m.feh, r, cmp, x, y, z, feh)) col<float>(T.FeH, row) = feh;
+	}
+	if (m.has_gfeh) {
 		gf_mwc r = restart(a, P, m.pl_gfeh, prow);
 		gfeh_row(m.gfeh, m.gfeh_comps, r, cmp, row, T.FeH_comp, T.FeH);
+	}
+}
+
+GF_D void rows_bond(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t prow, const gf_table &T,
+                    const gf_positions &P)
+{
+	if (col<int>(T.hidden, row)) return;
+	const int cmp = col<int>(T.comp, row);
+	const float x = col<float>(T.XYZ, row, 0), y = col<float>(T.XYZ, row, 1), z = col<float>(T.XYZ, row, 2);
+	gf_mwc r = restart(a, P, m.pl_bond, prow);
+	float v[3];
+	if (bond_row(m.bond, r, cmp, x, y, z, v)) {
+		col<float>(T.vcyl, row, 0) = v[0];
+		col<float>(T.vcyl, row, 1) = v[1];
+		col<float>(T.vcyl, row, 2) = v[2];
 	}
 }
 
@@ -224,13 +228,14 @@ GF_D void rows_astro(const gf_chain_mods &m, uint32_t row, const gf_table &T)
 	if (m.gal2other) mod_gal2other(m.gal2other, row, row + 1, T.hidden, T.lb, T.radec);
 }
 
-// GROUP: ROWS_KIN, ROWS_PHOT, ROWS_PERR or ROWS_ASTRO; NT: colour tables of the
-// photometry (1..4, at least ceil(ncolors / 4); see phot_tables())
+// GROUP: one of ROWS_*; NT: colour tables of the photometry (1..4, at
+// least ceil(ncolors / 4); see phot_tables())
 template <int GROUP, int NT>
 GF_D void chain_rows(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t prow, const gf_table &T,
                      const gf_positions &P)
 {
-	if (GROUP == ROWS_KIN) rows_kin(m, a, row, prow, T, P);
+	if (GROUP == ROWS_FEH) rows_feh(m, a, row, prow, T, P);
+	else if (GROUP == ROWS_BOND) rows_bond(m, a, row, prow, T, P);
 	else if (GROUP == ROWS_PHOT) rows_phot<NT>(m, row, T);
 	else if (GROUP == ROWS_PERR) rows_perr(m, a, row, prow, T, P);
 	else rows_astro(m, row, T);

@@ -51,8 +51,15 @@ struct gf_chain_launch {
 	uint32_t begin, step, end;
 };
 
-// The row pass's launch groups, in pipeline order: kinematics (FeH,
-// GaussianFeH, Bond2010), photometry, photometric errors, astrometry
-// (vel2pm, gal2other).  Photometry needs the kinematics' FeH, photometric
-// errors the photometry; astrometry needs only the kinematics.
-enum gf_row_group { GF_ROWS_KIN = 0, GF_ROWS_PHOT = 1, GF_ROWS_PERR = 2, GF_ROWS_ASTRO = 3, GF_ROWS_GROUPS = 4 };
+// The row pass's launch groups, in pipeline order: metallicity (FeH,
+// GaussianFeH), kinematics (Bond2010), photometry, photometric errors,
+// astrometry (vel2pm, gal2other).  Photometry needs the metallicity,
+// photometric errors the photometry, astrometry the kinematics.
+enum gf_row_group {
+	GF_ROWS_FEH = 0,
+	GF_ROWS_BOND = 1,
+	GF_ROWS_PHOT = 2,
+	GF_ROWS_PERR = 3,
+	GF_ROWS_ASTRO = 4,
+	GF_ROWS_GROUPS = 5
+};

@@ -72,7 +72,7 @@ int count_visible(gf_col hidden, uint64_t n, unsigned long long *counter, hipStr
 
 } // namespace gf
 
-constexpr int GF_CHAIN_LAUNCHES = 5;
+constexpr int GF_CHAIN_LAUNCHES = 6;   // the draw pass and the row pass's five groups
 
 enum gf_scratch_slot {
 	GF_SLOT_LFTAB = 0,

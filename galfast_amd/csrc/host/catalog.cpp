@@ -165,7 +165,7 @@ struct gf_catalog {
 	gf_rng rng{};
 	hipStream_t stream = nullptr;    // stream-order work: plans, emission, draw passes
 	hipStream_t rows = nullptr;      // the module chain's row passes
-	hipStream_t side = nullptr;      // the astrometry row pass, beside photometry
+	hipStream_t side = nullptr;      // kinematics and astrometry row passes, beside photometry
 	hipEvent_t ev[2] = {nullptr, nullptr};
 	// Two output chunks are in flight: while the row passes of chunk k run
 	// on `rows`, chunk k+1 is emitted and drawn on `stream` into the other
@@ -183,7 +183,7 @@ struct gf_catalog {
 	// Position planes of the draw passes, alternating per chain call: the
 	// draw pass into buffer b waits for the row passes that read it before.
 	DevBuf pos[2];
-	hipEvent_t drawn[2] = {}, pos_free[2] = {}, kin_done[2] = {}, side_done[2] = {};
+	hipEvent_t drawn[2] = {}, pos_free[2] = {}, pos_free_side[2] = {}, side_done[2] = {};
 	DevBuf vis_dev;                            // visible-row counter per slot
 	unsigned long long *vis_host = nullptr;    // pinned copies of the counters
 	int overlap = 1;
@@ -198,8 +198,8 @@ struct gf_catalog {
 		}
 		for (auto e : drawn) if (e) (void)hipEventDestroy(e);
 		for (auto e : pos_free) if (e) (void)hipEventDestroy(e);
-		for (auto e : kin_done) if (e) (void)hipEventDestroy(e);
 		for (auto e : side_done) if (e) (void)hipEventDestroy(e);
+		for (auto e : pos_free_side) if (e) (void)hipEventDestroy(e);
 		if (vis_host) (void)hipHostFree(vis_host);
 		if (rows) (void)hipStreamDestroy(rows);
 		if (side) (void)hipStreamDestroy(side);
@@ -923,11 +923,13 @@ void open_catalog(gf_catalog &c, const std::string &cmdfile)
 	for (int b = 0; b < 2; b++)
 		if (hipEventCreateWithFlags(&c.drawn[b], hipEventDisableTiming) != hipSuccess ||
 		    hipEventCreateWithFlags(&c.pos_free[b], hipEventDisableTiming) != hipSuccess ||
-		    hipEventCreateWithFlags(&c.kin_done[b], hipEventDisableTiming) != hipSuccess ||
-		    hipEventCreateWithFlags(&c.side_done[b], hipEventDisableTiming) != hipSuccess)
+		    hipEventCreateWithFlags(&c.side_done[b], hipEventDisableTiming) != hipSuccess ||
+		    hipEventCreateWithFlags(&c.pos_free_side[b], hipEventDisableTiming) != hipSuccess)
 			fail("event creation failed");
 	for (int b = 0; b < 2; b++)   // the position buffers start free
-		if (hipEventRecord(c.pos_free[b], c.rows) != hipSuccess) fail("event record failed");
+		if (hipEventRecord(c.pos_free[b], c.rows) != hipSuccess ||
+		    hipEventRecord(c.pos_free_side[b], c.side) != hipSuccess)
+			fail("event record failed");
 	c.vis_dev.alloc(2 * sizeof(unsigned long long));
 	if (hipHostMalloc((void **)&c.vis_host, 2 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
 		fail("pinned allocation failed");
@@ -1061,9 +1063,9 @@ int deliver(gf_catalog &c, int si, gf_chunk_fn fn, void *user, gf_run_stats &s)
 // read the streams' multipliers and run on c.rows as soon as the draw pass
 // of their rows is done, so that they overlap the next chunk's emission and
 // draws (the draw-type kernels run one lane per stream and leave most of the
-// device idle).  The astrometry pass needs only the kinematics and runs on
-// c.side beside photometry and photometric errors (memory-latency bound
-// texture lookups beside double-precision arithmetic).  Two chunk slots: chunk k is handed to the callback after
+// device idle).  The kinematics and astrometry passes depend on neither
+// metallicity nor photometry and run on c.side beside them (memory-latency
+// bound texture lookups beside double-precision arithmetic).  Two chunk slots: chunk k is handed to the callback after
 // chunk k+1 has been queued, and before chunk k+2 overwrites its slot.
 int run_catalog(gf_catalog &c, gf_chunk_fn fn, void *user, gf_run_stats *stats)
 {
@@ -1142,6 +1144,7 @@ int run_catalog(gf_catalog &c, gf_chunk_fn fn, void *user, gf_run_stats *stats)
 				const hipEvent_t *qe = e + EV_EMIT + EV_CALL * j;
 				// the draw pass overwrites the planes the row passes two calls back read
 				GF_HIP(hipStreamWaitEvent(A, c.pos_free[pb], 0));
+				GF_HIP(hipStreamWaitEvent(A, c.pos_free_side[pb], 0));
 				auto launch = [&](int g, hipStream_t st) -> int {
 					GF_HIP(hipEventRecord(qe[2 * g], st));
 					int r = gf::chain_launch(L, &c.rng, g, st);
@@ -1152,11 +1155,12 @@ int run_catalog(gf_catalog &c, gf_chunk_fn fn, void *user, gf_run_stats *stats)
 				if ((rc = launch(0, A))) return rc;
 				GF_HIP(hipEventRecord(c.drawn[pb], A));
 				GF_HIP(hipStreamWaitEvent(B, c.drawn[pb], 0));
-				if ((rc = launch(1 + GF_ROWS_KIN, B))) return rc;
-				GF_HIP(hipEventRecord(c.kin_done[pb], B));
-				GF_HIP(hipStreamWaitEvent(C, c.kin_done[pb], 0));
+				GF_HIP(hipStreamWaitEvent(C, c.drawn[pb], 0));
+				if ((rc = launch(1 + GF_ROWS_BOND, C))) return rc;
+				GF_HIP(hipEventRecord(c.pos_free_side[pb], C));
 				if ((rc = launch(1 + GF_ROWS_ASTRO, C))) return rc;
 				GF_HIP(hipEventRecord(c.side_done[pb], C));
+				if ((rc = launch(1 + GF_ROWS_FEH, B))) return rc;
 				if ((rc = launch(1 + GF_ROWS_PHOT, B))) return rc;
 				if ((rc = launch(1 + GF_ROWS_PERR, B))) return rc;
 				GF_HIP(hipEventRecord(c.pos_free[pb], B));

@@ -326,8 +326,12 @@ int serve_chain(const gf_serve_req *dreqs, int nreq, uint32_t nlanes, const gf_s
 		GF_LAUNCHED();
 	}
 	const unsigned g = gf::blocks_for(nrows, gfc::BS);
-	if (gfc::rows_on(hch, gfc::ROWS_KIN)) {
-		k_b_rows<gfc::ROWS_KIN, 1><<<g, gfc::BS, 0, st>>>(dreqs, nreq, dch, nrows, T, P, 1);
+	if (gfc::rows_on(hch, gfc::ROWS_FEH)) {
+		k_b_rows<gfc::ROWS_FEH, 1><<<g, gfc::BS, 0, st>>>(dreqs, nreq, dch, nrows, T, P, 1);
+		GF_LAUNCHED();
+	}
+	if (gfc::rows_on(hch, gfc::ROWS_BOND)) {
+		k_b_rows<gfc::ROWS_BOND, 1><<<g, gfc::BS, 0, st>>>(dreqs, nreq, dch, nrows, T, P, 1);
 		GF_LAUNCHED();
 	}
 	if (gfc::rows_on(hch, gfc::ROWS_PHOT)) {

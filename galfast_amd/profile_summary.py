@@ -29,7 +29,7 @@ import re
 
 COPY_BYTES = 1 << 30            # bench.py --copy-calibration copies 1 GiB
 PEAK_VALU_GINSTS = 256 * 4 * 2.4 / 4   # G wave64 vector instructions per second
-ROW_GROUPS = {"0": "KIN", "1": "PHOT", "2": "PERR", "3": "ASTRO"}
+ROW_GROUPS = {"0": "FEH", "1": "BOND", "2": "PHOT", "3": "PERR", "4": "ASTRO"}
 
 
 def short_name(full):

@@ -470,8 +470,8 @@ typedef struct gf_run_stats {
     double   t_plan, t_emit, t_chain, t_total;
     uint64_t visible;       /* rows with hidden == 0 */
     double   t_stage[6];    /* module chain launches: draw pass; row pass for
-                               FeH/GaussianFeH/Bond2010, photometry, photometric
-                               errors, vel2pm + gal2other; (spare) */
+                               FeH/GaussianFeH, Bond2010, photometry,
+                               photometric errors, vel2pm + gal2other */
     double   t_emit_rng, t_emit_rows;   /* the two launches inside t_emit */
 } gf_run_stats;
 

@@ -207,6 +207,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS))
+    ap.add_argument("--overlap", type=int, default=1, choices=(0, 1),
+                    help="0: every launch of the catalog on one stream (isolated kernel durations)")
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
@@ -243,6 +245,7 @@ def main():
     d = tempfile.mkdtemp(prefix="galfast_bench_%s_" % args.workload)
     cmd = configs.WRITERS[args.workload](d, args.seed)
     cat = Catalog(cmd, seed=args.seed, chunk_rows=30_000_000)
+    cat.set("overlap", args.overlap)
     info = cat.info()
 
     def step():

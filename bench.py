@@ -307,6 +307,19 @@ def main():
             traffic = tr[name]["bytes"]
     per_launch_s = t / max(n, 1)
     achieved = bytes_total / max(n, 1) / per_launch_s / 1e9 if per_launch_s > 0 else 0.0
+    # vector-instruction issue of the same kernel: its VALU instructions per
+    # launch from the committed counter runs (profiles/valu.json) over the
+    # launch duration measured here, as SIMD cycles per instruction (1,024
+    # SIMDs at 2.4 GHz; a wave64 FMA occupies a SIMD for 4 cycles)
+    valu_issue = None
+    vj = os.path.join(ROOT, "profiles", "valu.json")
+    if name and args.workload == "c1" and os.path.exists(vj):
+        with open(vj) as f:
+            vt = json.load(f)
+        if name in vt and vt[name]["valu_insts"] > 0:
+            valu_issue = {"kernel": name, "valu_insts_per_launch": vt[name]["valu_insts"],
+                          "simd_cycles_per_valu_inst": 1024 * 2.4e9 * per_launch_s / vt[name]["valu_insts"],
+                          "isolated_simd_cycles_per_valu_inst": vt[name]["simd_cycles_per_valu"]}
     out = {
         "metric": "catalog stars/sec (full pipeline) on 1 GPU vs plain; p50/p99 request latency",
         "value": value,
@@ -333,6 +346,7 @@ def main():
         "roofline": ({"bound": "hbm", "kernel": name, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                       "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                       "algorithmic_bytes_per_launch": bytes_total / max(n, 1)} if name else None),
+        "valu_issue": valu_issue,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cat, args.workload, args.cpu_budget, args.cpu_threads)

@@ -16,9 +16,11 @@ bytes per launch of each kernel, read by bench.py for roofline.traffic).
 GRBM_GUI_ACTIVE counters, at most 8 SQ counters a run); with SQ_WAVES and
 SQ_INSTS_VALU among them they give <tag>_instruction_mix.md and valu.json:
 the vector instructions each launch issues and, over the kernel-trace
-duration, the issue rate against the device's: 256 CUs x 4 SIMDs, one
-wave64 vector instruction per SIMD every 4 cycles (16 lanes a cycle) at
-2.4 GHz = 614.4 G wave-instructions/s.
+duration, the SIMD cycles per vector instruction: 1,024 SIMDs (256 CUs x 4)
+at 2.4 GHz times the launch's duration, over its vector instructions.  A
+wave64 FP32 or FP64 FMA occupies a SIMD for 4 cycles (16 lanes a cycle), so
+a launch near 4 cycles per instruction or below keeps the vector units
+issuing all the time.
 """
 import argparse
 import csv
@@ -28,7 +30,7 @@ import os
 import re
 
 COPY_BYTES = 1 << 30            # bench.py --copy-calibration copies 1 GiB
-PEAK_VALU_GINSTS = 256 * 4 * 2.4 / 4   # G wave64 vector instructions per second
+SIMD_GCYCLES = 256 * 4 * 2.4   # 1,024 SIMDs at 2.4 GHz: G SIMD-cycles per second
 ROW_GROUPS = {"0": "FEH", "1": "BOND", "2": "PHOT", "3": "PERR", "4": "ASTRO"}
 
 
@@ -100,21 +102,21 @@ def instruction_mix(a, ks, kt_cmd):
     classes = [c for c in sorted({c for k in kernels for c in per[k]}) if c.startswith("SQ_INSTS_")]
     lines = ["# Instruction mix per launch: %s" % a.tag, "", "Commands:", ""] + ["* `%s`" % c for c in pmc_cmds] + \
             ["* durations: `%s`" % kt_cmd, "",
-             "Issue rate = SQ_INSTS_VALU per launch / mean kernel-trace duration, against 256 CUs x 4 SIMDs x "
-             "2.4 GHz / 4 cycles per wave64 vector instruction = %.1f G/s.  Counts are per wave (64 lanes)." %
-             PEAK_VALU_GINSTS, "",
-             "| kernel | waves / launch | VALU / launch | VALU issue G/s | of peak | " +
+             "SIMD cycles per VALU instruction = 1,024 SIMDs x 2.4 GHz x mean kernel-trace duration / "
+             "SQ_INSTS_VALU per launch (a wave64 FMA occupies a SIMD for 4 cycles: near 4 or below, the vector "
+             "units issue all the time).  Counts are per wave (64 lanes).", "",
+             "| kernel | waves / launch | VALU / launch | mean us | SIMD cycles / VALU | " +
              " | ".join(c.replace("SQ_INSTS_", "") + " / wave" for c in classes) + " |",
              "|---|---|---|---|---|" + "---|" * len(classes)]
-    out = {"_peak_G_wave_insts_per_s": PEAK_VALU_GINSTS, "_program": a.program}
+    out = {"_simd_gcycles_per_s": SIMD_GCYCLES, "_program": a.program}
     for k in kernels:
         e = per[k]
         w = e["SQ_WAVES"]
         valu = e.get("SQ_INSTS_VALU", 0.0)
         avg = ks[k]["avg_ns"]
-        rate = valu / avg if avg else 0.0
-        out[k] = {"waves": w, "valu_insts": valu, "avg_ns": avg}
-        lines.append("| `%s` | %.0f | %.3g | %.0f | %.0f%% | " % (k, w, valu, rate, 100.0 * rate / PEAK_VALU_GINSTS) +
+        cpi = SIMD_GCYCLES * avg / valu if valu else 0.0
+        out[k] = {"waves": w, "valu_insts": valu, "avg_ns": avg, "simd_cycles_per_valu": cpi}
+        lines.append("| `%s` | %.0f | %.3g | %.1f | %.1f | " % (k, w, valu, avg / 1e3, cpi) +
                      " | ".join("%.1f" % (e.get(c, 0.0) / w) for c in classes) + " |")
     with open(os.path.join(a.out, "%s_instruction_mix.md" % a.tag), "w") as f:
         f.write("\n".join(lines) + "\n")
@@ -134,6 +136,9 @@ def main():
     ap.add_argument("--program", required=True, help="the profiled program (after rocprofv3's --)")
     ap.add_argument("--workload", default="")
     ap.add_argument("--pmc", nargs="*", default=[])
+    ap.add_argument("--kt-isolated", help="kernel trace of the same program with --overlap 0, whose durations "
+                    "the instruction-mix table uses")
+    ap.add_argument("--program-isolated", default="")
     a = ap.parse_args()
     os.makedirs(a.out, exist_ok=True)
 
@@ -150,7 +155,12 @@ def main():
     with open(os.path.join(a.out, "%s_kernel_stats.md" % a.tag), "w") as f:
         f.write("\n".join(lines) + "\n")
     if a.pmc:
-        instruction_mix(a, ks, kt_cmd)
+        if a.kt_isolated:
+            kt_cmd = "rocprofv3 --kernel-trace --stats --output-format csv -d %s -o kt -- %s" % (
+                a.kt_isolated, a.program_isolated or a.program)
+            instruction_mix(a, kernel_stats(a.kt_isolated), kt_cmd)
+        else:
+            instruction_mix(a, ks, kt_cmd)
 
     if not (a.fetch and a.write and a.copy_fetch and a.copy_write):
         return

@@ -84,5 +84,7 @@ enum gf_scratch_slot {
 	GF_SLOT_COUNT_W,
 	GF_SLOT_PARK,
 	GF_SLOT_UNITS,
+	GF_SLOT_ORDER,
+	GF_SLOT_ORDER_TMP,
 	GF_SLOT_NSLOTS
 };

@@ -187,6 +187,7 @@ struct gf_catalog {
 	DevBuf vis_dev;                            // visible-row counter per slot
 	unsigned long long *vis_host = nullptr;    // pinned copies of the counters
 	int overlap = 1;
+	int draw_mode = 0;   // 0: cumulative-table draw; 1: the reference's crawl, batch order and clipper
 
 	~gf_catalog()
 	{
@@ -1011,6 +1012,93 @@ double now_s()
 	return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// The catalog's module chain (device parameter blocks).
+gf_chain chain_of(gf_catalog &c)
+{
+	gf_chain ch{};
+	if (c.has_feh) ch.feh = &c.feh;
+	if (c.has_gfeh) { ch.gfeh = &c.gfeh; ch.gfeh_comps = c.gfeh_comps; }
+	if (c.has_mult) { ch.mult = &c.mult; ch.mult_comps = c.mult_comps; }
+	if (c.has_bond) ch.bond = &c.bond;
+	if (c.has_phot) { ch.phot = &c.phot; ch.phot_comps = c.phot_comps; }
+	if (c.has_perr) ch.perr = &c.perr;
+	if (c.has_v2pm) ch.v2pm = &c.v2pm;
+	ch.gal2other_coordsys = c.gal2other;
+	return ch;
+}
+
+gf_skygen_cols skygen_cols_of(const gf_table &t)
+{
+	return gf_skygen_cols{t.lb, t.projIdx, t.projXY, t.DM, t.absmag, t.XYZ, t.Am, t.AmInf, t.comp, t.hidden};
+}
+
+// The reference's draw, model by model (skygenHost<T>::drawSources,
+// skygen/skyconfig_impl.h:434-545, then os_clipper and the modules): the
+// diagonal crawl of kernel<1> with sp.nthreads threads on streams 0 .. nthreads-1,
+// the batch sorted with star_comp and truncated at its first hidden row, the
+// clipper, then the module chain over the visible rows, the stream of row r
+// being r / 256.  One batch per model: the table grows to hold the model's
+// stars.  (The reference suspends the crawl when its table fills and
+// resumes it for the next batch; which stars make a batch then depends on
+// the order of its threads' atomics.)
+int run_catalog_reference(gf_catalog &c, gf_chunk_fn fn, void *user, gf_run_stats *stats)
+{
+	gf_run_stats s{};
+	double t0 = now_s();
+	hipStream_t A = c.stream;
+	const gf_chain ch = chain_of(c);
+	const uint64_t sub = 65536ull * 256ull;   // rows the 65,536 streams cover per chain call
+	gf_catalog::Slot &drawn = c.slot[0], &sorted = c.slot[1];
+	for (auto &mp : c.models) {
+		Model &m = *mp;
+		if (!m.wpk.p) return GF_E_STATE;   // the count pass comes first, as in os_skygen::run
+		ensure_table(c, drawn, std::max<uint64_t>(drawn.cap, 1024));
+		gf_skygen_cols dc = skygen_cols_of(drawn.table);
+		uint64_t total = 0;
+		const gf_extinction *ext = c.has_ext ? &c.ext : nullptr;
+		const gf_pencil_beam *pix = (const gf_pencil_beam *)c.pix_dev.p;
+		int rc = gf_skygen_draw_compat(&c.sp, &m.m, pix, ext, nullptr, &c.rng, &dc, drawn.cap, &total, A);
+		if (rc == GF_E_CAPACITY) {   // nothing drawn, streams unchanged
+			ensure_table(c, drawn, total);
+			dc = skygen_cols_of(drawn.table);
+			rc = gf_skygen_draw_compat(&c.sp, &m.m, pix, ext, nullptr, &c.rng, &dc, drawn.cap, &total, A);
+		}
+		if (rc) return rc;
+		s.generated += total;
+		s.expected += m.counts.nstarsExpectedToGenerate;
+		s.n_emit++;
+		ensure_table(c, sorted, std::max<uint64_t>(total, 1));
+		gf_skygen_cols sc = skygen_cols_of(sorted.table);
+		uint64_t nv = 0;
+		rc = gf_skygen_order(&dc, total, &sc, &nv, A);
+		if (rc) return rc;
+		rc = gf_clip_footprint(&c.sp, &c.foot.f, sorted.table.projIdx, sorted.table.projXY, sorted.table.hidden, nv,
+		                       A);
+		if (rc) return rc;
+		for (uint64_t b0 = 0; b0 < nv; b0 += sub) {
+			gf_rows ks{(uint32_t)b0, 256, (uint32_t)std::min(nv, b0 + sub), 0};
+			rc = gf::module_chain(&ks, &ch, &c.rng, &sorted.table, A, nullptr, nullptr);
+			if (rc) return rc;
+			s.n_chain++;
+		}
+		s.rows_chain += nv;
+		uint64_t vis = 0;
+		if (nv) {
+			rc = gf_count_visible(sorted.table.hidden, nv, &vis, A);
+			if (rc) return rc;
+		}
+		s.visible += vis;
+		if (fn && nv) {
+			rc = fn(user, &sorted.table, nv, m.m.comp);
+			if (rc) return rc;
+		}
+	}
+	GF_HIP(hipStreamSynchronize(A));
+	s.t_total = now_s() - t0;
+	if (stats) *stats = s;
+	return 0;
+}
+
 // Events of one chunk: 3 around emission (on the catalog stream), then per
 // chain call 2 around each launch of the module chain (on the stream that
 // runs it).
@@ -1072,15 +1160,7 @@ int run_catalog(gf_catalog &c, gf_chunk_fn fn, void *user, gf_run_stats *stats)
 	gf_run_stats s{};
 	double t0 = now_s();
 	hipStream_t A = c.stream, B = c.overlap ? c.rows : c.stream, C = c.overlap ? c.side : c.stream;
-	gf_chain ch{};
-	if (c.has_feh) ch.feh = &c.feh;
-	if (c.has_gfeh) { ch.gfeh = &c.gfeh; ch.gfeh_comps = c.gfeh_comps; }
-	if (c.has_mult) { ch.mult = &c.mult; ch.mult_comps = c.mult_comps; }
-	if (c.has_bond) ch.bond = &c.bond;
-	if (c.has_phot) { ch.phot = &c.phot; ch.phot_comps = c.phot_comps; }
-	if (c.has_perr) ch.perr = &c.perr;
-	if (c.has_v2pm) ch.v2pm = &c.v2pm;
-	ch.gal2other_coordsys = c.gal2other;
+	const gf_chain ch = chain_of(c);
 	const uint64_t sub = 65536ull * 256ull;   // rows the 65,536 streams cover per chain call
 	const size_t pos_bytes = gf::chain_positions_bytes(&ch, (uint32_t)sub);
 	for (auto &pb : c.pos)
@@ -1227,6 +1307,10 @@ int gf_catalog_set(gf_catalog *c, const char *key, double value)
 	std::string k = key;
 	if (k == "chunk_rows") c->chunk_rows = (uint64_t)value;
 	else if (k == "overlap") c->overlap = value != 0.;
+	else if (k == "draw") {
+		if (value != 0. && value != 1.) return GF_E_ARG;
+		c->draw_mode = (int)value;
+	}
 	else if (k == "acc_mode") c->acc_mode = (int)value;
 	else if (k == "nstars") c->nstars_forced = value;
 	else return GF_E_ARG;
@@ -1286,7 +1370,7 @@ int gf_catalog_run(gf_catalog *c, gf_chunk_fn fn, void *user, gf_run_stats *stat
 	GF_CHECK(c, GF_E_ARG);
 	int rc;
 	try {
-		rc = run_catalog(*c, fn, user, stats);
+		rc = c->draw_mode == 1 ? run_catalog_reference(*c, fn, user, stats) : run_catalog(*c, fn, user, stats);
 	} catch (const ConfigError &e) {
 		g_last_error = e.msg;
 		rc = GF_E_CONFIG;

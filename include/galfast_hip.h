@@ -416,6 +416,21 @@ int gf_skygen_draw_compat(const gf_skygen_params *sp, const gf_model *model,
                           const gf_skygen_cols *out, uint64_t capacity,
                           uint64_t *nstars, void *stream);
 
+/* Reference row order of a drawn batch (skygenHost<T>::drawSources,
+ * skygen/skyconfig_impl.h:467-515, with star_comp :34-68): the visible rows
+ * of `in` (hidden == 0) sorted by (l, b, DM, M, Am) and gathered into `out`,
+ * whose capacity must hold them; *nvisible receives their number (the
+ * reference truncates the table there).  Rows equal on every key keep
+ * their input order. */
+int gf_skygen_order(const gf_skygen_cols *in, uint64_t n, const gf_skygen_cols *out,
+                    uint64_t *nvisible, void *stream);
+
+/* The clipper stage (os_clipper::process, skygen/os_skygen.cpp:997-1044)
+ * for the analytic footprints of this build: a visible row of [0, n) whose
+ * projection-plane position deprojects outside `foot` gets hidden = 1. */
+int gf_clip_footprint(const gf_skygen_params *sp, const gf_footprint *foot, gf_col projIdx,
+                      gf_col projXY, gf_col hidden, uint64_t n, void *stream);
+
 /* Draw plan for the cumulative-table draw below: per (pixel, DM-bin) star
  * counts and their exclusive prefix sums, device resident. */
 typedef struct gf_draw_plan gf_draw_plan;
@@ -485,7 +500,11 @@ typedef int (*gf_chunk_fn)(void *user, const gf_table *cols, uint64_t nrows, int
 /* Message of the last GF_E_CONFIG failure on this thread. */
 const char *gf_last_error(void);
 int  gf_catalog_open(const char *cmd_conf, uint32_t seed, const char *datadir, gf_catalog **out);
-/* Keys: "chunk_rows" (rows per output chunk, default 3e7), "overlap" (1,
+/* Keys: "chunk_rows" (rows per output chunk, default 3e7), "draw" (0, the
+ * default: the cumulative-table draw, chunks in (pixel, DM bin) order with
+ * hidden rows kept; 1: the reference's crawl, each model one batch in the
+ * order of drawSources -- hidden rows dropped, the rest sorted by l, b, DM,
+ * M, Am -- then the clipper and the modules), "overlap" (1,
  * the default: the module chain's row passes run on a second HIP stream
  * while the next rows are emitted and drawn; 0: one stream), "acc_mode"
  * (GF_ACC_*), "nstars" (force the expected total, os_skygen.cpp:886-901). */

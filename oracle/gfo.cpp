@@ -736,6 +736,64 @@ static bool dm_bin(const gf_skygen_params &sp, const gf_extinction &ext, const g
 	return true;
 }
 
+/* drawSources' batch order (skygen/skyconfig_impl.h:467-515): std::sort of
+ * the rows with star_comp (:34-68) -- hidden, then l, b, DM, M, Am, each with
+ * < -- and the table truncated at the first hidden row.  std::stable_sort
+ * here, so rows equal on every key keep their input order (std::sort leaves
+ * it unspecified).  Gathers the visible rows into out; returns their
+ * number. */
+GFO_API uint64_t gfo_order(const gf_skygen_cols *inp, uint64_t n, const gf_skygen_cols *outp)
+{
+	const gf_skygen_cols &in = *inp, &out = *outp;
+	std::vector<uint64_t> s(n);
+	for (uint64_t i = 0; i < n; i++) s[i] = i;
+	auto less = [&](uint64_t a, uint64_t b) {
+		int ha = C<int>(in.hidden, a), hb = C<int>(in.hidden, b);
+		if (ha != hb) return ha < hb;
+		double la = C<double>(in.lb, a, 0), lb = C<double>(in.lb, b, 0);
+		if (la < lb) return true;
+		if (!(la == lb)) return false;
+		double ba = C<double>(in.lb, a, 1), bb = C<double>(in.lb, b, 1);
+		if (ba < bb) return true;
+		if (!(ba == bb)) return false;
+		float da = C<float>(in.DM, a), db = C<float>(in.DM, b);
+		if (da < db) return true;
+		if (!(da == db)) return false;
+		float ma = C<float>(in.M, a), mb = C<float>(in.M, b);
+		if (ma < mb) return true;
+		if (!(ma == mb)) return false;
+		return C<float>(in.Am, a) < C<float>(in.Am, b);
+	};
+	std::stable_sort(s.begin(), s.end(), less);
+	uint64_t nv = 0;
+	while (nv < n && C<int>(in.hidden, s[nv]) == 0) nv++;
+	const gf_col *src[] = {&in.lb, &in.projIdx, &in.projXY, &in.DM, &in.M, &in.XYZ, &in.Am, &in.AmInf, &in.comp,
+	                       &in.hidden};
+	const gf_col *dst[] = {&out.lb, &out.projIdx, &out.projXY, &out.DM, &out.M, &out.XYZ, &out.Am, &out.AmInf,
+	                       &out.comp, &out.hidden};
+	for (int k = 0; k < 10; k++)
+		for (uint32_t p = 0; p < src[k]->width; p++)
+			for (uint64_t j = 0; j < nv; j++) {
+				if (src[k]->elem == 8) C<uint64_t>(*dst[k], j, p) = C<uint64_t>(*src[k], s[j], p);
+				else C<uint32_t>(*dst[k], j, p) = C<uint32_t>(*src[k], s[j], p);
+			}
+	return nv;
+}
+
+/* The clipper stage for the analytic footprints (os_clipper::process,
+ * skygen/os_skygen.cpp:997-1044): a visible row whose projection-plane
+ * position deprojects outside the footprint is hidden. */
+GFO_API void gfo_clip(const gf_skygen_params *sp, const gf_footprint *foot, const gf_skygen_cols *c, uint64_t n)
+{
+	if (foot->kind == 0) return;
+	for (uint64_t r = 0; r < n; r++) {
+		if (C<int>(c->hidden, r)) continue;
+		double l, b;
+		deproject(sp->proj[C<int>(c->projIdx, r)], l, b, C<float>(c->projXY, r, 0), C<float>(c->projXY, r, 1));
+		if (outside_foot(*foot, l, b)) C<int>(c->hidden, r) = 1;
+	}
+}
+
 GFO_API void gfo_count_bins(const gf_skygen_params *spp, const gf_model *mp, const gf_pencil_beam *pixels,
                             const gf_extinction *extp, gf_count_result *out, double *per_pixel, double *wpk)
 {

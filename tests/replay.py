@@ -96,6 +96,44 @@ def replay(view, chunk_rows=30_000_000, pixel_index=None, on_chunk=None, time_bu
             "stopped_early": stopped}
 
 
+def replay_reference(view, on_chunk=None):
+    """The catalog's reference draw mode on the oracle, model by model: the
+    crawl (sp.nthreads threads on streams 0 ..), the batch order of
+    drawSources (sort by hidden, l, b, DM, M, Am; truncate at the first
+    hidden row), the clipper, then the module chain over the visible rows.
+    Returns dict(generated, visible)."""
+    o = O.lib()
+    ns = view.nstreams
+    state = np.ctypeslib.as_array(view.rng_state0, shape=(3 * ns,)).copy()
+    sp = A.gf_skygen_params.from_buffer_copy(view.sp)
+    pix = (A.gf_pencil_beam * sp.npixels).from_address(view.pixels)
+    spec = catalog_spec(nbands=max(view.nbands, 1), nabsmag=max(view.nabsmag, 1))
+    foot = A.gf_footprint.from_buffer_copy(view.foot)
+    sub = 65536 * 256
+    generated = visible = 0
+    for mi in range(view.nmodels):
+        m = view.models[mi]
+        n = o.gfo_draw_compat(C.byref(sp), C.byref(m), C.cast(pix, C.c_void_p), _ext(view), None,
+                              O.ptr(state.copy()), ns, None, 0)
+        drawn = HostTable(max(n, 1), spec)
+        dcols = drawn.skygen_cols()
+        assert o.gfo_draw_compat(C.byref(sp), C.byref(m), C.cast(pix, C.c_void_p), _ext(view), None, O.ptr(state),
+                                 ns, C.byref(dcols), n) == n
+        generated += n
+        t = HostTable(max(n, 1), spec)
+        cols = t.skygen_cols()
+        nv = o.gfo_order(C.byref(dcols), n, C.byref(cols))
+        o.gfo_clip(C.byref(sp), C.byref(foot), C.byref(cols), nv)
+        tab = t.table()
+        for b0 in range(0, nv, sub):
+            ks = A.gf_rows(b0, 256, min(nv, b0 + sub), 0)
+            assert o.gfo_module_chain(C.byref(ks), C.byref(view.chain), O.ptr(state), ns, C.byref(tab)) == 0
+        visible += int((t.arr("hidden")[0, :nv] == 0).sum())
+        if on_chunk is not None and nv:
+            on_chunk(t, nv, m.comp)
+    return {"generated": generated, "visible": visible}
+
+
 def _shift(cols, row0):
     """Copy of a gf_skygen_cols / gf_table whose columns start row0 rows later."""
     out = type(cols).from_buffer_copy(cols)

@@ -70,6 +70,44 @@ def test_c0_catalog_matches_oracle_replay(c0):
     assert nvis > 0.5 * expected
 
 
+def check_reference_draw(cmd, seed):
+    """The catalog's reference draw mode: the crawl, the batch order of
+    drawSources (hidden rows dropped, the rest sorted by l, b, DM, M, Am),
+    the clipper and the module chain over the visible rows; one batch per
+    model, every column bit-identical to the oracle's restatement.
+    Returns (rows handed out, rows drawn, rows hidden by the clipper)."""
+    cat = Catalog(cmd, seed=seed)
+    cat.set("draw", 1)
+    cat.count()
+    view = R.host_view(G.lib(), cat._h)
+    spec = catalog_spec(nbands=view.nbands, nabsmag=view.nabsmag)
+    got = collect(cat, spec)
+    ref = []
+    info = R.replay_reference(view, on_chunk=lambda t, n, comp: ref.append((t, n, comp)))
+    cat.close()
+    assert len(got) == view.nmodels
+    assert [(n, c) for _, n, c in got] == [(n, c) for _, n, c in ref]
+    clipped = 0
+    for (a, n, _), (b, _, _) in zip(got, ref):
+        assert np.all(np.diff(b.arr("lb")[0, :n]) >= 0)
+        assert np.array_equal(a.arr("hidden")[:, :n], b.arr("hidden")[:, :n])
+        vis = b.arr("hidden")[0, :n] == 0
+        clipped += int((~vis).sum())
+        for c in spec:
+            x = np.ascontiguousarray(a.arr(c)[:, :n][:, vis])
+            y = np.ascontiguousarray(b.arr(c)[:, :n][:, vis])
+            assert np.array_equal(x.view(np.uint8), y.view(np.uint8)), c
+        for c in ("lb", "projIdx", "projXY", "DM", "absmag", "XYZ", "comp", "Am", "AmInf"):
+            assert np.array_equal(a.arr(c)[:, :n], b.arr(c)[:, :n]), c
+    return sum(n for _, n, _ in got), info["generated"], clipped
+
+
+def test_c0_reference_draw_matches_oracle(c0):
+    out, drawn, clipped = check_reference_draw(c0, 42)
+    assert 0 < out <= drawn
+    assert clipped > 0   # the rect footprint's edge pixels
+
+
 def test_c0_generated_matches_expected(c0):
     cat = Catalog(c0, seed=7)
     cat.count()
@@ -82,6 +120,13 @@ def test_c0_generated_matches_expected(c0):
 def c4s(tmp_path_factory):
     d = str(tmp_path_factory.mktemp("c4s"))
     return configs.write_c4_small(d)
+
+
+def test_c4_reference_draw_matches_oracle(c4s):
+    """With extinction, stars pushed past the faint limit are hidden by the
+    draw and dropped by the batch order."""
+    out, drawn, _ = check_reference_draw(c4s, 5)
+    assert 0 < out < drawn
 
 
 def test_c4_extinction_multiples_six_bands_match_oracle_replay(c4s):

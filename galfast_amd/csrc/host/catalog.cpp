@@ -1491,6 +1491,10 @@ struct Worker {
 	uint64_t table_cap = 0;
 	gf_table T{};
 	std::thread th;
+	// the module chain captured once (serve_chain_graph) and its batch arguments
+	hipGraphExec_t chain_exec = nullptr;
+	DevBuf chain_args;
+	gf_serve_chain_args chain_args_h{};
 };
 
 } // namespace
@@ -1697,9 +1701,24 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	}
 	GF_HIP(hipMemcpyAsync(w.reqs.p, rq.data(), rq.size() * sizeof(gf_serve_req), hipMemcpyHostToDevice, st));
 	grow<uint2>(w.pos, std::max<size_t>(1, (size_t)s.hchain.nplanes * nrows));
-	rc = gf::serve_chain((const gf_serve_req *)w.reqs.p, nr, loff[nr], (const gf_serve_chain *)s.dchain.p, s.hchain,
-	                     nrows, w.T, (uint2 *)w.pos.p, st);
-	if (rc) return rc;
+	if (s.cfg.eager_chain) {
+		rc = gf::serve_chain((const gf_serve_req *)w.reqs.p, nr, loff[nr], (const gf_serve_chain *)s.dchain.p,
+		                     s.hchain, nrows, w.T, (uint2 *)w.pos.p, st);
+		if (rc) return rc;
+	} else if (nrows) {
+		if (!w.chain_exec) {
+			w.chain_args.alloc(sizeof(gf_serve_chain_args));
+			rc = gf::serve_chain_graph((const gf_serve_chain_args *)w.chain_args.p, (const gf_serve_chain *)s.dchain.p,
+			                           s.hchain, st, &w.chain_exec);
+			if (rc) return rc;
+		}
+		// the previous batch's replay has finished (its results were read back)
+		w.chain_args_h = gf_serve_chain_args{(const gf_serve_req *)w.reqs.p, nr, loff[nr], nrows, w.T,
+		                                     gf_positions{(uint2 *)w.pos.p, nrows}};
+		GF_HIP(hipMemcpyAsync(w.chain_args.p, &w.chain_args_h, sizeof(gf_serve_chain_args), hipMemcpyHostToDevice,
+		                      st));
+		GF_HIP(hipGraphLaunch(w.chain_exec, st));
+	}
 	uint64_t maxn = 0;
 	for (int r = 0; r < nr; r++) maxn = std::max(maxn, roff[r + 1] - roff[r]);
 	grow<unsigned long long>(w.vis, (size_t)nr + 1);
@@ -1967,6 +1986,7 @@ void gf_server_destroy(gf_server *s)
 		if (t.joinable()) t.join();
 	for (auto &w : s->workers) {
 		if (w->th.joinable()) w->th.join();
+		if (w->chain_exec) (void)hipGraphExecDestroy(w->chain_exec);
 		if (w->st) (void)hipStreamDestroy(w->st);
 	}
 	if (s->base.stream) (void)hipStreamDestroy(s->base.stream);

@@ -36,6 +36,17 @@ struct gf_serve_req {
 // Module parameter blocks of the server (device copy).
 typedef gf_chain_mods gf_serve_chain;
 
+// What the captured module chain of a batch reads from device memory: the
+// batch's requests, lanes and rows, its table and position planes.
+struct gf_serve_chain_args {
+	const gf_serve_req *reqs;
+	int32_t nreq;
+	uint32_t nlanes;
+	uint64_t nrows;
+	gf_table T;
+	gf_positions P;
+};
+
 namespace gf {
 int serve_count(const gf_serve_seg *dsegs, int nseg, uint32_t nslots, const gf_model *dmodels,
                 const gf_pencil_beam *dpix, double *W, double *pix_sum, double *seg_expected, hipStream_t st);
@@ -50,6 +61,12 @@ int serve_emit_rows(const gf_serve_seg *dsegs, int nseg, const gf_model *dmodels
 // pos: hch.nplanes * nrows stream positions (device scratch of the caller)
 int serve_chain(const gf_serve_req *dreqs, int nreq, uint32_t nlanes, const gf_serve_chain *dch,
                 const gf_serve_chain &hch, uint64_t nrows, const gf_table &T, uint2 *pos, hipStream_t st);
+// The module chain of serve_chain captured as a HIP graph whose launches
+// read the batch from *dargs (fixed grids that stride over it), so that one
+// instantiation serves every batch: write the batch's gf_serve_chain_args to
+// dargs, then launch *exec on the same stream.
+int serve_chain_graph(const gf_serve_chain_args *dargs, const gf_serve_chain *dch, const gf_serve_chain &hch,
+                      hipStream_t st, hipGraphExec_t *exec);
 // vis[r] = rows of request r with hidden == 0 (max_rows: the largest request)
 int serve_visible(const gf_serve_req *dreqs, int nreq, uint64_t max_rows, const gf_col &hidden,
                   unsigned long long *vis, hipStream_t st);

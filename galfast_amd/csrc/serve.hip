@@ -189,12 +189,10 @@ __global__ void k_b_seg_rows(gf_serve_seg *segs, int nseg, const uint64_t *offs)
 // chunks s, s + S, s + 2S, ... and runs the draw pass on them in that order
 // (gf_chain.h); the row pass runs one lane per batch row.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(gfc::BS) k_b_draws(const gf_serve_req *reqs, int nreq, uint32_t nlanes,
-                                                     const gf_serve_chain *ch, gf_table T, gf_positions P)
+// One lane g of the batch's draw pass.
+GF_D void serve_draw_lane(const gf_serve_req *reqs, int nreq, uint32_t g, const gf_serve_chain &ch,
+                          const gf_table &T, const gf_positions &P, gfc::DrawLds &L)
 {
-	__shared__ gfc::DrawLds L;
-	uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-	if (g >= nlanes) return;
 	int ri = upper_index(reqs, nreq, [](const gf_serve_req &q) { return (uint64_t)q.lane0; }, g);
 	const gf_serve_req &rq = reqs[ri];
 	uint32_t s = g - rq.lane0;
@@ -204,9 +202,33 @@ __global__ void __launch_bounds__(gfc::BS) k_b_draws(const gf_serve_req *reqs, i
 	for (uint64_t c = s; c * 256 < nrows; c += rq.S) {
 		uint32_t r0 = (uint32_t)(rq.row0 + c * 256);
 		uint32_t r1 = (uint32_t)(rq.row0 + std::min<uint64_t>(nrows, (c + 1) * 256));
-		gfc::chunk_draws<true>(*ch, rng, r0, r1, r0, T, P, L, threadIdx.x);
+		gfc::chunk_draws<true>(ch, rng, r0, r1, r0, T, P, L, threadIdx.x);
 	}
 	gf_mwc_store(rq.state, rq.S, s, rng);
+}
+
+// One row of the batch's row pass group GROUP.
+template <int GROUP, int NT>
+GF_D void serve_row(const gf_serve_req *reqs, int nreq, const gf_serve_chain &ch, uint64_t row, const gf_table &T,
+                    const gf_positions &P, int need_stream)
+{
+	uint32_t a = 0;
+	if (need_stream) {
+		int ri = upper_index(reqs, nreq, [](const gf_serve_req &q) { return q.row0; }, row);
+		const gf_serve_req &rq = reqs[ri];
+		uint64_t c = (row - rq.row0) / 256;
+		a = rq.state[c % rq.S];
+	}
+	gfc::chain_rows<GROUP, NT>(ch, a, (uint32_t)row, row, T, P);
+}
+
+__global__ void __launch_bounds__(gfc::BS) k_b_draws(const gf_serve_req *reqs, int nreq, uint32_t nlanes,
+                                                     const gf_serve_chain *ch, gf_table T, gf_positions P)
+{
+	__shared__ gfc::DrawLds L;
+	uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+	if (g >= nlanes) return;
+	serve_draw_lane(reqs, nreq, g, *ch, T, P, L);
 }
 
 template <int GROUP, int NT>
@@ -215,14 +237,31 @@ __global__ void __launch_bounds__(gfc::BS) k_b_rows(const gf_serve_req *reqs, in
 {
 	uint64_t row = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (row >= nrows) return;
-	uint32_t a = 0;
-	if (need_stream) {
-		int ri = upper_index(reqs, nreq, [](const gf_serve_req &q) { return q.row0; }, row);
-		const gf_serve_req &rq = reqs[ri];
-		uint64_t c = (row - rq.row0) / 256;
-		a = rq.state[c % rq.S];
-	}
-	gfc::chain_rows<GROUP, NT>(*ch, a, (uint32_t)row, row, T, P);
+	serve_row<GROUP, NT>(reqs, nreq, *ch, row, T, P, need_stream);
+}
+
+// The same passes for the captured chain: fixed grids that stride over the
+// batch, which they read from the batch's argument block.
+constexpr unsigned GRAPH_DRAW_BLOCKS = 1024;   // 262,144 lanes a sweep
+constexpr unsigned GRAPH_ROW_BLOCKS = 8192;    // 2,097,152 rows a sweep
+
+__global__ void __launch_bounds__(gfc::BS) k_g_draws(const gf_serve_chain_args *args, const gf_serve_chain *ch)
+{
+	__shared__ gfc::DrawLds L;
+	const gf_serve_chain_args a = *args;
+	const uint32_t stride = gridDim.x * blockDim.x;
+	for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < a.nlanes; g += stride)
+		serve_draw_lane(a.reqs, a.nreq, g, *ch, a.T, a.P, L);
+}
+
+template <int GROUP, int NT>
+__global__ void __launch_bounds__(gfc::BS) k_g_rows(const gf_serve_chain_args *args, const gf_serve_chain *ch,
+                                                    int need_stream)
+{
+	const gf_serve_chain_args a = *args;
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	for (uint64_t row = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; row < a.nrows; row += stride)
+		serve_row<GROUP, NT>(a.reqs, a.nreq, *ch, row, a.T, a.P, need_stream);
 }
 
 // Visible rows of each request: a workgroup per (request, 64K-row slice).
@@ -352,6 +391,35 @@ int serve_chain(const gf_serve_req *dreqs, int nreq, uint32_t nlanes, const gf_s
 		GF_LAUNCHED();
 	}
 	return 0;
+}
+
+int serve_chain_graph(const gf_serve_chain_args *dargs, const gf_serve_chain *dch, const gf_serve_chain &hch,
+                      hipStream_t st, hipGraphExec_t *exec)
+{
+	GF_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+	// launches only: nothing that a capture would record as a memset or copy
+	const unsigned gr = GRAPH_ROW_BLOCKS, gd = GRAPH_DRAW_BLOCKS;
+	if (hch.nplanes || hch.has_mult) k_g_draws<<<gd, gfc::BS, 0, st>>>(dargs, dch);
+	if (gfc::rows_on(hch, gfc::ROWS_FEH)) k_g_rows<gfc::ROWS_FEH, 1><<<gr, gfc::BS, 0, st>>>(dargs, dch, 1);
+	if (gfc::rows_on(hch, gfc::ROWS_BOND)) k_g_rows<gfc::ROWS_BOND, 1><<<gr, gfc::BS, 0, st>>>(dargs, dch, 1);
+	if (gfc::rows_on(hch, gfc::ROWS_PHOT)) {
+		switch (gfc::phot_tables(hch)) {
+		case 1: k_g_rows<gfc::ROWS_PHOT, 1><<<gr, gfc::BS, 0, st>>>(dargs, dch, 0); break;
+		case 2: k_g_rows<gfc::ROWS_PHOT, 2><<<gr, gfc::BS, 0, st>>>(dargs, dch, 0); break;
+		case 3: k_g_rows<gfc::ROWS_PHOT, 3><<<gr, gfc::BS, 0, st>>>(dargs, dch, 0); break;
+		default: k_g_rows<gfc::ROWS_PHOT, 4><<<gr, gfc::BS, 0, st>>>(dargs, dch, 0); break;
+		}
+	}
+	if (gfc::rows_on(hch, gfc::ROWS_PERR)) k_g_rows<gfc::ROWS_PERR, 1><<<gr, gfc::BS, 0, st>>>(dargs, dch, 1);
+	if (gfc::rows_on(hch, gfc::ROWS_ASTRO)) k_g_rows<gfc::ROWS_ASTRO, 1><<<gr, gfc::BS, 0, st>>>(dargs, dch, 0);
+	hipError_t le = hipGetLastError();
+	hipGraph_t g = nullptr;
+	hipError_t ce = hipStreamEndCapture(st, &g);
+	if (le != hipSuccess) { if (g) (void)hipGraphDestroy(g); return (int)le; }
+	if (ce != hipSuccess) return (int)ce;
+	hipError_t ie = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+	(void)hipGraphDestroy(g);
+	return (int)ie;
 }
 
 } // namespace gf

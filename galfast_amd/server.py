@@ -18,7 +18,7 @@ class gf_result(C.Structure):
 
 class gf_server_config(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("max_batch", C.c_int32), ("workers", C.c_int32),
-                ("plain", C.c_int32), ("linger_us", C.c_int32)]
+                ("plain", C.c_int32), ("linger_us", C.c_int32), ("eager_chain", C.c_int32), ("pad", C.c_int32)]
 
 
 class gf_server_stats(C.Structure):
@@ -56,9 +56,10 @@ class Server:
     """
 
     def __init__(self, cmd_conf, on_result=None, hbm_budget=64 << 30, max_batch=64, workers=2, plain=False,
-                 linger_us=0, datadir=DATA_DIR):
+                 linger_us=0, eager_chain=False, datadir=DATA_DIR):
         L = _declare()
-        cfg = gf_server_config(hbm_budget, max_batch, workers, 1 if plain else 0, linger_us)
+        cfg = gf_server_config(hbm_budget, max_batch, workers, 1 if plain else 0, linger_us,
+                               1 if eager_chain else 0, 0)
         self.results = []
         self._user_cb = on_result
 

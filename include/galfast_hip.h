@@ -578,6 +578,9 @@ typedef struct gf_server_config {
     int32_t  plain;         /* 1: no batching and no overlap (one request at a time) */
     int32_t  linger_us;     /* a worker with fewer than max_batch ready requests waits up
                              * to this long for more before it launches */
+    int32_t  eager_chain;   /* 1: launch the module chain of each batch directly instead of
+                             * replaying the worker's captured HIP graph of it */
+    int32_t  pad;
 } gf_server_config;
 
 typedef struct gf_server_stats {

@@ -114,6 +114,17 @@ def test_request_rows_same_alone_and_in_mix(base, alone):
         same_rows(t, ref[rid][1], "request %d (reversed)" % rid)
 
 
+def test_captured_chain_matches_direct_launches(base, alone):
+    """Every batch replays the worker's captured HIP graph of the module
+    chain (plan, emission and the count run eagerly between replays); the
+    same requests with the chain launched directly give the same rows."""
+    ref, _ = alone
+    eager, st = serve(base, REQUESTS, plain=True, eager_chain=True)
+    assert st.completed == len(REQUESTS)
+    for rid, (res, t) in eager.items():
+        same_rows(t, ref[rid][1], "request %d (direct launches)" % rid)
+
+
 @pytest.mark.parametrize("rid", [1, 3, 5])
 def test_request_matches_oracle_replay(tmp_path, alone, rid):
     got, _ = alone

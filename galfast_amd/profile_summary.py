@@ -99,15 +99,21 @@ def instruction_mix(a, ks, kt_cmd):
             for k, (v, n) in counter_per_dispatch(d, c).items():
                 per.setdefault(k, {})[c] = v
     kernels = [k for k in sorted(ks, key=lambda k: -ks[k]["total_ns"]) if k in per and per[k].get("SQ_WAVES")]
-    classes = [c for c in sorted({c for k in kernels for c in per[k]}) if c.startswith("SQ_INSTS_")]
+    classes = [c for c in sorted({c for k in kernels for c in per[k]})
+               if c.startswith("SQ_INSTS_") and "_FLOPS_" not in c]
+    flops = [c for c in ("SQ_INSTS_VALU_FLOPS_FP32", "SQ_INSTS_VALU_FLOPS_FP64") if any(c in per[k] for k in kernels)]
     lines = ["# Instruction mix per launch: %s" % a.tag, "", "Commands:", ""] + ["* `%s`" % c for c in pmc_cmds] + \
             ["* durations: `%s`" % kt_cmd, "",
              "SIMD cycles per VALU instruction = 1,024 SIMDs x 2.4 GHz x mean kernel-trace duration / "
              "SQ_INSTS_VALU per launch (a wave64 FMA occupies a SIMD for 4 cycles: near 4 or below, the vector "
              "units issue all the time).  Counts are per wave (64 lanes).", "",
+             ] + (["FP32 / FP64 TFLOP/s = SQ_INSTS_VALU_FLOPS_FP32 / _FP64 (floating-point operations per wave "
+                   "instruction, an FMA counting 2) x 64 lanes x waves / mean duration; MI355X vector peaks: FP32 "
+                   "157.3 TFLOP/s (packed FP32 math), FP64 78.6 TFLOP/s.", ""] if flops else []) + [
              "| kernel | waves / launch | VALU / launch | mean us | SIMD cycles / VALU | " +
+             "".join("%s TFLOP/s | " % c[-4:] for c in flops) +
              " | ".join(c.replace("SQ_INSTS_", "") + " / wave" for c in classes) + " |",
-             "|---|---|---|---|---|" + "---|" * len(classes)]
+             "|---|---|---|---|---|" + "---|" * (len(flops) + len(classes))]
     out = {"_simd_gcycles_per_s": SIMD_GCYCLES, "_program": a.program}
     for k in kernels:
         e = per[k]
@@ -116,7 +122,12 @@ def instruction_mix(a, ks, kt_cmd):
         avg = ks[k]["avg_ns"]
         cpi = SIMD_GCYCLES * avg / valu if valu else 0.0
         out[k] = {"waves": w, "valu_insts": valu, "avg_ns": avg, "simd_cycles_per_valu": cpi}
-        lines.append("| `%s` | %.0f | %.3g | %.1f | %.1f | " % (k, w, valu, avg / 1e3, cpi) +
+        tf = []
+        for c in flops:
+            t = e.get(c, 0.0) * 64.0 / avg / 1e3 if avg else 0.0   # TFLOP/s
+            out[k][c[-4:].lower() + "_tflops"] = t
+            tf.append("%.2f | " % t)
+        lines.append("| `%s` | %.0f | %.3g | %.1f | %.1f | " % (k, w, valu, avg / 1e3, cpi) + "".join(tf) +
                      " | ".join("%.1f" % (e.get(c, 0.0) / w) for c in classes) + " |")
     with open(os.path.join(a.out, "%s_instruction_mix.md" % a.tag), "w") as f:
         f.write("\n".join(lines) + "\n")

@@ -43,6 +43,9 @@ def _declare():
     L.gf_catalog_run.argtypes = [C.c_void_p, CHUNK_FN, C.c_void_p, P(gf_run_stats)]
     L.gf_catalog_close.argtypes = [C.c_void_p]
     L.gf_catalog_close.restype = None
+    L.gf_catalog_component_ids.argtypes = [C.c_void_p, P(C.c_uint32), C.c_int]
+    L.gf_writer_open.argtypes = [C.c_void_p, C.c_char_p, C.c_int, P(C.c_void_p)]
+    L.gf_writer_close.argtypes = [C.c_void_p, P(C.c_uint64)]
     L._gf_catalog_declared = True
     return L
 
@@ -93,6 +96,27 @@ class Catalog:
             raise self._cb_error
         check(rc, "gf_catalog_run")
         return st
+
+    def component_ids(self):
+        n = lib().gf_catalog_component_ids(self._h, None, 0)
+        ids = (C.c_uint32 * max(n, 1))()
+        lib().gf_catalog_component_ids(self._h, ids, n)
+        return list(ids[:n])
+
+    def write(self, path, kind="fits"):
+        """Run the catalog into a file: the reference's output stage
+        (kind "fits": os_fitsout, "text": os_textout) for the visible rows.
+        Returns (run statistics, rows written)."""
+        L = lib()
+        w = C.c_void_p()
+        check(L.gf_writer_open(self._h, path.encode(), {"text": 0, "fits": 1}[kind], C.byref(w)), "gf_writer_open")
+        st = gf_run_stats()
+        rc = L.gf_catalog_run(self._h, C.cast(L.gf_writer_chunk, CHUNK_FN), w, C.byref(st))
+        rows = C.c_uint64()
+        rc2 = L.gf_writer_close(w, C.byref(rows))
+        check(rc, "gf_catalog_run")
+        check(rc2, "gf_writer_close")
+        return st, rows.value
 
     def close(self):
         if self._h:

@@ -86,5 +86,8 @@ enum gf_scratch_slot {
 	GF_SLOT_UNITS,
 	GF_SLOT_ORDER,
 	GF_SLOT_ORDER_TMP,
+	GF_SLOT_OUT,
+	GF_SLOT_OUT_COLS,
+	GF_SLOT_OUT_COUNT,
 	GF_SLOT_NSLOTS
 };

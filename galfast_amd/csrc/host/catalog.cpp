@@ -141,6 +141,7 @@ struct gf_catalog {
 	DevBuf phot_dev[8], phot_il;
 	int nbands = 0;
 	std::vector<std::string> band_names;
+	std::string bandset;   // the photometric system's name (photometry_host.cpp:94)
 	gf_photoerr_data perr{};
 	DevBuf perr_dev[2];
 	gf_vel2pm_data v2pm{};
@@ -702,6 +703,7 @@ void construct_photometry(gf_catalog &c, const Config &cfg)
 {
 	c.phot_comps = read_component_map(cfg, "comp", c.cm);
 	std::string bandset = cfg.get("bandset", "LSSTugrizy");
+	c.bandset = bandset;
 	c.band_names = split_ws(cfg.get("bands", "LSSTu LSSTg LSSTr LSSTi LSSTz LSSTy"));
 	c.nbands = (int)c.band_names.size();
 	int ncolors = c.nbands - 1;
@@ -1166,6 +1168,11 @@ int run_catalog(gf_catalog &c, gf_chunk_fn fn, void *user, gf_run_stats *stats)
 	for (auto &pb : c.pos)
 		if (pos_bytes && pb.bytes < pos_bytes) pb.alloc(pos_bytes);
 	unsigned long long *vis_dev = (unsigned long long *)c.vis_dev.p;
+	auto release = [](gf_draw_plan *p) {
+		(void)hipDeviceSynchronize();
+		gf_skygen_plan_free(p);
+	};
+	std::vector<std::unique_ptr<gf_draw_plan, decltype(release)>> plans;
 	uint32_t pb = 0;      // position buffer of the next chain call
 	uint32_t last = 0;    // position buffer of the last chain call queued
 	uint64_t k = 0;       // chunks queued
@@ -1179,12 +1186,11 @@ int run_catalog(gf_catalog &c, gf_chunk_fn fn, void *user, gf_run_stats *stats)
 		rc = gf_skygen_plan(&c.sp, &m.m, (const gf_pencil_beam *)c.pix_dev.p, c.has_ext ? &c.ext : nullptr,
 		                    (const double *)m.wpk.p, &c.rng, &plan, &total, A);
 		if (rc) return rc;
-		// the plan is read by the emission kernels on the catalog stream
-		auto release = [A](gf_draw_plan *p) {
-			(void)hipStreamSynchronize(A);
-			gf_skygen_plan_free(p);
-		};
-		std::unique_ptr<gf_draw_plan, decltype(release)> hold(plan, release);
+		// The plan is read by the emission kernels on the catalog stream; it
+		// is freed once the run is over, because freeing device memory waits
+		// for the whole device and would stall the next model's plan behind
+		// this model's last row passes.
+		plans.emplace_back(plan, release);
 		GF_HIP(hipEventRecord(c.ev[1], A));
 		GF_HIP(hipEventSynchronize(c.ev[1]));
 		s.t_plan += elapsed_s(c.ev[0], c.ev[1]);
@@ -1317,6 +1323,14 @@ int gf_catalog_set(gf_catalog *c, const char *key, double value)
 	return 0;
 }
 
+int gf_catalog_component_ids(const gf_catalog *c, uint32_t *ids, int cap)
+{
+	GF_CHECK(c && (ids || cap == 0), GF_E_ARG);
+	const int n = (int)c->cm.seq2comp.size();
+	for (int i = 0; i < n && i < cap; i++) ids[i] = c->cm.seq2comp[i];
+	return n;
+}
+
 int gf_catalog_info(const gf_catalog *c, gf_catalog_info_t *info)
 {
 	GF_CHECK(c && info, GF_E_ARG);
@@ -1384,6 +1398,108 @@ int gf_catalog_run(gf_catalog *c, gf_chunk_fn fn, void *user, gf_run_stats *stat
 	}
 	return rc;
 }
+
+} // extern "C"
+
+namespace gfh {
+
+// The columns the reference's output stages write for this job, in its
+// order (otable::getSortedColumnsForOutput, otable.cpp:502-540: ranked by
+// the first of lb, radec, XYZ, DM, absmag, comp, FeH, vcyl, pmlb, pmradec,
+// Am, AmInf that the column or one of its aliases is named, the others after
+// them by case-insensitive name), with the built-in and module column
+// definitions (otable.cpp:715-745; os_skygen.cpp:841-858;
+// photometry_host.cpp:108-122; photometricErrors_host.cpp:156-177;
+// unresolvedMultiples_host.cpp:62-77; GaussianFeH_host.cpp:46-47;
+// vel2pm_host.cpp:80-83; gal2other_host.cpp:74).  Hidden columns (hidden,
+// projIdx, projXY) are not written.
+void output_schema(const gf_catalog &c, std::vector<OutColumn> &out)
+{
+	struct Col {
+		OutColumn oc;
+		std::vector<std::string> aliases;
+	};
+	std::vector<Col> cols;
+	auto add = [&](std::string name, gf_col gf_table::*m, uint32_t w, char type, std::string fmt, std::string def,
+	               std::vector<std::string> aliases = {}, bool comp = false) {
+		Col k;
+		k.oc.name = name;
+		k.oc.member = m;
+		k.oc.width = w;
+		k.oc.type = type;
+		k.oc.fmt = fmt;
+		k.oc.def = def;
+		k.oc.map_comp = comp;
+		k.aliases = aliases;
+		cols.push_back(k);
+	};
+	auto fields = [](const std::vector<std::string> &names, const std::string &prefix) {
+		std::string f = "fieldNames=";
+		for (size_t i = 0; i < names.size(); i++) f += (i ? "," : "") + std::to_string(i) + ":" + prefix + names[i];
+		return f + ";";
+	};
+	const std::string absmag = "abs" + c.band;
+	add("lb", &gf_table::lb, 2, 'D', "% 13.8f", "lb[2]");
+	add("XYZ", &gf_table::XYZ, 3, 'E', "% 10.2f", "XYZ[3]");
+	add("DM", &gf_table::DM, 1, 'E', "% 7.3f", "DM");
+	if (c.has_mult)
+		add(absmag, &gf_table::absmag, 1, 'E', "% 7.3f", absmag + "{type=float;alias=absmag;band=" + c.band + ";}",
+		    {"absmag"});
+	else
+		add(absmag, &gf_table::absmag, 1, 'E', "% 7.3f",
+		    absmag + "{type=float;alias=M1;alias=absmag;band=" + c.band + ";}", {"M1", "absmag"});
+	add("comp", &gf_table::comp, 1, 'J', "%3d", "comp", {}, true);
+	add("Am", &gf_table::Am, 1, 'E', "%6.3f", "Am");
+	add("AmInf", &gf_table::AmInf, 1, 'E', "%6.3f", "AmInf");
+	if (c.has_feh || c.has_gfeh) add("FeH", &gf_table::FeH, 1, 'E', "% 6.3f", "FeH");
+	if (c.has_gfeh) add("FeH_comp", &gf_table::FeH_comp, 1, 'J', "%3d", "FeH_comp{type=int;fmt=%3d;}");
+	if (c.has_bond) add("vcyl", &gf_table::vcyl, 3, 'E', "% 7.1f", "vcyl[3]");
+	if (c.has_v2pm) {
+		const char *pm = c.v2pm.coordsys == 1 ? "pmradec" : "pmlb";
+		add(pm, &gf_table::pm, 3, 'E', "% 8.2f", std::string(pm) + "[3]");
+	}
+	if (c.gal2other == 1) add("radec", &gf_table::radec, 2, 'D', "% 13.8f", "radec[2]");
+	if (c.has_phot) {
+		const std::string n = std::to_string(c.nbands);
+		add(c.bandset, &gf_table::mags, (uint32_t)c.nbands, 'E', "% 7.3f",
+		    c.bandset + "[" + n + "]{type=float;class=magnitude;" + fields(c.band_names, "") + "}");
+		add(c.bandset + "PhotoFlags", &gf_table::photoFlags, 1, 'J', "% 4d",
+		    c.bandset + "PhotoFlags{type=int;class=flags;}");
+		if (c.has_perr)
+			add("obs" + c.bandset, &gf_table::obsMags, (uint32_t)c.nbands, 'E', "% 7.3f",
+			    "obs" + c.bandset + "[" + n + "]{type=float;class=magnitude;" + fields(c.band_names, "obs") + "}");
+	}
+	if (c.has_mult) {
+		add(absmag + "Sys", &gf_table::absmagSys, 2, 'E', "% 7.3f",
+		    absmag + "Sys[2]{type=float;class=magnitude;fieldNames=0:M1,1:M2;alias=absmagSys;band=" + c.band + ";}",
+		    {"absmagSys"});
+		add(absmag + "SysNcomp", &gf_table::absmagSysNcomp, 1, 'J', "%1d",
+		    absmag + "SysNcomp{type=int;fmt=%1d;}");
+	}
+	static const char *ranked[] = {"lb", "radec", "XYZ", "DM", "absmag", "comp", "FeH", "vcyl", "pmlb", "pmradec",
+	                               "Am", "AmInf"};
+	auto rank = [&](const Col &k) {
+		int r = 1000;
+		for (int i = 0; i < 12; i++) {
+			if (k.oc.name == ranked[i]) r = std::min(r, i + 1);
+			for (auto &a : k.aliases)
+				if (a == ranked[i]) r = std::min(r, i + 1);
+		}
+		return r;
+	};
+	std::stable_sort(cols.begin(), cols.end(), [&](const Col &a, const Col &b) {
+		int ra = rank(a), rb = rank(b);
+		return ra == rb ? strcasecmp(a.oc.name.c_str(), b.oc.name.c_str()) < 0 : ra < rb;
+	});
+	out.clear();
+	for (auto &k : cols) out.push_back(k.oc);
+}
+
+const std::vector<uint32_t> &component_ids(const gf_catalog &c) { return c.cm.seq2comp; }
+
+} // namespace gfh
+
+extern "C" {
 
 int gf_catalog_host_view_get(const gf_catalog *c, gf_catalog_host_view *v)
 {

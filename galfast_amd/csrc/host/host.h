@@ -4,6 +4,7 @@
 #pragma once
 
 #include <map>
+#include <strings.h>
 #include <memory>
 #include <string>
 #include <vector>
@@ -91,5 +92,19 @@ void texcoord_from_range(float imgx, float imgy, float x, float y, float tc[2]);
 // Coordinate transform of a model or module (common/transform.cpp:345-388).
 void load_transform(float T[3], float M[3][3], const std::string &center, const std::string &orientation,
                     const Config &cfg);
+
+// One column of the output stages: its name, the catalog table column, its
+// FITS form (width, 'E' / 'D' / 'J'), its text format and its otable
+// definition (the text header and the FITS METADATA extension).
+struct OutColumn {
+	std::string name;
+	gf_col gf_table::*member = nullptr;
+	uint32_t width = 1;
+	char type = 'E';
+	std::string fmt, def;
+	bool map_comp = false;   // write the user's component id
+};
+void output_schema(const gf_catalog &c, std::vector<OutColumn> &cols);
+const std::vector<uint32_t> &component_ids(const gf_catalog &c);
 
 } // namespace gfh

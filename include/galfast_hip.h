@@ -458,6 +458,40 @@ int gf_skygen_emit(const gf_draw_plan *plan, uint64_t row_begin, uint64_t row_en
 int gf_skygen_plan_free(gf_draw_plan *plan);
 
 /* ---------------------------------------------------------------------- */
+/* Output stages (os_fitsout, pipeline.cpp:767-1035; os_textout, :199-288)  */
+/* ---------------------------------------------------------------------- */
+
+/* One output column: a device column of col.width fields per row. */
+typedef struct gf_outcol {
+    gf_col          col;
+    const uint32_t *map;     /* int columns: a value v < nmap is written as map[v]
+                              * (device memory; the sequential component index to the
+                              * user's component id), or NULL */
+    uint32_t        nmap;
+    char            type;    /* 'E' float, 'D' double, 'J' int32 (the FITS TFORM codes) */
+    char            flags;   /* text: 1 = the ' ' flag (a space before a non-negative value) */
+    int16_t         width;   /* text: minimum field width */
+    int32_t         prec;    /* text: "%.Pf" digits after the point; -1 for "%d" */
+} gf_outcol;
+
+/* The visible rows (hidden == 0) of rows [0, nrows), in order, as FITS
+ * binary-table records: the columns' fields in order, big-endian, written
+ * to `out` (device memory, cap_rows records).  *nvisible receives the
+ * number of records (the rows os_fitsout writes, write_fits_rows :804-866).
+ * GF_E_CAPACITY if they do not fit. */
+int gf_fits_pack_rows(const gf_outcol *cols, int ncols, gf_col hidden, uint64_t nrows, void *out,
+                      uint64_t cap_rows, uint64_t *nvisible, void *stream);
+
+/* The visible rows as text lines (otable::serialize_body, otable.cpp:562-582,
+ * with fmtout, otable.h:39-94): each field as snprintf formats it with
+ * "%[ ]W.Pf" (floats promoted to double) or "%[ ]Wd", fields separated by
+ * one space, '\n' at the end of a line.  Values must be finite with
+ * |v| < 1e10 (GF_E_ARG otherwise).  *nbytes receives the text's length (also
+ * when it exceeds cap: GF_E_CAPACITY), *nvisible the number of lines. */
+int gf_text_format_rows(const gf_outcol *cols, int ncols, gf_col hidden, uint64_t nrows, char *out,
+                        uint64_t cap, uint64_t *nbytes, uint64_t *nvisible, void *stream);
+
+/* ---------------------------------------------------------------------- */
 /* Catalog pipeline (generate_catalog, pipeline.cpp:1313-1445)             */
 /* ---------------------------------------------------------------------- */
 
@@ -510,12 +544,31 @@ int  gf_catalog_open(const char *cmd_conf, uint32_t seed, const char *datadir, g
  * (GF_ACC_*), "nstars" (force the expected total, os_skygen.cpp:886-901). */
 int  gf_catalog_set(gf_catalog *c, const char *key, double value);
 int  gf_catalog_info(const gf_catalog *c, gf_catalog_info_t *info);
+/* The user's component id of each sequential component index (the `comp`
+ * column holds the index; componentMap, common/compmgr.h:35-49): writes up
+ * to cap ids, returns their number. */
+int  gf_catalog_component_ids(const gf_catalog *c, uint32_t *ids, int cap);
 /* Count pass for every model (os_skygen::run, skygen/os_skygen.cpp:876-884);
  * per_model (optional) receives nmodels results. */
 int  gf_catalog_count(gf_catalog *c, gf_count_result *per_model, double *expected);
 /* Draw every model and run the module chain chunk by chunk
  * (skygenHost<T>::drawSources, skygen/skyconfig_impl.h:434-545). */
 int  gf_catalog_run(gf_catalog *c, gf_chunk_fn fn, void *user, gf_run_stats *stats);
+
+/* Output stages for a catalog run (os_textout, pipeline.cpp:199-288;
+ * os_fitsout, :767-1035): the job's output columns in the reference's order
+ * and names, the visible rows of every chunk formatted (text) or packed
+ * (FITS binary table CATALOG, then a METADATA table with the column
+ * definitions) on the device and appended to `path`.  Pass
+ * gf_writer_chunk with the writer as the chunk callback of gf_catalog_run;
+ * gf_writer_close finishes the file, returns the rows written and frees the
+ * writer. */
+#define GF_OUT_TEXT 0
+#define GF_OUT_FITS 1
+typedef struct gf_writer gf_writer;
+int  gf_writer_open(const gf_catalog *c, const char *path, int kind, gf_writer **out);
+int  gf_writer_chunk(void *writer, const gf_table *cols, uint64_t nrows, int comp);
+int  gf_writer_close(gf_writer *w, uint64_t *rows);
 void gf_catalog_close(gf_catalog *c);
 
 /* Host-side copies of everything a catalog job was built from: the sky

@@ -163,6 +163,108 @@ def _plain_jobs(reqs, seed, tmpdir):
     return stars, time.perf_counter() - t_all, lat
 
 
+def output_bench(args):
+    """The output stages on the config-1 workload: the two device kernels
+    alone on the job's first 3e7-row chunk (gf_fits_pack_rows and
+    gf_text_format_rows over the rows as drawn, hidden ones included), and
+    whole jobs written through the FITS and text writers to /dev/null
+    (device packing or formatting, the copy to pinned host memory and the
+    host write; the count pass is not timed)."""
+    import ctypes as C
+    import torch
+    import galfast_amd as G
+    from galfast_amd import _abi as A, configs
+    from galfast_amd.catalog import CHUNK_FN, Catalog, gf_run_stats, _declare
+
+    L = G.lib()
+    _declare()
+    d = tempfile.mkdtemp(prefix="galfast_out_")
+    cmd = configs.WRITERS["c1"](d, args.seed)
+    res = {}
+
+    def grab(user, table, n, comp):
+        if not res:
+            torch.cuda.synchronize()   # the catalog's streams idle while the kernels are timed
+            res.update(_pack_and_format(L, A, table.contents, n))
+        return 0
+
+    cat = Catalog(cmd, seed=args.seed, chunk_rows=30_000_000)
+    cat.count()
+    st = gf_run_stats()
+    cb = CHUNK_FN(grab)
+    G.check(L.gf_catalog_run(cat._h, cb, None, C.byref(st)), "gf_catalog_run")
+    for kind in ("fits", "text"):
+        cat.count()
+        t0 = time.perf_counter()
+        st, rows = cat.write("/dev/null", kind)
+        dt = time.perf_counter() - t0
+        res["%s_file" % kind] = {"rows": rows, "s": dt, "stars_per_s": rows / dt}
+    cat.close()
+    return res
+
+
+# the config-1 output columns: table column, FITS type, text flag, width, precision
+OUTPUT_C1 = [("lb", "D", 1, 13, 8), ("radec", "D", 1, 13, 8), ("XYZ", "E", 1, 10, 2), ("DM", "E", 1, 7, 3),
+             ("absmag", "E", 1, 7, 3), ("comp", "J", 0, 3, -1), ("FeH", "E", 1, 6, 3), ("vcyl", "E", 1, 7, 1),
+             ("pm", "E", 1, 8, 2), ("Am", "E", 0, 6, 3), ("AmInf", "E", 0, 6, 3), ("obsMags", "E", 1, 7, 3),
+             ("mags", "E", 1, 7, 3), ("photoFlags", "J", 1, 4, -1)]
+
+
+def _pack_and_format(L, A, table, n, reps=3):
+    """Best of `reps` timed calls of each output kernel on one chunk."""
+    import ctypes as C
+    import torch
+    import galfast_amd as G
+
+    class gf_outcol(C.Structure):
+        _fields_ = [("col", A.gf_col), ("map", C.c_void_p), ("nmap", C.c_uint32), ("type", C.c_char),
+                    ("flags", C.c_char), ("width", C.c_int16), ("prec", C.c_int32)]
+
+    cols = (gf_outcol * len(OUTPUT_C1))()
+    rec = 0
+    for i, (name, t, fl, w, p) in enumerate(OUTPUT_C1):
+        c = getattr(table, name)
+        cols[i].col = c
+        cols[i].type = t.encode()
+        cols[i].flags = bytes([fl])
+        cols[i].width = w
+        cols[i].prec = p
+        rec += c.width * c.elem
+    L.gf_fits_pack_rows.argtypes = [C.c_void_p, C.c_int, A.gf_col, C.c_uint64, C.c_void_p, C.c_uint64,
+                                    C.POINTER(C.c_uint64), C.c_void_p]
+    L.gf_text_format_rows.argtypes = [C.c_void_p, C.c_int, A.gf_col, C.c_uint64, C.c_void_p, C.c_uint64,
+                                      C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_void_p]
+
+    def best(call):
+        ts = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            G.check(call(), "output kernel")
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return min(ts)
+
+    out = {}
+    buf = torch.empty(n * rec, dtype=torch.uint8, device="cuda")
+    nv = C.c_uint64()
+    dt = best(lambda: L.gf_fits_pack_rows(cols, len(cols), table.hidden, n, buf.data_ptr(), n, C.byref(nv), None))
+    # every row's hidden flag and each visible row's columns read, each record written
+    moved = 4 * n + 2 * nv.value * rec
+    out["fits_pack"] = {"rows": n, "records": nv.value, "record_bytes": rec, "ms": 1e3 * dt,
+                        "GBps": moved / dt / 1e9}
+    del buf
+    cap = n * 320
+    tbuf = torch.empty(cap, dtype=torch.uint8, device="cuda")
+    nb = C.c_uint64()
+    dt = best(lambda: L.gf_text_format_rows(cols, len(cols), table.hidden, n, tbuf.data_ptr(), cap, C.byref(nb),
+                                            C.byref(nv), None))
+    out["text_format"] = {"rows": n, "lines": nv.value, "bytes": nb.value, "ms": 1e3 * dt,
+                          "lines_per_s": nv.value / dt, "text_GBps": nb.value / dt / 1e9}
+    del tbuf
+    return out
+
+
 def serving(args):
     """Request-server measurement on the serving workload: a sample of the
     requests run as separate catalog jobs (the plain GPU path), the whole
@@ -214,6 +316,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--output-bench", action="store_true",
+                    help="also time the output stages (FITS and text) on the config-1 workload")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-batch", type=int, default=256)
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[10.0, 100.0, 1000.0])
@@ -237,6 +341,9 @@ def main():
               "pixels, see DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.output_bench:
+        import torch   # device buffers of the output timing; torch takes the device first
+        torch.cuda.init()
     import galfast_amd as G
     from galfast_amd import configs
     from galfast_amd.catalog import Catalog
@@ -351,6 +458,8 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cat, args.workload, args.cpu_budget, args.cpu_threads)
     cat.close()
+    if args.output_bench:
+        out["output"] = output_bench(args)
     if not args.no_serve:
         sv = serving(args)
         out["serving"] = sv

@@ -144,4 +144,10 @@ class gf_skygen_cols(C.Structure):
     _fields_ = [(n, gf_col) for n in SKYGEN_COLUMNS]
 
 
+class gf_countsmap_params(C.Structure):
+    _fields_ = [("x0", f32), ("dx", f32), ("y0", f32), ("dy", f32), ("z0", f32), ("dz", f32),
+                ("nx", i32), ("ny", i32), ("nz", i32), ("zmag_width", i32), ("ncoadd", i32), ("equalarea", i32),
+                ("coadd_offsets", vp), ("proj", (f64 * 4) * 2)]
+
+
 GF_ACC_FP64, GF_ACC_KAHAN32, GF_ACC_FP32 = 0, 1, 2

@@ -105,11 +105,12 @@ class Catalog:
 
     def write(self, path, kind="fits"):
         """Run the catalog into a file: the reference's output stage
-        (kind "fits": os_fitsout, "text": os_textout) for the visible rows.
-        Returns (run statistics, rows written)."""
+        (kind "fits": os_fitsout, "text": os_textout, "countsmap": the job's
+        os_countsMap stage) for the visible rows.  Returns (run statistics,
+        rows written or binned)."""
         L = lib()
         w = C.c_void_p()
-        check(L.gf_writer_open(self._h, path.encode(), {"text": 0, "fits": 1}[kind], C.byref(w)), "gf_writer_open")
+        check(L.gf_writer_open(self._h, path.encode(), {"text": 0, "fits": 1, "countsmap": 2}[kind], C.byref(w)), "gf_writer_open")
         st = gf_run_stats()
         rc = L.gf_catalog_run(self._h, C.cast(L.gf_writer_chunk, CHUNK_FN), w, C.byref(st))
         rows = C.c_uint64()

@@ -177,6 +177,18 @@ modules = feh.conf Bond2010.conf photometry.conf photoerr.conf vel2pm.conf gal2o
     return os.path.join(d, "cmd.conf")
 
 
+def add_countsmap(cmd, keys):
+    """Give the job in `cmd` (a cmd.conf path) a countsMap output stage with
+    the given {key: value} (xy, z, x0, dx, equalarea, "coadd.<name>",
+    "output all bins", ...)."""
+    d = os.path.dirname(cmd)
+    _w(os.path.join(d, "countsMap.conf"), "module = counts map\n" +
+       "\n".join("%s = %s" % (k, v) for k, v in keys.items()))
+    with open(cmd, "a") as f:
+        f.write("output = countsMap.conf\n")
+    return cmd
+
+
 def write_c3(d, seed=42, m1=25.0, beam=(90.0, 60.0, 1.0), dx=0.25, table_seed=42):
     """Serving base configuration: the config-1 models, tables (seeded with
     table_seed) and modules, a beam footprint, r in [14, m1], dx-degree

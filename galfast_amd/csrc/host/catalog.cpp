@@ -111,6 +111,7 @@ struct Footprint {
 struct gf_catalog {
 	Config globals, skygen;
 	std::vector<Config> model_cfgs, foot_cfgs, module_cfgs;
+	int countsmap = -1;   // module_cfgs index of the countsMap output stage
 	ComponentMap cm;
 	gf_skygen_params sp{};
 	std::vector<gf_pencil_beam> pixels;
@@ -878,6 +879,8 @@ void open_catalog(gf_catalog &c, const std::string &cmdfile)
 	for (auto &f : split_ws(cmd.get("models"))) c.model_cfgs.push_back(loadc(f));
 	for (auto &f : split_ws(cmd.get("footprints"))) c.foot_cfgs.push_back(loadc(f));
 	for (auto &f : split_ws(cmd.get("modules", ""))) c.module_cfgs.push_back(loadc(f));
+	// the output stages are module configurations too (galfast.cpp:227-233)
+	for (auto &f : split_ws(cmd.get("output", ""))) c.module_cfgs.push_back(loadc(f));
 	c.nstars_forced = cmd.getd("nstars", 0.);
 	if (c.foot_cfgs.size() != 1) fail("exactly one footprint configuration is supported");
 	for (auto &mc : c.model_cfgs) mc.globals = &c.globals;
@@ -908,7 +911,8 @@ void open_catalog(gf_catalog &c, const std::string &cmdfile)
 		else if (mod == "vel2pm") construct_vel2pm(c, mc);
 		else if (mod == "gal2other") construct_gal2other(c, mc);
 		else if (mod == "photometricerrors") { /* needs photometry; constructed below */ }
-		else if (mod == "fitsout" || mod == "textout" || mod == "countsmap") { /* output stages: not on the hot path */ }
+		else if (mod == "countsmap") c.countsmap = (int)(&mc - c.module_cfgs.data());
+		else if (mod == "fitsout" || mod == "textout") { /* the writer's kind and path choose the output */ }
 		else fail("unsupported module '" + mod + "'");
 	}
 	for (auto &mc : c.module_cfgs)
@@ -1496,6 +1500,13 @@ void output_schema(const gf_catalog &c, std::vector<OutColumn> &out)
 }
 
 const std::vector<uint32_t> &component_ids(const gf_catalog &c) { return c.cm.seq2comp; }
+
+const Config *countsmap_config(const gf_catalog &c)
+{
+	return c.countsmap >= 0 ? &c.module_cfgs[c.countsmap] : nullptr;
+}
+
+void set_last_error(const std::string &msg) { g_last_error = msg; }
 
 } // namespace gfh
 

@@ -106,5 +106,9 @@ struct OutColumn {
 };
 void output_schema(const gf_catalog &c, std::vector<OutColumn> &cols);
 const std::vector<uint32_t> &component_ids(const gf_catalog &c);
+// The job's countsMap output stage, or NULL.
+const Config *countsmap_config(const gf_catalog &c);
+// The message gf_last_error returns on this thread.
+void set_last_error(const std::string &msg);
 
 } // namespace gfh

@@ -491,6 +491,34 @@ int gf_fits_pack_rows(const gf_outcol *cols, int ncols, gf_col hidden, uint64_t 
 int gf_text_format_rows(const gf_outcol *cols, int ncols, gf_col hidden, uint64_t nrows, char *out,
                         uint64_t cap, uint64_t *nbytes, uint64_t *nvisible, void *stream);
 
+/* Sky-count maps (os_countsMap, pipeline.cpp:293-762): every visible row
+ * lands in the beam (X, Y) of its position column xy (degrees; with
+ * equalarea, first Lambert-projected onto the north (map 0, b > 0) or south
+ * (map 1) polar map, os_countsMap::threaded_process :418-494) and, for each
+ * element i < zmag_width of its z column, in z bin Z; a coadd c counts the
+ * row in every bin from the bin of min_i(z_i - coadd_offsets[c][i]) up.
+ * Bin b of an axis with origin a0 and width d holds values whose
+ * (int)((v - a0) / d + 0.5) is b, clamped to [0, n - 1] (find_bin :361-367).
+ * Counts are uint32 in [beam][Z][zwidth] order, beam = (X ny + Y) nmaps + map,
+ * zwidth = zmag_width + ncoadd, nmaps = equalarea ? 2 : 1. */
+typedef struct gf_countsmap_params {
+    float        x0, dx, y0, dy, z0, dz;
+    int32_t      nx, ny, nz;
+    int32_t      zmag_width, ncoadd, equalarea;
+    const float *coadd_offsets;   /* device, ncoadd x zmag_width, or NULL */
+    double       proj[2][4];      /* per map: l0, sin(phi1), cos(phi1), unused (radians) */
+} gf_countsmap_params;
+
+/* Adds the visible rows of [0, nrows) to `counts` (device, nmaps nx ny nz
+ * zwidth uint32); the coadd entries receive one count at the row's coadd
+ * bin only (gf_countsmap_finish makes them cumulative).  beams[beam] (device
+ * bytes) is set to 1 for every beam a row lands in; *total (device) grows by
+ * the number of z-bin counts added (m_total). */
+int gf_countsmap_bin(const gf_countsmap_params *p, gf_col xy, gf_col z, gf_col hidden, uint64_t nrows,
+                     uint32_t *counts, uint8_t *beams, unsigned long long *total, void *stream);
+/* The coadd entries of every (beam, coadd) made cumulative over Z. */
+int gf_countsmap_finish(const gf_countsmap_params *p, uint32_t *counts, void *stream);
+
 /* ---------------------------------------------------------------------- */
 /* Catalog pipeline (generate_catalog, pipeline.cpp:1313-1445)             */
 /* ---------------------------------------------------------------------- */
@@ -552,7 +580,10 @@ int  gf_catalog_component_ids(const gf_catalog *c, uint32_t *ids, int cap);
  * per_model (optional) receives nmodels results. */
 int  gf_catalog_count(gf_catalog *c, gf_count_result *per_model, double *expected);
 /* Draw every model and run the module chain chunk by chunk
- * (skygenHost<T>::drawSources, skygen/skyconfig_impl.h:434-545). */
+ * (skygenHost<T>::drawSources, skygen/skyconfig_impl.h:434-545).  The RNG
+ * streams persist: a second run on the same catalog continues them and
+ * draws another realization, a catalog opened again with the same seed
+ * repeats the first. */
 int  gf_catalog_run(gf_catalog *c, gf_chunk_fn fn, void *user, gf_run_stats *stats);
 
 /* Output stages for a catalog run (os_textout, pipeline.cpp:199-288;
@@ -565,6 +596,12 @@ int  gf_catalog_run(gf_catalog *c, gf_chunk_fn fn, void *user, gf_run_stats *sta
  * writer. */
 #define GF_OUT_TEXT 0
 #define GF_OUT_FITS 1
+/* GF_OUT_COUNTSMAP: the job's countsMap output stage (its configuration from
+ * the cmd.conf `output` or `modules` list; GF_E_CONFIG without one) binned
+ * on the device chunk by chunk and written to `path` as the reference's
+ * text table when the writer closes (os_countsMap::~os_countsMap :684-762);
+ * the rows it returns are the visible rows binned. */
+#define GF_OUT_COUNTSMAP 2
 typedef struct gf_writer gf_writer;
 int  gf_writer_open(const gf_catalog *c, const char *path, int kind, gf_writer **out);
 int  gf_writer_chunk(void *writer, const gf_table *cols, uint64_t nrows, int comp);

@@ -27,6 +27,10 @@
  *   errors                             test also rejecting a planted mistake
  *                                      (tests/test_oracle_modules.py)
  *   photometry ....................... closed form on polynomial colour tables
+ *   countsMap binning ................ known bins of bin centres and edges,
+ *                                      closed-form polar Lambert radius and
+ *                                      azimuth, coadd monotonicity, numpy
+ *                                      histogram (tests/test_oracle_countsmap.py)
  */
 #include <math.h>
 #include <stdint.h>
@@ -34,6 +38,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <algorithm>
+#include <map>
 #include <vector>
 
 #include "../include/galfast_hip.h"
@@ -792,6 +797,82 @@ GFO_API void gfo_clip(const gf_skygen_params *sp, const gf_footprint *foot, cons
 		deproject(sp->proj[C<int>(c->projIdx, r)], l, b, C<float>(c->projXY, r, 0), C<float>(c->projXY, r, 1));
 		if (outside_foot(*foot, l, b)) C<int>(c->hidden, r) = 1;
 	}
+}
+
+/* The countsMap stage's binning (os_countsMap::threaded_process,
+ * pipeline.cpp:418-494, find_bin :361-367, lambert::project
+ * common/projections.h:46-55), the reference's way: a std::map of beams
+ * (X, Y, map), each a vector of nz x zwidth int counts created on the beam's
+ * first row, coadds added to every bin from the depth bin up.  The result is
+ * added to counts / beams in the product's dense layout
+ * ([beam][Z][zwidth], beam = (X ny + Y) nmaps + map) and the number of
+ * z-element counts is returned.  coadd_offsets is a host pointer. */
+static int gfo_find_bin(double x, double x0, double dx, int n)
+{
+	const double v = (x - x0) / dx + 0.5;
+	/* the x86-64 conversion of an out-of-range or NaN value: INT_MIN */
+	int b = (v > -2147483649.0 && v < 2147483648.0) ? (int)v : (int)0x80000000u;
+	b = std::max(0, b);
+	b = std::min(n - 1, b);
+	return b;
+}
+
+GFO_API uint64_t gfo_countsmap_bin(const gf_countsmap_params *pp, gf_col xy, gf_col z, gf_col hidden, uint64_t n,
+                                   uint32_t *counts, uint8_t *beams)
+{
+	const gf_countsmap_params &p = *pp;
+	const int zm = p.zmag_width, zw = zm + p.ncoadd, nmaps = p.equalarea ? 2 : 1;
+	const double d2r = 0.01745329251994329576923691;
+	struct key {
+		int X, Y, map;
+		bool operator<(const key &a) const
+		{
+			return X < a.X || (X == a.X && Y < a.Y) || (X == a.X && Y == a.Y && map < a.map);
+		}
+	};
+	std::map<key, std::vector<int>> m;
+	uint64_t total = 0;
+	for (uint64_t row = 0; row < n; row++) {
+		if (C<int>(hidden, row)) continue;
+		key b;
+		double x = C<double>(xy, row, 0), y = C<double>(xy, row, 1);
+		b.map = 0;
+		if (p.equalarea) {
+			b.map = y > 0 ? 0 : 1;
+			const double *pr = p.proj[b.map];
+			const double l = x * d2r, phi = y * d2r;
+			double sp, cp, sdl, cdl;
+			gfo_sincos(phi, &sp, &cp);
+			gfo_sincos(l - pr[0], &sdl, &cdl);
+			const double denom = 1 + pr[1] * sp + pr[2] * cp * cdl;
+			if (!(denom < 1e-10)) {
+				const double kp = sqrt(2. / denom);
+				x = kp * cp * sdl;
+				y = kp * (pr[2] * sp - pr[1] * cp * cdl);
+			}
+		}
+		b.X = gfo_find_bin(x, p.x0, p.dx, p.nx);
+		b.Y = gfo_find_bin(y, p.y0, p.dy, p.ny);
+		std::vector<int> &c = m[b];
+		if (c.empty()) c.assign((size_t)p.nz * zw, 0);
+		for (int i = 0; i != zm; i++) {
+			int Z = gfo_find_bin(C<float>(z, row, i), p.z0, p.dz, p.nz);
+			c[(size_t)Z * zw + i]++;
+			total++;
+		}
+		for (int k = 0; k < p.ncoadd; k++) {
+			const float *coeffs = p.coadd_offsets + (size_t)k * zm;
+			float magCut = INFINITY;
+			for (int i = 0; i != zm; i++) magCut = std::min(magCut, C<float>(z, row, i) - coeffs[i]);
+			for (int Z = gfo_find_bin(magCut, p.z0, p.dz, p.nz); Z < p.nz; Z++) c[(size_t)Z * zw + zm + k]++;
+		}
+	}
+	for (auto &kv : m) {
+		const size_t beam = ((size_t)kv.first.X * p.ny + kv.first.Y) * nmaps + kv.first.map;
+		beams[beam] = 1;
+		for (size_t j = 0; j < kv.second.size(); j++) counts[beam * kv.second.size() + j] += (uint32_t)kv.second[j];
+	}
+	return total;
 }
 
 GFO_API void gfo_count_bins(const gf_skygen_params *spp, const gf_model *mp, const gf_pencil_beam *pixels,

@@ -45,6 +45,8 @@ def lib():
         _o.gfo_order.restype = C.c_uint64
         _o.gfo_order.argtypes = [P(A.gf_skygen_cols), C.c_uint64, P(A.gf_skygen_cols)]
         _o.gfo_clip.argtypes = [P(A.gf_skygen_params), P(A.gf_footprint), P(A.gf_skygen_cols), C.c_uint64]
+        _o.gfo_countsmap_bin.restype = C.c_uint64
+        _o.gfo_countsmap_bin.argtypes = [P(A.gf_countsmap_params), A.gf_col, A.gf_col, A.gf_col, C.c_uint64, vp, vp]
         _o.gfo_module_chain.restype = C.c_int
         _o.gfo_module_chain.argtypes = [P(A.gf_rows), P(A.gf_chain), vp, C.c_uint32, P(A.gf_table)]
         _o.gfo_galequ.argtypes = [C.c_double, C.c_double, P(C.c_double), P(C.c_double)]

@@ -180,6 +180,9 @@ def output_bench(args):
     _declare()
     d = tempfile.mkdtemp(prefix="galfast_out_")
     cmd = configs.WRITERS["c1"](d, args.seed)
+    # a sky-count map of the observed magnitudes: 1-degree (l, b) beams,
+    # half-magnitude bins from 10 to 30, one coadd
+    configs.add_countsmap(cmd, dict(COUNTSMAP_C1))
     res = {}
 
     def grab(user, table, n, comp):
@@ -193,7 +196,7 @@ def output_bench(args):
     st = gf_run_stats()
     cb = CHUNK_FN(grab)
     G.check(L.gf_catalog_run(cat._h, cb, None, C.byref(st)), "gf_catalog_run")
-    for kind in ("fits", "text"):
+    for kind in ("fits", "text", "countsmap"):
         cat.count()
         t0 = time.perf_counter()
         st, rows = cat.write("/dev/null", kind)
@@ -202,6 +205,9 @@ def output_bench(args):
     cat.close()
     return res
 
+
+COUNTSMAP_C1 = {"xy": "lb", "z": "obsSYN", "x0": 0.5, "x1": 359.5, "y0": -89.5, "y1": 89.5, "z0": 10, "z1": 30,
+                "dz": 0.5, "coadd.any": "0 0 0 0 0"}
 
 # the config-1 output columns: table column, FITS type, text flag, width, precision
 OUTPUT_C1 = [("lb", "D", 1, 13, 8), ("radec", "D", 1, 13, 8), ("XYZ", "E", 1, 10, 2), ("DM", "E", 1, 7, 3),
@@ -262,6 +268,21 @@ def _pack_and_format(L, A, table, n, reps=3):
     out["text_format"] = {"rows": n, "lines": nv.value, "bytes": nb.value, "ms": 1e3 * dt,
                           "lines_per_s": nv.value / dt, "text_GBps": nb.value / dt / 1e9}
     del tbuf
+    # the countsMap binning of the same rows (COUNTSMAP_C1's map)
+    p = A.gf_countsmap_params()
+    p.x0, p.dx, p.y0, p.dy, p.z0, p.dz = 0.5, 1.0, -89.5, 1.0, 10.0, 0.5
+    p.nx, p.ny, p.nz = 360, 180, 41
+    p.zmag_width, p.ncoadd, p.equalarea = 5, 1, 0
+    co = torch.zeros(5, dtype=torch.float32, device="cuda")
+    p.coadd_offsets = co.data_ptr()
+    zw = 6
+    counts = torch.zeros(360 * 180 * 41 * zw, dtype=torch.int32, device="cuda")
+    beams = torch.zeros(360 * 180, dtype=torch.uint8, device="cuda")
+    total = torch.zeros(1, dtype=torch.int64, device="cuda")
+    dt = best(lambda: L.gf_countsmap_bin(C.byref(p), table.lb, table.obsMags, table.hidden, n, counts.data_ptr(),
+                                         beams.data_ptr(), total.data_ptr(), None))
+    out["countsmap_bin"] = {"rows": n, "ms": 1e3 * dt, "rows_per_s": n / dt,
+                            "atomics_per_s": int(total.item()) // reps * (zw / 5) / dt}
     return out
 
 

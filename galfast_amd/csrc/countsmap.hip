@@ -48,38 +48,153 @@ GF_D void lambert_project(const double *pr, double &x, double &y)
 	y = kp * (pr[2] * sp - pr[1] * cp * cdl);
 }
 
+// Adds one to counts[key] for every active lane of the wave.  Lanes with
+// equal keys are combined into one atomic (rows arrive in sky order, so a
+// wave's rows mostly share a beam and a few z bins); after AGG_ROUNDS
+// distinct keys the remaining lanes add their own.
+constexpr int AGG_ROUNDS = 12;
+
+GF_D void wave_add(uint32_t *counts, uint64_t key, bool on)
+{
+	const int lane = (int)__lane_id();
+	uint64_t active = __ballot(on);
+	for (int round = 0; active; round++) {
+		if (round == AGG_ROUNDS) {
+			if ((active >> lane) & 1) atomicAdd(counts + key, 1u);
+			return;
+		}
+		const int leader = __ffsll((unsigned long long)active) - 1;
+		const uint64_t k0 = __shfl(key, leader);
+		const uint64_t same = __ballot(on && key == k0) & active;
+		if (lane == leader) atomicAdd(counts + k0, (uint32_t)__popcll(same));
+		active &= ~same;
+	}
+}
+
 __global__ void __launch_bounds__(CB) k_countsmap(const gf_countsmap_params p, gf_col xy, gf_col z, gf_col hidden,
                                                   uint32_t n, uint32_t *counts, uint8_t *beams,
                                                   unsigned long long *total)
 {
 	const int zm = p.zmag_width, zw = zm + p.ncoadd, nmaps = p.equalarea ? 2 : 1;
 	unsigned long long mine = 0;
-	for (uint64_t i = (uint64_t)blockIdx.x * CB + threadIdx.x; i < n; i += (uint64_t)gridDim.x * CB) {
+	// every lane of a wave takes the same trips, so that the wave's lanes
+	// combine their counts
+	const uint64_t wave0 = (uint64_t)blockIdx.x * CB + (threadIdx.x & ~63u);
+	for (uint64_t w = wave0; w < n; w += (uint64_t)gridDim.x * CB) {
+		const uint64_t i = w + __lane_id();
 		const uint32_t r = (uint32_t)i;
-		if (col<int>(hidden, r)) continue;
-		double x = col<double>(xy, r, 0), y = col<double>(xy, r, 1);
-		int map = 0;
-		if (p.equalarea) {
-			map = y > 0 ? 0 : 1;
-			lambert_project(p.proj[map], x, y);
+		const bool on = i < n && col<int>(hidden, r) == 0;
+		uint64_t base = 0;
+		if (on) {
+			double x = col<double>(xy, r, 0), y = col<double>(xy, r, 1);
+			int map = 0;
+			if (p.equalarea) {
+				map = y > 0 ? 0 : 1;
+				lambert_project(p.proj[map], x, y);
+			}
+			const int X = find_bin(x, p.x0, p.dx, p.nx), Y = find_bin(y, p.y0, p.dy, p.ny);
+			const uint64_t beam = ((uint64_t)X * p.ny + Y) * nmaps + map;
+			beams[beam] = 1;
+			base = beam * (uint64_t)(p.nz * zw);
+			mine += zm;
 		}
-		const int X = find_bin(x, p.x0, p.dx, p.nx), Y = find_bin(y, p.y0, p.dy, p.ny);
-		const uint64_t beam = ((uint64_t)X * p.ny + Y) * nmaps + map;
-		beams[beam] = 1;
-		uint32_t *c = counts + beam * (uint64_t)(p.nz * zw);
-		for (int i = 0; i < zm; i++)
-			atomicAdd(c + find_bin(col<float>(z, r, i), p.z0, p.dz, p.nz) * zw + i, 1u);
-		mine += zm;
-		// the coadd depth bin: the smallest z_i - offset_i over the bands
+		for (int e = 0; e < zm; e++) {
+			const uint64_t key = on ? base + (uint64_t)find_bin(col<float>(z, r, e), p.z0, p.dz, p.nz) * zw + e : 0;
+			wave_add(counts, key, on);
+		}
+		// the coadd depth bin: the smallest z_e - offset_e over the bands
 		// (std::min as the reference writes it: a NaN term is skipped)
 		for (int k = 0; k < p.ncoadd; k++) {
-			float cut = __builtin_inff();
-			for (int i = 0; i < zm; i++) {
-				const float m = col<float>(z, r, i) - p.coadd_offsets[k * zm + i];
-				cut = m < cut ? m : cut;
+			uint64_t key = 0;
+			if (on) {
+				float cut = __builtin_inff();
+				for (int e = 0; e < zm; e++) {
+					const float m = col<float>(z, r, e) - p.coadd_offsets[k * zm + e];
+					cut = m < cut ? m : cut;
+				}
+				key = base + (uint64_t)find_bin(cut, p.z0, p.dz, p.nz) * zw + zm + k;
 			}
-			atomicAdd(c + find_bin(cut, p.z0, p.dz, p.nz) * zw + zm + k, 1u);
+			wave_add(counts, key, on);
 		}
+	}
+	for (int o = 32; o > 0; o >>= 1) mine += __shfl_xor(mine, o);
+	if ((threadIdx.x & 63) == 0 && mine) atomicAdd(total, mine);
+}
+
+// The same binning with each block's counts gathered in LDS first: a block
+// takes tiles of TILE consecutive rows (a few sky pixels of a chunk in sky
+// order, so a few beams), gives each beam it meets one of LDS_BEAMS slots of
+// nz x zwidth counts, and adds the slots to the map once per tile.  Rows
+// whose beam finds no free slot go to the map directly.
+constexpr int LDS_BEAMS = 8;
+constexpr int TILE = CB * 16;
+
+__global__ void __launch_bounds__(CB) k_countsmap_lds(const gf_countsmap_params p, gf_col xy, gf_col z,
+                                                      gf_col hidden, uint32_t n, uint32_t *counts, uint8_t *beams,
+                                                      unsigned long long *total)
+{
+	extern __shared__ uint32_t h[];            // LDS_BEAMS x nz x zwidth
+	__shared__ uint32_t tag[LDS_BEAMS];        // beam + 1 of each slot, 0 = free
+	const int zm = p.zmag_width, zw = zm + p.ncoadd, nmaps = p.equalarea ? 2 : 1;
+	const int per = p.nz * zw;
+	unsigned long long mine = 0;
+	for (uint64_t t0 = (uint64_t)blockIdx.x * TILE; t0 < n; t0 += (uint64_t)gridDim.x * TILE) {
+		for (int j = threadIdx.x; j < LDS_BEAMS * per; j += CB) h[j] = 0;
+		if (threadIdx.x < LDS_BEAMS) tag[threadIdx.x] = 0;
+		__syncthreads();
+		for (int j = 0; j < TILE / CB; j++) {
+			const uint64_t i = t0 + (uint64_t)j * CB + threadIdx.x;
+			if (i >= n) break;
+			const uint32_t r = (uint32_t)i;
+			if (col<int>(hidden, r)) continue;
+			double x = col<double>(xy, r, 0), y = col<double>(xy, r, 1);
+			int map = 0;
+			if (p.equalarea) {
+				map = y > 0 ? 0 : 1;
+				lambert_project(p.proj[map], x, y);
+			}
+			const int X = find_bin(x, p.x0, p.dx, p.nx), Y = find_bin(y, p.y0, p.dy, p.ny);
+			const uint64_t beam = ((uint64_t)X * p.ny + Y) * nmaps + map;
+			// the beam's slot: the one tagged with it, else the first free one
+			const uint32_t want = (uint32_t)beam + 1u;
+			int slot = -1;
+			for (int q = 0; q < LDS_BEAMS; q++) {
+				const uint32_t old = atomicCAS(&tag[q], 0u, want);
+				if (old == 0u || old == want) { slot = q; break; }
+			}
+			uint32_t *c;
+			if (slot >= 0) {
+				c = h + slot * per;
+			} else {
+				beams[beam] = 1;
+				c = counts + beam * (uint64_t)per;
+			}
+			for (int e = 0; e < zm; e++) atomicAdd(c + find_bin(col<float>(z, r, e), p.z0, p.dz, p.nz) * zw + e, 1u);
+			mine += zm;
+			// the coadd depth bin: the smallest z_e - offset_e over the
+			// bands (std::min as the reference writes it: a NaN is skipped)
+			for (int k = 0; k < p.ncoadd; k++) {
+				float cut = __builtin_inff();
+				for (int e = 0; e < zm; e++) {
+					const float m = col<float>(z, r, e) - p.coadd_offsets[k * zm + e];
+					cut = m < cut ? m : cut;
+				}
+				atomicAdd(c + find_bin(cut, p.z0, p.dz, p.nz) * zw + zm + k, 1u);
+			}
+		}
+		__syncthreads();
+		for (int q = 0; q < LDS_BEAMS; q++) {
+			const uint32_t tg = tag[q];
+			if (tg == 0u) break;                   // slots fill in order
+			const uint64_t beam = tg - 1u;
+			if (threadIdx.x == 0) beams[beam] = 1;
+			uint32_t *dst = counts + beam * (uint64_t)per;
+			for (int j = threadIdx.x; j < per; j += CB) {
+				const uint32_t v = h[q * per + j];
+				if (v) atomicAdd(dst + j, v);
+			}
+		}
+		__syncthreads();
 	}
 	for (int o = 32; o > 0; o >>= 1) mine += __shfl_xor(mine, o);
 	if ((threadIdx.x & 63) == 0 && mine) atomicAdd(total, mine);
@@ -121,8 +236,16 @@ int gf_countsmap_bin(const gf_countsmap_params *p, gf_col xy, gf_col z, gf_col h
 	if (nrows == 0) return 0;
 	GF_CHECK(xy.ptr && xy.elem == 8 && xy.width >= 2, GF_E_ARG);
 	GF_CHECK(z.ptr && z.elem == 4 && z.width >= (uint32_t)p->zmag_width && hidden.ptr, GF_E_ARG);
-	const unsigned g = (unsigned)std::min<uint64_t>(gf::blocks_for(nrows, CB), 8192);
-	k_countsmap<<<g, CB, 0, gf::as_stream(stream)>>>(*p, xy, z, hidden, (uint32_t)nrows, counts, beams, total);
+	const size_t lds = (size_t)LDS_BEAMS * p->nz * (p->zmag_width + p->ncoadd) * sizeof(uint32_t);
+	const uint64_t nbeams = (uint64_t)p->nx * p->ny * (p->equalarea ? 2 : 1);
+	if (lds <= 48 * 1024 && nbeams < 0xffffffffull) {   // slots tag beams as beam + 1 in 32 bits
+		const unsigned g = (unsigned)std::min<uint64_t>((nrows + TILE - 1) / TILE, 4096);
+		k_countsmap_lds<<<g, CB, lds, gf::as_stream(stream)>>>(*p, xy, z, hidden, (uint32_t)nrows, counts, beams,
+		                                                      total);
+	} else {
+		const unsigned g = (unsigned)std::min<uint64_t>(gf::blocks_for(nrows, CB), 8192);
+		k_countsmap<<<g, CB, 0, gf::as_stream(stream)>>>(*p, xy, z, hidden, (uint32_t)nrows, counts, beams, total);
+	}
 	GF_LAUNCHED();
 	return 0;
 }

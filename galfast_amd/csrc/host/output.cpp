@@ -71,11 +71,18 @@ struct gf_writer {
 	std::vector<gf_outcol> oc;       // per chunk: the table's columns
 	uint32_t *dmap = nullptr;        // device copy of the component ids
 	uint32_t nmap = 0;
-	hipStream_t st = nullptr;
-	void *dbuf = nullptr;            // device output of one chunk
-	size_t dcap = 0;
-	char *hbuf = nullptr;            // pinned copy
-	size_t hcap = 0;
+	hipStream_t st = nullptr;        // packing / formatting
+	hipStream_t cs = nullptr;        // copies to the host
+	// two buffers of packed or formatted rows: a piece is copied to the host
+	// while the next one is produced and the one before is written
+	void *dbuf[2] = {};              // device
+	size_t dcap[2] = {};
+	char *hbuf[2] = {};              // pinned host copies
+	size_t hcap[2] = {};
+	hipEvent_t copied[2] = {};
+	int pending = -1;                // buffer whose copy is queued and not yet written
+	size_t pending_bytes = 0;
+	int next = 0;                    // buffer of the next piece
 	uint64_t rows = 0;
 	uint32_t rec_bytes = 0;          // FITS record length
 	long naxis2_at = -1;             // file offset of the CATALOG header's NAXIS2 card
@@ -85,10 +92,15 @@ struct gf_writer {
 
 	~gf_writer()
 	{
-		if (dbuf) (void)hipFree(dbuf);
-		if (hbuf) (void)hipHostFree(hbuf);
+		if (cs) (void)hipStreamSynchronize(cs);
+		for (int b = 0; b < 2; b++) {
+			if (dbuf[b]) (void)hipFree(dbuf[b]);
+			if (hbuf[b]) (void)hipHostFree(hbuf[b]);
+			if (copied[b]) (void)hipEventDestroy(copied[b]);
+		}
 		if (dmap) (void)hipFree(dmap);
 		if (st) (void)hipStreamDestroy(st);
+		if (cs) (void)hipStreamDestroy(cs);
 		if (f) fclose(f);
 	}
 };
@@ -163,22 +175,85 @@ int pad_data(FILE *f)
 	return pad ? write_all(f, z.data(), pad) : 0;
 }
 
-int ensure(gf_writer &w, size_t bytes)
+// Buffer b of at least `bytes` on the device and the host (its previous
+// contents are written out before it is reused).
+int ensure(gf_writer &w, int b, size_t bytes)
 {
-	if (w.dcap < bytes) {
-		if (w.dbuf) GF_HIP(hipFree(w.dbuf));
-		w.dbuf = nullptr;
-		w.dcap = 0;
-		GF_HIP(hipMalloc(&w.dbuf, bytes));
-		w.dcap = bytes;
+	if (w.dcap[b] < bytes) {
+		if (w.dbuf[b]) GF_HIP(hipFree(w.dbuf[b]));
+		w.dbuf[b] = nullptr;
+		w.dcap[b] = 0;
+		GF_HIP(hipMalloc(&w.dbuf[b], bytes));
+		w.dcap[b] = bytes;
 	}
-	if (w.hcap < bytes) {
-		if (w.hbuf) GF_HIP(hipHostFree(w.hbuf));
-		w.hbuf = nullptr;
-		w.hcap = 0;
-		GF_HIP(hipHostMalloc((void **)&w.hbuf, bytes, hipHostMallocDefault));
-		w.hcap = bytes;
+	if (w.hcap[b] < bytes) {
+		if (w.hbuf[b]) GF_HIP(hipHostFree(w.hbuf[b]));
+		w.hbuf[b] = nullptr;
+		w.hcap[b] = 0;
+		GF_HIP(hipHostMalloc((void **)&w.hbuf[b], bytes, hipHostMallocDefault));
+		w.hcap[b] = bytes;
 	}
+	return 0;
+}
+
+// Writes the piece whose copy was queued last, once the copy is done.
+int flush_pending(gf_writer &w)
+{
+	if (w.pending < 0) return 0;
+	const int b = w.pending;
+	w.pending = -1;
+	GF_HIP(hipEventSynchronize(w.copied[b]));
+	return w.pending_bytes ? write_all(w.f, w.hbuf[b], w.pending_bytes) : 0;
+}
+
+// The rows of a column from row r0 on.
+gf_col from_row(gf_col c, uint64_t r0)
+{
+	c.ptr = (char *)c.ptr + r0 * c.elem;
+	return c;
+}
+
+constexpr uint64_t PIECE_ROWS = 1ull << 22;   // rows packed or formatted per piece
+
+// One piece of rows [r0, r0 + n) of the chunk: packed or formatted into
+// buffer b on w.st, its copy to the host queued on w.cs, then the previous
+// piece written to the file while this one copies.
+int write_piece(gf_writer &w, const gf_table *t, uint64_t r0, uint64_t n)
+{
+	const int b = w.next;
+	std::vector<gf_outcol> oc = w.oc;
+	for (auto &o : oc) o.col = from_row(o.col, r0);
+	const gf_col hidden = from_row(t->hidden, r0);
+	int rc;
+	uint64_t nv = 0;
+	size_t bytes = 0;
+	if (w.kind == GF_OUT_FITS) {
+		if ((rc = ensure(w, b, std::max<size_t>(1, (size_t)n * w.rec_bytes)))) return rc;
+		rc = gf_fits_pack_rows(oc.data(), (int)oc.size(), hidden, n, w.dbuf[b], n, &nv, w.st);
+		if (rc) return rc;
+		bytes = (size_t)nv * w.rec_bytes;
+	} else {
+		if (!w.dbuf[b] && (rc = ensure(w, b, 1 << 20))) return rc;
+		uint64_t nb = 0;
+		rc = gf_text_format_rows(oc.data(), (int)oc.size(), hidden, n, (char *)w.dbuf[b], w.dcap[b], &nb, &nv, w.st);
+		if (rc == GF_E_CAPACITY) {
+			if ((rc = ensure(w, b, nb + nb / 8))) return rc;
+			rc = gf_text_format_rows(oc.data(), (int)oc.size(), hidden, n, (char *)w.dbuf[b], w.dcap[b], &nb, &nv,
+			                         w.st);
+		}
+		if (rc) return rc;
+		bytes = nb;
+	}
+	// the copy waits for the packing kernel, which may still run
+	GF_HIP(hipEventRecord(w.copied[b], w.st));
+	GF_HIP(hipStreamWaitEvent(w.cs, w.copied[b], 0));
+	if (bytes) GF_HIP(hipMemcpyAsync(w.hbuf[b], w.dbuf[b], bytes, hipMemcpyDeviceToHost, w.cs));
+	GF_HIP(hipEventRecord(w.copied[b], w.cs));
+	if ((rc = flush_pending(w))) return rc;
+	w.pending = b;
+	w.pending_bytes = bytes;
+	w.next = b ^ 1;
+	w.rows += nv;
 	return 0;
 }
 
@@ -467,6 +542,8 @@ int gf_writer_open(const gf_catalog *c, const char *path, int kind, gf_writer **
 		w->rec_bytes += k.width * (k.type == 'D' ? 8 : 4);
 	}
 	GF_HIP(hipStreamCreate(&w->st));
+	GF_HIP(hipStreamCreate(&w->cs));
+	for (auto &e : w->copied) GF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 	w->f = fopen(path, "wb");
 	if (!w->f) return GF_E_IO;
 	if (kind == GF_OUT_TEXT) {
@@ -522,35 +599,11 @@ int gf_writer_chunk(void *user, const gf_table *t, uint64_t nrows, int comp)
 		w->oc[i].col = t->*(w->cols[i].member);
 		if (!w->oc[i].col.ptr) return w->error = GF_E_ARG;
 	}
-	uint64_t nv = 0;
-	if (w->kind == GF_OUT_FITS) {
-		if ((rc = ensure(*w, std::max<size_t>(1, (size_t)nrows * w->rec_bytes)))) return w->error = rc;
-		rc = gf_fits_pack_rows(w->oc.data(), (int)w->oc.size(), t->hidden, nrows, w->dbuf, nrows, &nv, w->st);
-		if (rc) return w->error = rc;
-		const size_t bytes = (size_t)nv * w->rec_bytes;
-		if (bytes) {
-			GF_HIP(hipMemcpyAsync(w->hbuf, w->dbuf, bytes, hipMemcpyDeviceToHost, w->st));
-			GF_HIP(hipStreamSynchronize(w->st));
-			if ((rc = write_all(w->f, w->hbuf, bytes))) return w->error = rc;
-		}
-	} else {
-		uint64_t nb = 0;
-		if (!w->dbuf && (rc = ensure(*w, 1 << 20))) return w->error = rc;
-		rc = gf_text_format_rows(w->oc.data(), (int)w->oc.size(), t->hidden, nrows, (char *)w->dbuf, w->dcap, &nb,
-		                         &nv, w->st);
-		if (rc == GF_E_CAPACITY) {
-			if ((rc = ensure(*w, nb))) return w->error = rc;
-			rc = gf_text_format_rows(w->oc.data(), (int)w->oc.size(), t->hidden, nrows, (char *)w->dbuf, w->dcap,
-			                         &nb, &nv, w->st);
-		}
-		if (rc) return w->error = rc;
-		if (nb) {
-			GF_HIP(hipMemcpyAsync(w->hbuf, w->dbuf, nb, hipMemcpyDeviceToHost, w->st));
-			GF_HIP(hipStreamSynchronize(w->st));
-			if ((rc = write_all(w->f, w->hbuf, nb))) return w->error = rc;
-		}
-	}
-	w->rows += nv;
+	for (uint64_t r0 = 0; r0 < nrows; r0 += PIECE_ROWS)
+		if ((rc = write_piece(*w, t, r0, std::min(PIECE_ROWS, nrows - r0)))) return w->error = rc;
+	// the chunk's columns are the run's again once the callback returns:
+	// every piece has been packed or formatted (the copies may still run)
+	GF_HIP(hipStreamSynchronize(w->st));
 	return 0;
 }
 
@@ -558,6 +611,7 @@ int gf_writer_close(gf_writer *w, uint64_t *rows)
 {
 	if (!w) return GF_E_ARG;
 	int rc = w->error;
+	if (!rc && w->f) rc = flush_pending(*w);
 	if (!rc && w->kind == GF_OUT_COUNTSMAP && w->f) rc = countsmap_write(*w);
 	if (!rc && w->kind == GF_OUT_FITS && w->f) {
 		// the CATALOG table's row count, its data padded to a 2880-byte block,

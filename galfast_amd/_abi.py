@@ -60,7 +60,7 @@ class gf_vel2pm_data(C.Structure):
 class gf_photometry_data(C.Structure):
     _fields_ = [("ncolors", u32), ("bidx", u32), ("reddening", f32 * 17),
                 ("nFeH", i32), ("nMr", i32), ("tcFeH", f32 * 2), ("tcMr", f32 * 2),
-                ("color", vp * 4), ("cflags", vp * 4), ("texels", vp)]
+                ("color", vp * 4), ("cflags", vp * 4), ("texels", vp), ("flagbits", vp)]
 
 
 class gf_tex1d(C.Structure):

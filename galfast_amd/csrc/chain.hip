@@ -19,6 +19,7 @@ __global__ void __launch_bounds__(gfc::BS) k_chain_draws(const gf_chain_launch c
 	__shared__ gfc::DrawLds L;
 	uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= nthr) return;
+	__builtin_amdgcn_s_setprio(3);
 	uint64_t a = (uint64_t)c.begin + (uint64_t)c.step * t;
 	uint32_t r0 = (uint32_t)a, r1 = (uint32_t)std::min<uint64_t>(a + c.step, c.end);
 	gf_mwc r = gf_mwc_load(st, ns, t);
@@ -111,12 +112,9 @@ int chain_launch(const gf_chain_launch &L, const gf_rng *rng, int launch, hipStr
 		k_chain_rows<gfc::ROWS_BOND, 1><<<grows, gfc::BS, 0, st>>>(L, rng->state, 1);
 		break;
 	case 1 + gfc::ROWS_PHOT:
-		switch (gfc::phot_tables(L.m)) {
-		case 1: k_chain_rows<gfc::ROWS_PHOT, 1><<<grows, gfc::BS, 0, st>>>(L, rng->state, 0); break;
-		case 2: k_chain_rows<gfc::ROWS_PHOT, 2><<<grows, gfc::BS, 0, st>>>(L, rng->state, 0); break;
-		case 3: k_chain_rows<gfc::ROWS_PHOT, 3><<<grows, gfc::BS, 0, st>>>(L, rng->state, 0); break;
-		default: k_chain_rows<gfc::ROWS_PHOT, 4><<<grows, gfc::BS, 0, st>>>(L, rng->state, 0); break;
-		}
+		gfc::phot_dispatch(gfc::phot_variant(L.m), [&](auto nt) {
+			k_chain_rows<gfc::ROWS_PHOT, decltype(nt)::value><<<grows, gfc::BS, 0, st>>>(L, rng->state, 0);
+		});
 		break;
 	case 1 + gfc::ROWS_PERR:
 		k_chain_rows<gfc::ROWS_PERR, 1><<<grows, gfc::BS, 0, st>>>(L, rng->state, 1);

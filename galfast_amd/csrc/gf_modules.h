@@ -360,14 +360,25 @@ GF_D uint32_t sample_colors(const gf_photometry_data &lt, float *c, float FeH, f
 	const float b = lerp_coord(Mr, lt.tcMr[0], lt.tcMr[1], lt.nMr, j);
 	const int i1 = lt.nFeH > 1 ? i + 1 : i, j1 = lt.nMr > 1 ? j + 1 : j;
 	const int ntex = ((int)lt.ncolors + 3) / 4;
-	const float4 *tx = (const float4 *)lt.texels;
-	const size_t k00 = ((size_t)j * lt.nFeH + i) * 2 * ntex, k10 = ((size_t)j * lt.nFeH + i1) * 2 * ntex;
-	const size_t k01 = ((size_t)j1 * lt.nFeH + i) * 2 * ntex, k11 = ((size_t)j1 * lt.nFeH + i1) * 2 * ntex;
+	const float4 *tx = lt.flagbits ? (const float4 *)lt.texels : nullptr;
+	const size_t t00 = (size_t)j * lt.nFeH + i, t10 = (size_t)j * lt.nFeH + i1;
+	const size_t t01 = (size_t)j1 * lt.nFeH + i, t11 = (size_t)j1 * lt.nFeH + i1;
+	uint32_t b00 = 0, b10 = 0, b01 = 0, b11 = 0;
+	if (tx) {
+		b00 = lt.flagbits[t00]; b10 = lt.flagbits[t10];
+		b01 = lt.flagbits[t01]; b11 = lt.flagbits[t11];
+	}
 	for (int t = 0; t < NT && cidx < lt.ncolors; t++) {
 		float4 clr, f;
 		if (tx) {
-			clr = bilerp4(tx[k00 + t], tx[k10 + t], tx[k01 + t], tx[k11 + t], a, b);
-			f = bilerp4(tx[k00 + ntex + t], tx[k10 + ntex + t], tx[k01 + ntex + t], tx[k11 + ntex + t], a, b);
+			clr = bilerp4(tx[t00 * ntex + t], tx[t10 * ntex + t], tx[t01 * ntex + t], tx[t11 * ntex + t], a, b);
+			// the flag texels, 0 or 1, filtered as the colours are
+			auto unpack = [t](uint32_t bits) {
+				bits >>= 4 * t;
+				return make_float4((float)(bits & 1u), (float)((bits >> 1) & 1u), (float)((bits >> 2) & 1u),
+				                   (float)((bits >> 3) & 1u));
+			};
+			f = bilerp4(unpack(b00), unpack(b10), unpack(b01), unpack(b11), a, b);
 		} else {
 			clr = tex2d4(lt.color[t], lt.nFeH, lt.nMr, lt.tcFeH, lt.tcMr, FeH, Mr);
 			f = tex2d4(lt.cflags[t], lt.nFeH, lt.nMr, lt.tcFeH, lt.tcMr, FeH, Mr);
@@ -385,7 +396,9 @@ GF_D uint32_t sample_colors(const gf_photometry_data &lt, float *c, float FeH, f
 
 // os_photometry_kernel body (modules/photometry_gpu.cu.h:100-173), for at
 // most 4*NT colours (the caller checks ncolors <= 4*NT)
-template <int NT = 4>
+// SYS = false: only single stars (one absolute magnitude per row), the
+// register footprint of the lone-star path alone.
+template <int NT = 4, bool SYS = true>
 GF_D void mod_photometry(const gf_photometry_data &lt, gf_bitmap apply, uint32_t r0, uint32_t r1,
                          const gf_col &Am, const gf_col &flags, const gf_col &DM, const gf_col &Mr,
                          int nabsmag, const gf_col &mags, const gf_col &FeH, const gf_col &comp,
@@ -402,7 +415,8 @@ GF_D void mod_photometry(const gf_photometry_data &lt, gf_bitmap apply, uint32_t
 		const float dm = col<float>(DM, row), Am0 = col<float>(Am, row);
 		uint32_t flag = 0;
 		const float fMr0 = col<float>(Mr, row, 0);
-		const bool system = fMr0 < ABSMAG_NOT_PRESENT && nabsmag > 1 && col<float>(Mr, row, 1) < ABSMAG_NOT_PRESENT;
+		const bool system =
+			SYS && fMr0 < ABSMAG_NOT_PRESENT && nabsmag > 1 && col<float>(Mr, row, 1) < ABSMAG_NOT_PRESENT;
 		if (!system) {
 			// a lone star: its magnitudes are the colour sums themselves
 			// (-2.5 log10 10^(-0.4 M) = M, without the round trip)

@@ -139,7 +139,7 @@ struct gf_catalog {
 	gf_Bond2010_data bond{};
 	gf_photometry_data phot{};
 	PhotTables phot_tables;
-	DevBuf phot_dev[8], phot_il;
+	DevBuf phot_dev[8], phot_il, phot_fb;
 	int nbands = 0;
 	std::vector<std::string> band_names;
 	std::string bandset;   // the photometric system's name (photometry_host.cpp:94)
@@ -736,18 +736,24 @@ void construct_photometry(gf_catalog &c, const Config &cfg)
 		p.color[t] = (const float *)c.phot_dev[t].p;
 		p.cflags[t] = (const float *)c.phot_dev[4 + t].p;
 	}
-	// the interleaved copy: per texel the ntex colour float4s, then the flags
+	// the interleaved copy: per texel the ntex colour float4s; the flags (0
+	// or 1) as one bit per colour
 	{
 		const size_t ntexel = (size_t)pt.nFeH * pt.nMr, nt = (size_t)pt.ntex;
-		std::vector<float> il(ntexel * 8 * nt);
+		std::vector<float> il(ntexel * 4 * nt);
+		std::vector<uint16_t> fb(ntexel, 0);
 		for (size_t k = 0; k < ntexel; k++)
 			for (size_t t = 0; t < nt; t++)
 				for (int q = 0; q < 4; q++) {
-					il[(k * 2 * nt + t) * 4 + q] = pt.color[t * per + k * 4 + q];
-					il[(k * 2 * nt + nt + t) * 4 + q] = pt.flags[t * per + k * 4 + q];
+					il[(k * nt + t) * 4 + q] = pt.color[t * per + k * 4 + q];
+					const float f = pt.flags[t * per + k * 4 + q];
+					if (f != 0.f && f != 1.f) fail("photometry flags must be 0 or 1");
+					if (f != 0.f) fb[k] |= (uint16_t)(1u << (4 * t + q));
 				}
 		c.phot_il.upload(il);
+		c.phot_fb.upload(fb);
 		p.texels = (const float *)c.phot_il.p;
+		p.flagbits = (const uint16_t *)c.phot_fb.p;
 	}
 	p.nFeH = pt.nFeH;
 	p.nMr = pt.nMr;
@@ -969,6 +975,7 @@ void open_catalog(gf_catalog &c, const std::string &cmdfile)
 			c.phot_h.cflags[t] = t < c.phot_tables.ntex ? c.phot_tables.flags.data() + t * per : nullptr;
 		}
 		c.phot_h.texels = nullptr;
+		c.phot_h.flagbits = nullptr;
 	}
 	if (c.has_mult) {
 		c.mult_h = c.mult;

@@ -360,6 +360,7 @@ __global__ void __launch_bounds__(SK_BS) k_plan(const double *wpk, uint64_t U, u
 {
 	uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
 	if (s >= ns) return;
+	__builtin_amdgcn_s_setprio(3);   // the catalog's serial stream first among resident waves
 	gf_mwc rng = gf_mwc_load(st, ns, s);
 	for (uint64_t u = s; u < U; u += ns) {
 		double w = wpk[u];
@@ -440,6 +441,7 @@ __global__ void __launch_bounds__(SK_BS) k_emit_rng(gf_skygen_params sp, gf_mode
 	if (s >= ns) return;
 	const int K = sp.nm + sp.nM - 1;
 	uint64_t first = u_lo + ((uint64_t)s + ns - (u_lo % ns)) % ns;
+	__builtin_amdgcn_s_setprio(3);   // one wave per SIMD: first among the resident waves
 	gf_mwc rng = gf_mwc_load(st, ns, s);
 	bool touched = false;
 	for (uint64_t u = first; u < U; u += ns) {

@@ -177,11 +177,13 @@ typedef struct gf_photometry_data {
     float    tcFeH[2], tcMr[2];
     const float *color[4];   /* device, float4 texels */
     const float *cflags[4];  /* device, float4 texels */
-    /* optional (device): the same texels interleaved, texel k holding the
-     * ntex colour float4s then the ntex flag float4s at texels + 8*ntex*k,
-     * so that the two texels a lookup needs in one Mr row are one
-     * contiguous fetch; used instead of color/cflags when set */
+    /* optional (device): the colour texels interleaved, texel k holding the
+     * ntex colour float4s at texels + 4*ntex*k, so that the two texels a
+     * lookup needs in one Mr row are one contiguous fetch, and the flags as
+     * bits (flagbits[k] bit i set when colour i's flag texel is nonzero);
+     * used instead of color/cflags when both are set */
     const float *texels;
+    const uint16_t *flagbits;
 } gf_photometry_data;
 
 /* Tabulated 1D function on a uniform grid, sampled with the reference's

@@ -35,7 +35,7 @@ WORKLOADS = {
 }
 
 PEAK_HBM_GBS = 8000.0
-# Algorithmic bytes per row of each kernel (stated in DESIGN.md): every
+# Algorithmic bytes per row of each kernel: every
 # column a kernel must read, and every column it writes, once.  Hidden rows
 # cost the 4-byte hidden flag in the module kernels.
 # emit, per generated row: the random half writes the 20-byte parked star
@@ -358,8 +358,8 @@ def main():
         print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4}))
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the sky would shard by "
-              "pixels, see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (the sky would shard by "
+              "pixels across GPUs); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     if args.output_bench:

@@ -27,7 +27,7 @@ GF_D bool bit_isset(const gf_bitmap &m, int bit)
 // convention of sample_impl (common/cux.h:849-855): xi = (x - tc0)*tc1 + 0.5,
 // texel centres at integer + 0.5.  The reference's CUDA path filters in
 // hardware and its CPU path takes the nearest texel (common/cux.h:778-810);
-// this build evaluates the interpolation exactly in FP32.
+// here the interpolation is evaluated exactly in FP32.
 GF_D float lerp_coord(float x, float tc0, float tc1, int n, int &i)
 {
 	float xi = (x - tc0) * tc1 + 0.5f;

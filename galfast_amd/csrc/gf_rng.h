@@ -137,7 +137,7 @@ struct gf_mwc
 		if (aa == (float)na) return b * gamma_int(na);
 		if (na == 0) return b * gamma_frac(aa);
 		// the reference leaves the order of these two draws unspecified;
-		// this build draws the integer part first
+		// here the integer part is drawn first
 		float gi = gamma_int(na);
 		float gf = gamma_frac(aa - (float)na);
 		return b * (gi + gf);

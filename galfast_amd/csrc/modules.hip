@@ -1,7 +1,7 @@
 // Global kernels and C-ABI entry points of the per-star modules.  Thread t
 // of a launch owns rows [begin + step*t, min(begin + step*(t+1), end)) and
 // RNG stream t, as KERNEL/kernel_state do in the reference (common/gpu.h:124-156,
-// 240-257); the launch shape itself is this build's (256-lane workgroups).
+// 240-257); the workgroups are 256 lanes.
 #include "gf_runtime.h"
 #include "gf_modules.h"
 

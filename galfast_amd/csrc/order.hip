@@ -11,7 +11,7 @@
 // compare equal on every key keep their emission order (std::sort leaves
 // that order unspecified).  gf_clip_footprint is the clipper stage
 // (os_clipper::process, skygen/os_skygen.cpp:997-1044) for the analytic
-// footprints of this build: a visible row outside the footprint becomes
+// footprints (rect, beam): a visible row outside the footprint becomes
 // hidden, judged from its projection-plane position exactly as the draw
 // judges it.
 #include "gf_runtime.h"

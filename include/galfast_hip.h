@@ -428,7 +428,7 @@ int gf_skygen_order(const gf_skygen_cols *in, uint64_t n, const gf_skygen_cols *
                     uint64_t *nvisible, void *stream);
 
 /* The clipper stage (os_clipper::process, skygen/os_skygen.cpp:997-1044)
- * for the analytic footprints of this build: a visible row of [0, n) whose
+ * for the analytic footprints (rect, beam): a visible row of [0, n) whose
  * projection-plane position deprojects outside `foot` gets hidden = 1. */
 int gf_clip_footprint(const gf_skygen_params *sp, const gf_footprint *foot, gf_col projIdx,
                       gf_col projXY, gf_col hidden, uint64_t n, void *stream);

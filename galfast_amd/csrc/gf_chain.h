@@ -19,7 +19,6 @@
 #pragma once
 
 #include "gf_modules.h"
-#include <type_traits>
 #include "gf_chain_types.h"
 
 
@@ -211,8 +210,8 @@ template <int NT>
 GF_D void rows_phot(const gf_chain_mods &m, uint32_t row, const gf_table &T)
 {
 	const gf_col &Mr = m.has_mult ? T.absmagSys : T.absmag;
-	mod_photometry<(NT & 7), (NT & 8) != 0>(m.phot, m.phot_comps, row, row + 1, T.Am, T.photoFlags, T.DM, Mr,
-	                                        (int)Mr.width, T.mags, T.FeH, T.comp, T.hidden);
+	mod_photometry<NT>(m.phot, m.phot_comps, row, row + 1, T.Am, T.photoFlags, T.DM, Mr, (int)Mr.width, T.mags,
+	                   T.FeH, T.comp, T.hidden);
 }
 
 GF_D void rows_perr(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t prow, const gf_table &T,
@@ -229,8 +228,8 @@ GF_D void rows_astro(const gf_chain_mods &m, uint32_t row, const gf_table &T)
 	if (m.gal2other) mod_gal2other(m.gal2other, row, row + 1, T.hidden, T.lb, T.radec);
 }
 
-// GROUP: one of ROWS_*; NT: for ROWS_PHOT the photometry variant
-// (phot_variant()), otherwise 1
+// GROUP: one of ROWS_*; NT: colour tables of the photometry (1..4, at
+// least ceil(ncolors / 4); see phot_tables())
 template <int GROUP, int NT>
 GF_D void chain_rows(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t prow, const gf_table &T,
                      const gf_positions &P)
@@ -245,29 +244,6 @@ GF_D void chain_rows(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t 
 inline int phot_tables(const gf_chain_mods &m)
 {
 	return m.has_phot ? ((int)m.phot.ncolors + 3) / 4 : 1;
-}
-
-// The photometry launch's NT: the colour tables, plus 8 when rows can be
-// systems of several stars (unresolvedMultiples in the chain).
-inline int phot_variant(const gf_chain_mods &m)
-{
-	return phot_tables(m) | (m.has_mult ? 8 : 0);
-}
-
-// Calls f(std::integral_constant<int, NT>) for the photometry variant v.
-template <typename F>
-inline void phot_dispatch(int v, F &&f)
-{
-	switch (v) {
-	case 1: f(std::integral_constant<int, 1>()); break;
-	case 2: f(std::integral_constant<int, 2>()); break;
-	case 3: f(std::integral_constant<int, 3>()); break;
-	case 4: f(std::integral_constant<int, 4>()); break;
-	case 9: f(std::integral_constant<int, 9>()); break;
-	case 10: f(std::integral_constant<int, 10>()); break;
-	case 11: f(std::integral_constant<int, 11>()); break;
-	default: f(std::integral_constant<int, 12>()); break;
-	}
 }
 
 } // namespace gfc

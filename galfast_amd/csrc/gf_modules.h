@@ -396,9 +396,7 @@ GF_D uint32_t sample_colors(const gf_photometry_data &lt, float *c, float FeH, f
 
 // os_photometry_kernel body (modules/photometry_gpu.cu.h:100-173), for at
 // most 4*NT colours (the caller checks ncolors <= 4*NT)
-// SYS = false: only single stars (one absolute magnitude per row), the
-// register footprint of the lone-star path alone.
-template <int NT = 4, bool SYS = true>
+template <int NT = 4>
 GF_D void mod_photometry(const gf_photometry_data &lt, gf_bitmap apply, uint32_t r0, uint32_t r1,
                          const gf_col &Am, const gf_col &flags, const gf_col &DM, const gf_col &Mr,
                          int nabsmag, const gf_col &mags, const gf_col &FeH, const gf_col &comp,
@@ -415,8 +413,7 @@ GF_D void mod_photometry(const gf_photometry_data &lt, gf_bitmap apply, uint32_t
 		const float dm = col<float>(DM, row), Am0 = col<float>(Am, row);
 		uint32_t flag = 0;
 		const float fMr0 = col<float>(Mr, row, 0);
-		const bool system =
-			SYS && fMr0 < ABSMAG_NOT_PRESENT && nabsmag > 1 && col<float>(Mr, row, 1) < ABSMAG_NOT_PRESENT;
+		const bool system = fMr0 < ABSMAG_NOT_PRESENT && nabsmag > 1 && col<float>(Mr, row, 1) < ABSMAG_NOT_PRESENT;
 		if (!system) {
 			// a lone star: its magnitudes are the colour sums themselves
 			// (-2.5 log10 10^(-0.4 M) = M, without the round trip)

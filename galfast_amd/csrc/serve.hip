@@ -374,9 +374,12 @@ int serve_chain(const gf_serve_req *dreqs, int nreq, uint32_t nlanes, const gf_s
 		GF_LAUNCHED();
 	}
 	if (gfc::rows_on(hch, gfc::ROWS_PHOT)) {
-		gfc::phot_dispatch(gfc::phot_variant(hch), [&](auto nt) {
-			k_b_rows<gfc::ROWS_PHOT, decltype(nt)::value><<<g, gfc::BS, 0, st>>>(dreqs, nreq, dch, nrows, T, P, 0);
-		});
+		switch (gfc::phot_tables(hch)) {
+		case 1: k_b_rows<gfc::ROWS_PHOT, 1><<<g, gfc::BS, 0, st>>>(dreqs, nreq, dch, nrows, T, P, 0); break;
+		case 2: k_b_rows<gfc::ROWS_PHOT, 2><<<g, gfc::BS, 0, st>>>(dreqs, nreq, dch, nrows, T, P, 0); break;
+		case 3: k_b_rows<gfc::ROWS_PHOT, 3><<<g, gfc::BS, 0, st>>>(dreqs, nreq, dch, nrows, T, P, 0); break;
+		default: k_b_rows<gfc::ROWS_PHOT, 4><<<g, gfc::BS, 0, st>>>(dreqs, nreq, dch, nrows, T, P, 0); break;
+		}
 		GF_LAUNCHED();
 	}
 	if (gfc::rows_on(hch, gfc::ROWS_PERR)) {
@@ -400,9 +403,12 @@ int serve_chain_graph(const gf_serve_chain_args *dargs, const gf_serve_chain *dc
 	if (gfc::rows_on(hch, gfc::ROWS_FEH)) k_g_rows<gfc::ROWS_FEH, 1><<<gr, gfc::BS, 0, st>>>(dargs, dch, 1);
 	if (gfc::rows_on(hch, gfc::ROWS_BOND)) k_g_rows<gfc::ROWS_BOND, 1><<<gr, gfc::BS, 0, st>>>(dargs, dch, 1);
 	if (gfc::rows_on(hch, gfc::ROWS_PHOT)) {
-		gfc::phot_dispatch(gfc::phot_variant(hch), [&](auto nt) {
-			k_g_rows<gfc::ROWS_PHOT, decltype(nt)::value><<<gr, gfc::BS, 0, st>>>(dargs, dch, 0);
-		});
+		switch (gfc::phot_tables(hch)) {
+		case 1: k_g_rows<gfc::ROWS_PHOT, 1><<<gr, gfc::BS, 0, st>>>(dargs, dch, 0); break;
+		case 2: k_g_rows<gfc::ROWS_PHOT, 2><<<gr, gfc::BS, 0, st>>>(dargs, dch, 0); break;
+		case 3: k_g_rows<gfc::ROWS_PHOT, 3><<<gr, gfc::BS, 0, st>>>(dargs, dch, 0); break;
+		default: k_g_rows<gfc::ROWS_PHOT, 4><<<gr, gfc::BS, 0, st>>>(dargs, dch, 0); break;
+		}
 	}
 	if (gfc::rows_on(hch, gfc::ROWS_PERR)) k_g_rows<gfc::ROWS_PERR, 1><<<gr, gfc::BS, 0, st>>>(dargs, dch, 1);
 	if (gfc::rows_on(hch, gfc::ROWS_ASTRO)) k_g_rows<gfc::ROWS_ASTRO, 1><<<gr, gfc::BS, 0, st>>>(dargs, dch, 0);

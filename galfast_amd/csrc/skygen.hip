@@ -362,11 +362,25 @@ __global__ void __launch_bounds__(SK_BS) k_plan(const double *wpk, uint64_t U, u
 	if (s >= ns) return;
 	__builtin_amdgcn_s_setprio(3);   // the catalog's serial stream first among resident waves
 	gf_mwc rng = gf_mwc_load(st, ns, s);
-	for (uint64_t u = s; u < U; u += ns) {
-		double w = wpk[u];
+	// two units per trip, each unit's weight loaded a trip ahead (one wave
+	// per SIMD has nothing else to hide a load's latency behind); the two
+	// weights keep their registers, so no copy waits on a load in flight,
+	// and every trip issues the same loads (indices clamped to the last
+	// unit), so the waits count them exactly
+	const uint64_t step = 2ull * ns, last = U - 1;
+	double wa = wpk[s < last ? s : last];
+	double wb = wpk[s + (uint64_t)ns < last ? s + ns : last];
+	for (uint64_t u = s; u < U; u += step) {
 		uint32_t n = 0;
-		if (w > 0.0) n = rng.poisson((float)w);
+		if (wa > 0.0) n = rng.poisson((float)wa);
 		counts[u] = n;
+		wa = wpk[u + step < last ? u + step : last];
+		const uint64_t v = u + ns;
+		if (v >= U) break;
+		n = 0;
+		if (wb > 0.0) n = rng.poisson((float)wb);
+		counts[v] = n;
+		wb = wpk[v + step < last ? v + step : last];
 	}
 	gf_mwc_store(st, ns, s, rng);
 }

@@ -219,8 +219,16 @@ GF_D int lf_bin(const double *P, const uint16_t *G, double scale, int lo, int hi
 	int q = qd > 0.0 ? (qd < (double)(LF_GUESS - 1) ? (int)qd : LF_GUESS - 1) : 0;
 	int a = G[q];
 	a = a < lo ? lo : (a > hi ? hi : a);
-	while (a > lo && P[a] > tgt) a--;
-	while (a < hi && P[a + 1] <= tgt) a++;
+	// both neighbours of the guess are read at once; a walk down ends
+	// where the walk up would not start (P is nondecreasing)
+	const double pa = P[a], pa1 = P[a + 1];
+	if (a > lo && pa > tgt) {
+		a--;
+		while (a > lo && P[a] > tgt) a--;
+	} else if (a < hi && pa1 <= tgt) {
+		a++;
+		while (a < hi && P[a + 1] <= tgt) a++;
+	}
 	return a;
 }
 

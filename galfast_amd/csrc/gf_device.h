@@ -213,6 +213,40 @@ GF_D double lf_guess_scale(const double *P, int nM)
 	return r > 0.0 ? (double)LF_GUESS / r : 0.0;
 }
 
+// The guess table's thresholds: GP[q] = (P[G[q]], P[G[q] + 1]), so that a
+// draw reads its guess and both the guess's neighbours at once.
+GF_D void lf_guess_pairs(const double *P, const uint16_t *G, double2 *GP, int tid, int nthr)
+{
+	for (int q = tid; q < LF_GUESS; q += nthr) {
+		const int g = G[q];
+		GP[q] = make_double2(P[g], P[g + 1]);
+	}
+}
+
+// lf_bin with the thresholds of the guess from GP (the same bin).
+GF_D int lf_bin_pairs(const double *P, const uint16_t *G, const double2 *GP, double scale, int lo, int hi,
+                      double tgt)
+{
+	double qd = (tgt - P[0]) * scale;
+	int q = qd > 0.0 ? (qd < (double)(LF_GUESS - 1) ? (int)qd : LF_GUESS - 1) : 0;
+	const int g = G[q];
+	const double2 pp = GP[q];
+	int a = g < lo ? lo : (g > hi ? hi : g);
+	double pa = pp.x, pa1 = pp.y;
+	if (a != g) {
+		pa = P[a];
+		pa1 = P[a + 1];
+	}
+	if (a > lo && pa > tgt) {
+		a--;
+		while (a > lo && P[a] > tgt) a--;
+	} else if (a < hi && pa1 <= tgt) {
+		a++;
+		while (a < hi && P[a + 1] <= tgt) a++;
+	}
+	return a;
+}
+
 GF_D int lf_bin(const double *P, const uint16_t *G, double scale, int lo, int hi, double tgt)
 {
 	double qd = (tgt - P[0]) * scale;

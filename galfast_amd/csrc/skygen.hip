@@ -387,6 +387,14 @@ __global__ void __launch_bounds__(SK_BS) k_plan(const double *wpk, uint64_t U, u
 
 constexpr int EMIT_MAX_NM = 4096;
 
+// LDS of the emission: the LF prefix (nM + 1 doubles), the guess table
+// (LF_GUESS uint16) and, 16-byte aligned, the guess thresholds.
+GF_HD size_t lf_pairs_offset(int nM)
+{
+	const size_t b = ((size_t)nM + 1) * sizeof(double) + LF_GUESS * sizeof(uint16_t);
+	return (b + 15) & ~(size_t)15;
+}
+
 __global__ void __launch_bounds__(SK_BS) k_emit(gf_skygen_params sp, gf_model m, const gf_pencil_beam *pixels,
                                                 gf_extinction ext, gf_footprint foot, const double *Pg,
                                                 const uint64_t *offs, uint64_t U, uint64_t row_begin,
@@ -446,9 +454,12 @@ __global__ void __launch_bounds__(SK_BS) k_emit_rng(gf_skygen_params sp, gf_mode
 {
 	extern __shared__ double P[];
 	uint16_t *G = (uint16_t *)(P + sp.nM + 1);
+	double2 *GP = (double2 *)((char *)P + lf_pairs_offset(sp.nM));
 	for (int i = threadIdx.x; i <= sp.nM; i += blockDim.x) P[i] = Pg[i];
 	__syncthreads();
 	lf_guess_build(P, sp.nM, G, threadIdx.x, blockDim.x);
+	__syncthreads();
+	lf_guess_pairs(P, G, GP, threadIdx.x, blockDim.x);
 	__syncthreads();
 	const double gscale = lf_guess_scale(P, sp.nM);
 	uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -474,7 +485,7 @@ __global__ void __launch_bounds__(SK_BS) k_emit_rng(gf_skygen_params sp, gf_mode
 		uint64_t j1 = row_end < o1 ? row_end : o1;
 		for (uint64_t j = j0; j < j1; j++) {
 			double tgt = base + (double)rng.uniform() * span;
-			int iM = lf_bin(P, G, gscale, lo, hi, tgt);
+			int iM = lf_bin_pairs(P, G, GP, gscale, lo, hi, tgt);
 			float M = sp.M1 - iM * sp.dM;
 			float sx, sy, Mtmp = 0.f, mtmp = 0.f;
 			bool in = star_draws(sp, pix, M, k - iM, rng, sx, sy, Mtmp, mtmp);
@@ -706,7 +717,7 @@ int skygen_emit(const gf_draw_plan *plan, uint64_t row_begin, uint64_t row_end, 
 	}
 	GF_CHECK(u_lo < plan->U, GF_E_ARG);
 	size_t shb = ((size_t)plan->sp.nM + 1) * sizeof(double);
-	const size_t shb_guess = shb + LF_GUESS * sizeof(uint16_t);
+	const size_t shb_guess = lf_pairs_offset(plan->sp.nM) + LF_GUESS * sizeof(double2);
 	if (plan->ext.beam.data || plan->ext.north.data || plan->ext.south.data) {
 		// with extinction the star needs its DM bin's minimum extinction: one pass
 		if (ev) GF_HIP(hipEventRecord(ev[0], st));

@@ -222,3 +222,40 @@ def test_fits_and_text_output(tmp_path, cfg, order):
                             v = ids[v]
                         want.append(fmts[name] % (float(v) if fmts[name].endswith("f") else int(v)))
                 assert lines[r] == " ".join(want), (r, lines[r])
+
+
+def test_fits_pieces_across_chunks(tmp_path):
+    """A config-1-shaped job of ~1.2e7 stars in 4.5e6-row chunks: the writer packs
+    each chunk in pieces of 2^22 rows, copies one piece to the host while
+    packing the next, and carries the last piece of a chunk into the next
+    chunk's callback; the file must hold every visible row once, in order."""
+    d = str(tmp_path)
+    cmd = configs.write_c1(d)
+    kw = dict(chunk_rows=4_500_000)
+    cat = Catalog(cmd, seed=42, **kw)
+    cat.set("nstars", 12_000_000)
+    cat.count()
+    ids = np.array(cat.component_ids(), dtype=np.int64)
+    info = cat.info()
+    spec = catalog_spec(nbands=info.nbands, nabsmag=max(1, info.nabsmag))
+    got = collect(cat, spec)
+    cat.close()
+    assert len(got) >= 2 and max(n for _, n, _ in got) > (1 << 22)
+    cat = Catalog(cmd, seed=42, **kw)
+    cat.set("nstars", 12_000_000)
+    cat.count()
+    path = os.path.join(d, "sky.fits")
+    print("writing pieces", flush=True)
+    st, rows = cat.write(path, "fits")
+    cat.close()
+    hdr, cols, data = read_fits_tables(path)["CATALOG"]
+    nvis = int(hdr["NAXIS2"])
+    assert rows == nvis == st.visible
+    dt = [(name, {"E": ">f4", "D": ">f8", "J": ">i4"}[form[-1]], (int(form[:-1]),)) for name, form in cols]
+    arr = np.frombuffer(data, dtype=np.dtype(dt), count=nvis)
+    for name, form in cols:
+        parts = [t.arr(table_column(name))[:, :n][:, t.arr("hidden")[0, :n] == 0] for t, n, _ in got]
+        want = np.concatenate(parts, axis=1).T
+        if name == "comp":
+            want = ids[want]
+        assert np.array_equal(arr[name].astype(want.dtype), want), name

@@ -281,8 +281,9 @@ def _pack_and_format(L, A, table, n, reps=3):
     total = torch.zeros(1, dtype=torch.int64, device="cuda")
     dt = best(lambda: L.gf_countsmap_bin(C.byref(p), table.lb, table.obsMags, table.hidden, n, counts.data_ptr(),
                                          beams.data_ptr(), total.data_ptr(), None))
+    # read: every row's hidden flag, each visible row's (l, b) and 5 magnitudes
     out["countsmap_bin"] = {"rows": n, "ms": 1e3 * dt, "rows_per_s": n / dt,
-                            "atomics_per_s": int(total.item()) // reps * (zw / 5) / dt}
+                            "GBps": (4 * n + (16 + 20) * int(total.item()) // (5 * reps)) / dt / 1e9}
     return out
 
 

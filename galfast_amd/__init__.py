@@ -71,6 +71,7 @@ def _declare(L):
                                         P(C.c_uint64), vp]
     L.gf_skygen_order.argtypes = [P(gf_skygen_cols), C.c_uint64, P(gf_skygen_cols), P(C.c_uint64), vp]
     L.gf_clip_footprint.argtypes = [P(gf_skygen_params), P(gf_footprint), gf_col, gf_col, gf_col, C.c_uint64, vp]
+    L.gf_footprint_cover.argtypes = [P(gf_footprint), C.c_double, C.c_int32, vp, vp]
     L.gf_countsmap_bin.argtypes = [P(gf_countsmap_params), gf_col, gf_col, gf_col, C.c_uint64, vp, vp, vp, vp]
     L.gf_countsmap_finish.argtypes = [P(gf_countsmap_params), vp, vp]
     L.gf_skygen_plan.argtypes = [P(gf_skygen_params), P(gf_model), vp, P(gf_extinction), vp, P(gf_rng),

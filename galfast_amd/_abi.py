@@ -127,9 +127,17 @@ class gf_extinction(C.Structure):
     _fields_ = [("north", gf_tex3d), ("south", gf_tex3d), ("beam", gf_tex3d)]
 
 
+GF_FOOT_MAX = 8
+
+
+class gf_foot_region(C.Structure):
+    _fields_ = [("kind", i32), ("equ", i32), ("l0", f64), ("l1", f64), ("b0", f64), ("b1", f64),
+                ("cl", f64), ("cb", f64), ("sl", f64), ("sb", f64), ("cosr", f64), ("hole", f64),
+                ("bmin", f64)]
+
+
 class gf_footprint(C.Structure):
-    _fields_ = [("kind", i32), ("pad", i32), ("l0", f64), ("l1", f64), ("b0", f64), ("b1", f64),
-                ("bmin", f64), ("cl", f64), ("cb", f64), ("sl", f64), ("sb", f64), ("cosr", f64)]
+    _fields_ = [("nregion", i32), ("pad", i32), ("r", gf_foot_region * GF_FOOT_MAX)]
 
 
 class gf_count_result(C.Structure):

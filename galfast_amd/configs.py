@@ -177,6 +177,22 @@ modules = feh.conf Bond2010.conf photometry.conf photoerr.conf vel2pm.conf gal2o
     return os.path.join(d, "cmd.conf")
 
 
+def set_footprints(cmd, bodies):
+    """Replace the footprints of the job in `cmd` (a cmd.conf path) by one
+    footprint file per entry of `bodies` (the file text after
+    "module = footprint"); the job's footprint is their union."""
+    d = os.path.dirname(cmd)
+    names = []
+    for k, body in enumerate(bodies):
+        names.append("foot%d.conf" % k)
+        _w(os.path.join(d, names[-1]), "module = footprint\n" + body)
+    with open(cmd) as f:
+        lines = [ln for ln in f.read().splitlines() if not ln.strip().startswith("footprints")]
+    lines.append("footprints = " + " ".join(names))
+    _w(cmd, "\n".join(lines))
+    return cmd
+
+
 def add_countsmap(cmd, keys):
     """Give the job in `cmd` (a cmd.conf path) a countsMap output stage with
     the given {key: value} (xy, z, x0, dx, equalarea, "coadd.<name>",

@@ -208,16 +208,6 @@ GF_D void mod_Bond2010(const gf_Bond2010_data &par, gf_mwc &rng, uint32_t r0, ui
 	}
 }
 
-// J2000 Galactic <-> equatorial constants (modules/module_lib.h:62-71)
-namespace galequ_c {
-static constexpr double angp = 0.01745329251994329576923691 * 192.859508333;
-static constexpr double l0 = 0.01745329251994329576923691 * 32.932;
-static constexpr double ce = 0.88998740217659689;
-static constexpr double se = 0.45598511375586859;
-static constexpr double pi2 = 6.28318530717958647692528677;
-static constexpr double d2r = 0.01745329251994329576923691;
-}
-
 // transform_pm in double (modules/vel2pm_gpu.cu.h:134-164), with float
 // in/out parameters narrowed where the reference narrows them.
 GF_D void transform_pm(float &vlout, float &vbout, float l, float b, float vl, float vb,
@@ -305,13 +295,7 @@ GF_D void mod_gal2other(int coordsys, uint32_t r0, uint32_t r1, const gf_col &hi
 		double ly = col<double>(lb0, row, 1) * d2r;
 		double rx, ry;
 		if (coordsys == 1) {
-			double cb, sb, cl, sl;
-			gfm::sincos(ly, &sb, &cb);
-			gfm::sincos(lx - l0, &sl, &cl);
-			rx = gfm::atan2(cb * cl, sb * ce - cb * se * sl) + angp;
-			ry = gfm::asin(cb * ce * sl + sb * se);
-			while (rx < 0.) rx += pi2;
-			while (rx >= pi2) rx -= pi2;
+			gal_to_equ(lx, ly, rx, ry);
 		} else {
 			rx = ry = -9999.;
 		}

@@ -215,26 +215,8 @@ namespace {
 // Sky pixelization: square pixels of side dx in the north (pole b = +90, l0 =
 // 90 deg) and south (b = -90, l0 = 270 deg) Lambert projections
 // (skygen/os_skygen.cpp:239-283, 940-995), covered fractions by 8x8 point
-// sampling of the footprint predicate.
+// sampling of the footprint (gf_footprint_cover on the device).
 // ---------------------------------------------------------------------------
-bool in_footprint(const gf_footprint &f, double l, double b)
-{
-	if (f.kind == 1) {
-		double ld = l * 180. / M_PI, bd = b * 180. / M_PI;
-		ld = fmod(ld, 360.); if (ld < 0) ld += 360.;
-		if (bd < f.b0 || bd > f.b1) return false;
-		if (f.bmin > 0. && fabs(bd) < f.bmin) return false;
-		if (f.l1 - f.l0 >= 360.) return true;
-		double t = fmod(ld - f.l0, 360.); if (t < 0) t += 360.;
-		return t <= f.l1 - f.l0;
-	}
-	if (f.kind == 2) {
-		double d = cos(b) * cos(l) * f.cb * f.cl + cos(b) * sin(l) * f.cb * f.sl + sin(b) * f.sb;
-		return d >= f.cosr;
-	}
-	return true;
-}
-
 void lambert_init(gf_lambert &p, double l0, double b0)
 {
 	p.l0 = l0; p.b0 = b0;
@@ -265,119 +247,136 @@ void proj_pole(int hemi, double l, double b, double &x, double &y)
 	}
 }
 
-// Pixels of side dx covering a footprint, in both hemispheres, on grids
-// centred on the poles.  For a beam only the grid cells around the beam are
-// examined.
-std::vector<gf_pencil_beam> pixelize_footprint(const gf_footprint &fp, double dx)
+int pixel_grid_n(double dx) { return (int)ceil(2 * sqrt(2.0) / dx); }
+
+// The pixel of grid cell (hemi, i, j) with `inside` of its 64 samples in the
+// footprint.
+gf_pencil_beam cell_pixel(int hemi, int n, int i, int j, double dx, int inside)
+{
+	double x0 = -n * dx / 2, xc = x0 + i * dx + 0.5 * dx, yc = x0 + j * dx + 0.5 * dx;
+	double l, b;
+	deproj_pole(hemi, xc, yc, l, b);
+	gf_pencil_beam pb{};
+	pb.cl = (float)cos(l); pb.cb = (float)cos(b); pb.sl = (float)sin(l); pb.sb = (float)sin(b);
+	pb.coveredFraction = (float)inside / 64.f;
+	pb.dx = (float)dx;
+	pb.dA = pb.dx * pb.dx;
+	pb.projIdx = hemi;
+	pb.X = (float)xc; pb.Y = (float)yc;
+	pb.extIdx = -1;
+	return pb;
+}
+
+// Pixels of a one-beam footprint (a server request): only the grid cells
+// around the beam are sampled, with the device kernel's beam test, so the
+// pixels equal those gf_footprint_cover gives for the same beam.
+std::vector<gf_pencil_beam> pixelize_beam(const gf_footprint &fp, double dx)
 {
 	std::vector<gf_pencil_beam> out;
-	const int sub = 8;
-	const double R = sqrt(2.0);
-	int n = (int)ceil(2 * R / dx);
-	double x0 = -n * dx / 2, y0 = -n * dx / 2;
+	if (fp.nregion != 1 || fp.r[0].kind != 2 || fp.r[0].bmin > 0. || fp.r[0].hole > 0.)
+		fail("pixelize_beam: not a single beam");
+	const gf_foot_region &f = fp.r[0];
+	const int n = pixel_grid_n(dx);
+	const double x0 = -n * dx / 2, y0 = -n * dx / 2;
+	const double bc[3] = {f.cb * f.cl, f.cb * f.sl, f.sb};
+	const double r = acos(std::max(-1.0, std::min(1.0, f.cosr)));
 	for (int hemi = 0; hemi < 2; hemi++) {
-		int i0 = 0, i1 = n, j0 = 0, j1 = n;
-		if (fp.kind == 2) {
-			// grid cells around the beam: project a polar sampling of the cap
-			double cl0 = fp.cl, sl0 = fp.sl, cb0 = fp.cb, sb0 = fp.sb;
-			double r = acos(std::max(-1.0, std::min(1.0, fp.cosr)));
-			double xmin = 1e9, xmax = -1e9, ymin = 1e9, ymax = -1e9;
-			bool any = false;
-			// unit vectors of the centre and two tangent directions
-			double c[3] = {cb0 * cl0, cb0 * sl0, sb0};
-			double e1[3] = {-sl0, cl0, 0};
-			double e2[3] = {-sb0 * cl0, -sb0 * sl0, cb0};
-			for (int ir = 0; ir <= 24; ir++) {
-				double rr = r * ir / 24.0;
-				int na = ir == 0 ? 1 : 96;
-				for (int ia = 0; ia < na; ia++) {
-					double a = 2 * M_PI * ia / na;
-					double v[3];
-					for (int q = 0; q < 3; q++) v[q] = cos(rr) * c[q] + sin(rr) * (cos(a) * e1[q] + sin(a) * e2[q]);
-					double b = asin(std::max(-1.0, std::min(1.0, v[2]))), l = atan2(v[1], v[0]);
-					if ((hemi == 0 && b < 0) || (hemi == 1 && b >= 0)) continue;
-					double x, y;
-					proj_pole(hemi, l, b, x, y);
-					xmin = std::min(xmin, x); xmax = std::max(xmax, x);
-					ymin = std::min(ymin, y); ymax = std::max(ymax, y);
-					any = true;
-				}
+		// grid cells around the beam: project a polar sampling of the cap
+		double xmin = 1e9, xmax = -1e9, ymin = 1e9, ymax = -1e9;
+		bool any = false;
+		double e1[3] = {-f.sl, f.cl, 0}, e2[3] = {-f.sb * f.cl, -f.sb * f.sl, f.cb};
+		for (int ir = 0; ir <= 24; ir++) {
+			double rr = r * ir / 24.0;
+			int na = ir == 0 ? 1 : 96;
+			for (int ia = 0; ia < na; ia++) {
+				double a = 2 * M_PI * ia / na, v[3];
+				for (int q = 0; q < 3; q++) v[q] = cos(rr) * bc[q] + sin(rr) * (cos(a) * e1[q] + sin(a) * e2[q]);
+				double b = asin(std::max(-1.0, std::min(1.0, v[2]))), l = atan2(v[1], v[0]);
+				if ((hemi == 0 && b < 0) || (hemi == 1 && b >= 0)) continue;
+				double x, y;
+				proj_pole(hemi, l, b, x, y);
+				xmin = std::min(xmin, x); xmax = std::max(xmax, x);
+				ymin = std::min(ymin, y); ymax = std::max(ymax, y);
+				any = true;
 			}
-			if (!any) continue;
-			i0 = std::max(0, (int)floor((xmin - x0) / dx) - 2);
-			i1 = std::min(n, (int)floor((xmax - x0) / dx) + 3);
-			j0 = std::max(0, (int)floor((ymin - y0) / dx) - 2);
-			j1 = std::min(n, (int)floor((ymax - y0) / dx) + 3);
 		}
-		// a beam's membership test straight from the Lambert-plane point:
-		// the unit vector of (x, y) is (-x s, -y s, 1 - r^2/2) in the north
-		// and (x s, -y s, r^2/2 - 1) in the south, s = sqrt(1 - r^2/4)
-		const double bc[3] = {fp.cb * fp.cl, fp.cb * fp.sl, fp.sb};
+		if (!any) continue;
+		const int i0 = std::max(0, (int)floor((xmin - x0) / dx) - 2), i1 = std::min(n, (int)floor((xmax - x0) / dx) + 3);
+		const int j0 = std::max(0, (int)floor((ymin - y0) / dx) - 2), j1 = std::min(n, (int)floor((ymax - y0) / dx) + 3);
 		auto in_beam = [&](double x, double y) {
 			double r2 = x * x + y * y;
 			if (r2 > 2.0) return false;
 			double s = sqrt(std::max(0.0, 1.0 - 0.25 * r2)), z = 1.0 - 0.5 * r2;
 			double v0 = hemi == 0 ? -x * s : x * s, v1 = -y * s, v2 = hemi == 0 ? z : -z;
 			if ((hemi == 0 && v2 < 0) || (hemi == 1 && v2 >= 0)) return false;
-			return v0 * bc[0] + v1 * bc[1] + v2 * bc[2] >= fp.cosr;
+			return v0 * bc[0] + v1 * bc[1] + v2 * bc[2] >= f.cosr;
 		};
-		for (int i = i0; i < i1; i++) {
+		for (int i = i0; i < i1; i++)
 			for (int j = j0; j < j1; j++) {
 				double xa = x0 + i * dx, ya = y0 + j * dx;
 				double fx = (xa > 0 || xa + dx < 0) ? std::min(fabs(xa), fabs(xa + dx)) : 0.;
 				double fy = (ya > 0 || ya + dx < 0) ? std::min(fabs(ya), fabs(ya + dx)) : 0.;
 				if (fx * fx + fy * fy > 2.0) continue;
 				int inside = 0;
-				if (fp.kind == 2 && in_beam(xa, ya) && in_beam(xa + dx, ya) && in_beam(xa, ya + dx) &&
-				    in_beam(xa + dx, ya + dx)) {
-					inside = sub * sub;   // a cell with all four corners in the cap is covered
-				} else {
-					for (int a = 0; a < sub; a++)
-						for (int b2 = 0; b2 < sub; b2++) {
-							double x = xa + dx * (a + 0.5) / sub, y = ya + dx * (b2 + 0.5) / sub;
-							if (x * x + y * y > 2.0) continue;
-							if (fp.kind == 2) {
-								if (in_beam(x, y)) inside++;
-								continue;
-							}
-							double l, b;
-							deproj_pole(hemi, x, y, l, b);
-							if ((hemi == 0 && b < 0) || (hemi == 1 && b >= 0)) continue;
-							if (in_footprint(fp, l, b)) inside++;
-						}
-				}
-				if (!inside) continue;
-				double xc = xa + 0.5 * dx, yc = ya + 0.5 * dx;
-				double l, b;
-				deproj_pole(hemi, xc, yc, l, b);
-				gf_pencil_beam pb{};
-				pb.cl = (float)cos(l); pb.cb = (float)cos(b); pb.sl = (float)sin(l); pb.sb = (float)sin(b);
-				pb.coveredFraction = (float)inside / (float)(sub * sub);
-				pb.dx = (float)dx;
-				pb.dA = pb.dx * pb.dx;
-				pb.projIdx = hemi;
-				pb.X = (float)xc; pb.Y = (float)yc;
-				pb.extIdx = -1;
-				out.push_back(pb);
+				for (int a = 0; a < 8; a++)
+					for (int b2 = 0; b2 < 8; b2++)
+						inside += in_beam(xa + dx * (a + 0.5) / 8, ya + dx * (b2 + 0.5) / 8) ? 1 : 0;
+				if (inside) out.push_back(cell_pixel(hemi, n, i, j, dx, inside));
 			}
-		}
 	}
 	return out;
 }
 
+// The catalog's pixels: cell coverage on the device, the covered cells in
+// (hemisphere, i, j) order.
 void pixelize(gf_catalog &c, double dx)
 {
-	c.pixels = pixelize_footprint(c.foot.f, dx);
+	const int n = pixel_grid_n(dx);
+	const size_t cells = 2 * (size_t)n * n;
+	DevBuf d;
+	d.alloc(cells);
+	if (gf_footprint_cover(&c.foot.f, dx, n, (uint8_t *)d.p, nullptr)) fail("gf_footprint_cover failed");
+	std::vector<uint8_t> cover(cells);
+	if (hipMemcpy(cover.data(), d.p, cells, hipMemcpyDeviceToHost) != hipSuccess) fail("footprint cover copy failed");
+	c.pixels.clear();
+	for (int hemi = 0; hemi < 2; hemi++)
+		for (int i = 0; i < n; i++)
+			for (int j = 0; j < n; j++) {
+				int inside = cover[((size_t)hemi * n + i) * n + j];
+				if (inside) c.pixels.push_back(cell_pixel(hemi, n, i, j, dx, inside));
+			}
 	c.sp.npixels = (int)c.pixels.size();
 }
 
-void load_footprint(gf_catalog &c, const Config &fc)
+// Equatorial (ra, dec) -> Galactic (l, b), radians: the inverse rotation of
+// galequ (modules/gal2other_gpu.cu.h:39-60).
+void equ_to_gal(double ra, double dec, double &l, double &b)
 {
-	std::string type = fc.get("type");
+	const double angp = M_PI / 180. * 192.859508333, l0 = M_PI / 180. * 32.932;
+	const double ce = 0.88998740217659689, se = 0.45598511375586859;
+	double A = ra - angp;
+	double X = cos(dec) * cos(A), Y = cos(dec) * sin(A), Z = sin(dec);
+	double u0 = Y, u1 = -se * X + ce * Z, u2 = ce * X + se * Z;
+	b = asin(std::max(-1.0, std::min(1.0, u2)));
+	l = atan2(u1, u0) + l0;
+	l = fmod(l, 2 * M_PI);
+	if (l < 0) l += 2 * M_PI;
+}
+
+// One footprint configuration file -> one region (load_footprint,
+// footprint.cpp:729-744).
+gf_foot_region load_region(const Config &fc)
+{
+	std::string type = fc.get("type", "");
+	if (type.empty()) fail("The footprint configuration file must contain the 'type' keyword.");
 	std::string cs = fc.get("coordsys", "gal");
-	if (cs != "gal") fail("footprint coordsys '" + cs + "' is not supported (gal only)");
-	gf_footprint &f = c.foot.f;
+	if (cs != "gal" && cs != "equ")
+		fail("Don't know how to convert from coordinate system '" + cs + "' to galactic coordinates.");
+	gf_foot_region f{};
+	f.equ = cs == "equ";
 	if (type == "rect") {
+		// footprint.cpp:486-515: longitudes reduced to [0, 360), l0 >= l1
+		// wraps (l0 = l1 = 0 is the whole ring)
 		std::vector<double> r = fc.getv("rect", "0 1 0 1");
 		if (r.size() < 4) fail("rect needs l0 l1 b0 b1");
 		double l0 = fmod(r[0], 360.), l1 = fmod(r[1], 360.);
@@ -387,15 +386,30 @@ void load_footprint(gf_catalog &c, const Config &fc)
 		f.kind = 1;
 		f.l0 = l0; f.l1 = l1; f.b0 = r[2]; f.b1 = r[3];
 	} else if (type == "beam") {
+		// footprint.cpp:675-700: centre l b, radius r, optional hole radius
+		if (!fc.count("footprint_beam")) fail("Configuration key footprint_beam must be set");
 		std::vector<double> v = fc.getv("footprint_beam", "");
 		if (v.size() < 3) fail("footprint_beam needs l b r");
-		if (v.size() > 3 && v[3] != 0.) fail("beams with holes are not supported");
 		double l = v[0] * M_PI / 180., b = v[1] * M_PI / 180.;
+		if (f.equ) equ_to_gal(l, b, l, b);
 		f.kind = 2;
+		f.equ = 0;
 		f.cl = cos(l); f.cb = cos(b); f.sl = sin(l); f.sb = sin(b);
 		f.cosr = cos(v[2] * M_PI / 180.);
-	} else fail("unknown footprint type " + type);
+		double rhole = v.size() > 3 ? v[3] : 0.;
+		f.hole = rhole > 0. ? 1. - cos(rhole * M_PI / 180.) : 0.;
+	} else fail("The requested footprint type " + type + " is unknown.");
 	f.bmin = fc.getd("bmin", 0.);
+	return f;
+}
+
+void load_footprints(gf_catalog &c)
+{
+	gf_footprint &f = c.foot.f;
+	f = gf_footprint{};
+	if (c.foot_cfgs.size() > GF_FOOT_MAX)
+		fail("at most " + std::to_string(GF_FOOT_MAX) + " footprint configurations are supported");
+	for (auto &fc : c.foot_cfgs) f.r[f.nregion++] = load_region(fc);
 }
 
 // ---------------------------------------------------------------------------
@@ -888,12 +902,12 @@ void open_catalog(gf_catalog &c, const std::string &cmdfile)
 	// the output stages are module configurations too (galfast.cpp:227-233)
 	for (auto &f : split_ws(cmd.get("output", ""))) c.module_cfgs.push_back(loadc(f));
 	c.nstars_forced = cmd.getd("nstars", 0.);
-	if (c.foot_cfgs.size() != 1) fail("exactly one footprint configuration is supported");
+	if (c.foot_cfgs.empty()) fail("no footprint configuration given");
 	for (auto &mc : c.model_cfgs) mc.globals = &c.globals;
 
 	double dx;
 	load_sky_pixelization(c, dx);
-	load_footprint(c, c.foot_cfgs[0]);
+	load_footprints(c);
 	pixelize(c, dx);
 	load_extinction(c, c.skygen.get("extmaps", ""));
 	for (auto &mc : c.model_cfgs) load_model(c, mc);
@@ -1660,12 +1674,15 @@ namespace {
 
 void prepare(gf_server &s, SReq &q)
 {
-	gf_footprint &f = q.foot;
+	gf_footprint &fp = q.foot;
+	fp = gf_footprint{};
+	fp.nregion = 1;
+	gf_foot_region &f = fp.r[0];
 	double l = q.r.l * M_PI / 180., b = q.r.b * M_PI / 180.;
 	f.kind = 2;
 	f.cl = cos(l); f.cb = cos(b); f.sl = sin(l); f.sb = sin(b);
 	f.cosr = cos(q.r.radius * M_PI / 180.);
-	q.pix = pixelize_footprint(f, s.dx);
+	q.pix = pixelize_beam(fp, s.dx);
 	q.sp = s.base.sp;
 	q.sp.m1 = q.r.m1;
 	q.sp.nm = (int)round((q.sp.m1 - s.m0raw) / q.sp.dm);

@@ -76,6 +76,62 @@ __global__ void __launch_bounds__(OB) k_clip(gf_skygen_params sp, gf_footprint f
 	if (outside_footprint(foot, l, b)) col<int>(hidden, r) = 1;
 }
 
+// Sample point (x, y) of hemisphere `hemi`'s Lambert plane inside the
+// footprint?  Beams are tested on the point's unit vector, which the
+// equal-area projection gives without transcendentals:
+// (-x s, -y s, 1 - r^2/2) in the north, (x s, -y s, r^2/2 - 1) in the south,
+// s = sqrt(1 - r^2/4); rects and the zone of avoidance on the deprojected
+// (l, b).
+__device__ bool sample_inside(const gf_footprint &foot, int hemi, double x, double y)
+{
+	double r2 = x * x + y * y;
+	if (r2 > 2.0) return false;
+	double s = sqrt(fmax(0.0, 1.0 - 0.25 * r2)), z = 1.0 - 0.5 * r2;
+	double v0 = hemi == 0 ? -x * s : x * s, v1 = -y * s, v2 = hemi == 0 ? z : -z;
+	if ((hemi == 0 && v2 < 0) || (hemi == 1 && v2 >= 0)) return false;
+	if (foot.nregion <= 0) return true;
+	double l = 0., b = 0.;
+	bool have_lb = false;
+	for (int k = 0; k < foot.nregion && k < GF_FOOT_MAX; k++) {
+		const gf_foot_region &f = foot.r[k];
+		if (f.kind == 2 && f.bmin <= 0.) {
+			double d = v0 * (f.cb * f.cl) + v1 * (f.cb * f.sl) + v2 * f.sb;
+			if (d >= f.cosr && !(f.hole > 0. && 1. - d < f.hole)) return true;
+			continue;
+		}
+		if (!have_lb) { pole_deproject(hemi, x, y, l, b); have_lb = true; }
+		if (f.kind == 2) {
+			if (fabs(b * DBL_R2D) < f.bmin) continue;
+			double d = v0 * (f.cb * f.cl) + v1 * (f.cb * f.sl) + v2 * f.sb;
+			if (d >= f.cosr && !(f.hole > 0. && 1. - d < f.hole)) return true;
+			continue;
+		}
+		if (!outside_region(f, l, b)) return true;
+	}
+	return false;
+}
+
+// One thread per pixel-grid cell: the number of its 8 x 8 sample points
+// inside the footprint (os_skygen.cpp:940-995 clips polygons instead).
+__global__ void __launch_bounds__(OB) k_footprint_cover(gf_footprint foot, double dx, int n, uint8_t *cover)
+{
+	const uint64_t t = (uint64_t)blockIdx.x * OB + threadIdx.x;
+	const uint64_t nn = (uint64_t)n * n;
+	if (t >= 2 * nn) return;
+	const int hemi = (int)(t / nn);
+	const int i = (int)((t % nn) / n), j = (int)(t % n);
+	const double x0 = -n * dx / 2, xa = x0 + i * dx, ya = x0 + j * dx;
+	const double fx = (xa > 0 || xa + dx < 0) ? fmin(fabs(xa), fabs(xa + dx)) : 0.;
+	const double fy = (ya > 0 || ya + dx < 0) ? fmin(fabs(ya), fabs(ya + dx)) : 0.;
+	int inside = 0;
+	if (fx * fx + fy * fy <= 2.0) {
+		for (int a = 0; a < 8; a++)
+			for (int c = 0; c < 8; c++)
+				inside += sample_inside(foot, hemi, xa + dx * (a + 0.5) / 8, ya + dx * (c + 0.5) / 8) ? 1 : 0;
+	}
+	cover[t] = (uint8_t)inside;
+}
+
 int gather(const gf_col &src, const gf_col &dst, const uint32_t *idx, uint64_t n, hipStream_t st)
 {
 	GF_CHECK(src.width == dst.width && src.elem == dst.elem, GF_E_ARG);
@@ -163,8 +219,19 @@ int gf_clip_footprint(const gf_skygen_params *sp, const gf_footprint *foot, gf_c
                       gf_col hidden, uint64_t n, void *stream)
 {
 	GF_CHECK(sp && foot && n < (1ull << 32), GF_E_ARG);
-	if (n == 0 || foot->kind == 0) return 0;
+	GF_CHECK(foot->nregion >= 0 && foot->nregion <= GF_FOOT_MAX, GF_E_ARG);
+	if (n == 0 || foot->nregion == 0) return 0;
 	k_clip<<<gf::blocks_for(n, OB), OB, 0, gf::as_stream(stream)>>>(*sp, *foot, projIdx, projXY, hidden, n);
+	GF_LAUNCHED();
+	return 0;
+}
+
+int gf_footprint_cover(const gf_footprint *foot, double dx, int32_t n, uint8_t *cover, void *stream)
+{
+	GF_CHECK(foot && cover && dx > 0. && n > 0 && n <= 65536, GF_E_ARG);
+	GF_CHECK(foot->nregion >= 0 && foot->nregion <= GF_FOOT_MAX, GF_E_ARG);
+	const uint64_t cells = 2ull * (uint64_t)n * (uint64_t)n;
+	k_footprint_cover<<<gf::blocks_for(cells, OB), OB, 0, gf::as_stream(stream)>>>(*foot, dx, n, cover);
 	GF_LAUNCHED();
 	return 0;
 }

@@ -172,7 +172,8 @@ __global__ void __launch_bounds__(SB) k_b_emit_rows(const gf_serve_seg *segs, in
 		return;
 	}
 	gf_extinction ext{};
-	write_star(sg.sp, comp, ext, sg.foot, pix, v.x, v.y, v.z, v.w, 0.f, out, r);
+	// a request's footprint is one Galactic beam
+	write_star<false>(sg.sp, comp, ext, sg.foot, pix, v.x, v.y, v.z, v.w, 0.f, out, r);
 }
 
 // Row starts of the segments from the scanned offsets.

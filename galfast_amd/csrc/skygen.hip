@@ -497,7 +497,9 @@ __global__ void __launch_bounds__(SK_BS) k_emit_rng(gf_skygen_params sp, gf_mode
 	if (touched) gf_mwc_store(st, ns, s, rng);
 }
 
-// Deterministic half of the emit, one lane per row.
+// Deterministic half of the emit, one lane per row.  FULL: the footprint
+// test handles unions and equatorial rects.
+template <bool FULL>
 __global__ void __launch_bounds__(SK_BS) k_emit_rows(gf_skygen_params sp, int comp, const gf_pencil_beam *pixels,
                                                      gf_extinction ext, gf_footprint foot, uint32_t nrows,
                                                      gf_parked park, gf_skygen_cols out)
@@ -511,7 +513,7 @@ __global__ void __launch_bounds__(SK_BS) k_emit_rows(gf_skygen_params sp, int co
 		write_star_outside(out, row, pix, comp, v.x, v.y);
 		return;
 	}
-	write_star(sp, comp, ext, foot, pix, v.x, v.y, v.z, v.w, 0.f, out, row);
+	write_star<FULL>(sp, comp, ext, foot, pix, v.x, v.y, v.z, v.w, 0.f, out, row);
 }
 
 __global__ void k_find_unit(const uint64_t *offs, uint64_t U, uint64_t row, uint64_t *res)
@@ -747,8 +749,12 @@ int skygen_emit(const gf_draw_plan *plan, uint64_t row_begin, uint64_t row_end, 
 	    rng->state, rng->nstreams, park);
 	GF_LAUNCHED();
 	if (ev) GF_HIP(hipEventRecord(ev[1], st));
-	k_emit_rows<<<gf::blocks_for(n, SK_BS), SK_BS, 0, st>>>(plan->sp, plan->model.comp, plan->pixels, plan->ext, fp,
-	                                                          (uint32_t)n, park, *out);
+	if (footprint_is_simple(fp))
+		k_emit_rows<false><<<gf::blocks_for(n, SK_BS), SK_BS, 0, st>>>(plan->sp, plan->model.comp, plan->pixels,
+		                                                                 plan->ext, fp, (uint32_t)n, park, *out);
+	else
+		k_emit_rows<true><<<gf::blocks_for(n, SK_BS), SK_BS, 0, st>>>(plan->sp, plan->model.comp, plan->pixels,
+		                                                                plan->ext, fp, (uint32_t)n, park, *out);
 	GF_LAUNCHED();
 	if (ev) {
 		GF_HIP(hipEventRecord(ev[2], st));

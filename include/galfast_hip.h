@@ -366,14 +366,44 @@ typedef struct gf_extinction {
 } gf_extinction;
 
 /* Footprint predicate used to hide stars drawn outside the requested area
- * (the clipper, skygen/os_skygen.cpp:999-1044).  kind: 0 none, 1 rect in
- * Galactic l/b (degrees, with |b| >= bmin), 2 beam (centre l, b, radius). */
-typedef struct gf_footprint {
+ * (the clipper, skygen/os_skygen.cpp:999-1044) and to pixelize the sky.
+ * A footprint is the union of `nregion` regions, one per footprint
+ * configuration file (load_footprints, footprint.cpp:746-759); nregion 0
+ * means the whole sky.  Each region is what footprint.cpp builds for one
+ * file:
+ *   kind 1 rect (load_footprint_rect, footprint.cpp:486-560): lon l0..l1,
+ *          lat b0..b1 in degrees, in Galactic (equ = 0) or J2000
+ *          equatorial (equ = 1) coordinates; l0 < l1, l1 - l0 <= 360;
+ *   kind 2 beam (load_footprint_beam, footprint.cpp:675-700): Galactic
+ *          direction cosines of the centre, cos(radius), and
+ *          hole = 1 - cos(hole radius) (0 = no hole);
+ * minus the zone of avoidance |b| < bmin degrees, Galactic
+ * (clip_zone_of_avoidance, footprint.cpp:704-727). */
+#define GF_FOOT_MAX 8
+typedef struct gf_foot_region {
     int32_t kind;
-    int32_t pad;
-    double  l0, l1, b0, b1, bmin;   /* rect, degrees */
-    double  cl, cb, sl, sb, cosr;   /* beam centre direction and cos(radius) */
+    int32_t equ;
+    double  l0, l1, b0, b1;         /* rect, degrees */
+    double  cl, cb, sl, sb;         /* beam centre direction */
+    double  cosr, hole;             /* beam cos(radius), 1 - cos(hole radius) */
+    double  bmin;                   /* |b| < bmin removed, degrees */
+} gf_foot_region;
+
+typedef struct gf_footprint {
+    int32_t        nregion;
+    int32_t        pad;
+    gf_foot_region r[GF_FOOT_MAX];
 } gf_footprint;
+
+/* Sky pixelization (os_skygen::load_footprints, skygen/os_skygen.cpp:
+ * 235-283, 940-995): square pixels of side dx [radians of the Lambert
+ * plane] on a grid of n x n cells centred on each pole (north: pole
+ * b = +90 deg, l0 = 90 deg; south: b = -90 deg, l0 = 270 deg),
+ * n = ceil(2 sqrt(2) / dx).  Cell (h, i, j) spans x in [-n dx/2 + i dx, +dx),
+ * y likewise with j; its coverage is the number of 8 x 8 sample points
+ * (cell-centred) that lie in its own hemisphere and inside `foot`.
+ * Writes cover[(h n + i) n + j] (0..64) for the 2 n^2 cells. */
+int gf_footprint_cover(const gf_footprint *foot, double dx, int32_t n, uint8_t *cover, void *stream);
 
 /* Count-pass results (skygen/skyconfig_impl.h:112-184). */
 typedef struct gf_count_result {
@@ -428,7 +458,7 @@ int gf_skygen_order(const gf_skygen_cols *in, uint64_t n, const gf_skygen_cols *
                     uint64_t *nvisible, void *stream);
 
 /* The clipper stage (os_clipper::process, skygen/os_skygen.cpp:997-1044)
- * for the analytic footprints (rect, beam): a visible row of [0, n) whose
+ * for the analytic footprints (unions of rects and beams): a visible row of [0, n) whose
  * projection-plane position deprojects outside `foot` gets hidden = 1. */
 int gf_clip_footprint(const gf_skygen_params *sp, const gf_footprint *foot, gf_col projIdx,
                       gf_col projXY, gf_col hidden, uint64_t n, void *stream);

@@ -27,6 +27,10 @@
  *   errors                             test also rejecting a planted mistake
  *                                      (tests/test_oracle_modules.py)
  *   photometry ....................... closed form on polynomial colour tables
+ *   footprint regions, coverage ...... textbook Galactic pole / centre in
+ *                                      equatorial coordinates, cap, ring and
+ *                                      band areas in closed form, union
+ *                                      additivity (tests/test_oracle_footprint.py)
  *   countsMap binning ................ known bins of bin centres and edges,
  *                                      closed-form polar Lambert radius and
  *                                      azimuth, coadd monotonicity, numpy
@@ -440,14 +444,24 @@ static void deproject(const gf_lambert &p, double &l, double &b, float x, float 
 
 GFO_API void gfo_deproject(const gf_lambert *p, float x, float y, double *l, double *b) { deproject(*p, *l, *b, x, y); }
 
-static int outside_foot(const gf_footprint &f, double l, double b)
+GFO_API void gfo_galequ(double l, double b, double *ra, double *dec);
+
+/* One footprint region as footprint.cpp builds it from one configuration
+ * file: rect in Galactic or equatorial lon/lat (load_footprint_rect,
+ * footprint.cpp:486-560), beam with optional hole (load_footprint_beam,
+ * :675-700), minus the zone of avoidance |b| < bmin (:704-727).  The
+ * reference clips gpc polygons sampled along these boundaries; the oracle
+ * tests the boundaries themselves.  1 if (l, b) [radians] is outside. */
+static int outside_region(const gf_foot_region &f, double l, double b)
 {
+	if (f.bmin > 0. && fabs(b * R2D) < f.bmin) return 1;
 	if (f.kind == 1) {
-		double ld = l * R2D, bd = b * R2D;
+		double lon = l, lat = b;
+		if (f.equ) gfo_galequ(l, b, &lon, &lat);
+		double ld = lon * R2D, bd = lat * R2D;
 		while (ld < 0.) ld += 360.;
 		while (ld >= 360.) ld -= 360.;
 		if (bd < f.b0 || bd > f.b1) return 1;
-		if (f.bmin > 0. && fabs(bd) < f.bmin) return 1;
 		if (f.l1 - f.l0 >= 360.) return 0;
 		double t = ld - f.l0;
 		while (t < 0.) t += 360.;
@@ -459,9 +473,69 @@ static int outside_foot(const gf_footprint &f, double l, double b)
 		gfo_sincos(b, &sb, &cb);
 		gfo_sincos(l, &sl, &cl);
 		double d = cb * cl * f.cb * f.cl + cb * sl * f.cb * f.sl + sb * f.sb;
-		return d < f.cosr ? 1 : 0;
+		if (d < f.cosr) return 1;
+		return (f.hole > 0. && 1. - d < f.hole) ? 1 : 0;
 	}
-	return 0;
+	return 1;
+}
+
+/* The union of the footprint files (load_footprints, footprint.cpp:746-759). */
+static int outside_foot(const gf_footprint &f, double l, double b)
+{
+	if (f.nregion <= 0) return 0;
+	for (int k = 0; k < f.nregion && k < GF_FOOT_MAX; k++)
+		if (!outside_region(f.r[k], l, b)) return 0;
+	return 1;
+}
+
+GFO_API int gfo_outside_footprint(const gf_footprint *f, double l, double b) { return outside_foot(*f, l, b); }
+
+/* Pixel-grid coverage (os_skygen.cpp:235-283, 940-995 pixelizes the clipped
+ * polygons; here each cell of the pole-centred Lambert grid counts its 8 x 8
+ * cell-centred sample points inside the footprint).  cover[(h n + i) n + j]. */
+GFO_API void gfo_footprint_cover(const gf_footprint *foot, double dx, int n, uint8_t *cover)
+{
+	const double PI = 3.14159265358979323846;
+	for (int h = 0; h < 2; h++)
+		for (int i = 0; i < n; i++)
+			for (int j = 0; j < n; j++) {
+				const double x0 = -n * dx / 2, xa = x0 + i * dx, ya = x0 + j * dx;
+				const double fx = (xa > 0 || xa + dx < 0) ? std::min(fabs(xa), fabs(xa + dx)) : 0.;
+				const double fy = (ya > 0 || ya + dx < 0) ? std::min(fabs(ya), fabs(ya + dx)) : 0.;
+				int inside = 0;
+				if (fx * fx + fy * fy <= 2.0)
+					for (int a = 0; a < 8; a++)
+						for (int c = 0; c < 8; c++) {
+							double x = xa + dx * (a + 0.5) / 8, y = ya + dx * (c + 0.5) / 8;
+							double r2 = x * x + y * y;
+							if (r2 > 2.0) continue;
+							/* hemisphere of the sample: z = 1 - r^2/2 north, -z south */
+							double z = 1.0 - 0.5 * r2;
+							if ((h == 0 && z < 0) || (h == 1 && -z >= 0)) continue;
+							double sb = std::min(1.0, std::max(-1.0, z)), l, b;
+							if (h == 0) { b = gfo_asin(sb); l = 0.5 * PI + gfo_atan2(x, -y); }
+							else { b = -gfo_asin(sb); l = 1.5 * PI + gfo_atan2(x, y); }
+							while (l < 0.) l += 2 * PI;
+							while (l >= 2 * PI) l -= 2 * PI;
+							if (foot->nregion <= 0) { inside++; continue; }
+							bool in = false;
+							for (int k = 0; k < foot->nregion && k < GF_FOOT_MAX && !in; k++) {
+								const gf_foot_region &f = foot->r[k];
+								if (f.kind == 2) {
+									/* unit vector of the sample point straight from the plane */
+									double s = sqrt(std::max(0.0, 1.0 - 0.25 * r2));
+									double v0 = h == 0 ? -x * s : x * s, v1 = -y * s, v2 = h == 0 ? z : -z;
+									if (f.bmin > 0. && fabs(b * R2D) < f.bmin) continue;
+									double d = v0 * (f.cb * f.cl) + v1 * (f.cb * f.sl) + v2 * f.sb;
+									in = d >= f.cosr && !(f.hole > 0. && 1. - d < f.hole);
+								} else {
+									in = !outside_region(f, l, b);
+								}
+							}
+							inside += in ? 1 : 0;
+						}
+				cover[((size_t)h * n + i) * n + j] = (uint8_t)inside;
+			}
 }
 
 /* compute_pos (skygen/skygen.cu.h:216-229) */
@@ -790,7 +864,7 @@ GFO_API uint64_t gfo_order(const gf_skygen_cols *inp, uint64_t n, const gf_skyge
  * position deprojects outside the footprint is hidden. */
 GFO_API void gfo_clip(const gf_skygen_params *sp, const gf_footprint *foot, const gf_skygen_cols *c, uint64_t n)
 {
-	if (foot->kind == 0) return;
+	if (foot->nregion <= 0) return;
 	for (uint64_t r = 0; r < n; r++) {
 		if (C<int>(c->hidden, r)) continue;
 		double l, b;

@@ -247,10 +247,32 @@ def pixelize_rect(l0, l1, b0, b1, dx_deg, sub=8):
     return arr
 
 
-def rect_footprint(l0, l1, b0, b1, bmin=0.0):
+def rect_footprint(l0, l1, b0, b1, bmin=0.0, equ=0):
     f = A.gf_footprint()
-    f.kind = 1
-    f.l0, f.l1, f.b0, f.b1, f.bmin = l0, l1, b0, b1, bmin
+    f.nregion = 1
+    r = f.r[0]
+    r.kind, r.equ = 1, equ
+    r.l0, r.l1, r.b0, r.b1, r.bmin = l0, l1, b0, b1, bmin
+    return f
+
+
+def beam_region(l, b, radius, hole=0.0, bmin=0.0):
+    """A beam region (Galactic centre and radii in degrees)."""
+    r = A.gf_foot_region()
+    lr, br = np.radians(l), np.radians(b)
+    r.kind = 2
+    r.cl, r.cb, r.sl, r.sb = np.cos(lr), np.cos(br), np.sin(lr), np.sin(br)
+    r.cosr = np.cos(np.radians(radius))
+    r.hole = 1.0 - np.cos(np.radians(hole)) if hole > 0 else 0.0
+    r.bmin = bmin
+    return r
+
+
+def footprint(*regions):
+    f = A.gf_footprint()
+    f.nregion = len(regions)
+    for k, r in enumerate(regions):
+        f.r[k] = r
     return f
 
 

@@ -50,6 +50,8 @@ def lib():
         _o.gfo_module_chain.restype = C.c_int
         _o.gfo_module_chain.argtypes = [P(A.gf_rows), P(A.gf_chain), vp, C.c_uint32, P(A.gf_table)]
         _o.gfo_galequ.argtypes = [C.c_double, C.c_double, P(C.c_double), P(C.c_double)]
+        _o.gfo_outside_footprint.argtypes = [P(A.gf_footprint), C.c_double, C.c_double]
+        _o.gfo_footprint_cover.argtypes = [P(A.gf_footprint), C.c_double, C.c_int, C.c_void_p]
         _o.gfo_vcyl2pm.argtypes = [P(A.gf_vel2pm_data)] + [C.c_float] * 8 + [vp]
         _o.gfo_deproject.argtypes = [P(A.gf_lambert), C.c_float, C.c_float, P(C.c_double), P(C.c_double)]
         _o.gfo_resample_clip.argtypes = [vp, vp, C.c_int, C.c_float, C.c_int, vp, vp]

@@ -383,6 +383,9 @@ gf_foot_region load_region(const Config &fc)
 		if (l0 < 0) l0 += 360.;
 		if (l1 < 0) l1 += 360.;
 		if (l0 >= l1) { if (l0 == 0) l1 = 360.; else l0 -= 360.; }
+		// within 1e-5 of a full turn the reference builds the whole ring
+		// (footprint.cpp:515-521)
+		if (fabs((l1 - l0) / 360. - 1.) < 1e-5) { l0 = 0.; l1 = 360.; }
 		f.kind = 1;
 		f.l0 = l0; f.l1 = l1; f.b0 = r[2]; f.b1 = r[3];
 	} else if (type == "beam") {

@@ -117,3 +117,37 @@ def test_union_of_disjoint_beams_expectation_adds(tmp_path):
     (eu, nu), (ea, na), (eb, nb) = exp
     assert nu == na + nb and na > 0 and nb > 0
     assert abs(eu - (ea + eb)) <= 1e-9 * eu
+
+
+@pytest.mark.parametrize("body,msg", [
+    ("type = rect\ncoordsys = ecl\nrect = 0 10 0 10", "coordinate system 'ecl'"),
+    ("coordsys = gal\nrect = 0 10 0 10", "'type' keyword"),
+    ("type = polygon", "footprint type polygon is unknown"),
+    ("type = beam\ncoordsys = gal", "footprint_beam must be set"),
+])
+def test_footprint_configuration_errors(tmp_path, body, msg):
+    """The reference's footprint.cpp errors (:493-494, :680-689, :729-743)."""
+    cmd = configs.set_footprints(configs.write_c0(str(tmp_path)), [body])
+    with pytest.raises(G.GalfastError) as e:
+        Catalog(cmd, seed=1)
+    assert msg in str(e.value)
+
+
+def test_equatorial_beam_is_the_galactic_beam(tmp_path):
+    """A beam centred on the north Galactic pole given in equatorial
+    coordinates (ra 192.859508, dec 27.128336, J2000) covers the pixels of
+    the beam at b = 90 given in Galactic ones; the ring tolerance turns a
+    rect 1e-6 short of a full turn into the full ring."""
+    res = []
+    for k, body in enumerate(["type = beam\ncoordsys = equ\nfootprint_beam = 192.859508 27.128336 6",
+                              "type = beam\ncoordsys = gal\nfootprint_beam = 0 90 6",
+                              "type = rect\ncoordsys = gal\nrect = 0 359.999999 60 80",
+                              "type = rect\ncoordsys = gal\nrect = 0 360 60 80"]):
+        cmd = configs.set_footprints(configs.write_c0(str(tmp_path / ("j%d" % k))), [body])
+        cat = Catalog(cmd, seed=2)
+        res.append((cat.count(), cat.info().npixels, cat.info().footprint_covered_sr))
+        cat.close()
+    (e0, n0, a0), (e1, n1, a1), (e2, n2, a2), (e3, n3, a3) = res
+    assert n0 == n1 and abs(a0 - a1) <= 2e-4 * a1 and abs(e0 - e1) <= 2e-4 * e1
+    assert abs(a1 / (2 * math.pi * (1 - math.cos(6 * D2R))) - 1) < 5e-3
+    assert (e2, n2, a2) == (e3, n3, a3)

@@ -325,6 +325,36 @@ def serving(args):
     return out
 
 
+def footprint_cover_bench(reps=20):
+    """Sky-grid coverage of the config-1 footprint (|b| >= 10 deg, 0.5 deg
+    cells, both pole grids): gf_footprint_cover, wall clock over `reps`
+    launches on the null stream."""
+    import ctypes as C
+    import math
+    import galfast_amd as G
+    from galfast_amd import _abi as A
+    L = G.lib()
+    f = A.gf_footprint()
+    f.nregion = 1
+    r = f.r[0]
+    r.kind, r.l0, r.l1, r.b0, r.b1, r.bmin = 1, 0.0, 360.0, -90.0, 90.0, 10.0
+    dx = 0.5 * math.pi / 180
+    n = int(math.ceil(2 * math.sqrt(2.0) / dx))
+    buf = C.c_void_p()
+    G.check(L.gf_malloc(C.byref(buf), 2 * n * n), "gf_malloc")
+    try:
+        G.check(L.gf_footprint_cover(C.byref(f), dx, n, buf, None), "gf_footprint_cover")
+        G.check(L.gf_sync(None), "gf_sync")
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            G.check(L.gf_footprint_cover(C.byref(f), dx, n, buf, None), "gf_footprint_cover")
+        G.check(L.gf_sync(None), "gf_sync")
+        dt = (time.perf_counter() - t0) / reps
+    finally:
+        L.gf_free(buf)
+    return {"cells": 2 * n * n, "samples": 128 * n * n, "ms": 1e3 * dt}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -477,6 +507,8 @@ def main():
                       "algorithmic_bytes_per_launch": bytes_total / max(n, 1)} if name else None),
         "valu_issue": valu_issue,
     }
+    if args.workload == "c1":
+        out["footprint_cover"] = footprint_cover_bench()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cat, args.workload, args.cpu_budget, args.cpu_threads)
     cat.close()

@@ -81,6 +81,10 @@ def _declare(L):
     L.gf_skygen_plan_free.argtypes = [vp]
     for name in ("gf_init", "gf_sync", "gf_malloc", "gf_free"):
         getattr(L, name).restype = C.c_int
+    L.gf_init.argtypes = [C.c_int]
+    L.gf_sync.argtypes = [vp]
+    L.gf_malloc.argtypes = [P(vp), C.c_size_t]
+    L.gf_free.argtypes = [vp]
 
 
 def check(rc, what="galfast call"):

@@ -67,22 +67,62 @@ STAGE_NAMES = ["k_chain_draws", "k_chain_rows<FEH>", "k_chain_rows<BOND>", "k_ch
                "k_chain_rows<PERR>", "k_chain_rows<ASTRO>"]
 
 
+def host_cpu_info():
+    """The host's processor model (/proc/cpuinfo), logical CPU count (nproc),
+    the CPUs this process may run on (affinity) and the cgroup's CPU quota
+    (cpu.max, in CPUs; None when unlimited or unreadable)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+            if q != "max":
+                quota = float(q) / float(per)
+    except (OSError, ValueError):
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    return {"model": model, "nproc": os.cpu_count(), "affinity": aff, "cgroup_cpus": quota}
+
+
+def default_cpu_threads():
+    """Every core this process can use: the affinity set, capped by the
+    cgroup's CPU quota."""
+    info = host_cpu_info()
+    n = info["affinity"]
+    if info["cgroup_cpus"]:
+        n = min(n, max(1, int(info["cgroup_cpus"])))
+    return max(1, n)
+
+
 def cpu_baseline(cat, workload, budget_s, threads):
     """The oracle (a single-threaded C++ restatement of the reference's CPU
-    path) timed on the host: once on one thread over a block of pixels, and
-    on `threads` threads, each replaying its own block of pixels (the oracle
-    calls release the GIL, so the blocks run in parallel)."""
+    path) timed on the host: once on one thread, and on `threads` threads
+    at once (the oracle calls release the GIL, so they run in parallel).
+    Each thread replays its own sample of pixels drawn at random (fixed
+    seed) from the whole sky, so the sample's star mix is the job's."""
     from concurrent.futures import ThreadPoolExecutor
+    import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import galfast_amd as G
     import replay as R
     view = R.host_view(G.lib(), cat._h)
     npix = view.sp.npixels
     block = max(1, min(npix // max(threads, 1), 400))
-    start = npix // 4
+    perm = np.random.default_rng(12345).permutation(npix)
 
     def run(i, budget):
-        idx = list(range(start + i * block, min(npix, start + (i + 1) * block)))
+        idx = sorted(int(v) for v in perm[i * block:(i + 1) * block])
         return R.replay(view, chunk_rows=30_000_000, pixel_index=idx, time_budget=budget), len(idx)
 
     one, n1 = run(0, budget_s)
@@ -92,9 +132,13 @@ def cpu_baseline(cat, workload, budget_s, threads):
         res = list(ex.map(lambda i: run(i, budget_s), range(threads)))
     wall = time.perf_counter() - t0
     vis = sum(r["visible"] for r, _ in res)
+    host = host_cpu_info()
     return {"value": vis / wall, "unit": "stars/s", "cores": threads, "kind": "port",
-            "sample": "%d threads, each replaying its own block of %d of %d pixels on the oracle (%d visible stars "
-                      "in %.1f s wall)" % (threads, block, npix, vis, wall),
+            "host": host,
+            "sample": "%d threads, each replaying %d pixels drawn at random from the %d of the sky on the oracle "
+                      "(%d visible stars in %.1f s wall); host: %s, nproc %s, %s CPUs in the affinity set, "
+                      "cgroup quota %s CPUs" % (threads, block, npix, vis, wall, host["model"], host["nproc"],
+                                                host["affinity"], host["cgroup_cpus"]),
             "single_thread": {"value": rate1, "cores": 1,
                               "sample": "%d pixels (%d visible stars, %.1f s)" % (n1, one["visible"],
                                                                                     one["seconds"])}}
@@ -140,19 +184,23 @@ def _serve_once(conf, reqs, rate, warm, **kw):
 
 def _plain_jobs(reqs, seed, tmpdir):
     """The plain GPU path: every request as its own catalog job (its own
-    configuration, tables, 65,536-stream seeding, count, draw and modules;
-    nothing kept between requests).  Returns stars, seconds, per-request
-    latencies."""
+    configuration load, tables, 65,536-stream seeding, count, draw and
+    modules; nothing kept between requests).  The configuration files a
+    user would already have on disk are written before the clock starts.
+    Returns stars, seconds, per-request latencies."""
+    import numpy as np
     from galfast_amd import configs
     from galfast_amd.catalog import Catalog
     import galfast_amd as G
-    stars, lat = 0, []
-    t_all = time.perf_counter()
+    confs = []
     for rid, l, b, r, m1, rseed in reqs:
         d = os.path.join(tmpdir, "job%d" % rid)
         os.makedirs(d, exist_ok=True)
+        confs.append(configs.write_c3(d, seed, m1=float(np.float32(m1)), beam=(l, b, r)))
+    stars, lat = 0, []
+    t_all = time.perf_counter()
+    for conf, (rid, l, b, r, m1, rseed) in zip(confs, reqs):
         t0 = time.perf_counter()
-        conf = configs.write_c3(d, seed, m1=float(__import__("numpy").float32(m1)), beam=(l, b, r))
         cat = Catalog(conf, seed=rseed, chunk_rows=1 << 22)
         cat.count()
         st = cat.run()
@@ -365,7 +413,8 @@ def main():
                     help="0: every launch of the catalog on one stream (isolated kernel durations)")
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=None,
+                    help="threads of the CPU baseline (default: every core this process may use)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--output-bench", action="store_true",
@@ -510,7 +559,8 @@ def main():
     if args.workload == "c1":
         out["footprint_cover"] = footprint_cover_bench()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cat, args.workload, args.cpu_budget, args.cpu_threads)
+        out["cpu_baseline"] = cpu_baseline(cat, args.workload, args.cpu_budget,
+                                           args.cpu_threads or default_cpu_threads())
     cat.close()
     if args.output_bench:
         out["output"] = output_bench(args)

@@ -152,7 +152,8 @@ m0 = 14
 m1 = 24
 M0 = 3
 M1 = 15
-dx = 0.5""")
+dx = 0.5
+maxstars = 0""")
     _w(os.path.join(d, "foot.conf"), "module = footprint\ntype = rect\ncoordsys = gal\nrect = 0 360 -90 90\nbmin = 10")
     _w(os.path.join(d, "Bond2010.conf"), """
 module = Bond2010
@@ -355,7 +356,7 @@ lumfunc = lf.txt""")
     else:
         sky, foot = ("m0 = 14\nm1 = 27.5\ndx = 0.25", "rect = 60 100 20 50")
     _w(os.path.join(d, "skygen.conf"), "module = skygen\nband = SYNr\ndm = 0.05\nM0 = 3\nM1 = 15\n%s\n"
-       "extmaps = ext.conf" % sky)
+       "extmaps = ext.conf\nmaxstars = 0" % sky)
     _w(os.path.join(d, "foot.conf"), "module = footprint\ntype = rect\ncoordsys = gal\n%s" % foot)
     _w(os.path.join(d, "cmd.conf"), """
 module = config

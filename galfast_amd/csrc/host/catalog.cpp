@@ -124,6 +124,9 @@ struct gf_catalog {
 	uint64_t chunk_rows = 30000000;
 	int acc_mode = GF_ACC_FP64;
 	double nstars_forced = 0;
+	double maxstars = 1e8;        // skygen's maxstars (0: unlimited)
+	double counted = -1;          // expected stars of the last count (after nstars), -1 before a count
+	DevBuf per_pixel;             // covered-fraction weighted counts per pixel, for the self-check
 
 	// modules
 	bool has_feh = false, has_gfeh = false, has_mult = false, has_bond = false, has_phot = false,
@@ -295,6 +298,22 @@ std::vector<gf_pencil_beam> pixelize_beam(const gf_footprint &fp, double dx)
 				if ((hemi == 0 && b < 0) || (hemi == 1 && b >= 0)) continue;
 				double x, y;
 				proj_pole(hemi, l, b, x, y);
+				xmin = std::min(xmin, x); xmax = std::max(xmax, x);
+				ymin = std::min(ymin, y); ymax = std::max(ymax, y);
+				any = true;
+			}
+		}
+		// the arc of the equator inside the cap: a cap that crosses the
+		// equator by less than the sampling step still reaches both
+		// hemispheres, and its cells along the equator lie on this arc
+		const double bcen = asin(std::max(-1.0, std::min(1.0, f.sb)));
+		if (bcen - r <= 0 && bcen + r >= 0) {
+			const double lc = atan2(f.sl, f.cl);
+			const double cr = f.cb > 1e-12 ? f.cosr / f.cb : (f.cosr <= 0 ? -1.0 : 2.0);
+			const double half = cr >= 1.0 ? 0.0 : (cr <= -1.0 ? M_PI : acos(cr));
+			for (int t = 0; t <= 96; t++) {
+				double l = lc + half * (2.0 * t / 96.0 - 1.0), x, y;
+				proj_pole(hemi, l, 0.0, x, y);
 				xmin = std::min(xmin, x); xmax = std::max(xmax, x);
 				ymin = std::min(ymin, y); ymax = std::max(ymax, y);
 				any = true;
@@ -904,7 +923,10 @@ void open_catalog(gf_catalog &c, const std::string &cmdfile)
 	for (auto &f : split_ws(cmd.get("modules", ""))) c.module_cfgs.push_back(loadc(f));
 	// the output stages are module configurations too (galfast.cpp:227-233)
 	for (auto &f : split_ws(cmd.get("output", ""))) c.module_cfgs.push_back(loadc(f));
-	c.nstars_forced = cmd.getd("nstars", 0.);
+	// nstars / maxstars belong to the skygen stage (skygen/os_skygen.cpp:816-820);
+	// a command-file value is kept as a fallback
+	c.nstars_forced = c.skygen.count("nstars") ? c.skygen.getd("nstars") : cmd.getd("nstars", 0.);
+	c.maxstars = c.skygen.count("maxstars") ? c.skygen.getd("maxstars") : cmd.getd("maxstars", 1e8);
 	if (c.foot_cfgs.empty()) fail("no footprint configuration given");
 	for (auto &mc : c.model_cfgs) mc.globals = &c.globals;
 
@@ -1347,6 +1369,7 @@ int gf_catalog_set(gf_catalog *c, const char *key, double value)
 	}
 	else if (k == "acc_mode") c->acc_mode = (int)value;
 	else if (k == "nstars") c->nstars_forced = value;
+	else if (k == "maxstars") c->maxstars = value;
 	else return GF_E_ARG;
 	return 0;
 }
@@ -1378,31 +1401,52 @@ int gf_catalog_count(gf_catalog *c, gf_count_result *per_model, double *expected
 	GF_CHECK(c, GF_E_ARG);
 	double tot = 0;
 	const uint64_t K = (uint64_t)(c->sp.nm + c->sp.nM - 1);
-	for (size_t i = 0; i < c->models.size(); i++) {
-		Model &m = *c->models[i];
-		m.wpk.alloc((size_t)c->sp.npixels * K * sizeof(double));
-		int rc = gf_skygen_count(&c->sp, &m.m, (const gf_pencil_beam *)c->pix_dev.p, c->has_ext ? &c->ext : nullptr,
-		                         c->acc_mode, &m.counts,
-		                         nullptr, (double *)m.wpk.p, c->stream);
-		if (rc) return rc;
-		if (per_model) per_model[i] = m.counts;
-		tot += m.counts.nstarsExpected;
-	}
-	// nstars renormalisation (skygen/os_skygen.cpp:886-901)
-	if (c->nstars_forced > 0 && tot > 0) {
-		double norm = c->nstars_forced / tot;
-		for (auto &m : c->models) m->m.norm = (float)norm;
+	const size_t np = (size_t)c->sp.npixels;
+	c->counted = -1;
+	if (c->per_pixel.bytes < np * sizeof(double)) c->per_pixel.alloc(np * sizeof(double));
+	std::vector<double> pp(np), den(np, 0.0);
+	// every count starts from the model's own density (norm 1), so that a
+	// count is idempotent and a changed nstars scales the raw total
+	for (auto &m : c->models) m->m.norm = 1.f;
+	auto count_all = [&](bool check) -> int {
 		tot = 0;
 		for (size_t i = 0; i < c->models.size(); i++) {
 			Model &m = *c->models[i];
+			if (m.wpk.bytes < np * K * sizeof(double)) m.wpk.alloc(np * K * sizeof(double));
 			int rc = gf_skygen_count(&c->sp, &m.m, (const gf_pencil_beam *)c->pix_dev.p,
-			                         c->has_ext ? &c->ext : nullptr, c->acc_mode,
-			                         &m.counts, nullptr, (double *)m.wpk.p, c->stream);
+			                         c->has_ext ? &c->ext : nullptr, c->acc_mode, &m.counts,
+			                         check ? (double *)c->per_pixel.p : nullptr, (double *)m.wpk.p, c->stream);
 			if (rc) return rc;
+			if (check && np) {
+				GF_HIP(hipMemcpy(pp.data(), c->per_pixel.p, np * sizeof(double), hipMemcpyDeviceToHost));
+				// the second summation of skygenHost::integrateCounts: the
+				// per-pixel densities added in pixel order must agree with
+				// the total to 1e-5 (skygen/skyconfig_impl.h:387-402)
+				double total = 0;
+				for (size_t p = 0; p < np; p++) { total += pp[p]; den[p] += pp[p]; }
+				if (m.counts.nstarsExpected > 0 && !(fabs(total / m.counts.nstarsExpected - 1.) < 1e-5)) {
+					g_last_error = "The expected number of stars, computed in two different ways is inconsistent "
+					               "(nstarsExpected=" + std::to_string(m.counts.nstarsExpected) +
+					               ", total=" + std::to_string(total) + ")";
+					return GF_E_STATE;
+				}
+			}
 			if (per_model) per_model[i] = m.counts;
 			tot += m.counts.nstarsExpected;
 		}
+		return 0;
+	};
+	int rc = count_all(true);
+	if (rc) return rc;
+	// nstars renormalisation (skygen/os_skygen.cpp:883-901): the densities
+	// are scaled by nstars / (the model total at norm 1)
+	if (c->nstars_forced > 0 && tot > 0) {
+		double norm = c->nstars_forced / tot;
+		for (auto &m : c->models) m->m.norm = (float)norm;
+		rc = count_all(false);
+		if (rc) return rc;
 	}
+	c->counted = tot;
 	if (expected) *expected = tot;
 	return 0;
 }
@@ -1411,6 +1455,13 @@ int gf_catalog_run(gf_catalog *c, gf_chunk_fn fn, void *user, gf_run_stats *stat
 {
 	GF_CHECK(c, GF_E_ARG);
 	int rc;
+	// the expected number of stars may not exceed maxstars (skygen/os_skygen.cpp:908-917)
+	if (c->counted >= 0 && c->maxstars != 0 && c->counted > c->maxstars) {
+		g_last_error = "The expected number of generated stars (" + std::to_string(c->counted) + ") "
+		               "exceeds the configuration limit (" + std::to_string(c->maxstars) + "). Increase "
+		               "maxstars or set to zero for unlimited.";
+		return GF_E_CONFIG;
+	}
 	try {
 		rc = c->draw_mode == 1 ? run_catalog_reference(*c, fn, user, stats) : run_catalog(*c, fn, user, stats);
 	} catch (const ConfigError &e) {
@@ -1560,6 +1611,26 @@ int gf_catalog_host_view_get(const gf_catalog *c, gf_catalog_host_view *v)
 	v->rng_state0 = c->rng_state0.data();
 	if (c->has_ext) v->ext = c->ext_h;
 	return 0;
+}
+
+int gf_beam_pixels_host(double l, double b, double radius, double dx, gf_pencil_beam *out, int cap)
+{
+	GF_CHECK(radius > 0 && dx > 0 && cap >= 0 && (out || cap == 0), GF_E_ARG);
+	try {
+		gf_footprint fp{};
+		fp.nregion = 1;
+		gf_foot_region &f = fp.r[0];
+		f.kind = 2;
+		const double d2r = M_PI / 180.0;
+		f.cl = cos(l * d2r); f.sl = sin(l * d2r); f.cb = cos(b * d2r); f.sb = sin(b * d2r);
+		f.cosr = cos(radius * d2r);
+		std::vector<gf_pencil_beam> v = pixelize_beam(fp, dx * d2r);
+		for (int i = 0; i < (int)v.size() && i < cap; i++) out[i] = v[i];
+		return (int)v.size();
+	} catch (const ConfigError &e) {
+		g_last_error = e.msg;
+		return GF_E_CONFIG;
+	}
 }
 
 int gf_lf_texture_host(const double *X, const double *Y, int n, int nsamp, float *tex, float tc[2])

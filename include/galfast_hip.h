@@ -737,6 +737,13 @@ int gf_count_visible(gf_col hidden, uint64_t n, uint64_t *visible, void *stream)
  *   (modules/photometry_host.cpp:175-349), [tex][nMr][nFeH][4] floats,
  *   returning the number of float4 tables or a negative error. */
 int gf_lf_texture_host(const double *X, const double *Y, int n, int nsamp, float *tex, float tc[2]);
+/* The pixels of a server request's circular beam (centre l, b and radius
+ * in degrees, dx-degree cells), as gf_footprint_cover and the catalog's
+ * pixelization would give them, computed on the host: writes up to cap
+ * beams, returns the number of pixels.  No reference counterpart (the
+ * reference pixelizes footprints through its polygon clipper,
+ * skygen/os_skygen.cpp:940-995). */
+int gf_beam_pixels_host(double l, double b, double radius, double dx, gf_pencil_beam *out, int cap);
 int gf_photometry_tables_host(const double *rows, int nrows, int ncolors, double dFeH, double dMr,
                               float *colors, float *flags, size_t cap_floats, int *nFeH, int *nMr,
                               float tc[4]);

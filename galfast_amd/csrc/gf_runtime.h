@@ -67,6 +67,11 @@ int chain_setup(const gf_rows *ks, const gf_chain *chain, const gf_rng *rng, con
                 gf_chain_launch &L);
 bool chain_on(const gf_chain_launch &L, int launch);
 int chain_launch(const gf_chain_launch &L, const gf_rng *rng, int launch, hipStream_t st);
+// The device address of the no-allocation scratch array on `device`
+// (noalloc_setup, once per device; gf_init calls it) and on the current
+// device (noalloc_scratch, setting it up on first use).
+int noalloc_setup(int device);
+int noalloc_scratch(void **ptr);
 // Adds the number of rows [0, n) with hidden == 0 to *counter (device memory).
 int count_visible(gf_col hidden, uint64_t n, unsigned long long *counter, hipStream_t st);
 

@@ -157,6 +157,7 @@ const char *gf_strerror(int code)
 	case GF_E_NODEVICE: return "no HIP device";
 	case GF_E_BUDGET: return "request exceeds the device-memory budget";
 	case GF_E_STATE: return "call out of order";
+	case GF_E_SCRATCH: return "input exceeds the fixed scratch of a no-allocation entry point";
 	default:
 		if (code > 0) return hipGetErrorString((hipError_t)code);
 		return "unknown error";
@@ -170,6 +171,9 @@ int gf_init(int device)
 	if (e != hipSuccess || n == 0) return GF_E_NODEVICE;
 	GF_CHECK(device >= 0 && device < n, GF_E_ARG);
 	GF_HIP(hipSetDevice(device));
+	// the address of the no-allocation scratch on this device; a failure
+	// here is reported by gf_skygen_count_noalloc, which repeats the set-up
+	if (gf::noalloc_setup(device) != 0) (void)hipGetLastError();
 	return 0;
 }
 

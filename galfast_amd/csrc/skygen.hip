@@ -8,6 +8,7 @@
 // contributes LF(M) * rho * norm * D^3 * dA * POGSON * dm * dM.
 #include "gf_runtime.h"
 #include "gf_device.h"
+#include <mutex>
 
 namespace {
 
@@ -48,13 +49,17 @@ __global__ void k_lf_tables(gf_skygen_params sp, gf_model m, LFTables t)
 	}
 }
 
-int lf_tables(const gf_skygen_params &sp, const gf_model &m, LFTables &t, hipStream_t st)
+// Bytes of the LF tables of nM absolute-magnitude bins.
+size_t lf_tables_bytes(int nM)
+{
+	size_t n = (size_t)nM + 1;
+	return n * 4 + n * 8 + n * 4 + n * 4 + 64;
+}
+
+// The LF tables in the buffer p (lf_tables_bytes(sp.nM) bytes).
+int lf_tables_at(const gf_skygen_params &sp, const gf_model &m, void *p, LFTables &t, hipStream_t st)
 {
 	size_t n = (size_t)sp.nM + 1;
-	size_t bytes = n * 4 + n * 8 + n * 4 + n * 4 + 64;
-	void *p;
-	int rc = gf::scratch(GF_SLOT_LFTAB, bytes, &p);
-	if (rc) return rc;
 	char *c = (char *)p;
 	t.P = (double *)c; c += n * 8;
 	t.lfc = (float *)c; c += n * 4;
@@ -63,6 +68,14 @@ int lf_tables(const gf_skygen_params &sp, const gf_model &m, LFTables &t, hipStr
 	k_lf_tables<<<1, SK_BS, 0, st>>>(sp, m, t);
 	GF_LAUNCHED();
 	return 0;
+}
+
+int lf_tables(const gf_skygen_params &sp, const gf_model &m, LFTables &t, hipStream_t st)
+{
+	void *p;
+	int rc = gf::scratch(GF_SLOT_LFTAB, lf_tables_bytes(sp.nM), &p);
+	if (rc) return rc;
+	return lf_tables_at(sp, m, p, t, st);
 }
 
 // Distance of DM bin k: compute_pos evaluated at the diagonal's first cell in
@@ -215,6 +228,45 @@ __global__ void __launch_bounds__(SK_BS) k_count_final(const CountAcc *pix_out, 
 		res->ncells = snc[0];
 	}
 }
+
+// Count result of an empty pixel list.
+__global__ void k_count_empty(gf_count_result *res)
+{
+	*res = gf_count_result{0, 0, 0, 0};
+}
+
+// The count kernels in the accumulation arithmetic acc_mode: per-pixel sums
+// into pix_out (npixels entries), the job's totals into *dres (device).
+int launch_count(const gf_skygen_params &sp, const gf_model &m, const gf_pencil_beam *pixels,
+                 const gf_extinction &ex, int acc_mode, const LFTables &t, CountAcc *pix_out,
+                 gf_count_result *dres, double *per_pixel, double *wpk, hipStream_t st)
+{
+	switch (acc_mode) {
+	case GF_ACC_FP64:
+		k_count<GF_ACC_FP64><<<sp.npixels, SK_BS, 0, st>>>(sp, m, pixels, ex, t, pix_out, per_pixel, wpk);
+		GF_LAUNCHED();
+		k_count_final<GF_ACC_FP64><<<1, SK_BS, 0, st>>>(pix_out, sp.npixels, dres);
+		break;
+	case GF_ACC_KAHAN32:
+		k_count<GF_ACC_KAHAN32><<<sp.npixels, SK_BS, 0, st>>>(sp, m, pixels, ex, t, pix_out, per_pixel, wpk);
+		GF_LAUNCHED();
+		k_count_final<GF_ACC_KAHAN32><<<1, SK_BS, 0, st>>>(pix_out, sp.npixels, dres);
+		break;
+	default:
+		k_count<GF_ACC_FP32><<<sp.npixels, SK_BS, 0, st>>>(sp, m, pixels, ex, t, pix_out, per_pixel, wpk);
+		GF_LAUNCHED();
+		k_count_final<GF_ACC_FP32><<<1, SK_BS, 0, st>>>(pix_out, sp.npixels, dres);
+		break;
+	}
+	GF_LAUNCHED();
+	return 0;
+}
+
+// Layout of gf_skygen_count_noalloc's scratch: the LF tables of up to
+// GF_NOALLOC_MAX_NM bins, then the per-pixel partial sums of up to
+// GF_NOALLOC_MAX_PIXELS pixels.
+constexpr size_t NOALLOC_LF_BYTES = ((((size_t)GF_NOALLOC_MAX_NM + 1) * 20 + 64) + 255) & ~(size_t)255;
+constexpr size_t NOALLOC_BYTES = NOALLOC_LF_BYTES + (size_t)GF_NOALLOC_MAX_PIXELS * sizeof(CountAcc);
 
 // ---------------------------------------------------------------------------
 // Diagonal crawl of the reference (skygen/skygen.cu.h:231-358).
@@ -577,27 +629,38 @@ int gf_skygen_count(const gf_skygen_params *sp, const gf_model *model,
 	if (rc) return rc;
 	CountAcc *pix_out = (CountAcc *)p;
 	gf_count_result *dres = (gf_count_result *)((char *)p + (size_t)sp->npixels * sizeof(CountAcc));
-	switch (acc_mode) {
-	case GF_ACC_FP64:
-		k_count<GF_ACC_FP64><<<sp->npixels, SK_BS, 0, st>>>(*sp, *model, pixels, ex, t, pix_out, per_pixel, wpk);
-		GF_LAUNCHED();
-		k_count_final<GF_ACC_FP64><<<1, SK_BS, 0, st>>>(pix_out, sp->npixels, dres);
-		break;
-	case GF_ACC_KAHAN32:
-		k_count<GF_ACC_KAHAN32><<<sp->npixels, SK_BS, 0, st>>>(*sp, *model, pixels, ex, t, pix_out, per_pixel, wpk);
-		GF_LAUNCHED();
-		k_count_final<GF_ACC_KAHAN32><<<1, SK_BS, 0, st>>>(pix_out, sp->npixels, dres);
-		break;
-	default:
-		k_count<GF_ACC_FP32><<<sp->npixels, SK_BS, 0, st>>>(*sp, *model, pixels, ex, t, pix_out, per_pixel, wpk);
-		GF_LAUNCHED();
-		k_count_final<GF_ACC_FP32><<<1, SK_BS, 0, st>>>(pix_out, sp->npixels, dres);
-		break;
-	}
-	GF_LAUNCHED();
+	rc = launch_count(*sp, *model, pixels, ex, acc_mode, t, pix_out, dres, per_pixel, wpk, st);
+	if (rc) return rc;
 	GF_HIP(hipMemcpyAsync(out, dres, sizeof(gf_count_result), hipMemcpyDeviceToHost, st));
 	GF_HIP(hipStreamSynchronize(st));
 	return 0;
+}
+
+int gf_skygen_count_noalloc(const gf_skygen_params *sp, const gf_model *model,
+                            const gf_pencil_beam *pixels, const gf_extinction *ext,
+                            int acc_mode, gf_count_result *out_dev, double *per_pixel,
+                            double *wpk, void *stream)
+{
+	GF_CHECK(sp && model && pixels && out_dev && sp->npixels >= 0 && sp->nm > 0 && sp->nM > 0, GF_E_ARG);
+	GF_CHECK(acc_mode >= 0 && acc_mode <= 2, GF_E_ARG);
+	GF_CHECK(sp->npixels <= GF_NOALLOC_MAX_PIXELS && sp->nM <= GF_NOALLOC_MAX_NM, GF_E_SCRATCH);
+	void *base;
+	int rc = gf::noalloc_scratch(&base);
+	if (rc) return rc;
+	hipStream_t st = gf::as_stream(stream);
+	gf_extinction ex{};
+	if (ext) ex = *ext;
+	// the LF tables first, then the per-pixel partial sums (256-byte aligned)
+	LFTables t;
+	rc = lf_tables_at(*sp, *model, base, t, st);
+	if (rc) return rc;
+	CountAcc *pix_out = (CountAcc *)((char *)base + NOALLOC_LF_BYTES);
+	if (sp->npixels == 0) {
+		k_count_empty<<<1, 1, 0, st>>>(out_dev);
+		GF_LAUNCHED();
+		return 0;
+	}
+	return launch_count(*sp, *model, pixels, ex, acc_mode, t, pix_out, out_dev, per_pixel, wpk, st);
 }
 
 int gf_skygen_draw_compat(const gf_skygen_params *sp, const gf_model *model,
@@ -794,4 +857,54 @@ int lf_prefix(const gf_skygen_params &sp, const gf_model &m, double *dst, hipStr
 	GF_HIP(hipStreamSynchronize(st));
 	return 0;
 }
+} // namespace gf
+
+// The scratch of gf_skygen_count_noalloc (8.1 MiB, held by every device the
+// library runs on until the process ends).  At global scope, so that
+// ::scratch never names gf::scratch (the allocating cache).  Host code never
+// passes the array's name to a kernel: each device's address comes from
+// hipGetSymbolAddress (gf::noalloc_setup).
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace gf {
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[NOALLOC_MAX_DEVICES];
+}
+
+int noalloc_setup(int device)
+{
+	GF_CHECK(device >= 0 && device < NOALLOC_MAX_DEVICES, GF_E_ARG);
+	std::lock_guard<std::mutex> lk(g_noalloc_mu);
+	if (g_noalloc_ptr[device]) return 0;
+	int cur = 0;
+	GF_HIP(hipGetDevice(&cur));
+	if (cur != device) GF_HIP(hipSetDevice(device));
+	void *p = nullptr;
+	hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch));
+	if (cur != device) (void)hipSetDevice(cur);
+	if (e != hipSuccess) return (int)e;
+	if (!p) return GF_E_STATE;
+	g_noalloc_ptr[device] = p;
+	return 0;
+}
+
+int noalloc_scratch(void **ptr)
+{
+	int dev = 0;
+	GF_HIP(hipGetDevice(&dev));
+	GF_CHECK(dev >= 0 && dev < NOALLOC_MAX_DEVICES, GF_E_ARG);
+	{
+		std::lock_guard<std::mutex> lk(g_noalloc_mu);
+		if (g_noalloc_ptr[dev]) { *ptr = g_noalloc_ptr[dev]; return 0; }
+	}
+	int rc = noalloc_setup(dev);
+	if (rc) return rc;
+	std::lock_guard<std::mutex> lk(g_noalloc_mu);
+	*ptr = g_noalloc_ptr[dev];
+	return 0;
+}
+
 } // namespace gf

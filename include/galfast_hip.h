@@ -33,6 +33,7 @@ extern "C" {
 #define GF_E_NODEVICE    (-6)   /* no HIP device */
 #define GF_E_BUDGET      (-7)   /* request exceeds the device-memory budget */
 #define GF_E_STATE       (-8)   /* call out of order */
+#define GF_E_SCRATCH     (-9)   /* input exceeds a no-allocation entry point's fixed scratch */
 
 /* Human-readable text for an error code (static storage). */
 const char *gf_strerror(int code);
@@ -429,6 +430,24 @@ int gf_skygen_count(const gf_skygen_params *sp, const gf_model *model,
                     const gf_pencil_beam *pixels, const gf_extinction *ext,
                     int acc_mode, gf_count_result *out, double *per_pixel,
                     double *wpk, void *stream);
+
+/* gf_skygen_count without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same arguments and
+ * results as gf_skygen_count except that out_dev is DEVICE memory and the
+ * call neither copies to the host nor waits: the totals are in *out_dev
+ * once `stream` reaches that point.  Its scratch (LF tables and per-pixel
+ * partial sums) is a fixed device array of the library, one per GPU,
+ * limited to GF_NOALLOC_MAX_PIXELS pixels and GF_NOALLOC_MAX_NM
+ * absolute-magnitude bins (GF_E_SCRATCH beyond).  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (different
+ * streams, or host threads) race; calls ordered on one stream do not. */
+#define GF_NOALLOC_MAX_PIXELS 262144
+#define GF_NOALLOC_MAX_NM     4096
+int gf_skygen_count_noalloc(const gf_skygen_params *sp, const gf_model *model,
+                            const gf_pencil_beam *pixels, const gf_extinction *ext,
+                            int acc_mode, gf_count_result *out_dev, double *per_pixel,
+                            double *wpk, void *stream);
 
 /* Skygen output columns (skygen/skygen.h:170-177). */
 typedef struct gf_skygen_cols {

@@ -198,6 +198,59 @@ struct gf_mwc
 	}
 };
 
+// Jump-ahead.  With base 2^32 a stream's state (c, x), read as
+// z = c*2^32 + x, lives in Z/mZ for m = a*2^32 - 1: one step maps z to
+// a*(z mod 2^32) + (z div 2^32) = a*z mod m (a*2^32 = 1 mod m), so n steps
+// multiply z by a^n.  c < a holds for every seeded state and is kept by the
+// recurrence, so z < m; the two fixed points z = 0 and z = m are never
+// reached from a seeded state.
+//
+// Products mod m are Montgomery products with R = 2^64, made of two
+// reductions by 2^32: t*2^-32 = (t div 2^32) + a*(t mod 2^32) (mod m),
+// because 2^-32 = a (mod m).  A jump table holds a^(2^b) * R mod m; its
+// first entry a*R mod m is 2^32 exactly (a*2^64 = 2^32 mod m), and each next
+// entry is the Montgomery square of the previous one.  mul(z, table[b]) is
+// z * a^(2^b) mod m for z in plain form.
+constexpr int GF_JUMP_BITS = 64;   // jump tables cover n < 2^64
+
+struct gf_mwc_mod
+{
+	uint64_t m;
+	uint32_t a;
+
+	GF_HD explicit gf_mwc_mod(uint32_t a_) : m(((uint64_t)a_ << 32) - 1u), a(a_) {}
+
+	// x * Y * 2^-64 mod m for x, Y < m
+	GF_HD uint64_t mul(uint64_t x, uint64_t Y) const
+	{
+		const uint32_t x0 = (uint32_t)x, x1 = (uint32_t)(x >> 32);
+		const uint32_t y0 = (uint32_t)Y, y1 = (uint32_t)(Y >> 32);
+		// t = x*y0 < m*2^32; r = t*2^-32 mod m
+		const uint64_t p0 = (uint64_t)x0 * y0;
+		const uint64_t p1 = (uint64_t)x1 * y0 + (p0 >> 32);             // t div 2^32 < m
+		uint64_t r = p1 + (uint64_t)a * (uint32_t)p0;                    // < 2m - a
+		if (r < p1 || r >= m) r -= m;
+		// t = r + x*y1 < m*2^32; result = t*2^-32 mod m
+		const uint64_t q0 = (uint64_t)x0 * y1 + (uint32_t)r;
+		const uint64_t q1 = (uint64_t)x1 * y1 + (q0 >> 32) + (r >> 32);  // t div 2^32 < m
+		uint64_t s = q1 + (uint64_t)a * (uint32_t)q0;                    // < 2m - a
+		if (s < q1 || s >= m) s -= m;
+		return s;
+	}
+};
+
+GF_HD uint64_t gf_mwc_z(const gf_mwc &r) { return ((uint64_t)r.c << 32) | r.x; }
+GF_HD void gf_mwc_set_z(gf_mwc &r, uint64_t z) { r.c = (uint32_t)(z >> 32); r.x = (uint32_t)z; }
+
+// z * a^n mod m with the jump table of the stream (entry b at tab[b * stride]
+// for every set bit b of n).
+GF_HD uint64_t gf_mwc_jump_z(const gf_mwc_mod &md, uint64_t z, uint64_t n, const uint64_t *tab, size_t stride)
+{
+	for (int b = 0; n; b++, n >>= 1)
+		if (n & 1u) z = md.mul(z, tab[(size_t)b * stride]);
+	return z;
+}
+
 // Plane layout of a stream table: [a | c | x], each plane nstreams words
 // (common/cuda_rng.h:45-69).
 GF_HD gf_mwc gf_mwc_load(const uint32_t *st, uint32_t nstreams, uint32_t s)

@@ -72,6 +72,11 @@ int chain_launch(const gf_chain_launch &L, const gf_rng *rng, int launch, hipStr
 // device (noalloc_scratch, setting it up on first use).
 int noalloc_setup(int device);
 int noalloc_scratch(void **ptr);
+// Jump tables of the streams of rng (GF_JUMP_BITS * nstreams entries,
+// entry b of stream s at tab[b * nstreams + s]) and every stream n steps
+// ahead with such a table.
+int mwc_jump_table(const gf_rng *rng, uint64_t *tab, hipStream_t st);
+int mwc_jump(gf_rng *rng, uint64_t n, const uint64_t *tab, hipStream_t st);
 // Adds the number of rows [0, n) with hidden == 0 to *counter (device memory).
 int count_visible(gf_col hidden, uint64_t n, unsigned long long *counter, hipStream_t st);
 
@@ -94,5 +99,6 @@ enum gf_scratch_slot {
 	GF_SLOT_OUT,
 	GF_SLOT_OUT_COLS,
 	GF_SLOT_OUT_COUNT,
+	GF_SLOT_JUMP,
 	GF_SLOT_NSLOTS
 };

@@ -6,6 +6,7 @@
 #include "host.h"
 #include "../gf_runtime.h"
 #include "../gf_chain_types.h"
+#include "../gf_rng.h"
 
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -188,6 +189,7 @@ struct gf_catalog {
 	// Position planes of the draw passes, alternating per chain call: the
 	// draw pass into buffer b waits for the row passes that read it before.
 	DevBuf pos[2];
+	DevBuf jtab;          // jump tables of the 65,536 streams (GF_JUMP_BITS x 65536)
 	hipEvent_t drawn[2] = {}, pos_free[2] = {}, pos_free_side[2] = {}, side_done[2] = {};
 	DevBuf vis_dev;                            // visible-row counter per slot
 	unsigned long long *vis_host = nullptr;    // pinned copies of the counters
@@ -997,6 +999,9 @@ void open_catalog(gf_catalog &c, const std::string &cmdfile)
 	if (gf_mwc_seed_host(s32, primes.data(), (uint32_t)primes.size(), 65536, st.data()) != 65536)
 		fail("seeding failed");
 	if (gf_rng_alloc(&c.rng, 65536) || gf_rng_upload(&c.rng, st.data(), c.stream)) fail("stream upload failed");
+	c.jtab.alloc((size_t)GF_JUMP_BITS * 65536 * sizeof(uint64_t));
+	if (gf::mwc_jump_table(&c.rng, (uint64_t *)c.jtab.p, c.stream) || hipStreamSynchronize(c.stream) != hipSuccess)
+		fail("jump tables failed");
 	c.rng_state0 = st;
 
 	// host mirrors of the device parameter blocks

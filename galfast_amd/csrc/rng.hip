@@ -37,7 +37,53 @@ __global__ void mwc_sample_kernel(uint32_t *state, uint32_t ns, int kind, float 
 	gf_mwc_store(state, ns, s, r);
 }
 
+// Jump tables of the streams: tab[b * ns + s] = a_s^(2^b) * 2^64 mod m_s for
+// b < GF_JUMP_BITS (gf_mwc_mod).
+__global__ void k_jump_table(const uint32_t *state, uint32_t ns, uint64_t *tab)
+{
+	uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= ns) return;
+	const gf_mwc_mod md(state[s]);
+	uint64_t y = 1ull << 32;
+	for (int b = 0; b < GF_JUMP_BITS; b++) {
+		tab[(size_t)b * ns + s] = y;
+		y = md.mul(y, y);
+	}
+}
+
+// Every stream n steps ahead.
+__global__ void k_jump(uint32_t *state, uint32_t ns, uint64_t n, const uint64_t *tab)
+{
+	uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= ns) return;
+	gf_mwc r = gf_mwc_load(state, ns, s);
+	const gf_mwc_mod md(r.a);
+	gf_mwc_set_z(r, gf_mwc_jump_z(md, gf_mwc_z(r), n, tab + s, ns));
+	gf_mwc_store(state, ns, s, r);
+}
+
 } // namespace
+
+namespace gf {
+
+int mwc_jump_table(const gf_rng *rng, uint64_t *tab, hipStream_t st)
+{
+	GF_CHECK(rng && rng->state && tab, GF_E_ARG);
+	k_jump_table<<<gf::blocks_for(rng->nstreams, 256), 256, 0, st>>>(rng->state, rng->nstreams, tab);
+	GF_LAUNCHED();
+	return 0;
+}
+
+int mwc_jump(gf_rng *rng, uint64_t n, const uint64_t *tab, hipStream_t st)
+{
+	GF_CHECK(rng && rng->state && tab, GF_E_ARG);
+	if (!n) return 0;
+	k_jump<<<gf::blocks_for(rng->nstreams, 256), 256, 0, st>>>(rng->state, rng->nstreams, n, tab);
+	GF_LAUNCHED();
+	return 0;
+}
+
+} // namespace gf
 
 extern "C" {
 
@@ -100,6 +146,34 @@ int gf_rng_free(gf_rng *rng)
 	if (rng->state) GF_HIP(hipFree(rng->state));
 	rng->state = nullptr;
 	rng->nstreams = 0;
+	return 0;
+}
+
+int gf_mwc_jump(gf_rng *rng, uint64_t n, void *stream)
+{
+	GF_CHECK(rng && rng->state, GF_E_ARG);
+	if (!n) return 0;
+	hipStream_t st = gf::as_stream(stream);
+	void *p;
+	int rc = gf::scratch(GF_SLOT_JUMP, (size_t)GF_JUMP_BITS * rng->nstreams * sizeof(uint64_t), &p);
+	if (rc) return rc;
+	rc = gf::mwc_jump_table(rng, (uint64_t *)p, st);
+	if (rc) return rc;
+	return gf::mwc_jump(rng, n, (const uint64_t *)p, st);
+}
+
+int gf_mwc_jump_host(uint32_t *state, uint32_t nstreams, uint64_t n)
+{
+	GF_CHECK(state, GF_E_ARG);
+	uint64_t tab[GF_JUMP_BITS];
+	for (uint32_t s = 0; s < nstreams; s++) {
+		gf_mwc r = gf_mwc_load(state, nstreams, s);
+		const gf_mwc_mod md(r.a);
+		uint64_t y = 1ull << 32;
+		for (int b = 0; b < GF_JUMP_BITS; b++) { tab[b] = y; y = md.mul(y, y); }
+		gf_mwc_set_z(r, gf_mwc_jump_z(md, gf_mwc_z(r), n, tab, 1));
+		gf_mwc_store(state, nstreams, s, r);
+	}
 	return 0;
 }
 

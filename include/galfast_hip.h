@@ -117,6 +117,14 @@ int gf_rng_upload(gf_rng *rng, const uint32_t *host_state, void *stream);
 int gf_rng_download(const gf_rng *rng, uint32_t *host_state, void *stream);
 int gf_rng_free(gf_rng *rng);
 
+/* Move every stream n steps ahead in O(log n) (multiply-with-carry
+ * jump-ahead: with base 2^32 the state c*2^32 + x is multiplied by a^n
+ * modulo a*2^32 - 1).  Same states as n calls of uniform() per stream
+ * (common/cuda_rng.h:439-463); no counterpart in the reference, which only
+ * steps.  gf_mwc_jump_host does the same on a host stream table. */
+int gf_mwc_jump(gf_rng *rng, uint64_t n, void *stream);
+int gf_mwc_jump_host(uint32_t *state, uint32_t nstreams, uint64_t n);
+
 /* Sampler test kernel: stream s produces n values of the chosen sampler,
  * written to out[i*nstreams + s] (float for uniform/gaussian, uint32 for
  * poisson).  kind: 0 uniform, 1 gaussian(param), 2 poisson(param),

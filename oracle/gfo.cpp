@@ -265,6 +265,54 @@ static void rng_store(uint32_t *st, uint32_t ns, uint32_t s, const gfo_rng &r)
 	st[ns + s] = r.c; st[2 * ns + s] = r.x;
 }
 
+/* Jump-ahead of a multiply-with-carry stream (the recurrence of
+ * common/cuda_rng.h:439-463).  With base b = 2^32, read the state as
+ * z = c*2^32 + x.  One step maps z to a*(z mod 2^32) + (z div 2^32), which is
+ * a*z mod m for m = a*2^32 - 1 (because a*2^32 = 1 mod m), so n steps
+ * multiply z by a^n mod m.  Evaluated here with exact 128-bit products
+ * (square-and-multiply); pinned against stepping the recurrence
+ * (tests/test_oracle_rng.py). */
+static uint64_t gfo_mulmod(uint64_t x, uint64_t y, uint64_t m)
+{
+	return (uint64_t)(((unsigned __int128)x * y) % m);
+}
+static uint64_t gfo_powmod(uint64_t a, uint64_t n, uint64_t m)
+{
+	uint64_t r = 1 % m, b = a % m;
+	while (n) {
+		if (n & 1) r = gfo_mulmod(r, b, m);
+		b = gfo_mulmod(b, b, m);
+		n >>= 1;
+	}
+	return r;
+}
+static void gfo_jump(gfo_rng &r, uint64_t n)
+{
+	const uint64_t m = ((uint64_t)r.a << 32) - 1;
+	uint64_t z = ((uint64_t)r.c << 32) | r.x;
+	z = gfo_mulmod(z, gfo_powmod(r.a, n, m), m);
+	r.c = (uint32_t)(z >> 32);
+	r.x = (uint32_t)z;
+}
+/* every stream of st moved n steps ahead */
+GFO_API void gfo_mwc_jump(uint32_t *st, uint32_t ns, uint64_t n)
+{
+	for (uint32_t s = 0; s < ns; s++) {
+		gfo_rng r = rng_load(st, ns, s);
+		gfo_jump(r, n);
+		rng_store(st, ns, s, r);
+	}
+}
+/* every stream of st moved n steps ahead by stepping */
+GFO_API void gfo_mwc_step(uint32_t *st, uint32_t ns, uint64_t n)
+{
+	for (uint32_t s = 0; s < ns; s++) {
+		gfo_rng r = rng_load(st, ns, s);
+		for (uint64_t i = 0; i < n; i++) (void)r.uniform();
+		rng_store(st, ns, s, r);
+	}
+}
+
 GFO_API void gfo_mwc_sample(uint32_t *st, uint32_t ns, int kind, float param, uint32_t n, void *out)
 {
 	for (uint32_t s = 0; s < ns; s++) {

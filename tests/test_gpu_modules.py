@@ -214,3 +214,34 @@ def test_fused_chain_equals_unfused_and_oracle():
     fx.oracle_chain(feh=feh, bond=bond, phot=fx.phot_host, phot_comps=comps, perr=pe_h, v2pm=v2pm,
                     gal2other_coordsys=1)
     fx.compare(["FeH", "vcyl", "mags", "photoFlags", "obsMags", "pm", "radec"])
+
+
+def _gfeh_params():
+    p = A.gf_GaussianFeH_data()
+    p.ncomp = 3
+    for i, (m, s, w) in enumerate([(-0.5, 0.2, 0.6), (-1.0, 0.3, 0.3), (-1.5, 0.4, 0.1)]):
+        p.mean[i], p.sigma[i] = m, s
+    cs = np.cumsum([0.6, 0.3, 0.1])
+    for i in range(3):
+        p.pcum[i] = cs[i] / cs[-1]
+    return p
+
+
+def test_fused_chain_gaussianfeh_multiples_equals_oracle():
+    """GaussianFeH + unresolvedMultiples + Bond2010 + photometric errors (the
+    config-4 kind of chain) in one fused chain against the oracle."""
+    L = G.lib()
+    fx = Fixture()
+    comps = A.bitmap(0, 1, 2)
+    gfeh, bond = _gfeh_params(), gfx.bond_params()
+    md, mh = fx.mult(True, 1), fx.mult(False, 1)
+    pe_d, pe_h = fx.perr(True), fx.perr(False)
+    ch = A.gf_chain()
+    ch.gfeh, ch.gfeh_comps = C.pointer(gfeh), comps
+    ch.mult, ch.mult_comps = C.pointer(md), comps
+    ch.bond = C.pointer(bond)
+    ch.perr = C.pointer(pe_d)
+    G.check(L.gf_module_chain(C.byref(fx.rows()), C.byref(ch), C.byref(fx.drng.rng), C.byref(fx.dev.table()),
+                              stream_ptr()), "gf_module_chain")
+    fx.oracle_chain(gfeh=gfeh, gfeh_comps=comps, mult=mh, mult_comps=comps, bond=bond, perr=pe_h)
+    fx.compare(["FeH", "FeH_comp", "absmagSys", "absmagSysNcomp", "vcyl", "obsMags"])

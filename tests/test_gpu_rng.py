@@ -50,3 +50,18 @@ def test_poisson_sweep_of_means():
     # stream s draws with mean 0.01*(1+s): covers the product loop, the
     # gamma_int/gamma_large split and the binomial tail
     _run(3, 0.01, 4, seed=13, ns=65536)
+
+
+def test_device_jump_equals_oracle():
+    # gf_mwc_jump (device Montgomery products) against the oracle's 128-bit
+    # square-and-multiply, for every one of 65,536 seeded streams
+    O.lib().gfo_mwc_jump.argtypes = [C.c_void_p, C.c_uint32, C.c_uint64]
+    L = G.lib()
+    L.gf_mwc_jump.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p]
+    st = mwc_state(31, 65536)
+    d = DevRng(st)
+    ref = st.copy()
+    for n in (1, 1000, (1 << 33) + 17, (1 << 62) + 3):
+        G.check(L.gf_mwc_jump(C.byref(d.rng), n, stream_ptr()), "gf_mwc_jump")
+        O.lib().gfo_mwc_jump(O.ptr(ref), 65536, n)
+        assert np.array_equal(d.state(), ref), n

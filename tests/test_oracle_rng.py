@@ -119,3 +119,73 @@ def test_samplers_match_their_distributions(kind, param):
             sel = exp > 20
             chi2 = ((obs[sel] - exp[sel]) ** 2 / exp[sel]).sum()
             assert stats.chi2.sf(chi2, sel.sum() - 1) > 1e-4
+
+
+def _step_numpy(st, ns, n):
+    """n steps of every stream by the 64-bit recurrence, vectorised."""
+    a = st[:ns].astype(np.uint64)
+    c = st[ns:2 * ns].astype(np.uint64)
+    x = st[2 * ns:].astype(np.uint64)
+    for _ in range(n):
+        t = a * x + c
+        c = t >> np.uint64(32)
+        x = t & np.uint64(0xFFFFFFFF)
+    return np.concatenate([a, c, x]).astype(np.uint32)
+
+
+def test_mwc_jump_equals_stepping_for_every_multiplier():
+    # all 65,536 seeded streams (multipliers below and above 2^31, carries
+    # from the x86 overflow rule); jumps of n steps against n recurrence steps
+    O.lib().gfo_mwc_jump.argtypes = [C.c_void_p, C.c_uint32, C.c_uint64]
+    ns = 65536
+    st0 = mwc_state(2024, ns)
+    cur, done = st0.copy(), 0
+    for n in (1, 2, 3, 63, 64, 65, 700):
+        ref = _step_numpy(cur, ns, n - done)
+        got = st0.copy()
+        O.lib().gfo_mwc_jump(O.ptr(got), ns, n)
+        assert np.array_equal(got, ref), n
+        cur, done = ref, n
+    # a wrong modulus or exponent fails: one step fewer differs everywhere
+    got = st0.copy()
+    O.lib().gfo_mwc_jump(O.ptr(got), ns, 699)
+    assert (got[ns:] != cur[ns:]).reshape(2, ns).any(axis=0).all()
+
+
+def test_mwc_jump_long_distances_and_composition():
+    O.lib().gfo_mwc_jump.argtypes = [C.c_void_p, C.c_uint32, C.c_uint64]
+    O.lib().gfo_mwc_step.argtypes = [C.c_void_p, C.c_uint32, C.c_uint64]
+    ns = 64
+    st0 = mwc_state(99, 65536)
+    sub = np.concatenate([st0[j * 65536 + 54720: j * 65536 + 54784] for j in range(3)])   # a > 2^31 among them
+    assert (sub[:ns] > 2 ** 31).any() and (sub[:ns] < 2 ** 31).any()
+    n = (1 << 20) + 12345
+    a = sub.copy()
+    O.lib().gfo_mwc_step(O.ptr(a), ns, n)
+    b = sub.copy()
+    O.lib().gfo_mwc_jump(O.ptr(b), ns, n)
+    assert np.array_equal(a, b)
+    # jumps compose: 2^40 + 2^33 + 5 in three jumps and in one
+    c1 = sub.copy()
+    for k in (1 << 40, 1 << 33, 5):
+        O.lib().gfo_mwc_jump(O.ptr(c1), ns, k)
+    c2 = sub.copy()
+    O.lib().gfo_mwc_jump(O.ptr(c2), ns, (1 << 40) + (1 << 33) + 5)
+    assert np.array_equal(c1, c2)
+
+
+def test_product_host_jump_equals_oracle():
+    # the library's Montgomery products (the device code compiled for the
+    # host) against the oracle's 128-bit square-and-multiply
+    import galfast_amd as GA
+    L = GA.lib()
+    L.gf_mwc_jump_host.argtypes = [C.c_void_p, C.c_uint32, C.c_uint64]
+    O.lib().gfo_mwc_jump.argtypes = [C.c_void_p, C.c_uint32, C.c_uint64]
+    ns = 65536
+    st0 = mwc_state(5, ns)
+    for n in (1, 64, 8 * 1000 + 3, (1 << 31) + 8 * 17, (7 << 32) + (1 << 31), (1 << 63) + 12345, 2 ** 64 - 1):
+        a = st0.copy()
+        b = st0.copy()
+        assert L.gf_mwc_jump_host(O.ptr(a), ns, n) == 0
+        O.lib().gfo_mwc_jump(O.ptr(b), ns, n)
+        assert np.array_equal(a, b), n

@@ -38,11 +38,9 @@ PEAK_HBM_GBS = 8000.0
 # Algorithmic bytes per row of each kernel: every
 # column a kernel must read, and every column it writes, once.  Hidden rows
 # cost the 4-byte hidden flag in the module kernels.
-# emit, per generated row: the random half writes the 20-byte parked star
-# (x, y, M, m, pixel|hidden); the row half reads it and writes the 10
-# skygen columns (64 B)
-EMIT_RNG_BYTES_PER_ROW = 20
-EMIT_ROWS_BYTES_PER_ROW = 20 + 64
+# emission, per generated row: the 10 skygen columns (64 B) written; the
+# unit data (offsets, start state, pixel) is read once per unit, not per row
+EMIT_STARS_BYTES_PER_ROW = 64
 # (bytes per visible row, bytes per hidden row) of the module-chain launches
 # for the config-1 module set (FeH, Bond2010, photometry with 5 bands,
 # photometric errors, vel2pm, gal2other): every column a launch must read,
@@ -469,7 +467,7 @@ def main():
     t0 = time.perf_counter()
     acc = {"visible": 0, "generated": 0, "t_count": 0.0, "t_plan": 0.0, "t_emit": 0.0, "t_chain": 0.0,
            "n_emit": 0, "n_chain": 0, "expected": 0.0, "t_stage": [0.0] * len(STAGE_NAMES),
-           "t_emit_rng": 0.0, "t_emit_rows": 0.0}
+           "t_emit_stars": 0.0, "t_emit_units": 0.0}
     for _ in range(args.steps):
         expected, tc, st = step()
         acc["visible"] += st.visible
@@ -477,8 +475,8 @@ def main():
         acc["t_count"] += tc
         acc["t_plan"] += st.t_plan
         acc["t_emit"] += st.t_emit
-        acc["t_emit_rng"] += st.t_emit_rng
-        acc["t_emit_rows"] += st.t_emit_rows
+        acc["t_emit_stars"] += st.t_emit_stars
+        acc["t_emit_units"] += st.t_emit_units
         acc["t_chain"] += st.t_chain
         acc["n_emit"] += st.n_emit
         acc["n_chain"] += st.n_chain
@@ -492,8 +490,7 @@ def main():
     # roofline of the dominant kernel, from HIP-event durations recorded on
     # the catalog's stream around each launch
     hidden = acc["generated"] - acc["visible"]
-    kernels = {"k_emit_rng": (acc["t_emit_rng"], acc["n_emit"], EMIT_RNG_BYTES_PER_ROW * acc["generated"]),
-               "k_emit_rows": (acc["t_emit_rows"], acc["n_emit"], EMIT_ROWS_BYTES_PER_ROW * acc["generated"])}
+    kernels = {"k_emit_stars": (acc["t_emit_stars"], acc["n_emit"], EMIT_STARS_BYTES_PER_ROW * acc["generated"])}
     for q, name in enumerate(STAGE_NAMES):
         if acc["t_stage"][q] > 0:
             bv, bh = STAGE_BYTES[name]
@@ -548,8 +545,8 @@ def main():
                    "expected_per_step": acc["expected"] / args.steps},
         "breakdown_ms_per_step": dict(
             [(k, 1000.0 * acc[k] / args.steps) for k in ("t_count", "t_plan", "t_emit", "t_chain")] +
-            [("k_emit_rng", 1000.0 * acc["t_emit_rng"] / args.steps),
-             ("k_emit_rows", 1000.0 * acc["t_emit_rows"] / args.steps)] +
+            [("k_emit_stars", 1000.0 * acc["t_emit_stars"] / args.steps),
+             ("k_tile_units", 1000.0 * acc["t_emit_units"] / args.steps)] +
             [(STAGE_NAMES[q], 1000.0 * acc["t_stage"][q] / args.steps) for q in range(len(STAGE_NAMES))]),
         "roofline": ({"bound": "hbm", "kernel": name, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                       "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,

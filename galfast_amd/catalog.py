@@ -27,7 +27,7 @@ class gf_run_stats(C.Structure):
                 ("n_emit", C.c_uint64), ("n_chain", C.c_uint64), ("t_plan", C.c_double),
                 ("t_emit", C.c_double), ("t_chain", C.c_double), ("t_total", C.c_double),
                 ("visible", C.c_uint64), ("t_stage", C.c_double * 6),
-                ("t_emit_rng", C.c_double), ("t_emit_rows", C.c_double)]
+                ("t_emit_stars", C.c_double), ("t_emit_units", C.c_double)]
 
 
 def _declare():

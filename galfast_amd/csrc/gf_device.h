@@ -114,6 +114,34 @@ GF_D void lambert_deproject(const gf_lambert &p, double &l, double &b, const flo
 	l = p.l0 + gfm::atan2((double)x * sc, (double)p.cb * cc / ir - (double)ysb * sc);
 }
 
+// lambert_deproject that also gives the direction cosines (cos b cos l,
+// cos b sin l, sin b) of the point from the deprojection's own terms: with
+// rho = 1/ir, cos b sin(l - l0) = x sc / rho and cos b cos(l - l0) =
+// (cb cc rho - y sb sc) / rho (the atan2 arguments over rho), sin b the asin
+// argument; rotated by l0 with the projection's cos l0, sin l0.
+GF_D void lambert_deproject_dir(const gf_lambert &p, double &l, double &b, double &cbcl, double &cbsl, double &sb,
+                                const float x, const float y)
+{
+	double r2 = (double)(x * x + y * y);
+	if (r2 == 0.) {
+		b = p.b0; l = 0.;
+		cbcl = (double)p.cb; cbsl = 0.; sb = (double)p.sb;
+		return;
+	}
+	double ir = 1. / sqrt(r2);
+	double sc = 1. / ir * sqrt(1. - 0.25 * r2);
+	double cc = 1. - 0.5 * r2;
+	const double S = cc * (double)p.sb + (double)y * sc * (double)p.cb * ir;
+	b = gfm::asin(S);
+	float ysb = y * p.sb;
+	const double P = (double)x * sc, Q = (double)p.cb * cc / ir - (double)ysb * sc;
+	l = p.l0 + gfm::atan2(P, Q);
+	const double Pr = P * ir, Qr = Q * ir;
+	cbcl = (double)p.cl * Qr - (double)p.sl * Pr;
+	cbsl = (double)p.sl * Qr + (double)p.cl * Pr;
+	sb = S;
+}
+
 static constexpr float POGSON = 0.4605170185988091f;   // 0.2 ln 10 (skygen/skygen.cu.h:466)
 static constexpr double DBL_R2D = 57.2957795130823208767981548;
 static constexpr double DBL_D2R = 0.01745329251994329576923691;
@@ -330,14 +358,6 @@ GF_D int lf_bin(const double *P, const uint16_t *G, double scale, int lo, int hi
 	return a;
 }
 
-// A star parked by the random half of the emit for the row-parallel half:
-// position in the projection plane, M and m, and the pixel with bit 31 set
-// when the star fell outside the projection (hidden).
-struct gf_parked {
-	float4 *xyMm;
-	uint32_t *pix;
-};
-
 // The random part of one star of draw_stars (skygen/skygen.cu.h:376-385,
 // 404-405): position inside the pixel (2 uniforms) and, when that stays
 // inside the projection, the absolute and apparent magnitude (2 more).
@@ -377,20 +397,16 @@ GF_D void write_star(const gf_skygen_params &sp, int comp, const gf_extinction &
                      const gf_pencil_beam &pix, float x, float y, float Mtmp, float mtmp, float AmMin,
                      const gf_skygen_cols &c, uint32_t row)
 {
-	double l, b;
-	lambert_deproject(sp.proj[pix.projIdx], l, b, x, y);
+	double l, b, cbcl, cbsl, sbd;
+	lambert_deproject_dir(sp.proj[pix.projIdx], l, b, cbcl, cbsl, sbd, x, y);
 	int outside = outside_footprint<FULL>(foot, l, b);
-	double sld, cld, sbd, cbd;
-	gfm::sincos(l, &sld, &cld);
-	gfm::sincos(b, &sbd, &cbd);
-	float dcl = (float)cld, dcb = (float)cbd, dsl = (float)sld, dsb = (float)sbd;
 	l *= DBL_R2D;
 	if (l < 0.) l += 360.;
 	if (l > 360.) l -= 360.;
 	b *= DBL_R2D;
 	float DM = mtmp - Mtmp;
 	float D = gf_pow10f(0.2f * DM + 1.f);
-	float X = D * dcl * dcb, Y = D * dsl * dcb, Z = D * dsb;
+	float X = D * (float)cbcl, Y = D * (float)cbsl, Z = D * (float)sbd;
 	float Am0 = 0.f, Am1 = 0.f;
 	const gf_tex3d &em = pix.projIdx == 0 ? ext.north : ext.south;
 	if (em.data) {
@@ -415,5 +431,67 @@ GF_D void write_star(const gf_skygen_params &sp, int comp, const gf_extinction &
 	col<float>(c.Am, row) = Am0;
 	col<float>(c.AmInf, row) = AmInf;
 	col<int>(c.hidden, row) = hidden;
+}
+
+// The cumulative-table draw's use of the streams.  Unit u = p*K + k of a
+// model belongs to stream s = u mod ns with index i = u div ns; from the
+// streams' states at the plan, the unit's Poisson draw starts i*2^32 draws
+// on, its star j (0-based within the unit) i*2^32 + 2^31 + 8 j draws on (a
+// star takes 3 or 5), and once the model is planned the streams move
+// ceil(U / ns) * 2^32 draws on, past every unit.  Positions are reached by
+// jump-ahead (gf_mwc_mod) with the streams' jump tables.
+constexpr int GF_UNIT_SPAN_BITS = 32;
+constexpr int GF_EMIT_AT_BIT = 31;
+constexpr int GF_STAR_SPAN_BITS = 3;
+
+// The absolute-magnitude window and minimum extinction of a unit that holds
+// stars (its distance cut passed): DM bin k's first cell in crawl order as
+// cell_pos reads it, then dm_window; the distance itself only when a
+// distance cut is configured.
+GF_D bool unit_window(const gf_skygen_params &sp, const gf_extinction &ext, const gf_pencil_beam &pix, int k,
+                      float &Am, int &lo, int &hi)
+{
+	const int iM0 = k < sp.nM - 1 ? k : sp.nM - 1;
+	const int im0 = k - iM0;
+	const float M = sp.M1 - iM0 * sp.dM;
+	const float m = sp.m0 + im0 * sp.dm;
+	const float DM = m - M;
+	const int xidx = pix.extIdx / 2048, yidx = pix.extIdx % 2048;
+	Am = ext.beam.data ? tex3d(ext.beam, (float)xidx, (float)yidx, DM) : 0.f;
+	const float D = (sp.dmin != 0.f || sp.dmax != 0.f) ? gf_pow10f(0.2f * DM + 1.f) : 0.f;
+	return dm_window(sp, k, D, Am, lo, hi);
+}
+
+// The absolute-magnitude bin of a draw by binary search over the FP64 LF
+// prefix P: the smallest a in [lo, hi] with P[a + 1] > tgt, else hi (the
+// bin lf_bin_pairs finds with its guess table).
+GF_D int lf_bin_search(const double *P, int lo, int hi, double tgt)
+{
+	int a = lo, b = hi;
+	while (a < b) {
+		const int mid = (a + b) >> 1;
+		if (P[mid + 1] > tgt) b = mid; else a = mid + 1;
+	}
+	return a;
+}
+
+// One star of DM bin k: its absolute-magnitude bin from the FP64 cumulative
+// LF window [lo, hi] (one uniform; bin(lo, hi, tgt) finds it in P), the
+// draws of draw_stars and the deterministic rest, written to row `row` of
+// out.
+template <bool FULL, typename Bin>
+GF_D void emit_star(const gf_skygen_params &sp, int comp, const gf_extinction &ext, const gf_footprint &foot,
+                    const gf_pencil_beam &pix, int k, int lo, int hi, float Am, const double *P, Bin bin,
+                    gf_mwc &rng, const gf_skygen_cols &out, uint32_t row)
+{
+	const double tgt = P[lo] + (double)rng.uniform() * (P[hi + 1] - P[lo]);
+	const int iM = bin(lo, hi, tgt);
+	const float M = sp.M1 - iM * sp.dM;
+	float x, y, Mt = 0.f, mt = 0.f;
+	if (!star_draws(sp, pix, M, k - iM, rng, x, y, Mt, mt)) {
+		write_star_outside(out, row, pix, comp, x, y);
+		return;
+	}
+	write_star<FULL>(sp, comp, ext, foot, pix, x, y, Mt, mt, Am, out, row);
 }
 

@@ -12,9 +12,11 @@
 //
 //   expf, logf (the samplers, the density models): float arithmetic, see
 //         gf_expf / gf_logf; at most 1 ulp from the correctly rounded value.
-//   exp10f, log10f, powf, sinf, cosf, tanf: evaluated in IEEE double and
-//         rounded once to float (the argument scaling of exp10/pow needs the
-//         extra precision):
+//   exp10f (and the distance modulus's 10^(0.2 DM + 1)): float arithmetic,
+//         see gf_pow10f; at most 1 ulp.
+//   log10f, powf, sinf, cosf, tanf: evaluated in IEEE double and rounded
+//         once to float (the argument scaling of pow needs the extra
+//         precision):
 //     exp:  Cody-Waite reduction x = k*ln2 + r, |r| <= ln2/2, Taylor series
 //           of e^r to degree 10 (truncation < 3e-13 relative), scaling by 2^k.
 //     log:  x = 2^e * m, m in [sqrt(1/2), sqrt(2)), s = (m-1)/(m+1),
@@ -278,7 +280,35 @@ GF_HD float gf_logf(float x)
 	float ef = (float)e;
 	return __builtin_fmaf(ef, LN2_HI, -((hfsq - __builtin_fmaf(s, hfsq + R, ef * LN2_LO)) - f));
 }
-GF_HD float gf_exp10f(float x) { return (float)gfm::exp((double)x * gfm::LN10); }
+// 10^y in float arithmetic (Cody and Waite's argument reduction): y log2(10)
+// = k + r with k = rint(y Lh), r = (y Lh - k) + y Ll, log2(10) split into a
+// float head Lh and tail Ll (the fma forms y Lh - k with one rounding);
+// 2^r = e^(r ln 2) by its Taylor series to degree 7 (|r ln 2| <= 0.35:
+// truncation below 1e-8 relative); times 2^k.  Within 1 ulp of the
+// correctly rounded value.
+GF_HD float gf_pow10f(float y)
+{
+	const float Lh = 0x1.a934fp+1f, Ll = 0x1.2f346ep-24f;
+	if (y != y) return y;
+	if (y > 38.5318413f) return gfm::as_float(0x7f800000u);
+	if (y < -45.3534698f) return 0.f;
+	const float k = __builtin_rintf(y * Lh);
+	float r = __builtin_fmaf(y, Lh, -k);
+	r = __builtin_fmaf(y, Ll, r);
+	float p = 0x1.ffcbfcp-17f;                 // ln2^7 / 7!
+	p = __builtin_fmaf(p, r, 0x1.430912p-13f); // ln2^6 / 6!
+	p = __builtin_fmaf(p, r, 0x1.5d87fep-10f); // ln2^5 / 5!
+	p = __builtin_fmaf(p, r, 0x1.3b2ab6p-7f);  // ln2^4 / 4!
+	p = __builtin_fmaf(p, r, 0x1.c6b08ep-5f);  // ln2^3 / 3!
+	p = __builtin_fmaf(p, r, 0x1.ebfbe0p-3f);  // ln2^2 / 2!
+	p = __builtin_fmaf(p, r, 0x1.62e430p-1f);  // ln2
+	p = __builtin_fmaf(p, r, 1.f);
+	int ki = (int)k;
+	if (ki > 127) { p *= gfm::as_float(0x7f000000u); ki -= 127; }
+	if (ki < -126) { p *= gfm::as_float(0x00800000u); ki += 126; }
+	return p * gfm::as_float((uint32_t)(ki + 127) << 23);
+}
+GF_HD float gf_exp10f(float x) { return gf_pow10f(x); }
 GF_HD float gf_log10f(float x) { return (float)(gfm::log((double)x) * gfm::INV_LN10); }
 GF_HD float gf_sinf(float x) { double s, c; gfm::sincos((double)x, &s, &c); return (float)s; }
 GF_HD float gf_cosf(float x) { double s, c; gfm::sincos((double)x, &s, &c); return (float)c; }
@@ -296,16 +326,6 @@ GF_HD float gf_powf_nonneg(float x, float y, LogX logx)
 	if (x != x || y != y) return x + y;
 	if (x == 0.0f) return y > 0.0f ? 0.0f : gfm::as_float(0x7f800000u);
 	return (float)gfm::exp((double)y * logx());
-}
-
-// powf(10, y), the distance modulus's 10^(0.2 DM + 1): gf_powf(10, y) takes
-// the log of 10 in double, which is LN10 exactly (0x40026bb1bbb55516), so
-// this is the same exp of the same product without computing the log.
-GF_HD float gf_pow10f(float y)
-{
-	if (y == 0.0f) return 1.0f;
-	if (y != y) return 10.0f + y;
-	return (float)gfm::exp((double)y * gfm::LN10);
 }
 
 // x^y for float x >= 0 (and the integer-y cases of x < 0)

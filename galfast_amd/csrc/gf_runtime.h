@@ -38,14 +38,17 @@ int exclusive_scan_u32_u64(const uint32_t *in, uint64_t *offs, uint64_t n, hipSt
 
 struct gf_chain_fwd;
 // gf_skygen_emit with optional timing: ev holds 3 events, ms[0] and ms[1]
-// receive the durations of the random half (k_emit_rng) and the row half
-// (k_emit_rows) (and make the call wait for them).  On the extinction path
-// (one fused kernel) the fused kernel's time is the first and the second is
-// 0.  unit: the plan unit holding row_begin (from
-// skygen_chunk_units), or UINT64_MAX to look it up, which waits for the
-// stream.
+// receive the durations of the emission (k_emit_stars) and of the lookup of
+// its tiles' units (k_tile_units) (and make the call wait for them).  unit:
+// a unit at or below the one holding row_begin (from skygen_chunk_units),
+// or UINT64_MAX to search from unit 0.
 int skygen_emit(const gf_draw_plan *plan, uint64_t row_begin, uint64_t row_end, const gf_footprint *foot,
                 gf_rng *rng, const gf_skygen_cols *out, hipStream_t st, hipEvent_t *ev, float *ms, uint64_t unit);
+// gf_skygen_plan into *plan, reusing its buffers when *plan is non-NULL
+// (creating it otherwise).
+int skygen_plan(const gf_skygen_params *sp, const gf_model *model, const gf_pencil_beam *pixels,
+                const gf_extinction *ext, const double *wpk, gf_rng *rng, gf_draw_plan **plan, uint64_t *nstars,
+                hipStream_t st);
 // units[q] = the plan unit holding row q * chunk, q < nq (waits for the stream).
 int skygen_chunk_units(const gf_draw_plan *plan, uint64_t chunk, uint32_t nq, uint64_t *units, hipStream_t st);
 

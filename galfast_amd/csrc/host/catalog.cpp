@@ -100,6 +100,8 @@ struct Model {
 	int user_comp = 0;
 	gf_count_result counts{};
 	DevBuf wpk;
+	gf_draw_plan *plan = nullptr;   // the cumulative-table draw's plan, reused across runs
+	~Model() { gf_skygen_plan_free(plan); }
 };
 
 struct Footprint {
@@ -172,6 +174,8 @@ struct gf_catalog {
 	hipStream_t stream = nullptr;    // stream-order work: plans, emission, draw passes
 	hipStream_t rows = nullptr;      // the module chain's row passes
 	hipStream_t side = nullptr;      // kinematics and astrometry row passes, beside photometry
+	hipStream_t emit = nullptr;      // the cumulative-table draw's emission (independent of the draw passes)
+	hipEvent_t planned = nullptr, emitted[2] = {};
 	hipEvent_t ev[2] = {nullptr, nullptr};
 	// Two output chunks are in flight: while the row passes of chunk k run
 	// on `rows`, chunk k+1 is emitted and drawn on `stream` into the other
@@ -209,8 +213,12 @@ struct gf_catalog {
 		for (auto e : side_done) if (e) (void)hipEventDestroy(e);
 		for (auto e : pos_free_side) if (e) (void)hipEventDestroy(e);
 		if (vis_host) (void)hipHostFree(vis_host);
+		if (planned) (void)hipEventDestroy(planned);
+		for (auto e : emitted) if (e) (void)hipEventDestroy(e);
+		models.clear();   // plans hold device buffers: freed before the streams go
 		if (rows) (void)hipStreamDestroy(rows);
 		if (side) (void)hipStreamDestroy(side);
+		if (emit) (void)hipStreamDestroy(emit);
 	}
 };
 
@@ -968,8 +976,12 @@ void open_catalog(gf_catalog &c, const std::string &cmdfile)
 
 	c.pix_dev.upload(c.pixels);
 	if (hipStreamCreate(&c.stream) != hipSuccess || hipStreamCreate(&c.rows) != hipSuccess ||
-	    hipStreamCreate(&c.side) != hipSuccess)
+	    hipStreamCreate(&c.side) != hipSuccess || hipStreamCreate(&c.emit) != hipSuccess)
 		fail("stream creation failed");
+	if (hipEventCreateWithFlags(&c.planned, hipEventDisableTiming) != hipSuccess ||
+	    hipEventCreateWithFlags(&c.emitted[0], hipEventDisableTiming) != hipSuccess ||
+	    hipEventCreateWithFlags(&c.emitted[1], hipEventDisableTiming) != hipSuccess)
+		fail("event creation failed");
 	(void)hipEventCreate(&c.ev[0]);
 	(void)hipEventCreate(&c.ev[1]);
 	for (auto &sl : c.slot)
@@ -1187,9 +1199,9 @@ int deliver(gf_catalog &c, int si, gf_chunk_fn fn, void *user, gf_run_stats &s)
 	sl.pending = false;
 	s.visible += c.vis_host[si];
 	const hipEvent_t *e = sl.ev.data();
-	double a = elapsed_s(e[0], e[1]), b = elapsed_s(e[1], e[2]);
-	s.t_emit_rng += a;
-	s.t_emit_rows += b;
+	double a = elapsed_s(e[0], e[1]), b = elapsed_s(e[1], e[2]);   // the tiles' units, then the stars
+	s.t_emit_units += a;
+	s.t_emit_stars += b;
 	s.t_emit += a + b;
 	for (int j = 0; j < sl.nsub; j++) {
 		const hipEvent_t *q = e + EV_EMIT + EV_CALL * j;
@@ -1217,17 +1229,13 @@ int run_catalog(gf_catalog &c, gf_chunk_fn fn, void *user, gf_run_stats *stats)
 	gf_run_stats s{};
 	double t0 = now_s();
 	hipStream_t A = c.stream, B = c.overlap ? c.rows : c.stream, C = c.overlap ? c.side : c.stream;
+	hipStream_t E = c.overlap ? c.emit : c.stream;
 	const gf_chain ch = chain_of(c);
 	const uint64_t sub = 65536ull * 256ull;   // rows the 65,536 streams cover per chain call
 	const size_t pos_bytes = gf::chain_positions_bytes(&ch, (uint32_t)sub);
 	for (auto &pb : c.pos)
 		if (pos_bytes && pb.bytes < pos_bytes) pb.alloc(pos_bytes);
 	unsigned long long *vis_dev = (unsigned long long *)c.vis_dev.p;
-	auto release = [](gf_draw_plan *p) {
-		(void)hipDeviceSynchronize();
-		gf_skygen_plan_free(p);
-	};
-	std::vector<std::unique_ptr<gf_draw_plan, decltype(release)>> plans;
 	uint32_t pb = 0;      // position buffer of the next chain call
 	uint32_t last = 0;    // position buffer of the last chain call queued
 	uint64_t k = 0;       // chunks queued
@@ -1235,17 +1243,18 @@ int run_catalog(gf_catalog &c, gf_chunk_fn fn, void *user, gf_run_stats *stats)
 	for (auto &mp : c.models) {
 		Model &m = *mp;
 		if (!m.wpk.p) return GF_E_STATE;
-		gf_draw_plan *plan = nullptr;
 		uint64_t total = 0;
+		// a model's plan is rewritten only by its next run, which starts
+		// after every stream of this one has finished
 		GF_HIP(hipEventRecord(c.ev[0], A));
-		rc = gf_skygen_plan(&c.sp, &m.m, (const gf_pencil_beam *)c.pix_dev.p, c.has_ext ? &c.ext : nullptr,
-		                    (const double *)m.wpk.p, &c.rng, &plan, &total, A);
+		rc = gf::skygen_plan(&c.sp, &m.m, (const gf_pencil_beam *)c.pix_dev.p, c.has_ext ? &c.ext : nullptr,
+		                     (const double *)m.wpk.p, &c.rng, &m.plan, &total, A);
 		if (rc) return rc;
-		// The plan is read by the emission kernels on the catalog stream; it
-		// is freed once the run is over, because freeing device memory waits
-		// for the whole device and would stall the next model's plan behind
-		// this model's last row passes.
-		plans.emplace_back(plan, release);
+		gf_draw_plan *plan = m.plan;
+		// the emission reads the plan and the streams as they were at the
+		// plan (kept in the plan), never the streams the draw passes advance
+		GF_HIP(hipEventRecord(c.planned, A));
+		GF_HIP(hipStreamWaitEvent(E, c.planned, 0));
 		GF_HIP(hipEventRecord(c.ev[1], A));
 		GF_HIP(hipEventSynchronize(c.ev[1]));
 		s.t_plan += elapsed_s(c.ev[0], c.ev[1]);
@@ -1273,8 +1282,10 @@ int run_catalog(gf_catalog &c, gf_chunk_fn fn, void *user, gf_run_stats *stats)
 			const hipEvent_t *e = sl.ev.data();
 			gf_skygen_cols sc{sl.table.lb, sl.table.projIdx, sl.table.projXY, sl.table.DM, sl.table.absmag,
 			                  sl.table.XYZ, sl.table.Am, sl.table.AmInf, sl.table.comp, sl.table.hidden};
-			rc = gf::skygen_emit(plan, r0, r1, &c.foot.f, &c.rng, &sc, A, sl.ev.data(), nullptr, units[q]);
+			rc = gf::skygen_emit(plan, r0, r1, &c.foot.f, &c.rng, &sc, E, sl.ev.data(), nullptr, units[q]);
 			if (rc) return rc;
+			GF_HIP(hipEventRecord(c.emitted[si], E));
+			GF_HIP(hipStreamWaitEvent(A, c.emitted[si], 0));
 			s.n_emit++;
 			for (int j = 0; j < nsub; j++, pb ^= 1) {
 				const uint64_t b0 = (uint64_t)j * sub;
@@ -1335,6 +1346,7 @@ int run_catalog(gf_catalog &c, gf_chunk_fn fn, void *user, gf_run_stats *stats)
 	GF_HIP(hipStreamSynchronize(A));
 	GF_HIP(hipStreamSynchronize(B));
 	GF_HIP(hipStreamSynchronize(C));
+	GF_HIP(hipStreamSynchronize(E));
 	s.t_total = now_s() - t0;
 	if (stats) *stats = s;
 	return 0;
@@ -1714,7 +1726,7 @@ struct SReq {
 
 struct Worker {
 	hipStream_t st = nullptr;
-	DevBuf W, counts, offs, pixsum, segexp, pix, state, segs, segall, reqs, table, pos, park, vis;
+	DevBuf W, counts, offs, pixsum, segexp, pix, state, segs, segall, reqs, table, pos, zu, vis;
 	uint64_t table_cap = 0;
 	gf_table T{};
 	std::thread th;
@@ -1869,12 +1881,18 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 			g.S = q.S;
 			g.model = m;
 			g.req = r;
+			// the models of a request use its streams in turn
+			g.i0 = (uint32_t)((uint64_t)m * ((q.nunits_model + q.S - 1) / q.S));
 			segs[(size_t)m * nr + r] = g;
 			segall[(size_t)r * nm + m] = g;
 		}
 	}
 	GF_HIP(hipMemcpyAsync(w.segs.p, segs.data(), segs.size() * sizeof(gf_serve_seg), hipMemcpyHostToDevice, st));
+	GF_HIP(hipMemcpyAsync(w.segall.p, segall.data(), segall.size() * sizeof(gf_serve_seg), hipMemcpyHostToDevice, st));
 	gf_serve_seg *dsegs = (gf_serve_seg *)w.segs.p;
+	gf_serve_seg *dsegall = (gf_serve_seg *)w.segall.p;
+	const uint64_t *jtab = (const uint64_t *)s.base.jtab.p;
+	grow<uint64_t>(w.zu, U + 1);
 	const gf_model *dmodels = (const gf_model *)s.dmodels.p;
 	const gf_pencil_beam *dpix = (const gf_pencil_beam *)w.pix.p;
 	// count
@@ -1885,17 +1903,13 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	}
 	std::vector<double> segexp((size_t)nm * nr);
 	GF_HIP(hipMemcpyAsync(segexp.data(), w.segexp.p, segexp.size() * 8, hipMemcpyDeviceToHost, st));
-	// plan (one launch per model: a request's models use its streams in turn)
-	for (int m = 0; m < nm; m++) {
-		int rc = gf::serve_plan(dsegs + (size_t)m * nr, nr, loff[nr], (const double *)w.W.p, (uint32_t *)w.counts.p, st);
-		if (rc) return rc;
-	}
-	int rc = gf::exclusive_scan_u32_u64((const uint32_t *)w.counts.p, (uint64_t *)w.offs.p, U, st);
+	// plan: every unit of the batch at once (a segment's units start at its
+	// models' offset on its request's streams)
+	int rc = gf::serve_plan(dsegall, nr * nm, U, (const double *)w.W.p, jtab, (uint32_t *)w.counts.p,
+	                        (uint64_t *)w.zu.p, st);
 	if (rc) return rc;
-	for (int m = 0; m < nm; m++) {
-		rc = gf::serve_seg_rows(dsegs + (size_t)m * nr, nr, (const uint64_t *)w.offs.p, st);
-		if (rc) return rc;
-	}
+	rc = gf::exclusive_scan_u32_u64((const uint32_t *)w.counts.p, (uint64_t *)w.offs.p, U, st);
+	if (rc) return rc;
 	std::vector<uint64_t> roff(nr + 1);
 	for (int r = 0; r <= nr; r++)
 		GF_HIP(hipMemcpyAsync(&roff[r], (const uint64_t *)w.offs.p + uoff[r], 8, hipMemcpyDeviceToHost, st));
@@ -1907,29 +1921,27 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	const uint64_t nrows = roff[nr];
 	if (nrows >= (1ull << 32)) return GF_E_BUDGET;
 	worker_table(s, w, nrows);
-	GF_HIP(hipMemcpyAsync(w.segall.p, segall.data(), segall.size() * sizeof(gf_serve_seg), hipMemcpyHostToDevice, st));
 	// row0/row1 of the row-ordered table from the scanned offsets
-	rc = gf::serve_seg_rows((gf_serve_seg *)w.segall.p, nr * nm, (const uint64_t *)w.offs.p, st);
+	rc = gf::serve_seg_rows(dsegall, nr * nm, (const uint64_t *)w.offs.p, st);
 	if (rc) return rc;
 	gf_skygen_cols sc{w.T.lb, w.T.projIdx, w.T.projXY, w.T.DM, w.T.absmag, w.T.XYZ, w.T.Am, w.T.AmInf, w.T.comp, w.T.hidden};
-	grow<char>(w.park, std::max<size_t>(1, (size_t)nrows * (sizeof(float4) + sizeof(uint32_t))));
-	gf_parked park{(float4 *)w.park.p, (uint32_t *)((char *)w.park.p + (size_t)nrows * sizeof(float4))};
-	for (int m = 0; m < nm; m++) {
-		rc = gf::serve_emit(dsegs + (size_t)m * nr, nr, loff[nr], s.base.sp.nM, dpix, (const uint64_t *)w.offs.p,
-		                    park, st);
-		if (rc) return rc;
-	}
-	rc = gf::serve_emit_rows((const gf_serve_seg *)w.segall.p, nr * nm, dmodels, dpix, 0, nrows, park, sc, st);
+	rc = gf::serve_emit(dsegall, nr * nm, dmodels, dpix, (const uint64_t *)w.offs.p, (const uint64_t *)w.zu.p, jtab,
+	                    nrows, sc, st);
 	if (rc) return rc;
 	std::vector<gf_serve_req> rq(nr);
 	for (int r = 0; r < nr; r++) {
+		const SReq &q = *batch[r];
 		rq[r].state = dstate + soff[r];
 		rq[r].row0 = roff[r];
 		rq[r].row1 = roff[r + 1];
 		rq[r].lane0 = loff[r];
-		rq[r].S = batch[r]->S;
+		rq[r].S = q.S;
+		rq[r].advance = ((uint64_t)nm * ((q.nunits_model + q.S - 1) / q.S)) << GF_UNIT_SPAN_BITS;
 	}
 	GF_HIP(hipMemcpyAsync(w.reqs.p, rq.data(), rq.size() * sizeof(gf_serve_req), hipMemcpyHostToDevice, st));
+	// the streams move past every unit before the module chain draws from them
+	rc = gf::serve_advance((const gf_serve_req *)w.reqs.p, nr, loff[nr], jtab, st);
+	if (rc) return rc;
 	grow<uint2>(w.pos, std::max<size_t>(1, (size_t)s.hchain.nplanes * nrows));
 	if (s.cfg.eager_chain) {
 		rc = gf::serve_chain((const gf_serve_req *)w.reqs.p, nr, loff[nr], (const gf_serve_chain *)s.dchain.p,

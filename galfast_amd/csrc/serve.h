@@ -18,11 +18,11 @@ struct gf_serve_seg {
 	uint64_t row0, row1;    // output rows (after the plan scan)
 	uint32_t pix0, npix;    // the request's pixels in the batch pixel array
 	uint32_t slot0;         // first count-kernel workgroup of the segment in its launch
-	uint32_t lane0;         // first plan/emit lane of the segment in its launch
+	uint32_t lane0;         // first lane of the segment's request in per-stream launches
 	uint32_t S;             // streams of the request
 	int32_t model;          // index into the model array
 	int32_t req;            // request index in the batch
-	int32_t pad;
+	uint32_t i0;            // unit index of the segment's first unit on its streams (models before it)
 };
 
 // One request of a batch, for the module chain.
@@ -31,6 +31,7 @@ struct gf_serve_req {
 	uint64_t row0, row1;
 	uint32_t lane0;
 	uint32_t S;
+	uint64_t advance;       // draws its streams move on once all its models are planned
 };
 
 // Module parameter blocks of the server (device copy).
@@ -50,14 +51,18 @@ struct gf_serve_chain_args {
 namespace gf {
 int serve_count(const gf_serve_seg *dsegs, int nseg, uint32_t nslots, const gf_model *dmodels,
                 const gf_pencil_beam *dpix, double *W, double *pix_sum, double *seg_expected, hipStream_t st);
-int serve_plan(const gf_serve_seg *dsegs, int nseg, uint32_t nlanes, const double *W, uint32_t *counts,
-               hipStream_t st);
+// Plan of a batch: lane per unit over all nseg segments (in unit order);
+// counts and zu (the states the units' stars start from) per batch unit.
+// jtab: jump tables of the first 65,536 streams (stride 65,536).
+int serve_plan(const gf_serve_seg *dsegs, int nseg, uint64_t nunits, const double *W, const uint64_t *jtab,
+               uint32_t *counts, uint64_t *zu, hipStream_t st);
 int serve_seg_rows(gf_serve_seg *dsegs, int nseg, const uint64_t *offs, hipStream_t st);
-// park: n rows of parked stars (device scratch of the caller)
-int serve_emit(const gf_serve_seg *dsegs, int nseg, uint32_t nlanes, int nM, const gf_pencil_beam *dpix,
-               const uint64_t *offs, const gf_parked &park, hipStream_t st);
-int serve_emit_rows(const gf_serve_seg *dsegs, int nseg, const gf_model *dmodels, const gf_pencil_beam *dpix,
-                    uint64_t row0, uint64_t row1, const gf_parked &park, const gf_skygen_cols &out, hipStream_t st);
+// Emission: lane per batch row over all segments (in row order).
+int serve_emit(const gf_serve_seg *dsegs, int nseg, const gf_model *dmodels, const gf_pencil_beam *dpix,
+               const uint64_t *offs, const uint64_t *zu, const uint64_t *jtab, uint64_t nrows,
+               const gf_skygen_cols &out, hipStream_t st);
+// Every request's streams move on by its `advance` (nlanes = its streams in all).
+int serve_advance(const gf_serve_req *dreqs, int nreq, uint32_t nlanes, const uint64_t *jtab, hipStream_t st);
 // pos: hch.nplanes * nrows stream positions (device scratch of the caller)
 int serve_chain(const gf_serve_req *dreqs, int nreq, uint32_t nlanes, const gf_serve_chain *dch,
                 const gf_serve_chain &hch, uint64_t nrows, const gf_table &T, uint2 *pos, hipStream_t st);

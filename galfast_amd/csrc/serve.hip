@@ -81,99 +81,92 @@ __global__ void k_b_seg_expected(const gf_serve_seg *segs, int nseg, const doubl
 }
 
 // ---------------------------------------------------------------------------
-// Plan: lane (segment, s) draws the units u = s, s + S, ... of its segment
-// from stream s of the segment's request.
+// Plan: lane per batch unit.  Unit u of a segment belongs to stream u mod S
+// of its request with index i0 + u div S (the cumulative-table draw's
+// sub-streams, gf_device.h): its Poisson count and the state its stars
+// start from, from the request's streams as they were at the batch start.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(SB) k_b_plan(const gf_serve_seg *segs, int nseg, uint32_t nlanes, const double *W,
-                                               uint32_t *counts)
+constexpr uint32_t JSTRIDE = 65536;   // stride of the server's jump tables
+
+__global__ void __launch_bounds__(SB) k_b_plan_units(const gf_serve_seg *segs, int nseg, uint64_t nunits,
+                                                     const double *W, const uint64_t *jtab, uint32_t *counts,
+                                                     uint64_t *zu)
 {
-	uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-	if (g >= nlanes) return;
-	int si = upper_index(segs, nseg, [](const gf_serve_seg &s) { return (uint64_t)s.lane0; }, g);
-	const gf_serve_seg &sg = segs[si];
-	uint32_t s = g - sg.lane0;
-	if (s >= sg.S) return;
-	gf_mwc rng = gf_mwc_load(sg.state, sg.S, s);
-	for (uint64_t u = s; u < sg.nunits; u += sg.S) {
-		double w = W[sg.unit0 + u];
-		counts[sg.unit0 + u] = w > 0.0 ? rng.poisson((float)w) : 0u;
+	const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (g >= nunits) return;
+	const double w = W[g];
+	uint32_t n = 0;
+	if (w > 0.0) {
+		const int si = upper_index(segs, nseg, [](const gf_serve_seg &q) { return q.unit0; }, g);
+		const gf_serve_seg &sg = segs[si];
+		const uint64_t u = g - sg.unit0;
+		const uint32_t s = (uint32_t)(u % sg.S);
+		const uint64_t i = sg.i0 + u / sg.S;
+		gf_mwc r = gf_mwc_load(sg.state, sg.S, s);
+		const gf_mwc_mod md(r.a);
+		const uint64_t z = gf_mwc_jump_z(md, gf_mwc_z(r), i << GF_UNIT_SPAN_BITS, jtab + s, JSTRIDE);
+		gf_mwc_set_z(r, z);
+		n = r.poisson((float)w);
+		zu[g] = md.mul(z, jtab[(size_t)GF_EMIT_AT_BIT * JSTRIDE + s]);
 	}
-	gf_mwc_store(sg.state, sg.S, s, rng);
+	counts[g] = n;
 }
 
 // ---------------------------------------------------------------------------
-// Emit, random half: lane (segment, s) walks its units and parks the draws
-// of each star (as k_emit_rng does for the catalog).
+// Emission: lane per batch row.  The row's segment, its unit (binary search
+// over the segment's offsets), the star's state 8 draws per earlier star of
+// the unit on, and the star; the LF bin by binary search over the model's
+// prefix.  A request's footprint is one Galactic beam; no extinction.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(SB) k_b_emit_rng(const gf_serve_seg *segs, int nseg, uint32_t nlanes,
-                                                   const gf_pencil_beam *pixels, const uint64_t *offs,
-                                                   gf_parked park)
+__global__ void __launch_bounds__(SB) k_b_emit_stars(const gf_serve_seg *segs, int nseg, const gf_model *models,
+                                                     const gf_pencil_beam *pixels, const uint64_t *offs,
+                                                     const uint64_t *zu, const uint64_t *jtab, uint64_t nrows,
+                                                     gf_skygen_cols out)
 {
-	// every segment of a launch is of one model: its LF prefix and the bin
-	// guess table go to LDS
-	extern __shared__ double P[];
-	const int nM = segs[0].sp.nM;
-	uint16_t *G = (uint16_t *)(P + nM + 1);
-	for (int i = threadIdx.x; i <= nM; i += blockDim.x) P[i] = segs[0].lfP[i];
-	__syncthreads();
-	lf_guess_build(P, nM, G, threadIdx.x, blockDim.x);
-	__syncthreads();
-	const double gscale = lf_guess_scale(P, nM);
-	uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-	if (g >= nlanes) return;
-	int si = upper_index(segs, nseg, [](const gf_serve_seg &s) { return (uint64_t)s.lane0; }, g);
+	const uint64_t row = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (row >= nrows) return;
+	const int si = upper_index(segs, nseg, [](const gf_serve_seg &q) { return q.row0; }, row);
 	const gf_serve_seg &sg = segs[si];
-	uint32_t s = g - sg.lane0;
-	if (s >= sg.S) return;
 	const gf_skygen_params &sp = sg.sp;
-	const int K = sp.nm + sp.nM - 1;
-	gf_extinction ext{};
-	gf_mwc rng = gf_mwc_load(sg.state, sg.S, s);
-	for (uint64_t u = s; u < sg.nunits; u += sg.S) {
-		uint64_t o0 = offs[sg.unit0 + u], o1 = offs[sg.unit0 + u + 1];
-		if (o1 == o0) continue;
-		int p = (int)(u / K), k = (int)(u % K);
-		const gf_pencil_beam pix = pixels[sg.pix0 + p];
-		float D, Am, x, y, z;
-		int iM0 = k < sp.nM - 1 ? k : sp.nM - 1;
-		cell_pos(sp, ext, pix, sp.M1 - iM0 * sp.dM, k - iM0, D, Am, x, y, z);
-		int lo, hi;
-		dm_window(sp, k, D, Am, lo, hi);
-		double base = P[lo], span = P[hi + 1] - P[lo];
-		for (uint64_t j = o0; j < o1; j++) {
-			double tgt = base + (double)rng.uniform() * span;
-			int a = lf_bin(P, G, gscale, lo, hi, tgt);
-			float M = sp.M1 - a * sp.dM;
-			float sx, sy, Mtmp = 0.f, mtmp = 0.f;
-			bool in = star_draws(sp, pix, M, k - a, rng, sx, sy, Mtmp, mtmp);
-			park.xyMm[j] = make_float4(sx, sy, Mtmp, mtmp);
-			park.pix[j] = (sg.pix0 + (uint32_t)p) | (in ? 0u : 0x80000000u);
-		}
+	uint64_t a = sg.unit0, b = sg.unit0 + sg.nunits;   // offs[a] <= row < offs[b]
+	while (b - a > 1) {
+		const uint64_t mid = a + (b - a) / 2;
+		if (offs[mid] <= row) a = mid; else b = mid;
 	}
-	gf_mwc_store(sg.state, sg.S, s, rng);
+	const uint64_t u = a - sg.unit0;
+	const uint64_t jj = row - offs[a];
+	const uint32_t s = (uint32_t)(u % sg.S);
+	const int K = sp.nm + sp.nM - 1;
+	const int p = (int)(u / K), k = (int)(u % K);
+	gf_mwc r;
+	r.a = sg.state[s];
+	const gf_mwc_mod md(r.a);
+	gf_mwc_set_z(r, gf_mwc_jump_z(md, zu[a], jj << GF_STAR_SPAN_BITS, jtab + s, JSTRIDE));
+	const gf_pencil_beam pix = pixels[sg.pix0 + p];
+	const gf_extinction ext{};
+	float Am;
+	int lo, hi;
+	unit_window(sp, ext, pix, k, Am, lo, hi);
+	const double *P = sg.lfP;
+	emit_star<false>(sp, models[sg.model].comp, ext, sg.foot, pix, k, lo, hi, Am, P,
+	                 [&](int l, int h, double tg) { return lf_bin_search(P, l, h, tg); }, r, out, (uint32_t)row);
 }
 
-// Emit, deterministic half: one lane per batch row.
-__global__ void __launch_bounds__(SB) k_b_emit_rows(const gf_serve_seg *segs, int nseg, const gf_model *models,
-                                                    const gf_pencil_beam *pixels, uint64_t row0, uint64_t row1,
-                                                    gf_parked park, gf_skygen_cols out)
+// Every request's streams move on past all its units: lane per (request,
+// stream).
+__global__ void __launch_bounds__(SB) k_b_advance(const gf_serve_req *reqs, int nreq, uint32_t nlanes,
+                                                  const uint64_t *jtab)
 {
-	uint64_t row = row0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (row >= row1) return;
-	int si = upper_index(segs, nseg, [](const gf_serve_seg &s) { return s.row0; }, row);
-	const gf_serve_seg &sg = segs[si];
-	uint32_t r = (uint32_t)row;
-	const float4 v = park.xyMm[row];
-	const uint32_t ph = park.pix[row];
-	const gf_pencil_beam pix = pixels[ph & 0x7fffffffu];
-	int comp = models[sg.model].comp;
-	if (ph & 0x80000000u) {
-		write_star_outside(out, r, pix, comp, v.x, v.y);
-		return;
-	}
-	gf_extinction ext{};
-	// a request's footprint is one Galactic beam
-	write_star<false>(sg.sp, comp, ext, sg.foot, pix, v.x, v.y, v.z, v.w, 0.f, out, r);
+	const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+	if (g >= nlanes) return;
+	const int ri = upper_index(reqs, nreq, [](const gf_serve_req &q) { return (uint64_t)q.lane0; }, g);
+	const gf_serve_req &rq = reqs[ri];
+	const uint32_t s = g - rq.lane0;
+	if (s >= rq.S || rq.advance == 0) return;
+	gf_mwc r = gf_mwc_load(rq.state, rq.S, s);
+	const gf_mwc_mod md(r.a);
+	gf_mwc_set_z(r, gf_mwc_jump_z(md, gf_mwc_z(r), rq.advance, jtab + s, JSTRIDE));
+	gf_mwc_store(rq.state, rq.S, s, r);
 }
 
 // Row starts of the segments from the scanned offsets.
@@ -320,11 +313,11 @@ int serve_count(const gf_serve_seg *dsegs, int nseg, uint32_t nslots, const gf_m
 	return 0;
 }
 
-int serve_plan(const gf_serve_seg *dsegs, int nseg, uint32_t nlanes, const double *W, uint32_t *counts,
-               hipStream_t st)
+int serve_plan(const gf_serve_seg *dsegs, int nseg, uint64_t nunits, const double *W, const uint64_t *jtab,
+               uint32_t *counts, uint64_t *zu, hipStream_t st)
 {
-	if (!nlanes) return 0;
-	k_b_plan<<<gf::blocks_for(nlanes, SB), SB, 0, st>>>(dsegs, nseg, nlanes, W, counts);
+	if (!nunits) return 0;
+	k_b_plan_units<<<gf::blocks_for(nunits, SB), SB, 0, st>>>(dsegs, nseg, nunits, W, jtab, counts, zu);
 	GF_LAUNCHED();
 	return 0;
 }
@@ -336,22 +329,20 @@ int serve_seg_rows(gf_serve_seg *dsegs, int nseg, const uint64_t *offs, hipStrea
 	return 0;
 }
 
-int serve_emit(const gf_serve_seg *dsegs, int nseg, uint32_t nlanes, int nM, const gf_pencil_beam *dpix,
-               const uint64_t *offs, const gf_parked &park, hipStream_t st)
+int serve_emit(const gf_serve_seg *dsegs, int nseg, const gf_model *dmodels, const gf_pencil_beam *dpix,
+               const uint64_t *offs, const uint64_t *zu, const uint64_t *jtab, uint64_t nrows,
+               const gf_skygen_cols &out, hipStream_t st)
 {
-	if (!nlanes) return 0;
-	const size_t shb = ((size_t)nM + 1) * sizeof(double) + LF_GUESS * sizeof(uint16_t);
-	k_b_emit_rng<<<gf::blocks_for(nlanes, SB), SB, shb, st>>>(dsegs, nseg, nlanes, dpix, offs, park);
+	if (!nrows) return 0;
+	k_b_emit_stars<<<gf::blocks_for(nrows, SB), SB, 0, st>>>(dsegs, nseg, dmodels, dpix, offs, zu, jtab, nrows, out);
 	GF_LAUNCHED();
 	return 0;
 }
 
-int serve_emit_rows(const gf_serve_seg *dsegs, int nseg, const gf_model *dmodels, const gf_pencil_beam *dpix,
-                    uint64_t row0, uint64_t row1, const gf_parked &park, const gf_skygen_cols &out, hipStream_t st)
+int serve_advance(const gf_serve_req *dreqs, int nreq, uint32_t nlanes, const uint64_t *jtab, hipStream_t st)
 {
-	if (row1 <= row0) return 0;
-	k_b_emit_rows<<<gf::blocks_for(row1 - row0, SB), SB, 0, st>>>(dsegs, nseg, dmodels, dpix, row0, row1, park,
-	                                                               out);
+	if (!nlanes) return 0;
+	k_b_advance<<<gf::blocks_for(nlanes, SB), SB, 0, st>>>(dreqs, nreq, nlanes, jtab);
 	GF_LAUNCHED();
 	return 0;
 }

@@ -8,6 +8,7 @@
 // contributes LF(M) * rho * norm * D^3 * dA * POGSON * dm * dM.
 #include "gf_runtime.h"
 #include "gf_device.h"
+#include <algorithm>
 #include <mutex>
 
 namespace {
@@ -405,36 +406,29 @@ __global__ void __launch_bounds__(64) k_draw_compat(gf_skygen_params sp, gf_mode
 }
 
 // ---------------------------------------------------------------------------
-// Cumulative-table draw.
+// Cumulative-table draw on sub-streams (GF_UNIT_SPAN_BITS in gf_device.h).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(SK_BS) k_plan(const double *wpk, uint64_t U, uint32_t *st, uint32_t ns,
-                                                uint32_t *counts)
+// Plan: lane per unit.  The unit's Poisson count from its own position of
+// its stream, and the state where its stars' draws begin.
+__global__ void __launch_bounds__(SK_BS) k_plan_units(const double *wpk, uint64_t U, const uint32_t *base,
+                                                      uint32_t ns, const uint64_t *jtab, uint32_t *counts,
+                                                      uint64_t *zu)
 {
-	uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-	if (s >= ns) return;
-	__builtin_amdgcn_s_setprio(3);   // the catalog's serial stream first among resident waves
-	gf_mwc rng = gf_mwc_load(st, ns, s);
-	// two units per trip, each unit's weight loaded a trip ahead (one wave
-	// per SIMD has nothing else to hide a load's latency behind); the two
-	// weights keep their registers, so no copy waits on a load in flight,
-	// and every trip issues the same loads (indices clamped to the last
-	// unit), so the waits count them exactly
-	const uint64_t step = 2ull * ns, last = U - 1;
-	double wa = wpk[s < last ? s : last];
-	double wb = wpk[s + (uint64_t)ns < last ? s + ns : last];
-	for (uint64_t u = s; u < U; u += step) {
-		uint32_t n = 0;
-		if (wa > 0.0) n = rng.poisson((float)wa);
-		counts[u] = n;
-		wa = wpk[u + step < last ? u + step : last];
-		const uint64_t v = u + ns;
-		if (v >= U) break;
-		n = 0;
-		if (wb > 0.0) n = rng.poisson((float)wb);
-		counts[v] = n;
-		wb = wpk[v + step < last ? v + step : last];
+	const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (u >= U) return;
+	const double w = wpk[u];
+	uint32_t n = 0;
+	if (w > 0.0) {
+		const uint32_t s = (uint32_t)(u % ns);
+		const uint64_t i = u / ns;
+		gf_mwc r = gf_mwc_load(base, ns, s);
+		const gf_mwc_mod md(r.a);
+		const uint64_t z = gf_mwc_jump_z(md, gf_mwc_z(r), i << GF_UNIT_SPAN_BITS, jtab + s, ns);
+		gf_mwc_set_z(r, z);
+		n = r.poisson((float)w);
+		zu[u] = md.mul(z, jtab[(size_t)GF_EMIT_AT_BIT * ns + s]);
 	}
-	gf_mwc_store(st, ns, s, rng);
+	counts[u] = n;
 }
 
 constexpr int EMIT_MAX_NM = 4096;
@@ -447,136 +441,80 @@ GF_HD size_t lf_pairs_offset(int nM)
 	return (b + 15) & ~(size_t)15;
 }
 
-__global__ void __launch_bounds__(SK_BS) k_emit(gf_skygen_params sp, gf_model m, const gf_pencil_beam *pixels,
-                                                gf_extinction ext, gf_footprint foot, const double *Pg,
-                                                const uint64_t *offs, uint64_t U, uint64_t row_begin,
-                                                uint64_t row_end, uint64_t u_lo, uint32_t *st, uint32_t ns,
-                                                gf_skygen_cols out)
-{
-	extern __shared__ double P[];
-	for (int i = threadIdx.x; i <= sp.nM; i += blockDim.x) P[i] = Pg[i];
-	__syncthreads();
-	uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-	if (s >= ns) return;
-	const int K = sp.nm + sp.nM - 1;
-	uint64_t first = u_lo + ((uint64_t)s + ns - (u_lo % ns)) % ns;
-	gf_mwc rng = gf_mwc_load(st, ns, s);
-	bool touched = false;
-	for (uint64_t u = first; u < U; u += ns) {
-		uint64_t o0 = offs[u], o1 = offs[u + 1];
-		if (o0 >= row_end) break;
-		if (o1 <= row_begin || o1 == o0) continue;
-		touched = true;
-		int p = (int)(u / K), k = (int)(u % K);
-		const gf_pencil_beam pix = pixels[p];
-		float D, Am, x, y, z;
-		dm_bin(sp, ext, pix, k, D, Am, x, y, z);
-		int lo, hi;
-		dm_window(sp, k, D, Am, lo, hi);
-		double base = P[lo], span = P[hi + 1] - P[lo];
-		uint64_t j0 = row_begin > o0 ? row_begin : o0;
-		uint64_t j1 = row_end < o1 ? row_end : o1;
-		StarSink snk{out, true, j0 - row_begin};
-		for (uint64_t j = j0; j < j1; j++) {
-			// absolute-magnitude bin from the cumulative LF window
-			double tgt = base + (double)rng.uniform() * span;
-			int a = lo, b = hi;
-			while (a < b) {
-				int mid = (a + b) >> 1;
-				if (P[mid + 1] > tgt) b = mid; else a = mid + 1;
-			}
-			int iM = a;
-			int im = k - iM;
-			float M = sp.M1 - iM * sp.dM;
-			draw_one_star(sp, m, ext, foot, pix, M, im, Am, rng, snk);
-			snk.row++;
-		}
-	}
-	if (touched) gf_mwc_store(st, ns, s, rng);
-}
+constexpr unsigned EMIT_TILE = SK_BS;   // rows per tile of the emission
 
-// Random half of the emit for maps without extinction: per star the
-// absolute-magnitude bin and the star_draws values, parked in the output
-// columns (projXY = x, y; M = Mtmp; DM = mtmp; comp = pixel index;
-// hidden = 1 outside the projection).  k_emit_rows finishes every row.
-__global__ void __launch_bounds__(SK_BS) k_emit_rng(gf_skygen_params sp, gf_model m, const gf_pencil_beam *pixels,
-                                                    gf_extinction ext, const double *Pg, const uint64_t *offs,
-                                                    uint64_t U, uint64_t row_begin, uint64_t row_end, uint64_t u_lo,
-                                                    uint32_t *st, uint32_t ns, gf_parked park)
+// tu[q] = the unit of the first row of tile q (q < ntiles) and tu[ntiles]
+// = the unit of row row_end - 1: the largest u in [u_lo, U) with
+// offs[u] <= row (offs[u_lo] <= row_begin).
+__global__ void k_tile_units(const uint64_t *offs, uint64_t U, uint64_t row_begin, uint64_t row_end, uint64_t u_lo,
+                             uint32_t ntiles, uint64_t *tu)
 {
-	extern __shared__ double P[];
-	uint16_t *G = (uint16_t *)(P + sp.nM + 1);
-	double2 *GP = (double2 *)((char *)P + lf_pairs_offset(sp.nM));
-	for (int i = threadIdx.x; i <= sp.nM; i += blockDim.x) P[i] = Pg[i];
-	__syncthreads();
-	lf_guess_build(P, sp.nM, G, threadIdx.x, blockDim.x);
-	__syncthreads();
-	lf_guess_pairs(P, G, GP, threadIdx.x, blockDim.x);
-	__syncthreads();
-	const double gscale = lf_guess_scale(P, sp.nM);
-	uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-	if (s >= ns) return;
-	const int K = sp.nm + sp.nM - 1;
-	uint64_t first = u_lo + ((uint64_t)s + ns - (u_lo % ns)) % ns;
-	__builtin_amdgcn_s_setprio(3);   // one wave per SIMD: first among the resident waves
-	gf_mwc rng = gf_mwc_load(st, ns, s);
-	bool touched = false;
-	for (uint64_t u = first; u < U; u += ns) {
-		uint64_t o0 = offs[u], o1 = offs[u + 1];
-		if (o0 >= row_end) break;
-		if (o1 <= row_begin || o1 == o0) continue;
-		touched = true;
-		int p = (int)(u / K), k = (int)(u % K);
-		const gf_pencil_beam pix = pixels[p];
-		float D, Am, x, y, z;
-		dm_bin(sp, ext, pix, k, D, Am, x, y, z);
-		int lo, hi;
-		dm_window(sp, k, D, Am, lo, hi);
-		double base = P[lo], span = P[hi + 1] - P[lo];
-		uint64_t j0 = row_begin > o0 ? row_begin : o0;
-		uint64_t j1 = row_end < o1 ? row_end : o1;
-		for (uint64_t j = j0; j < j1; j++) {
-			double tgt = base + (double)rng.uniform() * span;
-			int iM = lf_bin_pairs(P, G, GP, gscale, lo, hi, tgt);
-			float M = sp.M1 - iM * sp.dM;
-			float sx, sy, Mtmp = 0.f, mtmp = 0.f;
-			bool in = star_draws(sp, pix, M, k - iM, rng, sx, sy, Mtmp, mtmp);
-			uint64_t row = j - row_begin;
-			park.xyMm[row] = make_float4(sx, sy, Mtmp, mtmp);
-			park.pix[row] = (uint32_t)p | (in ? 0u : 0x80000000u);
-		}
-	}
-	if (touched) gf_mwc_store(st, ns, s, rng);
-}
-
-// Deterministic half of the emit, one lane per row.  FULL: the footprint
-// test handles unions and equatorial rects.
-template <bool FULL>
-__global__ void __launch_bounds__(SK_BS) k_emit_rows(gf_skygen_params sp, int comp, const gf_pencil_beam *pixels,
-                                                     gf_extinction ext, gf_footprint foot, uint32_t nrows,
-                                                     gf_parked park, gf_skygen_cols out)
-{
-	uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
-	if (row >= nrows) return;
-	const float4 v = park.xyMm[row];
-	const uint32_t ph = park.pix[row];
-	const gf_pencil_beam pix = pixels[ph & 0x7fffffffu];
-	if (ph & 0x80000000u) {
-		write_star_outside(out, row, pix, comp, v.x, v.y);
-		return;
-	}
-	write_star<FULL>(sp, comp, ext, foot, pix, v.x, v.y, v.z, v.w, 0.f, out, row);
-}
-
-__global__ void k_find_unit(const uint64_t *offs, uint64_t U, uint64_t row, uint64_t *res)
-{
-	// largest u with offs[u] <= row (u < U)
-	uint64_t a = 0, b = U;   // offs[a] <= row; search in [a, b)
+	const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+	if (q > ntiles) return;
+	const uint64_t row = q < ntiles ? row_begin + (uint64_t)q * EMIT_TILE : row_end - 1;
+	uint64_t a = u_lo, b = U;
 	while (b - a > 1) {
-		uint64_t mid = (a + b) / 2;
+		const uint64_t mid = a + (b - a) / 2;
 		if (offs[mid] <= row) a = mid; else b = mid;
 	}
-	*res = a;
+	tu[q] = a;
+}
+
+// The emission's LF tables of a plan, built once: the prefix P, the guess
+// table G and its thresholds GP, laid out as in the emission's LDS
+// (lf_pairs_offset).
+__global__ void __launch_bounds__(SK_BS) k_lf_guess(const double *Pg, int nM, char *dst)
+{
+	double *P = (double *)dst;
+	uint16_t *G = (uint16_t *)(P + nM + 1);
+	double2 *GP = (double2 *)(dst + lf_pairs_offset(nM));
+	lf_guess_build(Pg, nM, G, threadIdx.x, blockDim.x);
+	for (int i = threadIdx.x; i <= nM; i += blockDim.x) P[i] = Pg[i];
+	__syncthreads();
+	lf_guess_pairs(Pg, G, GP, threadIdx.x, blockDim.x);
+}
+
+// Emission: lane per star, one tile of EMIT_TILE rows per workgroup.  A row
+// finds its unit between its tile's first and last units, jumps from the
+// unit's star state 8 draws per star before it, and draws the star.
+template <bool FULL>
+__global__ void __launch_bounds__(SK_BS) k_emit_stars(gf_skygen_params sp, int comp, const gf_pencil_beam *pixels,
+                                                      gf_extinction ext, gf_footprint foot, const char *lfg,
+                                                      uint32_t lfg_words, const uint64_t *offs, uint64_t row_begin,
+                                                      uint64_t row_end, const uint64_t *tu, const uint32_t *aplane,
+                                                      uint32_t ns, const uint64_t *zu, const uint64_t *jtab,
+                                                      gf_skygen_cols out)
+{
+	extern __shared__ double P[];
+	for (uint32_t i = threadIdx.x; i < lfg_words; i += blockDim.x) ((uint4 *)P)[i] = ((const uint4 *)lfg)[i];
+	__syncthreads();
+	const uint16_t *G = (const uint16_t *)(P + sp.nM + 1);
+	const double2 *GP = (const double2 *)((const char *)P + lf_pairs_offset(sp.nM));
+	const double gscale = lf_guess_scale(P, sp.nM);
+	const uint32_t q = blockIdx.x;
+	const uint64_t j = row_begin + (uint64_t)q * EMIT_TILE + threadIdx.x;
+	if (j >= row_end) return;
+	uint64_t a = tu[q], b = tu[q + 1] + 1;   // offs[a] <= j < offs[b]
+	while (b - a > 1) {
+		const uint64_t mid = a + (b - a) / 2;
+		if (offs[mid] <= j) a = mid; else b = mid;
+	}
+	const uint64_t u = a;
+	const uint64_t jj = j - offs[u];
+	const uint32_t s = (uint32_t)(u % ns);
+	const int K = sp.nm + sp.nM - 1;
+	const int p = (int)(u / K), k = (int)(u % K);
+	gf_mwc r;
+	r.a = aplane[s];
+	const gf_mwc_mod md(r.a);
+	gf_mwc_set_z(r, gf_mwc_jump_z(md, zu[u], jj << GF_STAR_SPAN_BITS, jtab + s, ns));
+	const gf_pencil_beam pix = pixels[p];
+	float Am;
+	int lo, hi;
+	unit_window(sp, ext, pix, k, Am, lo, hi);
+	emit_star<FULL>(sp, comp, ext, foot, pix, k, lo, hi, Am, P,
+	                [&](int l, int h, double tg) { return lf_bin_pairs(P, G, GP, gscale, l, h, tg); }, r, out,
+	                (uint32_t)(j - row_begin));
 }
 
 // The unit of row q * chunk for q < nq (the first unit of every output chunk).
@@ -602,10 +540,18 @@ struct gf_draw_plan {
 	gf_extinction ext;
 	uint64_t U;
 	uint64_t total;
+	uint32_t ns;          // streams
 	uint32_t *counts;
 	uint64_t *offs;
 	double *P;
 	uint64_t *tmp;
+	uint32_t *base;       // the streams [a | c | x] at the plan
+	uint64_t *jtab;       // their jump tables (GF_JUMP_BITS x ns)
+	uint64_t *zu;         // per unit: the state its stars' draws begin from
+	char *lfg;            // the emission's LF tables (k_lf_guess)
+	uint32_t lfg_words;   // their size in 16-byte words
+	// allocated element counts (a plan is reused across runs)
+	size_t cap_counts, cap_offs, cap_P, cap_tmp, cap_base, cap_jtab, cap_zu, cap_lfg;
 };
 
 extern "C" {
@@ -704,37 +650,11 @@ int gf_skygen_plan(const gf_skygen_params *sp, const gf_model *model,
                    const double *wpk, gf_rng *rng, gf_draw_plan **plan,
                    uint64_t *nstars, void *stream)
 {
-	GF_CHECK(sp && model && pixels && wpk && rng && rng->state && plan && nstars, GF_E_ARG);
-	GF_CHECK(sp->nM <= EMIT_MAX_NM, GF_E_ARG);
-	hipStream_t st = gf::as_stream(stream);
-	gf_draw_plan *pl = new gf_draw_plan{};
-	pl->sp = *sp;
-	pl->model = *model;
-	pl->pixels = pixels;
-	if (ext) pl->ext = *ext;
-	const uint64_t K = (uint64_t)(sp->nm + sp->nM - 1);
-	pl->U = (uint64_t)sp->npixels * K;
-	hipError_t e = hipMalloc((void **)&pl->counts, (pl->U + 1) * sizeof(uint32_t));
-	if (e == hipSuccess) e = hipMalloc((void **)&pl->offs, (pl->U + 1) * sizeof(uint64_t));
-	if (e == hipSuccess) e = hipMalloc((void **)&pl->P, ((size_t)sp->nM + 1) * sizeof(double));
-	if (e == hipSuccess) e = hipMalloc((void **)&pl->tmp, 2 * sizeof(uint64_t));
-	if (e != hipSuccess) { gf_skygen_plan_free(pl); return (int)e; }
-	LFTables t;
-	int rc = lf_tables(*sp, *model, t, st);
+	GF_CHECK(plan, GF_E_ARG);
+	*plan = nullptr;
+	gf_draw_plan *pl = nullptr;
+	int rc = gf::skygen_plan(sp, model, pixels, ext, wpk, rng, &pl, nstars, gf::as_stream(stream));
 	if (rc) { gf_skygen_plan_free(pl); return rc; }
-	e = hipMemcpyAsync(pl->P, t.P, ((size_t)sp->nM + 1) * sizeof(double), hipMemcpyDeviceToDevice, st);
-	if (e != hipSuccess) { gf_skygen_plan_free(pl); return (int)e; }
-	if (pl->U > 0) {
-		k_plan<<<gf::blocks_for(rng->nstreams, SK_BS), SK_BS, 0, st>>>(wpk, pl->U, rng->state, rng->nstreams, pl->counts);
-		e = hipGetLastError();
-		if (e != hipSuccess) { gf_skygen_plan_free(pl); return (int)e; }
-	}
-	rc = gf::exclusive_scan_u32_u64(pl->counts, pl->offs, pl->U, st);
-	if (rc) { gf_skygen_plan_free(pl); return rc; }
-	e = hipMemcpyAsync(&pl->total, pl->offs + pl->U, sizeof(uint64_t), hipMemcpyDeviceToHost, st);
-	if (e == hipSuccess) e = hipStreamSynchronize(st);
-	if (e != hipSuccess) { gf_skygen_plan_free(pl); return (int)e; }
-	*nstars = pl->total;
 	*plan = pl;
 	return 0;
 }
@@ -750,6 +670,72 @@ int gf_skygen_emit(const gf_draw_plan *plan, uint64_t row_begin, uint64_t row_en
 } // extern "C"
 
 namespace gf {
+
+namespace {
+template <typename T> hipError_t grow_dev(T *&p, size_t &cap, size_t n)
+{
+	if (p && cap >= n) return hipSuccess;
+	if (p) (void)hipFree(p);
+	p = nullptr;
+	cap = 0;
+	hipError_t e = hipMalloc((void **)&p, n * sizeof(T));
+	if (e == hipSuccess) cap = n;
+	return e;
+}
+} // namespace
+
+int skygen_plan(const gf_skygen_params *sp, const gf_model *model, const gf_pencil_beam *pixels,
+                const gf_extinction *ext, const double *wpk, gf_rng *rng, gf_draw_plan **plan, uint64_t *nstars,
+                hipStream_t st)
+{
+	GF_CHECK(sp && model && pixels && wpk && rng && rng->state && plan && nstars, GF_E_ARG);
+	GF_CHECK(sp->nM <= EMIT_MAX_NM && rng->nstreams > 0, GF_E_ARG);
+	gf_draw_plan *pl = *plan;
+	if (!pl) *plan = pl = new gf_draw_plan{};
+	pl->sp = *sp;
+	pl->model = *model;
+	pl->pixels = pixels;
+	pl->ext = ext ? *ext : gf_extinction{};
+	pl->total = 0;
+	const uint64_t K = (uint64_t)(sp->nm + sp->nM - 1);
+	const uint32_t ns = rng->nstreams;
+	pl->U = (uint64_t)sp->npixels * K;
+	pl->ns = ns;
+	const size_t lfg_bytes = lf_pairs_offset(sp->nM) + LF_GUESS * sizeof(double2);
+	pl->lfg_words = (uint32_t)((lfg_bytes + 15) / 16);
+	hipError_t e = grow_dev(pl->counts, pl->cap_counts, pl->U + 1);
+	if (e == hipSuccess) e = grow_dev(pl->offs, pl->cap_offs, pl->U + 1);
+	if (e == hipSuccess) e = grow_dev(pl->P, pl->cap_P, (size_t)sp->nM + 1);
+	if (e == hipSuccess) e = grow_dev(pl->tmp, pl->cap_tmp, 2);
+	if (e == hipSuccess) e = grow_dev(pl->base, pl->cap_base, (size_t)3 * ns);
+	if (e == hipSuccess) e = grow_dev(pl->jtab, pl->cap_jtab, (size_t)GF_JUMP_BITS * ns);
+	if (e == hipSuccess) e = grow_dev(pl->zu, pl->cap_zu, pl->U + 1);
+	if (e == hipSuccess) e = grow_dev(pl->lfg, pl->cap_lfg, (size_t)pl->lfg_words * 16);
+	if (e != hipSuccess) return (int)e;
+	LFTables t;
+	int rc = lf_tables(*sp, *model, t, st);
+	if (rc) return rc;
+	GF_HIP(hipMemcpyAsync(pl->P, t.P, ((size_t)sp->nM + 1) * sizeof(double), hipMemcpyDeviceToDevice, st));
+	GF_HIP(hipMemcpyAsync(pl->base, rng->state, (size_t)3 * ns * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+	rc = gf::mwc_jump_table(rng, pl->jtab, st);
+	if (rc) return rc;
+	k_lf_guess<<<1, SK_BS, 0, st>>>(pl->P, sp->nM, pl->lfg);
+	GF_LAUNCHED();
+	if (pl->U > 0) {
+		k_plan_units<<<gf::blocks_for(pl->U, SK_BS), SK_BS, 0, st>>>(wpk, pl->U, pl->base, ns, pl->jtab, pl->counts,
+		                                                             pl->zu);
+		GF_LAUNCHED();
+		// the streams move past every unit's draws
+		rc = gf::mwc_jump(rng, ((pl->U + ns - 1) / ns) << GF_UNIT_SPAN_BITS, pl->jtab, st);
+		if (rc) return rc;
+	}
+	rc = gf::exclusive_scan_u32_u64(pl->counts, pl->offs, pl->U, st);
+	if (rc) return rc;
+	GF_HIP(hipMemcpyAsync(&pl->total, pl->offs + pl->U, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+	GF_HIP(hipStreamSynchronize(st));
+	*nstars = pl->total;
+	return 0;
+}
 
 int skygen_chunk_units(const gf_draw_plan *plan, uint64_t chunk, uint32_t nq, uint64_t *units, hipStream_t st)
 {
@@ -768,63 +754,42 @@ int skygen_chunk_units(const gf_draw_plan *plan, uint64_t chunk, uint32_t nq, ui
 int skygen_emit(const gf_draw_plan *plan, uint64_t row_begin, uint64_t row_end, const gf_footprint *foot,
                 gf_rng *rng, const gf_skygen_cols *out, hipStream_t st, hipEvent_t *ev, float *ms, uint64_t unit)
 {
-	GF_CHECK(plan && rng && rng->state && out && row_begin <= row_end && row_end <= plan->total, GF_E_ARG);
+	(void)rng;   // the stars' draws come from the streams as they were at the plan
+	GF_CHECK(plan && out && row_begin <= row_end && row_end <= plan->total, GF_E_ARG);
 	if (ms) ms[0] = ms[1] = 0.f;
 	if (row_begin == row_end) return 0;
 	gf_footprint fp{};
 	if (foot) fp = *foot;
-	uint64_t u_lo = unit;
-	if (u_lo == UINT64_MAX) {
-		k_find_unit<<<1, 1, 0, st>>>(plan->offs, plan->U, row_begin, plan->tmp);
-		GF_LAUNCHED();
-		GF_HIP(hipMemcpyAsync(&u_lo, plan->tmp, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-		GF_HIP(hipStreamSynchronize(st));
-	}
-	GF_CHECK(u_lo < plan->U, GF_E_ARG);
-	size_t shb = ((size_t)plan->sp.nM + 1) * sizeof(double);
-	const size_t shb_guess = lf_pairs_offset(plan->sp.nM) + LF_GUESS * sizeof(double2);
-	if (plan->ext.beam.data || plan->ext.north.data || plan->ext.south.data) {
-		// with extinction the star needs its DM bin's minimum extinction: one pass
-		if (ev) GF_HIP(hipEventRecord(ev[0], st));
-		k_emit<<<gf::blocks_for(rng->nstreams, SK_BS), SK_BS, shb, st>>>(
-		    plan->sp, plan->model, plan->pixels, plan->ext, fp, plan->P, plan->offs, plan->U, row_begin, row_end,
-		    u_lo, rng->state, rng->nstreams, *out);
-		GF_LAUNCHED();
-		if (ev) {
-			GF_HIP(hipEventRecord(ev[1], st));
-			GF_HIP(hipEventRecord(ev[2], st));
-			if (ms) {
-				GF_HIP(hipEventSynchronize(ev[2]));
-				GF_HIP(hipEventElapsedTime(&ms[0], ev[0], ev[1]));
-			}
-		}
-		return 0;
-	}
-	uint64_t n = row_end - row_begin;
+	const uint64_t n = row_end - row_begin;
 	GF_CHECK(n < (1ull << 32), GF_E_ARG);
-	void *pk;
-	int rc = gf::scratch(GF_SLOT_PARK, (size_t)n * (sizeof(float4) + sizeof(uint32_t)), &pk);
+	const uint32_t ntiles = (uint32_t)((n + EMIT_TILE - 1) / EMIT_TILE);
+	void *p;
+	int rc = gf::scratch(GF_SLOT_PARK, ((size_t)ntiles + 1) * sizeof(uint64_t), &p);
 	if (rc) return rc;
-	gf_parked park{(float4 *)pk, (uint32_t *)((char *)pk + (size_t)n * sizeof(float4))};
+	uint64_t *tu = (uint64_t *)p;
+	const uint64_t u_lo = unit == UINT64_MAX ? 0 : unit;
+	GF_CHECK(u_lo < plan->U, GF_E_ARG);
 	if (ev) GF_HIP(hipEventRecord(ev[0], st));
-	k_emit_rng<<<gf::blocks_for(rng->nstreams, SK_BS), SK_BS, shb_guess, st>>>(
-	    plan->sp, plan->model, plan->pixels, plan->ext, plan->P, plan->offs, plan->U, row_begin, row_end, u_lo,
-	    rng->state, rng->nstreams, park);
+	k_tile_units<<<gf::blocks_for((uint64_t)ntiles + 1, 256), 256, 0, st>>>(plan->offs, plan->U, row_begin, row_end,
+	                                                                        u_lo, ntiles, tu);
 	GF_LAUNCHED();
 	if (ev) GF_HIP(hipEventRecord(ev[1], st));
+	const size_t shb = (size_t)plan->lfg_words * 16;
 	if (footprint_is_simple(fp))
-		k_emit_rows<false><<<gf::blocks_for(n, SK_BS), SK_BS, 0, st>>>(plan->sp, plan->model.comp, plan->pixels,
-		                                                                 plan->ext, fp, (uint32_t)n, park, *out);
+		k_emit_stars<false><<<ntiles, SK_BS, shb, st>>>(plan->sp, plan->model.comp, plan->pixels, plan->ext, fp,
+		                                                plan->lfg, plan->lfg_words, plan->offs, row_begin, row_end, tu,
+		                                                plan->base, plan->ns, plan->zu, plan->jtab, *out);
 	else
-		k_emit_rows<true><<<gf::blocks_for(n, SK_BS), SK_BS, 0, st>>>(plan->sp, plan->model.comp, plan->pixels,
-		                                                                plan->ext, fp, (uint32_t)n, park, *out);
+		k_emit_stars<true><<<ntiles, SK_BS, shb, st>>>(plan->sp, plan->model.comp, plan->pixels, plan->ext, fp,
+		                                               plan->lfg, plan->lfg_words, plan->offs, row_begin, row_end, tu,
+		                                               plan->base, plan->ns, plan->zu, plan->jtab, *out);
 	GF_LAUNCHED();
 	if (ev) {
 		GF_HIP(hipEventRecord(ev[2], st));
 		if (ms) {
 			GF_HIP(hipEventSynchronize(ev[2]));
-			GF_HIP(hipEventElapsedTime(&ms[0], ev[0], ev[1]));
-			GF_HIP(hipEventElapsedTime(&ms[1], ev[1], ev[2]));
+			GF_HIP(hipEventElapsedTime(&ms[0], ev[1], ev[2]));
+			GF_HIP(hipEventElapsedTime(&ms[1], ev[0], ev[1]));
 		}
 	}
 	return 0;
@@ -841,6 +806,10 @@ int gf_skygen_plan_free(gf_draw_plan *plan)
 	if (plan->offs) (void)hipFree(plan->offs);
 	if (plan->P) (void)hipFree(plan->P);
 	if (plan->tmp) (void)hipFree(plan->tmp);
+	if (plan->base) (void)hipFree(plan->base);
+	if (plan->jtab) (void)hipFree(plan->jtab);
+	if (plan->zu) (void)hipFree(plan->zu);
+	if (plan->lfg) (void)hipFree(plan->lfg);
 	delete plan;
 	return 0;
 }

@@ -608,7 +608,7 @@ typedef struct gf_run_stats {
     double   t_stage[6];    /* module chain launches: draw pass; row pass for
                                FeH/GaussianFeH, Bond2010, photometry,
                                photometric errors, vel2pm + gal2other */
-    double   t_emit_rng, t_emit_rows;   /* the two launches inside t_emit */
+    double   t_emit_stars, t_emit_units;   /* the two launches inside t_emit: the stars and their units */
 } gf_run_stats;
 
 /* Called once per output chunk, in row order, with device columns holding

@@ -490,6 +490,30 @@ static void deproject(const gf_lambert &p, double &l, double &b, float x, float 
 	l = p.l0 + gfo_atan2((double)x * sc, (double)p.cb * cc / ir - (double)ysb * sc);
 }
 
+/* deproject with the direction cosines (cos b cos l, cos b sin l, sin b)
+ * of the point from its own terms: cos b sin(l - l0) = x sc ir and
+ * cos b cos(l - l0) = (cb cc / ir - y sb sc) ir (the atan2 arguments times
+ * ir = 1/rho), sin b the asin argument, rotated by l0 (cl, sl of the
+ * projection). */
+static void deproject_dir(const gf_lambert &p, double &l, double &b, double &cbcl, double &cbsl, double &sbo,
+                          float x, float y)
+{
+	double r2 = (double)(x * x + y * y);
+	if (r2 == 0.) { b = p.b0; l = 0.; cbcl = (double)p.cb; cbsl = 0.; sbo = (double)p.sb; return; }
+	double ir = 1. / sqrt(r2);
+	double sc = 1. / ir * sqrt(1. - 0.25 * r2);
+	double cc = 1. - 0.5 * r2;
+	double S = cc * (double)p.sb + (double)y * sc * (double)p.cb * ir;
+	b = gfo_asin(S);
+	float ysb = y * p.sb;
+	double Pn = (double)x * sc, Qd = (double)p.cb * cc / ir - (double)ysb * sc;
+	l = p.l0 + gfo_atan2(Pn, Qd);
+	double Pr = Pn * ir, Qr = Qd * ir;
+	cbcl = (double)p.cl * Qr - (double)p.sl * Pr;
+	cbsl = (double)p.sl * Qr + (double)p.cl * Pr;
+	sbo = S;
+}
+
 GFO_API void gfo_deproject(const gf_lambert *p, float x, float y, double *l, double *b) { deproject(*p, *l, *b, x, y); }
 
 GFO_API void gfo_galequ(double l, double b, double *ra, double *dec);
@@ -592,7 +616,7 @@ static void cell_pos(const gf_skygen_params &sp, const gf_extinction &ext, const
 {
 	float m = sp.m0 + im * sp.dm;
 	float DM = m - M;
-	D = gfo_powf(10.f, 0.2f * DM + 1.f);
+	D = gfo_pow10f(0.2f * DM + 1.f);
 	int xidx = pix.extIdx / 2048, yidx = pix.extIdx % 2048;
 	Am = ext.beam.data ? tex3(ext.beam, (float)xidx, (float)yidx, DM) : 0.f;
 	x = D * pix.cl * pix.cb;
@@ -666,13 +690,9 @@ static void one_star(const gf_skygen_params &sp, const gf_model &m, const gf_ext
 		}
 		return;
 	}
-	double l, b;
-	deproject(sp.proj[pix.projIdx], l, b, x, y);
+	double l, b, cbcl, cbsl, sbd;
+	deproject_dir(sp.proj[pix.projIdx], l, b, cbcl, cbsl, sbd, x, y);
 	int outside = outside_foot(foot, l, b);
-	double sld, cld, sbd, cbd;
-	gfo_sincos(l, &sld, &cld);
-	gfo_sincos(b, &sbd, &cbd);
-	float cl = (float)cld, cb = (float)cbd, sl = (float)sld, sb = (float)sbd;
 	l *= R2D;
 	if (l < 0.) l += 360.;
 	if (l > 360.) l -= 360.;
@@ -680,7 +700,7 @@ static void one_star(const gf_skygen_params &sp, const gf_model &m, const gf_ext
 	float Mt = M + sp.dM * (rng.uniform() - 0.5f);
 	float mt = sp.m0 + sp.dm * ((float)im + rng.uniform() - 0.5f);
 	float DM = mt - Mt;
-	float D = gfo_powf(10.f, 0.2f * DM + 1.f);
+	float D = gfo_pow10f(0.2f * DM + 1.f);
 	const gf_tex3d &em = pix.projIdx == 0 ? ext.north : ext.south;
 	float Am0 = em.data ? tex3(em, x, y, DM) : 0.f;
 	float Am1 = em.data ? tex3(em, x, y, 100.f) : 0.f;
@@ -693,7 +713,8 @@ static void one_star(const gf_skygen_params &sp, const gf_model &m, const gf_ext
 		C<float>(s.c.projXY, r, 0) = x; C<float>(s.c.projXY, r, 1) = y;
 		C<double>(s.c.lb, r, 0) = l; C<double>(s.c.lb, r, 1) = b;
 		C<float>(s.c.M, r) = Mt; C<float>(s.c.DM, r) = DM;
-		C<float>(s.c.XYZ, r, 0) = D * cl * cb; C<float>(s.c.XYZ, r, 1) = D * sl * cb; C<float>(s.c.XYZ, r, 2) = D * sb;
+		C<float>(s.c.XYZ, r, 0) = D * (float)cbcl; C<float>(s.c.XYZ, r, 1) = D * (float)cbsl;
+		C<float>(s.c.XYZ, r, 2) = D * (float)sbd;
 		C<int>(s.c.comp, r) = m.comp;
 		C<float>(s.c.Am, r) = Am0; C<float>(s.c.AmInf, r) = AmInf;
 		C<int>(s.c.hidden, r) = hidden;
@@ -1037,19 +1058,55 @@ GFO_API void gfo_count_bins(const gf_skygen_params *spp, const gf_model *mp, con
 }
 
 /* Plan: unit u = p*K + k Poisson-drawn from stream u mod ns. */
-GFO_API void gfo_plan(const double *wpk, uint64_t U, uint32_t *st, uint32_t ns, uint32_t *counts)
+/* The cumulative-table draw's use of the streams (this build's own draw,
+ * statistically equivalent to the reference's per-cell draw of
+ * skygen/skygen.cu.h:366-457 by Poisson superposition).  Unit u = p*K + k
+ * of a model belongs to stream s = u mod ns with index i = u div ns; with
+ * the streams' states Z at the plan:
+ *   the plan's Poisson draw of u starts at Z_s advanced i*2^32 draws;
+ *   star j of u (0-based within the unit) starts at Z_s advanced
+ *   i*2^32 + 2^31 + 8 j draws (a star takes 3 or 5);
+ *   once the model is planned, the streams move on past every unit's draws,
+ *   ceil(U / ns) * 2^32 draws (gfo_plan_advance). */
+static const uint64_t GFO_UNIT_SPAN = 1ull << 32;
+static const uint64_t GFO_EMIT_AT = 1ull << 31;
+static const uint64_t GFO_STAR_SPAN = 8;
+
+/* Poisson counts of the U units; st (the streams at the plan) is only read */
+GFO_API void gfo_plan(const double *wpk, uint64_t U, const uint32_t *st, uint32_t ns, uint32_t *counts)
 {
+	for (uint32_t s = 0; s < ns && s < U; s++) {
+		gfo_rng start = rng_load(st, ns, s);
+		const uint64_t m = ((uint64_t)start.a << 32) - 1;
+		const uint64_t A = gfo_powmod(start.a, GFO_UNIT_SPAN, m);   /* one unit's draws */
+		for (uint64_t u = s; u < U; u += ns) {
+			gfo_rng r = start;
+			counts[u] = wpk[u] > 0.0 ? r.poisson((float)wpk[u]) : 0;
+			/* the next unit of the stream */
+			const uint64_t z = gfo_mulmod(((uint64_t)start.c << 32) | start.x, A, m);
+			start.c = (uint32_t)(z >> 32);
+			start.x = (uint32_t)z;
+		}
+	}
+}
+
+/* the streams once a model of U units is planned */
+GFO_API void gfo_plan_advance(uint32_t *st, uint32_t ns, uint64_t U)
+{
+	const uint64_t nI = (U + ns - 1) / ns;
 	for (uint32_t s = 0; s < ns; s++) {
 		gfo_rng r = rng_load(st, ns, s);
-		for (uint64_t u = s; u < U; u += ns) counts[u] = wpk[u] > 0.0 ? r.poisson((float)wpk[u]) : 0;
+		gfo_jump(r, nI * GFO_UNIT_SPAN);
 		rng_store(st, ns, s, r);
 	}
 }
 
-/* Emit rows [row_begin, row_end) of a plan with offsets offs (U+1). */
+/* Emit rows [row_begin, row_end) of a plan with offsets offs (U+1); st
+ * holds the streams as they were at the plan (only read). */
 GFO_API void gfo_emit(const gf_skygen_params *spp, const gf_model *mp, const gf_pencil_beam *pixels,
                       const gf_extinction *extp, const gf_footprint *footp, const uint64_t *offs, uint64_t U,
-                      uint64_t row_begin, uint64_t row_end, uint32_t *st, uint32_t ns, const gf_skygen_cols *out)
+                      uint64_t row_begin, uint64_t row_end, const uint32_t *st, uint32_t ns,
+                      const gf_skygen_cols *out)
 {
 	const gf_skygen_params &sp = *spp;
 	const gf_model &m = *mp;
@@ -1061,33 +1118,32 @@ GFO_API void gfo_emit(const gf_skygen_params *spp, const gf_model *mp, const gf_
 	std::vector<double> P;
 	lf_prefix(sp, m, lfc, P);
 	const int K = sp.nm + sp.nM - 1;
-	/* every stream walks its units in increasing order */
-	for (uint32_t s = 0; s < ns; s++) {
-		gfo_rng r = rng_load(st, ns, s);
-		bool touched = false;
-		for (uint64_t u = s; u < U; u += ns) {
-			uint64_t o0 = offs[u], o1 = offs[u + 1];
-			if (o0 >= row_end) break;
-			if (o1 <= row_begin || o1 == o0) continue;
-			touched = true;
-			int p = (int)(u / K), k = (int)(u % K);
-			const gf_pencil_beam &pix = pixels[p];
-			float D, Am, x, y, z;
-			int lo, hi;
-			dm_bin(sp, ext, pix, k, D, Am, x, y, z, lo, hi);
-			double base = P[lo], span = P[hi + 1] - P[lo];
-			uint64_t j0 = std::max(row_begin, o0), j1 = std::min(row_end, o1);
-			Sink snk{*out, true, j0 - row_begin};
-			for (uint64_t j = j0; j < j1; j++) {
-				double tgt = base + (double)r.uniform() * span;
-				int iM = lo;
-				while (iM < hi && !(P[iM + 1] > tgt)) iM++;
-				float M = sp.M1 - iM * sp.dM;
-				one_star(sp, m, ext, foot, pix, M, k - iM, Am, r, snk);
-				snk.row++;
-			}
+	if (row_begin >= row_end) return;
+	/* the unit of row_begin: the largest u with offs[u] <= row_begin */
+	uint64_t u = (uint64_t)(std::upper_bound(offs, offs + U + 1, row_begin) - offs) - 1;
+	for (; u < U && offs[u] < row_end; u++) {
+		const uint64_t o0 = offs[u], o1 = offs[u + 1];
+		if (o1 == o0) continue;
+		int p = (int)(u / K), k = (int)(u % K);
+		const gf_pencil_beam &pix = pixels[p];
+		float D, Am, x, y, z;
+		int lo, hi;
+		dm_bin(sp, ext, pix, k, D, Am, x, y, z, lo, hi);
+		double base = P[lo], span = P[hi + 1] - P[lo];
+		const uint64_t j0 = std::max(row_begin, o0), j1 = std::min(row_end, o1);
+		gfo_rng star = rng_load(st, ns, (uint32_t)(u % ns));
+		gfo_jump(star, (u / ns) * GFO_UNIT_SPAN + GFO_EMIT_AT + GFO_STAR_SPAN * (j0 - o0));
+		Sink snk{*out, true, j0 - row_begin};
+		for (uint64_t j = j0; j < j1; j++) {
+			gfo_rng r = star;
+			double tgt = base + (double)r.uniform() * span;
+			int iM = lo;
+			while (iM < hi && !(P[iM + 1] > tgt)) iM++;
+			float M = sp.M1 - iM * sp.dM;
+			one_star(sp, m, ext, foot, pix, M, k - iM, Am, r, snk);
+			snk.row++;
+			for (uint64_t q = 0; q < GFO_STAR_SPAN; q++) (void)star.uniform();   /* the next star */
 		}
-		if (touched) rng_store(st, ns, s, r);
 	}
 }
 

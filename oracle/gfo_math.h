@@ -204,7 +204,32 @@ static inline float gfo_logf(float x)
 	float ef = (float)e;
 	return fmaf(ef, hi, -((h - fmaf(s, h + R, ef * lo)) - f));
 }
-static inline float gfo_exp10f(float x) { return (float)gfo_exp((double)x * 2.30258509299404590109e+00); }
+/* 10^y in float arithmetic (Cody and Waite): y log2(10) = k + r, k =
+ * rint(y*Lh), r = fma(y, Lh, -k) + y*Ll with log2(10) = Lh + Ll (float head
+ * and tail); 2^r by the Taylor series of e^(r ln2) to degree 7; times 2^k */
+static inline float gfo_pow10f(float y)
+{
+	const float Lh = 0x1.a934fp+1f, Ll = 0x1.2f346ep-24f;
+	if (y != y) return y;
+	if (y > 38.5318413f) return INFINITY;
+	if (y < -45.3534698f) return 0.f;
+	float k = rintf(y * Lh);
+	float r = fmaf(y, Lh, -k);
+	r = fmaf(y, Ll, r);
+	static const float c[8] = {0x1.ffcbfcp-17f, 0x1.430912p-13f, 0x1.5d87fep-10f, 0x1.3b2ab6p-7f,
+	                           0x1.c6b08ep-5f,  0x1.ebfbe0p-3f,  0x1.62e430p-1f,  1.f};
+	float p = c[0];
+	for (int i = 1; i < 8; i++) p = fmaf(p, r, c[i]);
+	int ki = (int)k;
+	float s;
+	uint32_t u;
+	if (ki > 127) { u = 0x7f000000u; memcpy(&s, &u, 4); p *= s; ki -= 127; }
+	if (ki < -126) { u = 0x00800000u; memcpy(&s, &u, 4); p *= s; ki += 126; }
+	u = (uint32_t)(ki + 127) << 23;
+	memcpy(&s, &u, 4);
+	return p * s;
+}
+static inline float gfo_exp10f(float x) { return gfo_pow10f(x); }
 static inline float gfo_log10f(float x) { return (float)(gfo_log((double)x) * 4.34294481903251816668e-01); }
 static inline float gfo_sinf(float x) { double s, c; gfo_sincos((double)x, &s, &c); return (float)s; }
 static inline float gfo_cosf(float x) { double s, c; gfo_sincos((double)x, &s, &c); return (float)c; }

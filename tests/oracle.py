@@ -39,6 +39,9 @@ def lib():
         _o.gfo_count_bins.argtypes = [P(A.gf_skygen_params), P(A.gf_model), vp, P(A.gf_extinction),
                                       P(A.gf_count_result), vp, vp]
         _o.gfo_plan.argtypes = [vp, C.c_uint64, vp, C.c_uint32, vp]
+        _o.gfo_plan_advance.argtypes = [vp, C.c_uint32, C.c_uint64]
+        _o.gfo_mwc_jump.argtypes = [vp, C.c_uint32, C.c_uint64]
+        _o.gfo_mwc_step.argtypes = [vp, C.c_uint32, C.c_uint64]
         _o.gfo_emit.argtypes = [P(A.gf_skygen_params), P(A.gf_model), vp, P(A.gf_extinction),
                                 P(A.gf_footprint), vp, C.c_uint64, C.c_uint64, C.c_uint64, vp, C.c_uint32,
                                 P(A.gf_skygen_cols)]

@@ -66,7 +66,9 @@ def replay(view, chunk_rows=30_000_000, pixel_index=None, on_chunk=None, time_bu
         o.gfo_count_bins(C.byref(sp), C.byref(m), C.cast(pix, C.c_void_p), _ext(view), C.byref(res), None,
                          O.ptr(wpk))
         counts = np.zeros(U, dtype=np.uint32)
-        o.gfo_plan(O.ptr(wpk), U, O.ptr(state), ns, O.ptr(counts))
+        base = state.copy()          # the streams at the plan: the emission's too
+        o.gfo_plan(O.ptr(wpk), U, O.ptr(base), ns, O.ptr(counts))
+        o.gfo_plan_advance(O.ptr(state), ns, U)
         offs = np.zeros(U + 1, dtype=np.uint64)
         offs[1:] = np.cumsum(counts, dtype=np.uint64)
         total = int(offs[-1])
@@ -78,7 +80,7 @@ def replay(view, chunk_rows=30_000_000, pixel_index=None, on_chunk=None, time_bu
             t = HostTable(n, spec)
             cols = t.skygen_cols()
             o.gfo_emit(C.byref(sp), C.byref(m), C.cast(pix, C.c_void_p), _ext(view), C.byref(foot), O.ptr(offs),
-                       U, r0, r1, O.ptr(state), ns, C.byref(cols))
+                       U, r0, r1, O.ptr(base), ns, C.byref(cols))
             tab = t.table()
             for b0 in range(0, n, sub):
                 ks = A.gf_rows(b0, 256, min(n, b0 + sub), 0)
@@ -159,7 +161,8 @@ def request_streams(view, npix):
 def replay_request(view):
     """One server request on the oracle, given the host view of a catalog
     opened on the request's own beam, faint limit and seed: every model is
-    planned (in model order) on the request's S streams, then every model is
+    planned (in model order) on the request's S streams, each from the
+    streams as the previous models' plans left them, then every model is
     emitted, model after model, into one table; then the module chain runs
     over the request's 256-row chunks, chunk c on stream c.
     Returns (HostTable, expected)."""
@@ -179,20 +182,22 @@ def replay_request(view):
         o.gfo_count_bins(C.byref(sp), C.byref(m), C.cast(pix, C.c_void_p), None, C.byref(res), None, O.ptr(wpk))
         expected += float(wpk[wpk > 0].sum())
         counts = np.zeros(U, dtype=np.uint32)
-        o.gfo_plan(O.ptr(wpk), U, O.ptr(state), S, O.ptr(counts))
+        base = state.copy()
+        o.gfo_plan(O.ptr(wpk), U, O.ptr(base), S, O.ptr(counts))
+        o.gfo_plan_advance(O.ptr(state), S, U)
         offs = np.zeros(U + 1, dtype=np.uint64)
         offs[1:] = np.cumsum(counts, dtype=np.uint64)
-        plans.append(offs)
-    total = int(sum(int(p[-1]) for p in plans))
+        plans.append((offs, base))
+    total = int(sum(int(p[-1]) for p, _ in plans))
     spec = catalog_spec(nbands=max(view.nbands, 1), nabsmag=max(view.nabsmag, 1))
     t = HostTable(total, spec)
     row = 0
-    for mi, offs in enumerate(plans):
+    for mi, (offs, base) in enumerate(plans):
         n = int(offs[-1])
         if n:
             cols = _shift(t.skygen_cols(), row)
             o.gfo_emit(C.byref(sp), C.byref(view.models[mi]), C.cast(pix, C.c_void_p), None, C.byref(foot),
-                       O.ptr(offs), U, 0, n, O.ptr(state), S, C.byref(cols))
+                       O.ptr(offs), U, 0, n, O.ptr(base), S, C.byref(cols))
         row += n
     if total:
         assert (total + 255) // 256 <= S, "replay needs at most one chunk per stream"

@@ -104,6 +104,7 @@ def test_count_precision_study_c1(c1):
             st = state0.copy()
             counts = np.zeros(U, dtype=np.uint32)
             O.lib().gfo_plan(O.ptr(wd), U, O.ptr(st), view.nstreams, O.ptr(counts))
+            O.lib().gfo_plan_advance(O.ptr(st), view.nstreams, U)
             plans[name] = counts
             entry[name] = {"max_rel_err_W": float(rel.max()), "rel_err_total": float(abs(wd[pos].sum() - wo[pos].sum())
                                                                                     / wo[pos].sum()),

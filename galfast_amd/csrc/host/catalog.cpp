@@ -377,21 +377,6 @@ void pixelize(gf_catalog &c, double dx)
 	c.sp.npixels = (int)c.pixels.size();
 }
 
-// Equatorial (ra, dec) -> Galactic (l, b), radians: the inverse rotation of
-// galequ (modules/gal2other_gpu.cu.h:39-60).
-void equ_to_gal(double ra, double dec, double &l, double &b)
-{
-	const double angp = M_PI / 180. * 192.859508333, l0 = M_PI / 180. * 32.932;
-	const double ce = 0.88998740217659689, se = 0.45598511375586859;
-	double A = ra - angp;
-	double X = cos(dec) * cos(A), Y = cos(dec) * sin(A), Z = sin(dec);
-	double u0 = Y, u1 = -se * X + ce * Z, u2 = ce * X + se * Z;
-	b = asin(std::max(-1.0, std::min(1.0, u2)));
-	l = atan2(u1, u0) + l0;
-	l = fmod(l, 2 * M_PI);
-	if (l < 0) l += 2 * M_PI;
-}
-
 // One footprint configuration file -> one region (load_footprint,
 // footprint.cpp:729-744).
 gf_foot_region load_region(const Config &fc)
@@ -1644,6 +1629,27 @@ int gf_beam_pixels_host(double l, double b, double radius, double dx, gf_pencil_
 		std::vector<gf_pencil_beam> v = pixelize_beam(fp, dx * d2r);
 		for (int i = 0; i < (int)v.size() && i < cap; i++) out[i] = v[i];
 		return (int)v.size();
+	} catch (const ConfigError &e) {
+		g_last_error = e.msg;
+		return GF_E_CONFIG;
+	}
+}
+
+int gf_transform_host(const char *center, const char *orientation, double Rg, double z0, float T[3], float M[9])
+{
+	GF_CHECK(center && orientation && T && M, GF_E_ARG);
+	try {
+		Config cfg;
+		char buf[64];
+		snprintf(buf, sizeof buf, "%.17g", Rg);
+		cfg.kv["Rg"] = buf;
+		snprintf(buf, sizeof buf, "%.17g", z0);
+		cfg.kv["z0"] = buf;
+		float MM[3][3];
+		load_transform(T, MM, center, orientation, cfg);
+		for (int i = 0; i < 3; i++)
+			for (int j = 0; j < 3; j++) M[3 * i + j] = MM[i][j];
+		return 0;
 	} catch (const ConfigError &e) {
 		g_last_error = e.msg;
 		return GF_E_CONFIG;

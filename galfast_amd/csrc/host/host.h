@@ -90,6 +90,10 @@ Cube load_fits_cube(const std::string &file);
 void texcoord_from_range(float imgx, float imgy, float x, float y, float tc[2]);
 
 // Coordinate transform of a model or module (common/transform.cpp:345-388).
+// J2000 Galactic <-> equatorial, radians (galequ, modules/gal2other_gpu.cu.h:39-60,
+// and its inverse rotation).
+void gal_to_equ_host(double l, double b, double &ra, double &dec);
+void equ_to_gal(double ra, double dec, double &l, double &b);
 void load_transform(float T[3], float M[3][3], const std::string &center, const std::string &orientation,
                     const Config &cfg);
 

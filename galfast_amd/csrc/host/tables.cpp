@@ -229,53 +229,181 @@ PhotTables build_photometry_tables(const std::vector<std::vector<double>> &rows,
 	return pt;
 }
 
-// get_gal2xxx_matrix / xxx2galxyz / load_transform (common/transform.cpp:70-388)
-// for the coordinate systems the configurations use: gal and galplane.
+// J2000 Galactic <-> equatorial (the constants of galequ_constants,
+// modules/module_lib.h:62-71): (l, b) -> (ra, dec) as galequ
+// (modules/gal2other_gpu.cu.h:39-60) and its inverse rotation, radians.
+namespace {
+const double GE_ANGP = M_PI / 180. * 192.859508333, GE_L0 = M_PI / 180. * 32.932;
+const double GE_CE = 0.88998740217659689, GE_SE = 0.45598511375586859;
+}
+
+void gal_to_equ_host(double l, double b, double &ra, double &dec)
+{
+	const double cb = cos(b), sb = sin(b), cl = cos(l - GE_L0), sl = sin(l - GE_L0);
+	ra = atan2(cb * cl, sb * GE_CE - cb * GE_SE * sl) + GE_ANGP;
+	dec = asin(cb * GE_CE * sl + sb * GE_SE);
+	while (ra < 0.) ra += 2 * M_PI;
+	while (ra >= 2 * M_PI) ra -= 2 * M_PI;
+}
+
+void equ_to_gal(double ra, double dec, double &l, double &b)
+{
+	double A = ra - GE_ANGP;
+	double X = cos(dec) * cos(A), Y = cos(dec) * sin(A), Z = sin(dec);
+	double u0 = Y, u1 = -GE_SE * X + GE_CE * Z, u2 = GE_CE * X + GE_SE * Z;
+	b = asin(std::max(-1.0, std::min(1.0, u2)));
+	l = atan2(u1, u0) + GE_L0;
+	l = fmod(l, 2 * M_PI);
+	if (l < 0) l += 2 * M_PI;
+}
+
+namespace {
+
+// (l, b, D) -> Galactic cartesian (lbd2xyz, common/transform.cpp:58-68)
+void lbd2xyz(double v[3])
+{
+	const double cl = cos(v[0]), cb = cos(v[1]), sl = sin(v[0]), sb = sin(v[1]), d = v[2];
+	v[0] = d * cl * cb;
+	v[1] = d * sl * cb;
+	v[2] = d * sb;
+}
+
+// get_gal2xxx_matrix (common/transform.cpp:70-148): Galactic cartesian to
+// coordinate system cs.
+void gal2xxx(double go[3][3], const std::string &cs, const Config &cfg)
+{
+	if (cs == "gal" || cs == "galcentric") {
+		for (int i = 0; i < 3; i++)
+			for (int j = 0; j < 3; j++) go[i][j] = i == j ? 1. : 0.;
+	} else if (cs == "equ") {
+		// the Galactic axes in equatorial cartesian form the columns
+		double x[3], y[3], z[3];
+		gal_to_equ_host(0., 0., x[0], x[1]); x[2] = 1;
+		gal_to_equ_host(M_PI / 2, 0., y[0], y[1]); y[2] = 1;
+		gal_to_equ_host(0., M_PI / 2, z[0], z[1]); z[2] = 1;
+		lbd2xyz(x); lbd2xyz(y); lbd2xyz(z);
+		for (int i = 0; i < 3; i++) { go[i][0] = x[i]; go[i][1] = y[i]; go[i][2] = z[i]; }
+	} else if (cs == "galplane") {
+		double Rg = cfg.getd("Rg"), z0 = cfg.getd("z0");
+		double sina = -z0 / Rg, cosa = sqrt(1 - sina * sina);
+		double G[3][3] = {{-cosa, 0, sina}, {0, -1, 0}, {sina, 0, cosa}};
+		memcpy(go, G, sizeof(G));
+	} else {
+		fail("Unknown coordinate system " + cs + ".");
+	}
+}
+
+// xxx2galxyz (common/transform.cpp:150-224): a point given in coordinate
+// system cs (spherical: degrees; an empty type is the origin) to Galactic
+// cartesian.
+void xxx2galxyz(const Config &cfg, const std::string &cs, const std::string &type, double v[3])
+{
+	bool unkt = false;
+	if (type.empty()) v[0] = v[1] = v[2] = 0.;
+	if (cs == "gal") {
+		if (type == "sph") {
+			v[0] *= M_PI / 180.; v[1] *= M_PI / 180.;
+			lbd2xyz(v);
+		} else if (!(type == "xyz" || type.empty())) unkt = true;
+	} else if (cs == "equ") {
+		if (type == "sph" || type.empty()) {
+			v[0] *= M_PI / 180.; v[1] *= M_PI / 180.;
+			equ_to_gal(v[0], v[1], v[0], v[1]);
+			lbd2xyz(v);
+		} else unkt = true;
+	} else if (cs == "galcentric") {
+		if (type == "xyz" || type.empty()) {
+			double Rg = cfg.getd("Rg");
+			v[0] = Rg - v[0];
+			v[1] = -v[1];
+		} else unkt = true;
+	} else if (cs == "galplane") {
+		if (type == "xyz" || type.empty()) {
+			double Rg = cfg.getd("Rg"), z0 = cfg.getd("z0");
+			double sina = -z0 / Rg, cosa = sqrt(1 - sina * sina);
+			double x = cosa * v[0] - sina * v[2];
+			double z = sina * v[0] + cosa * v[2];
+			v[0] = Rg - x;
+			v[1] = -v[1];
+			v[2] = z;
+		} else unkt = true;
+	} else {
+		fail("Unknown coordinate system " + cs + ".");
+	}
+	if (unkt) fail("Unknown rotation type " + type + ".");
+}
+
+// get_rotation_matrix (common/transform.cpp:226-343): '<coordsys> [<kind>
+// params]' with kind axisangle (x y z angle[deg]) or euler (phi theta psi
+// [deg]), composed as rot * gal2xxx.
+void rotation_matrix(double M[3][3], const std::string &spec, const Config &cfg)
+{
+	std::istringstream in(spec);
+	std::string cs;
+	if (!(in >> cs)) {
+		for (int i = 0; i < 3; i++)
+			for (int j = 0; j < 3; j++) M[i][j] = i == j ? 1. : 0.;
+		return;
+	}
+	double R[3][3];
+	gal2xxx(R, cs, cfg);
+	std::string kind;
+	if (!(in >> kind)) {
+		memcpy(M, R, sizeof(R));
+		return;
+	}
+	double rot[3][3];
+	if (kind == "axisangle") {
+		double x, y, z, alpha;
+		if (!(in >> x >> y >> z >> alpha)) fail("Four numbers (x, y, z, angle) required for 'axisangle' rotation.");
+		double norm = sqrt(x * x + y * y + z * z);
+		x /= norm; y /= norm; z /= norm;
+		alpha = -alpha * M_PI / 180.;
+		double c = cos(alpha), sn = sin(alpha), Cc = 1. - c;
+		double xs = x * sn, ys = y * sn, zs = z * sn, xC = x * Cc, yC = y * Cc, zC = z * Cc;
+		double xyC = x * yC, yzC = y * zC, zxC = z * xC;
+		rot[0][0] = x * xC + c; rot[0][1] = xyC - zs;   rot[0][2] = zxC + ys;
+		rot[1][0] = xyC + zs;   rot[1][1] = y * yC + c; rot[1][2] = yzC - xs;
+		rot[2][0] = zxC - ys;   rot[2][1] = yzC + xs;   rot[2][2] = z * zC + c;
+	} else if (kind == "euler") {
+		double phi, theta, psi;
+		if (!(in >> phi >> theta >> psi)) fail("Three angles (phi, theta, psi) required for 'euler' rotation.");
+		phi *= M_PI / 180.; theta *= M_PI / 180.; psi *= M_PI / 180.;
+		rot[0][0] = cos(psi) * cos(phi) - cos(theta) * sin(phi) * sin(psi);
+		rot[0][1] = cos(psi) * sin(phi) + cos(theta) * cos(phi) * sin(psi);
+		rot[0][2] = sin(psi) * sin(theta);
+		rot[1][0] = -sin(psi) * cos(phi) - cos(theta) * sin(phi) * cos(psi);
+		rot[1][1] = -sin(psi) * sin(phi) + cos(theta) * cos(phi) * cos(psi);
+		rot[1][2] = cos(psi) * sin(theta);
+		rot[2][0] = sin(theta) * sin(phi);
+		rot[2][1] = -sin(theta) * cos(phi);
+		rot[2][2] = cos(theta);
+	} else {
+		fail("Unknown rotation type '" + kind + "'");
+	}
+	for (int i = 0; i < 3; i++)
+		for (int j = 0; j < 3; j++) {
+			double s = 0;
+			for (int k = 0; k < 3; k++) s += rot[i][k] * R[k][j];
+			M[i][j] = s;
+		}
+}
+
+} // namespace
+
+// load_transform (common/transform.cpp:345-388): xyz' = M (xyz + T) for a
+// centre '<coordsys> <type> v0 v1 v2' and an orientation as above.
 void load_transform(float T[3], float M[3][3], const std::string &center, const std::string &orientation,
                     const Config &cfg)
 {
 	double ctr[3] = {0, 0, 0};
-	{
-		std::istringstream ss(center);
-		std::string cs, type;
-		ss >> cs >> type >> ctr[0] >> ctr[1] >> ctr[2];
-		if (type.empty()) ctr[0] = ctr[1] = ctr[2] = 0.;
-		if (cs == "gal") {
-			if (type == "sph") {
-				double l = ctr[0] * M_PI / 180., b = ctr[1] * M_PI / 180., d = ctr[2];
-				ctr[0] = d * cos(l) * cos(b); ctr[1] = d * sin(l) * cos(b); ctr[2] = d * sin(b);
-			} else if (!(type == "xyz" || type.empty())) fail("unknown center type " + type);
-		} else if (cs == "galcentric") {
-			double Rg = cfg.getd("Rg");
-			ctr[0] = Rg - ctr[0];
-			ctr[1] = -ctr[1];
-		} else if (cs == "galplane") {
-			double Rg = cfg.getd("Rg"), z0 = cfg.getd("z0");
-			double sina = -z0 / Rg, cosa = sqrt(1 - sina * sina);
-			double x = cosa * ctr[0] - sina * ctr[2];
-			double z = sina * ctr[0] + cosa * ctr[2];
-			ctr[0] = Rg - x;
-			ctr[1] = -ctr[1];
-			ctr[2] = z;
-		} else fail("unsupported center coordinate system " + cs);
-	}
-	T[0] = (float)-ctr[0];
-	T[1] = (float)-ctr[1];
-	T[2] = (float)-ctr[2];
-	double R[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-	std::istringstream ss(orientation);
-	std::string cs;
-	ss >> cs;
-	if (cs == "galplane") {
-		double Rg = cfg.getd("Rg"), z0 = cfg.getd("z0");
-		double sina = -z0 / Rg, cosa = sqrt(1 - sina * sina);
-		double G[3][3] = {{-cosa, 0, sina}, {0, -1, 0}, {sina, 0, cosa}};
-		memcpy(R, G, sizeof(R));
-	} else if (!(cs.empty() || cs == "gal" || cs == "galcentric")) {
-		fail("unsupported orientation " + cs);
-	}
-	std::string kind;
-	if (ss >> kind) fail("rotations on top of the coordinate system are not supported: " + orientation);
+	std::string cs, type;
+	std::istringstream ss(center);
+	ss >> cs >> type >> ctr[0] >> ctr[1] >> ctr[2];
+	xxx2galxyz(cfg, cs, type, ctr);
+	for (int i = 0; i < 3; i++) T[i] = (float)-ctr[i];
+	double R[3][3];
+	rotation_matrix(R, orientation, cfg);
 	for (int i = 0; i < 3; i++)
 		for (int j = 0; j < 3; j++) M[i][j] = (float)R[i][j];
 }

@@ -764,6 +764,14 @@ int gf_count_visible(gf_col hidden, uint64_t n, uint64_t *visible, void *stream)
  *   (modules/photometry_host.cpp:175-349), [tex][nMr][nFeH][4] floats,
  *   returning the number of float4 tables or a negative error. */
 int gf_lf_texture_host(const double *X, const double *Y, int n, int nsamp, float *tex, float tc[2]);
+/* The translation T and rotation M (row-major) of a model or module's
+ * `center` and `orientation` keys, xyz' = M (xyz + T), given the
+ * definitions' Rg and z0 (load_transform, common/transform.cpp:345-388):
+ * centre systems gal (sph | xyz), equ (sph), galcentric, galplane;
+ * orientations gal, equ, galcentric, galplane, each optionally followed by
+ * `axisangle x y z angle` or `euler phi theta psi` (degrees).  GF_E_CONFIG
+ * (with gf_last_error) for anything else. */
+int gf_transform_host(const char *center, const char *orientation, double Rg, double z0, float T[3], float M[9]);
 /* The pixels of a server request's circular beam (centre l, b and radius
  * in degrees, dx-degree cells), as gf_footprint_cover and the catalog's
  * pixelization would give them, computed on the host: writes up to cap

@@ -384,6 +384,8 @@ static float tex3(const gf_tex3d &t, float x, float y, float z)
 	return c0 + c * (c1 - c0);
 }
 
+GFO_API float gfo_tex3d(const gf_tex3d *t, float x, float y, float z) { return tex3(*t, x, y, z); }
+
 /* ======================================================================== */
 /* Models (skygen/model_expDisk.h:71-121, model_powerLawEllipsoid.h:60-105)  */
 /* ======================================================================== */

@@ -325,8 +325,8 @@ def photometry_params(ntex, nF, nM, tc, color_ptrs, flag_ptrs, ncolors, bidx, re
 def multiples_tables(lf_M, lf_phi, fraction=0.5, npix=256):
     """secProb, cumLF, invCumLF as built by os_unresolvedMultiples::construct
     (modules/unresolvedMultiples_host.cpp:140-180)."""
-    x = np.asarray(lf_M, dtype=np.float64)
-    y = np.asarray(lf_phi, dtype=np.float64)
+    x = np.ascontiguousarray(lf_M, dtype=np.float64)
+    y = np.ascontiguousarray(lf_phi, dtype=np.float64)
     sec = np.full(2, fraction, dtype=np.float32)
     sec_tc = np.array([-100.0, 1.0 / 200.0], dtype=np.float32)
     dx = (x[-1] - x[0]) / (npix - 1)

@@ -1728,6 +1728,7 @@ struct SReq {
 	std::vector<uint32_t> state;   // [a | c | x] of S streams
 	uint64_t nunits_model = 0;     // npix * K
 	double expected = 0;
+	double counted = -1;           // expected rows from a count of an earlier batch, -1 before
 };
 
 struct Worker {
@@ -1740,6 +1741,20 @@ struct Worker {
 	hipGraphExec_t chain_exec = nullptr;
 	DevBuf chain_args;
 	gf_serve_chain_args chain_args_h{};
+	// device bytes held by the batch buffers (the graph's argument block aside)
+	uint64_t held() const
+	{
+		return W.bytes + counts.bytes + offs.bytes + pixsum.bytes + segexp.bytes + pix.bytes + state.bytes +
+		       segs.bytes + segall.bytes + reqs.bytes + table.bytes + pos.bytes + zu.bytes + vis.bytes;
+	}
+	void release()
+	{
+		for (DevBuf *b : {&W, &counts, &offs, &pixsum, &segexp, &pix, &state, &segs, &segall, &reqs, &table, &pos,
+		                  &zu, &vis})
+			b->free();
+		table_cap = 0;
+		T = gf_table{};
+	}
 };
 
 } // namespace
@@ -1758,7 +1773,7 @@ struct gf_server {
 	std::mutex mu;
 	std::condition_variable cv_sub, cv_ready, cv_idle;
 	std::deque<std::unique_ptr<SReq>> submitted, ready;
-	uint64_t n_submitted = 0, n_completed = 0, n_batches = 0, n_rows = 0, n_visible = 0;
+	uint64_t n_submitted = 0, n_completed = 0, n_batches = 0, n_rows = 0, n_visible = 0, batch_seq = 0;
 	double t_prep = 0, t_batch = 0;
 	double t_first = 0, t_last = 0;
 	bool stop = false;
@@ -1794,48 +1809,122 @@ void prepare(gf_server &s, SReq &q)
 	                 q.state.data());
 }
 
-size_t table_row_bytes(const gf_server &s)
-{
-	int nb = std::max(1, s.base.nbands), na = std::max(1, s.base.nabsmag);
-	return 16 + 4 + 8 + 4 + 4 + 12 + 4 + 4 + 4 + 4 + 4 + 4 + 4 * na + 4 + 12 + 4 * nb + 4 + 4 * nb + 12 + 16 + 128;
-}
+// The server table's columns (the catalog table of gf_table)
+struct ColDef { gf_col gf_table::*m; uint32_t elem, width; };
 
-void worker_table(gf_server &s, Worker &w, uint64_t cap)
+std::vector<ColDef> table_defs(const gf_server &s)
 {
-	if (w.table_cap >= cap && w.table.p) return;
-	cap = std::max<uint64_t>(cap, 1 << 16);
-	struct ColDef { gf_col gf_table::*m; uint32_t elem, width; };
-	int nb = s.base.nbands > 0 ? s.base.nbands : 1;
-	ColDef defs[] = {
+	const uint32_t nb = (uint32_t)std::max(1, s.base.nbands), na = (uint32_t)std::max(1, s.base.nabsmag);
+	return {
 		{&gf_table::lb, 8, 2}, {&gf_table::projIdx, 4, 1}, {&gf_table::projXY, 4, 2}, {&gf_table::DM, 4, 1},
 		{&gf_table::absmag, 4, 1}, {&gf_table::XYZ, 4, 3}, {&gf_table::Am, 4, 1}, {&gf_table::AmInf, 4, 1},
 		{&gf_table::comp, 4, 1}, {&gf_table::hidden, 4, 1}, {&gf_table::FeH, 4, 1}, {&gf_table::FeH_comp, 4, 1},
-		{&gf_table::absmagSys, 4, (uint32_t)std::max(s.base.nabsmag, 1)}, {&gf_table::absmagSysNcomp, 4, 1},
-		{&gf_table::vcyl, 4, 3}, {&gf_table::mags, 4, (uint32_t)nb}, {&gf_table::photoFlags, 4, 1},
-		{&gf_table::obsMags, 4, (uint32_t)nb}, {&gf_table::pm, 4, 3}, {&gf_table::radec, 8, 2},
+		{&gf_table::absmagSys, 4, na}, {&gf_table::absmagSysNcomp, 4, 1},
+		{&gf_table::vcyl, 4, 3}, {&gf_table::mags, 4, nb}, {&gf_table::photoFlags, 4, 1},
+		{&gf_table::obsMags, 4, nb}, {&gf_table::pm, 4, 3}, {&gf_table::radec, 8, 2},
 	};
+}
+
+uint32_t col_pitch(uint64_t cap, uint32_t elem) { return (uint32_t)(((cap * elem + 127) / 128) * 128); }
+
+size_t table_bytes(const gf_server &s, uint64_t cap)
+{
 	size_t total = 0;
-	for (auto &d : defs) total += ((cap * d.elem + 127) / 128) * 128 * d.width;
-	w.table.alloc(total);
+	for (auto &d : table_defs(s)) total += (size_t)col_pitch(cap, d.elem) * d.width;
+	return total;
+}
+
+void worker_table(gf_server &s, Worker &w, uint64_t cap, uint64_t share)
+{
+	if (w.table_cap >= cap && w.table.p) return;
+	// a quarter of headroom (at least 4,096 rows) when it stays within the share
+	const uint64_t others = w.held() - w.table.bytes;
+	uint64_t want = std::max<uint64_t>(cap + cap / 4, 4096);
+	if (others + table_bytes(s, want) > share) want = cap;
+	cap = want;
+	w.table.alloc(table_bytes(s, cap));
 	char *p = (char *)w.table.p;
-	for (auto &d : defs) {
-		uint32_t pitch = (uint32_t)(((cap * d.elem + 127) / 128) * 128);
+	for (auto &d : table_defs(s)) {
+		const uint32_t pitch = col_pitch(cap, d.elem);
 		w.T.*(d.m) = gf_col{p, pitch, d.width, d.elem, 0};
 		p += (size_t)pitch * d.width;
 	}
 	w.table_cap = cap;
 }
 
-template <typename T> void grow(DevBuf &b, size_t n)
+// Grow a worker buffer to n elements: with a quarter of headroom when the
+// worker's buffers stay within its share of the budget, else exactly.
+template <typename T> void grow(Worker &w, uint64_t share, DevBuf &b, size_t n)
 {
-	if (b.bytes < n * sizeof(T)) b.alloc(n * sizeof(T) + n * sizeof(T) / 4 + 256);
+	const size_t need = n * sizeof(T);
+	if (b.bytes >= need) return;
+	const uint64_t others = w.held() - b.bytes;
+	size_t want = need + need / 4 + 256;
+	if (others + want > share) want = need;
+	b.alloc(want);
 }
 
-int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch)
+// Device bytes of one request in a batch apart from its rows: per unit of
+// every model its weight, count, offset and emission base (8 + 4 + 8 + 8 B),
+// per pixel the pencil beam and its partial sum, per stream its state, its
+// segments and request record.
+uint64_t request_fixed_bytes(const gf_server &s, const SReq &q)
+{
+	const uint64_t nm = s.hmodels.size();
+	return q.nunits_model * nm * 28 + (uint64_t)q.pix.size() * (sizeof(gf_pencil_beam) + 8) + (uint64_t)q.S * 12 +
+	       nm * (2 * sizeof(gf_serve_seg) + 8) + sizeof(gf_serve_req) + 16;
+}
+
+// Device bytes per output row: the table's columns and the module chain's
+// stream positions.
+uint64_t row_device_bytes(const gf_server &s)
+{
+	uint64_t b = 0;
+	for (auto &d : table_defs(s)) b += (uint64_t)d.elem * d.width;
+	return b + 8 * (uint64_t)s.hchain.nplanes;
+}
+
+// Rows a request of expectation N stays below except with probability
+// ~1e-9: six Poisson standard deviations and a floor.
+uint64_t rows_bound(double N)
+{
+	return (uint64_t)ceil(N + 6.0 * sqrt(N) + 32.0);
+}
+
+// The rows a request is admitted with: its counted expectation's bound, or
+// before any count a rough bound from its area (3e8 stars per steradian).
+uint64_t admission_rows(const SReq &q)
+{
+	if (q.counted >= 0) return rows_bound(q.counted);
+	double area = 0;
+	for (auto &p : q.pix) area += (double)p.dA;
+	return rows_bound(area * 3.0e8);
+}
+
+uint64_t worker_share(const gf_server &s)
+{
+	return s.cfg.hbm_budget / (uint64_t)std::max(1, s.cfg.workers);
+}
+
+// Run one batch.  The count runs for every candidate of the batch; the
+// requests whose counted rows no longer fit the worker's share after the
+// first are moved to `deferred` (to go back to the queue, their count kept),
+// and a first request that does not fit the share alone to `rejected`.
+int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch,
+              std::vector<std::unique_ptr<SReq>> &deferred, std::vector<std::unique_ptr<SReq>> &rejected)
 {
 	hipStream_t st = w.st;
 	const int nm = (int)s.hmodels.size();
-	const int nr = (int)batch.size();
+	int nr = (int)batch.size();
+	const uint64_t share = worker_share(s);
+	const uint64_t rowb = row_device_bytes(s);
+	{
+		// the batch's admitted size; the buffers of earlier batches are
+		// released first when they would not leave room for it
+		uint64_t need = 0;
+		for (auto &q : batch) need += request_fixed_bytes(s, *q) + admission_rows(*q) * rowb;
+		if (w.held() + need > share) w.release();
+	}
 	// unit layout: request-major, model-minor
 	std::vector<uint64_t> uoff(nr + 1, 0);
 	std::vector<uint32_t> poff(nr + 1, 0), loff(nr + 1, 0);
@@ -1844,7 +1933,7 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 		poff[r + 1] = poff[r] + (uint32_t)batch[r]->pix.size();
 		loff[r + 1] = loff[r] + batch[r]->S;
 	}
-	const uint64_t U = uoff[nr];
+	const uint64_t Ucand = uoff[nr];
 	std::vector<gf_pencil_beam> pix;
 	pix.reserve(poff[nr]);
 	std::vector<uint32_t> state;
@@ -1855,16 +1944,16 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	}
 	state.resize(soff[nr]);
 	for (int r = 0; r < nr; r++) std::copy(batch[r]->state.begin(), batch[r]->state.end(), state.begin() + soff[r]);
-	grow<gf_pencil_beam>(w.pix, std::max<size_t>(pix.size(), 1));
-	grow<uint32_t>(w.state, std::max<size_t>(state.size(), 1));
-	grow<double>(w.W, U + 1);
-	grow<uint32_t>(w.counts, U + 1);
-	grow<uint64_t>(w.offs, U + 2);
-	grow<double>(w.pixsum, (size_t)poff[nr] + 1);
-	grow<double>(w.segexp, (size_t)nr * nm + 1);
-	grow<gf_serve_seg>(w.segs, (size_t)nr * nm + 1);
-	grow<gf_serve_seg>(w.segall, (size_t)nr * nm + 1);
-	grow<gf_serve_req>(w.reqs, (size_t)nr + 1);
+	grow<gf_pencil_beam>(w, share, w.pix, std::max<size_t>(pix.size(), 1));
+	grow<uint32_t>(w, share, w.state, std::max<size_t>(state.size(), 1));
+	grow<double>(w, share, w.W, Ucand + 1);
+	grow<uint32_t>(w, share, w.counts, Ucand + 1);
+	grow<uint64_t>(w, share, w.offs, Ucand + 2);
+	grow<double>(w, share, w.pixsum, (size_t)poff[nr] + 1);
+	grow<double>(w, share, w.segexp, (size_t)nr * nm + 1);
+	grow<gf_serve_seg>(w, share, w.segs, (size_t)nr * nm + 1);
+	grow<gf_serve_seg>(w, share, w.segall, (size_t)nr * nm + 1);
+	grow<gf_serve_req>(w, share, w.reqs, (size_t)nr + 1);
 	GF_HIP(hipMemcpyAsync(w.pix.p, pix.data(), pix.size() * sizeof(gf_pencil_beam), hipMemcpyHostToDevice, st));
 	GF_HIP(hipMemcpyAsync(w.state.p, state.data(), state.size() * 4, hipMemcpyHostToDevice, st));
 	uint32_t *dstate = (uint32_t *)w.state.p;
@@ -1898,7 +1987,7 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	gf_serve_seg *dsegs = (gf_serve_seg *)w.segs.p;
 	gf_serve_seg *dsegall = (gf_serve_seg *)w.segall.p;
 	const uint64_t *jtab = (const uint64_t *)s.base.jtab.p;
-	grow<uint64_t>(w.zu, U + 1);
+	grow<uint64_t>(w, share, w.zu, Ucand + 1);
 	const gf_model *dmodels = (const gf_model *)s.dmodels.p;
 	const gf_pencil_beam *dpix = (const gf_pencil_beam *)w.pix.p;
 	// count
@@ -1909,6 +1998,36 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	}
 	std::vector<double> segexp((size_t)nm * nr);
 	GF_HIP(hipMemcpyAsync(segexp.data(), w.segexp.p, segexp.size() * 8, hipMemcpyDeviceToHost, st));
+	GF_HIP(hipStreamSynchronize(st));
+	// admission by the counted expectation: the longest prefix of the
+	// candidates whose fixed bytes and bounded rows fit the share (units,
+	// pixels and streams are request-major, so a prefix keeps its layout)
+	{
+		uint64_t used = 0;
+		int k = 0;
+		for (int r = 0; r < nr; r++) {
+			SReq &q = *batch[r];
+			q.counted = 0;
+			for (int m = 0; m < nm; m++) q.counted += segexp[(size_t)m * nr + r];
+			const uint64_t need = request_fixed_bytes(s, q) + rows_bound(q.counted) * rowb;
+			if (k == r && used + need <= share) { used += need; k++; }
+		}
+		if (k == 0) {
+			rejected.push_back(std::move(batch[0]));
+			for (int r = 1; r < nr; r++) deferred.push_back(std::move(batch[r]));
+			batch.clear();
+			return 0;
+		}
+		for (int r = k; r < nr; r++) deferred.push_back(std::move(batch[r]));
+		batch.resize(k);
+		nr = k;
+	}
+	const uint64_t U = uoff[nr];
+	uint64_t bseq;
+	{
+		std::lock_guard<std::mutex> lk(s.mu);
+		bseq = s.batch_seq++;
+	}
 	// plan: every unit of the batch at once (a segment's units start at its
 	// models' offset on its request's streams)
 	int rc = gf::serve_plan(dsegall, nr * nm, U, (const double *)w.W.p, jtab, (uint32_t *)w.counts.p,
@@ -1920,13 +2039,14 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	for (int r = 0; r <= nr; r++)
 		GF_HIP(hipMemcpyAsync(&roff[r], (const uint64_t *)w.offs.p + uoff[r], 8, hipMemcpyDeviceToHost, st));
 	GF_HIP(hipStreamSynchronize(st));
-	for (int r = 0; r < nr; r++) {
-		batch[r]->expected = 0;
-		for (int m = 0; m < nm; m++) batch[r]->expected += segexp[(size_t)m * nr + r];
-	}
+	for (int r = 0; r < nr; r++) batch[r]->expected = batch[r]->counted;
 	const uint64_t nrows = roff[nr];
 	if (nrows >= (1ull << 32)) return GF_E_BUDGET;
-	worker_table(s, w, nrows);
+	// the rows drawn past their bound (a ~1e-9 event) may not fit the share
+	if (w.held() - w.table.bytes - w.pos.bytes + table_bytes(s, nrows) +
+	    (uint64_t)s.hchain.nplanes * nrows * sizeof(uint2) > share)
+		return GF_E_BUDGET;
+	worker_table(s, w, nrows, share);
 	// row0/row1 of the row-ordered table from the scanned offsets
 	rc = gf::serve_seg_rows(dsegall, nr * nm, (const uint64_t *)w.offs.p, st);
 	if (rc) return rc;
@@ -1948,7 +2068,7 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	// the streams move past every unit before the module chain draws from them
 	rc = gf::serve_advance((const gf_serve_req *)w.reqs.p, nr, loff[nr], jtab, st);
 	if (rc) return rc;
-	grow<uint2>(w.pos, std::max<size_t>(1, (size_t)s.hchain.nplanes * nrows));
+	grow<uint2>(w, share, w.pos, std::max<size_t>(1, (size_t)s.hchain.nplanes * nrows));
 	if (s.cfg.eager_chain) {
 		rc = gf::serve_chain((const gf_serve_req *)w.reqs.p, nr, loff[nr], (const gf_serve_chain *)s.dchain.p,
 		                     s.hchain, nrows, w.T, (uint2 *)w.pos.p, st);
@@ -1969,7 +2089,7 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	}
 	uint64_t maxn = 0;
 	for (int r = 0; r < nr; r++) maxn = std::max(maxn, roff[r + 1] - roff[r]);
-	grow<unsigned long long>(w.vis, (size_t)nr + 1);
+	grow<unsigned long long>(w, share, w.vis, (size_t)nr + 1);
 	rc = gf::serve_visible((const gf_serve_req *)w.reqs.p, nr, maxn, w.T.hidden, (unsigned long long *)w.vis.p, st);
 	if (rc) return rc;
 	std::vector<unsigned long long> visv(nr);
@@ -1992,6 +2112,8 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 		res.t_submit = q.t_submit;
 		res.t_start = q.t_start;
 		res.t_done = server_clock();
+		res.device_bytes = request_fixed_bytes(s, q) + rows_bound(q.counted) * rowb;
+		res.batch = bseq;
 		if (s.fn) {
 			std::lock_guard<std::mutex> lk(s.cb_mu);
 			s.fn(s.user, &res, &w.T, roff[r], n);
@@ -2020,6 +2142,8 @@ void fail_batch(gf_server &s, std::vector<std::unique_ptr<SReq>> &batch, int cod
 		res.t_submit = q->t_submit;
 		res.t_start = q->t_start;
 		res.t_done = server_clock();
+		if (q->counted >= 0)
+			res.device_bytes = request_fixed_bytes(s, *q) + rows_bound(q->counted) * row_device_bytes(s);
 		if (s.fn) {
 			std::lock_guard<std::mutex> lk(s.cb_mu);
 			s.fn(s.user, &res, nullptr, 0, 0);
@@ -2056,9 +2180,10 @@ void prep_loop(gf_server &s)
 void worker_loop(gf_server &s, Worker &w)
 {
 	(void)hipSetDevice(0);
-	const size_t rowb = table_row_bytes(s);
+	const uint64_t share = worker_share(s);
+	const uint64_t rowb = row_device_bytes(s);
 	for (;;) {
-		std::vector<std::unique_ptr<SReq>> batch;
+		std::vector<std::unique_ptr<SReq>> batch, deferred, rejected;
 		{
 			std::unique_lock<std::mutex> lk(s.mu);
 			auto &src = s.cfg.plain ? s.submitted : s.ready;
@@ -2067,31 +2192,54 @@ void worker_loop(gf_server &s, Worker &w)
 			if (!s.cfg.plain && s.cfg.linger_us > 0 && (int)src.size() < s.cfg.max_batch)
 				s.cv_ready.wait_for(lk, std::chrono::microseconds(s.cfg.linger_us),
 				                    [&] { return s.stop || (int)src.size() >= s.cfg.max_batch; });
-			// admission: up to max_batch requests whose estimated rows fit the
-			// worker's share of the device budget
-			uint64_t share = s.cfg.hbm_budget / (uint64_t)std::max(1, s.cfg.workers);
+			// admission: up to max_batch requests whose rows (counted when an
+			// earlier batch counted them, else bounded from their area) fit the
+			// worker's share of the device budget; a counted request that does
+			// not fit the share alone is rejected
 			uint64_t used = 0;
 			while (!src.empty() && (int)batch.size() < std::max(1, s.cfg.max_batch)) {
 				SReq &q = *src.front();
-				double area = 0;
-				for (auto &p : q.pix) area += (double)p.dA;
-				// rough bound: unit arrays plus rows at 3e8 stars per steradian
-				uint64_t est = (q.nunits_model + 1) * (uint64_t)(24 + 8 * s.hmodels.size()) +
-				               (uint64_t)(area * 3.0e8) * rowb;
-				if (!batch.empty() && used + est > share) break;
-				used += est;
+				const uint64_t need = request_fixed_bytes(s, q) + admission_rows(q) * rowb;
+				if (q.counted >= 0 && need > share) {
+					rejected.push_back(std::move(src.front()));
+					src.pop_front();
+					continue;
+				}
+				if (!batch.empty() && used + need > share) break;
+				used += need;
 				batch.push_back(std::move(src.front()));
 				src.pop_front();
 			}
 		}
+		if (!rejected.empty()) {
+			fail_batch(s, rejected, GF_E_BUDGET);
+			rejected.clear();
+		}
+		if (batch.empty()) continue;
 		double t0 = server_clock();
 		for (auto &q : batch) {
 			if (s.cfg.plain) prepare(s, *q);
 			q->t_start = t0;
 		}
 		double t1 = server_clock();
-		int rc = run_batch(s, w, batch);
+		int rc;
+		try {
+			rc = run_batch(s, w, batch, deferred, rejected);
+		} catch (const ConfigError &e) {   // a device allocation failed
+			g_last_error = e.msg;
+			w.release();
+			rc = GF_E_BUDGET;
+		}
 		if (rc) fail_batch(s, batch, rc);
+		if (!rejected.empty()) fail_batch(s, rejected, GF_E_BUDGET);
+		if (!deferred.empty()) {
+			{
+				std::lock_guard<std::mutex> lk(s.mu);
+				auto &src = s.cfg.plain ? s.submitted : s.ready;
+				for (auto it = deferred.rbegin(); it != deferred.rend(); ++it) src.push_front(std::move(*it));
+			}
+			s.cv_ready.notify_all();
+		}
 		{
 			std::lock_guard<std::mutex> lk(s.mu);
 			s.t_batch += server_clock() - t1;

@@ -13,7 +13,8 @@ class gf_request(C.Structure):
 class gf_result(C.Structure):
     _fields_ = [("id", C.c_uint64), ("status", C.c_int32), ("batch_size", C.c_int32), ("npixels", C.c_uint64),
                 ("generated", C.c_uint64), ("visible", C.c_uint64), ("expected", C.c_double),
-                ("t_submit", C.c_double), ("t_start", C.c_double), ("t_done", C.c_double)]
+                ("t_submit", C.c_double), ("t_start", C.c_double), ("t_done", C.c_double),
+                ("device_bytes", C.c_uint64), ("batch", C.c_uint64)]
 
 
 class gf_server_config(C.Structure):

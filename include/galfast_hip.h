@@ -712,6 +712,10 @@ typedef struct gf_result {
     uint64_t visible;       /* rows with hidden == 0 */
     double   expected;      /* expected stars (pixelized footprint) */
     double   t_submit, t_start, t_done;   /* seconds on the server's clock */
+    uint64_t device_bytes;  /* device bytes the request was admitted with (its units,
+                             * pixels and streams, and its rows bounded by the counted
+                             * expectation plus six Poisson deviations); 0 before a count */
+    uint64_t batch;         /* sequence number of the batch that carried it */
 } gf_result;
 
 /* Called on a server thread when a request completes.  cols are the batch's

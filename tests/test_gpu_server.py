@@ -146,3 +146,57 @@ def test_request_matches_oracle_replay(tmp_path, alone, rid):
     assert res.generated == ref.n
     same_rows(t, ref, "request %d vs oracle" % rid)
     assert res.visible == int((ref.arr("hidden")[0] == 0).sum())
+
+
+GF_E_BUDGET = -7
+BIG = (99, 60.0, 40.0, 6.0, 25.0, 19)    # ~113 deg^2 to r = 25
+
+
+def serve_status(conf, reqs, **kw):
+    """serve(), keeping failed requests (status != 0, no rows)."""
+    got = {}
+
+    def on_result(res, table, row0, nrows):
+        got[res.id] = (res, fetch_rows(table, row0, nrows) if res.status == 0 else None)
+
+    srv = Server(conf, on_result, **kw)
+    try:
+        for r in reqs:
+            srv.submit(*r)
+        srv.wait()
+    finally:
+        srv.close()
+    return got
+
+
+def test_budget_admission_by_counted_rows(base, alone):
+    """Requests are admitted by their counted expectation: a batch's requests
+    fit the worker's share of the budget together, a request that cannot fit
+    it alone fails with GF_E_BUDGET, and the others' rows are unchanged."""
+    ref, _ = alone
+    mix = REQUESTS[:4] + [BIG] + REQUESTS[4:]
+    roomy = serve_status(base, mix, max_batch=16, workers=1, linger_us=300000)
+    assert all(res.status == 0 for res, _ in roomy.values())
+    need = {rid: res.device_bytes for rid, (res, _) in roomy.items()}
+    small = sorted(need[r[0]] for r in REQUESTS)
+    budget = small[-1] + small[-2] // 2          # the largest small request fits, with few others
+    assert need[BIG[0]] > budget and sum(small) > budget
+    got = serve_status(base, mix, max_batch=16, workers=1, linger_us=300000, hbm_budget=budget)
+    assert set(got) == set(need)
+    assert got[BIG[0]][0].status == GF_E_BUDGET
+    per_batch = {}
+    for rid, (res, t) in got.items():
+        if rid == BIG[0]:
+            continue
+        assert res.status == 0, (rid, res.status)
+        r0, t0 = ref[rid]
+        assert (res.generated, res.visible, res.expected) == (r0.generated, r0.visible, r0.expected)
+        assert res.device_bytes == need[rid]
+        same_rows(t, t0, "request %d under a small budget" % rid)
+        per_batch[res.batch] = per_batch.get(res.batch, 0) + res.device_bytes
+    assert len(per_batch) > 1 and max(per_batch.values()) <= budget
+    # the same requests with two workers share the budget between them
+    got2 = serve_status(base, REQUESTS, max_batch=16, workers=2, linger_us=100000, hbm_budget=2 * budget)
+    for rid, (res, t) in got2.items():
+        assert res.status == 0
+        same_rows(t, ref[rid][1], "request %d, two workers" % rid)

@@ -314,14 +314,16 @@ def write_fits_cube(path, cube, axes, extra=None):
         f.write(data)
 
 
-def write_c4(d, seed=42, small=False):
+def write_c4(d, seed=42, small=False, foot=None, m1=None):
     """Config 4: 6 synthetic bands, unresolved binaries (fraction 0.5,
     companion magnitudes from the LF, fainter than the primary) and an
     analytic exponential dust disk (h = 100 pc, 0.5 mag/kpc in r) as north
     and south (X, Y, DM) extinction cubes, r in [14, 27.5] over
     l in [60, 100], b in [20, 50] (~970 deg^2) in 0.25 deg pixels.  small:
     the config-0 sky (10x10 deg, 1 deg pixels, r 15-21) moved to b in
-    [10, 20] with 64x64x31 cubes, for tests."""
+    [10, 20] with 64x64x31 cubes, for tests.  foot: the footprint file's
+    text after "module = footprint" instead of the rect; m1: the faint
+    limit."""
     write_common(d, seed)
     np.savetxt(os.path.join(d, "photosys.txt"), seeded_photosys(seed, ncolors=5), fmt="%.8f")
     m = np.round(np.arange(10.0, 30.0 + 0.005, 0.01), 6)
@@ -352,12 +354,14 @@ lumfunc = lf.txt""")
         write_fits_cube(os.path.join(d, "ext.%s.fits" % name), cube, axes, {"LAMBDA0": pole[0], "PHI1": pole[1]})
     _w(os.path.join(d, "ext.conf"), "north = ext.north.fits\nsouth = ext.south.fits")
     if small:
-        sky, foot = ("m0 = 15\nm1 = 21\ndx = 1", "rect = 85 95 10 20")
+        sky, body = ("m0 = 15\nm1 = %r\ndx = 1" % (21.0 if m1 is None else float(m1)),
+                     "type = rect\ncoordsys = gal\nrect = 85 95 10 20")
     else:
-        sky, foot = ("m0 = 14\nm1 = 27.5\ndx = 0.25", "rect = 60 100 20 50")
+        sky, body = ("m0 = 14\nm1 = %r\ndx = 0.25" % (27.5 if m1 is None else float(m1)),
+                     "type = rect\ncoordsys = gal\nrect = 60 100 20 50")
     _w(os.path.join(d, "skygen.conf"), "module = skygen\nband = SYNr\ndm = 0.05\nM0 = 3\nM1 = 15\n%s\n"
        "extmaps = ext.conf\nmaxstars = 0" % sky)
-    _w(os.path.join(d, "foot.conf"), "module = footprint\ntype = rect\ncoordsys = gal\n%s" % foot)
+    _w(os.path.join(d, "foot.conf"), "module = footprint\n%s" % (body if foot is None else foot))
     _w(os.path.join(d, "cmd.conf"), """
 module = config
 seed = %d
@@ -371,6 +375,27 @@ modules = feh.conf multiples.conf photometry.conf photoerr.conf""" % seed)
 
 def write_c4_small(d, seed=42):
     return write_c4(d, seed, small=True)
+
+
+def foot_body(l, b, radius, l1=0.0, b1=0.0):
+    """Footprint file text of a server request: a beam, or with radius 0 the
+    Galactic rect [l, l1] x [b, b1]."""
+    if radius > 0:
+        return "type = beam\ncoordsys = gal\nfootprint_beam = %r %r %r" % (float(l), float(b), float(radius))
+    return "type = rect\ncoordsys = gal\nrect = %r %r %r %r" % (float(l), float(l1), float(b), float(b1))
+
+
+def c4_tiles(nl=20, nb=10, l=(60.0, 100.0), b=(20.0, 50.0), m1=27.5, id0=1):
+    """The config-4 footprint as nl x nb Galactic rect sub-requests
+    (id, l0, b0, 0, m1, seed, l1, b1) that tile it exactly, seed per id."""
+    out = []
+    dl, db = (l[1] - l[0]) / nl, (b[1] - b[0]) / nb
+    for i in range(nl):
+        for j in range(nb):
+            rid = id0 + len(out)
+            out.append((rid, l[0] + i * dl, b[0] + j * db, 0.0, float(m1), request_seed(rid),
+                        l[0] + (i + 1) * dl, b[0] + (j + 1) * db))
+    return out
 
 
 WRITERS = {"c0": write_c0, "c1": write_c1, "c3": write_c3, "c4": write_c4, "c4s": write_c4_small}

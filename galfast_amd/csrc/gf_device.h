@@ -6,6 +6,7 @@
 #include "../../include/galfast_hip.h"
 #include "gf_math.h"
 #include "gf_rng.h"
+#include "gf_sky.h"
 
 #define GF_D __device__ __forceinline__
 
@@ -143,99 +144,6 @@ GF_D void lambert_deproject_dir(const gf_lambert &p, double &l, double &b, doubl
 }
 
 static constexpr float POGSON = 0.4605170185988091f;   // 0.2 ln 10 (skygen/skygen.cu.h:466)
-static constexpr double DBL_R2D = 57.2957795130823208767981548;
-static constexpr double DBL_D2R = 0.01745329251994329576923691;
-
-// J2000 Galactic <-> equatorial constants (modules/module_lib.h:62-71)
-namespace galequ_c {
-static constexpr double angp = 0.01745329251994329576923691 * 192.859508333;
-static constexpr double l0 = 0.01745329251994329576923691 * 32.932;
-static constexpr double ce = 0.88998740217659689;
-static constexpr double se = 0.45598511375586859;
-static constexpr double pi2 = 6.28318530717958647692528677;
-static constexpr double d2r = 0.01745329251994329576923691;
-}
-
-// Galactic (l, b) -> equatorial (ra, dec), radians, ra in [0, 2 pi)
-// (galequ, modules/gal2other_gpu.cu.h:39-60)
-GF_D void gal_to_equ(double l, double b, double &ra, double &dec)
-{
-	using namespace galequ_c;
-	double cb, sb, cl, sl;
-	gfm::sincos(b, &sb, &cb);
-	gfm::sincos(l - l0, &sl, &cl);
-	ra = gfm::atan2(cb * cl, sb * ce - cb * se * sl) + angp;
-	dec = gfm::asin(cb * ce * sl + sb * se);
-	while (ra < 0.) ra += pi2;
-	while (ra >= pi2) ra -= pi2;
-}
-
-// One footprint region (footprint.cpp:486-727): 1 if the Galactic direction
-// (l, b) [radians] lies outside it.  EQU = false compiles rects in Galactic
-// coordinates only (the equatorial transform's registers stay out of the
-// kernels that never need it).
-template <bool EQU = true>
-GF_D int outside_region(const gf_foot_region &f, double l, double b)
-{
-	const double R2D = DBL_R2D;
-	if (f.bmin > 0. && fabs(b * R2D) < f.bmin) return 1;
-	if (f.kind == 1) {
-		double lon = l, lat = b;
-		if (EQU && f.equ) gal_to_equ(l, b, lon, lat);
-		double ld = lon * R2D, bd = lat * R2D;
-		while (ld < 0.) ld += 360.;
-		while (ld >= 360.) ld -= 360.;
-		if (bd < f.b0 || bd > f.b1) return 1;
-		double lo = f.l0, hi = f.l1;
-		if (hi - lo >= 360.) return 0;
-		double t = ld - lo;
-		while (t < 0.) t += 360.;
-		while (t >= 360.) t -= 360.;
-		return t > (hi - lo) ? 1 : 0;
-	}
-	if (f.kind == 2) {
-		double cb, sb, cl, sl;
-		gfm::sincos(b, &sb, &cb);
-		gfm::sincos(l, &sl, &cl);
-		double d = cb * cl * f.cb * f.cl + cb * sl * f.cb * f.sl + sb * f.sb;
-		if (d < f.cosr) return 1;
-		return (f.hole > 0. && 1. - d < f.hole) ? 1 : 0;
-	}
-	return 1;
-}
-
-// The footprint (the union of its regions; no regions = the whole sky).
-// FULL = false: the footprint is one region, not an equatorial rect
-// (footprint_is_simple), and only that region is read.
-template <bool FULL = true>
-GF_D int outside_footprint(const gf_footprint &f, double l, double b)
-{
-	if (f.nregion <= 0) return 0;
-	if (!FULL) return outside_region<false>(f.r[0], l, b);
-	for (int k = 0; k < f.nregion && k < GF_FOOT_MAX; k++)
-		if (!outside_region(f.r[k], l, b)) return 0;
-	return 1;
-}
-
-inline bool footprint_is_simple(const gf_footprint &f)
-{
-	return f.nregion == 0 || (f.nregion == 1 && !(f.r[0].kind == 1 && f.r[0].equ));
-}
-
-// Deprojection of a pole-centred Lambert plane (hemi 0: pole b = +90 deg,
-// l0 = 90 deg; hemi 1: b = -90 deg, l0 = 270 deg), radians, l in [0, 2 pi)
-// (peyton::math::lambert::deproject with b0 = +-pi/2).
-GF_D void pole_deproject(int hemi, double x, double y, double &l, double &b)
-{
-	const double PI = 3.14159265358979323846;
-	double r2 = x * x + y * y;
-	double sb = fmin(1.0, fmax(-1.0, 1. - 0.5 * r2));
-	if (hemi == 0) { b = gfm::asin(sb); l = 0.5 * PI + gfm::atan2(x, -y); }
-	else { b = -gfm::asin(sb); l = 1.5 * PI + gfm::atan2(x, y); }
-	while (l < 0.) l += 2 * PI;
-	while (l >= 2 * PI) l -= 2 * PI;
-}
-
 // Valid absolute-magnitude-bin window of DM bin k of one pixel
 // (the cell skips of skygen/skygen.cu.h:551-559): cells (im = k - iM, iM)
 // with 0 <= im < nm, 0 <= iM < nM, not cut by the distance limits and with

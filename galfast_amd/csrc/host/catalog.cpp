@@ -7,6 +7,7 @@
 #include "../gf_runtime.h"
 #include "../gf_chain_types.h"
 #include "../gf_rng.h"
+#include "../gf_sky.h"
 
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -280,66 +281,73 @@ gf_pencil_beam cell_pixel(int hemi, int n, int i, int j, double dx, int inside)
 	return pb;
 }
 
-// Pixels of a one-beam footprint (a server request): only the grid cells
-// around the beam are sampled, with the device kernel's beam test, so the
-// pixels equal those gf_footprint_cover gives for the same beam.
-std::vector<gf_pencil_beam> pixelize_beam(const gf_footprint &fp, double dx)
+// The pixels of a one-region footprint -- a beam without a hole, or a rect
+// in Galactic coordinates -- on the catalog's pole grids: the cells around
+// the region's projection, each with the number of its 8 x 8 samples that
+// the device's coverage test (sample_inside) finds inside, in (hemisphere,
+// i, j) order: the pixels a catalog job on that footprint gets.
+std::vector<gf_pencil_beam> pixelize_region(const gf_footprint &fp, double dx)
 {
 	std::vector<gf_pencil_beam> out;
-	if (fp.nregion != 1 || fp.r[0].kind != 2 || fp.r[0].bmin > 0. || fp.r[0].hole > 0.)
-		fail("pixelize_beam: not a single beam");
+	if (fp.nregion != 1 || !((fp.r[0].kind == 2 && fp.r[0].hole <= 0.) || (fp.r[0].kind == 1 && !fp.r[0].equ)))
+		fail("pixelize_region: not a single beam or Galactic rect");
 	const gf_foot_region &f = fp.r[0];
 	const int n = pixel_grid_n(dx);
-	const double x0 = -n * dx / 2, y0 = -n * dx / 2;
-	const double bc[3] = {f.cb * f.cl, f.cb * f.sl, f.sb};
-	const double r = acos(std::max(-1.0, std::min(1.0, f.cosr)));
+	const double x0 = -n * dx / 2, y0 = -n * dx / 2, d2r = M_PI / 180.;
 	for (int hemi = 0; hemi < 2; hemi++) {
-		// grid cells around the beam: project a polar sampling of the cap
-		double xmin = 1e9, xmax = -1e9, ymin = 1e9, ymax = -1e9;
-		bool any = false;
-		double e1[3] = {-f.sl, f.cl, 0}, e2[3] = {-f.sb * f.cl, -f.sb * f.sl, f.cb};
-		for (int ir = 0; ir <= 24; ir++) {
-			double rr = r * ir / 24.0;
-			int na = ir == 0 ? 1 : 96;
-			for (int ia = 0; ia < na; ia++) {
-				double a = 2 * M_PI * ia / na, v[3];
-				for (int q = 0; q < 3; q++) v[q] = cos(rr) * bc[q] + sin(rr) * (cos(a) * e1[q] + sin(a) * e2[q]);
-				double b = asin(std::max(-1.0, std::min(1.0, v[2]))), l = atan2(v[1], v[0]);
-				if ((hemi == 0 && b < 0) || (hemi == 1 && b >= 0)) continue;
-				double x, y;
-				proj_pole(hemi, l, b, x, y);
-				xmin = std::min(xmin, x); xmax = std::max(xmax, x);
-				ymin = std::min(ymin, y); ymax = std::max(ymax, y);
-				any = true;
+		// points of the region in this hemisphere whose projections bound it
+		std::vector<std::pair<double, double>> lb;
+		if (f.kind == 2) {
+			// a polar sampling of the cap, and the arc of the equator inside
+			// it: a cap that crosses the equator by less than the sampling
+			// step still reaches both hemispheres along that arc
+			const double bc[3] = {f.cb * f.cl, f.cb * f.sl, f.sb};
+			const double r = acos(std::max(-1.0, std::min(1.0, f.cosr)));
+			const double e1[3] = {-f.sl, f.cl, 0}, e2[3] = {-f.sb * f.cl, -f.sb * f.sl, f.cb};
+			for (int ir = 0; ir <= 24; ir++) {
+				double rr = r * ir / 24.0;
+				int na = ir == 0 ? 1 : 96;
+				for (int ia = 0; ia < na; ia++) {
+					double a = 2 * M_PI * ia / na, v[3];
+					for (int q = 0; q < 3; q++) v[q] = cos(rr) * bc[q] + sin(rr) * (cos(a) * e1[q] + sin(a) * e2[q]);
+					lb.emplace_back(atan2(v[1], v[0]), asin(std::max(-1.0, std::min(1.0, v[2]))));
+				}
+			}
+			const double bcen = asin(std::max(-1.0, std::min(1.0, f.sb)));
+			if (bcen - r <= 0 && bcen + r >= 0) {
+				const double lc = atan2(f.sl, f.cl);
+				const double cr = f.cb > 1e-12 ? f.cosr / f.cb : (f.cosr <= 0 ? -1.0 : 2.0);
+				const double half = cr >= 1.0 ? 0.0 : (cr <= -1.0 ? M_PI : acos(cr));
+				for (int t = 0; t <= 96; t++) lb.emplace_back(lc + half * (2.0 * t / 96.0 - 1.0), 0.0);
+			}
+		} else {
+			// the rect's edges (its projection is bounded by their images),
+			// cut at the equator
+			double b0 = f.b0, b1 = f.b1;
+			if (hemi == 0) b0 = std::max(b0, 0.0); else b1 = std::min(b1, 0.0);
+			if (b0 > b1) continue;
+			const double l0 = f.l0, l1 = f.l1 - f.l0 >= 360. ? f.l0 + 360. : f.l1;
+			for (int t = 0; t <= 256; t++) {
+				const double l = l0 + (l1 - l0) * t / 256.0, b = b0 + (b1 - b0) * t / 256.0;
+				lb.emplace_back(l * d2r, b0 * d2r);
+				lb.emplace_back(l * d2r, b1 * d2r);
+				lb.emplace_back(l0 * d2r, b * d2r);
+				lb.emplace_back(l1 * d2r, b * d2r);
 			}
 		}
-		// the arc of the equator inside the cap: a cap that crosses the
-		// equator by less than the sampling step still reaches both
-		// hemispheres, and its cells along the equator lie on this arc
-		const double bcen = asin(std::max(-1.0, std::min(1.0, f.sb)));
-		if (bcen - r <= 0 && bcen + r >= 0) {
-			const double lc = atan2(f.sl, f.cl);
-			const double cr = f.cb > 1e-12 ? f.cosr / f.cb : (f.cosr <= 0 ? -1.0 : 2.0);
-			const double half = cr >= 1.0 ? 0.0 : (cr <= -1.0 ? M_PI : acos(cr));
-			for (int t = 0; t <= 96; t++) {
-				double l = lc + half * (2.0 * t / 96.0 - 1.0), x, y;
-				proj_pole(hemi, l, 0.0, x, y);
-				xmin = std::min(xmin, x); xmax = std::max(xmax, x);
-				ymin = std::min(ymin, y); ymax = std::max(ymax, y);
-				any = true;
-			}
+		double xmin = 1e9, xmax = -1e9, ymin = 1e9, ymax = -1e9;
+		bool any = false;
+		for (auto &p : lb) {
+			if ((hemi == 0 && p.second < 0) || (hemi == 1 && p.second > 0)) continue;
+			double x, y;
+			proj_pole(hemi, p.first, p.second, x, y);
+			xmin = std::min(xmin, x); xmax = std::max(xmax, x);
+			ymin = std::min(ymin, y); ymax = std::max(ymax, y);
+			any = true;
 		}
 		if (!any) continue;
 		const int i0 = std::max(0, (int)floor((xmin - x0) / dx) - 2), i1 = std::min(n, (int)floor((xmax - x0) / dx) + 3);
 		const int j0 = std::max(0, (int)floor((ymin - y0) / dx) - 2), j1 = std::min(n, (int)floor((ymax - y0) / dx) + 3);
-		auto in_beam = [&](double x, double y) {
-			double r2 = x * x + y * y;
-			if (r2 > 2.0) return false;
-			double s = sqrt(std::max(0.0, 1.0 - 0.25 * r2)), z = 1.0 - 0.5 * r2;
-			double v0 = hemi == 0 ? -x * s : x * s, v1 = -y * s, v2 = hemi == 0 ? z : -z;
-			if ((hemi == 0 && v2 < 0) || (hemi == 1 && v2 >= 0)) return false;
-			return v0 * bc[0] + v1 * bc[1] + v2 * bc[2] >= f.cosr;
-		};
 		for (int i = i0; i < i1; i++)
 			for (int j = j0; j < j1; j++) {
 				double xa = x0 + i * dx, ya = y0 + j * dx;
@@ -349,7 +357,7 @@ std::vector<gf_pencil_beam> pixelize_beam(const gf_footprint &fp, double dx)
 				int inside = 0;
 				for (int a = 0; a < 8; a++)
 					for (int b2 = 0; b2 < 8; b2++)
-						inside += in_beam(xa + dx * (a + 0.5) / 8, ya + dx * (b2 + 0.5) / 8) ? 1 : 0;
+						inside += sample_inside(fp, hemi, xa + dx * (a + 0.5) / 8, ya + dx * (b2 + 0.5) / 8) ? 1 : 0;
 				if (inside) out.push_back(cell_pixel(hemi, n, i, j, dx, inside));
 			}
 	}
@@ -379,6 +387,23 @@ void pixelize(gf_catalog &c, double dx)
 
 // One footprint configuration file -> one region (load_footprint,
 // footprint.cpp:729-744).
+// A rect region, longitudes as footprint.cpp:486-515 reduces them: to
+// [0, 360), l0 >= l1 wrapping through 0 (l0 = l1 = 0 is the whole ring),
+// and within 1e-5 of a full turn the whole ring (:515-521).
+gf_foot_region rect_region(double l0, double l1, double b0, double b1)
+{
+	gf_foot_region f{};
+	l0 = fmod(l0, 360.);
+	l1 = fmod(l1, 360.);
+	if (l0 < 0) l0 += 360.;
+	if (l1 < 0) l1 += 360.;
+	if (l0 >= l1) { if (l0 == 0) l1 = 360.; else l0 -= 360.; }
+	if (fabs((l1 - l0) / 360. - 1.) < 1e-5) { l0 = 0.; l1 = 360.; }
+	f.kind = 1;
+	f.l0 = l0; f.l1 = l1; f.b0 = b0; f.b1 = b1;
+	return f;
+}
+
 gf_foot_region load_region(const Config &fc)
 {
 	std::string type = fc.get("type", "");
@@ -393,15 +418,9 @@ gf_foot_region load_region(const Config &fc)
 		// wraps (l0 = l1 = 0 is the whole ring)
 		std::vector<double> r = fc.getv("rect", "0 1 0 1");
 		if (r.size() < 4) fail("rect needs l0 l1 b0 b1");
-		double l0 = fmod(r[0], 360.), l1 = fmod(r[1], 360.);
-		if (l0 < 0) l0 += 360.;
-		if (l1 < 0) l1 += 360.;
-		if (l0 >= l1) { if (l0 == 0) l1 = 360.; else l0 -= 360.; }
-		// within 1e-5 of a full turn the reference builds the whole ring
-		// (footprint.cpp:515-521)
-		if (fabs((l1 - l0) / 360. - 1.) < 1e-5) { l0 = 0.; l1 = 360.; }
-		f.kind = 1;
-		f.l0 = l0; f.l1 = l1; f.b0 = r[2]; f.b1 = r[3];
+		const bool equ = f.equ;
+		f = rect_region(r[0], r[1], r[2], r[3]);
+		f.equ = equ;
 	} else if (type == "beam") {
 		// footprint.cpp:675-700: centre l b, radius r, optional hole radius
 		if (!fc.count("footprint_beam")) fail("Configuration key footprint_beam must be set");
@@ -436,10 +455,10 @@ void load_footprints(gf_catalog &c)
 // find_extinction_minima (skygen/os_skygen.cpp:285-341): for every DM slice
 // z, the smallest extinction one slice closer over the map texels that can
 // overlap the pixel (texels outside the hemisphere with zero extinction
-// skipped); proj_x, proj_y: the pixel centre in the map's projection.
-void extinction_minima(Cube &beam, int extIdx, double proj_x, double proj_y, double dx, const Cube &tex)
+// skipped), folded into out[z]; proj_x, proj_y: the pixel centre in the
+// map's projection.
+void extinction_minima(float *out, double proj_x, double proj_y, double dx, const Cube &tex)
 {
-	const int xidx = extIdx / 2048, yidx = extIdx % 2048;
 	const float *tcx = tex.tc[0], *tcy = tex.tc[1];
 	double dr = 1.0001 / 2. * sqrt(2.) * (dx + std::max(1. / tcx[1], 1. / tcy[1]));
 	int xi = (int)((proj_x - dr - tcx[0]) * tcx[1]);  xi = std::max(0, std::min(xi - 1, tex.nx));
@@ -454,11 +473,46 @@ void extinction_minima(Cube &beam, int extIdx, double proj_x, double proj_y, dou
 			}
 			for (int z = 0; z < tex.nz; z++) {
 				float Ar = z == 0 ? 0.f : tex.at(x, y, z - 1);
-				float &b = beam.at(xidx, yidx, z);
-				b = std::min(b, Ar);
+				out[z] = std::min(out[z], Ar);
 			}
 		}
 	}
+}
+
+// The minima of every pixel, nz per pixel ([pixel][z]), over the north and
+// south maps that are loaded (os_skygen.cpp:773-784).
+std::vector<float> pixel_extinction_minima(const gf_catalog &c, const std::vector<gf_pencil_beam> &pix)
+{
+	const Cube &ref = c.ext_north.v.empty() ? c.ext_south : c.ext_north;
+	const int nz = ref.nz;
+	std::vector<float> out((size_t)pix.size() * nz, std::numeric_limits<float>::infinity());
+	for (size_t i = 0; i < pix.size(); i++) {
+		const gf_pencil_beam &p = pix[i];
+		double l = atan2((double)p.sl, (double)p.cl), b = atan2((double)p.sb, (double)p.cb);
+		double x, y;
+		if (!c.ext_north.v.empty()) { proj_pole(0, l, b, x, y); extinction_minima(&out[i * nz], x, y, p.dx, c.ext_north); }
+		if (!c.ext_south.v.empty()) { proj_pole(1, l, b, x, y); extinction_minima(&out[i * nz], x, y, p.dx, c.ext_south); }
+	}
+	return out;
+}
+
+// The minima as the draw samples them: a 3D texture with pixel g at texel
+// (g div 2048, g mod 2048) and the DM slices along z
+// (skygen/os_skygen.cpp:767-775).  Texels past the last pixel hold 0, not
+// the reference's infinity: a pixel's own texel is read with weight 1 and its
+// neighbour in x with weight 0, and 0 * infinity would make the sample NaN.
+void beam_texture(Cube &beam, const std::vector<float> &minima, size_t npix, int nz, const float tcz[2])
+{
+	beam.nx = std::max<int>(1, (int)(npix / 2048 + (npix % 2048 ? 1 : 0)));
+	beam.ny = std::max<int>(1, (int)std::min<size_t>(npix, 2048));
+	beam.nz = nz;
+	beam.v.assign((size_t)beam.nx * beam.ny * beam.nz, 0.f);
+	texcoord_from_range(0, (float)beam.nx, 0, (float)beam.nx, beam.tc[0]);
+	texcoord_from_range(0, (float)beam.ny, 0, (float)beam.ny, beam.tc[1]);
+	beam.tc[2][0] = tcz[0];
+	beam.tc[2][1] = tcz[1];
+	for (size_t g = 0; g < npix; g++)
+		for (int z = 0; z < nz; z++) beam.at((int)(g / 2048), (int)(g % 2048), z) = minima[g * nz + z];
 }
 
 gf_tex3d tex3d_of(const Cube &c, const float *data)
@@ -491,24 +545,10 @@ void load_extinction(gf_catalog &c, const std::string &econf)
 	if (!nf.empty() && !sf.empty() && c.ext_north.nz != c.ext_south.nz)
 		fail("north and south extinction maps differ in depth");
 	const Cube &ref = nf.empty() ? c.ext_south : c.ext_north;
-	Cube &beam = c.ext_beam;
 	const int np = (int)c.pixels.size();
-	beam.nx = std::max(1, np / 2048 + (np % 2048 ? 1 : 0));
-	beam.ny = std::max(1, std::min(np, 2048));
-	beam.nz = ref.nz;
-	beam.v.assign((size_t)beam.nx * beam.ny * beam.nz, std::numeric_limits<float>::infinity());
-	texcoord_from_range(0, (float)beam.nx, 0, (float)beam.nx, beam.tc[0]);
-	texcoord_from_range(0, (float)beam.ny, 0, (float)beam.ny, beam.tc[1]);
-	beam.tc[2][0] = ref.tc[2][0];
-	beam.tc[2][1] = ref.tc[2][1];
-	for (int i = 0; i < np; i++) {
-		gf_pencil_beam &p = c.pixels[i];
-		p.extIdx = i;
-		double l = atan2((double)p.sl, (double)p.cl), b = atan2((double)p.sb, (double)p.cb);
-		double x, y;
-		if (!nf.empty()) { proj_pole(0, l, b, x, y); extinction_minima(beam, i, x, y, p.dx, c.ext_north); }
-		if (!sf.empty()) { proj_pole(1, l, b, x, y); extinction_minima(beam, i, x, y, p.dx, c.ext_south); }
-	}
+	for (int i = 0; i < np; i++) c.pixels[i].extIdx = i;
+	beam_texture(c.ext_beam, pixel_extinction_minima(c, c.pixels), c.pixels.size(), ref.nz, ref.tc[2]);
+	Cube &beam = c.ext_beam;
 	c.ext_dev[0].upload(c.ext_north.v);
 	c.ext_dev[1].upload(c.ext_south.v);
 	c.ext_dev[2].upload(beam.v);
@@ -1626,7 +1666,24 @@ int gf_beam_pixels_host(double l, double b, double radius, double dx, gf_pencil_
 		const double d2r = M_PI / 180.0;
 		f.cl = cos(l * d2r); f.sl = sin(l * d2r); f.cb = cos(b * d2r); f.sb = sin(b * d2r);
 		f.cosr = cos(radius * d2r);
-		std::vector<gf_pencil_beam> v = pixelize_beam(fp, dx * d2r);
+		std::vector<gf_pencil_beam> v = pixelize_region(fp, dx * d2r);
+		for (int i = 0; i < (int)v.size() && i < cap; i++) out[i] = v[i];
+		return (int)v.size();
+	} catch (const ConfigError &e) {
+		g_last_error = e.msg;
+		return GF_E_CONFIG;
+	}
+}
+
+int gf_rect_pixels_host(double l0, double l1, double b0, double b1, double dx, gf_pencil_beam *out, int cap)
+{
+	GF_CHECK(l1 > l0 && l1 - l0 <= 360. && b0 >= -90. && b0 < b1 && b1 <= 90. && dx > 0 && cap >= 0 &&
+	         (out || cap == 0), GF_E_ARG);
+	try {
+		gf_footprint fp{};
+		fp.nregion = 1;
+		fp.r[0] = rect_region(l0, l1, b0, b1);
+		std::vector<gf_pencil_beam> v = pixelize_region(fp, dx * M_PI / 180.0);
 		for (int i = 0; i < (int)v.size() && i < cap; i++) out[i] = v[i];
 		return (int)v.size();
 	} catch (const ConfigError &e) {
@@ -1729,11 +1786,12 @@ struct SReq {
 	uint64_t nunits_model = 0;     // npix * K
 	double expected = 0;
 	double counted = -1;           // expected rows from a count of an earlier batch, -1 before
+	std::vector<float> extmin;     // extinction minima per pixel and DM slice, with extinction maps
 };
 
 struct Worker {
 	hipStream_t st = nullptr;
-	DevBuf W, counts, offs, pixsum, segexp, pix, state, segs, segall, reqs, table, pos, zu, vis;
+	DevBuf W, counts, offs, pixsum, segexp, pix, state, segs, segall, reqs, table, pos, zu, vis, extbeam;
 	uint64_t table_cap = 0;
 	gf_table T{};
 	std::thread th;
@@ -1745,12 +1803,12 @@ struct Worker {
 	uint64_t held() const
 	{
 		return W.bytes + counts.bytes + offs.bytes + pixsum.bytes + segexp.bytes + pix.bytes + state.bytes +
-		       segs.bytes + segall.bytes + reqs.bytes + table.bytes + pos.bytes + zu.bytes + vis.bytes;
+		       segs.bytes + segall.bytes + reqs.bytes + table.bytes + pos.bytes + zu.bytes + vis.bytes + extbeam.bytes;
 	}
 	void release()
 	{
 		for (DevBuf *b : {&W, &counts, &offs, &pixsum, &segexp, &pix, &state, &segs, &segall, &reqs, &table, &pos,
-		                  &zu, &vis})
+		                  &zu, &vis, &extbeam})
 			b->free();
 		table_cap = 0;
 		T = gf_table{};
@@ -1790,11 +1848,16 @@ void prepare(gf_server &s, SReq &q)
 	fp = gf_footprint{};
 	fp.nregion = 1;
 	gf_foot_region &f = fp.r[0];
-	double l = q.r.l * M_PI / 180., b = q.r.b * M_PI / 180.;
-	f.kind = 2;
-	f.cl = cos(l); f.cb = cos(b); f.sl = sin(l); f.sb = sin(b);
-	f.cosr = cos(q.r.radius * M_PI / 180.);
-	q.pix = pixelize_beam(fp, s.dx);
+	if (q.r.radius > 0) {
+		double l = q.r.l * M_PI / 180., b = q.r.b * M_PI / 180.;
+		f.kind = 2;
+		f.cl = cos(l); f.cb = cos(b); f.sl = sin(l); f.sb = sin(b);
+		f.cosr = cos(q.r.radius * M_PI / 180.);
+	} else {
+		f = rect_region(q.r.l, q.r.l1, q.r.b, q.r.b1);
+	}
+	q.pix = pixelize_region(fp, s.dx);
+	if (s.base.has_ext) q.extmin = pixel_extinction_minima(s.base, q.pix);
 	q.sp = s.base.sp;
 	q.sp.m1 = q.r.m1;
 	q.sp.nm = (int)round((q.sp.m1 - s.m0raw) / q.sp.dm);
@@ -1867,12 +1930,12 @@ template <typename T> void grow(Worker &w, uint64_t share, DevBuf &b, size_t n)
 // Device bytes of one request in a batch apart from its rows: per unit of
 // every model its weight, count, offset and emission base (8 + 4 + 8 + 8 B),
 // per pixel the pencil beam and its partial sum, per stream its state, its
-// segments and request record.
+// segments and request record, and its pixels' extinction minima.
 uint64_t request_fixed_bytes(const gf_server &s, const SReq &q)
 {
 	const uint64_t nm = s.hmodels.size();
 	return q.nunits_model * nm * 28 + (uint64_t)q.pix.size() * (sizeof(gf_pencil_beam) + 8) + (uint64_t)q.S * 12 +
-	       nm * (2 * sizeof(gf_serve_seg) + 8) + sizeof(gf_serve_req) + 16;
+	       nm * (2 * sizeof(gf_serve_seg) + 8) + sizeof(gf_serve_req) + 16 + q.extmin.size() * 4;
 }
 
 // Device bytes per output row: the table's columns and the module chain's
@@ -1944,6 +2007,8 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	}
 	state.resize(soff[nr]);
 	for (int r = 0; r < nr; r++) std::copy(batch[r]->state.begin(), batch[r]->state.end(), state.begin() + soff[r]);
+	if (s.base.has_ext)
+		for (size_t g = 0; g < pix.size(); g++) pix[g].extIdx = (int32_t)g;   // its texel of the batch's minima
 	grow<gf_pencil_beam>(w, share, w.pix, std::max<size_t>(pix.size(), 1));
 	grow<uint32_t>(w, share, w.state, std::max<size_t>(state.size(), 1));
 	grow<double>(w, share, w.W, Ucand + 1);
@@ -1990,9 +2055,26 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	grow<uint64_t>(w, share, w.zu, Ucand + 1);
 	const gf_model *dmodels = (const gf_model *)s.dmodels.p;
 	const gf_pencil_beam *dpix = (const gf_pencil_beam *)w.pix.p;
+	// the batch's extinction minima: pixel g of the batch at texel
+	// (g div 2048, g mod 2048), the maps themselves resident
+	gf_extinction ext{};
+	if (s.base.has_ext) {
+		const Cube &ref = s.base.ext_north.v.empty() ? s.base.ext_south : s.base.ext_north;
+		std::vector<float> minima;
+		minima.reserve((size_t)poff[nr] * ref.nz);
+		for (int r = 0; r < nr; r++) minima.insert(minima.end(), batch[r]->extmin.begin(), batch[r]->extmin.end());
+		Cube beam;
+		beam_texture(beam, minima, poff[nr], ref.nz, ref.tc[2]);
+		grow<float>(w, share, w.extbeam, beam.v.size());
+		GF_HIP(hipMemcpyAsync(w.extbeam.p, beam.v.data(), beam.v.size() * 4, hipMemcpyHostToDevice, st));
+		ext = s.base.ext;
+		ext.beam = tex3d_of(beam, (const float *)w.extbeam.p);
+		// the copy reads the host vector: wait before it goes out of scope
+		GF_HIP(hipStreamSynchronize(st));
+	}
 	// count
 	for (int m = 0; m < nm; m++) {
-		int rc = gf::serve_count(dsegs + (size_t)m * nr, nr, poff[nr], dmodels, dpix, (double *)w.W.p,
+		int rc = gf::serve_count(dsegs + (size_t)m * nr, nr, poff[nr], dmodels, ext, dpix, (double *)w.W.p,
 		                         (double *)w.pixsum.p, (double *)w.segexp.p + (size_t)m * nr, st);
 		if (rc) return rc;
 	}
@@ -2051,7 +2133,7 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	rc = gf::serve_seg_rows(dsegall, nr * nm, (const uint64_t *)w.offs.p, st);
 	if (rc) return rc;
 	gf_skygen_cols sc{w.T.lb, w.T.projIdx, w.T.projXY, w.T.DM, w.T.absmag, w.T.XYZ, w.T.Am, w.T.AmInf, w.T.comp, w.T.hidden};
-	rc = gf::serve_emit(dsegall, nr * nm, dmodels, dpix, (const uint64_t *)w.offs.p, (const uint64_t *)w.zu.p, jtab,
+	rc = gf::serve_emit(dsegall, nr * nm, dmodels, ext, dpix, (const uint64_t *)w.offs.p, (const uint64_t *)w.zu.p, jtab,
 	                    nrows, sc, st);
 	if (rc) return rc;
 	std::vector<gf_serve_req> rq(nr);
@@ -2330,7 +2412,10 @@ int gf_server_create(const char *cmd_conf, const char *datadir, const gf_server_
 
 int gf_server_submit(gf_server *s, const gf_request *req)
 {
-	GF_CHECK(s && req && req->radius > 0 && req->radius < 90, GF_E_ARG);
+	GF_CHECK(s && req, GF_E_ARG);
+	GF_CHECK((req->radius > 0 && req->radius < 90) ||
+	         (req->radius == 0 && req->l1 > req->l && req->l1 - req->l <= 360. && req->b >= -90. &&
+	          req->b < req->b1 && req->b1 <= 90.), GF_E_ARG);
 	auto q = std::make_unique<SReq>();
 	q->r = *req;
 	q->t_submit = server_clock();

@@ -76,41 +76,6 @@ __global__ void __launch_bounds__(OB) k_clip(gf_skygen_params sp, gf_footprint f
 	if (outside_footprint(foot, l, b)) col<int>(hidden, r) = 1;
 }
 
-// Sample point (x, y) of hemisphere `hemi`'s Lambert plane inside the
-// footprint?  Beams are tested on the point's unit vector, which the
-// equal-area projection gives without transcendentals:
-// (-x s, -y s, 1 - r^2/2) in the north, (x s, -y s, r^2/2 - 1) in the south,
-// s = sqrt(1 - r^2/4); rects and the zone of avoidance on the deprojected
-// (l, b).
-__device__ bool sample_inside(const gf_footprint &foot, int hemi, double x, double y)
-{
-	double r2 = x * x + y * y;
-	if (r2 > 2.0) return false;
-	double s = sqrt(fmax(0.0, 1.0 - 0.25 * r2)), z = 1.0 - 0.5 * r2;
-	double v0 = hemi == 0 ? -x * s : x * s, v1 = -y * s, v2 = hemi == 0 ? z : -z;
-	if ((hemi == 0 && v2 < 0) || (hemi == 1 && v2 >= 0)) return false;
-	if (foot.nregion <= 0) return true;
-	double l = 0., b = 0.;
-	bool have_lb = false;
-	for (int k = 0; k < foot.nregion && k < GF_FOOT_MAX; k++) {
-		const gf_foot_region &f = foot.r[k];
-		if (f.kind == 2 && f.bmin <= 0.) {
-			double d = v0 * (f.cb * f.cl) + v1 * (f.cb * f.sl) + v2 * f.sb;
-			if (d >= f.cosr && !(f.hole > 0. && 1. - d < f.hole)) return true;
-			continue;
-		}
-		if (!have_lb) { pole_deproject(hemi, x, y, l, b); have_lb = true; }
-		if (f.kind == 2) {
-			if (fabs(b * DBL_R2D) < f.bmin) continue;
-			double d = v0 * (f.cb * f.cl) + v1 * (f.cb * f.sl) + v2 * f.sb;
-			if (d >= f.cosr && !(f.hole > 0. && 1. - d < f.hole)) return true;
-			continue;
-		}
-		if (!outside_region(f, l, b)) return true;
-	}
-	return false;
-}
-
 // One thread per pixel-grid cell: the number of its 8 x 8 sample points
 // inside the footprint (os_skygen.cpp:940-995 clips polygons instead).
 __global__ void __launch_bounds__(OB) k_footprint_cover(gf_footprint foot, double dx, int n, uint8_t *cover)

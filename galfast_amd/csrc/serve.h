@@ -49,7 +49,7 @@ struct gf_serve_chain_args {
 };
 
 namespace gf {
-int serve_count(const gf_serve_seg *dsegs, int nseg, uint32_t nslots, const gf_model *dmodels,
+int serve_count(const gf_serve_seg *dsegs, int nseg, uint32_t nslots, const gf_model *dmodels, const gf_extinction &ext,
                 const gf_pencil_beam *dpix, double *W, double *pix_sum, double *seg_expected, hipStream_t st);
 // Plan of a batch: lane per unit over all nseg segments (in unit order);
 // counts and zu (the states the units' stars start from) per batch unit.
@@ -58,7 +58,8 @@ int serve_plan(const gf_serve_seg *dsegs, int nseg, uint64_t nunits, const doubl
                uint32_t *counts, uint64_t *zu, hipStream_t st);
 int serve_seg_rows(gf_serve_seg *dsegs, int nseg, const uint64_t *offs, hipStream_t st);
 // Emission: lane per batch row over all segments (in row order).
-int serve_emit(const gf_serve_seg *dsegs, int nseg, const gf_model *dmodels, const gf_pencil_beam *dpix,
+int serve_emit(const gf_serve_seg *dsegs, int nseg, const gf_model *dmodels, const gf_extinction &ext,
+               const gf_pencil_beam *dpix,
                const uint64_t *offs, const uint64_t *zu, const uint64_t *jtab, uint64_t nrows,
                const gf_skygen_cols &out, hipStream_t st);
 // Every request's streams move on by its `advance` (nlanes = its streams in all).

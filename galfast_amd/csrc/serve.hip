@@ -30,7 +30,8 @@ GF_D int upper_index(const T *arr, int n, F key, uint64_t v)
 // Count: one workgroup per (segment pixel), lanes over the segment's DM bins.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(SB) k_b_count(const gf_serve_seg *segs, int nseg, const gf_model *models,
-                                                const gf_pencil_beam *pixels, double *W, double *pix_sum)
+                                                const gf_extinction ext, const gf_pencil_beam *pixels, double *W,
+                                                double *pix_sum)
 {
 	const uint32_t gp = blockIdx.x;   // batch pixel-slot index over this launch's segments
 	int si = upper_index(segs, nseg, [](const gf_serve_seg &s) { return (uint64_t)s.slot0; }, gp);
@@ -42,7 +43,6 @@ __global__ void __launch_bounds__(SB) k_b_count(const gf_serve_seg *segs, int ns
 	const int K = sp.nm + sp.nM - 1;
 	const gf_pencil_beam pix = pixels[sg.pix0 + p];
 	const double *P = sg.lfP;
-	gf_extinction ext{};
 	double acc = 0.0;
 	for (int k = threadIdx.x; k < K; k += blockDim.x) {
 		float D, Am, x, y, z;
@@ -119,7 +119,8 @@ __global__ void __launch_bounds__(SB) k_b_plan_units(const gf_serve_seg *segs, i
 // prefix.  A request's footprint is one Galactic beam; no extinction.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(SB) k_b_emit_stars(const gf_serve_seg *segs, int nseg, const gf_model *models,
-                                                     const gf_pencil_beam *pixels, const uint64_t *offs,
+                                                     const gf_extinction ext, const gf_pencil_beam *pixels,
+                                                     const uint64_t *offs,
                                                      const uint64_t *zu, const uint64_t *jtab, uint64_t nrows,
                                                      gf_skygen_cols out)
 {
@@ -143,7 +144,6 @@ __global__ void __launch_bounds__(SB) k_b_emit_stars(const gf_serve_seg *segs, i
 	const gf_mwc_mod md(r.a);
 	gf_mwc_set_z(r, gf_mwc_jump_z(md, zu[a], jj << GF_STAR_SPAN_BITS, jtab + s, JSTRIDE));
 	const gf_pencil_beam pix = pixels[sg.pix0 + p];
-	const gf_extinction ext{};
 	float Am;
 	int lo, hi;
 	unit_window(sp, ext, pix, k, Am, lo, hi);
@@ -301,11 +301,11 @@ int serve_visible(const gf_serve_req *dreqs, int nreq, uint64_t max_rows, const 
 }
 
 
-int serve_count(const gf_serve_seg *dsegs, int nseg, uint32_t nslots, const gf_model *dmodels,
+int serve_count(const gf_serve_seg *dsegs, int nseg, uint32_t nslots, const gf_model *dmodels, const gf_extinction &ext,
                 const gf_pencil_beam *dpix, double *W, double *pix_sum, double *seg_expected, hipStream_t st)
 {
 	if (nslots) {
-		k_b_count<<<nslots, SB, 0, st>>>(dsegs, nseg, dmodels, dpix, W, pix_sum);
+		k_b_count<<<nslots, SB, 0, st>>>(dsegs, nseg, dmodels, ext, dpix, W, pix_sum);
 		GF_LAUNCHED();
 	}
 	k_b_seg_expected<<<gf::blocks_for(nseg, 64), 64, 0, st>>>(dsegs, nseg, pix_sum, seg_expected);
@@ -329,12 +329,14 @@ int serve_seg_rows(gf_serve_seg *dsegs, int nseg, const uint64_t *offs, hipStrea
 	return 0;
 }
 
-int serve_emit(const gf_serve_seg *dsegs, int nseg, const gf_model *dmodels, const gf_pencil_beam *dpix,
+int serve_emit(const gf_serve_seg *dsegs, int nseg, const gf_model *dmodels, const gf_extinction &ext,
+               const gf_pencil_beam *dpix,
                const uint64_t *offs, const uint64_t *zu, const uint64_t *jtab, uint64_t nrows,
                const gf_skygen_cols &out, hipStream_t st)
 {
 	if (!nrows) return 0;
-	k_b_emit_stars<<<gf::blocks_for(nrows, SB), SB, 0, st>>>(dsegs, nseg, dmodels, dpix, offs, zu, jtab, nrows, out);
+	k_b_emit_stars<<<gf::blocks_for(nrows, SB), SB, 0, st>>>(dsegs, nseg, dmodels, ext, dpix, offs, zu, jtab, nrows,
+	                                                         out);
 	GF_LAUNCHED();
 	return 0;
 }

@@ -7,7 +7,7 @@ from ._abi import gf_table
 
 class gf_request(C.Structure):
     _fields_ = [("id", C.c_uint64), ("l", C.c_double), ("b", C.c_double), ("radius", C.c_double),
-                ("m1", C.c_float), ("seed", C.c_uint32)]
+                ("m1", C.c_float), ("seed", C.c_uint32), ("l1", C.c_double), ("b1", C.c_double)]
 
 
 class gf_result(C.Structure):
@@ -86,8 +86,10 @@ class Server:
     def clock():
         return _declare().gf_server_clock()
 
-    def submit(self, rid, l, b, radius, m1, seed):
-        r = gf_request(rid, l, b, radius, m1, seed)
+    def submit(self, rid, l, b, radius, m1, seed, l1=0.0, b1=0.0):
+        """A beam (radius > 0) or, with radius 0, the Galactic rect
+        [l, l1] x [b, b1] (degrees)."""
+        r = gf_request(rid, l, b, radius, m1, seed, l1, b1)
         check(lib().gf_server_submit(self._h, C.byref(r)), "gf_server_submit")
 
     def wait(self):

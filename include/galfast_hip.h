@@ -693,14 +693,17 @@ int gf_catalog_host_view_get(const gf_catalog *c, gf_catalog_host_view *view);
 /* ---------------------------------------------------------------------- */
 
 /* A footprint-generation request: a circular beam (Galactic l, b, radius in
- * degrees), its own faint limit and its own seed.  Everything else (models,
- * absolute-magnitude binning, bright limit, pixel size, modules) comes from
- * the server's base configuration. */
+ * degrees) or, with radius = 0, the Galactic rect l in [l, l1], b in [b, b1]
+ * (degrees; 0 < l1 - l <= 360, -90 <= b < b1 <= 90), its own faint limit and
+ * its own seed.  Everything else (models, absolute-magnitude binning, bright
+ * limit, pixel size, extinction maps, modules) comes from the server's base
+ * configuration. */
 typedef struct gf_request {
     uint64_t id;
     double   l, b, radius;
     float    m1;
     uint32_t seed;
+    double   l1, b1;
 } gf_request;
 
 typedef struct gf_result {
@@ -783,6 +786,9 @@ int gf_transform_host(const char *center, const char *orientation, double Rg, do
  * reference pixelizes footprints through its polygon clipper,
  * skygen/os_skygen.cpp:940-995). */
 int gf_beam_pixels_host(double l, double b, double radius, double dx, gf_pencil_beam *out, int cap);
+/* The same for a server request's Galactic rect [l0, l1] x [b0, b1]
+ * (degrees). */
+int gf_rect_pixels_host(double l0, double l1, double b0, double b1, double dx, gf_pencil_beam *out, int cap);
 int gf_photometry_tables_host(const double *rows, int nrows, int ncolors, double dFeH, double dMr,
                               float *colors, float *flags, size_t cap_floats, int *nFeH, int *nMr,
                               float tc[4]);

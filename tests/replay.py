@@ -179,7 +179,8 @@ def replay_request(view):
         m = view.models[mi]
         res = A.gf_count_result()
         wpk = np.zeros(U)
-        o.gfo_count_bins(C.byref(sp), C.byref(m), C.cast(pix, C.c_void_p), None, C.byref(res), None, O.ptr(wpk))
+        o.gfo_count_bins(C.byref(sp), C.byref(m), C.cast(pix, C.c_void_p), _ext(view), C.byref(res), None,
+                         O.ptr(wpk))
         expected += float(wpk[wpk > 0].sum())
         counts = np.zeros(U, dtype=np.uint32)
         base = state.copy()
@@ -196,7 +197,7 @@ def replay_request(view):
         n = int(offs[-1])
         if n:
             cols = _shift(t.skygen_cols(), row)
-            o.gfo_emit(C.byref(sp), C.byref(view.models[mi]), C.cast(pix, C.c_void_p), None, C.byref(foot),
+            o.gfo_emit(C.byref(sp), C.byref(view.models[mi]), C.cast(pix, C.c_void_p), _ext(view), C.byref(foot),
                        O.ptr(offs), U, 0, n, O.ptr(base), S, C.byref(cols))
         row += n
     if total:

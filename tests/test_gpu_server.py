@@ -30,13 +30,13 @@ REQUESTS = [
 ]
 
 
-def fetch_rows(table, row0, n):
+def fetch_rows(table, row0, n, spec=SPEC):
     L = G.lib()
     L.gf_memcpy_d2h.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
-    h = HostTable(n, SPEC)
+    h = HostTable(n, spec)
     if n == 0:
         return h
-    for name in SPEC:
+    for name in spec:
         col = getattr(table, name)
         raw, p = h.buf[name]
         for j in range(col.width):
@@ -69,13 +69,13 @@ SKYGEN = ("lb", "projIdx", "projXY", "DM", "absmag", "XYZ", "Am", "AmInf", "comp
 UNWRITTEN = ("FeH_comp", "absmagSys", "absmagSysNcomp")
 
 
-def same_rows(a, b, what):
+def same_rows(a, b, what, spec=SPEC, unwritten=UNWRITTEN):
     """Skygen columns on every row; module columns on visible rows (the
     modules skip hidden rows and leave those cells as they were)."""
     assert a.n == b.n, "%s: %d vs %d rows" % (what, a.n, b.n)
     vis = b.arr("hidden")[0] == 0
-    for c in SPEC:
-        if c in UNWRITTEN:
+    for c in spec:
+        if c in unwritten:
             continue
         x, y = a.arr(c), b.arr(c)
         if c not in SKYGEN:
@@ -152,12 +152,12 @@ GF_E_BUDGET = -7
 BIG = (99, 60.0, 40.0, 6.0, 25.0, 19)    # ~113 deg^2 to r = 25
 
 
-def serve_status(conf, reqs, **kw):
+def serve_status(conf, reqs, spec=SPEC, **kw):
     """serve(), keeping failed requests (status != 0, no rows)."""
     got = {}
 
     def on_result(res, table, row0, nrows):
-        got[res.id] = (res, fetch_rows(table, row0, nrows) if res.status == 0 else None)
+        got[res.id] = (res, fetch_rows(table, row0, nrows, spec) if res.status == 0 else None)
 
     srv = Server(conf, on_result, **kw)
     try:
@@ -200,3 +200,48 @@ def test_budget_admission_by_counted_rows(base, alone):
     for rid, (res, t) in got2.items():
         assert res.status == 0
         same_rows(t, ref[rid][1], "request %d, two workers" % rid)
+
+
+# Requests on the small config-4 base (extinction maps, unresolved binaries,
+# six bands): Galactic rect tiles and a beam, (id, l, b, radius, m1, seed, l1, b1)
+C4_SPEC = catalog_spec(nbands=6, nabsmag=2)
+C4_REQUESTS = [
+    (201, 86.0, 11.0, 0.0, 21.0, 31, 89.0, 14.0),
+    (202, 90.0, 15.0, 2.0, 20.5, 32, 0.0, 0.0),
+    (203, 93.0, 16.0, 0.0, 21.0, 33, 95.0, 20.0),
+    (204, 358.0, -30.0, 0.0, 20.0, 34, 362.0, -27.0),    # across l = 0, south
+]
+
+
+@pytest.fixture(scope="module")
+def c4served(tmp_path_factory):
+    conf = configs.write_c4(str(tmp_path_factory.mktemp("c4s")), small=True)
+    alone = serve_status(conf, C4_REQUESTS, spec=C4_SPEC, plain=True)
+    mix = serve_status(conf, C4_REQUESTS, spec=C4_SPEC, max_batch=8, workers=1, linger_us=300000)
+    return alone, mix
+
+
+@pytest.mark.parametrize("rid", [r[0] for r in C4_REQUESTS])
+def test_rect_and_extinction_requests_match_oracle_replay(tmp_path, c4served, rid):
+    """A rect or beam request on a base with extinction maps: the same rows
+    alone and in a batch, and the oracle's replay of a catalog job on that
+    footprint (its pixels, extinction minima and draws)."""
+    alone, mix = c4served
+    req = next(r for r in C4_REQUESTS if r[0] == rid)
+    _, l, b, rad, m1, seed, l1, b1 = req
+    res, t = alone[rid]
+    assert res.status == 0 and res.generated > 0
+    assert max(x.batch_size for x, _ in mix.values()) > 1
+    same_rows(mix[rid][1], t, "request %d in a batch" % rid, C4_SPEC, ("FeH_comp",))
+    conf = configs.write_c4(str(tmp_path), small=True, m1=float(np.float32(m1)),
+                            foot=configs.foot_body(l, b, rad, l1, b1))
+    cat = Catalog(conf, seed=seed)
+    cat.count()
+    view = R.host_view(G.lib(), cat._h)
+    assert res.npixels == view.sp.npixels
+    ref, expected = R.replay_request(view)
+    assert abs(res.expected - expected) <= 1e-12 * expected
+    assert res.generated == ref.n
+    same_rows(t, ref, "request %d vs oracle" % rid, C4_SPEC, ("FeH_comp",))
+    assert (t.arr("Am")[0] > 0).mean() > 0.9        # the dust is there
+    cat.close()

@@ -1,8 +1,9 @@
-"""The server's per-request beam pixelization (host code of the library,
-no GPU) against the oracle's grid coverage of the same beam: every covered
-cell of both pole grids, with its sample count, in the catalog's pixel order.
-Beams that cross, touch or barely dip below the equator are the cases where
-the two hemispheres' grids both hold cells of the request."""
+"""The server's per-request pixelization of beams and Galactic rects (host
+code of the library, no GPU) against the oracle's grid coverage of the same
+region: every covered cell of both pole grids, with its sample count, in the
+catalog's pixel order.  Regions that cross, touch or barely dip below the
+equator are the cases where the two hemispheres' grids both hold cells of the
+request."""
 import ctypes as C
 import math
 
@@ -26,14 +27,18 @@ def lib_pixels(l, b, r, dx):
     return [(p.projIdx, p.X, p.Y, round(p.coveredFraction * 64)) for p in out[:n]]
 
 
-def oracle_pixels(l, b, r, dx):
+def oracle_pixels(l, b, r, dx, rect=None):
     f = A.gf_footprint()
     f.nregion = 1
     g = f.r[0]
-    g.kind = 2
-    g.cl, g.sl = math.cos(l * D2R), math.sin(l * D2R)
-    g.cb, g.sb = math.cos(b * D2R), math.sin(b * D2R)
-    g.cosr = math.cos(r * D2R)
+    if rect is None:
+        g.kind = 2
+        g.cl, g.sl = math.cos(l * D2R), math.sin(l * D2R)
+        g.cb, g.sb = math.cos(b * D2R), math.sin(b * D2R)
+        g.cosr = math.cos(r * D2R)
+    else:
+        g.kind = 1
+        g.l0, g.l1, g.b0, g.b1 = rect
     d = dx * D2R
     n = int(math.ceil(2 * math.sqrt(2.0) / d))
     cov = np.zeros(2 * n * n, np.uint8)
@@ -68,4 +73,30 @@ def test_beam_pixels_equal_grid_coverage(l, b, r):
     assert len(got) == len(want)
     assert got == want
     if abs(b) < r - 0.01:
+        assert {h for h, _, _, _ in want} == {0, 1}
+
+
+def lib_rect_pixels(l0, l1, b0, b1, dx):
+    L = GA.lib()
+    L.gf_rect_pixels_host.argtypes = [C.c_double] * 5 + [C.c_void_p, C.c_int]
+    n = L.gf_rect_pixels_host(l0, l1, b0, b1, dx, None, 0)
+    assert n >= 0
+    out = (A.gf_pencil_beam * max(n, 1))()
+    assert L.gf_rect_pixels_host(l0, l1, b0, b1, dx, out, n) == n
+    return [(p.projIdx, p.X, p.Y, round(p.coveredFraction * 64)) for p in out[:n]]
+
+
+@pytest.mark.parametrize("rect,oracle_rect", [
+    ((86.0, 89.0, 11.0, 14.0), (86.0, 89.0, 11.0, 14.0)),        # a config-4 tile
+    ((358.0, 362.0, -30.0, -27.0), (-2.0, 2.0, -30.0, -27.0)),   # across l = 0, as footprint files reduce it
+    ((10.0, 25.0, -3.0, 4.0), (10.0, 25.0, -3.0, 4.0)),          # across the equator
+    ((0.0, 360.0, 80.0, 90.0), (0.0, 360.0, 80.0, 90.0)),        # the polar cap
+    ((200.0, 230.0, -90.0, -60.0), (200.0, 230.0, -90.0, -60.0)),  # a wedge at the south pole
+])
+def test_rect_pixels_equal_grid_coverage(rect, oracle_rect):
+    dx = 0.5
+    got = lib_rect_pixels(*rect, dx)
+    want, n = oracle_pixels(0, 0, 0, dx, rect=oracle_rect)
+    assert len(got) == len(want) and got == want
+    if rect[2] < 0 < rect[3]:
         assert {h for h, _, _, _ in want} == {0, 1}

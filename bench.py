@@ -35,34 +35,58 @@ WORKLOADS = {
 }
 
 PEAK_HBM_GBS = 8000.0
-# Algorithmic bytes per row of each kernel: every
-# column a kernel must read, and every column it writes, once.  Hidden rows
-# cost the 4-byte hidden flag in the module kernels.
+# Algorithmic bytes per row of each kernel: every column a kernel must read,
+# and every column it writes, once.  Hidden rows cost the 4-byte hidden flag
+# in the module kernels.
 # emission, per generated row: the 10 skygen columns (64 B) written; the
 # unit data (offsets, start state, pixel) is read once per unit, not per row
 EMIT_STARS_BYTES_PER_ROW = 64
-# (bytes per visible row, bytes per hidden row) of the module-chain launches
-# for the config-1 module set (FeH, Bond2010, photometry with 5 bands,
-# photometric errors, vel2pm, gal2other): every column a launch must read,
-# and every column it writes, once.
-STAGE_BYTES = {
-    # draw pass: read hidden + comp of every row; write the 8-byte stream
-    # position of each of the 3 random-number modules for a visible row
-    "k_chain_draws": (4 + 4 + 3 * 8, 4 + 4),
-    # metallicity rows: read hidden, comp, XYZ, 1 position; write FeH
-    "k_chain_rows<FEH>": (4 + 4 + 12 + 8 + 4, 4),
-    # kinematics rows: read hidden, comp, XYZ, 1 position; write vcyl
-    "k_chain_rows<BOND>": (4 + 4 + 12 + 8 + 12, 4),
-    # photometry rows: read hidden, comp, FeH, absmag, DM, Am; write mags,
-    # photoFlags
-    "k_chain_rows<PHOT>": (4 + 4 + 4 + 4 + 4 + 4 + 20 + 4, 4),
-    # photometric-error rows: read hidden, mags, 1 position; write obsMags
-    "k_chain_rows<PERR>": (4 + 20 + 8 + 20, 4),
-    # astrometry rows: read hidden, lb, XYZ, vcyl; write pm, radec
-    "k_chain_rows<ASTRO>": (4 + 16 + 12 + 12 + 12 + 16, 4),
-}
 STAGE_NAMES = ["k_chain_draws", "k_chain_rows<FEH>", "k_chain_rows<BOND>", "k_chain_rows<PHOT>",
                "k_chain_rows<PERR>", "k_chain_rows<ASTRO>"]
+
+
+def stage_bytes(nbands, nabsmag, present):
+    """(bytes per visible row, bytes per hidden row) of the module-chain
+    launches for a job with nbands bands, nabsmag system components (2 or
+    more: unresolved multiples) and the row passes in `present`."""
+    nb = nbands
+    mult = nabsmag > 1
+    npos = sum(1 for k in ("k_chain_rows<FEH>", "k_chain_rows<BOND>", "k_chain_rows<PERR>") if k in present)
+    # draw pass: read hidden + comp of every row; per visible row write the
+    # 8-byte stream position of each random-number module; unresolved
+    # multiples run whole in it: read absmag, write absmagSys, its count and
+    # the system's absmag
+    draws = 4 + 4 + 8 * npos + ((4 + 4 * nabsmag + 4 + 4) if mult else 0)
+    mr = 4 * nabsmag if mult else 4
+    return {
+        "k_chain_draws": (draws, 4 + 4),
+        # metallicity rows: read hidden, comp, XYZ, 1 position; write FeH
+        "k_chain_rows<FEH>": (4 + 4 + 12 + 8 + 4, 4),
+        # kinematics rows: read hidden, comp, XYZ, 1 position; write vcyl
+        "k_chain_rows<BOND>": (4 + 4 + 12 + 8 + 12, 4),
+        # photometry rows: read hidden, comp, FeH, the absolute magnitude(s),
+        # DM, Am; write mags, photoFlags
+        "k_chain_rows<PHOT>": (4 + 4 + 4 + mr + 4 + 4 + 4 * nb + 4, 4),
+        # photometric-error rows: read hidden, mags, 1 position; write obsMags
+        "k_chain_rows<PERR>": (4 + 4 * nb + 8 + 4 * nb, 4),
+        # astrometry rows: read hidden, lb, XYZ, vcyl; write pm, radec
+        "k_chain_rows<ASTRO>": (4 + 16 + 12 + 12 + 12 + 16, 4),
+    }
+
+
+def table_row_bytes(nbands, nabsmag, present):
+    """Bytes of one output row: the skygen columns and the columns of the
+    modules that run."""
+    b = 64 + 4 + 4 * nbands + 4                       # skygen, FeH, mags, photoFlags
+    if nabsmag > 1:
+        b += 4 * nabsmag + 4                          # absmagSys, absmagSysNcomp
+    if "k_chain_rows<PERR>" in present:
+        b += 4 * nbands
+    if "k_chain_rows<BOND>" in present:
+        b += 12
+    if "k_chain_rows<ASTRO>" in present:
+        b += 12 + 16                                  # pm, radec
+    return b
 
 
 def host_cpu_info():
@@ -415,9 +439,11 @@ def main():
                     help="threads of the CPU baseline (default: every core this process may use)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-c4", action="store_true", help="skip the config-4 sub-record")
+    ap.add_argument("--c4-steps", type=int, default=3)
     ap.add_argument("--output-bench", action="store_true",
                     help="also time the output stages (FITS and text) on the config-1 workload")
-    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--serve-batch", type=int, default=256)
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[10.0, 100.0, 1000.0])
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="arrival window per Poisson rate")
@@ -444,14 +470,38 @@ def main():
         import torch   # device buffers of the output timing; torch takes the device first
         torch.cuda.init()
     import galfast_amd as G
+    G.check(G.lib().gf_init(0), "gf_init")
+    out, cat = measure(args, args.workload, args.steps, args.warmup, args.overlap)
+    if args.workload == "c1":
+        out["footprint_cover"] = footprint_cover_bench()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cat, args.workload, args.cpu_budget,
+                                           args.cpu_threads or default_cpu_threads())
+    cat.close()
+    if args.output_bench:
+        out["output"] = output_bench(args)
+    if not args.no_serve:
+        sv = serving(args)
+        out["serving"] = sv
+        top = sv["poisson"][max(sv["poisson"], key=float)] if sv["poisson"] else None
+        out["p50_ms"] = top["p50_ms"] if top else None
+        out["p99_ms"] = top["p99_ms"] if top else None
+    if args.workload == "c1" and not args.no_c4:
+        out["c4"] = c4_bench(args)
+    print(json.dumps(out))
+
+
+def measure(args, workload, steps, warmup, overlap=1):
+    """Time `steps` catalog steps of `workload` after `warmup` untimed ones.
+    Returns (the bench record without the CPU and serving parts, the open
+    catalog)."""
+    import galfast_amd as G
     from galfast_amd import configs
     from galfast_amd.catalog import Catalog
-
-    G.check(G.lib().gf_init(0), "gf_init")
-    d = tempfile.mkdtemp(prefix="galfast_bench_%s_" % args.workload)
-    cmd = configs.WRITERS[args.workload](d, args.seed)
+    d = tempfile.mkdtemp(prefix="galfast_bench_%s_" % workload)
+    cmd = configs.WRITERS[workload](d, args.seed)
     cat = Catalog(cmd, seed=args.seed, chunk_rows=30_000_000)
-    cat.set("overlap", args.overlap)
+    cat.set("overlap", overlap)
     info = cat.info()
 
     def step():
@@ -461,14 +511,14 @@ def main():
         st = cat.run()
         return expected, t1 - t0, st
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         step()
     G.lib().gf_sync(None)
     t0 = time.perf_counter()
     acc = {"visible": 0, "generated": 0, "t_count": 0.0, "t_plan": 0.0, "t_emit": 0.0, "t_chain": 0.0,
            "n_emit": 0, "n_chain": 0, "expected": 0.0, "t_stage": [0.0] * len(STAGE_NAMES),
            "t_emit_stars": 0.0, "t_emit_units": 0.0}
-    for _ in range(args.steps):
+    for _ in range(steps):
         expected, tc, st = step()
         acc["visible"] += st.visible
         acc["generated"] += st.generated
@@ -488,24 +538,22 @@ def main():
     value = acc["visible"] / elapsed
 
     # roofline of the dominant kernel, from HIP-event durations recorded on
-    # the catalog's stream around each launch
+    # the stream each launch runs on
     hidden = acc["generated"] - acc["visible"]
+    present = {STAGE_NAMES[q] for q in range(len(STAGE_NAMES)) if acc["t_stage"][q] > 0}
+    sbytes = stage_bytes(info.nbands, info.nabsmag, present)
     kernels = {"k_emit_stars": (acc["t_emit_stars"], acc["n_emit"], EMIT_STARS_BYTES_PER_ROW * acc["generated"])}
     for q, name in enumerate(STAGE_NAMES):
-        if acc["t_stage"][q] > 0:
-            bv, bh = STAGE_BYTES[name]
+        if name in present:
+            bv, bh = sbytes[name]
             kernels[name] = (acc["t_stage"][q], acc["n_chain"], bv * acc["visible"] + bh * hidden)
-    if args.workload != "c1":
-        # the per-launch algorithmic bytes above are those of the config-1
-        # module set and emit path
-        kernels = {}
-    name = max(kernels, key=lambda k: kernels[k][0]) if kernels else None
-    t, n, bytes_total = kernels[name] if name else (0.0, 1, 0.0)
+    name = max(kernels, key=lambda k: kernels[k][0])
+    t, n, bytes_total = kernels[name]
     # HBM bytes per launch of that kernel from the committed rocprofv3
-    # counter runs of this benchmark (profiles/traffic.json), if measured
+    # counter runs of this benchmark, if measured
     traffic = None
-    tj = os.path.join(ROOT, "profiles", "traffic.json")
-    if args.workload == "c1" and os.path.exists(tj):
+    tj = os.path.join(ROOT, "profiles", "traffic.json" if workload == "c1" else "traffic_%s.json" % workload)
+    if os.path.exists(tj):
         with open(tj) as f:
             tr = json.load(f)
         if name in tr:
@@ -513,60 +561,133 @@ def main():
     per_launch_s = t / max(n, 1)
     achieved = bytes_total / max(n, 1) / per_launch_s / 1e9 if per_launch_s > 0 else 0.0
     # vector-instruction issue of the same kernel: its VALU instructions per
-    # launch from the committed counter runs (profiles/valu.json) over the
-    # launch duration measured here, as SIMD cycles per instruction (1,024
-    # SIMDs at 2.4 GHz; a wave64 FMA occupies a SIMD for 4 cycles)
+    # launch from the committed counter runs over the launch duration
+    # measured here, as SIMD cycles per instruction (1,024 SIMDs at 2.4 GHz;
+    # a wave64 FMA occupies a SIMD for 4 cycles)
     valu_issue = None
-    vj = os.path.join(ROOT, "profiles", "valu.json")
-    if name and args.workload == "c1" and os.path.exists(vj):
+    vj = os.path.join(ROOT, "profiles", "valu.json" if workload == "c1" else "valu_%s.json" % workload)
+    if os.path.exists(vj):
         with open(vj) as f:
             vt = json.load(f)
         if name in vt and vt[name]["valu_insts"] > 0:
             valu_issue = {"kernel": name, "valu_insts_per_launch": vt[name]["valu_insts"],
                           "simd_cycles_per_valu_inst": 1024 * 2.4e9 * per_launch_s / vt[name]["valu_insts"],
                           "isolated_simd_cycles_per_valu_inst": vt[name]["simd_cycles_per_valu"]}
+    row_b = table_row_bytes(info.nbands, info.nabsmag, present)
+    rows_per_s = acc["generated"] / elapsed
     out = {
         "metric": "catalog stars/sec (full pipeline) on 1 GPU vs plain; p50/p99 request latency",
         "value": value,
         "unit": "stars/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1000.0 * elapsed / args.steps,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": 1000.0 * elapsed / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": "%s: %s" % (args.workload, WORKLOADS[args.workload]),
+        "config": {"workload": "%s: %s" % (workload, WORKLOADS[workload]),
                    "pixels": int(info.npixels), "nm": info.nm, "nM": info.nM,
-                   "stars_per_step": acc["visible"] // args.steps,
-                   "rows_per_step": acc["generated"] // args.steps,
-                   "expected_per_step": acc["expected"] / args.steps},
+                   "stars_per_step": acc["visible"] // steps,
+                   "rows_per_step": acc["generated"] // steps,
+                   "expected_per_step": acc["expected"] / steps},
         "breakdown_ms_per_step": dict(
-            [(k, 1000.0 * acc[k] / args.steps) for k in ("t_count", "t_plan", "t_emit", "t_chain")] +
-            [("k_emit_stars", 1000.0 * acc["t_emit_stars"] / args.steps),
-             ("k_tile_units", 1000.0 * acc["t_emit_units"] / args.steps)] +
-            [(STAGE_NAMES[q], 1000.0 * acc["t_stage"][q] / args.steps) for q in range(len(STAGE_NAMES))]),
-        "roofline": ({"bound": "hbm", "kernel": name, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
-                      "algorithmic_bytes_per_launch": bytes_total / max(n, 1)} if name else None),
+            [(k, 1000.0 * acc[k] / steps) for k in ("t_count", "t_plan", "t_emit", "t_chain")] +
+            [("k_emit_stars", 1000.0 * acc["t_emit_stars"] / steps),
+             ("k_tile_units", 1000.0 * acc["t_emit_units"] / steps)] +
+            [(STAGE_NAMES[q], 1000.0 * acc["t_stage"][q] / steps) for q in range(len(STAGE_NAMES))]),
+        "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": bytes_total / max(n, 1)},
+        # the whole job against HBM: every output row written once
+        "pipeline_roofline": {"bytes_per_row": row_b, "achieved": rows_per_s * row_b / 1e9,
+                              "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                              "frac": rows_per_s * row_b / 1e9 / PEAK_HBM_GBS},
         "valu_issue": valu_issue,
     }
-    if args.workload == "c1":
-        out["footprint_cover"] = footprint_cover_bench()
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cat, args.workload, args.cpu_budget,
-                                           args.cpu_threads or default_cpu_threads())
+    return out, cat
+
+
+def c4_bench(args):
+    """The deep config-4 job (extinction, binaries, 6 bands): a few timed
+    steps with its own roofline; the drawn stars per model and apparent-
+    magnitude bin beside the count's expectation per model; and the same
+    footprint served as 200 rect sub-requests tiling it."""
+    import numpy as np
+    import galfast_amd as G
+    rec, cat = measure(args, "c4", args.c4_steps, 1, 1)
+    out = {k: rec[k] for k in ("value", "unit", "ms_per_step", "steps", "config", "breakdown_ms_per_step",
+                               "roofline", "pipeline_roofline", "valu_issue")}
+    # expected stars per model (count) and drawn stars per model and 1-mag
+    # bin of the primary's r (untimed pass)
+    cat.count()
+    ids = cat.component_ids()
+    per = {"thin disk": 0, "thick disk": 1, "halo": 2}
+    bins = np.arange(14.0, 28.0, 1.0)
+    hist = np.zeros((len(ids), len(bins) - 1), dtype=np.int64)
+    L = G.lib()
+    import ctypes as C
+    L.gf_memcpy_d2h.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
+
+    def col(c, n, dt, plane=0):
+        a = np.empty(n, dtype=dt)
+        G.check(L.gf_memcpy_d2h(a.ctypes.data, c.ptr + plane * c.pitch, n * c.elem, None), "copy")
+        return a
+
+    def on_chunk(t, n, comp):
+        vis = col(t.hidden, n, np.int32) == 0
+        m = col(t.DM, n, np.float32) + col(t.absmagSys, n, np.float32, 0)
+        h, _ = np.histogram(m[vis], bins=bins)
+        hist[comp] += h
+
+    cat.run(on_chunk)
+    out["per_model"] = {name: {"expected": float(cat.per_model[i].nstarsExpected),
+                               "drawn_visible_per_mag": dict(("%g-%g" % (bins[j], bins[j + 1]), int(hist[i][j]))
+                                                             for j in range(len(bins) - 1))}
+                        for name, i in per.items() if i < len(ids)}
+    out["count_note"] = ("the luminosity function spans Mr 3-15 (the workload's LF tabulation), so at r = 27.5 "
+                         "the faintest stars it holds are seen to DM 12.5 (3.2 kpc); the dust disk removes the "
+                         "near-plane volume; over 970 deg^2 at b 20-50 this gives %.3g stars, not the ~1e9 "
+                         "the workload text estimates for an LF that continues to fainter dwarfs"
+                         % rec["config"]["expected_per_step"])
     cat.close()
-    if args.output_bench:
-        out["output"] = output_bench(args)
-    if not args.no_serve:
-        sv = serving(args)
-        out["serving"] = sv
-        out["p50_ms"] = sv["poisson"][max(sv["poisson"], key=float)]["p50_ms"] if sv["poisson"] else None
-        out["p99_ms"] = sv["poisson"][max(sv["poisson"], key=float)]["p99_ms"] if sv["poisson"] else None
-    print(json.dumps(out))
+    out["served_200"] = c4_served(args)
+    return out
+
+
+def c4_served(args):
+    """Config 4 as 200 Galactic rect sub-requests (20 x 10 tiles of 2 x 3
+    deg) submitted at once to one server; stars/s, latency, batches."""
+    import numpy as np
+    from galfast_amd import configs
+    from galfast_amd.server import Server
+    d = tempfile.mkdtemp(prefix="galfast_c4serve_")
+    conf = configs.write_c4(d, args.seed)
+    tiles = configs.c4_tiles()
+    srv = Server(conf, None, hbm_budget=200 << 30, max_batch=args.serve_batch, workers=2)
+    try:
+        warm = configs.c4_tiles(2, 1, l=(60.0, 62.0), b=(20.0, 23.0), id0=10 ** 7)
+        for r in warm:
+            srv.submit(*r)
+        srv.wait()
+        nwarm = len(srv.results)
+        for r in tiles:
+            srv.submit(*r)
+        srv.wait()
+        res = srv.results[nwarm:]
+    finally:
+        srv.close()
+    lat = np.array([x.t_done - x.t_submit for x in res])
+    wall = max(x.t_done for x in res) - min(x.t_submit for x in res)
+    stars = sum(x.visible for x in res)
+    return {"requests": len(res), "failed": sum(1 for x in res if x.status != 0), "stars": stars,
+            "rows": sum(x.generated for x in res), "expected": sum(x.expected for x in res),
+            "wall_s": wall, "stars_per_s": stars / wall,
+            "p50_ms": 1000.0 * float(np.percentile(lat, 50)), "p99_ms": 1000.0 * float(np.percentile(lat, 99)),
+            "batches": len(set(x.batch for x in res)),
+            "tiles": "20 x 10 Galactic rects of 2 x 3 deg over l 60-100, b 20-50, r to 27.5, seed per tile"}
 
 
 if __name__ == "__main__":

@@ -1146,45 +1146,52 @@ int run_catalog_reference(gf_catalog &c, gf_chunk_fn fn, void *user, gf_run_stat
 	for (auto &mp : c.models) {
 		Model &m = *mp;
 		if (!m.wpk.p) return GF_E_STATE;   // the count pass comes first, as in os_skygen::run
-		ensure_table(c, drawn, std::max<uint64_t>(drawn.cap, 1024));
-		gf_skygen_cols dc = skygen_cols_of(drawn.table);
-		uint64_t total = 0;
 		const gf_extinction *ext = c.has_ext ? &c.ext : nullptr;
 		const gf_pencil_beam *pix = (const gf_pencil_beam *)c.pix_dev.p;
-		int rc = gf_skygen_draw_compat(&c.sp, &m.m, pix, ext, nullptr, &c.rng, &dc, drawn.cap, &total, A);
-		if (rc == GF_E_CAPACITY) {   // nothing drawn, streams unchanged
-			ensure_table(c, drawn, total);
-			dc = skygen_cols_of(drawn.table);
-			rc = gf_skygen_draw_compat(&c.sp, &m.m, pix, ext, nullptr, &c.rng, &dc, drawn.cap, &total, A);
-		}
+		// batches of at most chunk_rows stars, the output table's capacity
+		// (drawSources, skyconfig_impl.h:434-534): each is sorted, truncated
+		// at its first hidden row, clipped and run through the modules
+		gf_draw_state *ds = nullptr;
+		uint64_t total = 0;
+		int rc = gf_skygen_draw_begin(&c.sp, &m.m, pix, ext, nullptr, &c.rng, &ds, &total, A);
 		if (rc) return rc;
+		std::unique_ptr<gf_draw_state, void (*)(gf_draw_state *)> hold(ds, gf_skygen_draw_end);
+		const uint64_t cap = std::max<uint64_t>(1, std::min<uint64_t>(c.chunk_rows, total));
+		ensure_table(c, drawn, cap);
+		ensure_table(c, sorted, cap);
+		gf_skygen_cols dc = skygen_cols_of(drawn.table);
+		gf_skygen_cols sc = skygen_cols_of(sorted.table);
 		s.generated += total;
 		s.expected += m.counts.nstarsExpectedToGenerate;
-		s.n_emit++;
-		ensure_table(c, sorted, std::max<uint64_t>(total, 1));
-		gf_skygen_cols sc = skygen_cols_of(sorted.table);
-		uint64_t nv = 0;
-		rc = gf_skygen_order(&dc, total, &sc, &nv, A);
-		if (rc) return rc;
-		rc = gf_clip_footprint(&c.sp, &c.foot.f, sorted.table.projIdx, sorted.table.projXY, sorted.table.hidden, nv,
-		                       A);
-		if (rc) return rc;
-		for (uint64_t b0 = 0; b0 < nv; b0 += sub) {
-			gf_rows ks{(uint32_t)b0, 256, (uint32_t)std::min(nv, b0 + sub), 0};
-			rc = gf::module_chain(&ks, &ch, &c.rng, &sorted.table, A, nullptr, nullptr);
+		for (;;) {
+			uint64_t n = 0;
+			rc = gf_skygen_draw_next(ds, &dc, cap, &n, A);
 			if (rc) return rc;
-			s.n_chain++;
-		}
-		s.rows_chain += nv;
-		uint64_t vis = 0;
-		if (nv) {
-			rc = gf_count_visible(sorted.table.hidden, nv, &vis, A);
+			if (n == 0) break;
+			s.n_emit++;
+			uint64_t nv = 0;
+			rc = gf_skygen_order(&dc, n, &sc, &nv, A);
 			if (rc) return rc;
-		}
-		s.visible += vis;
-		if (fn && nv) {
-			rc = fn(user, &sorted.table, nv, m.m.comp);
+			rc = gf_clip_footprint(&c.sp, &c.foot.f, sorted.table.projIdx, sorted.table.projXY, sorted.table.hidden,
+			                       nv, A);
 			if (rc) return rc;
+			for (uint64_t b0 = 0; b0 < nv; b0 += sub) {
+				gf_rows ks{(uint32_t)b0, 256, (uint32_t)std::min(nv, b0 + sub), 0};
+				rc = gf::module_chain(&ks, &ch, &c.rng, &sorted.table, A, nullptr, nullptr);
+				if (rc) return rc;
+				s.n_chain++;
+			}
+			s.rows_chain += nv;
+			uint64_t vis = 0;
+			if (nv) {
+				rc = gf_count_visible(sorted.table.hidden, nv, &vis, A);
+				if (rc) return rc;
+			}
+			s.visible += vis;
+			if (fn && nv) {
+				rc = fn(user, &sorted.table, nv, m.m.comp);
+				if (rc) return rc;
+			}
 		}
 	}
 	GF_HIP(hipStreamSynchronize(A));

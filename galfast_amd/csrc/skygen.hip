@@ -9,6 +9,7 @@
 #include "gf_runtime.h"
 #include "gf_device.h"
 #include <algorithm>
+#include <memory>
 #include <mutex>
 
 namespace {
@@ -354,7 +355,7 @@ GF_D void draw_one_star(const gf_skygen_params &sp, const gf_model &m, const gf_
 __global__ void __launch_bounds__(64) k_draw_compat(gf_skygen_params sp, gf_model m, const gf_pencil_beam *pixels,
                                                     gf_extinction ext, gf_footprint foot, uint32_t *st, uint32_t ns,
                                                     int pass, uint32_t *counts, const uint64_t *offs,
-                                                    gf_skygen_cols out)
+                                                    gf_skygen_cols out, int store = 0)
 {
 	const int tid = blockIdx.x * blockDim.x + threadIdx.x;
 	if (tid >= sp.nthreads) return;
@@ -402,7 +403,97 @@ __global__ void __launch_bounds__(64) k_draw_compat(gf_skygen_params sp, gf_mode
 		}
 	}
 	if (pass == 0) counts[tid] = n;
-	else gf_mwc_store(st, ns, tid, rng);
+	if (pass == 1 || store) gf_mwc_store(st, ns, tid, rng);
+}
+
+// The crawl state of the thread a batch of the capacity-limited draw stopped
+// inside (runtime_state, skygen/skygen.h:184-257): its place in the crawl,
+// the current cell's values, the stars of that cell still to draw and its
+// stream.
+struct gf_draw_resume {
+	int32_t tid, ilb, im, iM, k, bc;
+	float D, Am, R;
+	uint32_t ndraw;
+	uint32_t a, c, x;
+};
+
+// Rows [r0, r1) of the reference-order draw: thread t writes its stars
+// offs[t] .. offs[t+1] - 1 (the count pass's exclusive scan), from its
+// stream's state at the start of the draw (st0), or, for the thread whose
+// stars began in the previous batch, from the state that batch left in
+// res_in; the thread that reaches row r1 with stars still to draw leaves
+// its state in res_out.  The batches concatenated are the unlimited draw.
+__global__ void __launch_bounds__(64) k_draw_window(gf_skygen_params sp, gf_model m, const gf_pencil_beam *pixels,
+                                                    gf_extinction ext, gf_footprint foot, const uint32_t *st0,
+                                                    uint32_t ns, const uint64_t *offs, uint64_t r0, uint64_t r1,
+                                                    const gf_draw_resume *res_in, gf_draw_resume *res_out,
+                                                    gf_skygen_cols out)
+{
+	const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+	if (tid >= sp.nthreads) return;
+	const uint64_t o0 = offs[tid], o1 = offs[tid + 1];
+	if (o1 <= r0 || o0 >= r1) return;
+	const int block = 10;
+	const int nthreads = sp.nthreads, nm = sp.nm, nM = sp.nM;
+	int ilb = 0, im = 0, iM = 0, k = block * (tid - nthreads), bc = 0;
+	float D = 0.f, Am = 0.f, R = 0.f;
+	gf_pencil_beam pix;
+	gf_mwc rng;
+	unsigned int ndraw = 0;
+	uint64_t row = o0;
+	if (o0 < r0) {
+		const gf_draw_resume q = *res_in;
+		ilb = q.ilb; im = q.im; iM = q.iM; k = q.k; bc = q.bc;
+		D = q.D; Am = q.Am; R = q.R;
+		ndraw = q.ndraw;
+		rng.a = q.a; rng.c = q.c; rng.x = q.x;
+		pix = pixels[ilb];
+		row = r0;
+	} else {
+		rng = gf_mwc_load(st0, ns, tid);
+	}
+	StarSink s{out, true, row - r0};
+	for (;;) {
+		while (ndraw > 0) {
+			if (row == r1) {
+				*res_out = gf_draw_resume{tid, ilb, im, iM, k, bc, D, Am, R, ndraw, rng.a, rng.c, rng.x};
+				return;
+			}
+			draw_one_star(sp, m, ext, foot, pix, sp.M1 - iM * sp.dM, im, Am, rng, s);
+			s.row++;
+			row++;
+			ndraw--;
+		}
+		if (row >= o1) return;
+		bool moved;
+		if (bc == 0) {
+			bc = block;
+			k += block * nthreads;
+			diag_index_to_ij(ilb, im, iM, k, nm, nM);
+			if (ilb >= sp.npixels) return;
+			pix = pixels[ilb];
+			moved = true;
+		} else {
+			moved = crawl_advance(ilb, im, iM, pix, pixels, sp.npixels, nm, nM);
+		}
+		bc--;
+		float M = sp.M1 - iM * sp.dM;
+		if (moved) {
+			if (ilb >= sp.npixels) return;
+			float x, y, z;
+			cell_pos(sp, ext, pix, M, im, D, Am, x, y, z);
+			R = model_rho_xyz(m, x, y, z);
+		}
+		if ((sp.dmin != 0.f || sp.dmax != 0.f) && (sp.dmin > D || sp.dmax <= D)) continue;
+		float mm = sp.m0 + sp.dm * im + Am;
+		if (mm > sp.m1) continue;
+		float phi = tex1d(m.lf, M);
+		float rho = phi * R;
+		rho *= m.norm;
+		rho *= D * D * D;
+		rho *= pix.dA * POGSON * sp.dm * sp.dM;
+		ndraw = rng.poisson(rho);
+	}
 }
 
 // ---------------------------------------------------------------------------
@@ -643,6 +734,90 @@ int gf_skygen_draw_compat(const gf_skygen_params *sp, const gf_model *model,
 	k_draw_compat<<<grid, 64, 0, st>>>(*sp, *model, pixels, ex, fp, rng->state, rng->nstreams, 1, counts, offs, *out);
 	GF_LAUNCHED();
 	return 0;
+}
+
+// The capacity-limited reference draw (skygenHost::drawSources,
+// skygen/skyconfig_impl.h:434-534): the streams' starting states, each
+// thread's star count and their exclusive scan, the two resume records and
+// the next row to write.
+struct gf_draw_state {
+	gf_skygen_params sp;
+	gf_model m;
+	const gf_pencil_beam *pixels;
+	gf_extinction ext;
+	gf_footprint foot;
+	uint32_t ns;
+	uint32_t *st0 = nullptr;          // 3 * ns words
+	uint32_t *counts = nullptr;       // nthreads
+	uint64_t *offs = nullptr;         // nthreads + 1
+	gf_draw_resume *res = nullptr;    // 2
+	uint64_t total = 0, next = 0;
+	int batch = 0;
+	~gf_draw_state()
+	{
+		if (st0) (void)hipFree(st0);
+		if (counts) (void)hipFree(counts);
+		if (offs) (void)hipFree(offs);
+		if (res) (void)hipFree(res);
+	}
+};
+
+int gf_skygen_draw_begin(const gf_skygen_params *sp, const gf_model *model, const gf_pencil_beam *pixels,
+                         const gf_extinction *ext, const gf_footprint *foot, gf_rng *rng, gf_draw_state **state,
+                         uint64_t *nstars, void *stream)
+{
+	GF_CHECK(sp && model && pixels && rng && rng->state && state && nstars, GF_E_ARG);
+	GF_CHECK(sp->nthreads > 0 && (uint32_t)sp->nthreads <= rng->nstreams, GF_E_STREAMS);
+	*state = nullptr;
+	hipStream_t st = gf::as_stream(stream);
+	auto d = std::make_unique<gf_draw_state>();
+	d->sp = *sp;
+	d->m = *model;
+	d->pixels = pixels;
+	d->ext = ext ? *ext : gf_extinction{};
+	d->foot = foot ? *foot : gf_footprint{};
+	d->ns = rng->nstreams;
+	const int nt = sp->nthreads;
+	GF_HIP(hipMalloc(&d->st0, (size_t)3 * d->ns * sizeof(uint32_t)));
+	GF_HIP(hipMalloc(&d->counts, (size_t)nt * sizeof(uint32_t)));
+	GF_HIP(hipMalloc(&d->offs, ((size_t)nt + 1) * sizeof(uint64_t)));
+	GF_HIP(hipMalloc(&d->res, 2 * sizeof(gf_draw_resume)));
+	GF_HIP(hipMemcpyAsync(d->st0, rng->state, (size_t)3 * d->ns * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+	// the count pass leaves the streams where the whole draw leaves them
+	const unsigned grid = gf::blocks_for(nt, 64);
+	k_draw_compat<<<grid, 64, 0, st>>>(d->sp, d->m, pixels, d->ext, d->foot, rng->state, rng->nstreams, 0,
+	                                   d->counts, d->offs, gf_skygen_cols{}, 1);
+	GF_LAUNCHED();
+	int rc = gf::exclusive_scan_u32_u64(d->counts, d->offs, nt, st);
+	if (rc) return rc;
+	GF_HIP(hipMemcpyAsync(&d->total, d->offs + nt, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+	GF_HIP(hipStreamSynchronize(st));
+	*nstars = d->total;
+	*state = d.release();
+	return 0;
+}
+
+int gf_skygen_draw_next(gf_draw_state *d, const gf_skygen_cols *out, uint64_t capacity, uint64_t *nwritten,
+                        void *stream)
+{
+	GF_CHECK(d && out && nwritten && capacity > 0, GF_E_ARG);
+	hipStream_t st = gf::as_stream(stream);
+	const uint64_t r0 = d->next, r1 = std::min(d->total, r0 + capacity);
+	*nwritten = r1 - r0;
+	if (r1 == r0) return 0;
+	const gf_draw_resume *in = d->res + (d->batch & 1);
+	gf_draw_resume *outr = d->res + ((d->batch + 1) & 1);
+	k_draw_window<<<gf::blocks_for(d->sp.nthreads, 64), 64, 0, st>>>(d->sp, d->m, d->pixels, d->ext, d->foot, d->st0,
+	                                                                  d->ns, d->offs, r0, r1, in, outr, *out);
+	GF_LAUNCHED();
+	d->next = r1;
+	d->batch++;
+	return 0;
+}
+
+void gf_skygen_draw_end(gf_draw_state *d)
+{
+	delete d;
 }
 
 int gf_skygen_plan(const gf_skygen_params *sp, const gf_model *model,

@@ -492,6 +492,27 @@ int gf_clip_footprint(const gf_skygen_params *sp, const gf_footprint *foot, gf_c
 
 /* Draw plan for the cumulative-table draw below: per (pixel, DM-bin) star
  * counts and their exclusive prefix sums, device resident. */
+/* Capacity-limited, resumable reference-order draw: skygenHost::drawSources
+ * with runtime_state (skygen/skyconfig_impl.h:434-534, skygen/skygen.h:
+ * 184-257), where a draw kernel stops when the output table is full and the
+ * next call resumes every thread where it stopped.  gf_skygen_draw_begin
+ * keeps the streams' starting states, counts every thread's stars (moving
+ * the streams past the whole draw, as the unlimited draw leaves them) and
+ * returns their number; each gf_skygen_draw_next writes the next
+ * min(capacity, remaining) stars in (thread, crawl) order into `out`,
+ * resuming the thread the previous call stopped inside; *nwritten is 0 once
+ * all are written.  The calls' rows concatenated equal one
+ * gf_skygen_draw_compat of unlimited capacity.  The state holds device
+ * memory for the stream copy and 12 bytes per thread until
+ * gf_skygen_draw_end. */
+typedef struct gf_draw_state gf_draw_state;
+int  gf_skygen_draw_begin(const gf_skygen_params *sp, const gf_model *model, const gf_pencil_beam *pixels,
+                          const gf_extinction *ext, const gf_footprint *foot, gf_rng *rng, gf_draw_state **state,
+                          uint64_t *nstars, void *stream);
+int  gf_skygen_draw_next(gf_draw_state *state, const gf_skygen_cols *out, uint64_t capacity, uint64_t *nwritten,
+                         void *stream);
+void gf_skygen_draw_end(gf_draw_state *state);
+
 typedef struct gf_draw_plan gf_draw_plan;
 
 /* Poisson-draw the star count of every (pixel, DM-bin) pair from the

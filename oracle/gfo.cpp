@@ -845,6 +845,67 @@ GFO_API int64_t gfo_draw_compat(const gf_skygen_params *spp, const gf_model *mp,
 	return (int64_t)s.row;
 }
 
+/* Rows [r0, r1) of the reference-order draw from the streams' states at the
+ * start of the draw (st0, only read): every thread's crawl replayed in
+ * thread order, the stars before r0 drawn without being written (their
+ * draws move the stream all the same).  The capacity-limited draw of
+ * skygenHost::drawSources (skygen/skyconfig_impl.h:434-534) writes these
+ * rows in its batch [r0, r1).  Returns the number of rows written. */
+GFO_API uint64_t gfo_draw_window(const gf_skygen_params *spp, const gf_model *mp, const gf_pencil_beam *pixels,
+                                 const gf_extinction *extp, const gf_footprint *footp, const uint32_t *st0,
+                                 uint32_t ns, uint64_t r0, uint64_t r1, const gf_skygen_cols *out)
+{
+	const gf_skygen_params &sp = *spp;
+	const gf_model &m = *mp;
+	gf_extinction ext{};
+	if (extp) ext = *extp;
+	gf_footprint foot{};
+	if (footp) foot = *footp;
+	const int block = 10;
+	uint64_t row = 0;
+	for (int tid = 0; tid < sp.nthreads && row < r1; tid++) {
+		gfo_rng rng = rng_load(st0, ns, tid);
+		int ilb = 0, im = 0, iM = 0, k = block * (tid - sp.nthreads), bc = 0;
+		float D = 0, Am = 0, R = 0;
+		gf_pencil_beam pix{};
+		for (;;) {
+			bool moved;
+			if (bc == 0) {
+				bc = block;
+				k += block * sp.nthreads;
+				diag_ij(ilb, im, iM, k, sp.nm, sp.nM);
+				if (ilb >= sp.npixels) break;
+				pix = pixels[ilb];
+				moved = true;
+			} else {
+				moved = advance(ilb, im, iM, pix, pixels, sp.npixels, sp.nm, sp.nM);
+			}
+			bc--;
+			float M = sp.M1 - iM * sp.dM;
+			if (moved) {
+				if (ilb >= sp.npixels) break;
+				float x, y, z;
+				cell_pos(sp, ext, pix, M, im, D, Am, x, y, z);
+				R = rho_xyz(m, x, y, z);
+			}
+			if ((sp.dmin != 0.f || sp.dmax != 0.f) && (sp.dmin > D || sp.dmax <= D)) continue;
+			float mm = sp.m0 + sp.dm * im + Am;
+			if (mm > sp.m1) continue;
+			float rho = tex1(m.lf.data, m.lf.n, m.lf.tc, M) * R;
+			rho *= m.norm;
+			rho *= D * D * D;
+			rho *= pix.dA * POGSON * sp.dm * sp.dM;
+			unsigned int nd = rng.poisson(rho);
+			for (unsigned int q = 0; q < nd && row < r1; q++, row++) {
+				Sink s{*out, row >= r0, row >= r0 ? row - r0 : 0};
+				one_star(sp, m, ext, foot, pix, M, im, Am, rng, s);
+			}
+			if (row >= r1) break;
+		}
+	}
+	return row > r0 ? row - r0 : 0;
+}
+
 /* ---- cumulative-table draw: the build's (pixel, DM-bin) restatement ---- */
 
 /* Luminosity function at the absolute-magnitude bin centres, and its FP64

@@ -45,6 +45,10 @@ def lib():
         _o.gfo_emit.argtypes = [P(A.gf_skygen_params), P(A.gf_model), vp, P(A.gf_extinction),
                                 P(A.gf_footprint), vp, C.c_uint64, C.c_uint64, C.c_uint64, vp, C.c_uint32,
                                 P(A.gf_skygen_cols)]
+        _o.gfo_draw_window.restype = C.c_uint64
+        _o.gfo_draw_window.argtypes = [P(A.gf_skygen_params), P(A.gf_model), vp, P(A.gf_extinction),
+                                       P(A.gf_footprint), vp, C.c_uint32, C.c_uint64, C.c_uint64,
+                                       P(A.gf_skygen_cols)]
         _o.gfo_order.restype = C.c_uint64
         _o.gfo_order.argtypes = [P(A.gf_skygen_cols), C.c_uint64, P(A.gf_skygen_cols)]
         _o.gfo_clip.argtypes = [P(A.gf_skygen_params), P(A.gf_footprint), P(A.gf_skygen_cols), C.c_uint64]

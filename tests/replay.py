@@ -98,12 +98,14 @@ def replay(view, chunk_rows=30_000_000, pixel_index=None, on_chunk=None, time_bu
             "stopped_early": stopped}
 
 
-def replay_reference(view, on_chunk=None):
+def replay_reference(view, on_chunk=None, capacity=None):
     """The catalog's reference draw mode on the oracle, model by model: the
-    crawl (sp.nthreads threads on streams 0 ..), the batch order of
-    drawSources (sort by hidden, l, b, DM, M, Am; truncate at the first
-    hidden row), the clipper, then the module chain over the visible rows.
-    Returns dict(generated, visible)."""
+    crawl (sp.nthreads threads on streams 0 ..), in batches of at most
+    `capacity` stars (all at once by default) drawn from the streams' states
+    at the model's start while the count pass leaves the streams past the
+    whole draw; per batch the order of drawSources (sort by hidden, l, b,
+    DM, M, Am; truncate at the first hidden row), the clipper, then the
+    module chain over the visible rows.  Returns dict(generated, visible)."""
     o = O.lib()
     ns = view.nstreams
     state = np.ctypeslib.as_array(view.rng_state0, shape=(3 * ns,)).copy()
@@ -115,24 +117,28 @@ def replay_reference(view, on_chunk=None):
     generated = visible = 0
     for mi in range(view.nmodels):
         m = view.models[mi]
-        n = o.gfo_draw_compat(C.byref(sp), C.byref(m), C.cast(pix, C.c_void_p), _ext(view), None,
-                              O.ptr(state.copy()), ns, None, 0)
-        drawn = HostTable(max(n, 1), spec)
-        dcols = drawn.skygen_cols()
-        assert o.gfo_draw_compat(C.byref(sp), C.byref(m), C.cast(pix, C.c_void_p), _ext(view), None, O.ptr(state),
-                                 ns, C.byref(dcols), n) == n
+        st0 = state.copy()
+        n = o.gfo_draw_compat(C.byref(sp), C.byref(m), C.cast(pix, C.c_void_p), _ext(view), None, O.ptr(state), ns,
+                              None, 0)
         generated += n
-        t = HostTable(max(n, 1), spec)
-        cols = t.skygen_cols()
-        nv = o.gfo_order(C.byref(dcols), n, C.byref(cols))
-        o.gfo_clip(C.byref(sp), C.byref(foot), C.byref(cols), nv)
-        tab = t.table()
-        for b0 in range(0, nv, sub):
-            ks = A.gf_rows(b0, 256, min(nv, b0 + sub), 0)
-            assert o.gfo_module_chain(C.byref(ks), C.byref(view.chain), O.ptr(state), ns, C.byref(tab)) == 0
-        visible += int((t.arr("hidden")[0, :nv] == 0).sum())
-        if on_chunk is not None and nv:
-            on_chunk(t, nv, m.comp)
+        cap = max(1, min(capacity or n, n))
+        for r0 in range(0, n, cap):
+            r1 = min(n, r0 + cap)
+            drawn = HostTable(r1 - r0, spec)
+            dcols = drawn.skygen_cols()
+            assert o.gfo_draw_window(C.byref(sp), C.byref(m), C.cast(pix, C.c_void_p), _ext(view), None,
+                                     O.ptr(st0), ns, r0, r1, C.byref(dcols)) == r1 - r0
+            t = HostTable(r1 - r0, spec)
+            cols = t.skygen_cols()
+            nv = o.gfo_order(C.byref(dcols), r1 - r0, C.byref(cols))
+            o.gfo_clip(C.byref(sp), C.byref(foot), C.byref(cols), nv)
+            tab = t.table()
+            for b0 in range(0, nv, sub):
+                ks = A.gf_rows(b0, 256, min(nv, b0 + sub), 0)
+                assert o.gfo_module_chain(C.byref(ks), C.byref(view.chain), O.ptr(state), ns, C.byref(tab)) == 0
+            visible += int((t.arr("hidden")[0, :nv] == 0).sum())
+            if on_chunk is not None and nv:
+                on_chunk(t, nv, m.comp)
     return {"generated": generated, "visible": visible}
 
 

@@ -70,22 +70,29 @@ def test_c0_catalog_matches_oracle_replay(c0):
     assert nvis > 0.5 * expected
 
 
-def check_reference_draw(cmd, seed):
-    """The catalog's reference draw mode: the crawl, the batch order of
-    drawSources (hidden rows dropped, the rest sorted by l, b, DM, M, Am),
-    the clipper and the module chain over the visible rows; one batch per
-    model, every column bit-identical to the oracle's restatement.
+def check_reference_draw(cmd, seed, capacity=None, keep=None):
+    """The catalog's reference draw mode: the crawl in batches of at most
+    `capacity` stars (one batch per model by default), per batch the order
+    of drawSources (hidden rows dropped, the rest sorted by l, b, DM, M,
+    Am), the clipper and the module chain over the visible rows; every
+    column bit-identical to the oracle's restatement.  keep: a list that
+    receives the (table, rows) of every batch.
     Returns (rows handed out, rows drawn, rows hidden by the clipper)."""
     cat = Catalog(cmd, seed=seed)
     cat.set("draw", 1)
+    if capacity:
+        cat.set("chunk_rows", capacity)
     cat.count()
     view = R.host_view(G.lib(), cat._h)
     spec = catalog_spec(nbands=view.nbands, nabsmag=view.nabsmag)
     got = collect(cat, spec)
     ref = []
-    info = R.replay_reference(view, on_chunk=lambda t, n, comp: ref.append((t, n, comp)))
+    info = R.replay_reference(view, on_chunk=lambda t, n, comp: ref.append((t, n, comp)), capacity=capacity)
     cat.close()
-    assert len(got) == view.nmodels
+    if keep is not None:
+        keep.extend(got)
+    if capacity is None:
+        assert len(got) == view.nmodels
     assert [(n, c) for _, n, c in got] == [(n, c) for _, n, c in ref]
     clipped = 0
     for (a, n, _), (b, _, _) in zip(got, ref):
@@ -106,6 +113,40 @@ def test_c0_reference_draw_matches_oracle(c0):
     out, drawn, clipped = check_reference_draw(c0, 42)
     assert 0 < out <= drawn
     assert clipped > 0   # the rect footprint's edge pixels
+
+
+SKY_COLS = ("lb", "projIdx", "projXY", "DM", "absmag", "XYZ", "comp", "Am", "AmInf")
+
+
+def _visible_rows(batches, comp):
+    """The skygen columns of every handed-out row of model `comp` as one
+    sorted byte array."""
+    rows = []
+    for t, n, c in batches:
+        if c != comp:
+            continue
+        vis = t.arr("hidden")[0, :n] == 0
+        parts = [np.ascontiguousarray(t.arr(c)[:, :n][:, vis].T).view(np.uint8).reshape(int(vis.sum()), -1)
+                 for c in SKY_COLS]
+        rows.append(np.concatenate(parts, axis=1))
+    r = np.concatenate(rows)
+    return r[np.lexsort(r.T[::-1])]
+
+
+def test_c0_capacity_limited_reference_draw(c0):
+    """The draw stopped every 4,999 stars and resumed where each batch
+    stopped (skygenHost::drawSources with runtime_state): every batch
+    bit-identical to the oracle's batch, and the first model's stars handed
+    out over all its batches the same multiset as its one unlimited batch.
+    (The module chain runs between batches on the same streams, so later
+    models start from other stream states, as in the reference.)"""
+    limited, unlimited = [], []
+    check_reference_draw(c0, 42, capacity=4999, keep=limited)
+    check_reference_draw(c0, 42, keep=unlimited)
+    assert sum(1 for _, _, c in limited if c == 0) > 3
+    assert max(n for _, n, _ in limited) <= 4999
+    a, b = _visible_rows(limited, 0), _visible_rows(unlimited, 0)
+    assert a.shape == b.shape and np.array_equal(a, b)
 
 
 def test_c0_generated_matches_expected(c0):

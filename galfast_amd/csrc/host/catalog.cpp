@@ -95,6 +95,7 @@ gf_bitmap read_component_map(const Config &cfg, const std::string &key, Componen
 
 struct Model {
 	gf_model m{};
+	gf_skygen_params sp{};   // the job's binning with this model's absolute-magnitude range
 	Tex1D lf;
 	DevBuf lf_dev;
 	std::string name;
@@ -164,6 +165,7 @@ struct gf_catalog {
 
 	// host copies for gf_catalog_host_view
 	std::vector<gf_model> models_h;
+	std::vector<gf_skygen_params> models_sp_h;
 	gf_photometry_data phot_h{};
 	gf_multiples_data mult_h{};
 	gf_photoerr_data perr_h{};
@@ -971,14 +973,23 @@ void open_catalog(gf_catalog &c, const std::string &cmdfile)
 	pixelize(c, dx);
 	load_extinction(c, c.skygen.get("extmaps", ""));
 	for (auto &mc : c.model_cfgs) load_model(c, mc);
-	// hint_absmag: restrict the absolute-magnitude range to the LF's support
-	// (skygen/skyconfig_impl.h:258-263, skygen/model_lib.h:97-112); one range is
-	// shared by all models here, so only a common restriction is applied
+	// hint_absmag: each model's absolute-magnitude range restricted to its
+	// LF's support (skygen/skyconfig_impl.h:258-263, skygen/model_lib.h:
+	// 97-112), then reset_absmag (skygen/os_skygen.cpp:401-407)
 	for (auto &m : c.models) {
+		m->sp = c.sp;
 		float X0 = m->lf.tc[0];
 		float X1 = (float)((double)(m->lf.v.size() - 1) / m->lf.tc[1] + X0);
 		float M0 = c.sp.M0, M1 = (float)(c.sp.M1 + 0.5 * c.sp.dM);
-		if (X0 > M0 || X1 < M1) fail("luminosity function does not cover the absolute-magnitude range");
+		bool narrowed = false;
+		if (X0 > M0) { narrowed = true; M0 = X0; }
+		if (X1 < M1) { narrowed = true; M1 = X1; }
+		if (narrowed) {
+			m->sp.M0 = M0;
+			m->sp.nM = (int)round((M1 - M0) / c.sp.dM);
+			m->sp.M1 = (float)(M1 - 0.5 * c.sp.dM);
+			if (m->sp.nM < 1) fail("luminosity function of model " + m->name + " has no absolute magnitude in range");
+		}
 	}
 	// modules in pipeline order (pipeline.h:75-89)
 	for (auto &mc : c.module_cfgs) {
@@ -1043,10 +1054,12 @@ void open_catalog(gf_catalog &c, const std::string &cmdfile)
 
 	// host mirrors of the device parameter blocks
 	c.models_h.clear();
+	c.models_sp_h.clear();
 	for (auto &m : c.models) {
 		gf_model h = m->m;
 		h.lf.data = m->lf.v.data();
 		c.models_h.push_back(h);
+		c.models_sp_h.push_back(m->sp);
 	}
 	if (c.has_phot) {
 		c.phot_h = c.phot;
@@ -1153,7 +1166,7 @@ int run_catalog_reference(gf_catalog &c, gf_chunk_fn fn, void *user, gf_run_stat
 		// at its first hidden row, clipped and run through the modules
 		gf_draw_state *ds = nullptr;
 		uint64_t total = 0;
-		int rc = gf_skygen_draw_begin(&c.sp, &m.m, pix, ext, nullptr, &c.rng, &ds, &total, A);
+		int rc = gf_skygen_draw_begin(&m.sp, &m.m, pix, ext, nullptr, &c.rng, &ds, &total, A);
 		if (rc) return rc;
 		std::unique_ptr<gf_draw_state, void (*)(gf_draw_state *)> hold(ds, gf_skygen_draw_end);
 		const uint64_t cap = std::max<uint64_t>(1, std::min<uint64_t>(c.chunk_rows, total));
@@ -1279,7 +1292,7 @@ int run_catalog(gf_catalog &c, gf_chunk_fn fn, void *user, gf_run_stats *stats)
 		// a model's plan is rewritten only by its next run, which starts
 		// after every stream of this one has finished
 		GF_HIP(hipEventRecord(c.ev[0], A));
-		rc = gf::skygen_plan(&c.sp, &m.m, (const gf_pencil_beam *)c.pix_dev.p, c.has_ext ? &c.ext : nullptr,
+		rc = gf::skygen_plan(&m.sp, &m.m, (const gf_pencil_beam *)c.pix_dev.p, c.has_ext ? &c.ext : nullptr,
 		                     (const double *)m.wpk.p, &c.rng, &m.plan, &total, A);
 		if (rc) return rc;
 		gf_draw_plan *plan = m.plan;
@@ -1449,7 +1462,6 @@ int gf_catalog_count(gf_catalog *c, gf_count_result *per_model, double *expected
 {
 	GF_CHECK(c, GF_E_ARG);
 	double tot = 0;
-	const uint64_t K = (uint64_t)(c->sp.nm + c->sp.nM - 1);
 	const size_t np = (size_t)c->sp.npixels;
 	c->counted = -1;
 	if (c->per_pixel.bytes < np * sizeof(double)) c->per_pixel.alloc(np * sizeof(double));
@@ -1461,8 +1473,9 @@ int gf_catalog_count(gf_catalog *c, gf_count_result *per_model, double *expected
 		tot = 0;
 		for (size_t i = 0; i < c->models.size(); i++) {
 			Model &m = *c->models[i];
+			const uint64_t K = (uint64_t)(m.sp.nm + m.sp.nM - 1);
 			if (m.wpk.bytes < np * K * sizeof(double)) m.wpk.alloc(np * K * sizeof(double));
-			int rc = gf_skygen_count(&c->sp, &m.m, (const gf_pencil_beam *)c->pix_dev.p,
+			int rc = gf_skygen_count(&m.sp, &m.m, (const gf_pencil_beam *)c->pix_dev.p,
 			                         c->has_ext ? &c->ext : nullptr, c->acc_mode, &m.counts,
 			                         check ? (double *)c->per_pixel.p : nullptr, (double *)m.wpk.p, c->stream);
 			if (rc) return rc;
@@ -1645,6 +1658,7 @@ int gf_catalog_host_view_get(const gf_catalog *c, gf_catalog_host_view *v)
 	v->nmodels = (int)c->models_h.size();
 	v->nbands = c->nbands;
 	v->models = c->models_h.data();
+	v->model_sp = c->models_sp_h.data();
 	v->foot = c->foot.f;
 	gf_chain &ch = v->chain;
 	if (c->has_feh) ch.feh = &c->feh;
@@ -1790,7 +1804,11 @@ struct SReq {
 	gf_skygen_params sp{};
 	uint32_t S = 0;
 	std::vector<uint32_t> state;   // [a | c | x] of S streams
-	uint64_t nunits_model = 0;     // npix * K
+	uint64_t nunits_model = 0;     // npix * K of the job's binning: the size of the stream table
+	std::vector<gf_skygen_params> msp;   // per model: the binning with its absolute-magnitude range
+	std::vector<uint64_t> munits;        // per model: npix * K of its binning
+	uint64_t nunits = 0;                 // all models' units
+	uint64_t nI = 0;                     // stream indices all models' units take
 	double expected = 0;
 	double counted = -1;           // expected rows from a count of an earlier batch, -1 before
 	std::vector<float> extmin;     // extinction minima per pixel and DM slice, with extinction maps
@@ -1871,9 +1889,21 @@ void prepare(gf_server &s, SReq &q)
 	q.sp.npixels = (int)q.pix.size();
 	const uint64_t K = (uint64_t)(q.sp.nm + q.sp.nM - 1);
 	q.nunits_model = (uint64_t)q.pix.size() * K;
+	q.msp.clear();
+	q.munits.clear();
+	q.nunits = 0;
+	for (auto &m : s.base.models) {
+		gf_skygen_params sp = q.sp;
+		sp.M0 = m->sp.M0; sp.M1 = m->sp.M1; sp.nM = m->sp.nM;
+		q.msp.push_back(sp);
+		q.munits.push_back((uint64_t)q.pix.size() * (uint64_t)(sp.nm + sp.nM - 1));
+		q.nunits += q.munits.back();
+	}
 	uint64_t want = std::max<uint64_t>(64, q.nunits_model);
 	want = (want + 63) / 64 * 64;
 	q.S = (uint32_t)std::min<uint64_t>(65536, want);
+	q.nI = 0;
+	for (uint64_t u : q.munits) q.nI += (u + q.S - 1) / q.S;
 	q.state.resize(3 * (size_t)q.S);
 	gf_mwc_seed_host(gf_seed_from_pipeline_seed(q.r.seed), s.primes.data(), (uint32_t)s.primes.size(), q.S,
 	                 q.state.data());
@@ -1941,7 +1971,7 @@ template <typename T> void grow(Worker &w, uint64_t share, DevBuf &b, size_t n)
 uint64_t request_fixed_bytes(const gf_server &s, const SReq &q)
 {
 	const uint64_t nm = s.hmodels.size();
-	return q.nunits_model * nm * 28 + (uint64_t)q.pix.size() * (sizeof(gf_pencil_beam) + 8) + (uint64_t)q.S * 12 +
+	return q.nunits * 28 + (uint64_t)q.pix.size() * (sizeof(gf_pencil_beam) + 8) + (uint64_t)q.S * 12 +
 	       nm * (2 * sizeof(gf_serve_seg) + 8) + sizeof(gf_serve_req) + 16 + q.extmin.size() * 4;
 }
 
@@ -1999,7 +2029,7 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	std::vector<uint64_t> uoff(nr + 1, 0);
 	std::vector<uint32_t> poff(nr + 1, 0), loff(nr + 1, 0);
 	for (int r = 0; r < nr; r++) {
-		uoff[r + 1] = uoff[r] + batch[r]->nunits_model * nm;
+		uoff[r + 1] = uoff[r] + batch[r]->nunits;
 		poff[r + 1] = poff[r] + (uint32_t)batch[r]->pix.size();
 		loff[r + 1] = loff[r] + batch[r]->S;
 	}
@@ -2031,16 +2061,17 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	uint32_t *dstate = (uint32_t *)w.state.p;
 	// segments per model (launch order: requests), and all segments in row order
 	std::vector<gf_serve_seg> segs((size_t)nm * nr), segall((size_t)nr * nm);
-	for (int m = 0; m < nm; m++) {
-		for (int r = 0; r < nr; r++) {
-			SReq &q = *batch[r];
+	for (int r = 0; r < nr; r++) {
+		SReq &q = *batch[r];
+		uint64_t u0 = uoff[r], i0 = 0;
+		for (int m = 0; m < nm; m++) {
 			gf_serve_seg g{};
-			g.sp = q.sp;
+			g.sp = q.msp[m];
 			g.foot = q.foot;
 			g.lfP = (const double *)s.dlfP.p + (size_t)m * (s.base.sp.nM + 1);
 			g.state = dstate + soff[r];
-			g.unit0 = uoff[r] + (uint64_t)m * q.nunits_model;
-			g.nunits = q.nunits_model;
+			g.unit0 = u0;
+			g.nunits = q.munits[m];
 			g.pix0 = poff[r];
 			g.npix = (uint32_t)q.pix.size();
 			g.slot0 = poff[r];
@@ -2049,7 +2080,9 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 			g.model = m;
 			g.req = r;
 			// the models of a request use its streams in turn
-			g.i0 = (uint32_t)((uint64_t)m * ((q.nunits_model + q.S - 1) / q.S));
+			g.i0 = (uint32_t)i0;
+			u0 += q.munits[m];
+			i0 += (q.munits[m] + q.S - 1) / q.S;
 			segs[(size_t)m * nr + r] = g;
 			segall[(size_t)r * nm + m] = g;
 		}
@@ -2151,7 +2184,7 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 		rq[r].row1 = roff[r + 1];
 		rq[r].lane0 = loff[r];
 		rq[r].S = q.S;
-		rq[r].advance = ((uint64_t)nm * ((q.nunits_model + q.S - 1) / q.S)) << GF_UNIT_SPAN_BITS;
+		rq[r].advance = q.nI << GF_UNIT_SPAN_BITS;
 	}
 	GF_HIP(hipMemcpyAsync(w.reqs.p, rq.data(), rq.size() * sizeof(gf_serve_req), hipMemcpyHostToDevice, st));
 	// the streams move past every unit before the module chain draws from them
@@ -2381,7 +2414,9 @@ int gf_server_create(const char *cmd_conf, const char *datadir, const gf_server_
 		return GF_E_CONFIG;
 	}
 	for (size_t i = 0; i < s->hmodels.size(); i++) {
-		int rc = gf::lf_prefix(c.sp, s->hmodels[i], (double *)s->dlfP.p + i * ((size_t)c.sp.nM + 1), c.stream);
+		// each model's prefix over its own absolute-magnitude bins (at most the job's)
+		int rc = gf::lf_prefix(c.models[i]->sp, s->hmodels[i], (double *)s->dlfP.p + i * ((size_t)c.sp.nM + 1),
+		                       c.stream);
 		if (rc) return rc;
 	}
 	// module chain

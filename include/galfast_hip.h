@@ -705,6 +705,8 @@ typedef struct gf_catalog_host_view {
     uint32_t              nstreams;
     const uint32_t       *rng_state0;
     gf_extinction         ext;      /* host tables; all data NULL without extinction */
+    const gf_skygen_params *model_sp;   /* per model: sp with the model's absolute-magnitude
+                                         * range (its LF's support, hint_absmag) */
 } gf_catalog_host_view;
 
 int gf_catalog_host_view_get(const gf_catalog *c, gf_catalog_host_view *view);

@@ -21,7 +21,17 @@ class gf_catalog_host_view(C.Structure):
     _fields_ = [("sp", A.gf_skygen_params), ("pixels", C.c_void_p), ("nmodels", C.c_int32),
                 ("nbands", C.c_int32), ("models", C.POINTER(A.gf_model)), ("foot", A.gf_footprint),
                 ("chain", A.gf_chain), ("nabsmag", C.c_int32), ("nstreams", C.c_uint32),
-                ("rng_state0", C.POINTER(C.c_uint32)), ("ext", A.gf_extinction)]
+                ("rng_state0", C.POINTER(C.c_uint32)), ("ext", A.gf_extinction),
+                ("model_sp", C.POINTER(A.gf_skygen_params))]
+
+
+def model_sp(view, mi, npixels=None):
+    """The binning of model mi (its absolute-magnitude range), optionally
+    for a subset of npixels pixels."""
+    sp = A.gf_skygen_params.from_buffer_copy(view.model_sp[mi])
+    if npixels is not None:
+        sp.npixels = npixels
+    return sp
 
 
 def _ext(view):
@@ -49,10 +59,6 @@ def replay(view, chunk_rows=30_000_000, pixel_index=None, on_chunk=None, time_bu
         pix = allpix
     else:
         pix = (A.gf_pencil_beam * len(pixel_index))(*[allpix[i] for i in pixel_index])
-    sp = A.gf_skygen_params.from_buffer_copy(view.sp)
-    sp.npixels = len(pix)
-    K = sp.nm + sp.nM - 1
-    U = sp.npixels * K
     spec = catalog_spec(nbands=max(view.nbands, 1), nabsmag=max(view.nabsmag, 1))
     sub = 65536 * 256
     t0 = time.perf_counter()
@@ -61,6 +67,9 @@ def replay(view, chunk_rows=30_000_000, pixel_index=None, on_chunk=None, time_bu
     foot = A.gf_footprint.from_buffer_copy(view.foot)
     for mi in range(view.nmodels):
         m = view.models[mi]
+        sp = model_sp(view, mi, len(pix))
+        K = sp.nm + sp.nM - 1
+        U = sp.npixels * K
         res = A.gf_count_result()
         wpk = np.zeros(U)
         o.gfo_count_bins(C.byref(sp), C.byref(m), C.cast(pix, C.c_void_p), _ext(view), C.byref(res), None,
@@ -109,14 +118,14 @@ def replay_reference(view, on_chunk=None, capacity=None):
     o = O.lib()
     ns = view.nstreams
     state = np.ctypeslib.as_array(view.rng_state0, shape=(3 * ns,)).copy()
-    sp = A.gf_skygen_params.from_buffer_copy(view.sp)
-    pix = (A.gf_pencil_beam * sp.npixels).from_address(view.pixels)
+    pix = (A.gf_pencil_beam * view.sp.npixels).from_address(view.pixels)
     spec = catalog_spec(nbands=max(view.nbands, 1), nabsmag=max(view.nabsmag, 1))
     foot = A.gf_footprint.from_buffer_copy(view.foot)
     sub = 65536 * 256
     generated = visible = 0
     for mi in range(view.nmodels):
         m = view.models[mi]
+        sp = model_sp(view, mi)
         st0 = state.copy()
         n = o.gfo_draw_compat(C.byref(sp), C.byref(m), C.cast(pix, C.c_void_p), _ext(view), None, O.ptr(state), ns,
                               None, 0)
@@ -173,16 +182,17 @@ def replay_request(view):
     over the request's 256-row chunks, chunk c on stream c.
     Returns (HostTable, expected)."""
     o = O.lib()
-    sp = A.gf_skygen_params.from_buffer_copy(view.sp)
-    pix = (A.gf_pencil_beam * sp.npixels).from_address(view.pixels)
-    K = sp.nm + sp.nM - 1
-    U = sp.npixels * K
-    S, state = request_streams(view, sp.npixels)
+    npix = view.sp.npixels
+    pix = (A.gf_pencil_beam * npix).from_address(view.pixels)
+    S, state = request_streams(view, npix)
     foot = A.gf_footprint.from_buffer_copy(view.foot)
     plans = []
     expected = 0.0
     for mi in range(view.nmodels):
         m = view.models[mi]
+        sp = model_sp(view, mi)
+        K = sp.nm + sp.nM - 1
+        U = npix * K
         res = A.gf_count_result()
         wpk = np.zeros(U)
         o.gfo_count_bins(C.byref(sp), C.byref(m), C.cast(pix, C.c_void_p), _ext(view), C.byref(res), None,
@@ -194,12 +204,12 @@ def replay_request(view):
         o.gfo_plan_advance(O.ptr(state), S, U)
         offs = np.zeros(U + 1, dtype=np.uint64)
         offs[1:] = np.cumsum(counts, dtype=np.uint64)
-        plans.append((offs, base))
-    total = int(sum(int(p[-1]) for p, _ in plans))
+        plans.append((offs, base, sp, U))
+    total = int(sum(int(p[-1]) for p, _, _, _ in plans))
     spec = catalog_spec(nbands=max(view.nbands, 1), nabsmag=max(view.nabsmag, 1))
     t = HostTable(total, spec)
     row = 0
-    for mi, (offs, base) in enumerate(plans):
+    for mi, (offs, base, sp, U) in enumerate(plans):
         n = int(offs[-1])
         if n:
             cols = _shift(t.skygen_cols(), row)

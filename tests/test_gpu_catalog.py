@@ -199,3 +199,60 @@ def test_c4_extinction_multiples_six_bands_match_oracle_replay(c4s):
             assert not bad.any(), "%s: %d of %d differ" % (c, bad.sum(), bad.size)
     assert nvis > 0.3 * expected
     assert am_pos > 0.9 * nvis          # the dust disk reddens nearly every star
+
+
+@pytest.fixture(scope="module")
+def c0narrow(tmp_path_factory):
+    """Config 0 with the thin disk's luminosity function tabulated only over
+    Mr in [5, 12] (the job's range is 3-15)."""
+    d = str(tmp_path_factory.mktemp("c0n"))
+    cmd = configs.write_c0(d)
+    M, phi = configs.seeded_lf()
+    keep = (M >= 5.0 - 1e-9) & (M <= 12.0 + 1e-9)
+    np.savetxt(os.path.join(d, "lf_narrow.txt"), np.c_[M[keep], phi[keep]], fmt="%.10g")
+    path = os.path.join(d, "model.thin.conf")
+    with open(path) as f:
+        text = f.read()
+    with open(path, "w") as f:
+        f.write(text.replace("lumfunc = lf.txt", "lumfunc = lf_narrow.txt"))
+    return cmd
+
+
+def test_model_absolute_magnitude_range_narrowed_to_its_lf(c0narrow):
+    """hint_absmag + reset_absmag (skygen/skyconfig_impl.h:258-263,
+    skygen/model_lib.h:97-112, skygen/os_skygen.cpp:401-407): the thin
+    disk's bins cover only its LF's texture span, the other models keep
+    the job's; the catalog matches the oracle replay row for row, and
+    the thin disk's stars lie in its bins."""
+    cat = Catalog(c0narrow, seed=11, chunk_rows=40000)
+    cat.count()
+    view = R.host_view(G.lib(), cat._h)
+    m0 = view.models[0]
+    X0 = np.float32(m0.lf.tc[0])
+    X1 = np.float32(np.float64(m0.lf.n - 1) / np.float64(np.float32(m0.lf.tc[1])) + np.float64(X0))
+    dM = np.float32(view.sp.dM)
+    sp0 = R.model_sp(view, 0)
+    assert np.float32(sp0.M0) == X0 and X0 > np.float32(view.sp.M0)
+    assert sp0.nM == int(round(float((X1 - X0) / dM)))
+    assert np.float32(sp0.M1) == np.float32(np.float64(X1) - 0.5 * np.float64(dM))
+    assert sp0.nM < view.sp.nM
+    for mi in (1, 2):
+        sp = R.model_sp(view, mi)
+        assert (sp.M0, sp.M1, sp.nM) == (view.sp.M0, view.sp.M1, view.sp.nM)
+    spec = catalog_spec(nbands=view.nbands, nabsmag=view.nabsmag)
+    got = collect(cat, spec)
+    ref = []
+    R.replay(view, chunk_rows=40000, on_chunk=lambda t, n, comp: ref.append((t, n, comp)))
+    cat.close()
+    assert [(n, c) for _, n, c in got] == [(n, c) for _, n, c in ref]
+    thin = 0
+    for (a, n, comp), (b, _, _) in zip(got, ref):
+        vis = b.arr("hidden")[0] == 0
+        for c in ("lb", "DM", "absmag", "XYZ", "hidden", "FeH", "mags"):
+            assert np.array_equal(a.arr(c)[:, vis], b.arr(c)[:, vis]), c
+        if comp == 0:
+            M = a.arr("absmag")[0][:n]
+            assert M.min() >= np.float32(sp0.M1) - (sp0.nM - 0.5) * dM - 1e-5
+            assert M.max() <= np.float32(sp0.M1) + 0.5 * dM + 1e-5
+            thin += n
+    assert thin > 1000

@@ -245,3 +245,31 @@ def test_rect_and_extinction_requests_match_oracle_replay(tmp_path, c4served, ri
     same_rows(t, ref, "request %d vs oracle" % rid, C4_SPEC, ("FeH_comp",))
     assert (t.arr("Am")[0] > 0).mean() > 0.9        # the dust is there
     cat.close()
+
+
+def test_request_with_a_narrowed_model_matches_oracle_replay(tmp_path_factory):
+    """A model whose LF covers less than the job's absolute-magnitude range
+    is binned over its own range in a request too."""
+    import os
+    d = str(tmp_path_factory.mktemp("c0n"))
+    cmd = configs.write_c0(d)
+    M, phi = configs.seeded_lf()
+    keep = (M >= 5.0 - 1e-9) & (M <= 12.0 + 1e-9)
+    np.savetxt(os.path.join(d, "lf_narrow.txt"), np.c_[M[keep], phi[keep]], fmt="%.10g")
+    path = os.path.join(d, "model.thin.conf")
+    text = open(path).read()
+    open(path, "w").write(text.replace("lumfunc = lf.txt", "lumfunc = lf_narrow.txt"))
+    req = (301, 90.0, 60.0, 2.0, 21.0, 41, 0.0, 0.0)
+    got = serve_status(cmd, [req], plain=True)
+    res, t = got[301]
+    assert res.status == 0 and res.generated > 0
+    configs.set_footprints(cmd, [configs.foot_body(90.0, 60.0, 2.0)])
+    cat = Catalog(cmd, seed=41)
+    cat.count()
+    view = R.host_view(G.lib(), cat._h)
+    assert R.model_sp(view, 0).nM < view.sp.nM
+    ref, expected = R.replay_request(view)
+    assert abs(res.expected - expected) <= 1e-12 * expected
+    assert res.generated == ref.n
+    same_rows(t, ref, "narrowed request vs oracle")
+    cat.close()

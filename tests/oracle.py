@@ -8,7 +8,7 @@ import numpy as np
 from galfast_amd import _abi as A
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "libgforacle.so")
+ORACLE_SO = os.environ.get("GFO_LIB") or os.path.join(ROOT, "oracle", "_build", "libgforacle.so")
 
 _o = None
 

@@ -63,6 +63,7 @@ def _declare(L):
                                                    C.c_int, gf_col, gf_col, gf_col, gf_col, gf_col, vp]
     L.gf_photometric_errors_kernel.argtypes = [P(gf_rows), P(gf_photoerr_data), P(gf_rng), gf_col, gf_col,
                                                gf_col, vp]
+    L.gf_photoerr_records.argtypes = [C.c_int32, vp, vp, vp, vp, vp, vp]
     L.gf_module_chain.argtypes = [P(gf_rows), P(gf_chain), P(gf_rng), P(gf_table), vp]
     L.gf_skygen_count.argtypes = [P(gf_skygen_params), P(gf_model), vp, P(gf_extinction), C.c_int,
                                   P(gf_count_result), vp, vp, vp]

@@ -72,7 +72,8 @@ class gf_multiples_data(C.Structure):
 
 
 class gf_photoerr_data(C.Structure):
-    _fields_ = [("nbands", i32), ("koff", i32 * 18), ("kx", vp), ("ky", vp)]
+    _fields_ = [("nbands", i32), ("koff", i32 * 18), ("kx", vp), ("ky", vp), ("kq", vp),
+                ("gx0", C.c_double * 18), ("gscale", C.c_double * 18)]
 
 
 class gf_chain(C.Structure):

@@ -112,11 +112,15 @@ int chain_launch(const gf_chain_launch &L, const gf_rng *rng, int launch, hipStr
 		k_chain_rows<gfc::ROWS_BOND, 1><<<grows, gfc::BS, 0, st>>>(L, rng->state, 1);
 		break;
 	case 1 + gfc::ROWS_PHOT:
-		switch (gfc::phot_tables(L.m)) {
+		switch (gfc::phot_variant(L.m, L.t)) {
 		case 1: k_chain_rows<gfc::ROWS_PHOT, 1><<<grows, gfc::BS, 0, st>>>(L, rng->state, 0); break;
 		case 2: k_chain_rows<gfc::ROWS_PHOT, 2><<<grows, gfc::BS, 0, st>>>(L, rng->state, 0); break;
 		case 3: k_chain_rows<gfc::ROWS_PHOT, 3><<<grows, gfc::BS, 0, st>>>(L, rng->state, 0); break;
-		default: k_chain_rows<gfc::ROWS_PHOT, 4><<<grows, gfc::BS, 0, st>>>(L, rng->state, 0); break;
+		case 4: k_chain_rows<gfc::ROWS_PHOT, 4><<<grows, gfc::BS, 0, st>>>(L, rng->state, 0); break;
+		case 1 | gfc::PHOT_LONE: k_chain_rows<gfc::ROWS_PHOT, 1 | gfc::PHOT_LONE><<<grows, gfc::BS, 0, st>>>(L, rng->state, 0); break;
+		case 2 | gfc::PHOT_LONE: k_chain_rows<gfc::ROWS_PHOT, 2 | gfc::PHOT_LONE><<<grows, gfc::BS, 0, st>>>(L, rng->state, 0); break;
+		case 3 | gfc::PHOT_LONE: k_chain_rows<gfc::ROWS_PHOT, 3 | gfc::PHOT_LONE><<<grows, gfc::BS, 0, st>>>(L, rng->state, 0); break;
+		default: k_chain_rows<gfc::ROWS_PHOT, 4 | gfc::PHOT_LONE><<<grows, gfc::BS, 0, st>>>(L, rng->state, 0); break;
 		}
 		break;
 	case 1 + gfc::ROWS_PERR:

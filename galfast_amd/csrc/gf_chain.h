@@ -28,6 +28,7 @@ constexpr int BS = 256;      // lanes per workgroup of both passes
 constexpr int TILE = 16;     // rows per tile of the draw pass
 constexpr int MAXC = 256;    // rows of a chunk kept in LDS
 constexpr uint8_t HIDDEN = 0xFF, NOCOMP = 0xFE;
+constexpr int PHOT_LONE = 8;   // photometry launch variant flag (rows_phot)
 
 // LDS of one draw-pass workgroup: a tile of two 32-bit values per lane, and
 // the (hidden, comp) code of every row of each lane's chunk.
@@ -206,12 +207,13 @@ GF_D void rows_bond(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t p
 	}
 }
 
+// NT: the colour tables (1..4) plus PHOT_LONE when every row is a lone star
 template <int NT>
 GF_D void rows_phot(const gf_chain_mods &m, uint32_t row, const gf_table &T)
 {
 	const gf_col &Mr = m.has_mult ? T.absmagSys : T.absmag;
-	mod_photometry<NT>(m.phot, m.phot_comps, row, row + 1, T.Am, T.photoFlags, T.DM, Mr, (int)Mr.width, T.mags,
-	                   T.FeH, T.comp, T.hidden);
+	mod_photometry<(NT & 7), (NT & PHOT_LONE) != 0>(m.phot, m.phot_comps, row, row + 1, T.Am, T.photoFlags, T.DM,
+	                                                  Mr, (int)Mr.width, T.mags, T.FeH, T.comp, T.hidden);
 }
 
 GF_D void rows_perr(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t prow, const gf_table &T,
@@ -244,6 +246,15 @@ GF_D void chain_rows(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t 
 inline int phot_tables(const gf_chain_mods &m)
 {
 	return m.has_phot ? ((int)m.phot.ncolors + 3) / 4 : 1;
+}
+
+// The photometry launch variant: phot_tables(), plus PHOT_LONE when the
+// absolute-magnitude input has one column (no unresolved multiples), so that
+// no row is a system of several stars.
+inline int phot_variant(const gf_chain_mods &m, const gf_table &T)
+{
+	const gf_col &Mr = m.has_mult ? T.absmagSys : T.absmag;
+	return phot_tables(m) | (Mr.width == 1 ? PHOT_LONE : 0);
 }
 
 } // namespace gfc

@@ -379,8 +379,10 @@ GF_D uint32_t sample_colors(const gf_photometry_data &lt, float *c, float FeH, f
 }
 
 // os_photometry_kernel body (modules/photometry_gpu.cu.h:100-173), for at
-// most 4*NT colours (the caller checks ncolors <= 4*NT)
-template <int NT = 4>
+// most 4*NT colours (the caller checks ncolors <= 4*NT).  LONE: every row is
+// a lone star (one absolute-magnitude column), so the luminosity sum over a
+// system's components is compiled out.
+template <int NT = 4, bool LONE = false>
 GF_D void mod_photometry(const gf_photometry_data &lt, gf_bitmap apply, uint32_t r0, uint32_t r1,
                          const gf_col &Am, const gf_col &flags, const gf_col &DM, const gf_col &Mr,
                          int nabsmag, const gf_col &mags, const gf_col &FeH, const gf_col &comp,
@@ -397,7 +399,8 @@ GF_D void mod_photometry(const gf_photometry_data &lt, gf_bitmap apply, uint32_t
 		const float dm = col<float>(DM, row), Am0 = col<float>(Am, row);
 		uint32_t flag = 0;
 		const float fMr0 = col<float>(Mr, row, 0);
-		const bool system = fMr0 < ABSMAG_NOT_PRESENT && nabsmag > 1 && col<float>(Mr, row, 1) < ABSMAG_NOT_PRESENT;
+		const bool system =
+		    !LONE && fMr0 < ABSMAG_NOT_PRESENT && nabsmag > 1 && col<float>(Mr, row, 1) < ABSMAG_NOT_PRESENT;
 		if (!system) {
 			// a lone star: its magnitudes are the colour sums themselves
 			// (-2.5 log10 10^(-0.4 M) = M, without the round trip)
@@ -524,12 +527,74 @@ GF_D double spline_eval(const double *kx, const double *ky, int n, double x)
 	return ky[lo] + D * (x - kx[lo]);
 }
 
+// spline_eval over the interval records q[0 .. n-1] of one band (the layout
+// of gf_photoerr_data::kq) when the guessed interval gi was wrong: the
+// interval by binary search, with the same arithmetic.
+GF_D double spline_eval_rec_search(const double4 *q, int n, double x)
+{
+	int lo = 0, hi = n - 1;
+	while (hi > lo + 1) {
+		int mid = (lo + hi) >> 1;
+		if (q[mid].x > x) hi = mid; else lo = mid;
+	}
+	const double4 r = q[lo];
+	return r.y + r.z * (x - r.x);
+}
+
+// The guessed interval of x among n knots from the band's first knot x0
+// and scale (n-1)/(x_last-x0), clamped to [0, n-2].
+GF_D int spline_guess(int n, double x, double x0, double scale)
+{
+	const double g = (x - x0) * scale;
+	return g < 0. ? 0 : (g > (double)(n - 2) ? n - 2 : (int)g);
+}
+
+// Whether record r of interval gi (of n knots) holds x: spline_eval's interval.
+GF_D bool spline_holds(const double4 &r, int gi, int n, double x)
+{
+	return (gi == 0 || r.x <= x) && (gi == n - 2 || r.w > x);
+}
+
 // Photometric errors (modules/photometricErrors_host.cpp:112-131) drawn
 // from the chunk's MWC stream, rows in order and the bands of a row in
 // order.  One visible row: a deviate for each band that has an error curve.
 GF_D void perr_row(const gf_photoerr_data &par, gf_mwc &rng, uint32_t row, const gf_col &magTrue,
                    const gf_col &magObs)
 {
+	constexpr int NB = 6;   // bands whose error lookups are all issued before the draws
+	if (par.kq && par.nbands <= NB) {
+		// the magnitudes, then every band's guessed interval record (one
+		// 32-byte fetch each, all in flight together), then the values; a
+		// wrong guess (irregular knots) falls back to the binary search
+		const double4 *q = (const double4 *)par.kq;
+		float mag[NB], sigma[NB];
+		double4 r[NB];
+		int gi[NB];
+#pragma unroll
+		for (int b = 0; b < NB; b++)
+			if (b < par.nbands) mag[b] = col<float>(magTrue, row, b);
+#pragma unroll
+		for (int b = 0; b < NB; b++) {
+			const int n = b < par.nbands ? par.koff[b + 1] - par.koff[b] : 0;
+			gi[b] = n > 1 ? spline_guess(n, (double)mag[b], par.gx0[b], par.gscale[b]) : 0;
+			if (n > 0) r[b] = q[par.koff[b] + gi[b]];
+		}
+#pragma unroll
+		for (int b = 0; b < NB; b++) {
+			const int n = b < par.nbands ? par.koff[b + 1] - par.koff[b] : 0;
+			const double x = (double)mag[b];
+			double v = n == 1 ? r[b].y : r[b].y + r[b].z * (x - r[b].x);
+			if (n > 1 && !spline_holds(r[b], gi[b], n, x)) v = spline_eval_rec_search(q + par.koff[b], n, x);
+			sigma[b] = (float)v;
+		}
+#pragma unroll
+		for (int b = 0; b < NB; b++) {
+			if (b >= par.nbands) break;
+			const bool curve = par.koff[b + 1] > par.koff[b];
+			col<float>(magObs, row, b) = curve ? mag[b] + rng.gaussian(sigma[b]) : mag[b];
+		}
+		return;
+	}
 	for (int b = 0; b < par.nbands; b++) {
 		int k0 = par.koff[b], k1 = par.koff[b + 1];
 		float mag = col<float>(magTrue, row, b);

@@ -152,7 +152,7 @@ struct gf_catalog {
 	std::vector<std::string> band_names;
 	std::string bandset;   // the photometric system's name (photometry_host.cpp:94)
 	gf_photoerr_data perr{};
-	DevBuf perr_dev[2];
+	DevBuf perr_dev[3];
 	gf_vel2pm_data v2pm{};
 
 	// extinction (os_skygen::load_extinction_maps): north/south maps and the
@@ -927,6 +927,13 @@ void construct_photoerr(gf_catalog &c, const Config &cfg)
 	c.perr_dev[1].upload(ky);
 	c.perr.kx = (const double *)c.perr_dev[0].p;
 	c.perr.ky = (const double *)c.perr_dev[1].p;
+	// interval records: one fetch and no division per band and star
+	std::vector<double> kq(4 * kx.size());
+	if (gf_photoerr_records(c.perr.nbands, c.perr.koff, kx.data(), ky.data(), kq.data(), c.perr.gx0,
+	                        c.perr.gscale))
+		fail("photometricErrors: bad error-curve knots");
+	c.perr_dev[2].upload(kq);
+	c.perr.kq = kq.empty() ? nullptr : (const double *)c.perr_dev[2].p;
 	c.has_perr = true;
 }
 
@@ -1081,6 +1088,7 @@ void open_catalog(gf_catalog &c, const std::string &cmdfile)
 		c.perr_h = c.perr;
 		c.perr_h.kx = c.perr_kx.data();
 		c.perr_h.ky = c.perr_ky.data();
+		c.perr_h.kq = nullptr;   // the host view holds the knots only
 	}
 }
 

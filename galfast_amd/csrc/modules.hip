@@ -245,4 +245,26 @@ int gf_photometric_errors_kernel(const gf_rows *ks, const gf_photoerr_data *par,
 	return 0;
 }
 
+int gf_photoerr_records(int32_t nbands, const int32_t *koff, const double *kx, const double *ky, double *kq,
+                        double *gx0, double *gscale)
+{
+	GF_CHECK(nbands >= 0 && nbands <= 17 && koff && gx0 && gscale, GF_E_ARG);
+	GF_CHECK(koff[0] == 0 && (koff[nbands] == 0 || (kx && ky && kq)), GF_E_ARG);
+	for (int b = 0; b < nbands; b++) {
+		const int k0 = koff[b], k1 = koff[b + 1];
+		GF_CHECK(k1 >= k0, GF_E_ARG);
+		gx0[b] = k1 > k0 ? kx[k0] : 0.0;
+		gscale[b] = k1 - k0 > 1 ? (double)(k1 - k0 - 1) / (kx[k1 - 1] - kx[k0]) : 0.0;
+		for (int j = k0; j < k1; j++) {
+			double *r = kq + 4 * (size_t)j;
+			const bool last = j + 1 == k1;
+			r[0] = kx[j];
+			r[1] = ky[j];
+			r[2] = last ? 0.0 : (ky[j + 1] - ky[j]) / (kx[j + 1] - kx[j]);
+			r[3] = last ? kx[j] : kx[j + 1];
+		}
+	}
+	return 0;
+}
+
 } // extern "C"

@@ -219,7 +219,24 @@ typedef struct gf_photoerr_data {
     int32_t koff[18];
     const double *kx;   /* device knots: magnitude */
     const double *ky;   /* device knots: sigma */
+    /* Optional (NULL: evaluate from kx, ky): device records of 4 doubles, one
+     * per knot j of band b (koff[b] <= j < koff[b+1]): {kx[j], ky[j],
+     * (ky[j+1]-ky[j]) / (kx[j+1]-kx[j]), kx[j+1]}, the band's last knot
+     * {kx[j], ky[j], 0, kx[j]}; and per band the first knot gx0[b] and
+     * gscale[b] = (n-1) / (kx[last]-kx[first]) (n knots, 0 when n = 1), from
+     * which the interval of a magnitude is guessed.  The slopes are the
+     * quotients the evaluation would form, so the results are the same;
+     * gf_photoerr_records fills all three. */
+    const double *kq;
+    double gx0[18];
+    double gscale[18];
 } gf_photoerr_data;
+
+/* Fill the interval records `kq` (4 * koff[nbands] doubles, host memory),
+ * gx0 and gscale (nbands doubles each) of gf_photoerr_data from host copies
+ * of its knots. */
+int gf_photoerr_records(int32_t nbands, const int32_t *koff, const double *kx, const double *ky, double *kq,
+                        double *gx0, double *gscale);
 
 /* os_FeH_kernel (modules/FeH_gpu.cu.h:43-87; called at FeH_host.cpp:76). */
 int gf_os_FeH_kernel(const gf_rows *ks, const gf_FeH_data *par, gf_rng *rng,

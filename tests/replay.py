@@ -222,3 +222,17 @@ def replay_request(view):
         rc = o.gfo_module_chain(C.byref(ks), C.byref(view.chain), O.ptr(state), S, C.byref(t.table()))
         assert rc == 0
     return t, expected
+
+
+def unwritten_columns(view):
+    """The columns no module of the job writes: they hold whatever their
+    memory held, on the device and in the oracle's buffers alike."""
+    ch = view.chain
+    unwritten = {"FeH_comp"} if not ch.gfeh else set()
+    for ptr, cols in ((ch.mult, ("absmagSys", "absmagSysNcomp")), (ch.bond, ("vcyl",)), (ch.v2pm, ("pm",)),
+                      (ch.phot, ("mags", "photoFlags")), (ch.perr, ("obsMags",)), (ch.feh or ch.gfeh, ("FeH",))):
+        if not ptr:
+            unwritten |= set(cols)
+    if not ch.gal2other_coordsys:
+        unwritten.add("radec")
+    return unwritten

@@ -190,6 +190,66 @@ def test_photometric_errors(fx):
     fx.compare(["obsMags"])
 
 
+def _irregular_knots(nbands, rng):
+    """Error curves whose knots the interval guess often misses: irregular
+    spacing, one band of a single knot, one of two, one without a curve."""
+    kx, ky, koff = [], [], [0]
+    for b in range(nbands):
+        if b == 2:
+            m = np.array([20.0])
+        elif b == 3:
+            m = np.array([12.0, 27.0])
+        elif b == nbands - 1:
+            m = np.zeros(0)
+        else:
+            m = np.sort(rng.uniform(10.0, 30.0, 300 + 100 * b))
+            m[0], m[-1] = 10.0, 30.0
+        kx.append(m)
+        ky.append(0.02 + np.exp(0.8 * (m - 24.0)))
+        koff.append(koff[-1] + len(m))
+    return np.concatenate(kx), np.concatenate(ky), koff
+
+
+@pytest.mark.parametrize("irregular", [False, True])
+def test_photometric_errors_interval_records(fx, irregular):
+    """The kernel with gf_photoerr_records' interval records gives the bits
+    of the evaluation from the knots (the oracle's)."""
+    import torch
+    rng = np.random.default_rng(7)
+    mags = rng.uniform(8, 32, fx.dev.buf["mags"][0].numel()).astype(np.float32)
+    fx.dev.buf["mags"][0].copy_(torch.from_numpy(mags))
+    fx.host.buf["mags"][0][:] = mags
+    nb = fx.perr(False).nbands
+    if irregular:
+        kx, ky, koff = _irregular_knots(nb, rng)
+    else:
+        kx, ky, koff = gfx.photoerr_knots(nb)
+    koffa = np.array(koff, dtype=np.int32)
+    kq, gx0, gs = np.zeros(4 * len(kx)), np.zeros(18), np.zeros(18)
+    G.check(G.lib().gf_photoerr_records(nb, koffa.ctypes.data, kx.ctypes.data, ky.ctypes.data, kq.ctypes.data,
+                                        gx0.ctypes.data, gs.ctypes.data), "gf_photoerr_records")
+    dx, dy, dq = dev_array(kx), dev_array(ky), dev_array(kq)
+
+    def par(ptrs):
+        p = A.gf_photoerr_data()
+        p.nbands = nb
+        for i, k in enumerate(koff):
+            p.koff[i] = k
+        p.kx, p.ky = ptrs[0], ptrs[1]
+        if len(ptrs) > 2:
+            p.kq = ptrs[2]
+            for b in range(nb):
+                p.gx0[b], p.gscale[b] = gx0[b], gs[b]
+        return p
+    pd = par([dx.data_ptr(), dy.data_ptr(), dq.data_ptr()])
+    ph = par([kx.ctypes.data, ky.ctypes.data])
+    G.check(G.lib().gf_photometric_errors_kernel(C.byref(fx.rows()), C.byref(pd), C.byref(fx.drng.rng),
+                                                 fx.dev.col("mags"), fx.dev.col("obsMags"), fx.dev.col("hidden"),
+                                                 stream_ptr()))
+    fx.oracle_chain(perr=ph)
+    fx.compare(["obsMags"])
+
+
 def test_stream_limit_is_enforced(fx):
     p = gfx.feh_params()
     rows = A.gf_rows(0, 256, 65536 * 256 + 1, 0)

@@ -18,6 +18,7 @@ import galfast_amd as G
 from galfast_amd import configs
 from galfast_amd.catalog import Catalog
 from galfast_amd.table import catalog_spec
+import replay as R
 
 pytestmark = pytest.mark.gpu
 
@@ -60,6 +61,8 @@ def run(cmd, overlap):
     cat.count()
     info = cat.info()
     spec = catalog_spec(nbands=info.nbands, nabsmag=max(1, info.nabsmag))
+    for c in R.unwritten_columns(R.host_view(G.lib(), cat._h)):
+        spec.pop(c, None)
     chunks = []
     st = cat.run(lambda t, n, comp: chunks.append((n, comp, sample(t, n, spec))))
     cat.close()

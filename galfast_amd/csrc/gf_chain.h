@@ -135,9 +135,8 @@ GF_D void chunk_draws(const gf_chain_mods &m, gf_mwc &rng, uint32_t r0, uint32_t
 	}
 }
 
-GF_D gf_mwc restart(uint32_t a, const gf_positions &P, int plane, uint64_t prow)
+GF_D gf_mwc restart_at(uint32_t a, uint2 q)
 {
-	uint2 q = P.pos[plane * P.pitch + prow];
 	gf_mwc r;
 	r.a = a;
 	r.c = q.x;
@@ -145,12 +144,16 @@ GF_D gf_mwc restart(uint32_t a, const gf_positions &P, int plane, uint64_t prow)
 	return r;
 }
 
+GF_D uint2 position(const gf_positions &P, int plane, uint64_t prow) { return P.pos[plane * P.pitch + prow]; }
+
 // The row pass, in five groups so that each launch stays small enough in
 // registers to keep several waves per SIMD, and so that independent groups
 // can run side by side: FEH (FeH, GaussianFeH), BOND (Bond2010), PHOT
 // (photometry), PERR (photometric errors), ASTRO (vel2pm, gal2other).  a is
 // the multiplier of the row's stream and prow its index in the position
-// planes.
+// planes.  Each group reads all of a row's inputs before it tests the row's
+// hidden flag, so that the loads travel together (the columns exist for
+// every row; a hidden row's values are read and left unused).
 enum {
 	ROWS_FEH = GF_ROWS_FEH,
 	ROWS_BOND = GF_ROWS_BOND,
@@ -178,16 +181,20 @@ inline bool rows_stream(const gf_chain_mods &m, int group)
 GF_D void rows_feh(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t prow, const gf_table &T,
                    const gf_positions &P)
 {
-	if (col<int>(T.hidden, row)) return;
+	const int hidden = col<int>(T.hidden, row);
 	const int cmp = col<int>(T.comp, row);
 	if (m.has_feh) {
 		const float x = col<float>(T.XYZ, row, 0), y = col<float>(T.XYZ, row, 1), z = col<float>(T.XYZ, row, 2);
-		gf_mwc r = restart(a, P, m.pl_feh, prow);
+		const uint2 q = position(P, m.pl_feh, prow);
+		if (hidden) return;
+		gf_mwc r = restart_at(a, q);
 		float feh;
 		if (feh_row(m.feh, r, cmp, x, y, z, feh)) col<float>(T.FeH, row) = feh;
 	}
 	if (m.has_gfeh) {
-		gf_mwc r = restart(a, P, m.pl_gfeh, prow);
+		const uint2 q = position(P, m.pl_gfeh, prow);
+		if (hidden) return;
+		gf_mwc r = restart_at(a, q);
 		gfeh_row(m.gfeh, m.gfeh_comps, r, cmp, row, T.FeH_comp, T.FeH);
 	}
 }
@@ -195,10 +202,12 @@ GF_D void rows_feh(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t pr
 GF_D void rows_bond(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t prow, const gf_table &T,
                     const gf_positions &P)
 {
-	if (col<int>(T.hidden, row)) return;
+	const int hidden = col<int>(T.hidden, row);
 	const int cmp = col<int>(T.comp, row);
 	const float x = col<float>(T.XYZ, row, 0), y = col<float>(T.XYZ, row, 1), z = col<float>(T.XYZ, row, 2);
-	gf_mwc r = restart(a, P, m.pl_bond, prow);
+	const uint2 q = position(P, m.pl_bond, prow);
+	if (hidden) return;
+	gf_mwc r = restart_at(a, q);
 	float v[3];
 	if (bond_row(m.bond, r, cmp, x, y, z, v)) {
 		col<float>(T.vcyl, row, 0) = v[0];
@@ -211,23 +220,50 @@ GF_D void rows_bond(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t p
 template <int NT>
 GF_D void rows_phot(const gf_chain_mods &m, uint32_t row, const gf_table &T)
 {
+	constexpr bool LONE = (NT & PHOT_LONE) != 0;
 	const gf_col &Mr = m.has_mult ? T.absmagSys : T.absmag;
-	mod_photometry<(NT & 7), (NT & PHOT_LONE) != 0>(m.phot, m.phot_comps, row, row + 1, T.Am, T.photoFlags, T.DM,
-	                                                  Mr, (int)Mr.width, T.mags, T.FeH, T.comp, T.hidden);
+	const int nabsmag = (int)Mr.width;
+	const int hidden = col<int>(T.hidden, row);
+	const int cmp = col<int>(T.comp, row);
+	const float feh = col<float>(T.FeH, row), dm = col<float>(T.DM, row), am = col<float>(T.Am, row);
+	const float mr0 = col<float>(Mr, row, 0);
+	const float mr1 = !LONE && nabsmag > 1 ? col<float>(Mr, row, 1) : ABSMAG_NOT_PRESENT;
+	if (hidden || !bit_isset(m.phot_comps, cmp)) return;
+	phot_row<(NT & 7), LONE>(m.phot, row, feh, dm, am, mr0, mr1, T.photoFlags, Mr, nabsmag, T.mags);
 }
 
 GF_D void rows_perr(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t prow, const gf_table &T,
                     const gf_positions &P)
 {
-	if (col<int>(T.hidden, row)) return;
-	gf_mwc r = restart(a, P, m.pl_perr, prow);
+	const int hidden = col<int>(T.hidden, row);
+	const uint2 q = position(P, m.pl_perr, prow);
+	if (perr_fast(m.perr)) {
+		float mag[PERR_FAST_BANDS];
+#pragma unroll
+		for (int b = 0; b < PERR_FAST_BANDS; b++)
+			if (b < m.perr.nbands) mag[b] = col<float>(T.mags, row, b);
+		if (hidden) return;
+		gf_mwc r = restart_at(a, q);
+		perr_row_fast(m.perr, r, row, mag, T.obsMags);
+		return;
+	}
+	if (hidden) return;
+	gf_mwc r = restart_at(a, q);
 	perr_row(m.perr, r, row, T.mags, T.obsMags);
 }
 
 GF_D void rows_astro(const gf_chain_mods &m, uint32_t row, const gf_table &T)
 {
-	if (m.has_v2pm) mod_vel2pm(m.v2pm, row, row + 1, T.lb, T.XYZ, T.vcyl, T.pm, T.hidden);
-	if (m.gal2other) mod_gal2other(m.gal2other, row, row + 1, T.hidden, T.lb, T.radec);
+	const int hidden = col<int>(T.hidden, row);
+	const double l = col<double>(T.lb, row, 0), b = col<double>(T.lb, row, 1);
+	if (m.has_v2pm) {
+		const float x = col<float>(T.XYZ, row, 0), y = col<float>(T.XYZ, row, 1), z = col<float>(T.XYZ, row, 2);
+		const float vr = col<float>(T.vcyl, row, 0), vp = col<float>(T.vcyl, row, 1), vz = col<float>(T.vcyl, row, 2);
+		if (hidden) return;
+		vel2pm_row(m.v2pm, row, l, b, x, y, z, vr, vp, vz, T.pm);
+	}
+	if (hidden) return;
+	if (m.gal2other) gal2other_row(m.gal2other, row, l, b, T.radec);
 }
 
 // GROUP: one of ROWS_*; NT: colour tables of the photometry (1..4, at

@@ -232,21 +232,18 @@ GF_D void transform_pm(float &vlout, float &vbout, float l, float b, float vl, f
 	}
 }
 
-// os_vel2pm_kernel body (modules/vel2pm_gpu.cu.h:268-300) with vcyl2pm
-// (:221-251), vel_cyl2xyz (:112-121) and vel_xyz2lbr (:81-94).
-GF_D void mod_vel2pm(const gf_vel2pm_data &par, uint32_t r0, uint32_t r1, const gf_col &lb0,
-                     const gf_col &XYZ, const gf_col &vcyl, const gf_col &pmout, const gf_col &hidden)
+// One row of os_vel2pm_kernel (modules/vel2pm_gpu.cu.h:268-300) with
+// vcyl2pm (:221-251), vel_cyl2xyz (:112-121) and vel_xyz2lbr (:81-94), from
+// the row's (l, b) in degrees, XYZ and cylindrical velocity.
+GF_D void vel2pm_row(const gf_vel2pm_data &par, uint32_t row, double l_deg, double b_deg, float x, float y, float z,
+                     float vr, float vphi, float vz0, const gf_col &pmout)
 {
-	for (uint32_t row = r0; row < r1; row++) {
-		if (col<int>(hidden, row)) continue;
-		float l = 0.017453292519943295769f * (float)col<double>(lb0, row, 0);
-		float b = 0.017453292519943295769f * (float)col<double>(lb0, row, 1);
-		float X = par.Rg - col<float>(XYZ, row, 0);
-		float Y = -col<float>(XYZ, row, 1);
-		float Z = col<float>(XYZ, row, 2);
-		float vr = col<float>(vcyl, row, 0);
-		float vphi = col<float>(vcyl, row, 1);
-		float vz0 = col<float>(vcyl, row, 2);
+	{
+		float l = 0.017453292519943295769f * (float)l_deg;
+		float b = 0.017453292519943295769f * (float)b_deg;
+		float X = par.Rg - x;
+		float Y = -y;
+		float Z = z;
 
 		float rho = gf_sqrtf(X * X + Y * Y);
 		float cphi = X / rho;
@@ -284,23 +281,42 @@ GF_D void mod_vel2pm(const gf_vel2pm_data &par, uint32_t r0, uint32_t r1, const 
 	}
 }
 
-// os_gal2other_kernel body with galequ (modules/gal2other_gpu.cu.h:39-95)
+// os_vel2pm_kernel body over rows [r0, r1).
+GF_D void mod_vel2pm(const gf_vel2pm_data &par, uint32_t r0, uint32_t r1, const gf_col &lb0,
+                     const gf_col &XYZ, const gf_col &vcyl, const gf_col &pmout, const gf_col &hidden)
+{
+	for (uint32_t row = r0; row < r1; row++) {
+		if (col<int>(hidden, row)) continue;
+		vel2pm_row(par, row, col<double>(lb0, row, 0), col<double>(lb0, row, 1), col<float>(XYZ, row, 0),
+		           col<float>(XYZ, row, 1), col<float>(XYZ, row, 2), col<float>(vcyl, row, 0),
+		           col<float>(vcyl, row, 1), col<float>(vcyl, row, 2), pmout);
+	}
+}
+
+// One row of os_gal2other_kernel with galequ (modules/gal2other_gpu.cu.h:39-95)
+// from the row's (l, b) in degrees.
+GF_D void gal2other_row(int coordsys, uint32_t row, double l_deg, double b_deg, const gf_col &out)
+{
+	using namespace galequ_c;
+	double lx = l_deg * d2r;
+	double ly = b_deg * d2r;
+	double rx, ry;
+	if (coordsys == 1) {
+		gal_to_equ(lx, ly, rx, ry);
+	} else {
+		rx = ry = -9999.;
+	}
+	col<double>(out, row, 0) = rx / d2r;
+	col<double>(out, row, 1) = ry / d2r;
+}
+
+// os_gal2other_kernel body over rows [r0, r1).
 GF_D void mod_gal2other(int coordsys, uint32_t r0, uint32_t r1, const gf_col &hidden,
                         const gf_col &lb0, const gf_col &out)
 {
-	using namespace galequ_c;
 	for (uint32_t row = r0; row < r1; row++) {
 		if (col<int>(hidden, row)) continue;
-		double lx = col<double>(lb0, row, 0) * d2r;
-		double ly = col<double>(lb0, row, 1) * d2r;
-		double rx, ry;
-		if (coordsys == 1) {
-			gal_to_equ(lx, ly, rx, ry);
-		} else {
-			rx = ry = -9999.;
-		}
-		col<double>(out, row, 0) = rx / d2r;
-		col<double>(out, row, 1) = ry / d2r;
+		gal2other_row(coordsys, row, col<double>(lb0, row, 0), col<double>(lb0, row, 1), out);
 	}
 }
 
@@ -378,29 +394,22 @@ GF_D uint32_t sample_colors(const gf_photometry_data &lt, float *c, float FeH, f
 	return flags;
 }
 
-// os_photometry_kernel body (modules/photometry_gpu.cu.h:100-173), for at
-// most 4*NT colours (the caller checks ncolors <= 4*NT).  LONE: every row is
-// a lone star (one absolute-magnitude column), so the luminosity sum over a
-// system's components is compiled out.
-template <int NT = 4, bool LONE = false>
-GF_D void mod_photometry(const gf_photometry_data &lt, gf_bitmap apply, uint32_t r0, uint32_t r1,
-                         const gf_col &Am, const gf_col &flags, const gf_col &DM, const gf_col &Mr,
-                         int nabsmag, const gf_col &mags, const gf_col &FeH, const gf_col &comp,
-                         const gf_col &hidden)
+// One row of os_photometry_kernel (modules/photometry_gpu.cu.h:100-173) for
+// a visible row of a selected component, from the row's [Fe/H], distance
+// modulus, extinction and first two absolute magnitudes (fMr1 is read only
+// when nabsmag > 1), for at most 4*NT colours (the caller checks ncolors <=
+// 4*NT).  LONE: every row is a lone star (one absolute-magnitude column), so
+// the luminosity sum over a system's components is compiled out.
+template <int NT, bool LONE>
+GF_D void phot_row(const gf_photometry_data &lt, uint32_t row, float fFeH, float dm, float Am0, float fMr0,
+                   float fMr1, const gf_col &flags, const gf_col &Mr, int nabsmag, const gf_col &mags)
 {
 	constexpr int NC = 4 * NT;
 	float c[NC];
 	const int nb = (int)lt.ncolors < NC ? (int)lt.ncolors : NC;
-	for (uint32_t row = r0; row < r1; row++) {
-		if (col<int>(hidden, row)) continue;
-		int cmp = col<int>(comp, row);
-		if (!bit_isset(apply, cmp)) continue;
-		float fFeH = col<float>(FeH, row);
-		const float dm = col<float>(DM, row), Am0 = col<float>(Am, row);
+	{
 		uint32_t flag = 0;
-		const float fMr0 = col<float>(Mr, row, 0);
-		const bool system =
-		    !LONE && fMr0 < ABSMAG_NOT_PRESENT && nabsmag > 1 && col<float>(Mr, row, 1) < ABSMAG_NOT_PRESENT;
+		const bool system = !LONE && fMr0 < ABSMAG_NOT_PRESENT && nabsmag > 1 && fMr1 < ABSMAG_NOT_PRESENT;
 		if (!system) {
 			// a lone star: its magnitudes are the colour sums themselves
 			// (-2.5 log10 10^(-0.4 M) = M, without the round trip)
@@ -451,6 +460,23 @@ GF_D void mod_photometry(const gf_photometry_data &lt, gf_bitmap apply, uint32_t
 		flag = __brev(flag);
 		flag >>= (32 - nb - 1);
 		col<int>(flags, row) = (int)flag;
+	}
+}
+
+// os_photometry_kernel body (modules/photometry_gpu.cu.h:100-173) over rows
+// [r0, r1), for at most 4*NT colours.
+template <int NT = 4, bool LONE = false>
+GF_D void mod_photometry(const gf_photometry_data &lt, gf_bitmap apply, uint32_t r0, uint32_t r1,
+                         const gf_col &Am, const gf_col &flags, const gf_col &DM, const gf_col &Mr,
+                         int nabsmag, const gf_col &mags, const gf_col &FeH, const gf_col &comp,
+                         const gf_col &hidden)
+{
+	for (uint32_t row = r0; row < r1; row++) {
+		if (col<int>(hidden, row)) continue;
+		if (!bit_isset(apply, col<int>(comp, row))) continue;
+		const float fMr1 = !LONE && nabsmag > 1 ? col<float>(Mr, row, 1) : ABSMAG_NOT_PRESENT;
+		phot_row<NT, LONE>(lt, row, col<float>(FeH, row), col<float>(DM, row), col<float>(Am, row),
+		                   col<float>(Mr, row, 0), fMr1, flags, Mr, nabsmag, mags);
 	}
 }
 
@@ -555,44 +581,59 @@ GF_D bool spline_holds(const double4 &r, int gi, int n, double x)
 	return (gi == 0 || r.x <= x) && (gi == n - 2 || r.w > x);
 }
 
+// Bands whose error lookups are all issued before the draws (interval
+// records present): perr_row_fast.
+constexpr int PERR_FAST_BANDS = 6;
+
+GF_D bool perr_fast(const gf_photoerr_data &par) { return par.kq && par.nbands <= PERR_FAST_BANDS; }
+
+// Photometric errors of one visible row from its true magnitudes mag[0 ..
+// nbands-1] (perr_fast(par)): every band's guessed interval record (one
+// 32-byte fetch each, all in flight together), then the values (a wrong
+// guess, from irregular knots, falls back to the binary search), then the
+// deviates in band order.
+GF_D void perr_row_fast(const gf_photoerr_data &par, gf_mwc &rng, uint32_t row, const float mag[PERR_FAST_BANDS],
+                        const gf_col &magObs)
+{
+	constexpr int NB = PERR_FAST_BANDS;
+	const double4 *q = (const double4 *)par.kq;
+	float sigma[NB];
+	double4 r[NB];
+	int gi[NB];
+#pragma unroll
+	for (int b = 0; b < NB; b++) {
+		const int n = b < par.nbands ? par.koff[b + 1] - par.koff[b] : 0;
+		gi[b] = n > 1 ? spline_guess(n, (double)mag[b], par.gx0[b], par.gscale[b]) : 0;
+		if (n > 0) r[b] = q[par.koff[b] + gi[b]];
+	}
+#pragma unroll
+	for (int b = 0; b < NB; b++) {
+		const int n = b < par.nbands ? par.koff[b + 1] - par.koff[b] : 0;
+		const double x = (double)mag[b];
+		double v = n == 1 ? r[b].y : r[b].y + r[b].z * (x - r[b].x);
+		if (n > 1 && !spline_holds(r[b], gi[b], n, x)) v = spline_eval_rec_search(q + par.koff[b], n, x);
+		sigma[b] = (float)v;
+	}
+#pragma unroll
+	for (int b = 0; b < NB; b++) {
+		if (b >= par.nbands) break;
+		const bool curve = par.koff[b + 1] > par.koff[b];
+		col<float>(magObs, row, b) = curve ? mag[b] + rng.gaussian(sigma[b]) : mag[b];
+	}
+}
+
 // Photometric errors (modules/photometricErrors_host.cpp:112-131) drawn
 // from the chunk's MWC stream, rows in order and the bands of a row in
 // order.  One visible row: a deviate for each band that has an error curve.
 GF_D void perr_row(const gf_photoerr_data &par, gf_mwc &rng, uint32_t row, const gf_col &magTrue,
                    const gf_col &magObs)
 {
-	constexpr int NB = 6;   // bands whose error lookups are all issued before the draws
-	if (par.kq && par.nbands <= NB) {
-		// the magnitudes, then every band's guessed interval record (one
-		// 32-byte fetch each, all in flight together), then the values; a
-		// wrong guess (irregular knots) falls back to the binary search
-		const double4 *q = (const double4 *)par.kq;
-		float mag[NB], sigma[NB];
-		double4 r[NB];
-		int gi[NB];
+	if (perr_fast(par)) {
+		float mag[PERR_FAST_BANDS];
 #pragma unroll
-		for (int b = 0; b < NB; b++)
+		for (int b = 0; b < PERR_FAST_BANDS; b++)
 			if (b < par.nbands) mag[b] = col<float>(magTrue, row, b);
-#pragma unroll
-		for (int b = 0; b < NB; b++) {
-			const int n = b < par.nbands ? par.koff[b + 1] - par.koff[b] : 0;
-			gi[b] = n > 1 ? spline_guess(n, (double)mag[b], par.gx0[b], par.gscale[b]) : 0;
-			if (n > 0) r[b] = q[par.koff[b] + gi[b]];
-		}
-#pragma unroll
-		for (int b = 0; b < NB; b++) {
-			const int n = b < par.nbands ? par.koff[b + 1] - par.koff[b] : 0;
-			const double x = (double)mag[b];
-			double v = n == 1 ? r[b].y : r[b].y + r[b].z * (x - r[b].x);
-			if (n > 1 && !spline_holds(r[b], gi[b], n, x)) v = spline_eval_rec_search(q + par.koff[b], n, x);
-			sigma[b] = (float)v;
-		}
-#pragma unroll
-		for (int b = 0; b < NB; b++) {
-			if (b >= par.nbands) break;
-			const bool curve = par.koff[b + 1] > par.koff[b];
-			col<float>(magObs, row, b) = curve ? mag[b] + rng.gaussian(sigma[b]) : mag[b];
-		}
+		perr_row_fast(par, rng, row, mag, magObs);
 		return;
 	}
 	for (int b = 0; b < par.nbands; b++) {

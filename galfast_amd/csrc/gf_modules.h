@@ -354,24 +354,26 @@ template <int NT = 4>
 GF_D uint32_t sample_colors(const gf_photometry_data &lt, float *c, float FeH, float Mr)
 {
 	uint32_t flags = 0, cidx = 0;
-	// texel coordinates, shared by every table (interleaved layout)
+	// texel coordinates, shared by every table
 	int i, j;
 	const float a = lerp_coord(FeH, lt.tcFeH[0], lt.tcFeH[1], lt.nFeH, i);
 	const float b = lerp_coord(Mr, lt.tcMr[0], lt.tcMr[1], lt.nMr, j);
-	const int i1 = lt.nFeH > 1 ? i + 1 : i, j1 = lt.nMr > 1 ? j + 1 : j;
 	const int ntex = ((int)lt.ncolors + 3) / 4;
+	// the lookup quad's record (quad layout): the four corners' float4s per
+	// table and their flag bits, one contiguous fetch each
 	const float4 *tx = lt.flagbits ? (const float4 *)lt.texels : nullptr;
-	const size_t t00 = (size_t)j * lt.nFeH + i, t10 = (size_t)j * lt.nFeH + i1;
-	const size_t t01 = (size_t)j1 * lt.nFeH + i, t11 = (size_t)j1 * lt.nFeH + i1;
+	const size_t k = (size_t)j * lt.nFeH + i;
 	uint32_t b00 = 0, b10 = 0, b01 = 0, b11 = 0;
 	if (tx) {
-		b00 = lt.flagbits[t00]; b10 = lt.flagbits[t10];
-		b01 = lt.flagbits[t01]; b11 = lt.flagbits[t11];
+		const uint2 fw = *(const uint2 *)(lt.flagbits + 4 * k);
+		b00 = fw.x & 0xffffu; b10 = fw.x >> 16;
+		b01 = fw.y & 0xffffu; b11 = fw.y >> 16;
 	}
 	for (int t = 0; t < NT && cidx < lt.ncolors; t++) {
 		float4 clr, f;
 		if (tx) {
-			clr = bilerp4(tx[t00 * ntex + t], tx[t10 * ntex + t], tx[t01 * ntex + t], tx[t11 * ntex + t], a, b);
+			const float4 *qd = tx + 4 * ((size_t)ntex * k + t);
+			clr = bilerp4(qd[0], qd[1], qd[2], qd[3], a, b);
 			// the flag texels, 0 or 1, filtered as the colours are
 			auto unpack = [t](uint32_t bits) {
 				bits >>= 4 * t;

@@ -809,21 +809,35 @@ void construct_photometry(gf_catalog &c, const Config &cfg)
 		p.color[t] = (const float *)c.phot_dev[t].p;
 		p.cflags[t] = (const float *)c.phot_dev[4 + t].p;
 	}
-	// the interleaved copy: per texel the ntex colour float4s; the flags (0
-	// or 1) as one bit per colour
+	// the copy by lookup quad: per lower corner (i, j) the four corner
+	// texels' ntex colour float4s, and each corner's flags (0 or 1) as one
+	// bit per colour
 	{
-		const size_t ntexel = (size_t)pt.nFeH * pt.nMr, nt = (size_t)pt.ntex;
-		std::vector<float> il(ntexel * 4 * nt);
-		std::vector<uint16_t> fb(ntexel, 0);
+		const int nF = pt.nFeH, nM = pt.nMr;
+		const size_t ntexel = (size_t)nF * nM, nt = (size_t)pt.ntex;
+		std::vector<uint16_t> bits(ntexel, 0);
 		for (size_t k = 0; k < ntexel; k++)
 			for (size_t t = 0; t < nt; t++)
 				for (int q = 0; q < 4; q++) {
-					il[(k * nt + t) * 4 + q] = pt.color[t * per + k * 4 + q];
 					const float f = pt.flags[t * per + k * 4 + q];
 					if (f != 0.f && f != 1.f) fail("photometry flags must be 0 or 1");
-					if (f != 0.f) fb[k] |= (uint16_t)(1u << (4 * t + q));
+					if (f != 0.f) bits[k] |= (uint16_t)(1u << (4 * t + q));
 				}
-		c.phot_il.upload(il);
+		std::vector<float> qd(ntexel * nt * 16);
+		std::vector<uint16_t> fb(ntexel * 4);
+		for (int j = 0; j < nM; j++)
+			for (int i = 0; i < nF; i++) {
+				const size_t k = (size_t)j * nF + i;
+				const int i1 = std::min(i + 1, nF - 1), j1 = std::min(j + 1, nM - 1);
+				const size_t corner[4] = {k, (size_t)j * nF + i1, (size_t)j1 * nF + i, (size_t)j1 * nF + i1};
+				for (int cn = 0; cn < 4; cn++) {
+					fb[4 * k + cn] = bits[corner[cn]];
+					for (size_t t = 0; t < nt; t++)
+						for (int q = 0; q < 4; q++)
+							qd[((k * nt + t) * 4 + cn) * 4 + q] = pt.color[t * per + corner[cn] * 4 + q];
+				}
+			}
+		c.phot_il.upload(qd);
 		c.phot_fb.upload(fb);
 		p.texels = (const float *)c.phot_il.p;
 		p.flagbits = (const uint16_t *)c.phot_fb.p;

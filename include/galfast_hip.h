@@ -186,11 +186,14 @@ typedef struct gf_photometry_data {
     float    tcFeH[2], tcMr[2];
     const float *color[4];   /* device, float4 texels */
     const float *cflags[4];  /* device, float4 texels */
-    /* optional (device): the colour texels interleaved, texel k holding the
-     * ntex colour float4s at texels + 4*ntex*k, so that the two texels a
-     * lookup needs in one Mr row are one contiguous fetch, and the flags as
-     * bits (flagbits[k] bit i set when colour i's flag texel is nonzero);
-     * used instead of color/cflags when both are set */
+    /* optional (device): the colour texels by lookup quad, so that a lookup
+     * is one contiguous fetch.  Quad k = j*nFeH + i holds the four texels a
+     * lookup with lower corner (i, j) interpolates, (i, j), (i1, j),
+     * (i, j1), (i1, j1) with i1 = min(i+1, nFeH-1), j1 = min(j+1, nMr-1):
+     * for each of the ntex colour tables t the four corners' float4s at
+     * texels + 16*(ntex*k + t); flagbits[4k + corner] holds that corner's
+     * flags as bits (bit 4t+c set when colour 4t+c's flag texel is
+     * nonzero).  Used instead of color/cflags when both are set. */
     const float *texels;
     const uint16_t *flagbits;
 } gf_photometry_data;

@@ -166,6 +166,57 @@ def test_photometry(fx):
     fx.compare(["mags", "photoFlags"])
 
 
+def quad_layout(colors, flags, ntex, nF, nM):
+    """The catalog's lookup-quad copy of the colour and flag texels: per
+    lower corner (i, j) the four corners' float4s of every table and each
+    corner's flag bits (bit 4t+c for colour 4t+c)."""
+    sz = nF * nM * 4
+    col = np.stack([colors[t * sz:(t + 1) * sz].reshape(nM, nF, 4) for t in range(ntex)])   # t, j, i, c
+    flg = np.stack([flags[t * sz:(t + 1) * sz].reshape(nM, nF, 4) for t in range(ntex)])
+    bits = np.zeros((nM, nF), dtype=np.uint16)
+    for t in range(ntex):
+        for c in range(4):
+            bits |= ((flg[t, :, :, c] != 0).astype(np.uint16) << np.uint16(4 * t + c))
+    j, i = np.meshgrid(np.arange(nM), np.arange(nF), indexing="ij")
+    i1, j1 = np.minimum(i + 1, nF - 1), np.minimum(j + 1, nM - 1)
+    corners = [(j, i), (j, i1), (j1, i), (j1, i1)]
+    qd = np.stack([np.stack([col[t][cj, ci] for cj, ci in corners], axis=2) for t in range(ntex)], axis=2)
+    fb = np.stack([bits[cj, ci] for cj, ci in corners], axis=2)
+    return np.ascontiguousarray(qd, dtype=np.float32).ravel(), np.ascontiguousarray(fb).ravel()
+
+
+@pytest.mark.parametrize("nbands,system", [(5, False), (7, False), (7, True)])
+def test_photometry_quad_layout(nbands, system):
+    """The photometry kernel reading the lookup-quad copy gives the oracle's
+    bits (the oracle samples the reference's separate texel tables), for one
+    and two colour tables, lone stars and binaries."""
+    fx = Fixture(nbands=nbands)
+    rows = gfx.synthetic_photosys(ncolors=nbands - 1)
+    ntex, nF, nM, tc, colors, flags = gfx.photometry_tables(rows, nbands - 1)
+    qd, fb = quad_layout(colors, flags, ntex, nF, nM)
+    dq, dfb = dev_array(qd), dev_array(fb)
+    pd = fx.phot_dev      # this test's own fixture
+    pd.texels, pd.flagbits = dq.data_ptr(), dfb.data_ptr()
+    comps = A.bitmap(0, 1, 2)
+    if system:
+        md, mh = fx.mult(True, 1), fx.mult(False, 1)
+        G.check(G.lib().gf_os_unresolvedMultiples_kernel(C.byref(fx.rows()), comps, C.byref(md),
+                                                         C.byref(fx.drng.rng), 2, fx.dev.col("absmag"),
+                                                         fx.dev.col("absmagSys"), fx.dev.col("absmagSysNcomp"),
+                                                         fx.dev.col("comp"), fx.dev.col("hidden"), stream_ptr()))
+    Mr, nabs = (fx.dev.col("absmagSys"), 2) if system else (fx.dev.col("absmag"), 1)
+    G.check(G.lib().gf_os_photometry_kernel(C.byref(fx.rows()), comps, C.byref(pd), fx.dev.col("Am"),
+                                            fx.dev.col("photoFlags"), fx.dev.col("DM"), Mr, nabs,
+                                            fx.dev.col("mags"), fx.dev.col("FeH"), fx.dev.col("comp"),
+                                            fx.dev.col("hidden"), stream_ptr()))
+    if system:
+        fx.oracle_chain(mult=mh, mult_comps=comps, phot=fx.phot_host, phot_comps=comps)
+        fx.compare(["absmagSys", "mags", "photoFlags"])
+    else:
+        fx.oracle_chain(phot=fx.phot_host, phot_comps=comps)
+        fx.compare(["mags", "photoFlags"])
+
+
 @pytest.mark.parametrize("algo", [1, 2, 3])
 def test_unresolvedMultiples(fx, algo):
     comps = A.bitmap(0, 1, 2)

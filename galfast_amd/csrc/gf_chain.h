@@ -209,7 +209,7 @@ GF_D void rows_bond(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t p
 	if (hidden) return;
 	gf_mwc r = restart_at(a, q);
 	float v[3];
-	if (bond_row(m.bond, r, cmp, x, y, z, v)) {
+	if (m.bond_one_disk ? bond_row<true>(m.bond, r, cmp, x, y, z, v) : bond_row<false>(m.bond, r, cmp, x, y, z, v)) {
 		col<float>(T.vcyl, row, 0) = v[0];
 		col<float>(T.vcyl, row, 1) = v[1];
 		col<float>(T.vcyl, row, 2) = v[2];

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
 #include "../../include/galfast_hip.h"
 
 // Module parameter blocks of a chain (device copy), with the index of each
@@ -22,9 +23,13 @@ struct gf_chain_mods {
 	int gal2other;
 	int pl_feh, pl_gfeh, pl_bond, pl_perr;
 	int nplanes;
+	int bond_one_disk;   // Bond2010's two disk moment sets are the same bits
 };
 
-// Assign the position planes of the enabled modules.
+// Assign the position planes of the enabled modules, and note whether
+// Bond2010's disk_1 and disk_2 are identical (then the per-star choice
+// between them, drawn as always, selects the same numbers, and the kernel
+// may read disk_1 as a value common to every lane).
 inline void gf_chain_planes(gf_chain_mods &m)
 {
 	int n = 0;
@@ -33,6 +38,7 @@ inline void gf_chain_planes(gf_chain_mods &m)
 	m.pl_bond = m.has_bond ? n++ : -1;
 	m.pl_perr = m.has_perr ? n++ : -1;
 	m.nplanes = n;
+	m.bond_one_disk = m.has_bond && memcmp(&m.bond.disk_1, &m.bond.disk_2, sizeof(gf_vmoments)) == 0;
 }
 
 // Stream positions: plane p holds the (c, x) of row i at pos[p * pitch + i].

@@ -322,7 +322,10 @@ GF_HD float gf_fabsf(float x) { return __builtin_fabsf(x); }
 template <typename LogX>
 GF_HD float gf_powf_nonneg(float x, float y, LogX logx)
 {
-	if (y == 0.0f || x == 1.0f) return 1.0f;
+	// y first and alone: a kernel's exponents are the same for every lane,
+	// so this test is a scalar branch there
+	if (y == 0.0f) return 1.0f;
+	if (x == 1.0f) return 1.0f;
 	if (x != x || y != y) return x + y;
 	if (x == 0.0f) return y > 0.0f ? 0.0f : gfm::as_float(0x7f800000u);
 	return (float)gfm::exp((double)y * logx());

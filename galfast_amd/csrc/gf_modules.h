@@ -144,7 +144,10 @@ GF_D void bond_trivar(float y[3], const float s[6], gf_mwc &rng)
 }
 
 // One row of os_Bond2010_kernel (modules/Bond2010_gpu.cu.h:185-216); false
-// when the row's component is not handled by the module.
+// when the row's component is not handled by the module.  ONE_DISK: disk_1
+// and disk_2 hold the same bits, so the disk moments are read from disk_1
+// whatever the draw chooses (the uniform is drawn as always).
+template <bool ONE_DISK = false>
 GF_D bool bond_row(const gf_Bond2010_data &par, gf_mwc &rng, int cmp, float x, float y, float z, float v[3])
 {
 	transform3(x, y, z, par.T, par.M);
@@ -154,7 +157,7 @@ GF_D bool bond_row(const gf_Bond2010_data &par, gf_mwc &rng, int cmp, float x, f
 	if (bit_isset(par.comp_thin, cmp) || bit_isset(par.comp_thick, cmp)) {
 		// get_disk_kinematics (modules/Bond2010_gpu.cu.h:130-140)
 		float p = rng.uniform();
-		const gf_vmoments &disk = p < par.fk ? par.disk_1 : par.disk_2;
+		const gf_vmoments &disk = ONE_DISK || p < par.fk ? par.disk_1 : par.disk_2;
 		for (int i = 0; i < 3; i++) v[i] = bond_modfun(bp, disk.m[i]);
 		if (v[1] > 0.f) v[1] = 0.f;
 		float s[6];

@@ -2162,9 +2162,16 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 			batch.clear();
 			return 0;
 		}
-		for (int r = k; r < nr; r++) deferred.push_back(std::move(batch[r]));
-		batch.resize(k);
-		nr = k;
+		if (k < nr) {
+			// the buffers were sized for every candidate, the deferred ones
+			// included: run the prefix again from released buffers, so that
+			// what the worker holds is the prefix's own (its counts are kept,
+			// so the second count admits all of it)
+			for (int r = k; r < nr; r++) deferred.push_back(std::move(batch[r]));
+			batch.resize(k);
+			w.release();
+			return run_batch(s, w, batch, deferred, rejected);
+		}
 	}
 	const uint64_t U = uoff[nr];
 	uint64_t bseq;

@@ -188,13 +188,23 @@ def test_budget_admission_by_counted_rows(base, alone):
     for rid, (res, t) in got.items():
         if rid == BIG[0]:
             continue
-        assert res.status == 0, (rid, res.status)
+        assert res.status == 0, (rid, res.status, G.lib().gf_last_error(), {k: v[0].status for k, v in got.items()},
+                                 {k: v for k, v in need.items()}, budget)
         r0, t0 = ref[rid]
         assert (res.generated, res.visible, res.expected) == (r0.generated, r0.visible, r0.expected)
         assert res.device_bytes == need[rid]
         same_rows(t, t0, "request %d under a small budget" % rid)
         per_batch[res.batch] = per_batch.get(res.batch, 0) + res.device_bytes
     assert len(per_batch) > 1 and max(per_batch.values()) <= budget
+    # a request that fits alone also succeeds when it shares its first batch
+    # with one that does not (whichever the preparation threads finish
+    # first leads the batch, so the pair is served several times)
+    fit = max(REQUESTS, key=lambda r: need[r[0]])
+    for _ in range(4):
+        pair = serve_status(base, [fit, BIG], max_batch=16, workers=1, linger_us=300000, hbm_budget=budget)
+        assert pair[BIG[0]][0].status == GF_E_BUDGET
+        assert pair[fit[0]][0].status == 0, G.lib().gf_last_error()
+        same_rows(pair[fit[0]][1], ref[fit[0]][1], "request %d beside an over-budget one" % fit[0])
     # the same requests with two workers share the budget between them
     got2 = serve_status(base, REQUESTS, max_batch=16, workers=2, linger_us=100000, hbm_budget=2 * budget)
     for rid, (res, t) in got2.items():

@@ -9,8 +9,11 @@ stars per second.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1]
 
-The CPU baseline replays a bounded sample of the same job (a contiguous
-block of sky pixels) on the single-threaded oracle on the host.
+The CPU baseline replays a bounded sample of the same job (blocks of sky
+pixels drawn at random) on the oracle, one replay per host core, and once
+on a single core.  The serving record times the request server on the
+config-3 stream of footprint requests beside one catalog job per request;
+the c4 sub-record repeats the catalog measurement on config 4.
 """
 import argparse
 import json
@@ -18,6 +21,12 @@ import os
 import sys
 import tempfile
 import time
+
+# the catalog's four HIP streams each on a hardware queue of their own
+# (galfast_amd/__init__.py): at least 8 queues for this process, whatever
+# lower value the environment holds; set before anything initialises HIP
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)

@@ -9,6 +9,17 @@ CPU fallback: importing :func:`lib` without the built library raises.
 import ctypes as C
 import os
 
+# A catalog run keeps four HIP streams busy at once (plan and draw passes,
+# emission, two row-pass streams).  HIP maps a process's streams onto its
+# hardware queues, four by default with one taken by the null stream; two
+# streams that share a queue run strictly one after the other, so the
+# emission of the next chunk would wait behind the kinematics passes of the
+# current one.  Eight queues give every stream its own.  HIP reads this
+# once, when it initialises in the process: it takes effect only when the
+# package is imported before the first GPU call, and a value the
+# environment already sets is left as it is.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+
 from . import _abi
 from ._abi import *  # noqa: F401,F403
 

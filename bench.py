@@ -582,6 +582,18 @@ def measure(args, workload, steps, warmup, overlap=1):
             valu_issue = {"kernel": name, "valu_insts_per_launch": vt[name]["valu_insts"],
                           "simd_cycles_per_valu_inst": 1024 * 2.4e9 * per_launch_s / vt[name]["valu_insts"],
                           "isolated_simd_cycles_per_valu_inst": vt[name]["simd_cycles_per_valu"]}
+    # the count integral's vector arithmetic, from the same counter runs
+    # (SQ_INSTS_VALU_FLOPS_FP32 / _FP64 over the isolated launch)
+    count_valu = None
+    if os.path.exists(vj):
+        with open(vj) as f:
+            vt = json.load(f)
+        ck = next((k for k in vt if k.startswith("k_count<")), None)
+        if ck and "fp32_tflops" in vt[ck]:
+            count_valu = {"kernel": ck, "fp32_tflops": vt[ck]["fp32_tflops"], "fp64_tflops": vt[ck]["fp64_tflops"],
+                          "peak_fp32_tflops": 157.3, "peak_fp64_tflops": 78.6,
+                          "simd_cycles_per_valu_inst": vt[ck]["simd_cycles_per_valu"],
+                          "source": os.path.relpath(vj, ROOT)}
     row_b = table_row_bytes(info.nbands, info.nabsmag, present)
     rows_per_s = acc["generated"] / elapsed
     out = {
@@ -615,6 +627,7 @@ def measure(args, workload, steps, warmup, overlap=1):
                               "peak": PEAK_HBM_GBS, "unit": "GB/s",
                               "frac": rows_per_s * row_b / 1e9 / PEAK_HBM_GBS},
         "valu_issue": valu_issue,
+        "count_valu": count_valu,
     }
     return out, cat
 

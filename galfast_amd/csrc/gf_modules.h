@@ -8,23 +8,25 @@
 static constexpr float ABSMAG_NOT_PRESENT = 99.999f;   // modules/module_lib.h:57
 
 // One row of os_FeH_kernel (modules/FeH_gpu.cu.h:60-84); false when the row's
-// component is not handled by the module.
+// component is not handled by the module.  Disk rows draw their component
+// choice first; the Gaussian deviate, the last draw of both branches, is
+// drawn once after them (offs + g and g + offs are the same sum).
 GF_D bool feh_row(const gf_FeH_data &par, gf_mwc &rng, int cmp, float x, float y, float z, float &feh)
 {
-	if (bit_isset(par.comp_thin, cmp) || bit_isset(par.comp_thick, cmp)) {
+	const bool disk = bit_isset(par.comp_thin, cmp) || bit_isset(par.comp_thick, cmp);
+	if (!disk && !bit_isset(par.comp_halo, cmp)) return false;
+	int i = 2;
+	float aZ = 0.f;
+	if (disk) {
 		float p = rng.uniform() * (par.A[0] + par.A[1]);
-		int i = p < par.A[0] ? 0 : 1;
+		i = p < par.A[0] ? 0 : 1;
 		transform3(x, y, z, par.T, par.M);
 		float muD = par.muInf + par.DeltaMu * gf_expf(-gf_fabsf(z) / par.Hmu);
-		float aZ = muD - 0.067f;
-		feh = rng.gaussian(par.sigma[i]) + aZ + par.offs[i];
-		return true;
+		aZ = muD - 0.067f;
 	}
-	if (bit_isset(par.comp_halo, cmp)) {
-		feh = par.offs[2] + rng.gaussian(par.sigma[2]);
-		return true;
-	}
-	return false;
+	const float g = rng.gaussian(par.sigma[i]);
+	feh = disk ? g + aZ + par.offs[i] : g + par.offs[2];
+	return true;
 }
 
 // The draws feh_row makes for a row of component cmp, without the
@@ -144,9 +146,15 @@ GF_D void bond_trivar(float y[3], const float s[6], gf_mwc &rng)
 }
 
 // One row of os_Bond2010_kernel (modules/Bond2010_gpu.cu.h:185-216); false
-// when the row's component is not handled by the module.  ONE_DISK: disk_1
-// and disk_2 hold the same bits, so the disk moments are read from disk_1
-// whatever the draw chooses (the uniform is drawn as always).
+// when the row's component is not handled by the module.  The moments are
+// evaluated in the component's own branch (get_disk_kinematics, :130-140;
+// get_halo_kinematics, :142-159), whose parameters are the same for every
+// lane; the three Gaussian deviates of trivar_gaussK_draw, which both
+// branches draw last, are drawn after the branches, once per wave however
+// its rows mix the components (every row's draws stay in their order: a
+// disk row's uniform, then its deviates).  ONE_DISK: disk_1 and disk_2 hold
+// the same bits, so the disk moments are read from disk_1 whatever the draw
+// chooses (the uniform is drawn as always).
 template <bool ONE_DISK = false>
 GF_D bool bond_row(const gf_Bond2010_data &par, gf_mwc &rng, int cmp, float x, float y, float z, float v[3])
 {
@@ -154,23 +162,39 @@ GF_D bool bond_row(const gf_Bond2010_data &par, gf_mwc &rng, int cmp, float x, f
 	const float Rsquared = (float)(1e-6 * (double)(x * x + y * y));
 	const float Z = (float)(1e-3 * (double)z);
 	BondPow bp(Rsquared, Z);
-	if (bit_isset(par.comp_thin, cmp) || bit_isset(par.comp_thick, cmp)) {
-		// get_disk_kinematics (modules/Bond2010_gpu.cu.h:130-140)
+	const bool disk = bit_isset(par.comp_thin, cmp) || bit_isset(par.comp_thick, cmp);
+	if (!disk && !bit_isset(par.comp_halo, cmp)) return false;
+	float s[6];
+	if (disk) {
 		float p = rng.uniform();
-		const gf_vmoments &disk = ONE_DISK || p < par.fk ? par.disk_1 : par.disk_2;
-		for (int i = 0; i < 3; i++) v[i] = bond_modfun(bp, disk.m[i]);
+		const bool first = ONE_DISK || p < par.fk;
+		// the chosen set's coefficients by value (with ONE_DISK, disk_1's,
+		// the same for every lane)
+		auto pick = [first](const float (&a)[5], const float (&b)[5], float c[5]) {
+#pragma unroll
+			for (int e = 0; e < 5; e++) c[e] = first ? a[e] : b[e];
+		};
+#pragma unroll
+		for (int i = 0; i < 3; i++) {
+			float c[5];
+			pick(par.disk_1.m[i], par.disk_2.m[i], c);
+			v[i] = bond_modfun(bp, c);
+		}
 		if (v[1] > 0.f) v[1] = 0.f;
-		float s[6];
-		for (int i = 0; i < 6; i++) s[i] = bond_modfun(bp, disk.ss[i]);
-		bond_trivar(v, s, rng);
-		return true;
-	}
-	if (bit_isset(par.comp_halo, cmp)) {
-		// get_halo_kinematics (modules/Bond2010_gpu.cu.h:142-159)
+#pragma unroll
+		for (int i = 0; i < 6; i++) {
+			float c[5];
+			pick(par.disk_1.ss[i], par.disk_2.ss[i], c);
+			s[i] = bond_modfun(bp, c);
+		}
+	} else {
+#pragma unroll
 		for (int i = 0; i < 3; i++) v[i] = bond_modfun(bp, par.halo.m[i]);
-		float s[6];
+#pragma unroll
 		for (int i = 0; i < 6; i++) s[i] = bond_modfun(bp, par.halo.ss[i]);
-		bond_trivar(v, s, rng);
+	}
+	bond_trivar(v, s, rng);
+	if (!disk) {
 		float invR = 1.f / gf_sqrtf(Rsquared + Z * Z);
 		float cosa = gf_sqrtf(Rsquared) * invR;
 		float sina = Z * invR;
@@ -179,9 +203,8 @@ GF_D bool bond_row(const gf_Bond2010_data &par, gf_mwc &rng, int cmp, float x, f
 		v[0] = vR;
 		v[1] = v[2];
 		v[2] = vZ;
-		return true;
 	}
-	return false;
+	return true;
 }
 
 // The draws bond_row makes for a row of component cmp.

@@ -147,13 +147,32 @@ GF_D double bin_expectation(const gf_skygen_params &sp, const gf_model &m, const
 	}
 }
 
-template <int MODE>
+// The count's prefix table in LDS (LDS = true, launched with nM + 1 entries
+// of dynamic shared memory) when it has at most COUNT_LDS_NM + 1 entries,
+// else read from global memory.
+constexpr int COUNT_LDS_NM = 4095;
+
+template <int MODE, bool LDS>
 __global__ void __launch_bounds__(SK_BS) k_count(gf_skygen_params sp, gf_model m, const gf_pencil_beam *pixels,
                                                  gf_extinction ext, LFTables t, CountAcc *pix_out,
                                                  double *per_pixel, double *wpk)
 {
 	const int p = blockIdx.x;
 	const int K = sp.nm + sp.nM - 1;
+	if (LDS) {
+		// the mode's prefix table, staged once per workgroup
+		extern __shared__ double count_prefix[];
+		if (MODE == GF_ACC_FP64) {
+			for (int i = threadIdx.x; i <= sp.nM; i += blockDim.x) count_prefix[i] = t.P[i];
+			t.P = count_prefix;
+		} else {
+			float *f = (float *)count_prefix;
+			const float *src = MODE == GF_ACC_KAHAN32 ? t.Pk : t.Pf;
+			for (int i = threadIdx.x; i <= sp.nM; i += blockDim.x) f[i] = src[i];
+			if (MODE == GF_ACC_KAHAN32) t.Pk = f; else t.Pf = f;
+		}
+		__syncthreads();
+	}
 	const gf_pencil_beam pix = pixels[p];
 	Acc<MODE> acc;
 	double mx = 0.0, nc = 0.0;
@@ -239,23 +258,37 @@ __global__ void k_count_empty(gf_count_result *res)
 
 // The count kernels in the accumulation arithmetic acc_mode: per-pixel sums
 // into pix_out (npixels entries), the job's totals into *dres (device).
+// k_count in accumulation MODE, its prefix table (entries of type T) staged
+// in LDS when it fits.
+template <int MODE, typename T>
+void count_kernel(const gf_skygen_params &sp, const gf_model &m, const gf_pencil_beam *pixels,
+                  const gf_extinction &ex, const LFTables &t, CountAcc *pix_out, double *per_pixel, double *wpk,
+                  hipStream_t st)
+{
+	if (sp.nM <= COUNT_LDS_NM)
+		k_count<MODE, true><<<sp.npixels, SK_BS, ((size_t)sp.nM + 1) * sizeof(T), st>>>(sp, m, pixels, ex, t, pix_out,
+		                                                                                per_pixel, wpk);
+	else
+		k_count<MODE, false><<<sp.npixels, SK_BS, 0, st>>>(sp, m, pixels, ex, t, pix_out, per_pixel, wpk);
+}
+
 int launch_count(const gf_skygen_params &sp, const gf_model &m, const gf_pencil_beam *pixels,
                  const gf_extinction &ex, int acc_mode, const LFTables &t, CountAcc *pix_out,
                  gf_count_result *dres, double *per_pixel, double *wpk, hipStream_t st)
 {
 	switch (acc_mode) {
 	case GF_ACC_FP64:
-		k_count<GF_ACC_FP64><<<sp.npixels, SK_BS, 0, st>>>(sp, m, pixels, ex, t, pix_out, per_pixel, wpk);
+		count_kernel<GF_ACC_FP64, double>(sp, m, pixels, ex, t, pix_out, per_pixel, wpk, st);
 		GF_LAUNCHED();
 		k_count_final<GF_ACC_FP64><<<1, SK_BS, 0, st>>>(pix_out, sp.npixels, dres);
 		break;
 	case GF_ACC_KAHAN32:
-		k_count<GF_ACC_KAHAN32><<<sp.npixels, SK_BS, 0, st>>>(sp, m, pixels, ex, t, pix_out, per_pixel, wpk);
+		count_kernel<GF_ACC_KAHAN32, float>(sp, m, pixels, ex, t, pix_out, per_pixel, wpk, st);
 		GF_LAUNCHED();
 		k_count_final<GF_ACC_KAHAN32><<<1, SK_BS, 0, st>>>(pix_out, sp.npixels, dres);
 		break;
 	default:
-		k_count<GF_ACC_FP32><<<sp.npixels, SK_BS, 0, st>>>(sp, m, pixels, ex, t, pix_out, per_pixel, wpk);
+		count_kernel<GF_ACC_FP32, float>(sp, m, pixels, ex, t, pix_out, per_pixel, wpk, st);
 		GF_LAUNCHED();
 		k_count_final<GF_ACC_FP32><<<1, SK_BS, 0, st>>>(pix_out, sp.npixels, dres);
 		break;

@@ -7,9 +7,10 @@
 // moves the stream past them (feh_skip, bond_skip, ...: the uniform draws
 // and rejection loops only, none of the arithmetic that turns them into
 // values).  unresolvedMultiples, whose number of draws depends on the row's
-// data, runs in full here.
+// data, draws its companions here; with two absolute-magnitude planes the
+// FEH row pass sums the system's magnitude, with more the draw pass does.
 //
-// Row pass (one lane per row, four launches): every module in pipeline
+// Row pass (one lane per row, five launches): every module in pipeline
 // order (FeH, GaussianFeH, Bond2010 | photometry | photometric errors |
 // vel2pm, gal2other) for the row, the random-number modules restarting the
 // stream from the recorded position.  A row's draws are the same numbers in the same order
@@ -78,6 +79,84 @@ GF_D void tile_store(uint2 *dst, int n, const DrawLds &L, int lane)
 	}
 }
 
+// Write the lane's n values of LDS column src to dst.
+GF_D void tile_store1(uint32_t *dst, int n, const uint32_t (*src)[BS], int lane)
+{
+	if (n == TILE && ((uintptr_t)dst & 15) == 0) {
+		uint4 *v = (uint4 *)dst;
+#pragma unroll
+		for (int i = 0; i < TILE / 4; i++)
+			v[i] = make_uint4(src[4 * i][lane], src[4 * i + 1][lane], src[4 * i + 2][lane], src[4 * i + 3][lane]);
+	} else {
+		for (int i = 0; i < n; i++) dst[i] = src[i][lane];
+	}
+}
+
+GF_D bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
+
+// The companion draws of unresolvedMultiples over the chunk with two
+// absolute-magnitude planes, a tile at a time: the tile's absmag comes in
+// through vector loads; the primary's and the companion's magnitudes and
+// the component count leave through vector stores when every row of a whole
+// tile takes part, otherwise row by row for the rows that take part.  The
+// system magnitudes follow in the FEH row pass (rows_mult_total).
+template <bool CACHED>
+GF_D void mult_walk(const gf_chain_mods &m, gf_mwc &rng, uint32_t r0, uint32_t r1, const gf_table &T, DrawLds &L,
+                    int lane)
+{
+	for (uint32_t base = r0; base < r1; base += TILE) {
+		const int n = (int)(r1 - base < (uint32_t)TILE ? r1 - base : TILE);
+		if (!CACHED) {
+			tile_fetch(&col<uint32_t>(T.hidden, base), n, L.a, lane, 1u);
+			tile_fetch(&col<uint32_t>(T.comp, base), n, L.b, lane, 0u);
+			for (int i = 0; i < n; i++) L.hc[i][lane] = hc_code(L.a[i][lane], L.b[i][lane]);
+		}
+		const uint32_t h0 = CACHED ? base - r0 : 0;
+		bool all = n == TILE;
+		for (int i = 0; i < n; i++) {
+			const uint8_t code = L.hc[h0 + i][lane];
+			all = all && code != HIDDEN && bit_isset(m.mult_comps, code == NOCOMP ? -1 : (int)code);
+		}
+		uint32_t *S0 = &col<uint32_t>(T.absmagSys, base, 0), *S1 = &col<uint32_t>(T.absmagSys, base, 1);
+		uint32_t *Nc = &col<uint32_t>(T.absmagSysNcomp, base);
+		tile_fetch(&col<uint32_t>(T.absmag, base), n, L.a, lane, 0u);
+		const bool vec = all && aligned16(S0) && aligned16(S1) && aligned16(Nc);
+		uint32_t two = 0;   // bit i: row base+i has a companion (vector path)
+		for (int i = 0; i < n; i++) {
+			if (!vec) {
+				const uint8_t code = L.hc[h0 + i][lane];
+				if (code == HIDDEN || !bit_isset(m.mult_comps, code == NOCOMP ? -1 : (int)code)) continue;
+			}
+			const float M1 = __uint_as_float(L.a[i][lane]);
+			float M2 = ABSMAG_NOT_PRESENT;
+			bool has = false;
+			multiples_draw(m.mult, rng, M1, 2, [&](int, float v, bool h) { M2 = v; has = h; });
+			if (vec) {
+				L.b[i][lane] = __float_as_uint(M2);
+				two |= (uint32_t)has << i;
+			} else {
+				S0[i] = __float_as_uint(M1);
+				S1[i] = __float_as_uint(M2);
+				Nc[i] = has ? 2u : 1u;
+			}
+		}
+		if (vec) {
+			tile_store1(S0, TILE, L.a, lane);
+			tile_store1(S1, TILE, L.b, lane);
+			uint4 *v = (uint4 *)Nc;
+#pragma unroll
+			for (int i = 0; i < TILE / 4; i++)
+				v[i] = make_uint4(1u + ((two >> (4 * i)) & 1u), 1u + ((two >> (4 * i + 1)) & 1u),
+				                  1u + ((two >> (4 * i + 2)) & 1u), 1u + ((two >> (4 * i + 3)) & 1u));
+		}
+	}
+}
+
+// Whether unresolvedMultiples is split between the passes (its draws in the
+// draw pass, the system magnitude in the FEH row pass): with two
+// absolute-magnitude planes.
+GF_D bool mult_split(const gf_chain_mods &m, const gf_table &T) { return m.has_mult && T.absmagSys.width == 2; }
+
 // One module's walk over the chunk: record each visible row's position in
 // plane pos, then step past the row's draws with skip(rng, comp).
 template <bool CACHED, typename Skip>
@@ -122,9 +201,13 @@ GF_D void chunk_draws(const gf_chain_mods &m, gf_mwc &rng, uint32_t r0, uint32_t
 	if (m.has_gfeh)
 		draw_walk<CACHED>(rng, r0, r1, prow0, P.pos + m.pl_gfeh * P.pitch, T, L, lane,
 		                  [&](gf_mwc &r, int cmp) { gfeh_skip(m.gfeh, m.gfeh_comps, r, cmp); });
-	if (m.has_mult)
-		mod_multiples(m.mult, m.mult_comps, rng, r0, r1, (int)T.absmagSys.width, T.absmag, T.absmagSys,
-		              T.absmagSysNcomp, T.comp, T.hidden);
+	if (m.has_mult) {
+		if (mult_split(m, T))
+			mult_walk<CACHED>(m, rng, r0, r1, T, L, lane);
+		else
+			mod_multiples(m.mult, m.mult_comps, rng, r0, r1, (int)T.absmagSys.width, T.absmag, T.absmagSys,
+			              T.absmagSysNcomp, T.comp, T.hidden);
+	}
 	if (m.has_bond)
 		draw_walk<CACHED>(rng, r0, r1, prow0, P.pos + m.pl_bond * P.pitch, T, L, lane,
 		                  [&](gf_mwc &r, int cmp) { bond_skip(m.bond, r, cmp); });
@@ -148,7 +231,8 @@ GF_D uint2 position(const gf_positions &P, int plane, uint64_t prow) { return P.
 
 // The row pass, in five groups so that each launch stays small enough in
 // registers to keep several waves per SIMD, and so that independent groups
-// can run side by side: FEH (FeH, GaussianFeH), BOND (Bond2010), PHOT
+// can run side by side: FEH (FeH, GaussianFeH, the system magnitude of
+// unresolvedMultiples), BOND (Bond2010), PHOT
 // (photometry), PERR (photometric errors), ASTRO (vel2pm, gal2other).  a is
 // the multiplier of the row's stream and prow its index in the position
 // planes.  Each group reads all of a row's inputs before it tests the row's
@@ -165,7 +249,7 @@ enum {
 
 inline bool rows_on(const gf_chain_mods &m, int group)
 {
-	if (group == ROWS_FEH) return m.has_feh || m.has_gfeh;
+	if (group == ROWS_FEH) return m.has_feh || m.has_gfeh || m.has_mult;
 	if (group == ROWS_BOND) return m.has_bond;
 	if (group == ROWS_PHOT) return m.has_phot;
 	if (group == ROWS_PERR) return m.has_perr;
@@ -178,11 +262,24 @@ inline bool rows_stream(const gf_chain_mods &m, int group)
 	return group == ROWS_FEH || group == ROWS_BOND || group == ROWS_PERR;
 }
 
+// The system magnitude of a row the draw pass gave a companion draw
+// (mult_split): the luminosity sum of its one or two components.
+GF_D void rows_mult_total(const gf_chain_mods &m, uint32_t row, int hidden, int cmp, const gf_table &T)
+{
+	const float M1 = col<float>(T.absmagSys, row, 0), M2 = col<float>(T.absmagSys, row, 1);
+	const int n = col<int>(T.absmagSysNcomp, row);
+	if (hidden || !bit_isset(m.mult_comps, cmp)) return;
+	float Ltot = multiples_lum(M1);
+	if (n == 2) Ltot += multiples_lum(M2);
+	col<float>(T.absmag, row) = multiples_mag(Ltot);
+}
+
 GF_D void rows_feh(const gf_chain_mods &m, uint32_t a, uint32_t row, uint64_t prow, const gf_table &T,
                    const gf_positions &P)
 {
 	const int hidden = col<int>(T.hidden, row);
 	const int cmp = col<int>(T.comp, row);
+	if (mult_split(m, T)) rows_mult_total(m, row, hidden, cmp, T);
 	if (m.has_feh) {
 		const float x = col<float>(T.XYZ, row, 0), y = col<float>(T.XYZ, row, 1), z = col<float>(T.XYZ, row, 2);
 		const uint2 q = position(P, m.pl_feh, prow);

@@ -508,8 +508,43 @@ GF_D void mod_photometry(const gf_photometry_data &lt, gf_bitmap apply, uint32_t
 	}
 }
 
-// draw_companion and os_unresolvedMultiples_kernel body
-// (modules/unresolvedMultiples_gpu.cu.h:48-122)
+// The companion draws of one row of os_unresolvedMultiples_kernel
+// (draw_companion, modules/unresolvedMultiples_gpu.cu.h:48-78, called from
+// :104-117) whose primary has magnitude M1: put(i, M, has) receives the
+// magnitude of companion i in [1, nabsmag), ABSMAG_NOT_PRESENT when it has
+// none.
+template <typename Put>
+GF_D void multiples_draw(const gf_multiples_data &par, gf_mwc &rng, float M1, int nabsmag, Put put)
+{
+	for (int i = 1; i < nabsmag; i++) {
+		float M2 = 0.f;
+		bool has = false;
+		float psec = tex1d(par.secProb, M1);
+		float u = rng.uniform();
+		if (!(u > psec)) {
+			has = true;
+			if (par.algo == 3) {
+				M2 = M1;
+			} else {
+				float pprim = tex1d(par.cumLF, M1);
+				u = rng.uniform();
+				if (par.algo == 1) u = (float)((double)pprim + (double)u * (1. - (double)pprim));
+				M2 = tex1d(par.invCumLF, u);
+				if (par.algo == 1 && M2 < M1) M2 = M1;
+			}
+		}
+		put(i, has ? M2 : ABSMAG_NOT_PRESENT, has);
+	}
+}
+
+// The system's luminosity sum (modules/unresolvedMultiples_gpu.cu.h:104-118):
+// each present component adds 10^(-0.4 M) in component order, and the total
+// magnitude is -2.5 log10 of the sum.
+GF_D float multiples_lum(float M) { return gf_exp10f(-0.4f * M); }
+GF_D float multiples_mag(float Ltot) { return (float)(-2.5 * (double)gf_log10f(Ltot)); }
+
+// os_unresolvedMultiples_kernel (modules/unresolvedMultiples_gpu.cu.h:80-123)
+// over rows [r0, r1)
 GF_D void mod_multiples(const gf_multiples_data &par, gf_bitmap apply, gf_mwc &rng, uint32_t r0, uint32_t r1,
                         int nabsmag, const gf_col &M, const gf_col &Msys, const gf_col &ncomp,
                         const gf_col &comp, const gf_col &hidden)
@@ -520,36 +555,17 @@ GF_D void mod_multiples(const gf_multiples_data &par, gf_bitmap apply, gf_mwc &r
 		if (!bit_isset(apply, cmp)) continue;
 		float M1 = col<float>(M, row);
 		col<float>(Msys, row, 0) = M1;
-		float Ltot = gf_exp10f(-0.4f * M1);
-		int ncomps = 1;
-		for (int i = 1; i < nabsmag; i++) {
-			float M2 = 0.f;
-			bool has = false;
-			float psec = tex1d(par.secProb, M1);
-			float u = rng.uniform();
-			if (!(u > psec)) {
-				has = true;
-				if (par.algo == 3) {
-					M2 = M1;
-				} else {
-					float pprim = tex1d(par.cumLF, M1);
-					u = rng.uniform();
-					if (par.algo == 1) u = (float)((double)pprim + (double)u * (1. - (double)pprim));
-					M2 = tex1d(par.invCumLF, u);
-					if (par.algo == 1 && M2 < M1) M2 = M1;
-				}
-			}
+		float Ltot = multiples_lum(M1);
+		int n = 1;
+		multiples_draw(par, rng, M1, nabsmag, [&](int i, float v, bool has) {
+			col<float>(Msys, row, i) = v;
 			if (has) {
-				col<float>(Msys, row, i) = M2;
-				Ltot += gf_exp10f(-0.4f * M2);
-				ncomps++;
-			} else {
-				col<float>(Msys, row, i) = ABSMAG_NOT_PRESENT;
+				Ltot += multiples_lum(v);
+				n++;
 			}
-		}
-		float Mtot = (float)(-2.5 * (double)gf_log10f(Ltot));
-		col<float>(M, row) = Mtot;
-		col<int>(ncomp, row) = ncomps;
+		});
+		col<float>(M, row) = multiples_mag(Ltot);
+		col<int>(ncomp, row) = n;
 	}
 }
 

@@ -367,11 +367,12 @@ def _pack_and_format(L, A, table, n, reps=3):
 
 
 def serving(args):
-    """Request-server measurement on the serving workload: a sample of the
-    requests run as separate catalog jobs (the plain GPU path), the whole
-    request set served one at a time with resident tables (unbatched: no
-    batching, no overlap) and with continuous batching (all submitted at
-    once), then Poisson arrivals at each rate for the latency percentiles."""
+    """Request-server measurement on the serving workload: the whole request
+    set served with continuous batching (all submitted at once), Poisson
+    arrivals at each rate for the latency percentiles, the whole set served
+    one at a time with resident tables (unbatched: no batching, no overlap),
+    and a sample of the requests run as separate catalog jobs (the plain GPU
+    path)."""
     from galfast_amd import configs
     d = tempfile.mkdtemp(prefix="galfast_serve_")
     conf = configs.write_c3(d, args.seed)
@@ -381,6 +382,18 @@ def serving(args):
                        "|b|>5 deg, r 14-m1 with m1 uniform in [20,25], seed per request id; config-1 models "
                        "and modules, 0.25 deg pixels; HBM budget 200 GB" % len(reqs)}
     import numpy as np
+    # The phases run from the quietest host state to the noisiest: the plain
+    # jobs open and free a catalog (gigabytes of device buffers) per request,
+    # and for seconds afterwards the host runs the server's CPU work (beam
+    # pixelization, stream seeding) at a third of its speed, so they go last.
+    out["batched"] = _serve_once(conf, reqs, None, warm, max_batch=args.serve_batch, workers=2)
+    # Poisson arrivals: at each rate the first min(all, rate x 10 s) requests
+    out["poisson"] = {}
+    for rate in args.serve_rates:
+        n = max(20, min(len(reqs), int(rate * args.serve_seconds)))
+        out["poisson"]["%g" % rate] = _serve_once(conf, reqs[:n], rate, warm, max_batch=args.serve_batch,
+                                                  workers=2)
+    out["unbatched"] = _serve_once(conf, reqs, None, warm, plain=True)
     # plain: one catalog job per request, on every k-th request (a bounded
     # sample of the same request stream)
     k = max(1, len(reqs) // args.plain_jobs)
@@ -391,16 +404,8 @@ def serving(args):
                         "p50_ms": 1000.0 * float(np.percentile(plat, 50)),
                         "p99_ms": 1000.0 * float(np.percentile(plat, 99)),
                         "sample": "every %d-th request of the stream" % k}
-    out["unbatched"] = _serve_once(conf, reqs, None, warm, plain=True)
-    out["batched"] = _serve_once(conf, reqs, None, warm, max_batch=args.serve_batch, workers=2)
     out["served_vs_plain"] = out["batched"]["stars_per_s"] / out["plain_job"]["stars_per_s"]
     out["served_vs_unbatched"] = out["batched"]["stars_per_s"] / out["unbatched"]["stars_per_s"]
-    # Poisson arrivals: at each rate the first min(all, rate x 10 s) requests
-    out["poisson"] = {}
-    for rate in args.serve_rates:
-        n = max(20, min(len(reqs), int(rate * args.serve_seconds)))
-        out["poisson"]["%g" % rate] = _serve_once(conf, reqs[:n], rate, warm, max_batch=args.serve_batch,
-                                                  workers=2)
     return out
 
 

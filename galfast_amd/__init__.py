@@ -56,6 +56,7 @@ def _declare(L):
     L.gf_seed_from_pipeline_seed.restype = C.c_uint32
     L.gf_seed_from_pipeline_seed.argtypes = [C.c_uint32]
     L.gf_mwc_seed_host.argtypes = [C.c_uint32, P(C.c_uint32), C.c_uint32, C.c_uint32, P(C.c_uint32)]
+    L.gf_mwc_seed_device.argtypes = [C.c_uint32, vp, C.c_uint32, C.c_uint32, vp, vp]
     L.gf_rng_alloc.argtypes = [P(gf_rng), C.c_uint32]
     L.gf_rng_upload.argtypes = [P(gf_rng), P(C.c_uint32), vp]
     L.gf_rng_download.argtypes = [P(gf_rng), P(C.c_uint32), vp]

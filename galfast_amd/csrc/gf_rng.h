@@ -213,6 +213,14 @@ struct gf_mwc
 // z * a^(2^b) mod m for z in plain form.
 constexpr int GF_JUMP_BITS = 64;   // jump tables cover n < 2^64
 
+// One stream table to seed on the device: nstreams streams from `seed`, at
+// word `offset` of the output ([a | c | x], each plane nstreams words).
+struct gf_seed_seg {
+	uint32_t seed;
+	uint32_t nstreams;
+	uint64_t offset;
+};
+
 struct gf_mwc_mod
 {
 	uint64_t m;

@@ -64,6 +64,7 @@ int module_chain(const gf_rows *ks, const gf_chain *chain, gf_rng *rng, const gf
 // draw pass of their rows is done).  pos: the position planes
 // (chain_positions_bytes), or NULL for a per-thread scratch buffer.
 struct gf_chain_launch;
+struct gf_seed_seg;
 namespace gf {
 size_t chain_positions_bytes(const gf_chain *chain, uint32_t nrows);
 int chain_setup(const gf_rows *ks, const gf_chain *chain, const gf_rng *rng, const gf_table *t, uint2 *pos,
@@ -80,6 +81,12 @@ int noalloc_scratch(void **ptr);
 // ahead with such a table.
 int mwc_jump_table(const gf_rng *rng, uint64_t *tab, hipStream_t st);
 int mwc_jump(gf_rng *rng, uint64_t n, const uint64_t *tab, hipStream_t st);
+// gf_mwc_seed_host on the device for nsegs stream tables at once (segs_dev
+// in device memory, max_streams the largest nstreams among them): table g
+// gets segs_dev[g].nstreams streams seeded from segs_dev[g].seed at word
+// segs_dev[g].offset of out, multipliers primes[0 ..] (device memory).
+int mwc_seed_streams(const gf_seed_seg *segs_dev, uint32_t nsegs, uint32_t max_streams, const uint32_t *primes,
+                     uint32_t *out, hipStream_t st);
 // Adds the number of rows [0, n) with hidden == 0 to *counter (device memory).
 int count_visible(gf_col hidden, uint64_t n, unsigned long long *counter, hipStream_t st);
 
@@ -103,5 +110,6 @@ enum gf_scratch_slot {
 	GF_SLOT_OUT_COLS,
 	GF_SLOT_OUT_COUNT,
 	GF_SLOT_JUMP,
+	GF_SLOT_SEED,
 	GF_SLOT_NSLOTS
 };

@@ -300,15 +300,18 @@ std::vector<gf_pencil_beam> pixelize_region(const gf_footprint &fp, double dx)
 		// points of the region in this hemisphere whose projections bound it
 		std::vector<std::pair<double, double>> lb;
 		if (f.kind == 2) {
-			// a polar sampling of the cap, and the arc of the equator inside
-			// it: a cap that crosses the equator by less than the sampling
-			// step still reaches both hemispheres along that arc
+			// the cap's centre and boundary circle, and the arc of the equator
+			// inside it: the projection maps the hemisphere's part of the cap
+			// onto a region bounded by the images of those two curves, so
+			// their samples bound it (to within the sampling step, which the
+			// two-cell margin below covers); a cap that crosses the equator by
+			// less than the step still reaches both hemispheres along the arc
 			const double bc[3] = {f.cb * f.cl, f.cb * f.sl, f.sb};
 			const double r = acos(std::max(-1.0, std::min(1.0, f.cosr)));
 			const double e1[3] = {-f.sl, f.cl, 0}, e2[3] = {-f.sb * f.cl, -f.sb * f.sl, f.cb};
-			for (int ir = 0; ir <= 24; ir++) {
-				double rr = r * ir / 24.0;
-				int na = ir == 0 ? 1 : 96;
+			for (int ir = 0; ir <= 1; ir++) {
+				const double rr = ir ? r : 0.0;
+				const int na = ir == 0 ? 1 : 192;
 				for (int ia = 0; ia < na; ia++) {
 					double a = 2 * M_PI * ia / na, v[3];
 					for (int q = 0; q < 3; q++) v[q] = cos(rr) * bc[q] + sin(rr) * (cos(a) * e1[q] + sin(a) * e2[q]);
@@ -350,12 +353,30 @@ std::vector<gf_pencil_beam> pixelize_region(const gf_footprint &fp, double dx)
 		if (!any) continue;
 		const int i0 = std::max(0, (int)floor((xmin - x0) / dx) - 2), i1 = std::min(n, (int)floor((xmax - x0) / dx) + 3);
 		const int j0 = std::max(0, (int)floor((ymin - y0) / dx) - 2), j1 = std::min(n, (int)floor((ymax - y0) / dx) + 3);
+		// a beam's cells well inside or well outside the cap need no samples:
+		// within a cell of the pole grids whose corners are all at plane
+		// radius < sqrt 2 the unit vector moves by at most sqrt 2 times the
+		// plane distance, and no sample is farther than dx / sqrt 2 from the
+		// centre, so every sample's cap product is within dx of the centre's
+		const bool beam = f.kind == 2 && f.bmin <= 0. && f.hole <= 0.;
+		const double margin = 1.01 * dx + 1e-9;
 		for (int i = i0; i < i1; i++)
 			for (int j = j0; j < j1; j++) {
 				double xa = x0 + i * dx, ya = y0 + j * dx;
 				double fx = (xa > 0 || xa + dx < 0) ? std::min(fabs(xa), fabs(xa + dx)) : 0.;
 				double fy = (ya > 0 || ya + dx < 0) ? std::min(fabs(ya), fabs(ya + dx)) : 0.;
 				if (fx * fx + fy * fy > 2.0) continue;
+				if (beam) {
+					const double gx = std::max(fabs(xa), fabs(xa + dx)), gy = std::max(fabs(ya), fabs(ya + dx));
+					if (gx * gx + gy * gy < 2.0 - 1e-9) {
+						const double xc = xa + 0.5 * dx, yc = ya + 0.5 * dx, r2 = xc * xc + yc * yc;
+						const double sc = sqrt(1.0 - 0.25 * r2), z = 1.0 - 0.5 * r2;
+						const double v0 = hemi == 0 ? -xc * sc : xc * sc, v1 = -yc * sc, v2 = hemi == 0 ? z : -z;
+						const double dc = v0 * (f.cb * f.cl) + v1 * (f.cb * f.sl) + v2 * f.sb;
+						if (dc - f.cosr > margin) { out.push_back(cell_pixel(hemi, n, i, j, dx, 64)); continue; }
+						if (f.cosr - dc > margin) continue;
+					}
+				}
 				int inside = 0;
 				for (int a = 0; a < 8; a++)
 					for (int b2 = 0; b2 < 8; b2++)
@@ -1825,7 +1846,7 @@ struct SReq {
 	std::vector<gf_pencil_beam> pix;
 	gf_skygen_params sp{};
 	uint32_t S = 0;
-	std::vector<uint32_t> state;   // [a | c | x] of S streams
+	uint32_t seed32 = 0;           // the seed of its S streams (seeded on the device with its batch)
 	uint64_t nunits_model = 0;     // npix * K of the job's binning: the size of the stream table
 	std::vector<gf_skygen_params> msp;   // per model: the binning with its absolute-magnitude range
 	std::vector<uint64_t> munits;        // per model: npix * K of its binning
@@ -1838,7 +1859,7 @@ struct SReq {
 
 struct Worker {
 	hipStream_t st = nullptr;
-	DevBuf W, counts, offs, pixsum, segexp, pix, state, segs, segall, reqs, table, pos, zu, vis, extbeam;
+	DevBuf W, counts, offs, pixsum, segexp, pix, state, seeds, segs, segall, reqs, table, pos, zu, vis, extbeam;
 	uint64_t table_cap = 0;
 	gf_table T{};
 	std::thread th;
@@ -1850,12 +1871,13 @@ struct Worker {
 	uint64_t held() const
 	{
 		return W.bytes + counts.bytes + offs.bytes + pixsum.bytes + segexp.bytes + pix.bytes + state.bytes +
-		       segs.bytes + segall.bytes + reqs.bytes + table.bytes + pos.bytes + zu.bytes + vis.bytes + extbeam.bytes;
+		       seeds.bytes + segs.bytes + segall.bytes + reqs.bytes + table.bytes + pos.bytes + zu.bytes + vis.bytes +
+		       extbeam.bytes;
 	}
 	void release()
 	{
-		for (DevBuf *b : {&W, &counts, &offs, &pixsum, &segexp, &pix, &state, &segs, &segall, &reqs, &table, &pos,
-		                  &zu, &vis, &extbeam})
+		for (DevBuf *b : {&W, &counts, &offs, &pixsum, &segexp, &pix, &state, &seeds, &segs, &segall, &reqs, &table,
+		                  &pos, &zu, &vis, &extbeam})
 			b->free();
 		table_cap = 0;
 		T = gf_table{};
@@ -1873,6 +1895,7 @@ struct gf_server {
 	gf_result_fn fn = nullptr;
 	void *user = nullptr;
 	DevBuf dmodels, dlfP, dchain;
+	DevBuf dprimes;                  // the stream multipliers, for seeding requests' streams on the device
 	std::vector<gf_model> hmodels;
 	gf_serve_chain hchain{};
 	std::mutex mu;
@@ -1926,9 +1949,7 @@ void prepare(gf_server &s, SReq &q)
 	q.S = (uint32_t)std::min<uint64_t>(65536, want);
 	q.nI = 0;
 	for (uint64_t u : q.munits) q.nI += (u + q.S - 1) / q.S;
-	q.state.resize(3 * (size_t)q.S);
-	gf_mwc_seed_host(gf_seed_from_pipeline_seed(q.r.seed), s.primes.data(), (uint32_t)s.primes.size(), q.S,
-	                 q.state.data());
+	q.seed32 = gf_seed_from_pipeline_seed(q.r.seed);
 }
 
 // The server table's columns (the catalog table of gf_table)
@@ -1989,12 +2010,23 @@ template <typename T> void grow(Worker &w, uint64_t share, DevBuf &b, size_t n)
 // Device bytes of one request in a batch apart from its rows: per unit of
 // every model its weight, count, offset and emission base (8 + 4 + 8 + 8 B),
 // per pixel the pencil beam and its partial sum, per stream its state, its
-// segments and request record, and its pixels' extinction minima.
+// seeding record, segments and request record, and its pixels' extinction
+// minima.
 uint64_t request_fixed_bytes(const gf_server &s, const SReq &q)
 {
 	const uint64_t nm = s.hmodels.size();
 	return q.nunits * 28 + (uint64_t)q.pix.size() * (sizeof(gf_pencil_beam) + 8) + (uint64_t)q.S * 12 +
-	       nm * (2 * sizeof(gf_serve_seg) + 8) + sizeof(gf_serve_req) + 16 + q.extmin.size() * 4;
+	       sizeof(gf_seed_seg) + nm * (2 * sizeof(gf_serve_seg) + 8) + sizeof(gf_serve_req) + 16 +
+	       q.extmin.size() * 4;
+}
+
+// Device bytes of a batch beyond its requests' own: each buffer's spare end
+// element and the table's column pitches (rounded up to 128 bytes).
+uint64_t batch_slack(const gf_server &s)
+{
+	uint64_t planes = 0;
+	for (auto &d : table_defs(s)) planes += d.width;
+	return planes * 128 + 4096;
 }
 
 // Device bytes per output row: the table's columns and the module chain's
@@ -2040,13 +2072,20 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	int nr = (int)batch.size();
 	const uint64_t share = worker_share(s);
 	const uint64_t rowb = row_device_bytes(s);
+	// the batch's admitted size; the buffers of earlier batches are released
+	// first when they would not leave room for it.  The buffers other than
+	// the table grow with headroom only while that leaves the share room for
+	// the table of the admitted rows.
+	uint64_t reserve = batch_slack(s);
 	{
-		// the batch's admitted size; the buffers of earlier batches are
-		// released first when they would not leave room for it
 		uint64_t need = 0;
-		for (auto &q : batch) need += request_fixed_bytes(s, *q) + admission_rows(*q) * rowb;
+		for (auto &q : batch) {
+			need += request_fixed_bytes(s, *q) + admission_rows(*q) * rowb;
+			reserve += admission_rows(*q) * rowb;
+		}
 		if (w.held() + need > share) w.release();
 	}
+	const uint64_t fshare = share > reserve ? share - reserve : 0;
 	// unit layout: request-major, model-minor
 	std::vector<uint64_t> uoff(nr + 1, 0);
 	std::vector<uint32_t> poff(nr + 1, 0), loff(nr + 1, 0);
@@ -2058,29 +2097,38 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	const uint64_t Ucand = uoff[nr];
 	std::vector<gf_pencil_beam> pix;
 	pix.reserve(poff[nr]);
-	std::vector<uint32_t> state;
+	// the requests' stream tables, seeded on the device (gf_mwc_seed_host's
+	// tables, 3 S words each)
 	std::vector<size_t> soff(nr + 1, 0);
+	std::vector<gf_seed_seg> seeds(nr);
+	uint32_t maxS = 0;
 	for (int r = 0; r < nr; r++) {
 		pix.insert(pix.end(), batch[r]->pix.begin(), batch[r]->pix.end());
-		soff[r + 1] = soff[r] + batch[r]->state.size();
+		soff[r + 1] = soff[r] + 3 * (size_t)batch[r]->S;
+		seeds[r] = gf_seed_seg{batch[r]->seed32, batch[r]->S, (uint64_t)soff[r]};
+		maxS = std::max(maxS, batch[r]->S);
 	}
-	state.resize(soff[nr]);
-	for (int r = 0; r < nr; r++) std::copy(batch[r]->state.begin(), batch[r]->state.end(), state.begin() + soff[r]);
 	if (s.base.has_ext)
 		for (size_t g = 0; g < pix.size(); g++) pix[g].extIdx = (int32_t)g;   // its texel of the batch's minima
-	grow<gf_pencil_beam>(w, share, w.pix, std::max<size_t>(pix.size(), 1));
-	grow<uint32_t>(w, share, w.state, std::max<size_t>(state.size(), 1));
-	grow<double>(w, share, w.W, Ucand + 1);
-	grow<uint32_t>(w, share, w.counts, Ucand + 1);
-	grow<uint64_t>(w, share, w.offs, Ucand + 2);
-	grow<double>(w, share, w.pixsum, (size_t)poff[nr] + 1);
-	grow<double>(w, share, w.segexp, (size_t)nr * nm + 1);
-	grow<gf_serve_seg>(w, share, w.segs, (size_t)nr * nm + 1);
-	grow<gf_serve_seg>(w, share, w.segall, (size_t)nr * nm + 1);
-	grow<gf_serve_req>(w, share, w.reqs, (size_t)nr + 1);
+	grow<gf_pencil_beam>(w, fshare, w.pix, std::max<size_t>(pix.size(), 1));
+	grow<uint32_t>(w, fshare, w.state, std::max<size_t>(soff[nr], 1));
+	grow<gf_seed_seg>(w, fshare, w.seeds, std::max<size_t>(seeds.size(), 1));
+	grow<double>(w, fshare, w.W, Ucand + 1);
+	grow<uint32_t>(w, fshare, w.counts, Ucand + 1);
+	grow<uint64_t>(w, fshare, w.offs, Ucand + 2);
+	grow<double>(w, fshare, w.pixsum, (size_t)poff[nr] + 1);
+	grow<double>(w, fshare, w.segexp, (size_t)nr * nm + 1);
+	grow<gf_serve_seg>(w, fshare, w.segs, (size_t)nr * nm + 1);
+	grow<gf_serve_seg>(w, fshare, w.segall, (size_t)nr * nm + 1);
+	grow<gf_serve_req>(w, fshare, w.reqs, (size_t)nr + 1);
 	GF_HIP(hipMemcpyAsync(w.pix.p, pix.data(), pix.size() * sizeof(gf_pencil_beam), hipMemcpyHostToDevice, st));
-	GF_HIP(hipMemcpyAsync(w.state.p, state.data(), state.size() * 4, hipMemcpyHostToDevice, st));
+	GF_HIP(hipMemcpyAsync(w.seeds.p, seeds.data(), seeds.size() * sizeof(gf_seed_seg), hipMemcpyHostToDevice, st));
 	uint32_t *dstate = (uint32_t *)w.state.p;
+	{
+		int rc = gf::mwc_seed_streams((const gf_seed_seg *)w.seeds.p, (uint32_t)nr, maxS,
+		                              (const uint32_t *)s.dprimes.p, dstate, st);
+		if (rc) return rc;
+	}
 	// segments per model (launch order: requests), and all segments in row order
 	std::vector<gf_serve_seg> segs((size_t)nm * nr), segall((size_t)nr * nm);
 	for (int r = 0; r < nr; r++) {
@@ -2114,7 +2162,7 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	gf_serve_seg *dsegs = (gf_serve_seg *)w.segs.p;
 	gf_serve_seg *dsegall = (gf_serve_seg *)w.segall.p;
 	const uint64_t *jtab = (const uint64_t *)s.base.jtab.p;
-	grow<uint64_t>(w, share, w.zu, Ucand + 1);
+	grow<uint64_t>(w, fshare, w.zu, Ucand + 1);
 	const gf_model *dmodels = (const gf_model *)s.dmodels.p;
 	const gf_pencil_beam *dpix = (const gf_pencil_beam *)w.pix.p;
 	// the batch's extinction minima: pixel g of the batch at texel
@@ -2127,7 +2175,7 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 		for (int r = 0; r < nr; r++) minima.insert(minima.end(), batch[r]->extmin.begin(), batch[r]->extmin.end());
 		Cube beam;
 		beam_texture(beam, minima, poff[nr], ref.nz, ref.tc[2]);
-		grow<float>(w, share, w.extbeam, beam.v.size());
+		grow<float>(w, fshare, w.extbeam, beam.v.size());
 		GF_HIP(hipMemcpyAsync(w.extbeam.p, beam.v.data(), beam.v.size() * 4, hipMemcpyHostToDevice, st));
 		ext = s.base.ext;
 		ext.beam = tex3d_of(beam, (const float *)w.extbeam.p);
@@ -2147,7 +2195,7 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	// candidates whose fixed bytes and bounded rows fit the share (units,
 	// pixels and streams are request-major, so a prefix keeps its layout)
 	{
-		uint64_t used = 0;
+		uint64_t used = batch_slack(s);
 		int k = 0;
 		for (int r = 0; r < nr; r++) {
 			SReq &q = *batch[r];
@@ -2219,7 +2267,7 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	// the streams move past every unit before the module chain draws from them
 	rc = gf::serve_advance((const gf_serve_req *)w.reqs.p, nr, loff[nr], jtab, st);
 	if (rc) return rc;
-	grow<uint2>(w, share, w.pos, std::max<size_t>(1, (size_t)s.hchain.nplanes * nrows));
+	grow<uint2>(w, fshare, w.pos, std::max<size_t>(1, (size_t)s.hchain.nplanes * nrows));
 	if (s.cfg.eager_chain) {
 		rc = gf::serve_chain((const gf_serve_req *)w.reqs.p, nr, loff[nr], (const gf_serve_chain *)s.dchain.p,
 		                     s.hchain, nrows, w.T, (uint2 *)w.pos.p, st);
@@ -2240,7 +2288,7 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	}
 	uint64_t maxn = 0;
 	for (int r = 0; r < nr; r++) maxn = std::max(maxn, roff[r + 1] - roff[r]);
-	grow<unsigned long long>(w, share, w.vis, (size_t)nr + 1);
+	grow<unsigned long long>(w, fshare, w.vis, (size_t)nr + 1);
 	rc = gf::serve_visible((const gf_serve_req *)w.reqs.p, nr, maxn, w.T.hidden, (unsigned long long *)w.vis.p, st);
 	if (rc) return rc;
 	std::vector<unsigned long long> visv(nr);
@@ -2347,11 +2395,11 @@ void worker_loop(gf_server &s, Worker &w)
 			// earlier batch counted them, else bounded from their area) fit the
 			// worker's share of the device budget; a counted request that does
 			// not fit the share alone is rejected
-			uint64_t used = 0;
+			uint64_t used = batch_slack(s);
 			while (!src.empty() && (int)batch.size() < std::max(1, s.cfg.max_batch)) {
 				SReq &q = *src.front();
 				const uint64_t need = request_fixed_bytes(s, q) + admission_rows(q) * rowb;
-				if (q.counted >= 0 && need > share) {
+				if (q.counted >= 0 && need + batch_slack(s) > share) {
 					rejected.push_back(std::move(src.front()));
 					src.pop_front();
 					continue;
@@ -2432,6 +2480,12 @@ int gf_server_create(const char *cmd_conf, const char *datadir, const gf_server_
 		if (!f) { g_last_error = "cannot read safeprimes32.u32"; return GF_E_CONFIG; }
 		f.read((char *)s->primes.data(), s->primes.size() * 4);
 		s->primes.resize(f.gcount() / 4);
+	}
+	try {
+		s->dprimes.upload(s->primes);
+	} catch (const ConfigError &e) {
+		g_last_error = e.msg;
+		return GF_E_CONFIG;
 	}
 	// models and their LF prefix tables (device)
 	for (auto &m : c.models) s->hmodels.push_back(m->m);

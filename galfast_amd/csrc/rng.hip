@@ -16,10 +16,47 @@ struct Rand48 {
 };
 
 // (int)v with the x86-64 cvttsd2si result for out-of-range values.
-inline int32_t trunc_to_int_x86(double v)
+__host__ __device__ inline int32_t trunc_to_int_x86(double v)
 {
 	if (!(v > -2147483649.0 && v < 2147483648.0)) return INT32_MIN;
 	return (int32_t)v;
+}
+
+// The drand48 state n steps after X: the n-th power of the step
+// X <- (0x5DEECE66D X + 0xB) mod 2^48, formed by squaring (the step k times
+// is X <- (A X + C) mod 2^48; twice that is A^2 and C (A + 1)).  Products
+// are taken mod 2^64, whose low 48 bits are the products mod 2^48.
+__device__ inline uint64_t rand48_skip(uint64_t X, uint64_t n)
+{
+	const uint64_t M48 = 0xFFFFFFFFFFFFull;
+	uint64_t A = 0x5DEECE66Dull, C = 0xBull;
+	for (; n; n >>= 1) {
+		if (n & 1u) X = (A * X + C) & M48;
+		C = (C * (A + 1u)) & M48;
+		A = (A * A) & M48;
+	}
+	return X;
+}
+
+// gf_mwc_seed_host on the device: stream i of table g takes the generator's
+// draws 2i+1 (its carry, drand48 times its multiplier) and 2i+2 (its value,
+// lrand48), so every lane jumps to its own pair.  Block row y seeds table
+// segs[y].
+__global__ void k_seed_streams(const gf_seed_seg *segs, const uint32_t *primes, uint32_t *out)
+{
+	const gf_seed_seg g = segs[blockIdx.y];
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= g.nstreams) return;
+	const uint64_t M48 = 0xFFFFFFFFFFFFull;
+	uint64_t X = rand48_skip(((uint64_t)g.seed << 16) | 0x330Eull, 2 * (uint64_t)i);
+	X = (0x5DEECE66Dull * X + 0xBull) & M48;
+	const double u = (double)X * (1.0 / 281474976710656.0);
+	X = (0x5DEECE66Dull * X + 0xBull) & M48;
+	const uint32_t a = primes[i];
+	uint32_t *t = out + g.offset;
+	t[i] = a;
+	t[g.nstreams + i] = (uint32_t)trunc_to_int_x86(u * (double)a);
+	t[2 * (size_t)g.nstreams + i] = (uint32_t)(X >> 17);
 }
 
 __global__ void mwc_sample_kernel(uint32_t *state, uint32_t ns, int kind, float param, uint32_t n, void *out)
@@ -74,6 +111,16 @@ int mwc_jump_table(const gf_rng *rng, uint64_t *tab, hipStream_t st)
 	return 0;
 }
 
+int mwc_seed_streams(const gf_seed_seg *segs_dev, uint32_t nsegs, uint32_t max_streams, const uint32_t *primes,
+                     uint32_t *out, hipStream_t st)
+{
+	GF_CHECK(segs_dev && primes && out && nsegs <= 65535u, GF_E_ARG);
+	if (!nsegs || !max_streams) return 0;
+	k_seed_streams<<<dim3((max_streams + 255) / 256, nsegs), 256, 0, st>>>(segs_dev, primes, out);
+	GF_LAUNCHED();
+	return 0;
+}
+
 int mwc_jump(gf_rng *rng, uint64_t n, const uint64_t *tab, hipStream_t st)
 {
 	GF_CHECK(rng && rng->state && tab, GF_E_ARG);
@@ -111,6 +158,23 @@ int gf_mwc_seed_host(uint32_t seed, const uint32_t *primes, uint32_t nprimes,
 		out_state[2 * nstreams + i] = r.lrand();
 	}
 	return (int)nstreams;
+}
+
+int gf_mwc_seed_device(uint32_t seed, const uint32_t *primes, uint32_t nprimes, uint32_t nstreams, uint32_t *state,
+                       void *stream)
+{
+	GF_CHECK(primes && state && nstreams > 0 && nstreams % 64 == 0 && nstreams <= nprimes, GF_E_ARG);
+	hipStream_t st = gf::as_stream(stream);
+	void *p;
+	int rc = gf::scratch(GF_SLOT_SEED, sizeof(gf_seed_seg), &p);
+	if (rc) return rc;
+	const gf_seed_seg g{seed, nstreams, 0};
+	GF_HIP(hipMemcpyAsync(p, &g, sizeof g, hipMemcpyHostToDevice, st));
+	rc = gf::mwc_seed_streams((const gf_seed_seg *)p, 1, nstreams, primes, state, st);
+	if (rc) return rc;
+	// the copy reads g on this stack frame
+	GF_HIP(hipStreamSynchronize(st));
+	return 0;
 }
 
 int gf_rng_alloc(gf_rng *rng, uint32_t nstreams)

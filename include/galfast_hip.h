@@ -110,6 +110,14 @@ uint32_t gf_seed_from_pipeline_seed(uint32_t pipeline_seed);
 int gf_mwc_seed_host(uint32_t seed, const uint32_t *primes, uint32_t nprimes,
                      uint32_t nstreams, uint32_t *out_state);
 
+/* mwc_impl::srand (common/cuda_rng.h:466-497) on the device: the same
+ * tables as gf_mwc_seed_host, each stream's two drand48/lrand48 draws
+ * reached by jump-ahead, so that every stream is seeded by its own lane.
+ * primes and state are device memory; nstreams must be a multiple of 64
+ * and at most nprimes; state receives 3*nstreams words [a | c | x]. */
+int gf_mwc_seed_device(uint32_t seed, const uint32_t *primes, uint32_t nprimes,
+                       uint32_t nstreams, uint32_t *state, void *stream);
+
 /* Allocate / copy device stream state (rng_base::upload/download,
  * common/cuda_rng.h:74-103). */
 int gf_rng_alloc(gf_rng *rng, uint32_t nstreams);

@@ -65,3 +65,34 @@ def test_device_jump_equals_oracle():
         G.check(L.gf_mwc_jump(C.byref(d.rng), n, stream_ptr()), "gf_mwc_jump")
         O.lib().gfo_mwc_jump(O.ptr(ref), 65536, n)
         assert np.array_equal(d.state(), ref), n
+
+
+@pytest.mark.parametrize("seed,ns", [(0, 64), (1234, 4096), (16777215, 65536), (2 ** 32 - 1, 70720)])
+def test_device_seeding_equals_host_seeding(seed, ns):
+    """gf_mwc_seed_device (every stream's drand48/lrand48 pair reached by
+    jump-ahead, a lane per stream) writes gf_mwc_seed_host's tables and the
+    oracle's, including the multipliers above 2^31 (their carries overflow
+    to INT32_MIN) at the end of the prime list."""
+    from gfx import PRIMES
+    T = torch()
+    host = np.zeros(3 * ns, dtype=np.uint32)
+    assert G.lib().gf_mwc_seed_host(seed, PRIMES.ctypes.data_as(C.POINTER(C.c_uint32)), len(PRIMES), ns,
+                                    host.ctypes.data_as(C.POINTER(C.c_uint32))) == ns
+    assert np.array_equal(host, mwc_state(seed, ns))
+    primes = T.from_numpy(PRIMES.view(np.int32).copy()).cuda()
+    out = T.zeros(3 * ns, dtype=T.int32, device="cuda")
+    G.check(G.lib().gf_mwc_seed_device(seed, primes.data_ptr(), len(PRIMES), ns, out.data_ptr(), stream_ptr()),
+            "gf_mwc_seed_device")
+    sync()
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), host)
+
+
+def test_device_seeding_rejects_bad_counts():
+    from gfx import PRIMES
+    T = torch()
+    primes = T.from_numpy(PRIMES.view(np.int32).copy()).cuda()
+    out = T.zeros(3 * 128, dtype=T.int32, device="cuda")
+    L = G.lib()
+    assert L.gf_mwc_seed_device(1, primes.data_ptr(), len(PRIMES), 100, out.data_ptr(), stream_ptr()) != 0
+    assert L.gf_mwc_seed_device(1, primes.data_ptr(), 64, 128, out.data_ptr(), stream_ptr()) != 0
+    assert L.gf_mwc_seed_device(1, None, len(PRIMES), 64, out.data_ptr(), stream_ptr()) != 0

@@ -64,6 +64,10 @@ def oracle_pixels(l, b, r, dx, rect=None):
     (10.0, 45.0, 3.0),       # one hemisphere only
     (100.0, 25.0, 30.0),     # a wide cap: the equator crossings lie between boundary samples
     (250.0, -28.0, 30.5),    # and from the south
+    (140.0, 62.0, 5.642),    # 100 deg^2: most cells wholly inside, counted without samples
+    (33.0, -71.0, 0.0564),   # 0.01 deg^2, smaller than a cell
+    (5.0, 87.0, 6.0),        # over the north pole
+    (181.0, -86.5, 9.0),     # over the south pole
 ])
 def test_beam_pixels_equal_grid_coverage(l, b, r):
     dx = 0.25

@@ -63,14 +63,17 @@ def stage_bytes(nbands, nabsmag, present):
     npos = sum(1 for k in ("k_chain_rows<FEH>", "k_chain_rows<BOND>", "k_chain_rows<PERR>") if k in present)
     # draw pass: read hidden + comp of every row; per visible row write the
     # 8-byte stream position of each random-number module; unresolved
-    # multiples run whole in it: read absmag, write absmagSys, its count and
-    # the system's absmag
-    draws = 4 + 4 + 8 * npos + ((4 + 4 * nabsmag + 4 + 4) if mult else 0)
+    # multiples draw their companions in it: read absmag, write absmagSys
+    # and its count (and, with more than two planes, the system's absmag)
+    split = nabsmag == 2
+    draws = 4 + 4 + 8 * npos + ((4 + 4 * nabsmag + 4 + (0 if split else 4)) if mult else 0)
     mr = 4 * nabsmag if mult else 4
     return {
         "k_chain_draws": (draws, 4 + 4),
-        # metallicity rows: read hidden, comp, XYZ, 1 position; write FeH
-        "k_chain_rows<FEH>": (4 + 4 + 12 + 8 + 4, 4),
+        # metallicity rows: read hidden, comp, XYZ, 1 position; write FeH;
+        # with two magnitude planes also the system's magnitude: read
+        # absmagSys and its count, write absmag
+        "k_chain_rows<FEH>": (4 + 4 + 12 + 8 + 4 + ((4 * nabsmag + 4 + 4) if mult and split else 0), 4),
         # kinematics rows: read hidden, comp, XYZ, 1 position; write vcyl
         "k_chain_rows<BOND>": (4 + 4 + 12 + 8 + 12, 4),
         # photometry rows: read hidden, comp, FeH, the absolute magnitude(s),
@@ -210,7 +213,8 @@ def _serve_once(conf, reqs, rate, warm, **kw):
     return {"requests": len(res), "failed": sum(1 for x in res if x.status != 0), "stars": stars,
             "rows": sum(x.generated for x in res), "wall_s": wall, "stars_per_s": stars / wall,
             "p50_ms": 1000.0 * float(np.percentile(lat, 50)), "p99_ms": 1000.0 * float(np.percentile(lat, 99)),
-            "mean_batch": len(res) / max(nb, 1), "host_prep_s": st.t_prep, "worker_batch_s": st.t_batch}
+            "mean_batch": len(res) / max(nb, 1), "host_prep_s": st.t_prep, "worker_batch_s": st.t_batch,
+            "worker_device_wait_s": st.t_wait, "worker_deliver_s": st.t_deliver}
 
 
 def _plain_jobs(reqs, seed, tmpdir):

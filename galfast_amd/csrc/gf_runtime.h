@@ -94,6 +94,11 @@ int count_visible(gf_col hidden, uint64_t n, unsigned long long *counter, hipStr
 
 constexpr int GF_CHAIN_LAUNCHES = 6;   // the draw pass and the row pass's five groups
 
+// Rows of the largest table a catalog chunk or a server batch may use: a
+// column's pitch (gf_col.pitch, 32 bits) must hold its 8-byte planes, rounded
+// up to 128 bytes.
+constexpr uint64_t GF_MAX_TABLE_ROWS = ((1ull << 32) - 128) / 8;
+
 enum gf_scratch_slot {
 	GF_SLOT_LFTAB = 0,
 	GF_SLOT_PARTIALS,

@@ -1466,7 +1466,10 @@ int gf_catalog_set(gf_catalog *c, const char *key, double value)
 {
 	GF_CHECK(c && key, GF_E_ARG);
 	std::string k = key;
-	if (k == "chunk_rows") c->chunk_rows = (uint64_t)value;
+	if (k == "chunk_rows") {
+		if (!(value >= 1.0 && value <= (double)GF_MAX_TABLE_ROWS)) return GF_E_ARG;
+		c->chunk_rows = (uint64_t)value;
+	}
 	else if (k == "overlap") c->overlap = value != 0.;
 	else if (k == "draw") {
 		if (value != 0. && value != 1.) return GF_E_ARG;
@@ -1859,7 +1862,7 @@ struct SReq {
 
 struct Worker {
 	hipStream_t st = nullptr;
-	DevBuf W, counts, offs, pixsum, segexp, pix, state, seeds, segs, segall, reqs, table, pos, zu, vis, extbeam;
+	DevBuf W, counts, offs, pixsum, segexp, pix, state, seeds, segs, segall, reqs, table, pos, zu, vis, extbeam, roff;
 	uint64_t table_cap = 0;
 	gf_table T{};
 	std::thread th;
@@ -1872,12 +1875,12 @@ struct Worker {
 	{
 		return W.bytes + counts.bytes + offs.bytes + pixsum.bytes + segexp.bytes + pix.bytes + state.bytes +
 		       seeds.bytes + segs.bytes + segall.bytes + reqs.bytes + table.bytes + pos.bytes + zu.bytes + vis.bytes +
-		       extbeam.bytes;
+		       extbeam.bytes + roff.bytes;
 	}
 	void release()
 	{
 		for (DevBuf *b : {&W, &counts, &offs, &pixsum, &segexp, &pix, &state, &seeds, &segs, &segall, &reqs, &table,
-		                  &pos, &zu, &vis, &extbeam})
+		                  &pos, &zu, &vis, &extbeam, &roff})
 			b->free();
 		table_cap = 0;
 		T = gf_table{};
@@ -1902,7 +1905,7 @@ struct gf_server {
 	std::condition_variable cv_sub, cv_ready, cv_idle;
 	std::deque<std::unique_ptr<SReq>> submitted, ready;
 	uint64_t n_submitted = 0, n_completed = 0, n_batches = 0, n_rows = 0, n_visible = 0, batch_seq = 0;
-	double t_prep = 0, t_batch = 0;
+	double t_prep = 0, t_batch = 0, t_wait = 0, t_deliver = 0;
 	double t_first = 0, t_last = 0;
 	bool stop = false;
 	std::vector<std::thread> prep;
@@ -1980,9 +1983,10 @@ size_t table_bytes(const gf_server &s, uint64_t cap)
 void worker_table(gf_server &s, Worker &w, uint64_t cap, uint64_t share)
 {
 	if (w.table_cap >= cap && w.table.p) return;
-	// a quarter of headroom (at least 4,096 rows) when it stays within the share
+	// a quarter of headroom (at least 4,096 rows, at most GF_MAX_TABLE_ROWS)
+	// when it stays within the share
 	const uint64_t others = w.held() - w.table.bytes;
-	uint64_t want = std::max<uint64_t>(cap + cap / 4, 4096);
+	uint64_t want = std::min<uint64_t>(std::max<uint64_t>(cap + cap / 4, 4096), GF_MAX_TABLE_ROWS);
 	if (others + table_bytes(s, want) > share) want = cap;
 	cap = want;
 	w.table.alloc(table_bytes(s, cap));
@@ -2064,6 +2068,17 @@ uint64_t worker_share(const gf_server &s)
 // requests whose counted rows no longer fit the worker's share after the
 // first are moved to `deferred` (to go back to the queue, their count kept),
 // and a first request that does not fit the share alone to `rejected`.
+// A worker's wait for its stream, added to the server's device-wait time.
+int wait_stream(gf_server &s, hipStream_t st)
+{
+	const double t0 = server_clock();
+	GF_HIP(hipStreamSynchronize(st));
+	const double dt = server_clock() - t0;
+	std::lock_guard<std::mutex> lk(s.mu);
+	s.t_wait += dt;
+	return 0;
+}
+
 int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch,
               std::vector<std::unique_ptr<SReq>> &deferred, std::vector<std::unique_ptr<SReq>> &rejected)
 {
@@ -2180,7 +2195,7 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 		ext = s.base.ext;
 		ext.beam = tex3d_of(beam, (const float *)w.extbeam.p);
 		// the copy reads the host vector: wait before it goes out of scope
-		GF_HIP(hipStreamSynchronize(st));
+		if (int wrc = wait_stream(s, st)) return wrc;
 	}
 	// count
 	for (int m = 0; m < nm; m++) {
@@ -2190,19 +2205,24 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	}
 	std::vector<double> segexp((size_t)nm * nr);
 	GF_HIP(hipMemcpyAsync(segexp.data(), w.segexp.p, segexp.size() * 8, hipMemcpyDeviceToHost, st));
-	GF_HIP(hipStreamSynchronize(st));
+	if (int wrc = wait_stream(s, st)) return wrc;
 	// admission by the counted expectation: the longest prefix of the
 	// candidates whose fixed bytes and bounded rows fit the share (units,
 	// pixels and streams are request-major, so a prefix keeps its layout)
 	{
-		uint64_t used = batch_slack(s);
+		uint64_t used = batch_slack(s), rows = 0;
 		int k = 0;
 		for (int r = 0; r < nr; r++) {
 			SReq &q = *batch[r];
 			q.counted = 0;
 			for (int m = 0; m < nm; m++) q.counted += segexp[(size_t)m * nr + r];
 			const uint64_t need = request_fixed_bytes(s, q) + rows_bound(q.counted) * rowb;
-			if (k == r && used + need <= share) { used += need; k++; }
+			// and the prefix's rows fit one table
+			if (k == r && used + need <= share && rows + rows_bound(q.counted) <= GF_MAX_TABLE_ROWS) {
+				used += need;
+				rows += rows_bound(q.counted);
+				k++;
+			}
 		}
 		if (k == 0) {
 			rejected.push_back(std::move(batch[0]));
@@ -2235,17 +2255,24 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	rc = gf::exclusive_scan_u32_u64((const uint32_t *)w.counts.p, (uint64_t *)w.offs.p, U, st);
 	if (rc) return rc;
 	std::vector<uint64_t> roff(nr + 1);
-	for (int r = 0; r <= nr; r++)
-		GF_HIP(hipMemcpyAsync(&roff[r], (const uint64_t *)w.offs.p + uoff[r], 8, hipMemcpyDeviceToHost, st));
-	GF_HIP(hipStreamSynchronize(st));
+	grow<uint64_t>(w, fshare, w.roff, (size_t)nr + 1);
+	rc = gf::serve_req_rows(dsegall, nr, nm, (const uint64_t *)w.offs.p, U, (uint64_t *)w.roff.p, st);
+	if (rc) return rc;
+	GF_HIP(hipMemcpyAsync(roff.data(), w.roff.p, roff.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+	if (int wrc = wait_stream(s, st)) return wrc;
 	for (int r = 0; r < nr; r++) batch[r]->expected = batch[r]->counted;
 	const uint64_t nrows = roff[nr];
-	if (nrows >= (1ull << 32)) return GF_E_BUDGET;
+	if (nrows > GF_MAX_TABLE_ROWS) return GF_E_BUDGET;
 	// the rows drawn past their bound (a ~1e-9 event) may not fit the share
 	if (w.held() - w.table.bytes - w.pos.bytes + table_bytes(s, nrows) +
 	    (uint64_t)s.hchain.nplanes * nrows * sizeof(uint2) > share)
 		return GF_E_BUDGET;
-	worker_table(s, w, nrows, share);
+	{
+		// the table's headroom leaves room for the stream positions
+		const uint64_t posb = (uint64_t)s.hchain.nplanes * nrows * sizeof(uint2);
+		const uint64_t need_pos = w.pos.bytes >= posb ? 0 : posb - w.pos.bytes;
+		worker_table(s, w, nrows, share > need_pos ? share - need_pos : 0);
+	}
 	// row0/row1 of the row-ordered table from the scanned offsets
 	rc = gf::serve_seg_rows(dsegall, nr * nm, (const uint64_t *)w.offs.p, st);
 	if (rc) return rc;
@@ -2293,8 +2320,9 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	if (rc) return rc;
 	std::vector<unsigned long long> visv(nr);
 	GF_HIP(hipMemcpyAsync(visv.data(), w.vis.p, (size_t)nr * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-	GF_HIP(hipStreamSynchronize(st));
+	if (int wrc = wait_stream(s, st)) return wrc;
 	// results
+	const double td0 = server_clock();
 	uint64_t vis_total = 0;
 	for (int r = 0; r < nr; r++) {
 		SReq &q = *batch[r];
@@ -2323,6 +2351,7 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	}
 	{
 		std::lock_guard<std::mutex> lk(s.mu);
+		s.t_deliver += server_clock() - td0;
 		s.n_batches++;
 		s.n_rows += nrows;
 		s.n_visible += vis_total;
@@ -2576,6 +2605,8 @@ int gf_server_stats_get(gf_server *s, gf_server_stats *st)
 	st->t_last_done = s->t_last;
 	st->t_prep = s->t_prep;
 	st->t_batch = s->t_batch;
+	st->t_wait = s->t_wait;
+	st->t_deliver = s->t_deliver;
 	return 0;
 }
 

@@ -57,6 +57,11 @@ int serve_count(const gf_serve_seg *dsegs, int nseg, uint32_t nslots, const gf_m
 int serve_plan(const gf_serve_seg *dsegs, int nseg, uint64_t nunits, const double *W, const uint64_t *jtab,
                uint32_t *counts, uint64_t *zu, hipStream_t st);
 int serve_seg_rows(gf_serve_seg *dsegs, int nseg, const uint64_t *offs, hipStream_t st);
+// roff[r] (r <= nreq): request r's first row, roff[nreq] the batch's rows,
+// from the plan's scanned offsets (segall: nm segments per request,
+// request-major; nunits: the batch's units)
+int serve_req_rows(const gf_serve_seg *dsegall, int nreq, int nm, const uint64_t *offs, uint64_t nunits,
+                   uint64_t *roff, hipStream_t st);
 // Emission: lane per batch row over all segments (in row order).
 int serve_emit(const gf_serve_seg *dsegs, int nseg, const gf_model *dmodels, const gf_extinction &ext,
                const gf_pencil_beam *dpix,

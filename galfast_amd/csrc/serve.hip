@@ -178,6 +178,17 @@ __global__ void k_b_seg_rows(gf_serve_seg *segs, int nseg, const uint64_t *offs)
 	segs[si].row1 = offs[segs[si].unit0 + segs[si].nunits];
 }
 
+// roff[r] = the first row of request r (its first segment's first unit's
+// offset), roff[nreq] = offs[nunits], the batch's rows.  segall holds nm
+// segments per request, request-major.
+__global__ void k_b_req_rows(const gf_serve_seg *segall, int nreq, int nm, const uint64_t *offs, uint64_t nunits,
+                             uint64_t *roff)
+{
+	int r = blockIdx.x * blockDim.x + threadIdx.x;
+	if (r > nreq) return;
+	roff[r] = offs[r < nreq ? segall[(size_t)r * nm].unit0 : nunits];
+}
+
 // ---------------------------------------------------------------------------
 // Module chain over requests: lane (request, s) owns the request's 256-row
 // chunks s, s + S, s + 2S, ... and runs the draw pass on them in that order
@@ -318,6 +329,14 @@ int serve_plan(const gf_serve_seg *dsegs, int nseg, uint64_t nunits, const doubl
 {
 	if (!nunits) return 0;
 	k_b_plan_units<<<gf::blocks_for(nunits, SB), SB, 0, st>>>(dsegs, nseg, nunits, W, jtab, counts, zu);
+	GF_LAUNCHED();
+	return 0;
+}
+
+int serve_req_rows(const gf_serve_seg *dsegall, int nreq, int nm, const uint64_t *offs, uint64_t nunits,
+                   uint64_t *roff, hipStream_t st)
+{
+	k_b_req_rows<<<gf::blocks_for((uint64_t)nreq + 1, 64), 64, 0, st>>>(dsegall, nreq, nm, offs, nunits, roff);
 	GF_LAUNCHED();
 	return 0;
 }

@@ -25,7 +25,8 @@ class gf_server_config(C.Structure):
 class gf_server_stats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("batches", C.c_uint64),
                 ("rows", C.c_uint64), ("visible", C.c_uint64), ("t_first_submit", C.c_double),
-                ("t_last_done", C.c_double), ("t_prep", C.c_double), ("t_batch", C.c_double)]
+                ("t_last_done", C.c_double), ("t_prep", C.c_double), ("t_batch", C.c_double),
+                ("t_wait", C.c_double), ("t_deliver", C.c_double)]
 
 
 RESULT_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(gf_result), C.POINTER(gf_table), C.c_uint64, C.c_uint64)

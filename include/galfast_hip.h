@@ -795,6 +795,8 @@ typedef struct gf_server_stats {
     double   t_first_submit, t_last_done;
     double   t_prep;    /* host seconds spent preparing requests (all threads) */
     double   t_batch;   /* seconds workers spent running batches (all workers) */
+    double   t_wait;    /* of which waiting for their device streams */
+    double   t_deliver; /* of which handing results to the callback */
 } gf_server_stats;
 
 typedef struct gf_server gf_server;

@@ -152,6 +152,21 @@ def test_c0_capacity_limited_reference_draw(c0):
     assert a.shape == b.shape and np.array_equal(a, b)
 
 
+def test_chunk_rows_bounded_by_the_column_pitch(c0):
+    """A chunk's 8-byte columns must keep a 32-bit plane pitch: chunk_rows
+    above ((2^32 - 128) / 8) rows (and below one) is refused, the limit
+    itself is accepted."""
+    cat = Catalog(c0, seed=7)
+    try:
+        limit = ((1 << 32) - 128) // 8
+        cat.set("chunk_rows", limit)
+        for bad in (limit + 1, 1 << 32, 0):
+            with pytest.raises(G.GalfastError):
+                cat.set("chunk_rows", bad)
+    finally:
+        cat.close()
+
+
 def test_c0_generated_matches_expected(c0):
     cat = Catalog(c0, seed=7)
     cat.count()

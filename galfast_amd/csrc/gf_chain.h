@@ -32,12 +32,18 @@ constexpr uint8_t HIDDEN = 0xFF, NOCOMP = 0xFE;
 constexpr int PHOT_LONE = 8;   // photometry launch variant flag (rows_phot)
 
 // LDS of one draw-pass workgroup: a tile of two 32-bit values per lane, and
-// the (hidden, comp) code of every row of each lane's chunk.
-struct DrawLds {
+// the (hidden, comp) code of ROWS rows of each lane's chunk: all MAXC rows
+// when the chunk's codes are read once for every module (CACHED walks,
+// 96 KiB, one workgroup per CU), or the current tile's (16 KiB less per
+// lane: four workgroups per CU, for launches with more chunks than lanes).
+template <int ROWS>
+struct DrawLdsN {
 	uint32_t a[TILE][BS];
 	uint32_t b[TILE][BS];
-	uint8_t hc[MAXC][BS];
+	uint8_t hc[ROWS][BS];
 };
+using DrawLds = DrawLdsN<MAXC>;
+using DrawLdsTile = DrawLdsN<TILE>;
 
 // Fetch n <= TILE consecutive 32-bit values starting at src into the lane's
 // LDS column (vector loads when the run is whole and aligned).
@@ -67,7 +73,8 @@ GF_D uint8_t hc_code(uint32_t hidden, uint32_t comp)
 }
 
 // Write the lane's n recorded positions (LDS columns a = c, b = x) to dst.
-GF_D void tile_store(uint2 *dst, int n, const DrawLds &L, int lane)
+template <class Lds>
+GF_D void tile_store(uint2 *dst, int n, const Lds &L, int lane)
 {
 	if (n == TILE && ((uintptr_t)dst & 15) == 0) {
 		uint4 *v = (uint4 *)dst;
@@ -100,8 +107,8 @@ GF_D bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 // the component count leave through vector stores when every row of a whole
 // tile takes part, otherwise row by row for the rows that take part.  The
 // system magnitudes follow in the FEH row pass (rows_mult_total).
-template <bool CACHED>
-GF_D void mult_walk(const gf_chain_mods &m, gf_mwc &rng, uint32_t r0, uint32_t r1, const gf_table &T, DrawLds &L,
+template <bool CACHED, class Lds>
+GF_D void mult_walk(const gf_chain_mods &m, gf_mwc &rng, uint32_t r0, uint32_t r1, const gf_table &T, Lds &L,
                     int lane)
 {
 	for (uint32_t base = r0; base < r1; base += TILE) {
@@ -159,9 +166,9 @@ GF_D bool mult_split(const gf_chain_mods &m, const gf_table &T) { return m.has_m
 
 // One module's walk over the chunk: record each visible row's position in
 // plane pos, then step past the row's draws with skip(rng, comp).
-template <bool CACHED, typename Skip>
+template <bool CACHED, class Lds, typename Skip>
 GF_D void draw_walk(gf_mwc &rng, uint32_t r0, uint32_t r1, uint64_t prow0, uint2 *plane, const gf_table &T,
-                    DrawLds &L, int lane, Skip skip)
+                    Lds &L, int lane, Skip skip)
 {
 	for (uint32_t base = r0; base < r1; base += TILE) {
 		int n = (int)(r1 - base < (uint32_t)TILE ? r1 - base : TILE);
@@ -183,11 +190,12 @@ GF_D void draw_walk(gf_mwc &rng, uint32_t r0, uint32_t r1, uint64_t prow0, uint2
 
 // The draw pass over one chunk [r0, r1) whose rows have positions at
 // prow0 + (row - r0) of every plane.
-template <bool CACHED>
+template <bool CACHED, class Lds>
 GF_D void chunk_draws(const gf_chain_mods &m, gf_mwc &rng, uint32_t r0, uint32_t r1, uint64_t prow0,
-                      const gf_table &T, const gf_positions &P, DrawLds &L, int lane)
+                      const gf_table &T, const gf_positions &P, Lds &L, int lane)
 {
-	if (CACHED) {
+	static_assert(!CACHED || sizeof(L.hc) >= sizeof(uint8_t[MAXC][BS]), "the cached walk keeps every row's code");
+	if constexpr (CACHED) {
 		for (uint32_t base = r0; base < r1; base += TILE) {
 			int n = (int)(r1 - base < (uint32_t)TILE ? r1 - base : TILE);
 			tile_fetch(&col<uint32_t>(T.hidden, base), n, L.a, lane, 1u);

@@ -196,7 +196,7 @@ __global__ void k_b_req_rows(const gf_serve_seg *segall, int nreq, int nm, const
 // ---------------------------------------------------------------------------
 // One lane g of the batch's draw pass.
 GF_D void serve_draw_lane(const gf_serve_req *reqs, int nreq, uint32_t g, const gf_serve_chain &ch,
-                          const gf_table &T, const gf_positions &P, gfc::DrawLds &L)
+                          const gf_table &T, const gf_positions &P, gfc::DrawLdsTile &L)
 {
 	int ri = upper_index(reqs, nreq, [](const gf_serve_req &q) { return (uint64_t)q.lane0; }, g);
 	const gf_serve_req &rq = reqs[ri];
@@ -207,7 +207,7 @@ GF_D void serve_draw_lane(const gf_serve_req *reqs, int nreq, uint32_t g, const 
 	for (uint64_t c = s; c * 256 < nrows; c += rq.S) {
 		uint32_t r0 = (uint32_t)(rq.row0 + c * 256);
 		uint32_t r1 = (uint32_t)(rq.row0 + std::min<uint64_t>(nrows, (c + 1) * 256));
-		gfc::chunk_draws<true>(ch, rng, r0, r1, r0, T, P, L, threadIdx.x);
+		gfc::chunk_draws<false>(ch, rng, r0, r1, r0, T, P, L, threadIdx.x);
 	}
 	gf_mwc_store(rq.state, rq.S, s, rng);
 }
@@ -230,7 +230,7 @@ GF_D void serve_row(const gf_serve_req *reqs, int nreq, const gf_serve_chain &ch
 __global__ void __launch_bounds__(gfc::BS) k_b_draws(const gf_serve_req *reqs, int nreq, uint32_t nlanes,
                                                      const gf_serve_chain *ch, gf_table T, gf_positions P)
 {
-	__shared__ gfc::DrawLds L;
+	__shared__ gfc::DrawLdsTile L;
 	uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
 	if (g >= nlanes) return;
 	serve_draw_lane(reqs, nreq, g, *ch, T, P, L);
@@ -252,7 +252,7 @@ constexpr unsigned GRAPH_ROW_BLOCKS = 8192;    // 2,097,152 rows a sweep
 
 __global__ void __launch_bounds__(gfc::BS) k_g_draws(const gf_serve_chain_args *args, const gf_serve_chain *ch)
 {
-	__shared__ gfc::DrawLds L;
+	__shared__ gfc::DrawLdsTile L;
 	const gf_serve_chain_args a = *args;
 	const uint32_t stride = gridDim.x * blockDim.x;
 	for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < a.nlanes; g += stride)

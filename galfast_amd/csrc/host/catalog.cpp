@@ -11,6 +11,7 @@
 
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <cmath>
 #include <stdio.h>
 #include <string.h>
 
@@ -1520,7 +1521,9 @@ int gf_catalog_count(gf_catalog *c, gf_count_result *per_model, double *expected
 		for (size_t i = 0; i < c->models.size(); i++) {
 			Model &m = *c->models[i];
 			const uint64_t K = (uint64_t)(m.sp.nm + m.sp.nM - 1);
-			if (m.wpk.bytes < np * K * sizeof(double)) m.wpk.alloc(np * K * sizeof(double));
+			// (allocated also for a footprint without pixels: a run checks
+			// for it to know that the count came first)
+			if (!m.wpk.p || m.wpk.bytes < np * K * sizeof(double)) m.wpk.alloc(np * K * sizeof(double));
 			int rc = gf_skygen_count(&m.sp, &m.m, (const gf_pencil_beam *)c->pix_dev.p,
 			                         c->has_ext ? &c->ext : nullptr, c->acc_mode, &m.counts,
 			                         check ? (double *)c->per_pixel.p : nullptr, (double *)m.wpk.p, c->stream);
@@ -2570,6 +2573,9 @@ int gf_server_submit(gf_server *s, const gf_request *req)
 	GF_CHECK((req->radius > 0 && req->radius < 90) ||
 	         (req->radius == 0 && req->l1 > req->l && req->l1 - req->l <= 360. && req->b >= -90. &&
 	          req->b < req->b1 && req->b1 <= 90.), GF_E_ARG);
+	GF_CHECK(std::isfinite(req->l) && std::isfinite(req->b) && req->b >= -90. && req->b <= 90., GF_E_ARG);
+	// at least one apparent-magnitude bin between the server's m0 and m1
+	GF_CHECK(std::isfinite(req->m1) && std::round((req->m1 - s->m0raw) / s->base.sp.dm) >= 1.0, GF_E_ARG);
 	auto q = std::make_unique<SReq>();
 	q->r = *req;
 	q->t_submit = server_clock();

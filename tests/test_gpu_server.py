@@ -119,6 +119,37 @@ def test_request_rows_same_alone_and_in_mix(base, alone):
         same_rows(t, ref[rid][1], "request %d (reversed)" % rid)
 
 
+def test_request_without_stars_beside_others(base, alone):
+    """A beam of 3e-6 deg^2 whose faint limit leaves one apparent-magnitude
+    bin draws no star; batched with ordinary requests it completes with no
+    rows and leaves theirs unchanged."""
+    ref, _ = alone
+    empty = (50, 140.0, 30.0, 0.001, 14.05, 77)
+    got, st = serve(base, REQUESTS[:3] + [empty], max_batch=8, workers=1, linger_us=300000)
+    assert st.completed == 4
+    res, t = got[50]
+    assert res.expected < 1e-3 and res.generated == 0 and res.visible == 0 and t.n == 0
+    assert res.batch_size > 1
+    for rid in (1, 2, 3):
+        same_rows(got[rid][1], ref[rid][1], "request %d beside an empty one" % rid)
+
+
+def test_submit_rejects_malformed_requests(base):
+    srv = Server(base, None, plain=True)
+    try:
+        for bad in [(1, 90.0, 60.0, 0.5, 14.0, 1),            # no magnitude bin between m0 = 14 and m1
+                    (2, 90.0, 60.0, 0.5, float("nan"), 1),
+                    (3, float("nan"), 60.0, 0.5, 22.0, 1),
+                    (4, 90.0, 95.0, 0.5, 22.0, 1),            # |b| > 90
+                    (5, 90.0, 60.0, 0.0, 22.0, 1)]:           # rect with l1 <= l
+            with pytest.raises(G.GalfastError):
+                srv.submit(*bad)
+        srv.submit(6, 90.0, 60.0, 0.5, 14.05, 1)            # one bin: accepted
+        srv.wait()
+    finally:
+        srv.close()
+
+
 def test_captured_chain_matches_direct_launches(base, alone):
     """Every batch replays the worker's captured HIP graph of the module
     chain (plan, emission and the count run eagerly between replays); the

@@ -178,34 +178,29 @@ def cpu_baseline(cat, workload, budget_s, threads):
                                                                                     one["seconds"])}}
 
 
-def _serve_once(conf, reqs, rate, warm, **kw):
-    """Serve reqs (after `warm` warm-up requests) with Poisson arrivals at
-    `rate` req/s (None: all at once); latency = done - submit on the
-    server's clock."""
+def _serve_phase(srv, reqs, rate, id_offset=0):
+    """Submit reqs to the open server srv, their ids shifted by id_offset,
+    with Poisson arrivals at `rate` req/s (None: all at once), and wait for
+    them; the phase's record (latency = done - submit on the server's
+    clock, worker times as the phase's share of the server's totals)."""
     import numpy as np
     from galfast_amd.server import Server
-    srv = Server(conf, None, hbm_budget=200 << 30, **kw)
-    try:
-        for r in warm:
-            srv.submit(*r)
-        srv.wait()
-        nwarm = len(srv.results)
-        rng = np.random.default_rng(1)
-        t = Server.clock()
-        for r in reqs:
-            if rate:
-                t += rng.exponential(1.0 / rate)
-                while True:
-                    dt = t - Server.clock()
-                    if dt <= 0:
-                        break
-                    time.sleep(min(dt, 0.001))
-            srv.submit(*r)
-        srv.wait()
-        res = srv.results[nwarm:]
-        st = srv.stats()
-    finally:
-        srv.close()
+    n0 = len(srv.results)
+    st0 = srv.stats()
+    rng = np.random.default_rng(1)
+    t = Server.clock()
+    for r in reqs:
+        if rate:
+            t += rng.exponential(1.0 / rate)
+            while True:
+                dt = t - Server.clock()
+                if dt <= 0:
+                    break
+                time.sleep(min(dt, 0.001))
+        srv.submit(r[0] + id_offset, *r[1:])
+    srv.wait()
+    res = srv.results[n0:]
+    st = srv.stats()
     lat = np.array([x.t_done - x.t_submit for x in res])
     wall = max(x.t_done for x in res) - min(x.t_submit for x in res)
     stars = sum(x.visible for x in res)
@@ -213,8 +208,28 @@ def _serve_once(conf, reqs, rate, warm, **kw):
     return {"requests": len(res), "failed": sum(1 for x in res if x.status != 0), "stars": stars,
             "rows": sum(x.generated for x in res), "wall_s": wall, "stars_per_s": stars / wall,
             "p50_ms": 1000.0 * float(np.percentile(lat, 50)), "p99_ms": 1000.0 * float(np.percentile(lat, 99)),
-            "mean_batch": len(res) / max(nb, 1), "host_prep_s": st.t_prep, "worker_batch_s": st.t_batch,
-            "worker_device_wait_s": st.t_wait, "worker_deliver_s": st.t_deliver}
+            "mean_batch": len(res) / max(nb, 1), "host_prep_s": st.t_prep - st0.t_prep,
+            "worker_batch_s": st.t_batch - st0.t_batch, "worker_device_wait_s": st.t_wait - st0.t_wait,
+            "worker_deliver_s": st.t_deliver - st0.t_deliver}
+
+
+WARM_IDS = 10 ** 8   # id offset of a warm-up pass over a request stream
+
+
+def _serve_once(conf, reqs, rate, warm, full_warm=False, **kw):
+    """Serve reqs on a new server after `warm` warm-up requests (and, with
+    full_warm, after a first pass over reqs themselves under other ids, so
+    that the workers' batch buffers have reached the sizes the stream's
+    batches need when the timed pass starts)."""
+    from galfast_amd.server import Server
+    srv = Server(conf, None, hbm_budget=200 << 30, **kw)
+    try:
+        _serve_phase(srv, warm, None)
+        if full_warm:
+            _serve_phase(srv, reqs, None, WARM_IDS)
+        return _serve_phase(srv, reqs, rate)
+    finally:
+        srv.close()
 
 
 def _plain_jobs(reqs, seed, tmpdir):
@@ -390,14 +405,24 @@ def serving(args):
     # jobs open and free a catalog (gigabytes of device buffers) per request,
     # and for seconds afterwards the host runs the server's CPU work (beam
     # pixelization, stream seeding) at a third of its speed, so they go last.
-    out["batched"] = _serve_once(conf, reqs, None, warm, max_batch=args.serve_batch, workers=2)
-    # Poisson arrivals: at each rate the first min(all, rate x 10 s) requests
-    out["poisson"] = {}
-    for rate in args.serve_rates:
-        n = max(20, min(len(reqs), int(rate * args.serve_seconds)))
-        out["poisson"]["%g" % rate] = _serve_once(conf, reqs[:n], rate, warm, max_batch=args.serve_batch,
-                                                  workers=2)
-    out["unbatched"] = _serve_once(conf, reqs, None, warm, plain=True)
+    # One batching server serves every batched phase, after a warm-up pass
+    # over the stream: device allocations (which cost seconds on a device
+    # that has just freed tens of gigabytes) happen before the timed passes,
+    # as in a long-running server.
+    from galfast_amd.server import Server
+    srv = Server(conf, None, hbm_budget=200 << 30, max_batch=args.serve_batch, workers=2)
+    try:
+        _serve_phase(srv, warm, None)
+        _serve_phase(srv, reqs, None, WARM_IDS)
+        out["batched"] = _serve_phase(srv, reqs, None)
+        # Poisson arrivals: at each rate the first min(all, rate x 10 s) requests
+        out["poisson"] = {}
+        for i, rate in enumerate(args.serve_rates):
+            n = max(20, min(len(reqs), int(rate * args.serve_seconds)))
+            out["poisson"]["%g" % rate] = _serve_phase(srv, reqs[:n], rate, (i + 2) * WARM_IDS)
+    finally:
+        srv.close()
+    out["unbatched"] = _serve_once(conf, reqs, None, warm, full_warm=True, plain=True)
     # plain: one catalog job per request, on every k-th request (a bounded
     # sample of the same request stream)
     k = max(1, len(reqs) // args.plain_jobs)
@@ -466,6 +491,9 @@ def main():
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[10.0, 100.0, 1000.0])
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="arrival window per Poisson rate")
     ap.add_argument("--plain-jobs", type=int, default=40, help="requests timed as separate catalog jobs")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only serve the request stream with continuous batching (all submitted at once) and "
+                         "print that record (the command the serving profiles are taken with)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (the reference kernel for scaling "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -478,6 +506,17 @@ def main():
         y.copy_(x)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4}))
+        return
+    if args.serve_only:
+        import galfast_amd as G
+        from galfast_amd import configs
+        G.check(G.lib().gf_init(0), "gf_init")
+        d = tempfile.mkdtemp(prefix="galfast_serve_")
+        conf = configs.write_c3(d, args.seed)
+        reqs = configs.c3_requests(args.serve_requests, seed=args.seed)
+        warm = configs.c3_requests(16, seed=args.seed + 1, id0=10 ** 7)
+        print(json.dumps({"batched": _serve_once(conf, reqs, None, warm, full_warm=True, max_batch=args.serve_batch,
+                                                 workers=2)}))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (the sky would shard by "
@@ -690,7 +729,8 @@ def c4_bench(args):
 
 def c4_served(args):
     """Config 4 as 200 Galactic rect sub-requests (20 x 10 tiles of 2 x 3
-    deg) submitted at once to one server; stars/s, latency, batches."""
+    deg) submitted at once to one server, after a warm-up pass over the same
+    tiles; stars/s, latency, batches."""
     import numpy as np
     from galfast_amd import configs
     from galfast_amd.server import Server
@@ -703,6 +743,9 @@ def c4_served(args):
         for r in warm:
             srv.submit(*r)
         srv.wait()
+        # a warm-up pass over the tiles under other ids: the workers' batch
+        # buffers reach their sizes before the timed pass
+        _serve_phase(srv, tiles, None, WARM_IDS)
         nwarm = len(srv.results)
         for r in tiles:
             srv.submit(*r)

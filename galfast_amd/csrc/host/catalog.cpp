@@ -2082,6 +2082,31 @@ int wait_stream(gf_server &s, hipStream_t st)
 	return 0;
 }
 
+// Device bytes a batch of `units` units, `pixels` pixels, `words` stream
+// table words, nr requests of nm models, `extf` extinction-minima floats and
+// `rows` table rows needs beyond what the worker's buffers already hold: for
+// each buffer the part of its need above its size (the sizes run_batch grows
+// them to).
+uint64_t beyond_held(const gf_server &s, const Worker &w, uint64_t units, uint64_t pixels, uint64_t words, int nr,
+                     int nm, uint64_t extf, uint64_t rows)
+{
+	auto over = [](const DevBuf &b, uint64_t need) -> uint64_t { return need > b.bytes ? need - b.bytes : 0; };
+	const uint64_t nseg = (uint64_t)nr * nm + 1, nq = (uint64_t)nr + 1;
+	uint64_t e = over(w.pix, std::max<uint64_t>(pixels, 1) * sizeof(gf_pencil_beam)) +
+	             over(w.state, std::max<uint64_t>(words, 1) * 4) +
+	             over(w.seeds, std::max<uint64_t>(nr, 1) * sizeof(gf_seed_seg)) + over(w.W, (units + 1) * 8) +
+	             over(w.counts, (units + 1) * 4) + over(w.offs, (units + 2) * 8) + over(w.zu, (units + 1) * 8) +
+	             over(w.pixsum, (pixels + 1) * 8) + over(w.segexp, nseg * 8) +
+	             over(w.segs, nseg * sizeof(gf_serve_seg)) + over(w.segall, nseg * sizeof(gf_serve_seg)) +
+	             over(w.reqs, nq * sizeof(gf_serve_req)) + over(w.vis, nq * 8) + over(w.roff, nq * 8) +
+	             over(w.extbeam, extf * 4) + over(w.pos, (uint64_t)s.hchain.nplanes * rows * sizeof(uint2));
+	if (!(w.table.p && w.table_cap >= rows)) {
+		const uint64_t tb = table_bytes(s, std::min<uint64_t>(rows, GF_MAX_TABLE_ROWS));
+		e += tb > w.table.bytes ? tb - w.table.bytes : 0;
+	}
+	return e + batch_slack(s);
+}
+
 int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch,
               std::vector<std::unique_ptr<SReq>> &deferred, std::vector<std::unique_ptr<SReq>> &rejected)
 {
@@ -2091,17 +2116,25 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	const uint64_t share = worker_share(s);
 	const uint64_t rowb = row_device_bytes(s);
 	// the batch's admitted size; the buffers of earlier batches are released
-	// first when they would not leave room for it.  The buffers other than
-	// the table grow with headroom only while that leaves the share room for
-	// the table of the admitted rows.
+	// first when growing them for it would not stay within the share (they
+	// are reused as they are otherwise: an allocation and a free can cost
+	// far more than the batch).  The buffers other than the table grow with
+	// headroom only while that leaves the share room for the table of the
+	// admitted rows.
 	uint64_t reserve = batch_slack(s);
 	{
-		uint64_t need = 0;
+		uint64_t units = 0, pixels = 0, words = 0, extf = 0, rows = 0;
 		for (auto &q : batch) {
-			need += request_fixed_bytes(s, *q) + admission_rows(*q) * rowb;
 			reserve += admission_rows(*q) * rowb;
+			units += q->nunits;
+			pixels += q->pix.size();
+			words += 3 * (uint64_t)q->S;
+			extf += q->extmin.size();
+			rows += admission_rows(*q);
 		}
-		if (w.held() + need > share) w.release();
+		// the batch's minima cube pads its pixels to rows of 2,048 (beam_texture)
+		if (extf && pixels) extf = (pixels + 2047) * (extf / pixels);
+		if (w.held() + beyond_held(s, w, units, pixels, words, nr, nm, extf, rows) > share) w.release();
 	}
 	const uint64_t fshare = share > reserve ? share - reserve : 0;
 	// unit layout: request-major, model-minor

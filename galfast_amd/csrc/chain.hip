@@ -23,7 +23,7 @@ __global__ void __launch_bounds__(gfc::BS) k_chain_draws(const gf_chain_launch c
 	uint64_t a = (uint64_t)c.begin + (uint64_t)c.step * t;
 	uint32_t r0 = (uint32_t)a, r1 = (uint32_t)std::min<uint64_t>(a + c.step, c.end);
 	gf_mwc r = gf_mwc_load(st, ns, t);
-	gfc::chunk_draws<CACHED>(c.m, r, r0, r1, (uint64_t)(r0 - c.begin), c.t, c.P, L, threadIdx.x);
+	gfc::chunk_draws<CACHED>(c.m, r, r0, r1, (uint64_t)(r0 - c.begin), c.t, c.P, L, threadIdx.x, c.vis);
 	gf_mwc_store(st, ns, t, r);
 }
 
@@ -81,6 +81,11 @@ int chain_setup(const gf_rows *ks, const gf_chain *chain, const gf_rng *rng, con
 		L.P.pitch = n;
 	}
 	return 0;
+}
+
+bool chain_counts_visible(const gf_chain_launch &L)
+{
+	return chain_on(L, 0) && L.step <= (uint32_t)gfc::MAXC;
 }
 
 bool chain_on(const gf_chain_launch &L, int launch)

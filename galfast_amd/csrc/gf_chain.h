@@ -190,18 +190,36 @@ GF_D void draw_walk(gf_mwc &rng, uint32_t r0, uint32_t r1, uint64_t prow0, uint2
 
 // The draw pass over one chunk [r0, r1) whose rows have positions at
 // prow0 + (row - r0) of every plane.
+// Adds every active lane's count (at most 511) to *out with one atomic per
+// wave: the sum bit by bit from ballots, which see only the lanes that are
+// active (lanes past the launch's end have returned).
+GF_D void wave_add(unsigned long long *out, uint32_t count)
+{
+	unsigned long long sum = 0;
+#pragma unroll
+	for (int b = 0; b < 9; b++) sum += (unsigned long long)__popcll(__ballot((count >> b) & 1u)) << b;
+	const unsigned long long active = __ballot(1);
+	if ((int)__lane_id() == __ffsll((long long)active) - 1) atomicAdd(out, sum);
+}
+
+// vis (CACHED only): when set, the chunk's visible rows are added to it.
 template <bool CACHED, class Lds>
 GF_D void chunk_draws(const gf_chain_mods &m, gf_mwc &rng, uint32_t r0, uint32_t r1, uint64_t prow0,
-                      const gf_table &T, const gf_positions &P, Lds &L, int lane)
+                      const gf_table &T, const gf_positions &P, Lds &L, int lane, unsigned long long *vis = nullptr)
 {
 	static_assert(!CACHED || sizeof(L.hc) >= sizeof(uint8_t[MAXC][BS]), "the cached walk keeps every row's code");
 	if constexpr (CACHED) {
+		uint32_t nvis = 0;
 		for (uint32_t base = r0; base < r1; base += TILE) {
 			int n = (int)(r1 - base < (uint32_t)TILE ? r1 - base : TILE);
 			tile_fetch(&col<uint32_t>(T.hidden, base), n, L.a, lane, 1u);
 			tile_fetch(&col<uint32_t>(T.comp, base), n, L.b, lane, 0u);
-			for (int i = 0; i < n; i++) L.hc[base - r0 + i][lane] = hc_code(L.a[i][lane], L.b[i][lane]);
+			for (int i = 0; i < n; i++) {
+				L.hc[base - r0 + i][lane] = hc_code(L.a[i][lane], L.b[i][lane]);
+				nvis += L.a[i][lane] == 0u ? 1u : 0u;
+			}
 		}
+		if (vis) wave_add(vis, nvis);
 	}
 	if (m.has_feh)
 		draw_walk<CACHED>(rng, r0, r1, prow0, P.pos + m.pl_feh * P.pitch, T, L, lane,

@@ -55,6 +55,7 @@ struct gf_chain_launch {
 	gf_table t;
 	gf_positions P;
 	uint32_t begin, step, end;
+	unsigned long long *vis;   // when set, the draw pass adds the call's visible rows to *vis
 };
 
 // The row pass's launch groups, in pipeline order: metallicity (FeH,

@@ -70,6 +70,8 @@ size_t chain_positions_bytes(const gf_chain *chain, uint32_t nrows);
 int chain_setup(const gf_rows *ks, const gf_chain *chain, const gf_rng *rng, const gf_table *t, uint2 *pos,
                 gf_chain_launch &L);
 bool chain_on(const gf_chain_launch &L, int launch);
+// Whether the launch's draw pass adds its visible rows to L.vis when set.
+bool chain_counts_visible(const gf_chain_launch &L);
 int chain_launch(const gf_chain_launch &L, const gf_rng *rng, int launch, hipStream_t st);
 // The device address of the no-allocation scratch array on `device`
 // (noalloc_setup, once per device; gf_init calls it) and on the current

@@ -1376,12 +1376,18 @@ int run_catalog(gf_catalog &c, gf_chunk_fn fn, void *user, gf_run_stats *stats)
 			GF_HIP(hipEventRecord(c.emitted[si], E));
 			GF_HIP(hipStreamWaitEvent(A, c.emitted[si], 0));
 			s.n_emit++;
+			bool counted = false;   // the draw passes count the chunk's visible rows
 			for (int j = 0; j < nsub; j++, pb ^= 1) {
 				const uint64_t b0 = (uint64_t)j * sub;
 				gf_rows ks{(uint32_t)b0, 256, (uint32_t)std::min(n, b0 + sub), 0};
 				gf_chain_launch L;
 				rc = gf::chain_setup(&ks, &ch, &c.rng, &sl.table, pos_bytes ? (uint2 *)c.pos[pb].p : nullptr, L);
 				if (rc) return rc;
+				if (j == 0 && gf::chain_counts_visible(L)) {
+					counted = true;
+					GF_HIP(hipMemsetAsync(vis_dev + si, 0, sizeof(unsigned long long), A));
+				}
+				if (counted) L.vis = vis_dev + si;
 				const hipEvent_t *qe = e + EV_EMIT + EV_CALL * j;
 				// the draw pass overwrites the planes the row passes two calls back read
 				GF_HIP(hipStreamWaitEvent(A, c.pos_free[pb], 0));
@@ -1411,9 +1417,11 @@ int run_catalog(gf_catalog &c, gf_chunk_fn fn, void *user, gf_run_stats *stats)
 			// the chunk is complete when the astrometry passes are
 			GF_HIP(hipStreamWaitEvent(B, c.side_done[last], 0));
 			s.rows_chain += n;
-			GF_HIP(hipMemsetAsync(vis_dev + si, 0, sizeof(unsigned long long), B));
-			rc = gf::count_visible(sl.table.hidden, n, vis_dev + si, B);
-			if (rc) return rc;
+			if (!counted) {
+				GF_HIP(hipMemsetAsync(vis_dev + si, 0, sizeof(unsigned long long), B));
+				rc = gf::count_visible(sl.table.hidden, n, vis_dev + si, B);
+				if (rc) return rc;
+			}
 			GF_HIP(hipMemcpyAsync(c.vis_host + si, vis_dev + si, sizeof(unsigned long long), hipMemcpyDeviceToHost,
 			                      B));
 			GF_HIP(hipEventRecord(sl.done, B));

@@ -52,9 +52,12 @@ def test_c0_catalog_matches_oracle_replay(c0):
     assert 5e4 < expected < 2e5
     view = R.host_view(G.lib(), cat._h)
     spec = catalog_spec(nbands=view.nbands, nabsmag=view.nabsmag)
-    got = collect(cat, spec)
+    got = []
+    st = cat.run(lambda t, n, comp: got.append((fetch(t, n, spec), n, comp)))
     ref = []
-    R.replay(view, chunk_rows=40000, on_chunk=lambda t, n, comp: ref.append((t, n, comp)))
+    rs = R.replay(view, chunk_rows=40000, on_chunk=lambda t, n, comp: ref.append((t, n, comp)))
+    # the run's visible count (added up by the draw passes) is the replay's
+    assert st.visible == rs["visible"] and st.generated == rs["generated"]
     assert [(n, c) for _, n, c in got] == [(n, c) for _, n, c in ref]
     # every column of every visible row is bit-identical
     exact = ["lb", "projIdx", "projXY", "DM", "absmag", "XYZ", "comp", "Am", "AmInf", "hidden", "FeH", "mags",

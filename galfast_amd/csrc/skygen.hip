@@ -598,9 +598,12 @@ __global__ void __launch_bounds__(SK_BS) k_lf_guess(const double *Pg, int nM, ch
 	lf_guess_pairs(Pg, G, GP, threadIdx.x, blockDim.x);
 }
 
+constexpr unsigned EMIT_OFFS_LDS = 512;   // tile unit offsets staged in LDS (wider ranges read global memory)
+
 // Emission: lane per star, one tile of EMIT_TILE rows per workgroup.  A row
-// finds its unit between its tile's first and last units, jumps from the
-// unit's star state 8 draws per star before it, and draws the star.
+// finds its unit between its tile's first and last units (their offsets
+// staged in LDS when the range is at most EMIT_OFFS_LDS units), jumps from
+// the unit's star state 8 draws per star before it, and draws the star.
 template <bool FULL>
 __global__ void __launch_bounds__(SK_BS) k_emit_stars(gf_skygen_params sp, int comp, const gf_pencil_beam *pixels,
                                                       gf_extinction ext, gf_footprint foot, const char *lfg,
@@ -610,21 +613,38 @@ __global__ void __launch_bounds__(SK_BS) k_emit_stars(gf_skygen_params sp, int c
                                                       gf_skygen_cols out)
 {
 	extern __shared__ double P[];
+	__shared__ uint64_t so[EMIT_OFFS_LDS];
+	const uint32_t q = blockIdx.x;
+	const uint64_t ua = tu[q], ub = tu[q + 1] + 1;   // offs[ua] <= the tile's rows < offs[ub], ub <= U
+	const bool staged = ub - ua < EMIT_OFFS_LDS;     // so[k] = offs[ua + k], k <= ub - ua
 	for (uint32_t i = threadIdx.x; i < lfg_words; i += blockDim.x) ((uint4 *)P)[i] = ((const uint4 *)lfg)[i];
+	if (staged)
+		for (uint32_t i = threadIdx.x; i <= (uint32_t)(ub - ua); i += blockDim.x) so[i] = offs[ua + i];
 	__syncthreads();
 	const uint16_t *G = (const uint16_t *)(P + sp.nM + 1);
 	const double2 *GP = (const double2 *)((const char *)P + lf_pairs_offset(sp.nM));
 	const double gscale = lf_guess_scale(P, sp.nM);
-	const uint32_t q = blockIdx.x;
 	const uint64_t j = row_begin + (uint64_t)q * EMIT_TILE + threadIdx.x;
 	if (j >= row_end) return;
-	uint64_t a = tu[q], b = tu[q + 1] + 1;   // offs[a] <= j < offs[b]
-	while (b - a > 1) {
-		const uint64_t mid = a + (b - a) / 2;
-		if (offs[mid] <= j) a = mid; else b = mid;
+	uint64_t u, o;   // the row's unit and its first row
+	if (staged) {
+		uint32_t a = 0, b = (uint32_t)(ub - ua);
+		while (b - a > 1) {
+			const uint32_t mid = a + (b - a) / 2;
+			if (so[mid] <= j) a = mid; else b = mid;
+		}
+		u = ua + a;
+		o = so[a];
+	} else {
+		uint64_t a = ua, b = ub;
+		while (b - a > 1) {
+			const uint64_t mid = a + (b - a) / 2;
+			if (offs[mid] <= j) a = mid; else b = mid;
+		}
+		u = a;
+		o = offs[u];
 	}
-	const uint64_t u = a;
-	const uint64_t jj = j - offs[u];
+	const uint64_t jj = j - o;
 	const uint32_t s = (uint32_t)(u % ns);
 	const int K = sp.nm + sp.nM - 1;
 	const int p = (int)(u / K), k = (int)(u % K);

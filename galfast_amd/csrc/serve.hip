@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(SB) k_b_plan_units(const gf_serve_seg *segs, i
 		const uint64_t i = sg.i0 + u / sg.S;
 		gf_mwc r = gf_mwc_load(sg.state, sg.S, s);
 		const gf_mwc_mod md(r.a);
-		const uint64_t z = gf_mwc_jump_z(md, gf_mwc_z(r), i << GF_UNIT_SPAN_BITS, jtab + s, JSTRIDE);
+		const uint64_t z = gf_mwc_jump_z(md, gf_mwc_z(r), i, jtab + (size_t)GF_UNIT_SPAN_BITS * JSTRIDE + s, JSTRIDE);
 		gf_mwc_set_z(r, z);
 		n = r.poisson((float)w);
 		zu[g] = md.mul(z, jtab[(size_t)GF_EMIT_AT_BIT * JSTRIDE + s]);
@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(SB) k_b_emit_stars(const gf_serve_seg *segs, i
 	gf_mwc r;
 	r.a = sg.state[s];
 	const gf_mwc_mod md(r.a);
-	gf_mwc_set_z(r, gf_mwc_jump_z(md, zu[a], jj << GF_STAR_SPAN_BITS, jtab + s, JSTRIDE));
+	gf_mwc_set_z(r, gf_mwc_jump_z(md, zu[a], jj, jtab + (size_t)GF_STAR_SPAN_BITS * JSTRIDE + s, JSTRIDE));
 	const gf_pencil_beam pix = pixels[sg.pix0 + p];
 	float Am;
 	int lo, hi;

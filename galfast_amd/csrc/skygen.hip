@@ -547,7 +547,8 @@ __global__ void __launch_bounds__(SK_BS) k_plan_units(const double *wpk, uint64_
 		const uint64_t i = u / ns;
 		gf_mwc r = gf_mwc_load(base, ns, s);
 		const gf_mwc_mod md(r.a);
-		const uint64_t z = gf_mwc_jump_z(md, gf_mwc_z(r), i << GF_UNIT_SPAN_BITS, jtab + s, ns);
+		// i << GF_UNIT_SPAN_BITS draws: the table from that bit on (the same products)
+		const uint64_t z = gf_mwc_jump_z(md, gf_mwc_z(r), i, jtab + (size_t)GF_UNIT_SPAN_BITS * ns + s, ns);
 		gf_mwc_set_z(r, z);
 		n = r.poisson((float)w);
 		zu[u] = md.mul(z, jtab[(size_t)GF_EMIT_AT_BIT * ns + s]);
@@ -651,7 +652,7 @@ __global__ void __launch_bounds__(SK_BS) k_emit_stars(gf_skygen_params sp, int c
 	gf_mwc r;
 	r.a = aplane[s];
 	const gf_mwc_mod md(r.a);
-	gf_mwc_set_z(r, gf_mwc_jump_z(md, zu[u], jj << GF_STAR_SPAN_BITS, jtab + s, ns));
+	gf_mwc_set_z(r, gf_mwc_jump_z(md, zu[u], jj, jtab + (size_t)GF_STAR_SPAN_BITS * ns + s, ns));
 	const gf_pencil_beam pix = pixels[p];
 	float Am;
 	int lo, hi;

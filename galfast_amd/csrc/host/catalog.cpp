@@ -2324,23 +2324,30 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 	rc = gf::serve_emit(dsegall, nr * nm, dmodels, ext, dpix, (const uint64_t *)w.offs.p, (const uint64_t *)w.zu.p, jtab,
 	                    nrows, sc, st);
 	if (rc) return rc;
+	// the module chain's lanes: per request the streams that own a 256-row
+	// chunk of its rows (stream s owns chunks s, s + S, ...; streams past
+	// the request's chunk count have nothing to draw and are left out)
 	std::vector<gf_serve_req> rq(nr);
+	std::vector<uint32_t> aoff(nr + 1, 0);
 	for (int r = 0; r < nr; r++) {
 		const SReq &q = *batch[r];
+		const uint64_t chunks = (roff[r + 1] - roff[r] + 255) / 256;
+		aoff[r + 1] = aoff[r] + (uint32_t)std::min<uint64_t>(q.S, chunks);
 		rq[r].state = dstate + soff[r];
 		rq[r].row0 = roff[r];
 		rq[r].row1 = roff[r + 1];
-		rq[r].lane0 = loff[r];
+		rq[r].lane0 = aoff[r];
 		rq[r].S = q.S;
 		rq[r].advance = q.nI << GF_UNIT_SPAN_BITS;
 	}
+	const uint32_t nlanes = aoff[nr];
 	GF_HIP(hipMemcpyAsync(w.reqs.p, rq.data(), rq.size() * sizeof(gf_serve_req), hipMemcpyHostToDevice, st));
 	// the streams move past every unit before the module chain draws from them
-	rc = gf::serve_advance((const gf_serve_req *)w.reqs.p, nr, loff[nr], jtab, st);
+	rc = gf::serve_advance((const gf_serve_req *)w.reqs.p, nr, nlanes, jtab, st);
 	if (rc) return rc;
 	grow<uint2>(w, fshare, w.pos, std::max<size_t>(1, (size_t)s.hchain.nplanes * nrows));
 	if (s.cfg.eager_chain) {
-		rc = gf::serve_chain((const gf_serve_req *)w.reqs.p, nr, loff[nr], (const gf_serve_chain *)s.dchain.p,
+		rc = gf::serve_chain((const gf_serve_req *)w.reqs.p, nr, nlanes, (const gf_serve_chain *)s.dchain.p,
 		                     s.hchain, nrows, w.T, (uint2 *)w.pos.p, st);
 		if (rc) return rc;
 	} else if (nrows) {
@@ -2351,7 +2358,7 @@ int run_batch(gf_server &s, Worker &w, std::vector<std::unique_ptr<SReq>> &batch
 			if (rc) return rc;
 		}
 		// the previous batch's replay has finished (its results were read back)
-		w.chain_args_h = gf_serve_chain_args{(const gf_serve_req *)w.reqs.p, nr, loff[nr], nrows, w.T,
+		w.chain_args_h = gf_serve_chain_args{(const gf_serve_req *)w.reqs.p, nr, nlanes, nrows, w.T,
 		                                     gf_positions{(uint2 *)w.pos.p, nrows}};
 		GF_HIP(hipMemcpyAsync(w.chain_args.p, &w.chain_args_h, sizeof(gf_serve_chain_args), hipMemcpyHostToDevice,
 		                      st));

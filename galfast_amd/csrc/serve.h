@@ -29,7 +29,7 @@ struct gf_serve_seg {
 struct gf_serve_req {
 	uint32_t *state;
 	uint64_t row0, row1;
-	uint32_t lane0;
+	uint32_t lane0;         // first lane of the request: it has one per stream that owns a 256-row chunk
 	uint32_t S;
 	uint64_t advance;       // draws its streams move on once all its models are planned
 };

@@ -152,8 +152,8 @@ __global__ void __launch_bounds__(SB) k_b_emit_stars(const gf_serve_seg *segs, i
 	                 [&](int l, int h, double tg) { return lf_bin_search(P, l, h, tg); }, r, out, (uint32_t)row);
 }
 
-// Every request's streams move on past all its units: lane per (request,
-// stream).
+// Every request's streams that own a chunk of its rows move on past all its
+// units: lane per (request, stream); the others draw nothing more.
 __global__ void __launch_bounds__(SB) k_b_advance(const gf_serve_req *reqs, int nreq, uint32_t nlanes,
                                                   const uint64_t *jtab)
 {
